@@ -1,0 +1,1055 @@
+/* Oracle: plain-C restatement of DelFEM's FEM hot path. TEST INFRASTRUCTURE ONLY: the checker the HIP
+ * product is compared against, never the product. Built with -ffp-contract=off so that every product
+ * and sum is rounded as in the reference (g++ -O2 on x86-64 emits no FMA).
+ *
+ * Every function names the reference lines it restates. Parity pinning: tests/test_oracle_vs_reference.py
+ * compares each function bit for bit with the reference compiled from its own sources (oracle/_ref), and
+ * tests/test_oracle_math.py checks what the mathematics fixes (closed forms, rigid-body null space, row
+ * sums, exact factorisation of banded matrices, CG termination on tiny SPD systems). The only function
+ * without a reference counterpart is cr_traction_loads (the reference has no surface traction,
+ * src/femeqn/eqn_linear_solid3d.cpp:312-314); it is pinned by closed form (sum of loads = t * area).
+ */
+#include "cpu_ref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* pattern: src/femfield/elem_ary.cpp:675-724 (MakePointSurElem), :821-892 (MakePointSurPoint with        */
+/* isnt_self), include/delfem/indexed_array.h:113-132 (Sort): per row, the distinct neighbours, ascending. */
+
+static int* point_sur_elem(int nnode, int nelem, int npoel, const int* lnods, int** index_out) {
+  int* index = (int*)calloc((size_t)nnode + 1, sizeof(int));
+  for (long e = 0; e < nelem; e++)
+    for (int k = 0; k < npoel; k++) index[lnods[e * npoel + k] + 1]++;
+  for (int i = 0; i < nnode; i++) index[i + 1] += index[i];
+  int* arr = (int*)malloc(sizeof(int) * (size_t)(index[nnode] > 0 ? index[nnode] : 1));
+  int* fill = (int*)malloc(sizeof(int) * (size_t)(nnode + 1));
+  memcpy(fill, index, sizeof(int) * (size_t)(nnode + 1));
+  for (long e = 0; e < nelem; e++)
+    for (int k = 0; k < npoel; k++) arr[fill[lnods[e * npoel + k]]++] = (int)e;
+  free(fill);
+  *index_out = index;
+  return arr;
+}
+
+static int cmp_u(const void* a, const void* b) {
+  const unsigned x = *(const unsigned*)a, y = *(const unsigned*)b;
+  return (x > y) - (x < y);
+}
+
+long cr_pattern_count(int nnode, int nelem, int npoel, const int* lnods, unsigned* rowptr) {
+  int* esi;
+  int* es = point_sur_elem(nnode, nelem, npoel, lnods, &esi);
+  int* mark = (int*)malloc(sizeof(int) * (size_t)(nnode > 0 ? nnode : 1));
+  for (int i = 0; i < nnode; i++) mark[i] = -1;
+  long cnt = 0;
+  rowptr[0] = 0;
+  for (int i = 0; i < nnode; i++) {
+    mark[i] = i;
+    for (int k = esi[i]; k < esi[i + 1]; k++) {
+      const int e = es[k];
+      for (int a = 0; a < npoel; a++) {
+        const int j = lnods[(long)e * npoel + a];
+        if (mark[j] != i) { mark[j] = i; cnt++; }
+      }
+    }
+    rowptr[i + 1] = (unsigned)cnt;
+  }
+  free(mark); free(es); free(esi);
+  return cnt;
+}
+
+void cr_pattern_fill(int nnode, int nelem, int npoel, const int* lnods, const unsigned* rowptr,
+                     unsigned* colind) {
+  int* esi;
+  int* es = point_sur_elem(nnode, nelem, npoel, lnods, &esi);
+  int* mark = (int*)malloc(sizeof(int) * (size_t)(nnode > 0 ? nnode : 1));
+  for (int i = 0; i < nnode; i++) mark[i] = -1;
+  for (int i = 0; i < nnode; i++) {
+    unsigned pos = rowptr[i];
+    mark[i] = i;
+    for (int k = esi[i]; k < esi[i + 1]; k++) {
+      const int e = es[k];
+      for (int a = 0; a < npoel; a++) {
+        const int j = lnods[(long)e * npoel + a];
+        if (mark[j] != i) { mark[j] = i; colind[pos++] = (unsigned)j; }
+      }
+    }
+    qsort(&colind[rowptr[i]], rowptr[i + 1] - rowptr[i], sizeof(unsigned), cmp_u);
+  }
+  free(mark); free(es); free(esi);
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Mearge: src/matvec/matdia_blkcrs.cpp:165-258. emat is [nno][nno][bs] (bs = b*b). */
+
+static void merge(int nno, const int* no, int bs, const double* emat, const unsigned* rowptr,
+                  const unsigned* colind, double* crs, double* dia) {
+  for (int a = 0; a < nno; a++) {
+    const int i = no[a];
+    for (int c = 0; c < nno; c++) {
+      const double* pin = &emat[(a * nno + c) * bs];
+      if (a == c) {
+        double* pout = &dia[(long)i * bs];
+        for (int k = 0; k < bs; k++) pout[k] += pin[k];
+      } else {
+        const unsigned j = (unsigned)no[c];
+        long pos = -1;
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++)
+          if (colind[q] == j) { pos = q; break; }
+        if (pos < 0) continue; /* pairs missing from the pattern are skipped (:242) */
+        double* pout = &crs[pos * bs];
+        for (int k = 0; k < bs; k++) pout[k] += pin[k];
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* P1 geometry: src/femeqn/ker_emat_tri.cpp:13-15 (TriArea), :36-52 (TriDlDx);                          */
+/* include/delfem/femeqn/ker_emat_tet.h:32-39 (TetVolume, with its 0.1666666666666 literal), :42-77.     */
+
+static double tri_area(const double* p0, const double* p1, const double* p2) {
+  return 0.5 * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+static void tri_dldx(double dldx[3][2], const double* p0, const double* p1, const double* p2) {
+  const double area = tri_area(p0, p1, p2);
+  const double tmp1 = 0.5 / area;
+  dldx[0][0] = tmp1 * (p1[1] - p2[1]);
+  dldx[1][0] = tmp1 * (p2[1] - p0[1]);
+  dldx[2][0] = tmp1 * (p0[1] - p1[1]);
+  dldx[0][1] = tmp1 * (p2[0] - p1[0]);
+  dldx[1][1] = tmp1 * (p0[0] - p2[0]);
+  dldx[2][1] = tmp1 * (p1[0] - p0[0]);
+}
+
+static double tet_volume(const double* p0, const double* p1, const double* p2, const double* p3) {
+  double vol = (p1[0] - p0[0]) * ((p2[1] - p0[1]) * (p3[2] - p0[2]) - (p3[1] - p0[1]) * (p2[2] - p0[2]))
+             - (p1[1] - p0[1]) * ((p2[0] - p0[0]) * (p3[2] - p0[2]) - (p3[0] - p0[0]) * (p2[2] - p0[2]))
+             + (p1[2] - p0[2]) * ((p2[0] - p0[0]) * (p3[1] - p0[1]) - (p3[0] - p0[0]) * (p2[1] - p0[1]));
+  vol *= 0.1666666666666;
+  return vol;
+}
+
+static void tet_dldx(double d[4][3], const double* p0, const double* p1, const double* p2, const double* p3) {
+  const double vol = tet_volume(p0, p1, p2, p3);
+  const double t = 1.0 / (vol * 6.0);
+  d[0][0] = -t * ((p2[1] - p1[1]) * (p3[2] - p1[2]) - (p3[1] - p1[1]) * (p2[2] - p1[2]));
+  d[0][1] = +t * ((p2[0] - p1[0]) * (p3[2] - p1[2]) - (p3[0] - p1[0]) * (p2[2] - p1[2]));
+  d[0][2] = -t * ((p2[0] - p1[0]) * (p3[1] - p1[1]) - (p3[0] - p1[0]) * (p2[1] - p1[1]));
+  d[1][0] = +t * ((p3[1] - p2[1]) * (p0[2] - p2[2]) - (p0[1] - p2[1]) * (p3[2] - p2[2]));
+  d[1][1] = -t * ((p3[0] - p2[0]) * (p0[2] - p2[2]) - (p0[0] - p2[0]) * (p3[2] - p2[2]));
+  d[1][2] = +t * ((p3[0] - p2[0]) * (p0[1] - p2[1]) - (p0[0] - p2[0]) * (p3[1] - p2[1]));
+  d[2][0] = -t * ((p0[1] - p3[1]) * (p1[2] - p3[2]) - (p1[1] - p3[1]) * (p0[2] - p3[2]));
+  d[2][1] = +t * ((p0[0] - p3[0]) * (p1[2] - p3[2]) - (p1[0] - p3[0]) * (p0[2] - p3[2]));
+  d[2][2] = -t * ((p0[0] - p3[0]) * (p1[1] - p3[1]) - (p1[0] - p3[0]) * (p0[1] - p3[1]));
+  d[3][0] = +t * ((p1[1] - p0[1]) * (p2[2] - p0[2]) - (p2[1] - p0[1]) * (p1[2] - p0[2]));
+  d[3][1] = -t * ((p1[0] - p0[0]) * (p2[2] - p0[2]) - (p2[0] - p0[0]) * (p1[2] - p0[2]));
+  d[3][2] = +t * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* element kernels                                                                                       */
+
+/* src/femeqn/eqn_poisson.cpp:49-125 (2D), :249-323 (3D). p = (alpha, source). */
+static void elem_poisson(int ndim, const double co[][3], const double* val, const double* p, double* emat,
+                         double* eres) {
+  const double alpha = p[0], source = p[1];
+  if (ndim == 2) {
+    const double area = tri_area(co[0], co[1], co[2]);
+    double dldx[3][2];
+    tri_dldx(dldx, co[0], co[1], co[2]);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        emat[i * 3 + j] = alpha * area * (dldx[i][0] * dldx[j][0] + dldx[i][1] * dldx[j][1]);
+    for (int i = 0; i < 3; i++) eres[i] = source * area * 0.33333333333333333;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) eres[i] -= emat[i * 3 + j] * val[j];
+  } else {
+    const double vol = tet_volume(co[0], co[1], co[2], co[3]);
+    double dldx[4][3];
+    tet_dldx(dldx, co[0], co[1], co[2], co[3]);
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++)
+        emat[i * 4 + j] =
+            alpha * vol * (dldx[i][0] * dldx[j][0] + dldx[i][1] * dldx[j][1] + dldx[i][2] * dldx[j][2]);
+    for (int i = 0; i < 4; i++) eres[i] = source * vol * 0.25;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) eres[i] -= emat[i * 4 + j] * val[j];
+  }
+}
+
+/* src/femeqn/eqn_diffusion.cpp:240-345. p = (rho, alpha, source), glob = (dt, gamma). */
+static void elem_diffusion2d(const double co[][3], const double* val, const double* vval, const double* p,
+                             const double* glob, double* emat, double* eres) {
+  const double rho = p[0], alpha = p[1], source = p[2], dt = glob[0], gamma = glob[1];
+  double eC[3][3], eM[3][3];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double dldx[3][2];
+  tri_dldx(dldx, co[0], co[1], co[2]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) eC[i][j] = alpha * area * (dldx[i][0] * dldx[j][0] + dldx[i][1] * dldx[j][1]);
+  {
+    const double dtmp1 = rho * area * 0.08333333333333333;
+    for (int i = 0; i < 3; i++) {
+      for (int j = 0; j < 3; j++) eM[i][j] = dtmp1;
+      eM[i][i] += dtmp1;
+    }
+  }
+  for (int i = 0; i < 3; i++) eres[i] = source * area * 0.333333333333333333;
+  {
+    const double dtmp1 = gamma * dt;
+    for (int i = 0; i < 9; i++) emat[i] = (&eM[0][0])[i] + dtmp1 * (&eC[0][0])[i];
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) eres[i] -= eC[i][j] * (val[j] + dt * vval[j]) + eM[i][j] * vval[j];
+}
+
+/* src/femeqn/eqn_linear_solid2d.cpp:505-598. p = (lambda, myu, rho, gx, gy). emat [3][3][2][2]. */
+static void elem_solid2d(const double co[][3], const double disp[][3], const double* p, double* emat,
+                         double* eres) {
+  const double lambda = p[0], myu = p[1], rho = p[2], g_x = p[3], g_y = p[4];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double dldx[3][2];
+  tri_dldx(dldx, co[0], co[1], co[2]);
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++) {
+      double* e = &emat[(i * 3 + j) * 4];
+      e[0] = area * (lambda + myu) * dldx[i][0] * dldx[j][0];
+      e[1] = area * (lambda * dldx[i][0] * dldx[j][1] + myu * dldx[j][0] * dldx[i][1]);
+      e[2] = area * (lambda * dldx[i][1] * dldx[j][0] + myu * dldx[j][1] * dldx[i][0]);
+      e[3] = area * (lambda + myu) * dldx[i][1] * dldx[j][1];
+      const double dtmp1 = (dldx[i][1] * dldx[j][1] + dldx[i][0] * dldx[j][0]) * area * myu;
+      e[0] += dtmp1;
+      e[3] += dtmp1;
+    }
+  }
+  for (int i = 0; i < 3; i++) {
+    eres[i * 2 + 0] = area * rho * g_x * 0.33333333333333333;
+    eres[i * 2 + 1] = area * rho * g_y * 0.33333333333333333;
+  }
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++) {
+      const double* e = &emat[(i * 3 + j) * 4];
+      eres[i * 2 + 0] -= e[0] * disp[j][0] + e[1] * disp[j][1];
+      eres[i * 2 + 1] -= e[2] * disp[j][0] + e[3] * disp[j][1];
+    }
+  }
+}
+
+/* src/femeqn/eqn_linear_solid3d.cpp:241-334. p = (lambda, myu, rho, gx, gy, gz). emat [4][4][3][3]. */
+static void elem_solid3d(const double co[][3], const double disp[][3], const double* p, double* emat,
+                         double* eres) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double g[3] = {p[3], p[4], p[5]};
+  const double vol = tet_volume(co[0], co[1], co[2], co[3]);
+  double d[4][3];
+  tet_dldx(d, co[0], co[1], co[2], co[3]);
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++) {
+      double* e = &emat[(i * 4 + j) * 9];
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          e[a * 3 + b] = vol * (lambda * d[i][a] * d[j][b] + myu * d[j][a] * d[i][b]);
+      const double dtmp1 = d[i][0] * d[j][0] + d[i][1] * d[j][1] + d[i][2] * d[j][2];
+      e[0] += vol * myu * dtmp1;
+      e[4] += vol * myu * dtmp1;
+      e[8] += vol * myu * dtmp1;
+    }
+  }
+  for (int i = 0; i < 4; i++) {
+    eres[i * 3 + 0] = vol * rho * g[0] * 0.25;
+    eres[i * 3 + 1] = vol * rho * g[1] * 0.25;
+    eres[i * 3 + 2] = vol * rho * g[2] * 0.25;
+    for (int j = 0; j < 4; j++) {
+      const double* e = &emat[(i * 4 + j) * 9];
+      eres[i * 3 + 0] -= e[0] * disp[j][0] + e[1] * disp[j][1] + e[2] * disp[j][2];
+      eres[i * 3 + 1] -= e[3] * disp[j][0] + e[4] * disp[j][1] + e[5] * disp[j][2];
+      eres[i * 3 + 2] -= e[6] * disp[j][0] + e[7] * disp[j][1] + e[8] * disp[j][2];
+    }
+  }
+}
+
+/* src/femeqn/eqn_st_venant.cpp:47-156 (AddElemMatFin_StVenant2D), :157-272 (element loop).
+ * p = (lambda, myu, rho, gx, gy). */
+static void elem_stvk2d(const double co[][3], const double disp[][3], const double* p, double* emat,
+                        double* eres) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double g[2] = {p[3], p[4]};
+  double efin[3][2] = {{0, 0}, {0, 0}, {0, 0}}, efex[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int i = 0; i < 36; i++) emat[i] = 0.0;
+  const double area = tri_area(co[0], co[1], co[2]);
+  double dndx[3][2];
+  tri_dldx(dndx, co[0], co[1], co[2]);
+  double dudx[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int n = 0; n < 3; n++)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) dudx[a][b] += disp[n][a] * dndx[n][b];
+  const double detwei = area;
+  double strain[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) {
+      strain[a][b] = 0.5 * (dudx[a][b] + dudx[b][a]);
+      for (int k = 0; k < 2; k++) strain[a][b] += 0.5 * dudx[k][a] * dudx[k][b];
+    }
+  double d2s[3][2][2][2];
+  {
+    double z[2][2];
+    for (int a = 0; a < 2; a++) {
+      for (int b = 0; b < 2; b++) z[a][b] = dudx[a][b];
+      z[a][a] += 1.0;
+    }
+    for (int k = 0; k < 3; k++)
+      for (int jd = 0; jd < 2; jd++)
+        for (int id = 0; id < 2; id++)
+          for (int kd = 0; kd < 2; kd++) d2s[k][kd][id][jd] = dndx[k][jd] * z[kd][id];
+  }
+  double stress[2][2];
+  {
+    double dtmp1 = 0.0;
+    for (int a = 0; a < 2; a++) {
+      for (int b = 0; b < 2; b++) stress[a][b] = 2.0 * myu * strain[a][b];
+      dtmp1 += strain[a][a];
+    }
+    for (int a = 0; a < 2; a++) stress[a][a] += lambda * dtmp1;
+  }
+  for (int k = 0; k < 3; k++)
+    for (int kd = 0; kd < 2; kd++)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) efin[k][kd] += detwei * stress[a][b] * d2s[k][kd][a][b];
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++) {
+      double* e = &emat[(i * 3 + j) * 4];
+      for (int id = 0; id < 2; id++)
+        for (int jd = 0; jd < 2; jd++) {
+          {
+            double dtmp1 = 0.0;
+            for (int gd = 0; gd < 2; gd++)
+              for (int hd = 0; hd < 2; hd++)
+                dtmp1 += d2s[i][id][gd][hd] * d2s[j][jd][gd][hd] + d2s[i][id][gd][hd] * d2s[j][jd][hd][gd];
+            e[id * 2 + jd] += detwei * myu * dtmp1;
+          }
+          {
+            double dtmp1 = 0.0, dtmp2 = 0.0;
+            for (int gd = 0; gd < 2; gd++) {
+              dtmp1 += d2s[i][id][gd][gd];
+              dtmp2 += d2s[j][jd][gd][gd];
+            }
+            e[id * 2 + jd] += detwei * lambda * dtmp1 * dtmp2;
+          }
+        }
+      {
+        double dtmp1 = 0.0;
+        for (int gd = 0; gd < 2; gd++)
+          for (int hd = 0; hd < 2; hd++) dtmp1 += stress[gd][hd] * dndx[i][hd] * dndx[j][gd];
+        for (int id = 0; id < 2; id++) e[id * 2 + id] += detwei * dtmp1;
+      }
+    }
+  }
+  for (int n = 0; n < 3; n++)
+    for (int a = 0; a < 2; a++) efex[n][a] += area * rho * g[a] / 3.0;
+  for (int n = 0; n < 3; n++)
+    for (int a = 0; a < 2; a++) eres[n * 2 + a] = efex[n][a] - efin[n][a];
+}
+
+/* src/femeqn/eqn_st_venant.cpp:724-834 (SetElemMatFin_StVenant3D), :838-939 (element loop).
+ * p = (lambda, myu, rho, gx, gy, gz). */
+static void elem_stvk3d(const double co[][3], const double disp[][3], const double* p, double* emat,
+                        double* eres) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double g[3] = {p[3], p[4], p[5]};
+  double efin[4][3], efex[4][3];
+  for (int i = 0; i < 144; i++) emat[i] = 0.0;
+  for (int i = 0; i < 12; i++) { (&efex[0][0])[i] = 0.0; (&efin[0][0])[i] = 0.0; }
+  const double vol = tet_volume(co[0], co[1], co[2], co[3]);
+  double dndx[4][3];
+  tet_dldx(dndx, co[0], co[1], co[2], co[3]);
+  double u[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int n = 0; n < 4; n++)
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) u[a][b] += disp[n][a] * dndx[n][b];
+  const double detwei = vol;
+  double s[6];
+  {
+    double e[6];
+    e[0] = u[0][0] + 0.5 * (u[0][0] * u[0][0] + u[1][0] * u[1][0] + u[2][0] * u[2][0]);
+    e[1] = u[1][1] + 0.5 * (u[0][1] * u[0][1] + u[1][1] * u[1][1] + u[2][1] * u[2][1]);
+    e[2] = u[2][2] + 0.5 * (u[0][2] * u[0][2] + u[1][2] * u[1][2] + u[2][2] * u[2][2]);
+    e[3] = 0.5 * (u[0][1] + u[1][0] + u[0][0] * u[0][1] + u[1][0] * u[1][1] + u[2][0] * u[2][1]);
+    e[4] = 0.5 * (u[1][2] + u[2][1] + u[0][1] * u[0][2] + u[1][1] * u[1][2] + u[2][1] * u[2][2]);
+    e[5] = 0.5 * (u[2][0] + u[0][2] + u[0][2] * u[0][0] + u[1][2] * u[1][0] + u[2][2] * u[2][0]);
+    const double dtmp1 = e[0] + e[1] + e[2];
+    s[0] = 2.0 * myu * e[0] + lambda * dtmp1;
+    s[1] = 2.0 * myu * e[1] + lambda * dtmp1;
+    s[2] = 2.0 * myu * e[2] + lambda * dtmp1;
+    s[3] = 2.0 * myu * e[3];
+    s[4] = 2.0 * myu * e[4];
+    s[5] = 2.0 * myu * e[5];
+  }
+  double d2s[4][3][6];
+  {
+    const double z[3][3] = {{u[0][0] + 1, u[0][1], u[0][2]},
+                            {u[1][0], u[1][1] + 1, u[1][2]},
+                            {u[2][0], u[2][1], u[2][2] + 1}};
+    for (int k = 0; k < 4; k++)
+      for (int kd = 0; kd < 3; kd++) {
+        d2s[k][kd][0] = dndx[k][0] * z[kd][0];
+        d2s[k][kd][1] = dndx[k][1] * z[kd][1];
+        d2s[k][kd][2] = dndx[k][2] * z[kd][2];
+        d2s[k][kd][3] = (dndx[k][0] * z[kd][1] + dndx[k][1] * z[kd][0]);
+        d2s[k][kd][4] = (dndx[k][1] * z[kd][2] + dndx[k][2] * z[kd][1]);
+        d2s[k][kd][5] = (dndx[k][2] * z[kd][0] + dndx[k][0] * z[kd][2]);
+      }
+  }
+  for (int k = 0; k < 4; k++)
+    for (int kd = 0; kd < 3; kd++) {
+      double dtmp1 = 0;
+      dtmp1 += s[0] * d2s[k][kd][0];
+      dtmp1 += s[1] * d2s[k][kd][1];
+      dtmp1 += s[2] * d2s[k][kd][2];
+      dtmp1 += s[3] * d2s[k][kd][3];
+      dtmp1 += s[4] * d2s[k][kd][4];
+      dtmp1 += s[5] * d2s[k][kd][5];
+      efin[k][kd] = dtmp1 * detwei;
+    }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < i; j++)
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          double dtmp1 = 0.0;
+          dtmp1 += d2s[i][a][0] * d2s[j][b][0];
+          dtmp1 += d2s[i][a][1] * d2s[j][b][1];
+          dtmp1 += d2s[i][a][2] * d2s[j][b][2];
+          dtmp1 += d2s[i][a][3] * d2s[j][b][3] * 0.5;
+          dtmp1 += d2s[i][a][4] * d2s[j][b][4] * 0.5;
+          dtmp1 += d2s[i][a][5] * d2s[j][b][5] * 0.5;
+          const double dtmp2 = d2s[i][a][0] + d2s[i][a][1] + d2s[i][a][2];
+          const double dtmp3 = d2s[j][b][0] + d2s[j][b][1] + d2s[j][b][2];
+          const double dtmp4 = detwei * (myu * 2 * dtmp1 + lambda * dtmp2 * dtmp3);
+          emat[(i * 4 + j) * 9 + a * 3 + b] = dtmp4;
+          emat[(j * 4 + i) * 9 + b * 3 + a] = dtmp4;
+        }
+  for (int i = 0; i < 4; i++)
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double dtmp1 = 0.0;
+        dtmp1 += d2s[i][a][0] * d2s[i][b][0];
+        dtmp1 += d2s[i][a][1] * d2s[i][b][1];
+        dtmp1 += d2s[i][a][2] * d2s[i][b][2];
+        dtmp1 += d2s[i][a][3] * d2s[i][b][3] * 0.5;
+        dtmp1 += d2s[i][a][4] * d2s[i][b][4] * 0.5;
+        dtmp1 += d2s[i][a][5] * d2s[i][b][5] * 0.5;
+        const double dtmp2 = d2s[i][a][0] + d2s[i][a][1] + d2s[i][a][2];
+        const double dtmp3 = d2s[i][b][0] + d2s[i][b][1] + d2s[i][b][2];
+        emat[(i * 4 + i) * 9 + a * 3 + b] = detwei * (myu * 2 * dtmp1 + lambda * dtmp2 * dtmp3);
+      }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      double dtmp1 = 0.0;
+      dtmp1 += s[0] * dndx[i][0] * dndx[j][0];
+      dtmp1 += s[1] * dndx[i][1] * dndx[j][1];
+      dtmp1 += s[2] * dndx[i][2] * dndx[j][2];
+      dtmp1 += s[3] * (dndx[i][0] * dndx[j][1] + dndx[i][1] * dndx[j][0]);
+      dtmp1 += s[4] * (dndx[i][1] * dndx[j][2] + dndx[i][2] * dndx[j][1]);
+      dtmp1 += s[5] * (dndx[i][2] * dndx[j][0] + dndx[i][0] * dndx[j][2]);
+      emat[(i * 4 + j) * 9 + 0] += detwei * dtmp1;
+      emat[(i * 4 + j) * 9 + 4] += detwei * dtmp1;
+      emat[(i * 4 + j) * 9 + 8] += detwei * dtmp1;
+    }
+  for (int n = 0; n < 4; n++)
+    for (int a = 0; a < 3; a++) efex[n][a] += vol * rho * g[a] / 4.0;
+  for (int n = 0; n < 4; n++)
+    for (int a = 0; a < 3; a++) eres[n * 3 + a] = efex[n][a] - efin[n][a];
+}
+
+int cr_assemble(int kind, int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                const double* params, int npar, int pstride, const double* glob, const double* u,
+                const double* v, int len, const unsigned* rowptr, const unsigned* colind, double* crs,
+                double* dia, double* res) {
+  (void)nnode;
+  (void)npar;
+  const int nno = (ndim == 2) ? 3 : 4;
+  const int bs = len * len;
+  double emat[144], eres[12], co[4][3], dsp[4][3], val[4], vval[4];
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * nno];
+    const double* p = &params[e * pstride];
+    for (int a = 0; a < nno; a++) {
+      for (int d = 0; d < 3; d++) co[a][d] = (d < ndim) ? coords[(long)no[a] * ndim + d] : 0.0;
+      for (int d = 0; d < 3; d++) dsp[a][d] = (d < len && u) ? u[(long)no[a] * len + d] : 0.0;
+      val[a] = u ? u[(long)no[a] * len] : 0.0;
+      vval[a] = v ? v[(long)no[a] * len] : 0.0;
+    }
+    switch (kind) {
+      case CR_POISSON: elem_poisson(ndim, co, val, p, emat, eres); break;
+      case CR_DIFFUSION2D: elem_diffusion2d(co, val, vval, p, glob, emat, eres); break;
+      case CR_SOLID2D: elem_solid2d(co, dsp, p, emat, eres); break;
+      case CR_SOLID3D: elem_solid3d(co, dsp, p, emat, eres); break;
+      case CR_STVK2D: elem_stvk2d(co, dsp, p, emat, eres); break;
+      case CR_STVK3D: elem_stvk3d(co, dsp, p, emat, eres); break;
+      default: return -1;
+    }
+    merge(nno, no, bs, emat, rowptr, colind, crs, dia);
+    /* CVector_Blk::AddValue, include/delfem/matvec/vector_blk.h:170-181 */
+    for (int a = 0; a < nno; a++)
+      for (int d = 0; d < len; d++) res[(long)no[a] * len + d] += eres[a * len + d];
+  }
+  return 0;
+}
+
+void cr_add_nodal_loads(int nload, const int* node, const double* loads, int len, double* res) {
+  for (int k = 0; k < nload; k++)
+    for (int d = 0; d < len; d++) res[(long)node[k] * len + d] += loads[(long)k * len + d];
+}
+
+void cr_traction_loads(int nface, const int* faces, const double* trac, const double* coords, double* res) {
+  for (int f = 0; f < nface; f++) {
+    const double* p0 = &coords[(long)faces[f * 3 + 0] * 3];
+    const double* p1 = &coords[(long)faces[f * 3 + 1] * 3];
+    const double* p2 = &coords[(long)faces[f * 3 + 2] * 3];
+    const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double area = 0.5 * sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 3; d++) res[(long)faces[f * 3 + k] * 3 + d] += trac[f * 3 + d] * area / 3.0;
+  }
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* src/matvec/matdia_blkcrs.cpp:261-343 (SetBoundaryCondition, fixed block size branch :313-342),        */
+/* include/delfem/matvec/bcflag_blk.h:102-129 (SetZeroToBCDof), src/ls/linearsystem.cpp:92-128.          */
+double cr_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
+                 const int* flags, double* res) {
+  const int bl = len, bs = len * len;
+  for (long i = 0; i < n; i++) {
+    for (int id = 0; id < bl; id++) {
+      if (flags[i * bl + id] == 0) continue;
+      for (int k = 0; k < bl; k++) dia[i * bs + k * bl + id] = 0.0;
+      for (int j = 0; j < bl; j++) dia[i * bs + id * bl + j] = 0.0;
+      dia[i * bs + id * bl + id] = 1.0;
+      for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++)
+        for (int j = 0; j < bl; j++) crs[(long)q * bs + id * bl + j] = 0.0;
+    }
+  }
+  const unsigned ncrs = rowptr[n];
+  for (unsigned q = 0; q < ncrs; q++) {
+    const unsigned j1 = colind[q];
+    for (int jd = 0; jd < bl; jd++) {
+      if (flags[(long)j1 * bl + jd] == 0) continue;
+      for (int id = 0; id < bl; id++) crs[(long)q * bs + id * bl + jd] = 0.0;
+    }
+  }
+  double sq = 0.0;
+  for (long i = 0; i < (long)n * len; i++) {
+    if (flags[i] == 1) res[i] = 0;
+  }
+  for (long i = 0; i < (long)n * len; i++) sq += res[i] * res[i];
+  return sqrt(sq);
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* src/matvec/matdia_blkcrs.cpp:921-1094 (MatVec, b = 1, 2, 3 specialisations, general b)               */
+void cr_matvec(int n, int len, const unsigned* rowptr, const unsigned* colind, const double* crs,
+               const double* dia, double alpha, const double* x, double beta, double* y) {
+  if (len == 1) {
+    for (long i = 0; i < n; i++) {
+      double* iy = y + i;
+      (*iy) *= beta;
+      for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) (*iy) += alpha * crs[q] * x[colind[q]];
+      (*iy) += alpha * dia[i] * x[i];
+    }
+  } else if (len == 2) {
+    for (long i = 0; i < n; i++) {
+      double* iy = y + i * 2;
+      iy[0] *= beta;
+      iy[1] *= beta;
+      for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        const double* jx = x + (long)colind[q] * 2;
+        iy[0] += alpha * (crs[q * 4] * jx[0] + crs[q * 4 + 1] * jx[1]);
+        iy[1] += alpha * (crs[q * 4 + 2] * jx[0] + crs[q * 4 + 3] * jx[1]);
+      }
+      const double* ix = x + i * 2;
+      iy[0] += alpha * (dia[i * 4] * ix[0] + dia[i * 4 + 1] * ix[1]);
+      iy[1] += alpha * (dia[i * 4 + 2] * ix[0] + dia[i * 4 + 3] * ix[1]);
+    }
+  } else if (len == 3) {
+    for (long i = 0; i < n; i++) {
+      double* iy = y + i * 3;
+      iy[0] *= beta;
+      iy[1] *= beta;
+      iy[2] *= beta;
+      for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        const double* jx = x + (long)colind[q] * 3;
+        const double* a = crs + (long)q * 9;
+        iy[0] += alpha * (a[0] * jx[0] + a[1] * jx[1] + a[2] * jx[2]);
+        iy[1] += alpha * (a[3] * jx[0] + a[4] * jx[1] + a[5] * jx[2]);
+        iy[2] += alpha * (a[6] * jx[0] + a[7] * jx[1] + a[8] * jx[2]);
+      }
+      const double* ix = x + i * 3;
+      const double* a = dia + i * 9;
+      iy[0] += alpha * (a[0] * ix[0] + a[1] * ix[1] + a[2] * ix[2]);
+      iy[1] += alpha * (a[3] * ix[0] + a[4] * ix[1] + a[5] * ix[2]);
+      iy[2] += alpha * (a[6] * ix[0] + a[7] * ix[1] + a[8] * ix[2]);
+    }
+  } else {
+    const int bl = len, bs = len * len;
+    for (long i = 0; i < n; i++) {
+      double* iy = y + i * bl;
+      for (int d = 0; d < bl; d++) iy[d] *= beta;
+      for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        const double* jx = x + (long)colind[q] * bl;
+        for (int a = 0; a < bl; a++)
+          for (int b = 0; b < bl; b++) iy[a] += alpha * crs[(long)q * bs + a * bl + b] * jx[b];
+      }
+      const double* ix = x + i * bl;
+      for (int a = 0; a < bl; a++)
+        for (int b = 0; b < bl; b++) iy[a] += alpha * dia[i * bs + a * bl + b] * ix[b];
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* ILU(0). Symbolic: src/matvec/matdiafrac_blkcrs.cpp:227-287 (m_DiaInd), and with an ordering           */
+/* src/matvec/mat_blkcrs.cpp:493-549 (permuted, std::sort per row). Values: matdia_blkcrs.cpp:673-760.   */
+void cr_ilu0_pattern(int n, const unsigned* rowptr, const unsigned* colind, const int* order,
+                     unsigned* frowptr, unsigned* fcolind, unsigned* diaind) {
+  if (!order) {
+    memcpy(frowptr, rowptr, sizeof(unsigned) * ((size_t)n + 1));
+    memcpy(fcolind, colind, sizeof(unsigned) * rowptr[n]);
+  } else {
+    int* inv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) inv[order[i]] = i;
+    frowptr[0] = 0;
+    for (int i0 = 0; i0 < n; i0++) {
+      const int i1 = order[i0];
+      frowptr[i0 + 1] = frowptr[i0] + (rowptr[i1 + 1] - rowptr[i1]);
+    }
+    for (int i0 = 0; i0 < n; i0++) {
+      const int i1 = order[i0];
+      unsigned pos = frowptr[i0];
+      for (unsigned q = rowptr[i1]; q < rowptr[i1 + 1]; q++) fcolind[pos++] = (unsigned)inv[colind[q]];
+      qsort(&fcolind[frowptr[i0]], frowptr[i0 + 1] - frowptr[i0], sizeof(unsigned), cmp_u);
+    }
+    free(inv);
+  }
+  for (int i = 0; i < n; i++) {
+    diaind[i] = frowptr[i + 1];
+    for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++)
+      if (fcolind[q] > (unsigned)i) { diaind[i] = q; break; }
+  }
+}
+
+void cr_ilu0_setvalue(int n, int len, const unsigned* rowptr, const unsigned* colind, const double* crs,
+                      const double* dia, const int* order, const unsigned* frowptr, const unsigned* fcolind,
+                      double* fcrs, double* fdia) {
+  const int bs = len * len;
+  memset(fcrs, 0, sizeof(double) * (size_t)frowptr[n] * bs);
+  if (!order) {
+    memcpy(fcrs, crs, sizeof(double) * (size_t)rowptr[n] * bs);
+    memcpy(fdia, dia, sizeof(double) * (size_t)n * bs);
+    return;
+  }
+  int* inv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; i++) inv[order[i]] = i;
+  for (int i0 = 0; i0 < n; i0++) {
+    const int i1 = order[i0];
+    for (unsigned q1 = rowptr[i1]; q1 < rowptr[i1 + 1]; q1++) {
+      const unsigned j0 = (unsigned)inv[colind[q1]];
+      for (unsigned q0 = frowptr[i0]; q0 < frowptr[i0 + 1]; q0++)
+        if (fcolind[q0] == j0) {
+          memcpy(&fcrs[(size_t)q0 * bs], &crs[(size_t)q1 * bs], sizeof(double) * bs);
+          break;
+        }
+    }
+    memcpy(&fdia[(size_t)i0 * bs], &dia[(size_t)i1 * bs], sizeof(double) * bs);
+  }
+  free(inv);
+}
+
+/* src/matvec/matdiafrac_blkcrs.cpp:774-1107 (DoILUDecomp), with the b = 1, 2, 3 forms and the general   */
+/* form through CalcSubMatPr (:62-73), CalcInvMat (:107-136) and CalcMatPr (:45-60).                     */
+static void inv_general(double* a, int n, int* info) {
+  *info = 0;
+  for (int i = 0; i < n; i++) {
+    if (fabs(a[i * n + i]) < 1.0e-30) { *info = 1; return; }
+    if (a[i * n + i] < 0.0) (*info)--;
+    double tmp1 = 1.0 / a[i * n + i];
+    a[i * n + i] = 1.0;
+    for (int k = 0; k < n; k++) a[i * n + k] *= tmp1;
+    for (int j = 0; j < n; j++) {
+      if (j != i) {
+        tmp1 = a[j * n + i];
+        a[j * n + i] = 0.0;
+        for (int k = 0; k < n; k++) a[j * n + k] -= tmp1 * a[i * n + k];
+      }
+    }
+  }
+}
+
+int cr_ilu0_factor(int n, int len, const unsigned* frowptr, const unsigned* fcolind, const unsigned* diaind,
+                   double* fcrs, double* fdia) {
+  const int bl = len, bs = len * len;
+  const unsigned nmax_sing = 10;
+  unsigned icnt_sing = 0;
+  int* row2crs = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int j = 0; j < n; j++) row2crs[j] = -1;
+  double tmp[64];
+  int ok = 1;
+  for (int i = 0; i < n && ok; i++) {
+    for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) row2crs[fcolind[q]] = (int)q;
+    for (unsigned ik = frowptr[i]; ik < diaind[i]; ik++) {
+      const unsigned k = fcolind[ik];
+      const double* pik = &fcrs[(size_t)ik * bs];
+      for (unsigned kj = diaind[k]; kj < frowptr[k + 1]; kj++) {
+        const unsigned j = fcolind[kj];
+        const double* pkj = &fcrs[(size_t)kj * bs];
+        double* pij;
+        if (j != (unsigned)i) {
+          const int ij = row2crs[j];
+          if (ij == -1) continue;
+          pij = &fcrs[(size_t)ij * bs];
+        } else {
+          pij = &fdia[(size_t)i * bs];
+        }
+        if (bl == 1) {
+          pij[0] -= pik[0] * pkj[0];
+        } else if (bl == 2) {
+          pij[0] -= pik[0] * pkj[0] + pik[1] * pkj[2];
+          pij[1] -= pik[0] * pkj[1] + pik[1] * pkj[3];
+          pij[2] -= pik[2] * pkj[0] + pik[3] * pkj[2];
+          pij[3] -= pik[2] * pkj[1] + pik[3] * pkj[3];
+        } else if (bl == 3) {
+          for (int a = 0; a < 3; a++) {
+            pij[a * 3 + 0] -= pik[a * 3 + 0] * pkj[0] + pik[a * 3 + 1] * pkj[3] + pik[a * 3 + 2] * pkj[6];
+            pij[a * 3 + 1] -= pik[a * 3 + 0] * pkj[1] + pik[a * 3 + 1] * pkj[4] + pik[a * 3 + 2] * pkj[7];
+            pij[a * 3 + 2] -= pik[a * 3 + 0] * pkj[2] + pik[a * 3 + 1] * pkj[5] + pik[a * 3 + 2] * pkj[8];
+          }
+        } else {
+          for (int a = 0; a < bl; a++)
+            for (int b = 0; b < bl; b++)
+              for (int c = 0; c < bl; c++) pij[a * bl + b] -= pik[a * bl + c] * pkj[c * bl + b];
+        }
+      }
+    }
+    double* pii = &fdia[(size_t)i * bs];
+    int singular = 0;
+    if (bl == 1) {
+      const double iiv = pii[0];
+      if (fabs(iiv) > 1.0e-30) pii[0] = 1.0 / iiv;
+      else singular = 1;
+    } else if (bl == 2) {
+      const double det = pii[0] * pii[3] - pii[1] * pii[2];
+      if (fabs(det) > 1.0e-30) {
+        const double inv_det = 1.0 / det;
+        const double dtmp1 = pii[0];
+        pii[0] = inv_det * pii[3];
+        pii[1] = -inv_det * pii[1];
+        pii[2] = -inv_det * pii[2];
+        pii[3] = inv_det * dtmp1;
+      } else singular = 1;
+    } else if (bl == 3) {
+      const double* a = pii;
+      const double det = a[0] * a[4] * a[8] + a[3] * a[7] * a[2] + a[6] * a[1] * a[5] - a[0] * a[7] * a[5] -
+                         a[6] * a[4] * a[2] - a[3] * a[1] * a[8];
+      if (fabs(det) > 1.0e-30) {
+        /* CalcInvMat3, :139-158 (it recomputes det) */
+        const double det2 = a[0] * a[4] * a[8] + a[3] * a[7] * a[2] + a[6] * a[1] * a[5] -
+                            a[0] * a[7] * a[5] - a[6] * a[4] * a[2] - a[3] * a[1] * a[8];
+        const double inv_det = 1.0 / det2;
+        double t[9];
+        for (int k = 0; k < 9; k++) t[k] = pii[k];
+        pii[0] = inv_det * (t[4] * t[8] - t[5] * t[7]);
+        pii[1] = inv_det * (t[2] * t[7] - t[1] * t[8]);
+        pii[2] = inv_det * (t[1] * t[5] - t[2] * t[4]);
+        pii[3] = inv_det * (t[5] * t[6] - t[3] * t[8]);
+        pii[4] = inv_det * (t[0] * t[8] - t[2] * t[6]);
+        pii[5] = inv_det * (t[2] * t[3] - t[0] * t[5]);
+        pii[6] = inv_det * (t[3] * t[7] - t[4] * t[6]);
+        pii[7] = inv_det * (t[1] * t[6] - t[0] * t[7]);
+        pii[8] = inv_det * (t[0] * t[4] - t[1] * t[3]);
+      } else singular = 1;
+    } else {
+      int info = 0;
+      inv_general(pii, bl, &info);
+      if (info == 1) singular = 1;
+    }
+    if (singular) {
+      icnt_sing++;
+      if (icnt_sing > nmax_sing) { ok = 0; break; }
+    }
+    for (unsigned ij = diaind[i]; ij < frowptr[i + 1]; ij++) {
+      double* pij = &fcrs[(size_t)ij * bs];
+      if (bl == 1) {
+        pij[0] = pij[0] * pii[0];
+      } else if (bl == 2) {
+        for (int k = 0; k < 4; k++) tmp[k] = pij[k];
+        pij[0] = pii[0] * tmp[0] + pii[1] * tmp[2];
+        pij[1] = pii[0] * tmp[1] + pii[1] * tmp[3];
+        pij[2] = pii[2] * tmp[0] + pii[3] * tmp[2];
+        pij[3] = pii[2] * tmp[1] + pii[3] * tmp[3];
+      } else if (bl == 3) {
+        for (int k = 0; k < 9; k++) tmp[k] = pij[k];
+        for (int a = 0; a < 3; a++) {
+          pij[a * 3 + 0] = pii[a * 3 + 0] * tmp[0] + pii[a * 3 + 1] * tmp[3] + pii[a * 3 + 2] * tmp[6];
+          pij[a * 3 + 1] = pii[a * 3 + 0] * tmp[1] + pii[a * 3 + 1] * tmp[4] + pii[a * 3 + 2] * tmp[7];
+          pij[a * 3 + 2] = pii[a * 3 + 0] * tmp[2] + pii[a * 3 + 1] * tmp[5] + pii[a * 3 + 2] * tmp[8];
+        }
+      } else {
+        for (int a = 0; a < bl; a++)
+          for (int b = 0; b < bl; b++) {
+            tmp[a * bl + b] = 0.0;
+            for (int c = 0; c < bl; c++) tmp[a * bl + b] += pii[a * bl + c] * pij[c * bl + b];
+          }
+        for (int k = 0; k < bs; k++) pij[k] = tmp[k];
+      }
+    }
+    for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) row2crs[fcolind[q]] = -1;
+  }
+  free(row2crs);
+  return ok;
+}
+
+/* src/matvec/matdiafrac_blkcrs.cpp:332-507 (ForwardSubstitution), :509-669 (BackwardSubstitution),      */
+/* src/ls/preconditioner.cpp:301-335 (SolvePrecond), src/matvec/ordering_blk.cpp:136-166 (permutation).  */
+static void fwd_bwd(int n, int bl, const unsigned* rp, const unsigned* ci, const unsigned* di, const double* fcrs,
+                    const double* fdia, double* v) {
+  const int bs = bl * bl;
+  double t[8];
+  for (long i = 0; i < n; i++) {
+    for (int d = 0; d < bl; d++) t[d] = v[i * bl + d];
+    for (unsigned q = rp[i]; q < di[i]; q++) {
+      const long j = ci[q];
+      const double* a = &fcrs[(size_t)q * bs];
+      if (bl == 1) {
+        t[0] -= a[0] * v[j];
+      } else if (bl == 2) {
+        const double v0 = v[j * 2], v1 = v[j * 2 + 1];
+        t[0] -= a[0] * v0 + a[1] * v1;
+        t[1] -= a[2] * v0 + a[3] * v1;
+      } else if (bl == 3) {
+        const double v0 = v[j * 3], v1 = v[j * 3 + 1], v2 = v[j * 3 + 2];
+        t[0] -= a[0] * v0 + a[1] * v1 + a[2] * v2;
+        t[1] -= a[3] * v0 + a[4] * v1 + a[5] * v2;
+        t[2] -= a[6] * v0 + a[7] * v1 + a[8] * v2;
+      } else {
+        for (int r = 0; r < bl; r++)
+          for (int c = 0; c < bl; c++) t[r] -= a[r * bl + c] * v[j * bl + c];
+      }
+    }
+    const double* a = &fdia[(size_t)i * bs];
+    if (bl == 1) {
+      v[i] = a[0] * t[0];
+    } else if (bl == 2) {
+      v[i * 2] = a[0] * t[0] + a[1] * t[1];
+      v[i * 2 + 1] = a[2] * t[0] + a[3] * t[1];
+    } else if (bl == 3) {
+      v[i * 3] = a[0] * t[0] + a[1] * t[1] + a[2] * t[2];
+      v[i * 3 + 1] = a[3] * t[0] + a[4] * t[1] + a[5] * t[2];
+      v[i * 3 + 2] = a[6] * t[0] + a[7] * t[1] + a[8] * t[2];
+    } else {
+      for (int r = 0; r < bl; r++) {
+        double s = 0.0;
+        for (int c = 0; c < bl; c++) s += a[r * bl + c] * t[c];
+        v[i * bl + r] = s;
+      }
+    }
+  }
+  for (long i = (long)n - 1; i >= 0; i--) {
+    for (int d = 0; d < bl; d++) t[d] = v[i * bl + d];
+    for (unsigned q = di[i]; q < rp[i + 1]; q++) {
+      const long j = ci[q];
+      const double* a = &fcrs[(size_t)q * bs];
+      if (bl == 1) {
+        t[0] -= a[0] * v[j];
+      } else if (bl == 2) {
+        const double v0 = v[j * 2], v1 = v[j * 2 + 1];
+        t[0] -= a[0] * v0 + a[1] * v1;
+        t[1] -= a[2] * v0 + a[3] * v1;
+      } else if (bl == 3) {
+        const double v0 = v[j * 3], v1 = v[j * 3 + 1], v2 = v[j * 3 + 2];
+        t[0] -= a[0] * v0 + a[1] * v1 + a[2] * v2;
+        t[1] -= a[3] * v0 + a[4] * v1 + a[5] * v2;
+        t[2] -= a[6] * v0 + a[7] * v1 + a[8] * v2;
+      } else {
+        for (int r = 0; r < bl; r++)
+          for (int c = 0; c < bl; c++) t[r] -= a[r * bl + c] * v[j * bl + c];
+      }
+    }
+    for (int d = 0; d < bl; d++) v[i * bl + d] = t[d];
+  }
+}
+
+void cr_ilu_solve(int n, int len, const unsigned* frowptr, const unsigned* fcolind, const unsigned* diaind,
+                  const double* fcrs, const double* fdia, const int* order, double* vec, double* work) {
+  if (!order) {
+    fwd_bwd(n, len, frowptr, fcolind, diaind, fcrs, fdia, vec);
+    return;
+  }
+  for (long i = 0; i < n; i++)
+    for (int d = 0; d < len; d++) work[i * len + d] = vec[(long)order[i] * len + d];
+  fwd_bwd(n, len, frowptr, fcolind, diaind, fcrs, fdia, work);
+  for (long i = 0; i < n; i++)
+    for (int d = 0; d < len; d++) vec[(long)order[i] * len + d] = work[i * len + d];
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Krylov loops on handle semantics: src/ls/solver_ls_iter.cpp:100-172 (Solve_PCG), :291-408             */
+/* (Solve_PBiCGSTAB); vector ops src/matvec/vector_blk.cpp:37-112; MATVEC src/ls/linearsystem.cpp:250-291.*/
+static double v_dot(long m, const double* a, const double* b) {
+  double d = 0.0;
+  for (long i = 0; i < m; i++) d += a[i] * b[i];
+  return d;
+}
+static void v_axpy(long m, double alpha, const double* x, double* y) {
+  if (alpha == 0) return; /* CLinearSystem::AXPY returns early on alpha == 0 (linearsystem.cpp:302) */
+  for (long i = 0; i < m; i++) y[i] += alpha * x[i];
+}
+static void v_scal(long m, double alpha, double* x) {
+  for (long i = 0; i < m; i++) x[i] *= alpha;
+}
+static void precond(const cr_system* s, double* v, double* work) {
+  cr_ilu_solve(s->n, s->len, s->frowptr, s->fcolind, s->diaind, s->fcrs, s->fdia, s->order, v, work);
+}
+static void matvec(const cr_system* s, double alpha, const double* x, double beta, double* y) {
+  if (alpha == 0.0) { v_scal((long)s->n * s->len, beta, y); return; }
+  cr_matvec(s->n, s->len, s->rowptr, s->colind, s->crs, s->dia, alpha, x, beta, y);
+}
+
+int cr_solve_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const double tol = *conv_ratio;
+  const unsigned mx_iter = *iter;
+  const long m = (long)s->n * s->len;
+  double* p = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* z = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* w = (double*)calloc((size_t)m + 1, sizeof(double));
+  int ret = 0;
+  v_scal(m, 0.0, x);
+  double sq_inv_norm_res0;
+  {
+    const double sq0 = v_dot(m, r, r);
+    if (sq0 < 1.0e-30) {
+      *conv_ratio = 0.0;
+      *iter = 0;
+      ret = 1;
+      goto done;
+    }
+    sq_inv_norm_res0 = 1.0 / sq0;
+  }
+  memcpy(z, r, sizeof(double) * m);
+  precond(s, z, w);
+  memcpy(p, z, sizeof(double) * m);
+  {
+    double inpro_rz = v_dot(m, r, z);
+    for (unsigned it = 0; it < mx_iter; it++) {
+      double alpha;
+      {
+        matvec(s, 1.0, p, 0.0, z);
+        const double pap = v_dot(m, p, z);
+        alpha = inpro_rz / pap;
+      }
+      v_axpy(m, -alpha, z, r);
+      v_axpy(m, alpha, p, x);
+      {
+        const double sq = v_dot(m, r, r);
+        if (sq * sq_inv_norm_res0 < tol * tol) {
+          *conv_ratio = sqrt(sq * sq_inv_norm_res0);
+          *iter = it;
+          ret = 1;
+          goto done;
+        }
+      }
+      double beta;
+      {
+        memcpy(z, r, sizeof(double) * m);
+        precond(s, z, w);
+        const double rz_new = v_dot(m, r, z);
+        beta = rz_new / inpro_rz;
+        inpro_rz = rz_new;
+      }
+      v_scal(m, beta, p);
+      v_axpy(m, 1.0, z, p);
+    }
+    {
+      const double sq = v_dot(m, r, r);
+      *conv_ratio = sqrt(sq * sq_inv_norm_res0);
+      ret = 0;
+    }
+  }
+done:
+  free(p); free(z); free(w);
+  return ret;
+}
+
+int cr_solve_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const double tol = *conv_ratio;
+  const unsigned max_iter = *iter;
+  const long m = (long)s->n * s->len;
+  double* buf = (double*)calloc((size_t)m * 8 + 1, sizeof(double));
+  double *vs = buf, *vMs = buf + m, *vAMs = buf + 2 * m, *vp = buf + 3 * m, *vMp = buf + 4 * m,
+         *vAMp = buf + 5 * m, *vr2 = buf + 6 * m, *w = buf + 7 * m;
+  int ret = 1;
+  double sq_inv_norm_res_ini;
+  {
+    const double sq0 = v_dot(m, r, r);
+    if (sq0 < 1.0e-60) {
+      *conv_ratio = sqrt(sq0);
+      *iter = 0;
+      goto done;
+    }
+    sq_inv_norm_res_ini = 1.0 / sq0;
+  }
+  v_scal(m, 0.0, x);
+  memcpy(vr2, r, sizeof(double) * m);
+  memcpy(vp, r, sizeof(double) * m);
+  *iter = max_iter;
+  for (unsigned it = 1; it < max_iter; it++) {
+    memcpy(vMp, vp, sizeof(double) * m);
+    precond(s, vMp, w);
+    const double r_r2 = v_dot(m, r, vr2);
+    matvec(s, 1.0, vMp, 0.0, vAMp);
+    double alpha;
+    {
+      const double den = v_dot(m, vAMp, vr2);
+      alpha = r_r2 / den;
+    }
+    memcpy(vs, r, sizeof(double) * m);
+    v_axpy(m, -alpha, vAMp, vs);
+    memcpy(vMs, vs, sizeof(double) * m);
+    precond(s, vMs, w);
+    matvec(s, 1.0, vMs, 0.0, vAMs);
+    double omega;
+    {
+      const double den = v_dot(m, vAMs, vAMs);
+      const double num = v_dot(m, vs, vAMs);
+      omega = num / den;
+    }
+    v_axpy(m, alpha, vMp, x);
+    v_axpy(m, omega, vMs, x);
+    memcpy(r, vs, sizeof(double) * m);
+    v_axpy(m, -omega, vAMs, r);
+    {
+      const double sq = v_dot(m, r, r);
+      const double sqc = sq * sq_inv_norm_res_ini;
+      if (sqc < tol * tol) {
+        *conv_ratio = sqrt(sq * sq_inv_norm_res_ini);
+        *iter = it;
+        goto done;
+      }
+    }
+    double beta;
+    {
+      const double tmp1 = v_dot(m, r, vr2);
+      beta = tmp1 * alpha / (r_r2 * omega);
+    }
+    v_scal(m, beta, vp);
+    v_axpy(m, 1.0, r, vp);
+    v_axpy(m, -beta * omega, vAMp, vp);
+  }
+done:
+  free(buf);
+  return ret;
+}

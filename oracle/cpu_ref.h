@@ -1,0 +1,84 @@
+/* Oracle: plain-C restatement of DelFEM's FEM hot path (test infrastructure only).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this. It is the checker the
+ * HIP product is compared against, never part of the product. Each function follows the reference file
+ * and lines cited beside it, in the same arithmetic order, so that on the same inputs it reproduces the
+ * reference bit for bit (pinned against the reference itself, compiled from its sources into
+ * oracle/_ref/, by tests/test_oracle_vs_reference.py).
+ *
+ * Storage is the reference's block-CRS (CMatDia_BlkCrs, include/delfem/matvec/mat_blkcrs.h:183-192 and
+ * matdia_blkcrs.h:119): rowptr[n+1] (the reference's m_colInd_Blk), colind[ncrs] (its m_rowPtr_Blk),
+ * crs[ncrs*b*b] off-diagonal blocks row-major, dia[n*b*b] diagonal blocks, the diagonal kept out of CRS.
+ */
+#ifndef DFM_ORACLE_CPU_REF_H
+#define DFM_ORACLE_CPU_REF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element kinds, shared with the product's C-ABI */
+enum { CR_POISSON = 0, CR_SOLID2D = 1, CR_SOLID3D = 2, CR_STVK2D = 3, CR_STVK3D = 4, CR_DIFFUSION2D = 5 };
+
+/* CElemAry::MakePattern_FEM: rowptr[n+1]; returns ncrs. */
+long cr_pattern_count(int nnode, int nelem, int npoel, const int* lnods, unsigned* rowptr);
+void cr_pattern_fill(int nnode, int nelem, int npoel, const int* lnods, const unsigned* rowptr,
+                     unsigned* colind);
+
+/* InitializeMarge + AddLinSys_* element loop + Mearge (values accumulate into crs/dia/res, which the
+ * caller zeroes first). params: per element, npar doubles, stride pstride (0 = one set for all). */
+int cr_assemble(int kind, int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                const double* params, int npar, int pstride, const double* glob, const double* u,
+                const double* v, int len, const unsigned* rowptr, const unsigned* colind, double* crs,
+                double* dia, double* res);
+
+/* res[node*len + d] += loads[k*len + d] for k in order (consistent traction loads; reference extension) */
+void cr_add_nodal_loads(int nload, const int* node, const double* loads, int len, double* res);
+
+/* P1 consistent surface loads: face triangles (3 nodes) with a traction vector each -> nodal loads,
+ * t*A/3 per node, accumulated into res in face order. 3D only. */
+void cr_traction_loads(int nface, const int* faces, const double* trac, const double* coords, double* res);
+
+/* CMatDia_BlkCrs::SetBoundaryCondition + CBCFlag::SetZeroToBCDof; returns ||res|| (FinalizeMarge) */
+double cr_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
+                 const int* flags, double* res);
+
+/* CMatDia_BlkCrs::MatVec: y = beta*y + alpha*A*x */
+void cr_matvec(int n, int len, const unsigned* rowptr, const unsigned* colind, const double* crs,
+               const double* dia, double alpha, const double* x, double beta, double* y);
+
+/* ILU(0) symbolic (CMatDiaFrac_BlkCrs(0,mat[,order])): the factor pattern (permuted and sorted when
+ * order != NULL; order[new] = old) and m_DiaInd. */
+void cr_ilu0_pattern(int n, const unsigned* rowptr, const unsigned* colind, const int* order,
+                     unsigned* frowptr, unsigned* fcolind, unsigned* diaind);
+/* SetValue_Initialize: copy A into the factor pattern (permuted when order != NULL) */
+void cr_ilu0_setvalue(int n, int len, const unsigned* rowptr, const unsigned* colind, const double* crs,
+                      const double* dia, const int* order, const unsigned* frowptr, const unsigned* fcolind,
+                      double* fcrs, double* fdia);
+/* DoILUDecomp: returns 1 on success, 0 when more than 10 singular pivots were met */
+int cr_ilu0_factor(int n, int len, const unsigned* frowptr, const unsigned* fcolind, const unsigned* diaind,
+                   double* fcrs, double* fdia);
+/* CPreconditioner_ILU::SolvePrecond: ForwardSubstitution + BackwardSubstitution, in place, with the
+ * ordering's permutation in and out (work: n*len doubles, used only with an ordering) */
+void cr_ilu_solve(int n, int len, const unsigned* frowptr, const unsigned* fcolind, const unsigned* diaind,
+                  const double* fcrs, const double* fdia, const int* order, double* vec, double* work);
+
+/* A system with its ILU(0) preconditioner, for the Krylov loops. */
+typedef struct {
+  int n, len;
+  const unsigned *rowptr, *colind;
+  const double *crs, *dia;
+  const unsigned *frowptr, *fcolind, *diaind;
+  const double *fcrs, *fdia;
+  const int* order;
+} cr_system;
+
+/* LsSol::Solve_PCG / Solve_PBiCGSTAB on handles -1 (res, overwritten) and -2 (upd).
+ * conv_ratio and iter are in/out as in the reference. Returns the reference's bool. */
+int cr_solve_pcg(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+int cr_solve_pbicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

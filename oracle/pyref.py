@@ -1,0 +1,285 @@
+"""ctypes wrappers over the oracle libraries (test infrastructure only; see oracle/__init__.py)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CPU_REF_SO = os.path.join(HERE, "libcpu_ref.so")
+REF_SO = os.path.join(HERE, "_ref", "libdfm_ref.so")
+
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D = range(6)
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+_cr = None
+
+
+def cpu_ref():
+    global _cr
+    if _cr is None:
+        if not os.path.exists(CPU_REF_SO):
+            raise RuntimeError(f"{CPU_REF_SO} is not built (make -C oracle)")
+        _cr = C.CDLL(CPU_REF_SO)
+        _cr.cr_pattern_count.restype = C.c_long
+        _cr.cr_set_bc.restype = C.c_double
+    return _cr
+
+
+class CpuSystem:
+    """Block-CRS system assembled by the restatement."""
+
+    def __init__(self, ndim, coords, lnods, len_):
+        self.ndim, self.len = ndim, len_
+        self.coords = _f64(coords)
+        self.lnods = _i32(lnods)
+        self.nnode = self.coords.size // ndim
+        self.npoel = 3 if ndim == 2 else 4
+        self.nelem = self.lnods.size // self.npoel
+        lib = cpu_ref()
+        self.rowptr = np.zeros(self.nnode + 1, dtype=np.uint32)
+        ncrs = lib.cr_pattern_count(self.nnode, self.nelem, self.npoel, self.lnods.ctypes.data_as(C.c_void_p),
+                                    self.rowptr.ctypes.data_as(C.c_void_p))
+        self.colind = np.zeros(max(ncrs, 1), dtype=np.uint32)[:ncrs]
+        lib.cr_pattern_fill(self.nnode, self.nelem, self.npoel, self.lnods.ctypes.data_as(C.c_void_p),
+                            self.rowptr.ctypes.data_as(C.c_void_p), self.colind.ctypes.data_as(C.c_void_p))
+        bs = len_ * len_
+        self.crs = np.zeros(ncrs * bs)
+        self.dia = np.zeros(self.nnode * bs)
+        self.res = np.zeros(self.nnode * len_)
+
+    def zero(self):
+        self.crs[:] = 0.0
+        self.dia[:] = 0.0
+        self.res[:] = 0.0
+
+    def assemble(self, kind, params, u=None, v=None, glob=(0.0, 0.0)):
+        """params: (npar,) for all elements or (nelem, npar) per element."""
+        params = _f64(params)
+        if params.ndim == 1:
+            npar, pstride = params.size, 0
+        else:
+            npar, pstride = params.shape[1], params.shape[1]
+        glob = _f64(glob)
+        u = None if u is None else _f64(u)
+        v = None if v is None else _f64(v)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        rc = cpu_ref().cr_assemble(kind, self.ndim, self.nnode, p(self.coords), self.nelem, p(self.lnods),
+                                   p(params), npar, pstride, p(glob), p(u), p(v), self.len, p(self.rowptr),
+                                   p(self.colind), p(self.crs), p(self.dia), p(self.res))
+        assert rc == 0
+
+    def traction(self, faces, trac):
+        faces, trac = _i32(faces), _f64(trac)
+        cpu_ref().cr_traction_loads(faces.size // 3, faces.ctypes.data_as(C.c_void_p),
+                                    trac.ctypes.data_as(C.c_void_p), self.coords.ctypes.data_as(C.c_void_p),
+                                    self.res.ctypes.data_as(C.c_void_p))
+
+    def set_bc(self, flags):
+        flags = _i32(flags)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        return cpu_ref().cr_set_bc(self.nnode, self.len, p(self.rowptr), p(self.colind), p(self.crs),
+                                   p(self.dia), p(flags), p(self.res))
+
+    def matvec(self, alpha, x, beta, y):
+        y = _f64(y).copy()
+        x = _f64(x)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        cpu_ref().cr_matvec(self.nnode, self.len, p(self.rowptr), p(self.colind), p(self.crs), p(self.dia),
+                            C.c_double(alpha), p(x), C.c_double(beta), p(y))
+        return y
+
+    def ilu0(self, order=None):
+        n, bs = self.nnode, self.len * self.len
+        ncrs = int(self.rowptr[-1])
+        f = {"frowptr": np.zeros(n + 1, np.uint32), "fcolind": np.zeros(max(ncrs, 1), np.uint32)[:ncrs],
+             "diaind": np.zeros(n, np.uint32), "fcrs": np.zeros(ncrs * bs), "fdia": np.zeros(n * bs)}
+        order = None if order is None else _i32(order)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        lib = cpu_ref()
+        lib.cr_ilu0_pattern(n, p(self.rowptr), p(self.colind), p(order), p(f["frowptr"]), p(f["fcolind"]),
+                            p(f["diaind"]))
+        lib.cr_ilu0_setvalue(n, self.len, p(self.rowptr), p(self.colind), p(self.crs), p(self.dia), p(order),
+                             p(f["frowptr"]), p(f["fcolind"]), p(f["fcrs"]), p(f["fdia"]))
+        ok = lib.cr_ilu0_factor(n, self.len, p(f["frowptr"]), p(f["fcolind"]), p(f["diaind"]), p(f["fcrs"]),
+                                p(f["fdia"]))
+        f["ok"] = bool(ok)
+        f["order"] = order
+        return f
+
+    def ilu_apply(self, f, r):
+        v = _f64(r).copy()
+        work = np.zeros_like(v)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        cpu_ref().cr_ilu_solve(self.nnode, self.len, p(f["frowptr"]), p(f["fcolind"]), p(f["diaind"]),
+                               p(f["fcrs"]), p(f["fdia"]), p(f["order"]), p(v), p(work))
+        return v
+
+    def _cr_system(self, f):
+        class S(C.Structure):
+            _fields_ = [("n", C.c_int), ("len", C.c_int), ("rowptr", C.c_void_p), ("colind", C.c_void_p),
+                        ("crs", C.c_void_p), ("dia", C.c_void_p), ("frowptr", C.c_void_p),
+                        ("fcolind", C.c_void_p), ("diaind", C.c_void_p), ("fcrs", C.c_void_p),
+                        ("fdia", C.c_void_p), ("order", C.c_void_p)]
+        p = lambda a: None if a is None else a.ctypes.data
+        return S(self.nnode, self.len, p(self.rowptr), p(self.colind), p(self.crs), p(self.dia),
+                 p(f["frowptr"]), p(f["fcolind"]), p(f["diaind"]), p(f["fcrs"]), p(f["fdia"]), p(f["order"]))
+
+    def solve(self, f, method="pcg", tol=1e-10, max_iter=1000, res=None):
+        """Returns (ok, conv_ratio, iteration, update, residual_after)."""
+        s = self._cr_system(f)
+        r = (self.res if res is None else _f64(res)).copy()
+        x = np.zeros_like(r)
+        conv = C.c_double(tol)
+        it = C.c_uint(max_iter)
+        fn = cpu_ref().cr_solve_pcg if method == "pcg" else cpu_ref().cr_solve_pbicgstab
+        ok = fn(C.byref(s), r.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p), C.byref(conv),
+                C.byref(it))
+        return bool(ok), conv.value, it.value, x, r
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def reference():
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            raise RuntimeError(f"{REF_SO} is not built (make -C oracle ref, where the reference sources exist)")
+        _ref = C.CDLL(REF_SO)
+        _ref.ref_sys_create.restype = C.c_void_p
+        for name in ["ref_sys_destroy", "ref_sys_ncrs", "ref_sys_pattern", "ref_sys_set_field", "ref_sys_get_field",
+                     "ref_sys_set_bc", "ref_sys_assemble", "ref_sys_get_matrix", "ref_sys_get_vector",
+                     "ref_sys_set_vector", "ref_sys_matvec", "ref_ilu0_factor", "ref_ilu_apply", "ref_sys_solve",
+                     "ref_sys_update_field"]:
+            getattr(_ref, name).argtypes = None
+        _ref.ref_sys_destroy.argtypes = [C.c_void_p]
+        _ref.ref_sys_ncrs.argtypes = [C.c_void_p]
+    return _ref
+
+
+class RefSystem:
+    """The reference DelFEM driven through its own API (oracle/ref_capi.cpp)."""
+
+    def __init__(self, ndim, coords, lnods, len_, groups=None):
+        self.ndim, self.len = ndim, len_
+        self.coords = _f64(coords)
+        self.lnods = _i32(lnods)
+        self.nnode = self.coords.size // ndim
+        npoel = 3 if ndim == 2 else 4
+        self.nelem = self.lnods.size // npoel
+        self.groups = None if groups is None else _i32(groups)
+        ngroup = 1 if groups is None else int(self.groups.max()) + 1
+        self.ngroup = ngroup
+        lib = reference()
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        self.h = C.c_void_p(lib.ref_sys_create(ndim, self.nnode, p(self.coords), self.nelem, p(self.lnods), ngroup,
+                                               p(self.groups), len_))
+        self.ncrs = lib.ref_sys_ncrs(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            reference().ref_sys_destroy(self.h)
+            self.h = None
+
+    def pattern(self):
+        rp = np.zeros(self.nnode + 1, np.uint32)
+        ci = np.zeros(max(self.ncrs, 1), np.uint32)
+        reference().ref_sys_pattern(self.h, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p))
+        return rp, ci[: self.ncrs]
+
+    def set_field(self, u=None, v=None):
+        u = None if u is None else _f64(u)
+        v = None if v is None else _f64(v)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        reference().ref_sys_set_field(self.h, p(u), p(v))
+
+    def get_field(self):
+        u = np.zeros(self.nnode * self.len)
+        reference().ref_sys_get_field(self.h, u.ctypes.data_as(C.c_void_p))
+        return u
+
+    def set_bc(self, flags):
+        flags = _i32(flags)
+        reference().ref_sys_set_bc(self.h, flags.ctypes.data_as(C.c_void_p))
+
+    def assemble(self, kind, group_params, glob=(0.0, 0.0)):
+        gp = _f64(group_params)
+        if gp.ndim == 1:
+            gp = gp[None, :]
+        assert gp.shape[0] == self.ngroup
+        glob = _f64(glob)
+        rn = C.c_double(0.0)
+        rc = reference().ref_sys_assemble(self.h, kind, gp.ctypes.data_as(C.c_void_p), gp.shape[1],
+                                          glob.ctypes.data_as(C.c_void_p), C.byref(rn))
+        assert rc == 0, rc
+        return rn.value
+
+    def matrix(self):
+        bs = self.len * self.len
+        crs = np.zeros(self.ncrs * bs)
+        dia = np.zeros(self.nnode * bs)
+        reference().ref_sys_get_matrix(self.h, crs.ctypes.data_as(C.c_void_p), dia.ctypes.data_as(C.c_void_p))
+        return crs, dia
+
+    def vector(self, iv):
+        out = np.zeros(self.nnode * self.len)
+        reference().ref_sys_get_vector(self.h, C.c_int(iv), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def set_vector(self, iv, vals):
+        vals = _f64(vals)
+        reference().ref_sys_set_vector(self.h, C.c_int(iv), vals.ctypes.data_as(C.c_void_p))
+
+    def matvec(self, alpha, x, beta, y):
+        x, y = _f64(x), _f64(y).copy()
+        reference().ref_sys_matvec(self.h, C.c_double(alpha), x.ctypes.data_as(C.c_void_p), C.c_double(beta),
+                                   y.ctypes.data_as(C.c_void_p))
+        return y
+
+    def ilu0(self, order=None):
+        bs = self.len * self.len
+        rp = np.zeros(self.nnode + 1, np.uint32)
+        ci = np.zeros(max(self.ncrs, 1), np.uint32)
+        crs = np.zeros(max(self.ncrs, 1) * bs)
+        dia = np.zeros(self.nnode * bs)
+        order = None if order is None else _i32(order)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        ok = reference().ref_ilu0_factor(self.h, p(order), p(rp), p(ci), p(crs), p(dia))
+        return bool(ok), rp, ci[: self.ncrs], crs[: self.ncrs * bs], dia
+
+    def ilu_apply(self, r, lev=0, order=None):
+        r = _f64(r)
+        z = np.zeros_like(r)
+        order = None if order is None else _i32(order)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        ok = reference().ref_ilu_apply(self.h, C.c_int(lev), p(order), p(r), p(z))
+        assert ok
+        return z
+
+    def solve(self, method="pcg", tol=1e-10, max_iter=1000, lev=0, order=None):
+        conv = C.c_double(tol)
+        it = C.c_uint(max_iter)
+        order = None if order is None else _i32(order)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        ok = reference().ref_sys_solve(self.h, C.c_int(0 if method == "pcg" else 1), C.c_int(lev), p(order),
+                                       C.byref(conv), C.byref(it))
+        return ok, conv.value, it.value
+
+    def update_field(self):
+        reference().ref_sys_update_field(self.h)

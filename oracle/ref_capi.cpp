@@ -1,0 +1,325 @@
+// Test-infrastructure driver: a C interface over the *reference* DelFEM (compiled from its own sources by
+// oracle/Makefile into oracle/_ref/libdfm_ref.so). It exists only so that tests can pin the plain-C
+// restatement (oracle/cpu_ref.c) against the real reference. Nothing in the product links it.
+//
+// It drives the reference exactly through its public API, in the order of the reference's own low-level
+// usage (test_glut_lowlev/solid2d/main.cpp:183-298 there):
+//   CFieldWorld::AddMesh(IMesh) -> MakeField_FieldElemDim -> CLinearSystem_Field::AddPattern_Field
+//   -> CBCFlag::SetBC -> InitializeMarge -> Fem::Eqn::AddLinSys_* -> FinalizeMarge
+//   -> CPreconditioner_ILU::SetLinearSystem/SetValue -> LsSol::Solve_PCG / Solve_PBiCGSTAB.
+// Per-element materials are expressed with one element array per material group, which the reference
+// supports natively (one (lambda,mu) or alpha per element array, include/delfem/eqnsys_solid.h:155-245).
+
+#include <cassert>
+#include <cstring>
+#include <vector>
+
+#include "delfem/field_world.h"
+#include "delfem/field.h"
+#include "delfem/mesh_interface.h"
+#include "delfem/femls/linearsystem_field.h"
+#include "delfem/ls/linearsystem.h"
+#include "delfem/ls/preconditioner.h"
+#include "delfem/ls/solver_ls_iter.h"
+#include "delfem/matvec/matdia_blkcrs.h"
+#include "delfem/matvec/matdiafrac_blkcrs.h"
+#include "delfem/matvec/ordering_blk.h"
+#include "delfem/matvec/vector_blk.h"
+#include "delfem/matvec/bcflag_blk.h"
+#include "delfem/femeqn/eqn_poisson.h"
+#include "delfem/femeqn/eqn_diffusion.h"
+#include "delfem/femeqn/eqn_linear_solid2d.h"
+#include "delfem/femeqn/eqn_linear_solid3d.h"
+#include "delfem/femeqn/eqn_st_venant.h"
+
+namespace {
+
+// IMesh adapter: one element array per material group, element order preserved inside each group.
+class CGroupMesh : public Msh::IMesh {
+ public:
+  unsigned int ndim;
+  std::vector<double> coord;
+  std::vector<std::vector<int> > groups;  // lnods per group
+  Msh::MSH_TYPE type;
+  virtual unsigned int GetDimention() const { return ndim; }
+  virtual void GetInfo(unsigned int id_msh, unsigned int& id_cad, unsigned int& id_before_ext,
+                       unsigned int& inum_ext, int& ilayer) const {
+    id_cad = id_msh; id_before_ext = 0; inum_ext = 0; ilayer = 0;
+  }
+  virtual void GetCoord(std::vector<double>& c) const { c = coord; }
+  virtual Msh::MSH_TYPE GetConnectivity(unsigned int id_msh, std::vector<int>& lnods) const {
+    lnods = groups[id_msh - 1];
+    return type;
+  }
+  virtual std::vector<unsigned int> GetAry_ID() const {
+    std::vector<unsigned int> ids;
+    for (unsigned int i = 0; i < groups.size(); i++) ids.push_back(i + 1);
+    return ids;
+  }
+  virtual std::vector<unsigned int> GetIncludeElemIDAry(unsigned int) const {
+    return std::vector<unsigned int>();
+  }
+};
+
+struct RefSys {
+  Fem::Field::CFieldWorld world;
+  Fem::Ls::CLinearSystem_Field ls;
+  CGroupMesh mesh;
+  unsigned int id_base, id_field;
+  std::vector<unsigned int> id_ea;       // per group
+  std::vector<unsigned int> id_partial;  // per group partial field (3D linear solid takes a field)
+  int ndim, nnode, len;
+};
+
+MatVec::CMatDia_BlkCrs& Mat(RefSys* s) { return s->ls.m_ls.GetMatrix(0); }
+
+}  // namespace
+
+extern "C" {
+
+// kind of field: len = 1 (scalar) or ndim (vector). Element groups are 0..ngroup-1 per element.
+void* ref_sys_create(int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                     int ngroup, const int* elem_group, int len) {
+  RefSys* s = new RefSys;
+  s->ndim = ndim; s->nnode = nnode; s->len = len;
+  const int npoel = (ndim == 2) ? 3 : 4;
+  s->mesh.ndim = ndim;
+  s->mesh.type = (ndim == 2) ? Msh::TRI : Msh::TET;
+  s->mesh.coord.assign(coords, coords + (size_t)nnode * ndim);
+  s->mesh.groups.resize(ngroup);
+  for (int ie = 0; ie < nelem; ie++) {
+    const int g = elem_group ? elem_group[ie] : 0;
+    for (int k = 0; k < npoel; k++) s->mesh.groups[g].push_back(lnods[(size_t)ie * npoel + k]);
+  }
+  s->id_base = s->world.AddMesh(s->mesh);
+  Fem::Field::FIELD_TYPE ft = Fem::Field::SCALAR;
+  if (len == 2) ft = Fem::Field::VECTOR2;
+  if (len == 3) ft = Fem::Field::VECTOR3;
+  s->id_field = s->world.MakeField_FieldElemDim(s->id_base, ndim, ft,
+                                                Fem::Field::VALUE | Fem::Field::VELOCITY,
+                                                Fem::Field::CORNER);
+  const Fem::Field::CField& field = s->world.GetField(s->id_field);
+  s->id_ea = field.GetAryIdEA();
+  assert((int)s->id_ea.size() == ngroup);
+  s->ls.AddPattern_Field(s->id_field, s->world);
+  if (ndim == 3 && len == 3 && ngroup > 1) {
+    for (int g = 0; g < ngroup; g++) {
+      s->id_partial.push_back(s->world.GetPartialField(s->id_field, s->id_ea[g]));
+    }
+  }
+  return s;
+}
+
+void ref_sys_destroy(void* h) { delete (RefSys*)h; }
+
+int ref_sys_ncrs(void* h) { return (int)Mat((RefSys*)h).NCrs(); }
+
+void ref_sys_pattern(void* h, unsigned int* rowptr, unsigned int* colind) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CMatDia_BlkCrs& m = Mat(s);
+  unsigned int pos = 0;
+  rowptr[0] = 0;
+  for (int i = 0; i < s->nnode; i++) {
+    unsigned int np = 0;
+    const unsigned int* cols = m.GetPtrIndPSuP(i, np);
+    for (unsigned int k = 0; k < np; k++) colind[pos + k] = cols[k];
+    pos += np;
+    rowptr[i + 1] = pos;
+  }
+}
+
+// nodal field values: u = VALUE (n*len), v = VELOCITY (n*len) or NULL
+void ref_sys_set_field(void* h, const double* u, const double* v) {
+  RefSys* s = (RefSys*)h;
+  Fem::Field::CField& field = s->world.GetField(s->id_field);
+  Fem::Field::CNodeAry::CNodeSeg& ns_u =
+      field.GetNodeSeg(Fem::Field::CORNER, true, s->world, Fem::Field::VALUE);
+  Fem::Field::CNodeAry::CNodeSeg& ns_v =
+      field.GetNodeSeg(Fem::Field::CORNER, true, s->world, Fem::Field::VELOCITY);
+  for (int i = 0; i < s->nnode; i++) {
+    for (int d = 0; d < s->len; d++) {
+      ns_u.SetValue(i, d, u ? u[(size_t)i * s->len + d] : 0.0);
+      ns_v.SetValue(i, d, v ? v[(size_t)i * s->len + d] : 0.0);
+    }
+  }
+}
+
+void ref_sys_get_field(void* h, double* u) {
+  RefSys* s = (RefSys*)h;
+  Fem::Field::CField& field = s->world.GetField(s->id_field);
+  const Fem::Field::CNodeAry::CNodeSeg& ns_u =
+      field.GetNodeSeg(Fem::Field::CORNER, true, s->world, Fem::Field::VALUE);
+  for (int i = 0; i < s->nnode; i++) ns_u.GetValue(i, &u[(size_t)i * s->len]);
+}
+
+void ref_sys_set_bc(void* h, const int* flags) {
+  RefSys* s = (RefSys*)h;
+  s->ls.m_ls.ClearFixedBoundaryCondition();
+  MatVec::CBCFlag& bc = s->ls.m_ls.GetBCFlag(0);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++)
+      if (flags[(size_t)i * s->len + d]) bc.SetBC(i, d);
+}
+
+// kind: 0 Poisson (alpha, source), 1 LinearSolid2D (lambda, myu, rho, gx, gy),
+//       2 LinearSolid3D (lambda, myu, rho, gx, gy, gz), 3 StVenant2D (lambda, myu, rho, gx, gy),
+//       4 StVenant3D (lambda, myu, rho, gx, gy, gz), 5 Diffusion (rho, alpha, source) with glob = (dt, gamma)
+// params: per group, npar doubles each. Returns FinalizeMarge's residual norm in *res_norm.
+int ref_sys_assemble(void* h, int kind, const double* params, int npar, const double* glob,
+                     double* res_norm) {
+  RefSys* s = (RefSys*)h;
+  s->ls.InitializeMarge();
+  const int ngroup = (int)s->id_ea.size();
+  for (int g = 0; g < ngroup; g++) {
+    const double* p = &params[(size_t)g * npar];
+    const unsigned int id_ea = s->id_ea[g];
+    bool ok = true;
+    switch (kind) {
+      case 0: ok = Fem::Eqn::AddLinSys_Poisson(s->ls, p[0], p[1], s->world, s->id_field, id_ea); break;
+      case 1:
+        ok = Fem::Eqn::AddLinSys_LinearSolid2D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], s->world,
+                                                      s->id_field, id_ea);
+        break;
+      case 2: {
+        const unsigned int idf = s->id_partial.empty() ? s->id_field : s->id_partial[g];
+        ok = Fem::Eqn::AddLinSys_LinearSolid3D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], p[5], s->world, idf);
+        if (s->id_partial.empty()) g = ngroup;  // one call covers every element array
+        break;
+      }
+      case 3:
+        ok = Fem::Eqn::AddLinSys_StVenant2D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], s->world,
+                                                   s->id_field, id_ea);
+        break;
+      case 4:
+        ok = Fem::Eqn::AddLinSys_StVenant3D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], p[5], s->world,
+                                                   s->id_field, id_ea);
+        break;
+      case 5:
+        ok = Fem::Eqn::AddLinSys_Diffusion(glob[0], glob[1], s->ls, p[0], p[1], p[2], s->world,
+                                           s->id_field, id_ea);
+        break;
+      default: return -1;
+    }
+    if (!ok) return -2;
+  }
+  *res_norm = s->ls.FinalizeMarge();
+  return 0;
+}
+
+// Matrix values: crs[ncrs*len*len] row-major blocks in CRS order, dia[n*len*len].
+void ref_sys_get_matrix(void* h, double* crs, double* dia) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CMatDia_BlkCrs& m = Mat(s);
+  const int bs = s->len * s->len;
+  size_t pos = 0;
+  for (int i = 0; i < s->nnode; i++) {
+    unsigned int np = 0;
+    const double* v = m.GetPtrValPSuP(i, np);
+    if (np) std::memcpy(&crs[pos * bs], v, sizeof(double) * np * bs);
+    pos += np;
+    std::memcpy(&dia[(size_t)i * bs], m.GetPtrValDia(i), sizeof(double) * bs);
+  }
+}
+
+// iv: -1 residual, -2 update
+void ref_sys_get_vector(void* h, int iv, double* out) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CVector_Blk& v = s->ls.m_ls.GetVector(iv, 0);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++) out[(size_t)i * s->len + d] = v.GetValue(i, d);
+}
+
+void ref_sys_set_vector(void* h, int iv, const double* in) {
+  RefSys* s = (RefSys*)h;
+  MatVec::CVector_Blk& v = s->ls.m_ls.GetVector(iv, 0);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++) v.SetValue(i, d, in[(size_t)i * s->len + d]);
+}
+
+// y = alpha*A*x + beta*y through CLinearSystem::MATVEC on handles -1 (x) and -2 (y).
+void ref_sys_matvec(void* h, double alpha, const double* x, double beta, double* y) {
+  RefSys* s = (RefSys*)h;
+  ref_sys_set_vector(h, -1, x);
+  ref_sys_set_vector(h, -2, y);
+  s->ls.m_ls.MATVEC(alpha, -1, beta, -2);
+  ref_sys_get_vector(h, -2, y);
+}
+
+// ILU(0) numeric factor of the current matrix, optionally under an ordering (new->old, as in
+// CPreconditioner_ILU::SetOrdering). Writes the factor pattern (rowptr/colind, in the new numbering) and
+// values (L unscaled, D^-1, D^-1 U). Returns DoILUDecomp's result (1 ok, 0 failed).
+int ref_ilu0_factor(void* h, const int* order, unsigned int* rowptr, unsigned int* colind, double* crs,
+                    double* dia) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CMatDia_BlkCrs& m = Mat(s);
+  MatVec::CMatDiaFrac_BlkCrs* f = 0;
+  MatVec::COrdering_Blk ord;
+  if (order) {
+    std::vector<int> o(order, order + s->nnode);
+    ord.SetOrdering(o);
+    f = new MatVec::CMatDiaFrac_BlkCrs(0, m, ord);
+    f->SetValue_Initialize(m, ord);
+  } else {
+    f = new MatVec::CMatDiaFrac_BlkCrs(0, m);
+    f->SetValue_Initialize(m);
+  }
+  const bool ok = f->DoILUDecomp();
+  const int bs = s->len * s->len;
+  size_t pos = 0;
+  rowptr[0] = 0;
+  for (int i = 0; i < s->nnode; i++) {
+    unsigned int np = 0;
+    const unsigned int* cols = f->GetPtrIndPSuP(i, np);
+    const double* v = f->GetPtrValPSuP(i, np);
+    for (unsigned int k = 0; k < np; k++) colind[pos + k] = cols[k];
+    if (np) std::memcpy(&crs[pos * bs], v, sizeof(double) * np * bs);
+    pos += np;
+    rowptr[i + 1] = (unsigned int)pos;
+    std::memcpy(&dia[(size_t)i * bs], f->GetPtrValDia(i), sizeof(double) * bs);
+  }
+  delete f;
+  return ok ? 1 : 0;
+}
+
+// z = M^-1 r with the reference's CPreconditioner_ILU (ILU(lev) or ordered ILU(0)).
+int ref_ilu_apply(void* h, int lev, const int* order, const double* r, double* z) {
+  RefSys* s = (RefSys*)h;
+  LsSol::CPreconditioner_ILU prec;
+  prec.SetFillInLevel(lev);
+  if (order) prec.SetOrdering(std::vector<int>(order, order + s->nnode));
+  prec.SetLinearSystem(s->ls.m_ls);
+  if (!prec.SetValue(s->ls.m_ls)) return 0;
+  if (s->ls.m_ls.GetTmpVectorArySize() < 1) s->ls.m_ls.ReSizeTmpVecSolver(1);
+  MatVec::CVector_Blk& v = s->ls.m_ls.GetVector(0, 0);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++) v.SetValue(i, d, r[(size_t)i * s->len + d]);
+  prec.SolvePrecond(s->ls.m_ls, 0);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++) z[(size_t)i * s->len + d] = v.GetValue(i, d);
+  return 1;
+}
+
+// Solve with the reference's Krylov loops on the assembled system: method 0 Solve_PCG, 1 Solve_PBiCGSTAB.
+// lev = ILU fill level (ignored with an ordering). Returns the solver's bool; conv/iter are in/out as
+// in the reference. The update (handle -2) is left in the system (read it with ref_sys_get_vector).
+int ref_sys_solve(void* h, int method, int lev, const int* order, double* conv_ratio, unsigned int* iter) {
+  RefSys* s = (RefSys*)h;
+  LsSol::CPreconditioner_ILU prec;
+  prec.SetFillInLevel(lev);
+  if (order) prec.SetOrdering(std::vector<int>(order, order + s->nnode));
+  prec.SetLinearSystem(s->ls.m_ls);
+  if (!prec.SetValue(s->ls.m_ls)) return -1;
+  LsSol::CLinearSystemPreconditioner lsp(s->ls.m_ls, prec);
+  bool ok = false;
+  if (method == 0) ok = LsSol::Solve_PCG(*conv_ratio, *iter, lsp);
+  else ok = LsSol::Solve_PBiCGSTAB(*conv_ratio, *iter, lsp);
+  return ok ? 1 : 0;
+}
+
+// u_field += update (CLinearSystem_Field::UpdateValueOfField, VALUE)
+void ref_sys_update_field(void* h) {
+  RefSys* s = (RefSys*)h;
+  s->ls.UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+"""Pins the oracle's restatement (oracle/cpu_ref.c) against the reference DelFEM itself, compiled from its
+own sources into oracle/_ref/ (make -C oracle ref). Bit-exact comparisons throughout: same inputs, same
+arithmetic order. Skipped where the reference sources are not present (the GPU host)."""
+import numpy as np
+import pytest
+
+import meshgen as M
+from oracle import pyref as P
+
+pytestmark = pytest.mark.skipif(not P.ref_available(), reason="oracle/_ref not built (reference sources absent)")
+
+LAM2, MU2 = 1.0 * 0.3 / (1 - 0.3 ** 2), 1.0 / (2 * 1.3)  # plane stress E=1, nu=0.3
+LAM3, MU3 = 0.3 / (1.3 * 0.4), 1.0 / 2.6
+
+
+def mesh2d(nx=8, ny=4, seed=11):
+    return M.rect_tri(nx, ny, 1.0, 0.2 * ny / 4, seed)
+
+
+def mesh3d(nx=3, ny=2, nz=2, seed=12):
+    return M.box_tet(nx, ny, nz, 1.0, 0.7, 0.6, seed)
+
+
+def clamp_flags(coords, ndim, length, axis=0):
+    x = coords.reshape(-1, ndim)[:, axis]
+    f = np.zeros((x.size, length), np.int32)
+    f[x == x.min()] = 1
+    return f.ravel()
+
+
+def both(ndim, coords, lnods, length, groups=None):
+    return P.RefSystem(ndim, coords, lnods, length, groups), P.CpuSystem(ndim, coords, lnods, length)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_pattern_bit_exact(ndim):
+    co, ln = mesh2d(11, 7) if ndim == 2 else mesh3d(4, 3, 3)
+    ref, cpu = both(ndim, co, ln, 1)
+    rp, ci = ref.pattern()
+    assert np.array_equal(rp, cpu.rowptr) and np.array_equal(ci, cpu.colind)
+    # diagonal excluded, rows ascending (src/femfield/elem_ary.cpp:838, matdiafrac_blkcrs.cpp:195-207)
+    for i in range(cpu.nnode):
+        row = cpu.colind[cpu.rowptr[i]:cpu.rowptr[i + 1]]
+        assert i not in row and np.all(np.diff(row.astype(np.int64)) > 0)
+
+
+def _rand_u(n, length, seed, scale):
+    return np.random.default_rng(seed).uniform(-scale, scale, n * length)
+
+
+CASES = [
+    # (name, ndim, len, kind, params(per element row), uses u, uses v)
+    ("poisson2d", 2, 1, P.POISSON, lambda r: [r.uniform(0.5, 2.0), 1.0], True, False),
+    ("poisson3d", 3, 1, P.POISSON, lambda r: [r.uniform(0.5, 2.0), 0.7], True, False),
+    ("diffusion2d", 2, 1, P.DIFFUSION2D, lambda r: [1.1, r.uniform(0.5, 2.0), 0.3], True, True),
+    ("solid2d", 2, 2, P.SOLID2D, lambda r: [LAM2 * r.uniform(0.5, 2), MU2, 1.0, 0.0, -0.01], True, False),
+    ("solid3d", 3, 3, P.SOLID3D, lambda r: [LAM3, MU3 * r.uniform(0.5, 2), 1.0, 0.01, 0.0, -0.02], True, False),
+    ("stvk2d", 2, 2, P.STVK2D, lambda r: [LAM2, MU2 * r.uniform(0.5, 2), 1.0, 0.0, -0.05], True, False),
+    ("stvk3d", 3, 3, P.STVK3D, lambda r: [LAM3 * r.uniform(0.5, 2), MU3, 1.0, 0.0, 0.0, -0.05], True, False),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
+def test_assembly_bit_exact(case, per_element):
+    name, ndim, length, kind, pfun, use_u, use_v = case
+    co, ln = mesh2d(4, 2) if ndim == 2 else mesh3d(2, 2, 1)
+    if not per_element:
+        co, ln = mesh2d(9, 5) if ndim == 2 else mesh3d(3, 3, 2)
+    npoel = 3 if ndim == 2 else 4
+    nelem = ln.size // npoel
+    n = co.size // ndim
+    rng = np.random.default_rng(5)
+    if per_element:
+        params = np.array([pfun(rng) for _ in range(nelem)])
+        groups = np.arange(nelem, dtype=np.int32)  # one element array per element, in element order
+    else:
+        params = np.array(pfun(rng))
+        groups = None
+    ref, cpu = both(ndim, co, ln, length, groups)
+    u = _rand_u(n, length, 7, 0.05) if use_u else None
+    v = _rand_u(n, length, 8, 0.05) if use_v else None
+    ref.set_field(u, v)
+    flags = clamp_flags(co, ndim, length)
+    ref.set_bc(flags)
+    glob = (0.05, 0.6)
+    rn = ref.assemble(kind, params if per_element else params[None, :], glob)
+    cpu.assemble(kind, params, u, v, glob)
+    rn2 = cpu.set_bc(flags)
+    crs, dia = ref.matrix()
+    assert np.array_equal(crs, cpu.crs), np.abs(crs - cpu.crs).max()
+    assert np.array_equal(dia, cpu.dia), np.abs(dia - cpu.dia).max()
+    assert np.array_equal(ref.vector(-1), cpu.res)
+    assert rn == rn2
+
+
+@pytest.mark.parametrize("length", [1, 2, 3])
+def test_matvec_bit_exact(length):
+    ndim = 2 if length != 3 else 3
+    co, ln = mesh2d(7, 5) if ndim == 2 else mesh3d(3, 2, 2)
+    ref, cpu = both(ndim, co, ln, length)
+    kind = {1: P.POISSON, 2: P.SOLID2D, 3: P.SOLID3D}[length]
+    par = {1: [1.0, 1.0], 2: [LAM2, MU2, 1.0, 0.0, -0.01], 3: [LAM3, MU3, 1.0, 0.0, 0.0, -0.01]}[length]
+    flags = clamp_flags(co, ndim, length)
+    ref.set_bc(flags)
+    ref.assemble(kind, np.array(par)[None, :])
+    cpu.assemble(kind, par)
+    cpu.set_bc(flags)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=cpu.nnode * length)
+    y = rng.normal(size=cpu.nnode * length)
+    for alpha, beta in [(1.0, 0.0), (-0.7, 1.3)]:
+        assert np.array_equal(ref.matvec(alpha, x, beta, y), cpu.matvec(alpha, x, beta, y))
+
+
+def _assembled(ndim, length, nxyz, seed=21):
+    co, ln = mesh2d(*nxyz) if ndim == 2 else mesh3d(*nxyz)
+    ref, cpu = both(ndim, co, ln, length)
+    kind = {1: P.POISSON, 2: P.SOLID2D, 3: P.SOLID3D}[length]
+    par = {1: [1.3, 1.0], 2: [LAM2, MU2, 1.0, 0.0, -0.01], 3: [LAM3, MU3, 1.0, 0.0, 0.0, -0.01]}[length]
+    flags = clamp_flags(co, ndim, length)
+    ref.set_bc(flags)
+    ref.assemble(kind, np.array(par)[None, :])
+    cpu.assemble(kind, par)
+    cpu.set_bc(flags)
+    return ref, cpu
+
+
+ILU_CASES = [(2, 1, (9, 6)), (2, 2, (9, 4)), (3, 3, (3, 3, 2)), (3, 1, (3, 3, 3))]
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
+@pytest.mark.parametrize("ordered", [False, True], ids=["natural", "ordered"])
+def test_ilu0_bit_exact(ndim, length, nxyz, ordered):
+    ref, cpu = _assembled(ndim, length, nxyz)
+    order = np.random.default_rng(4).permutation(cpu.nnode).astype(np.int32) if ordered else None
+    ok, rp, ci, crs, dia = ref.ilu0(order)
+    f = cpu.ilu0(order)
+    assert ok and f["ok"]
+    assert np.array_equal(rp, f["frowptr"]) and np.array_equal(ci, f["fcolind"])
+    assert np.array_equal(crs, f["fcrs"]) and np.array_equal(dia, f["fdia"])
+    r = np.random.default_rng(9).normal(size=cpu.nnode * length)
+    assert np.array_equal(ref.ilu_apply(r, 0, order), cpu.ilu_apply(f, r))
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
+@pytest.mark.parametrize("method", ["pcg", "pbicgstab"])
+@pytest.mark.parametrize("ordered", [False, True], ids=["natural", "ordered"])
+def test_krylov_bit_exact(ndim, length, nxyz, method, ordered):
+    ref, cpu = _assembled(ndim, length, nxyz)
+    order = np.random.default_rng(4).permutation(cpu.nnode).astype(np.int32) if ordered else None
+    f = cpu.ilu0(order)
+    ok, conv, it, x, _ = cpu.solve(f, method, 1e-10, 1000)
+    rok, rconv, rit = ref.solve(method, 1e-10, 1000, 0, order)
+    assert (ok, conv, it) == (bool(rok), rconv, rit)
+    assert np.array_equal(ref.vector(-2), x)
+
+
+def test_pcg_nonconvergence_matches():
+    """max_iter too small: Solve_PCG returns false, leaves iteration at max, reports the ratio."""
+    ref, cpu = _assembled(2, 1, (9, 6))
+    f = cpu.ilu0()
+    ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-12, 2)
+    rok, rconv, rit = ref.solve("pcg", 1e-12, 2, 0, None)
+    assert not ok and rok == 0 and it == rit == 2 and conv == rconv
