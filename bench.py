@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""DOF-solves/s at 1e-8 relative residual, mixed precision vs FP64, with p50/p99 solve latency, on one GPU.
+
+Workload (default): 2D Poisson/diffusion, P1 triangles on the unit square 1024 x 1024 (1,050,625 nodes),
+seeded +-10 % node jitter, per-element conductivity lognormal(0,1), f = 1, u = 0 on the boundary; synthetic
+inputs generated on the host and resident in HBM before timing. One step = one solve of the hot path:
+assembly -> Dirichlet rows/columns -> ILU(0) numeric factorisation -> PCG to a 1e-8 true relative residual.
+`value` is the mixed-precision path (FP32 matrix/factor/SpMV/ILU inside FP64 iterative refinement on the
+true residual); the plain FP64 path is timed beside it on the same system.
+
+The roofline object is for the dominant kernel of the mixed path's inner loop, the fused block-CRS SpMV +
+dot kernel, timed with HIP events on the library's stream. The CPU baseline is the oracle's single-threaded C
+restatement of the reference, timed on a bounded sample of the same workload (see its `sample` field).
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOF-solves/s at 1e-8 rel. residual, mixed vs FP64; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, gpu_iters, budget_s):
+    """The oracle (plain-C port of the reference arithmetic, one thread) on the full mesh: setup
+    (pattern, assembly, Dirichlet, ILU(0) in the reference's natural order) and a bounded number of PCG
+    iterations; the solve time is extrapolated with the iteration count of the GPU FP64 solve."""
+    import numpy as np
+
+    from oracle import pyref as P
+
+    t0 = time.perf_counter()
+    cpu = P.CpuSystem(w["ndim"], w["coords"], w["lnods"], w["len"])
+    cpu.assemble(w["kind"], w["params"])
+    cpu.set_bc(w["bc"])
+    t1 = time.perf_counter()
+    f = cpu.ilu0()
+    t2 = time.perf_counter()
+    k1 = 5
+    cpu.solve(f, "pcg", 1e-30, k1)
+    t3 = time.perf_counter()
+    per = (t3 - t2) / k1
+    k2 = max(k1 + 5, min(200, int(budget_s / max(per, 1e-6))))
+    cpu.solve(f, "pcg", 1e-30, k2)
+    t4 = time.perf_counter()
+    t_iter = ((t4 - t3) - (t3 - t2)) / (k2 - k1)
+    t_pcg0 = (t3 - t2) - k1 * t_iter
+    total = (t1 - t0) + (t2 - t1) + t_pcg0 + gpu_iters * t_iter
+    ndof = cpu.nnode * cpu.len
+    return {
+        "value": ndof / total,
+        "unit": "DOF-solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{w['name']} full mesh ({ndof} dofs): pattern+assembly+BC {t1 - t0:.2f}s, ILU(0) natural "
+                   f"{t2 - t1:.2f}s, {k2} PCG iterations at {t_iter * 1e3:.1f} ms/iter; solve extrapolated to "
+                   f"{gpu_iters} iterations (GPU FP64 count) -> {total:.1f}s per solve"),
+        "ms_per_pcg_iter": t_iter * 1e3,
+        "solve_s_extrapolated": total,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--ordering", default="blockcolor", choices=["natural", "multicolor", "blockcolor"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--max-iter", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: every GPU box of this build has one; --gpus {args.gpus} is not "
+              f"supported (requests and batches shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from delfem_amd import _native as N
+    from delfem_amd import workloads as W
+
+    t0 = time.perf_counter()
+    w = W.make(args.config)
+    ctx = N.Context(0)
+    order_kind = {"natural": N.ORDER_NATURAL, "multicolor": N.ORDER_MULTICOLOR,
+                  "blockcolor": N.ORDER_BLOCKCOLOR}[args.ordering]
+    s = W.build_system(ctx, w, order_kind)
+    s.set_params(w["params"])
+    s.set_loads(w["faces"], w["trac"])
+    nph = s.schedule_info()
+    ndof = s.ndof
+    log(f"[bench] {w['name']} ndof={ndof} ncrs={s.ncrs} ordering={args.ordering} phases={nph} "
+        f"setup {time.perf_counter() - t0:.1f}s")
+    tol = w["tol"]
+
+    def step(precision):
+        ctx.synchronize()
+        a = time.perf_counter()
+        s.assemble_resident(w["kind"], norm=False)
+        s.factor()
+        ok, conv, it = s.solve_pcg(tol, args.max_iter, precision)
+        ctx.synchronize()
+        b = time.perf_counter()
+        return b - a, ok, conv, it
+
+    res = {}
+    for prec, name in [(N.MIXED, "mixed"), (N.FP64, "fp64")]:
+        for _ in range(args.warmup):
+            step(prec)
+        lat, its = [], []
+        ok_all = True
+        conv = 0.0
+        for k in range(args.steps):
+            dt, ok, conv, it = step(prec)
+            lat.append(dt)
+            its.append(it)
+            ok_all &= ok
+        true_rel = s.true_residual()
+        st = s.last_solve_stats()
+        res[name] = dict(lat=lat, its=its, ok=ok_all, conv=conv, true_rel=true_rel, stats=st)
+        log(f"[bench] {name}: {statistics.median(lat) * 1e3:.1f} ms/solve, iters {its[-1]}, conv {conv:.2e}, "
+            f"true rel residual {true_rel:.2e}, solve {st['ms']:.1f} ms, {st['ms_per_iter'] * 1e3:.1f} us/iter")
+
+    # dominant kernel of the mixed path: fused SpMV + dot in FP32 (timed with HIP events on the lib's stream)
+    spmv32_ms, spmv32_b = s.bench_kernel(1, 50)
+    spmv64_ms, spmv64_b = s.bench_kernel(0, 50)
+    ilu32_ms, ilu32_b = s.bench_kernel(3, 20)
+    ilu64_ms, ilu64_b = s.bench_kernel(2, 20)
+    achieved = spmv32_b / (spmv32_ms * 1e-3) / 1e9
+    log(f"[bench] spmv fp32 {spmv32_ms * 1e3:.1f} us ({achieved:.0f} GB/s), spmv fp64 {spmv64_ms * 1e3:.1f} us "
+        f"({spmv64_b / spmv64_ms / 1e6:.0f} GB/s), ilu fp32 {ilu32_ms * 1e3:.1f} us "
+        f"({ilu32_b / ilu32_ms / 1e6:.0f} GB/s), ilu fp64 {ilu64_ms * 1e3:.1f} us ({ilu64_b / ilu64_ms / 1e6:.0f} GB/s)")
+
+    mixed, fp64 = res["mixed"], res["fp64"]
+    K = args.steps
+    t_mixed = sum(mixed["lat"])
+    t_fp64 = sum(fp64["lat"])
+    value = ndof * K / t_mixed
+    value64 = ndof * K / t_fp64
+    lat_sorted = sorted(mixed["lat"])
+
+    def pct(v, q):
+        if len(v) == 1:
+            return v[0]
+        i = q * (len(v) - 1)
+        lo = int(i)
+        hi = min(lo + 1, len(v) - 1)
+        return v[lo] + (v[hi] - v[lo]) * (i - lo)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{w['name']}: " + {"c1": "2D plane-stress elasticity P1, 128x32 quads",
+                                             "c2": "2D Poisson P1, unit square 1024x1024, lognormal(0,1) conductivity",
+                                             "c3": "3D elasticity P1 tets, 96^3 hexes x 6",
+                                             "c5": "3D StVK cantilever, 64x16x16 hexes x 6 (first Newton step)"}[
+                             w["name"]]),
+            "dofs": ndof,
+            "nnz_blocks_offdiag": s.ncrs,
+            "block": w["len"],
+            "tol_true_rel_residual": tol,
+            "ordering": args.ordering,
+            "ilu_phases_fwd_bwd": [nph[0], nph[1]],
+            "step": "assemble + Dirichlet + ILU(0) factor + PCG solve",
+        },
+        "fp64_value": value64,
+        "mixed_over_fp64": value / value64,
+        "p50_ms": pct(lat_sorted, 0.5) * 1e3,
+        "p99_ms": pct(lat_sorted, 0.99) * 1e3,
+        "fp64_p50_ms": pct(sorted(fp64["lat"]), 0.5) * 1e3,
+        "iters_mixed_inner": mixed["its"][-1],
+        "iters_fp64": fp64["its"][-1],
+        "true_rel_residual_mixed": mixed["true_rel"],
+        "true_rel_residual_fp64": fp64["true_rel"],
+        "converged": bool(mixed["ok"] and fp64["ok"]),
+        "ms_per_pcg_iter_mixed": mixed["stats"]["ms_per_iter"],
+        "ms_per_pcg_iter_fp64": fp64["stats"]["ms_per_iter"],
+        "solve_ms_mixed": mixed["stats"]["ms"],
+        "solve_ms_fp64": fp64["stats"]["ms"],
+        "roofline": {
+            "kernel": "k_pcg_spmv_dot<B=%d,FP32> (fused block-CRS SpMV + dot)" % w["len"],
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": spmv32_ms * 1e3,
+            "algorithmic_bytes": spmv32_b,
+        },
+        "kernels": {
+            "spmv_fp64": {"us": spmv64_ms * 1e3, "GB/s": spmv64_b / spmv64_ms / 1e6, "bytes": spmv64_b},
+            "ilu_apply_fp32": {"us": ilu32_ms * 1e3, "GB/s": ilu32_b / ilu32_ms / 1e6, "bytes": ilu32_b},
+            "ilu_apply_fp64": {"us": ilu64_ms * 1e3, "GB/s": ilu64_b / ilu64_ms / 1e6, "bytes": ilu64_b},
+        },
+    }
+    s.close()
+    ctx.close()
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle port, 1 thread) ...")
+        try:
+            out["cpu_baseline"] = cpu_baseline(w, fp64["its"][-1] + 1, args.cpu_budget)
+            out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        except Exception as e:  # the baseline is reported, never the measured product
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,308 @@
+"""ctypes binding of libdfm_gpu.so (include/dfm_gpu.h). Thin: every call goes straight to the C-ABI.
+
+The product runs only through the HIP library; there is no CPU fallback. Loading fails loudly when the
+library has not been built (python -c "import __graft_entry__; __graft_entry__.build()")."""
+import atexit
+import ctypes as C
+import os
+import weakref
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdfm_gpu.so")
+
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D = range(6)
+ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKCOLOR = range(3)
+FP64, MIXED = 0, 1
+ERR_SINGULAR = -5
+NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3}
+
+_lib = None
+_live = weakref.WeakSet()  # systems and contexts still open, closed in order at interpreter exit
+
+
+@atexit.register
+def _close_all():
+    objs = list(_live)
+    for o in objs:
+        if isinstance(o, System):
+            o.close()
+    for o in objs:
+        if isinstance(o, Context):
+            o.close()
+
+
+class DfmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dfm error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.dfm_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise DfmError(rc, lib().dfm_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def device_bytes():
+    b = C.c_longlong(0)
+    _check(lib().dfm_device_bytes(C.byref(b)))
+    return b.value
+
+
+def device_count():
+    c = C.c_int(0)
+    _check(lib().dfm_device_count(C.byref(c)))
+    return c.value
+
+
+# ---- seeded meshes -----------------------------------------------------------------------------------------
+def rect_tri(nx, ny, lx, ly, seed, jitter=0.1):
+    coords = np.empty((nx + 1) * (ny + 1) * 2)
+    tri = np.empty(nx * ny * 6, dtype=np.int32)
+    _check(lib().dfm_mesh_rect_tri(nx, ny, C.c_double(lx), C.c_double(ly), C.c_ulonglong(seed),
+                                   C.c_double(jitter), _p(coords), _p(tri)))
+    return coords, tri
+
+
+def box_tet(nx, ny, nz, lx, ly, lz, seed, jitter=0.1):
+    coords = np.empty((nx + 1) * (ny + 1) * (nz + 1) * 3)
+    tet = np.empty(nx * ny * nz * 24, dtype=np.int32)
+    _check(lib().dfm_mesh_box_tet(nx, ny, nz, C.c_double(lx), C.c_double(ly), C.c_double(lz), C.c_ulonglong(seed),
+                                  C.c_double(jitter), _p(coords), _p(tet)))
+    return coords, tet
+
+
+def rng_uniform(seed, stream, n):
+    out = np.empty(n)
+    _check(lib().dfm_rng_uniform(C.c_ulonglong(seed), C.c_ulonglong(stream), C.c_size_t(n), _p(out)))
+    return out
+
+
+def rng_normal(seed, stream, n):
+    out = np.empty(n)
+    _check(lib().dfm_rng_normal(C.c_ulonglong(seed), C.c_ulonglong(stream), C.c_size_t(n), _p(out)))
+    return out
+
+
+# ---- context / system ----------------------------------------------------------------------------------------
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().dfm_ctx_create(device, C.byref(self.h)))
+        _live.add(self)
+
+    def synchronize(self):
+        _check(lib().dfm_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().dfm_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class System:
+    """Device linear system: CLinearSystem_Field + CPreconditioner_ILU(0) on one GPU."""
+
+    def __init__(self, ctx, ndim, coords, lnods, length):
+        self.ctx = ctx
+        self.ndim, self.len = ndim, length
+        coords, lnods = _f64(coords), _i32(lnods)
+        self.nnode = coords.size // ndim
+        self.nelem = lnods.size // (ndim + 1)
+        self.h = C.c_void_p()
+        _check(lib().dfm_sys_create_mesh(ctx.h, ndim, self.nnode, _p(coords), self.nelem, _p(lnods), length,
+                                         C.byref(self.h)))
+        n, ln, nc = C.c_int(), C.c_int(), C.c_longlong()
+        _check(lib().dfm_sys_info(self.h, C.byref(n), C.byref(ln), C.byref(nc)))
+        self.ncrs = nc.value
+        _live.add(self)
+
+    def close(self):
+        if self.h:
+            lib().dfm_sys_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ndof(self):
+        return self.nnode * self.len
+
+    def pattern(self):
+        rp = np.empty(self.nnode + 1, np.uint32)
+        ci = np.empty(max(self.ncrs, 1), np.uint32)
+        _check(lib().dfm_sys_get_pattern(self.h, _p(rp), _p(ci)))
+        return rp, ci[: self.ncrs]
+
+    def make_ordering(self, kind, tile=0):
+        order = np.empty(self.nnode, np.int32)
+        nc = C.c_int(0)
+        _check(lib().dfm_sys_make_ordering(self.h, kind, tile, _p(order), C.byref(nc)))
+        return order, nc.value
+
+    def set_ordering(self, order):
+        _check(lib().dfm_sys_set_ordering(self.h, _p(_i32(order))))
+
+    def schedule_info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().dfm_sys_schedule_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_bc(self, flags):
+        _check(lib().dfm_sys_set_bc(self.h, _p(_i32(flags))))
+
+    def set_field(self, u=None, v=None):
+        _check(lib().dfm_sys_set_field(self.h, _p(_f64(u)), _p(_f64(v))))
+
+    def get_field(self):
+        u = np.empty(self.ndof)
+        _check(lib().dfm_sys_get_field(self.h, _p(u)))
+        return u
+
+    def update_field(self):
+        _check(lib().dfm_sys_update_field(self.h))
+
+    def assemble(self, kind, params, glob=None, faces=None, trac=None):
+        params = _f64(params)
+        if params.ndim == 1:
+            npar, pstride = params.size, 0
+        else:
+            npar, pstride = params.shape[1], params.shape[1]
+        glob = _f64(glob) if glob is not None else None
+        faces, trac = _i32(faces), _f64(trac)
+        nface = 0 if faces is None else faces.size // 3
+        rn = C.c_double(0.0)
+        _check(lib().dfm_sys_assemble(self.h, kind, _p(params), npar, pstride, _p(glob), nface, _p(faces), _p(trac),
+                                      C.byref(rn)))
+        return rn.value
+
+    def set_params(self, params):
+        params = _f64(params)
+        npar, pstride = (params.size, 0) if params.ndim == 1 else (params.shape[1], params.shape[1])
+        _check(lib().dfm_sys_set_params(self.h, _p(params), npar, pstride))
+
+    def set_loads(self, faces=None, trac=None):
+        faces, trac = _i32(faces), _f64(trac)
+        nface = 0 if faces is None else faces.size // 3
+        _check(lib().dfm_sys_set_loads(self.h, nface, _p(faces), _p(trac)))
+
+    def assemble_resident(self, kind, glob=None, norm=True):
+        glob = _f64(glob) if glob is not None else None
+        rn = C.c_double(0.0)
+        _check(lib().dfm_sys_assemble_resident(self.h, kind, _p(glob), C.byref(rn) if norm else None))
+        return rn.value
+
+    def bench_kernel(self, which, reps=20):
+        ms, nbytes = C.c_double(), C.c_double()
+        _check(lib().dfm_sys_bench_kernel(self.h, which, reps, C.byref(ms), C.byref(nbytes)))
+        return ms.value, nbytes.value
+
+    def get_matrix(self):
+        bs = self.len * self.len
+        crs = np.empty(max(self.ncrs, 1) * bs)
+        dia = np.empty(self.nnode * bs)
+        _check(lib().dfm_sys_get_matrix(self.h, _p(crs), _p(dia)))
+        return crs[: self.ncrs * bs], dia
+
+    # handle ops (ILinearSystemPreconditioner_Sol)
+    def resize_tmp(self, n):
+        _check(lib().dfm_sys_resize_tmp(self.h, n))
+
+    def dot(self, i, j):
+        out = C.c_double()
+        _check(lib().dfm_sys_dot(self.h, i, j, C.byref(out)))
+        return out.value
+
+    def copy(self, i, j):
+        _check(lib().dfm_sys_copy(self.h, i, j))
+
+    def scal(self, a, i):
+        _check(lib().dfm_sys_scal(self.h, C.c_double(a), i))
+
+    def axpy(self, a, i, j):
+        _check(lib().dfm_sys_axpy(self.h, C.c_double(a), i, j))
+
+    def matvec(self, a, i, b, j):
+        _check(lib().dfm_sys_matvec(self.h, C.c_double(a), i, C.c_double(b), j))
+
+    def get_vector(self, iv):
+        out = np.empty(self.ndof)
+        _check(lib().dfm_sys_get_vector(self.h, iv, _p(out)))
+        return out
+
+    def set_vector(self, iv, vals):
+        _check(lib().dfm_sys_set_vector(self.h, iv, _p(_f64(vals))))
+
+    # preconditioner
+    def factor(self):
+        _check(lib().dfm_prec_factor(self.h))
+
+    def prec_apply(self, iv):
+        _check(lib().dfm_prec_apply(self.h, iv))
+
+    def get_factor(self):
+        bs = self.len * self.len
+        f = {"frowptr": np.empty(self.nnode + 1, np.uint32), "fcolind": np.empty(max(self.ncrs, 1), np.uint32),
+             "diaind": np.empty(self.nnode, np.uint32), "fcrs": np.empty(max(self.ncrs, 1) * bs),
+             "fdia": np.empty(self.nnode * bs)}
+        _check(lib().dfm_prec_get_factor(self.h, _p(f["frowptr"]), _p(f["fcolind"]), _p(f["diaind"]), _p(f["fcrs"]),
+                                         _p(f["fdia"])))
+        f["fcolind"] = f["fcolind"][: self.ncrs]
+        f["fcrs"] = f["fcrs"][: self.ncrs * bs]
+        return f
+
+    # solvers
+    def solve_pcg(self, tol, max_iter, precision=FP64):
+        conv = C.c_double(tol)
+        it = C.c_uint(max_iter)
+        ok = _check(lib().dfm_solve_pcg(self.h, precision, C.byref(conv), C.byref(it)))
+        return bool(ok), conv.value, it.value
+
+    def solve_pbicgstab(self, tol, max_iter):
+        conv = C.c_double(tol)
+        it = C.c_uint(max_iter)
+        ok = _check(lib().dfm_solve_pbicgstab(self.h, C.byref(conv), C.byref(it)))
+        return bool(ok), conv.value, it.value
+
+    def true_residual(self):
+        rel = C.c_double()
+        _check(lib().dfm_sys_true_residual(self.h, C.byref(rel)))
+        return rel.value
+
+    def last_solve_stats(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_uint()
+        _check(lib().dfm_sys_last_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"ms": a.value, "ms_per_iter": b.value, "iters": c.value}

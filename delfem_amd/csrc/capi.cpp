@@ -1,0 +1,761 @@
+// C-ABI of libdfm_gpu.so (include/dfm_gpu.h): host orchestration of the device system.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+#include "dfm_internal.hpp"
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace dfm {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+static std::atomic<long long> g_dev_bytes{0};
+
+hipError_t DevAlloc::alloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) g_dev_bytes += (long long)bytes;
+  return e;
+}
+void DevAlloc::release(void* p) {
+  size_t sz = 0;
+  // the size is recovered from the allocation itself for the accounting
+  if (hipMemPtrGetInfo(p, &sz) == hipSuccess) g_dev_bytes -= (long long)sz;
+  hipFree(p);
+}
+
+long long device_bytes_in_use() { return g_dev_bytes.load(); }
+
+Sys::~Sys() {
+  d_rowptr.release(); d_colind.release(); d_diaind.release(); d_lsplit.release(); d_usplit.release();
+  d_order.release(); d_inv.release(); d_elsup_ptr.release(); d_elsup.release(); d_lnods.release(); d_bc.release();
+  d_coords.release(); d_u.release(); d_v.release();
+  d_crs.release(); d_dia.release(); d_crs32.release(); d_dia32.release();
+  d_fcrs.release(); d_fdia.release(); d_fcrs32.release(); d_fdia32.release();
+  d_res.release(); d_upd.release(); d_rhs.release();
+  for (auto* t : d_tmp) { t->release(); delete t; }
+  d_work.release(); d_work32.release(); d_partial.release(); d_counter.release(); d_scal.release(); d_iflag.release();
+  for (Sweep* w : {&fwd, &bwd}) { w->d_grp.release(); w->d_rowlist.release(); w->d_lev.release(); }
+  d_par.release(); d_lptr.release(); d_loads.release();
+  if (h_pinned) hipHostFree(h_pinned);
+  if (h_pinned_d) hipHostFree(h_pinned_d);
+}
+
+template <typename T>
+static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+  hipError_t e = b.alloc(v.size());
+  if (e != hipSuccess || v.empty()) return e;
+  e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
+// device pattern, ordering maps and sweep schedules for the current order
+static int upload_structure(Sys& s) {
+  hipStream_t st = s.ctx->stream;
+  DFM_HIP(upload(s.d_rowptr, s.rowptr, st));
+  DFM_HIP(upload(s.d_colind, s.colind, st));
+  DFM_HIP(upload(s.d_diaind, s.diaind, st));
+  DFM_HIP(upload(s.d_lsplit, s.lsplit, st));
+  DFM_HIP(upload(s.d_usplit, s.usplit, st));
+  DFM_HIP(upload(s.d_order, s.order, st));
+  DFM_HIP(upload(s.d_inv, s.inv, st));
+  for (Sweep* w : {&s.fwd, &s.bwd}) {
+    DFM_HIP(upload(w->d_grp, w->grp, st));
+    DFM_HIP(upload(w->d_rowlist, w->rowlist, st));
+    DFM_HIP(upload(w->d_lev, w->lev, st));
+  }
+  return DFM_OK;
+}
+
+static int apply_order(Sys& s, const std::vector<int>& order, const std::vector<int>& color,
+                       const std::vector<int>& tile, int ncolors) {
+  s.order = order;
+  s.inv.assign(s.n, 0);
+  for (int i = 0; i < s.n; i++) s.inv[order[i]] = i;
+  s.natural = true;
+  for (int i = 0; i < s.n; i++)
+    if (order[i] != i) { s.natural = false; break; }
+  s.color_of = color;
+  s.tile_of = tile;
+  s.ncolors = ncolors;
+  try {
+    build_device_pattern(s);
+    build_schedule(s);
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+    return DFM_ERR_ARG;
+  }
+  s.assembled = s.factored = s.factored32 = false;
+  return upload_structure(s);
+}
+
+static double* vec_ptr(Sys& s, int iv) {
+  if (iv == -1) return s.d_res.p;
+  if (iv == -2) return s.d_upd.p;
+  if (iv >= 0 && iv < (int)s.d_tmp.size()) return s.d_tmp[iv]->p;
+  return nullptr;
+}
+
+static long long mdof(const Sys& s) { return (long long)s.n * s.len; }
+
+}  // namespace dfm
+
+using namespace dfm;
+
+#define SYS(h) (h ? &h->s : nullptr)
+#define NEED(p, name) DFM_CHECK((p) != nullptr, DFM_ERR_ARG, std::string(name) + ": null argument")
+
+extern "C" {
+
+const char* dfm_last_error(void) { return g_err.c_str(); }
+
+int dfm_device_count(int* count) {
+  NEED(count, "dfm_device_count");
+  int c = 0;
+  DFM_HIP(hipGetDeviceCount(&c));
+  *count = c;
+  return DFM_OK;
+}
+
+int dfm_ctx_create(int device, dfm_ctx* out) {
+  NEED(out, "dfm_ctx_create");
+  DFM_HIP(hipSetDevice(device));
+  dfm_ctx c = new dfm_ctx_s;
+  c->c.device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_error(std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    return DFM_ERR_HIP;
+  }
+  *out = c;
+  return DFM_OK;
+}
+
+static void ctx_release(dfm_ctx ctx) {
+  if (--ctx->c.refs > 0) return;
+  (void)hipStreamSynchronize(ctx->c.stream);
+  (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+// Systems keep their context alive: destroying a context with live systems defers the release.
+int dfm_ctx_destroy(dfm_ctx ctx) {
+  if (!ctx || ctx->c.destroyed) return DFM_OK;
+  ctx->c.destroyed = true;
+  ctx_release(ctx);
+  return DFM_OK;
+}
+
+int dfm_ctx_synchronize(dfm_ctx ctx) {
+  NEED(ctx, "dfm_ctx_synchronize");
+  DFM_HIP(hipStreamSynchronize(ctx->c.stream));
+  return DFM_OK;
+}
+
+int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, int nelem, const int* lnods, int len,
+                        dfm_sys* out) {
+  NEED(ctx, "dfm_sys_create_mesh");
+  NEED(out, "dfm_sys_create_mesh");
+  DFM_CHECK(ndim == 2 || ndim == 3, DFM_ERR_ARG, "dfm_sys_create_mesh: ndim must be 2 or 3");
+  DFM_CHECK(len >= 1 && len <= 3, DFM_ERR_ARG, "dfm_sys_create_mesh: len must be 1, 2 or 3");
+  DFM_CHECK(nnode > 0 && nelem > 0 && coords && lnods, DFM_ERR_ARG, "dfm_sys_create_mesh: empty mesh");
+  const int npoel = ndim + 1;
+  for (long long k = 0; k < (long long)nelem * npoel; k++)
+    DFM_CHECK(lnods[k] >= 0 && lnods[k] < nnode, DFM_ERR_ARG, "dfm_sys_create_mesh: node index out of range");
+  dfm_sys h = new dfm_sys_s;
+  Sys& s = h->s;
+  s.ctx = &ctx->c;
+  s.ndim = ndim; s.n = nnode; s.len = len; s.bs = len * len; s.npoel = npoel; s.nelem = nelem;
+  s.coords.assign(coords, coords + (size_t)nnode * ndim);
+  s.lnods.assign(lnods, lnods + (size_t)nelem * npoel);
+  build_pattern(nnode, nelem, npoel, lnods, s.rowptr0, s.colind0, s.elsup_ptr, s.elsup);
+  s.ncrs = s.rowptr0[nnode];
+  hipStream_t st = s.ctx->stream;
+  const long long m = mdof(s);
+  std::vector<int> ident(nnode);
+  std::iota(ident.begin(), ident.end(), 0);
+  int rc = apply_order(s, ident, {}, {}, 0);
+  if (rc != DFM_OK) { delete h; return rc; }
+  auto fail = [&](hipError_t e) {
+    set_error(std::string("dfm_sys_create_mesh: ") + hipGetErrorString(e));
+    delete h;
+    return e == hipErrorOutOfMemory ? DFM_ERR_NOMEM : DFM_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = upload(s.d_coords, s.coords, st)) != hipSuccess) return fail(e);
+  if ((e = upload(s.d_lnods, s.lnods, st)) != hipSuccess) return fail(e);
+  if ((e = upload(s.d_elsup_ptr, s.elsup_ptr, st)) != hipSuccess) return fail(e);
+  if ((e = upload(s.d_elsup, s.elsup, st)) != hipSuccess) return fail(e);
+  if ((e = s.d_crs.alloc((size_t)s.ncrs * s.bs)) != hipSuccess) return fail(e);
+  if ((e = s.d_dia.alloc((size_t)nnode * s.bs)) != hipSuccess) return fail(e);
+  if ((e = s.d_res.alloc(m)) != hipSuccess) return fail(e);
+  if ((e = s.d_upd.alloc(m)) != hipSuccess) return fail(e);
+  if ((e = s.d_rhs.alloc(m)) != hipSuccess) return fail(e);
+  if ((e = s.d_u.alloc(m)) != hipSuccess) return fail(e);
+  if ((e = s.d_v.alloc(m)) != hipSuccess) return fail(e);
+  if ((e = s.d_bc.alloc(m)) != hipSuccess) return fail(e);
+  // per-block partial sums: up to 2 values per block of the widest reduction grid (SpMV rows at 32 lanes)
+  if ((e = s.d_partial.alloc(8 * 2048)) != hipSuccess)
+    return fail(e);
+  if ((e = s.d_counter.alloc(8)) != hipSuccess) return fail(e);
+  if ((e = s.d_scal.alloc(16)) != hipSuccess) return fail(e);
+  if ((e = s.d_iflag.alloc(8)) != hipSuccess) return fail(e);
+  if ((e = hipHostMalloc((void**)&s.h_pinned, 16 * sizeof(int))) != hipSuccess) return fail(e);
+  if ((e = hipHostMalloc((void**)&s.h_pinned_d, 16 * sizeof(double))) != hipSuccess) return fail(e);
+  if ((e = hipMemsetAsync(s.d_counter.p, 0, 8 * sizeof(unsigned), st)) != hipSuccess) return fail(e);
+  if ((e = hipMemsetAsync(s.d_bc.p, 0, m * sizeof(int), st)) != hipSuccess) return fail(e);
+  if ((e = launch_fill(s.d_u.p, 0.0, m, st)) != hipSuccess) return fail(e);
+  if ((e = launch_fill(s.d_v.p, 0.0, m, st)) != hipSuccess) return fail(e);
+  if ((e = launch_fill(s.d_res.p, 0.0, m, st)) != hipSuccess) return fail(e);
+  if ((e = launch_fill(s.d_upd.p, 0.0, m, st)) != hipSuccess) return fail(e);
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);
+  h->owner = ctx;
+  ctx->c.refs++;
+  *out = h;
+  return DFM_OK;
+}
+
+int dfm_sys_destroy(dfm_sys sys) {
+  if (!sys) return DFM_OK;
+  (void)hipStreamSynchronize(sys->s.ctx->stream);
+  dfm_ctx owner = sys->owner;
+  delete sys;
+  if (owner) ctx_release(owner);
+  return DFM_OK;
+}
+
+int dfm_sys_info(dfm_sys sys, int* nblk, int* len, long long* ncrs) {
+  NEED(sys, "dfm_sys_info");
+  if (nblk) *nblk = sys->s.n;
+  if (len) *len = sys->s.len;
+  if (ncrs) *ncrs = sys->s.ncrs;
+  return DFM_OK;
+}
+
+int dfm_sys_get_pattern(dfm_sys sys, unsigned* rowptr, unsigned* colind) {
+  NEED(sys, "dfm_sys_get_pattern");
+  const Sys& s = sys->s;
+  if (rowptr) std::memcpy(rowptr, s.rowptr0.data(), sizeof(unsigned) * (s.n + 1));
+  if (colind) std::memcpy(colind, s.colind0.data(), sizeof(unsigned) * s.ncrs);
+  return DFM_OK;
+}
+
+// the last ordering built by dfm_sys_make_ordering, with its colouring (kept so that dfm_sys_set_ordering
+// with that same order can use contiguous groups)
+struct Pending {
+  std::vector<int> order, color, tile;
+  int ncolors = 0;
+};
+static thread_local Pending g_pending;
+
+int dfm_sys_make_ordering(dfm_sys sys, int kind, int tile, int* order_out, int* ncolors_out) {
+  NEED(sys, "dfm_sys_make_ordering");
+  Sys& s = sys->s;
+  Pending p;
+  if (kind == DFM_ORDER_NATURAL) {
+    p.order.resize(s.n);
+    std::iota(p.order.begin(), p.order.end(), 0);
+  } else if (kind == DFM_ORDER_MULTICOLOR) {
+    make_multicolor(s, p.order, p.color, p.ncolors);
+  } else if (kind == DFM_ORDER_BLOCKCOLOR) {
+    make_blockcolor(s, tile, p.order, p.color, p.tile, p.ncolors);
+  } else {
+    DFM_CHECK(false, DFM_ERR_ARG, "dfm_sys_make_ordering: unknown kind");
+  }
+  if (order_out) std::memcpy(order_out, p.order.data(), sizeof(int) * s.n);
+  if (ncolors_out) *ncolors_out = p.ncolors;
+  g_pending = std::move(p);
+  return DFM_OK;
+}
+
+int dfm_sys_set_ordering(dfm_sys sys, const int* order) {
+  NEED(sys, "dfm_sys_set_ordering");
+  Sys& s = sys->s;
+  std::vector<int> ord(s.n);
+  if (order) {
+    std::vector<char> seen(s.n, 0);
+    for (int i = 0; i < s.n; i++) {
+      DFM_CHECK(order[i] >= 0 && order[i] < s.n && !seen[order[i]], DFM_ERR_ARG,
+                "dfm_sys_set_ordering: not a permutation");
+      seen[order[i]] = 1;
+      ord[i] = order[i];
+    }
+  } else {
+    std::iota(ord.begin(), ord.end(), 0);
+  }
+  std::vector<int> color, tile;
+  int nc = 0;
+  if (order && g_pending.order == ord && !g_pending.color.empty()) {
+    color = g_pending.color;
+    tile = g_pending.tile;
+    nc = g_pending.ncolors;
+  }
+  // Dirichlet flags follow the relabelling
+  std::vector<int> bc((size_t)mdof(s));
+  DFM_HIP(hipMemcpy(bc.data(), s.d_bc.p, bc.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> bc0(bc.size());
+  for (int i0 = 0; i0 < s.n; i0++)
+    for (int d = 0; d < s.len; d++) bc0[(size_t)s.order[i0] * s.len + d] = bc[(size_t)i0 * s.len + d];
+  const int rc = apply_order(s, ord, color, tile, nc);
+  if (rc != DFM_OK) return rc;
+  for (int i0 = 0; i0 < s.n; i0++)
+    for (int d = 0; d < s.len; d++) bc[(size_t)i0 * s.len + d] = bc0[(size_t)s.order[i0] * s.len + d];
+  DFM_HIP(hipMemcpy(s.d_bc.p, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice));
+  return DFM_OK;
+}
+
+int dfm_sys_schedule_info(dfm_sys sys, int* nphase_fwd, int* nphase_bwd, int* max_group_levels) {
+  NEED(sys, "dfm_sys_schedule_info");
+  const Sys& s = sys->s;
+  if (nphase_fwd) *nphase_fwd = s.fwd.nphase();
+  if (nphase_bwd) *nphase_bwd = s.bwd.nphase();
+  if (max_group_levels) {
+    int mx = 0;
+    for (int v : s.fwd.phase_nlev) mx = std::max(mx, v);
+    for (int v : s.bwd.phase_nlev) mx = std::max(mx, v);
+    *max_group_levels = mx;
+  }
+  return DFM_OK;
+}
+
+int dfm_sys_set_bc(dfm_sys sys, const int* flags) {
+  NEED(sys, "dfm_sys_set_bc");
+  NEED(flags, "dfm_sys_set_bc");
+  Sys& s = sys->s;
+  std::vector<int> bc((size_t)mdof(s));
+  for (int i0 = 0; i0 < s.n; i0++)
+    for (int d = 0; d < s.len; d++) bc[(size_t)i0 * s.len + d] = flags[(size_t)s.order[i0] * s.len + d] ? 1 : 0;
+  DFM_HIP(hipMemcpy(s.d_bc.p, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice));
+  return DFM_OK;
+}
+
+int dfm_sys_set_field(dfm_sys sys, const double* u, const double* v) {
+  NEED(sys, "dfm_sys_set_field");
+  Sys& s = sys->s;
+  const size_t bytes = sizeof(double) * mdof(s);
+  if (u) DFM_HIP(hipMemcpy(s.d_u.p, u, bytes, hipMemcpyHostToDevice));
+  else DFM_HIP(launch_fill(s.d_u.p, 0.0, mdof(s), s.ctx->stream));
+  if (v) DFM_HIP(hipMemcpy(s.d_v.p, v, bytes, hipMemcpyHostToDevice));
+  else DFM_HIP(launch_fill(s.d_v.p, 0.0, mdof(s), s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_get_field(dfm_sys sys, double* u) {
+  NEED(sys, "dfm_sys_get_field");
+  NEED(u, "dfm_sys_get_field");
+  Sys& s = sys->s;
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  DFM_HIP(hipMemcpy(u, s.d_u.p, sizeof(double) * mdof(s), hipMemcpyDeviceToHost));
+  return DFM_OK;
+}
+
+int dfm_sys_update_field(dfm_sys sys) {
+  NEED(sys, "dfm_sys_update_field");
+  Sys& s = sys->s;
+  DFM_HIP(launch_update_field(s.n, s.len, s.d_order.p, s.d_upd.p, s.d_u.p, s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_set_params(dfm_sys sys, const double* params, int npar, int pstride) {
+  NEED(sys, "dfm_sys_set_params");
+  NEED(params, "dfm_sys_set_params");
+  Sys& s = sys->s;
+  DFM_CHECK(npar > 0 && (pstride == 0 || pstride >= npar), DFM_ERR_ARG, "dfm_sys_set_params: bad npar/pstride");
+  const size_t npv = pstride ? (size_t)s.nelem * pstride : (size_t)npar;
+  if (s.d_par.n != npv) DFM_HIP(s.d_par.alloc(npv));
+  DFM_HIP(hipMemcpy(s.d_par.p, params, npv * sizeof(double), hipMemcpyHostToDevice));
+  s.npar = npar;
+  s.pstride = pstride;
+  return DFM_OK;
+}
+
+// consistent P1 surface loads t*A/3 per node (a reference extension), kept per node in face order so that
+// the assembly adds them after the element contributions in a fixed order
+int dfm_sys_set_loads(dfm_sys sys, int nface, const int* faces, const double* trac) {
+  NEED(sys, "dfm_sys_set_loads");
+  Sys& s = sys->s;
+  if (nface <= 0) {
+    s.has_loads = false;
+    return DFM_OK;
+  }
+  DFM_CHECK(s.ndim == 3 && s.len == 3 && faces && trac, DFM_ERR_ARG,
+            "dfm_sys_set_loads: surface loads need a 3D vector field, faces and tractions");
+  std::vector<int> lptr(s.n + 1, 0);
+  for (long long f = 0; f < nface; f++)
+    for (int k = 0; k < 3; k++) {
+      const int nd = faces[f * 3 + k];
+      DFM_CHECK(nd >= 0 && nd < s.n, DFM_ERR_ARG, "dfm_sys_set_loads: face node out of range");
+      lptr[nd + 1]++;
+    }
+  for (int i = 0; i < s.n; i++) lptr[i + 1] += lptr[i];
+  std::vector<double> loads((size_t)lptr[s.n] * 3, 0.0);
+  std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+  for (long long f = 0; f < nface; f++) {
+    const double* p0 = &s.coords[(size_t)faces[f * 3 + 0] * 3];
+    const double* p1 = &s.coords[(size_t)faces[f * 3 + 1] * 3];
+    const double* p2 = &s.coords[(size_t)faces[f * 3 + 2] * 3];
+    const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double area = 0.5 * std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int k = 0; k < 3; k++) {
+      const int slot = fill[faces[f * 3 + k]]++;
+      for (int d = 0; d < 3; d++) loads[(size_t)slot * 3 + d] = trac[f * 3 + d] * area / 3.0;
+    }
+  }
+  hipStream_t st = s.ctx->stream;
+  DFM_HIP(upload(s.d_lptr, lptr, st));
+  DFM_HIP(upload(s.d_loads, loads, st));
+  s.has_loads = true;
+  return DFM_OK;
+}
+
+int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double* res_norm) {
+  NEED(sys, "dfm_sys_assemble_resident");
+  Sys& s = sys->s;
+  static const int need_npar[6] = {2, 5, 6, 5, 6, 3};
+  static const int need_len[6] = {1, 2, 3, 2, 3, 1};
+  DFM_CHECK(kind >= 0 && kind < 6, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
+  DFM_CHECK(s.d_par.p != nullptr, DFM_ERR_STATE, "dfm_sys_assemble: no element parameters set");
+  DFM_CHECK(s.npar >= need_npar[kind], DFM_ERR_ARG, "dfm_sys_assemble: too few parameters for the element kind");
+  DFM_CHECK(need_len[kind] == s.len, DFM_ERR_ARG, "dfm_sys_assemble: element kind does not match dofs per node");
+  DFM_CHECK((kind == DFM_SOLID3D || kind == DFM_STVK3D || kind == DFM_POISSON) || s.ndim == 2, DFM_ERR_ARG,
+            "dfm_sys_assemble: element kind needs a 2D mesh");
+  DFM_CHECK((kind != DFM_SOLID3D && kind != DFM_STVK3D) || s.ndim == 3, DFM_ERR_ARG,
+            "dfm_sys_assemble: element kind needs a 3D mesh");
+  hipStream_t st = s.ctx->stream;
+  const long long m = mdof(s);
+  // InitializeMarge (src/ls/linearsystem.cpp:74-88): matrix, residual and update set to zero
+  DFM_HIP(launch_fill(s.d_crs.p, 0.0, (long long)s.ncrs * s.bs, st));
+  DFM_HIP(launch_fill(s.d_dia.p, 0.0, (long long)s.n * s.bs, st));
+  DFM_HIP(launch_fill(s.d_res.p, 0.0, m, st));
+  DFM_HIP(launch_fill(s.d_upd.p, 0.0, m, st));
+  AsmArgs a{};
+  a.kind = kind; a.ndim = s.ndim; a.npoel = s.npoel; a.len = s.len; a.n = s.n;
+  a.coords = s.d_coords.p; a.lnods = s.d_lnods.p; a.elsup_ptr = s.d_elsup_ptr.p; a.elsup = s.d_elsup.p;
+  a.order = s.d_order.p; a.inv = s.d_inv.p;
+  a.params = s.d_par.p; a.npar = s.npar; a.pstride = s.pstride;
+  a.glob0 = glob ? glob[0] : 0.0; a.glob1 = glob ? glob[1] : 0.0;
+  a.u = s.d_u.p; a.v = s.d_v.p;
+  a.load_ptr = s.has_loads ? s.d_lptr.p : nullptr;
+  a.loads = s.has_loads ? s.d_loads.p : nullptr;
+  a.rowptr = s.d_rowptr.p; a.colind = s.d_colind.p; a.crs = s.d_crs.p; a.dia = s.d_dia.p; a.res = s.d_res.p;
+  DFM_HIP(launch_assemble(a, st));
+  // FinalizeMarge (:92-128): Dirichlet rows/columns, residual zeroed at fixed dofs, ||r|| returned
+  DFM_HIP(launch_set_bc(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_bc.p, st));
+  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
+  s.assembled = true;
+  s.factored = s.factored32 = false;
+  if (res_norm) {
+    DFM_HIP(launch_dot(s.d_res.p, s.d_res.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + 15, st));
+    DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
+    DFM_HIP(hipStreamSynchronize(st));
+    *res_norm = std::sqrt(s.h_pinned_d[0]);
+  }
+  return DFM_OK;
+}
+
+int dfm_sys_assemble(dfm_sys sys, int kind, const double* params, int npar, int pstride, const double* glob,
+                     int nface, const int* faces, const double* trac, double* res_norm) {
+  NEED(sys, "dfm_sys_assemble");
+  NEED(params, "dfm_sys_assemble");
+  int rc = dfm_sys_set_params(sys, params, npar, pstride);
+  if (rc != DFM_OK) return rc;
+  rc = dfm_sys_set_loads(sys, nface, faces, trac);
+  if (rc != DFM_OK) return rc;
+  rc = dfm_sys_assemble_resident(sys, kind, glob, res_norm);
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(hipStreamSynchronize(sys->s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia) {
+  NEED(sys, "dfm_sys_get_matrix");
+  Sys& s = sys->s;
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  std::vector<double> c((size_t)s.ncrs * s.bs), d((size_t)s.n * s.bs);
+  DFM_HIP(hipMemcpy(c.data(), s.d_crs.p, c.size() * sizeof(double), hipMemcpyDeviceToHost));
+  DFM_HIP(hipMemcpy(d.data(), s.d_dia.p, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (crs)
+    for (long long q = 0; q < s.ncrs; q++)
+      std::memcpy(&crs[(size_t)q * s.bs], &c[(size_t)s.perm_crs[q] * s.bs], sizeof(double) * s.bs);
+  if (dia)
+    for (int i0 = 0; i0 < s.n; i0++)
+      std::memcpy(&dia[(size_t)s.order[i0] * s.bs], &d[(size_t)i0 * s.bs], sizeof(double) * s.bs);
+  return DFM_OK;
+}
+
+int dfm_sys_resize_tmp(dfm_sys sys, int ntmp) {
+  NEED(sys, "dfm_sys_resize_tmp");
+  Sys& s = sys->s;
+  DFM_CHECK(ntmp >= 0, DFM_ERR_ARG, "dfm_sys_resize_tmp: negative size");
+  // like CLinearSystem::ReSizeTmpVecSolver, only growth changes anything (src/ls/linearsystem.cpp:208-244)
+  while ((int)s.d_tmp.size() < ntmp) {
+    auto* b = new DBuf<double>();
+    DFM_HIP(b->alloc(mdof(s)));
+    DFM_HIP(launch_fill(b->p, 0.0, mdof(s), s.ctx->stream));
+    s.d_tmp.push_back(b);
+  }
+  return DFM_OK;
+}
+
+int dfm_sys_ntmp(dfm_sys sys, int* ntmp) {
+  NEED(sys, "dfm_sys_ntmp");
+  NEED(ntmp, "dfm_sys_ntmp");
+  *ntmp = (int)sys->s.d_tmp.size();
+  return DFM_OK;
+}
+
+int dfm_sys_dot(dfm_sys sys, int iv1, int iv2, double* out) {
+  NEED(sys, "dfm_sys_dot");
+  NEED(out, "dfm_sys_dot");
+  Sys& s = sys->s;
+  double* a = vec_ptr(s, iv1);
+  double* b = vec_ptr(s, iv2);
+  DFM_CHECK(a && b, DFM_ERR_ARG, "dfm_sys_dot: bad vector handle");
+  hipStream_t st = s.ctx->stream;
+  DFM_HIP(launch_dot(a, b, mdof(s), s.d_partial.p, s.d_counter.p, s.d_scal.p + 15, st));
+  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipStreamSynchronize(st));
+  *out = s.h_pinned_d[0];
+  return DFM_OK;
+}
+
+int dfm_sys_copy(dfm_sys sys, int iv1, int iv2) {
+  NEED(sys, "dfm_sys_copy");
+  Sys& s = sys->s;
+  double* a = vec_ptr(s, iv1);
+  double* b = vec_ptr(s, iv2);
+  DFM_CHECK(a && b, DFM_ERR_ARG, "dfm_sys_copy: bad vector handle");
+  if (iv1 == iv2) return DFM_OK;
+  DFM_HIP(launch_copy(a, b, mdof(s), s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_scal(dfm_sys sys, double alpha, int iv) {
+  NEED(sys, "dfm_sys_scal");
+  Sys& s = sys->s;
+  double* a = vec_ptr(s, iv);
+  DFM_CHECK(a, DFM_ERR_ARG, "dfm_sys_scal: bad vector handle");
+  DFM_HIP(launch_scal(a, alpha, mdof(s), s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_axpy(dfm_sys sys, double alpha, int iv1, int iv2) {
+  NEED(sys, "dfm_sys_axpy");
+  Sys& s = sys->s;
+  double* a = vec_ptr(s, iv1);
+  double* b = vec_ptr(s, iv2);
+  DFM_CHECK(a && b, DFM_ERR_ARG, "dfm_sys_axpy: bad vector handle");
+  DFM_CHECK(iv1 != iv2, DFM_ERR_ARG, "dfm_sys_axpy: iv1 == iv2 is not supported (src/ls/linearsystem.cpp:318-321)");
+  if (alpha == 0) return DFM_OK;
+  DFM_HIP(launch_axpy(alpha, a, b, mdof(s), s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_matvec(dfm_sys sys, double alpha, int iv1, double beta, int iv2) {
+  NEED(sys, "dfm_sys_matvec");
+  Sys& s = sys->s;
+  double* x = vec_ptr(s, iv1);
+  double* y = vec_ptr(s, iv2);
+  DFM_CHECK(x && y, DFM_ERR_ARG, "dfm_sys_matvec: bad vector handle");
+  if (alpha == 0.0) return dfm_sys_scal(sys, beta, iv2);
+  DFM_CHECK(iv1 != iv2, DFM_ERR_ARG, "dfm_sys_matvec: iv1 == iv2 is not supported (src/ls/linearsystem.cpp:275-278)");
+  DFM_CHECK(s.assembled, DFM_ERR_STATE, "dfm_sys_matvec: matrix not assembled");
+  DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, alpha, x,
+                      beta, y, s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_get_vector(dfm_sys sys, int iv, double* out) {
+  NEED(sys, "dfm_sys_get_vector");
+  NEED(out, "dfm_sys_get_vector");
+  Sys& s = sys->s;
+  double* a = vec_ptr(s, iv);
+  DFM_CHECK(a, DFM_ERR_ARG, "dfm_sys_get_vector: bad vector handle");
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  std::vector<double> v((size_t)mdof(s));
+  DFM_HIP(hipMemcpy(v.data(), a, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i0 = 0; i0 < s.n; i0++)
+    for (int d = 0; d < s.len; d++) out[(size_t)s.order[i0] * s.len + d] = v[(size_t)i0 * s.len + d];
+  return DFM_OK;
+}
+
+int dfm_sys_set_vector(dfm_sys sys, int iv, const double* in) {
+  NEED(sys, "dfm_sys_set_vector");
+  NEED(in, "dfm_sys_set_vector");
+  Sys& s = sys->s;
+  double* a = vec_ptr(s, iv);
+  DFM_CHECK(a, DFM_ERR_ARG, "dfm_sys_set_vector: bad vector handle");
+  std::vector<double> v((size_t)mdof(s));
+  for (int i0 = 0; i0 < s.n; i0++)
+    for (int d = 0; d < s.len; d++) v[(size_t)i0 * s.len + d] = in[(size_t)s.order[i0] * s.len + d];
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  DFM_HIP(hipMemcpy(a, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  return DFM_OK;
+}
+
+int dfm_prec_factor(dfm_sys sys) {
+  NEED(sys, "dfm_prec_factor");
+  Sys& s = sys->s;
+  DFM_CHECK(s.assembled, DFM_ERR_STATE, "dfm_prec_factor: matrix not assembled");
+  hipStream_t st = s.ctx->stream;
+  if (!s.d_fcrs.p) DFM_HIP(s.d_fcrs.alloc((size_t)s.ncrs * s.bs));
+  if (!s.d_fdia.p) DFM_HIP(s.d_fdia.alloc((size_t)s.n * s.bs));
+  // SetValue_Initialize: the factor pattern is the matrix pattern (ILU(0)), values copied
+  DFM_HIP(launch_copy(s.d_crs.p, s.d_fcrs.p, (long long)s.ncrs * s.bs, st));
+  DFM_HIP(launch_copy(s.d_dia.p, s.d_fdia.p, (long long)s.n * s.bs, st));
+  DFM_HIP(hipMemsetAsync(s.d_iflag.p + 4, 0, sizeof(int), st));
+  SweepDev fw{s.fwd.nphase(), s.fwd.phase_gptr.data(), s.fwd.phase_nlev.data(), s.fwd.d_grp.p,
+              s.fwd.rowlist.empty() ? nullptr : s.fwd.d_rowlist.p, s.fwd.d_lev.p};
+  DFM_HIP(launch_ilu_factor(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_diaind.p, s.d_fcrs.p, s.d_fdia.p, fw,
+                            s.d_iflag.p + 4, st));
+  int nsing = 0;
+  DFM_HIP(hipMemcpyAsync(s.h_pinned + 8, s.d_iflag.p + 4, sizeof(int), hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipStreamSynchronize(st));
+  nsing = s.h_pinned[8];
+  s.factored32 = false;
+  if (nsing > 10) {
+    s.factored = false;
+    set_error("dfm_prec_factor: more than 10 singular pivots (DoILUDecomp returns false)");
+    return DFM_ERR_SINGULAR;
+  }
+  s.factored = true;
+  return DFM_OK;
+}
+
+int dfm_prec_apply(dfm_sys sys, int iv) {
+  NEED(sys, "dfm_prec_apply");
+  Sys& s = sys->s;
+  double* v = vec_ptr(s, iv);
+  DFM_CHECK(v, DFM_ERR_ARG, "dfm_prec_apply: bad vector handle");
+  DFM_CHECK(s.factored, DFM_ERR_STATE, "dfm_prec_apply: preconditioner not factored");
+  DFM_HIP(precond_apply64(s, v, v, nullptr, s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_prec_get_factor(dfm_sys sys, unsigned* frowptr, unsigned* fcolind, unsigned* diaind, double* fcrs,
+                        double* fdia) {
+  NEED(sys, "dfm_prec_get_factor");
+  Sys& s = sys->s;
+  DFM_CHECK(s.factored, DFM_ERR_STATE, "dfm_prec_get_factor: preconditioner not factored");
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  if (frowptr) std::memcpy(frowptr, s.rowptr.data(), sizeof(unsigned) * (s.n + 1));
+  if (fcolind) std::memcpy(fcolind, s.colind.data(), sizeof(unsigned) * s.ncrs);
+  if (diaind) std::memcpy(diaind, s.diaind.data(), sizeof(unsigned) * s.n);
+  if (fcrs) DFM_HIP(hipMemcpy(fcrs, s.d_fcrs.p, sizeof(double) * s.ncrs * s.bs, hipMemcpyDeviceToHost));
+  if (fdia) DFM_HIP(hipMemcpy(fdia, s.d_fdia.p, sizeof(double) * s.n * s.bs, hipMemcpyDeviceToHost));
+  return DFM_OK;
+}
+
+static int ensure_work(Sys& s, int nvec64, int nvec32) {
+  const long long m = mdof(s);
+  if (s.d_work.n < (size_t)(nvec64 * m)) DFM_HIP(s.d_work.alloc((size_t)nvec64 * m));
+  if (nvec32 && s.d_work32.n < (size_t)(nvec32 * m)) DFM_HIP(s.d_work32.alloc((size_t)nvec32 * m));
+  return DFM_OK;
+}
+
+static int ensure_fp32(Sys& s) {
+  if (s.factored32) return DFM_OK;
+  hipStream_t st = s.ctx->stream;
+  if (!s.d_crs32.p) DFM_HIP(s.d_crs32.alloc((size_t)s.ncrs * s.bs));
+  if (!s.d_dia32.p) DFM_HIP(s.d_dia32.alloc((size_t)s.n * s.bs));
+  if (!s.d_fcrs32.p) DFM_HIP(s.d_fcrs32.alloc((size_t)s.ncrs * s.bs));
+  if (!s.d_fdia32.p) DFM_HIP(s.d_fdia32.alloc((size_t)s.n * s.bs));
+  DFM_HIP(launch_to_f32(s.d_crs.p, s.d_crs32.p, (long long)s.ncrs * s.bs, st));
+  DFM_HIP(launch_to_f32(s.d_dia.p, s.d_dia32.p, (long long)s.n * s.bs, st));
+  DFM_HIP(launch_to_f32(s.d_fcrs.p, s.d_fcrs32.p, (long long)s.ncrs * s.bs, st));
+  DFM_HIP(launch_to_f32(s.d_fdia.p, s.d_fdia32.p, (long long)s.n * s.bs, st));
+  s.factored32 = true;
+  return DFM_OK;
+}
+
+int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration) {
+  NEED(sys, "dfm_solve_pcg");
+  NEED(conv_ratio, "dfm_solve_pcg");
+  NEED(iteration, "dfm_solve_pcg");
+  Sys& s = sys->s;
+  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED, DFM_ERR_ARG, "dfm_solve_pcg: bad precision");
+  DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_solve_pcg: assemble and factor first");
+  int rc = ensure_work(s, 2, precision == DFM_FP64 ? 0 : 5);
+  if (rc != DFM_OK) return rc;
+  if (precision == DFM_MIXED) {
+    rc = ensure_fp32(s);
+    if (rc != DFM_OK) return rc;
+  }
+  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, mdof(s), s.ctx->stream));
+  return solve_pcg(s, precision, conv_ratio, iteration);
+}
+
+int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
+  NEED(sys, "dfm_solve_pbicgstab");
+  NEED(conv_ratio, "dfm_solve_pbicgstab");
+  NEED(iteration, "dfm_solve_pbicgstab");
+  Sys& s = sys->s;
+  DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_solve_pbicgstab: assemble and factor first");
+  const int rc = ensure_work(s, 7, 0);
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, mdof(s), s.ctx->stream));
+  return solve_pbicgstab(s, conv_ratio, iteration);
+}
+
+int dfm_sys_true_residual(dfm_sys sys, double* rel) {
+  NEED(sys, "dfm_sys_true_residual");
+  NEED(rel, "dfm_sys_true_residual");
+  Sys& s = sys->s;
+  const long long m = mdof(s);
+  hipStream_t st = s.ctx->stream;
+  const int rc = ensure_work(s, 1, 0);
+  if (rc != DFM_OK) return rc;
+  double* t = s.d_work.p;
+  DFM_HIP(launch_copy(s.d_rhs.p, t, m, st));
+  DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
+                      s.d_upd.p, 1.0, t, st));
+  DFM_HIP(launch_dot(t, t, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + 14, st));
+  DFM_HIP(launch_dot(s.d_rhs.p, s.d_rhs.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + 15, st));
+  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + 14, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipStreamSynchronize(st));
+  *rel = s.h_pinned_d[1] > 0 ? std::sqrt(s.h_pinned_d[0] / s.h_pinned_d[1]) : 0.0;
+  return DFM_OK;
+}
+
+int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes) {
+  NEED(sys, "dfm_sys_bench_kernel");
+  Sys& s = sys->s;
+  DFM_CHECK(which >= 0 && which <= 3 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
+  DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble and factor first");
+  int rc = ensure_work(s, 2, 5);
+  if (rc != DFM_OK) return rc;
+  if (which == 1 || which == 3) {
+    rc = ensure_fp32(s);
+    if (rc != DFM_OK) return rc;
+  }
+  return bench_kernel(s, which, reps, avg_ms, bytes);
+}
+
+int dfm_device_bytes(long long* bytes) {
+  NEED(bytes, "dfm_device_bytes");
+  *bytes = device_bytes_in_use();
+  return DFM_OK;
+}
+
+int dfm_sys_last_solve_stats(dfm_sys sys, double* ms_total, double* ms_per_iter, unsigned* inner_iters) {
+  NEED(sys, "dfm_sys_last_solve_stats");
+  if (ms_total) *ms_total = sys->s.last_ms;
+  if (ms_per_iter) *ms_per_iter = sys->s.last_ms_iter;
+  if (inner_iters) *inner_iters = sys->s.last_inner;
+  return DFM_OK;
+}
+
+}  // extern "C"

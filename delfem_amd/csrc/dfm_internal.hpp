@@ -1,0 +1,159 @@
+// Internal declarations of libdfm_gpu.so: device buffers, the system object and its ILU schedule.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dfm_gpu.h"
+
+namespace dfm {
+
+void set_error(const std::string& msg);
+
+#define DFM_HIP(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) {                                                                        \
+      ::dfm::set_error(std::string(#call) + ": " + hipGetErrorString(e_) + " at " + __FILE__ + ":" + \
+                       std::to_string(__LINE__));                                                  \
+      return DFM_ERR_HIP;                                                                          \
+    }                                                                                              \
+  } while (0)
+
+#define DFM_CHECK(cond, code, msg) \
+  do {                             \
+    if (!(cond)) {                 \
+      ::dfm::set_error(msg);       \
+      return code;                 \
+    }                              \
+  } while (0)
+
+// Device allocation with byte accounting (the serving admission budget reads it).
+struct DevAlloc {
+  static hipError_t alloc(void** p, size_t bytes);
+  static void release(void* p);
+};
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return DevAlloc::alloc((void**)&p, count * sizeof(T));
+  }
+  void release() {
+    if (p) DevAlloc::release(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int refs = 1;            // the handle itself + one per live system
+  bool destroyed = false;  // dfm_ctx_destroy called; freed when the last system goes
+};
+
+// Host-side schedule of one triangular sweep: phases run one after another (kernel boundaries); a phase is
+// a set of independent groups; a group (<= 64 rows, one wavefront) is either a contiguous row range of the
+// device numbering with intra-group levels, or a list of independent rows (indirect mode).
+struct Sweep {
+  std::vector<int> phase_gptr;  // [nphase+1] group index ranges, in execution order
+  std::vector<int> phase_nlev;  // [nphase] number of intra-group levels (1 = no intra-group dependency)
+  std::vector<int> grp;         // [2*ngroup] (start, end) per group in execution order: row ids
+                                // (contiguous mode) or offsets into rowlist (indirect mode)
+  std::vector<int> rowlist;     // indirect mode: rows of the groups, grp indexes into it; else empty
+  std::vector<uint8_t> lev;     // [n] intra-group level of each row for this sweep's direction
+  DBuf<int> d_grp, d_rowlist;
+  DBuf<uint8_t> d_lev;
+  int nphase() const { return (int)phase_nlev.size(); }
+};
+
+struct Sys {
+  Ctx* ctx = nullptr;
+  int ndim = 0, n = 0, len = 0, bs = 0, npoel = 0, nelem = 0;
+  long long ncrs = 0;
+
+  // mesh (original numbering)
+  std::vector<double> coords;
+  std::vector<int> lnods;
+  std::vector<unsigned> rowptr0, colind0;  // pattern, original numbering
+  std::vector<int> elsup_ptr, elsup;       // node -> elements, ascending element index
+
+  // ordering: order[new] = old, inv[old] = new
+  std::vector<int> order, inv;
+  bool natural = true;
+  int ncolors = 0;
+  std::vector<int> color_of;  // per device row (MC/BMC), empty for natural / external orderings
+  std::vector<int> tile_of;   // per device row (BMC)
+
+  // device-numbering pattern (shared by the permuted matrix and the ILU(0) factor)
+  std::vector<unsigned> rowptr, colind, diaind, lsplit, usplit;
+  std::vector<long long> perm_crs;  // original crs index -> device crs index
+  Sweep fwd, bwd;
+
+  // device arrays
+  DBuf<unsigned> d_rowptr, d_colind, d_diaind, d_lsplit, d_usplit;
+  DBuf<int> d_order, d_inv, d_elsup_ptr, d_elsup, d_lnods, d_bc;
+  DBuf<double> d_coords, d_u, d_v;
+  DBuf<double> d_crs, d_dia;         // matrix, device numbering, FP64
+  DBuf<float> d_crs32, d_dia32;      // FP32 copy (mixed precision)
+  DBuf<double> d_fcrs, d_fdia;       // ILU(0) factor FP64
+  DBuf<float> d_fcrs32, d_fdia32;    // ILU(0) factor FP32
+  DBuf<double> d_res, d_upd, d_rhs;  // handles -1, -2 and a copy of the assembled right-hand side
+  std::vector<DBuf<double>*> d_tmp;  // handles 0..
+  DBuf<double> d_work;               // solver work vectors
+  DBuf<float> d_work32;
+  DBuf<double> d_partial;            // per-block partial sums
+  DBuf<unsigned> d_counter;          // last-block counters
+  DBuf<double> d_scal;               // device scalars of the Krylov loops
+  DBuf<int> d_iflag;                 // device int scalars (done flag, iteration)
+  int* h_pinned = nullptr;           // host-pinned mirror of the done flag + iteration
+  double* h_pinned_d = nullptr;
+
+  // resident assembly inputs
+  DBuf<double> d_par;
+  int npar = 0, pstride = 0;
+  DBuf<int> d_lptr;
+  DBuf<double> d_loads;
+  bool has_loads = false;
+
+  bool assembled = false, factored = false, factored32 = false;
+  double last_ms = 0.0, last_ms_iter = 0.0;
+  unsigned last_inner = 0;
+  int last_nrel = 0;  // reliable updates in the last mixed solve
+
+  ~Sys();
+};
+
+// host symbolic routines (host_symbolic.cpp)
+void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vector<unsigned>& rowptr,
+                   std::vector<unsigned>& colind, std::vector<int>& elsup_ptr, std::vector<int>& elsup);
+void build_device_pattern(Sys& s);  // from order/inv: rowptr, colind, diaind, perm_crs
+void build_schedule(Sys& s);        // fwd/bwd sweeps, lsplit/usplit
+void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors);
+void make_blockcolor(const Sys& s, int tile, std::vector<int>& order, std::vector<int>& color_of_new,
+                     std::vector<int>& tile_of_new, int& ncolors);
+
+// mesh generation (host_mesh.cpp)
+uint64_t rng_u64(uint64_t seed, uint64_t stream, uint64_t idx);
+double rng_u01(uint64_t seed, uint64_t stream, uint64_t idx);
+double rng_normal(uint64_t seed, uint64_t stream, uint64_t idx);
+
+}  // namespace dfm
+
+struct dfm_ctx_s {
+  dfm::Ctx c;
+};
+struct dfm_sys_s {
+  dfm::Sys s;
+  dfm_ctx_s* owner = nullptr;
+};

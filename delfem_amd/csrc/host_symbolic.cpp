@@ -1,0 +1,344 @@
+// Host symbolic phase: CRS pattern from the mesh, ILU(0) orderings, the device-numbering pattern and the
+// phase/group/level schedules of the triangular sweeps and the numeric factorisation.
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <numeric>
+#include <stdexcept>
+
+#include "dfm_internal.hpp"
+
+namespace dfm {
+
+// CElemAry::MakePattern_FEM (src/femfield/elem_ary.cpp:225-245): node -> elements (MakePointSurElem,
+// :675-724, ascending element index), then per node the distinct neighbours through its elements, self
+// excluded (MakePointSurPoint with isnt_self, :821-892), ascending (CIndexedArray::Sort,
+// include/delfem/indexed_array.h:113-132).
+void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vector<unsigned>& rowptr,
+                   std::vector<unsigned>& colind, std::vector<int>& elsup_ptr, std::vector<int>& elsup) {
+  elsup_ptr.assign((size_t)nnode + 1, 0);
+  for (long long e = 0; e < nelem; e++)
+    for (int k = 0; k < npoel; k++) elsup_ptr[lnods[e * npoel + k] + 1]++;
+  for (int i = 0; i < nnode; i++) elsup_ptr[i + 1] += elsup_ptr[i];
+  elsup.assign((size_t)elsup_ptr[nnode], 0);
+  {
+    std::vector<int> fill(elsup_ptr.begin(), elsup_ptr.end() - 1);
+    for (long long e = 0; e < nelem; e++)
+      for (int k = 0; k < npoel; k++) elsup[fill[lnods[e * npoel + k]]++] = (int)e;
+  }
+  rowptr.assign((size_t)nnode + 1, 0);
+  std::vector<int> mark((size_t)nnode, -1);
+  for (int i = 0; i < nnode; i++) {
+    mark[i] = i;
+    unsigned c = 0;
+    for (int k = elsup_ptr[i]; k < elsup_ptr[i + 1]; k++) {
+      const int* no = &lnods[(long long)elsup[k] * npoel];
+      for (int a = 0; a < npoel; a++)
+        if (mark[no[a]] != i) { mark[no[a]] = i; c++; }
+    }
+    rowptr[i + 1] = rowptr[i] + c;
+  }
+  colind.assign(rowptr[nnode], 0);
+  std::fill(mark.begin(), mark.end(), -1);
+  for (int i = 0; i < nnode; i++) {
+    mark[i] = i;
+    unsigned pos = rowptr[i];
+    for (int k = elsup_ptr[i]; k < elsup_ptr[i + 1]; k++) {
+      const int* no = &lnods[(long long)elsup[k] * npoel];
+      for (int a = 0; a < npoel; a++)
+        if (mark[no[a]] != i) { mark[no[a]] = i; colind[pos++] = (unsigned)no[a]; }
+    }
+    std::sort(colind.begin() + rowptr[i], colind.begin() + rowptr[i + 1]);
+  }
+}
+
+// Permuted pattern as CMat_BlkCrs::AddPattern(rhs, order, order) builds it (src/matvec/mat_blkcrs.cpp:493-549:
+// new row i0 takes old row order[i0], columns relabelled and sorted) and m_DiaInd
+// (src/matvec/matdiafrac_blkcrs.cpp:263-277).
+void build_device_pattern(Sys& s) {
+  const int n = s.n;
+  s.rowptr.assign((size_t)n + 1, 0);
+  for (int i0 = 0; i0 < n; i0++) {
+    const int i1 = s.order[i0];
+    s.rowptr[i0 + 1] = s.rowptr[i0] + (s.rowptr0[i1 + 1] - s.rowptr0[i1]);
+  }
+  s.colind.assign(s.rowptr[n], 0);
+  s.perm_crs.assign(s.rowptr0[n], 0);
+  std::vector<std::pair<unsigned, unsigned>> tmp;
+  for (int i0 = 0; i0 < n; i0++) {
+    const int i1 = s.order[i0];
+    tmp.clear();
+    for (unsigned q = s.rowptr0[i1]; q < s.rowptr0[i1 + 1]; q++) tmp.push_back({(unsigned)s.inv[s.colind0[q]], q});
+    std::sort(tmp.begin(), tmp.end());
+    unsigned pos = s.rowptr[i0];
+    for (auto& t : tmp) {
+      s.colind[pos] = t.first;
+      s.perm_crs[t.second] = pos;
+      pos++;
+    }
+  }
+  s.diaind.assign((size_t)n, 0);
+  for (int i = 0; i < n; i++) {
+    s.diaind[i] = s.rowptr[i + 1];
+    for (unsigned q = s.rowptr[i]; q < s.rowptr[i + 1]; q++)
+      if (s.colind[q] > (unsigned)i) { s.diaind[i] = q; break; }
+  }
+}
+
+static const int kGroup = 64;  // rows per group: one wavefront, one row per lane
+
+// Level sets of the whole matrix (natural or arbitrary orderings): every level is a phase of independent
+// rows, handed out in chunks of 64 through a row list.
+static void level_sweeps(Sys& s) {
+  const int n = s.n;
+  for (int dir = 0; dir < 2; dir++) {
+    Sweep& w = dir == 0 ? s.fwd : s.bwd;
+    std::vector<int> lev((size_t)n, 0);
+    int nlev = 0;
+    if (dir == 0) {
+      for (int i = 0; i < n; i++) {
+        int l = 0;
+        for (unsigned q = s.rowptr[i]; q < s.diaind[i]; q++) l = std::max(l, lev[s.colind[q]] + 1);
+        lev[i] = l;
+        nlev = std::max(nlev, l + 1);
+      }
+    } else {
+      for (int i = n - 1; i >= 0; i--) {
+        int l = 0;
+        for (unsigned q = s.diaind[i]; q < s.rowptr[i + 1]; q++) l = std::max(l, lev[s.colind[q]] + 1);
+        lev[i] = l;
+        nlev = std::max(nlev, l + 1);
+      }
+    }
+    std::vector<int> cnt((size_t)nlev + 1, 0);
+    for (int i = 0; i < n; i++) cnt[lev[i] + 1]++;
+    for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+    w.rowlist.assign((size_t)n, 0);
+    {
+      std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+      for (int i = 0; i < n; i++) w.rowlist[fill[lev[i]]++] = i;
+    }
+    // groups are (start, end) pairs into the row list, in execution order
+    w.phase_gptr.assign(1, 0);
+    w.phase_nlev.clear();
+    w.grp.clear();
+    for (int l = 0; l < nlev; l++) {
+      for (int r = cnt[l]; r < cnt[l + 1]; r += kGroup) {
+        w.grp.push_back(r);
+        w.grp.push_back(std::min(r + kGroup, cnt[l + 1]));
+      }
+      w.phase_gptr.push_back((int)w.grp.size() / 2);
+      w.phase_nlev.push_back(1);
+    }
+    w.lev.assign((size_t)n, 0);
+  }
+  s.lsplit = s.diaind;
+  s.usplit = s.diaind;
+}
+
+// Contiguous groups (MC / BMC orderings): group g = device rows [gstart[g], gstart[g+1]) of colour
+// colour_of_group[g]; groups of one colour are independent, dependencies inside a group are resolved by
+// intra-group levels.
+static void group_sweeps(Sys& s, const std::vector<int>& gstart, const std::vector<int>& gcolor, int ncolors) {
+  const int n = s.n;
+  const int ngroup = (int)gstart.size() - 1;
+  std::vector<int> grp_of((size_t)n);
+  for (int g = 0; g < ngroup; g++)
+    for (int r = gstart[g]; r < gstart[g + 1]; r++) grp_of[r] = g;
+  s.lsplit.assign((size_t)n, 0);
+  s.usplit.assign((size_t)n, 0);
+  std::vector<int> flev((size_t)n, 0), blev((size_t)n, 0);
+  for (int i = 0; i < n; i++) {
+    const int g = grp_of[i], g0 = gstart[g];
+    unsigned ls = s.diaind[i];
+    for (unsigned q = s.rowptr[i]; q < s.diaind[i]; q++) {
+      const int j = (int)s.colind[q];
+      if (j >= g0) { ls = q; break; }
+      if (gcolor[grp_of[j]] >= gcolor[g]) throw std::runtime_error("ordering: lower dependency inside a colour");
+    }
+    s.lsplit[i] = ls;
+    int l = 0;
+    for (unsigned q = ls; q < s.diaind[i]; q++) l = std::max(l, flev[s.colind[q]] + 1);
+    flev[i] = l;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    const int g = grp_of[i], g1 = gstart[g + 1];
+    unsigned us = s.rowptr[i + 1];
+    for (unsigned q = s.diaind[i]; q < s.rowptr[i + 1]; q++) {
+      const int j = (int)s.colind[q];
+      if (j >= g1) {
+        if (us == s.rowptr[i + 1]) us = q;
+        if (gcolor[grp_of[j]] <= gcolor[g]) throw std::runtime_error("ordering: upper dependency inside a colour");
+      }
+    }
+    s.usplit[i] = us;
+    int l = 0;
+    for (unsigned q = s.diaind[i]; q < us; q++) l = std::max(l, blev[s.colind[q]] + 1);
+    blev[i] = l;
+  }
+  // groups are ordered by colour in the device numbering
+  std::vector<int> cstart((size_t)ncolors + 1, 0);
+  for (int g = 0; g < ngroup; g++) cstart[gcolor[g] + 1]++;
+  for (int c = 0; c < ncolors; c++) cstart[c + 1] += cstart[c];
+  for (int g = 1; g < ngroup; g++)
+    if (gcolor[g] < gcolor[g - 1]) throw std::runtime_error("ordering: groups not colour-major");
+  for (int dir = 0; dir < 2; dir++) {
+    Sweep& w = dir == 0 ? s.fwd : s.bwd;
+    const std::vector<int>& lev = dir == 0 ? flev : blev;
+    w.rowlist.clear();
+    w.lev.assign((size_t)n, 0);
+    for (int i = 0; i < n; i++) w.lev[i] = (uint8_t)lev[i];
+    // phase p runs colour p (forward) or colour ncolors-1-p (backward); group lists in execution order
+    w.grp.clear();
+    w.phase_gptr.assign(1, 0);
+    w.phase_nlev.clear();
+    std::vector<int> gl;  // group starts in execution order, then the final end
+    for (int p = 0; p < ncolors; p++) {
+      const int c = dir == 0 ? p : ncolors - 1 - p;
+      int nl = 1;
+      for (int g = cstart[c]; g < cstart[c + 1]; g++) {
+        int mx = 0;
+        for (int r = gstart[g]; r < gstart[g + 1]; r++) mx = std::max(mx, lev[r]);
+        nl = std::max(nl, mx + 1);
+        gl.push_back(g);
+      }
+      w.phase_gptr.push_back((int)gl.size());
+      w.phase_nlev.push_back(nl);
+    }
+    // groups are stored as (start, end) pairs in execution order: grp[2k], grp[2k+1]
+    w.grp.resize(gl.size() * 2);
+    for (size_t k = 0; k < gl.size(); k++) {
+      w.grp[2 * k] = gstart[gl[k]];
+      w.grp[2 * k + 1] = gstart[gl[k] + 1];
+    }
+  }
+}
+
+void build_schedule(Sys& s) {
+  if (s.color_of.empty()) {
+    level_sweeps(s);
+    return;
+  }
+  // contiguous groups: tiles (BMC) or 64-row chunks of each colour (MC)
+  std::vector<int> gstart(1, 0), gcolor;
+  const int n = s.n;
+  int i = 0;
+  while (i < n) {
+    const int c = s.color_of[i];
+    const int t = s.tile_of.empty() ? -1 : s.tile_of[i];
+    int j = i + 1;
+    while (j < n && s.color_of[j] == c && (t < 0 ? (j - i) < kGroup : s.tile_of[j] == t)) j++;
+    if (j - i > kGroup) throw std::runtime_error("ordering: tile larger than 64 rows");
+    gstart.push_back(j);
+    gcolor.push_back(c);
+    i = j;
+  }
+  group_sweeps(s, gstart, gcolor, s.ncolors);
+}
+
+// Greedy node colouring in natural order; order = colour-major, natural inside a colour.
+void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors) {
+  const int n = s.n;
+  std::vector<int> col((size_t)n, -1);
+  std::vector<int> used;
+  ncolors = 0;
+  for (int i = 0; i < n; i++) {
+    used.assign((size_t)ncolors + 1, 0);
+    for (unsigned q = s.rowptr0[i]; q < s.rowptr0[i + 1]; q++) {
+      const int c = col[s.colind0[q]];
+      if (c >= 0) used[c] = 1;
+    }
+    int c = 0;
+    while (used[c]) c++;
+    col[i] = c;
+    ncolors = std::max(ncolors, c + 1);
+  }
+  order.resize(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return col[a] < col[b]; });
+  color_of_new.resize(n);
+  for (int k = 0; k < n; k++) color_of_new[k] = col[order[k]];
+}
+
+// Block multicolour ordering. Nodes are binned into geometric tiles of about `tedge` element spacings per
+// axis (8 x 8 nodes in 2D, 4 x 4 x 4 in 3D by default, so a tile fits one wavefront and its intra-tile
+// dependency chain stays short); a tile holding more than 64 nodes is split into chunks of 64 in natural
+// order. Tiles are coloured greedily on the tile adjacency graph, in the lexicographic order of their
+// (z, y, x) bins; the order is colour-major, tile-major, natural inside a tile. Any such order is a
+// permutation that CPreconditioner_ILU::SetOrdering accepts, so the factorisation it induces is the
+// reference's ILU(0) under that ordering.
+void make_blockcolor(const Sys& s, int tedge, std::vector<int>& order, std::vector<int>& color_of_new,
+                     std::vector<int>& tile_of_new, int& ncolors) {
+  const int n = s.n, nd = s.ndim, npoel = s.npoel;
+  if (tedge <= 0) tedge = (nd == 2) ? 8 : 4;
+  // element spacing from the mean element measure (right triangles of h^2/2, Kuhn tets of h^3/6)
+  double meas = 0.0;
+  for (int e = 0; e < s.nelem; e++) {
+    const int* no = &s.lnods[(size_t)e * npoel];
+    const double* p0 = &s.coords[(size_t)no[0] * nd];
+    double d[3][3];
+    for (int k = 1; k < npoel; k++)
+      for (int c = 0; c < nd; c++) d[k - 1][c] = s.coords[(size_t)no[k] * nd + c] - p0[c];
+    if (nd == 2) meas += 0.5 * std::fabs(d[0][0] * d[1][1] - d[0][1] * d[1][0]);
+    else
+      meas += std::fabs(d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) - d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
+                        d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0])) / 6.0;
+  }
+  meas /= std::max(1, s.nelem);
+  const double h = (nd == 2) ? std::sqrt(2.0 * meas) : std::cbrt(6.0 * meas);
+  double lo[3] = {1e300, 1e300, 1e300};
+  for (int i = 0; i < n; i++)
+    for (int c = 0; c < nd; c++) lo[c] = std::min(lo[c], s.coords[(size_t)i * nd + c]);
+  std::vector<long long> key(n);
+  for (int i = 0; i < n; i++) {
+    long long t[3] = {0, 0, 0};
+    for (int c = 0; c < nd; c++) {
+      const long long g = (long long)std::floor((s.coords[(size_t)i * nd + c] - lo[c]) / h + 0.5);
+      t[c] = g / tedge;
+    }
+    key[i] = (t[2] << 42) | (t[1] << 21) | t[0];
+  }
+  // tiles in lexicographic (z, y, x) bin order; nodes in natural order inside a tile, split at 64
+  std::vector<int> byk(n);
+  std::iota(byk.begin(), byk.end(), 0);
+  std::stable_sort(byk.begin(), byk.end(), [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<int> tile(n);
+  int ntile = 0;
+  for (int k = 0; k < n;) {
+    int e = k;
+    while (e < n && key[byk[e]] == key[byk[k]]) e++;
+    for (int c = k; c < e; c += kGroup) {
+      for (int j = c; j < std::min(e, c + kGroup); j++) tile[byk[j]] = ntile;
+      ntile++;
+    }
+    k = e;
+  }
+  std::vector<std::vector<int>> tadj((size_t)ntile);
+  for (int i = 0; i < n; i++)
+    for (unsigned q = s.rowptr0[i]; q < s.rowptr0[i + 1]; q++) {
+      const int a = tile[i], c = tile[s.colind0[q]];
+      if (a != c) tadj[a].push_back(c);
+    }
+  std::vector<int> tcol((size_t)ntile, -1);
+  ncolors = 0;
+  std::vector<char> used;
+  for (int t = 0; t < ntile; t++) {
+    used.assign((size_t)ncolors + 1, 0);
+    for (int c : tadj[t])
+      if (tcol[c] >= 0) used[tcol[c]] = 1;
+    int c = 0;
+    while (used[c]) c++;
+    tcol[t] = c;
+    ncolors = std::max(ncolors, c + 1);
+  }
+  order = byk;  // already tile-major (tiles in creation order), natural inside a tile
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tcol[tile[a]] < tcol[tile[b]]; });
+  color_of_new.resize(n);
+  tile_of_new.resize(n);
+  for (int k = 0; k < n; k++) {
+    color_of_new[k] = tcol[tile[order[k]]];
+    tile_of_new[k] = tile[order[k]];
+  }
+}
+
+}  // namespace dfm

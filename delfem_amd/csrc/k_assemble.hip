@@ -1,0 +1,477 @@
+// Element assembly on the device: one thread per matrix row (device numbering) gathers the contributions
+// of the elements around its node in ascending element index, which is the order in which the reference's
+// element loop merges them (CMatDia_BlkCrs::Mearge, src/matvec/matdia_blkcrs.cpp:165-258, called per element
+// from AddLinSys_*). Every entry is therefore the same sum, in the same order, as in the reference: the
+// result is bit-identical, deterministic and needs no atomics. Each element row is recomputed by each of
+// its nodes (3 or 4 times); the element kernels are cheap next to the memory traffic of the sweep.
+//
+// The element arithmetic follows the reference expression by expression; the library is compiled with
+// -ffp-contract=off so that no product is fused into an FMA.
+#include "dfm_internal.hpp"
+#include "kernels.hpp"
+
+namespace dfm {
+
+// ---------------------------------------------------------------------------------------------------------
+// P1 geometry: src/femeqn/ker_emat_tri.cpp:13-15 (TriArea), :36-52 (TriDlDx);
+// include/delfem/femeqn/ker_emat_tet.h:32-39 (TetVolume, with its 0.1666666666666 literal), :42-77 (TetDlDx)
+
+__device__ inline double tri_area(const double* p0, const double* p1, const double* p2) {
+  return 0.5 * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+__device__ inline void tri_dldx(double d[3][2], const double co[4][3]) {
+  const double* p0 = co[0];
+  const double* p1 = co[1];
+  const double* p2 = co[2];
+  const double area = tri_area(p0, p1, p2);
+  const double t = 0.5 / area;
+  d[0][0] = t * (p1[1] - p2[1]);
+  d[1][0] = t * (p2[1] - p0[1]);
+  d[2][0] = t * (p0[1] - p1[1]);
+  d[0][1] = t * (p2[0] - p1[0]);
+  d[1][1] = t * (p0[0] - p2[0]);
+  d[2][1] = t * (p1[0] - p0[0]);
+}
+
+__device__ inline double tet_volume(const double co[4][3]) {
+  const double* p0 = co[0];
+  const double* p1 = co[1];
+  const double* p2 = co[2];
+  const double* p3 = co[3];
+  double vol = (p1[0] - p0[0]) * ((p2[1] - p0[1]) * (p3[2] - p0[2]) - (p3[1] - p0[1]) * (p2[2] - p0[2])) -
+               (p1[1] - p0[1]) * ((p2[0] - p0[0]) * (p3[2] - p0[2]) - (p3[0] - p0[0]) * (p2[2] - p0[2])) +
+               (p1[2] - p0[2]) * ((p2[0] - p0[0]) * (p3[1] - p0[1]) - (p3[0] - p0[0]) * (p2[1] - p0[1]));
+  vol *= 0.1666666666666;
+  return vol;
+}
+
+__device__ inline void tet_dldx(double d[4][3], const double co[4][3]) {
+  const double* p0 = co[0];
+  const double* p1 = co[1];
+  const double* p2 = co[2];
+  const double* p3 = co[3];
+  const double vol = tet_volume(co);
+  const double t = 1.0 / (vol * 6.0);
+  d[0][0] = -t * ((p2[1] - p1[1]) * (p3[2] - p1[2]) - (p3[1] - p1[1]) * (p2[2] - p1[2]));
+  d[0][1] = +t * ((p2[0] - p1[0]) * (p3[2] - p1[2]) - (p3[0] - p1[0]) * (p2[2] - p1[2]));
+  d[0][2] = -t * ((p2[0] - p1[0]) * (p3[1] - p1[1]) - (p3[0] - p1[0]) * (p2[1] - p1[1]));
+  d[1][0] = +t * ((p3[1] - p2[1]) * (p0[2] - p2[2]) - (p0[1] - p2[1]) * (p3[2] - p2[2]));
+  d[1][1] = -t * ((p3[0] - p2[0]) * (p0[2] - p2[2]) - (p0[0] - p2[0]) * (p3[2] - p2[2]));
+  d[1][2] = +t * ((p3[0] - p2[0]) * (p0[1] - p2[1]) - (p0[0] - p2[0]) * (p3[1] - p2[1]));
+  d[2][0] = -t * ((p0[1] - p3[1]) * (p1[2] - p3[2]) - (p1[1] - p3[1]) * (p0[2] - p3[2]));
+  d[2][1] = +t * ((p0[0] - p3[0]) * (p1[2] - p3[2]) - (p1[0] - p3[0]) * (p0[2] - p3[2]));
+  d[2][2] = -t * ((p0[0] - p3[0]) * (p1[1] - p3[1]) - (p1[0] - p3[0]) * (p0[1] - p3[1]));
+  d[3][0] = +t * ((p1[1] - p0[1]) * (p2[2] - p0[2]) - (p2[1] - p0[1]) * (p1[2] - p0[2]));
+  d[3][1] = -t * ((p1[0] - p0[0]) * (p2[2] - p0[2]) - (p2[0] - p0[0]) * (p1[2] - p0[2]));
+  d[3][2] = +t * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Element rows. Each computes block row `a` of the element matrix, em[c*B*B + r*B + s] for block (a, c),
+// and the element residual of node a, er[0..B).
+
+// src/femeqn/eqn_poisson.cpp:49-125 (2D), :249-323 (3D). p = (alpha, source)
+template <int NDIM>
+__device__ inline void row_poisson(int a, const double co[4][3], const double* val, const double* p, double* em,
+                                   double* er) {
+  const double alpha = p[0], source = p[1];
+  if (NDIM == 2) {
+    const double area = tri_area(co[0], co[1], co[2]);
+    double d[3][2];
+    tri_dldx(d, co);
+    for (int j = 0; j < 3; j++) em[j] = alpha * area * (d[a][0] * d[j][0] + d[a][1] * d[j][1]);
+    double r = source * area * 0.33333333333333333;
+    for (int j = 0; j < 3; j++) r -= em[j] * val[j];
+    er[0] = r;
+  } else {
+    const double vol = tet_volume(co);
+    double d[4][3];
+    tet_dldx(d, co);
+    for (int j = 0; j < 4; j++) em[j] = alpha * vol * (d[a][0] * d[j][0] + d[a][1] * d[j][1] + d[a][2] * d[j][2]);
+    double r = source * vol * 0.25;
+    for (int j = 0; j < 4; j++) r -= em[j] * val[j];
+    er[0] = r;
+  }
+}
+
+// src/femeqn/eqn_diffusion.cpp:240-345. p = (rho, alpha, source), glob = (dt, gamma)
+__device__ inline void row_diffusion2d(int a, const double co[4][3], const double* val, const double* vval,
+                                       const double* p, double dt, double gamma, double* em, double* er) {
+  const double rho = p[0], alpha = p[1], source = p[2];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double d[3][2];
+  tri_dldx(d, co);
+  double eC[3], eM[3];
+  for (int j = 0; j < 3; j++) eC[j] = alpha * area * (d[a][0] * d[j][0] + d[a][1] * d[j][1]);
+  const double m = rho * area * 0.08333333333333333;
+  for (int j = 0; j < 3; j++) eM[j] = m;
+  eM[a] += m;
+  double r = source * area * 0.333333333333333333;
+  const double gdt = gamma * dt;
+  for (int j = 0; j < 3; j++) em[j] = eM[j] + gdt * eC[j];
+  for (int j = 0; j < 3; j++) r -= eC[j] * (val[j] + dt * vval[j]) + eM[j] * vval[j];
+  er[0] = r;
+}
+
+// src/femeqn/eqn_linear_solid2d.cpp:505-598. p = (lambda, myu, rho, gx, gy)
+__device__ inline void row_solid2d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
+                                   double* er) {
+  const double lambda = p[0], myu = p[1], rho = p[2], g_x = p[3], g_y = p[4];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double d[3][2];
+  tri_dldx(d, co);
+  for (int j = 0; j < 3; j++) {
+    double* e = &em[j * 4];
+    e[0] = area * (lambda + myu) * d[a][0] * d[j][0];
+    e[1] = area * (lambda * d[a][0] * d[j][1] + myu * d[j][0] * d[a][1]);
+    e[2] = area * (lambda * d[a][1] * d[j][0] + myu * d[j][1] * d[a][0]);
+    e[3] = area * (lambda + myu) * d[a][1] * d[j][1];
+    const double t = (d[a][1] * d[j][1] + d[a][0] * d[j][0]) * area * myu;
+    e[0] += t;
+    e[3] += t;
+  }
+  double r0 = area * rho * g_x * 0.33333333333333333;
+  double r1 = area * rho * g_y * 0.33333333333333333;
+  for (int j = 0; j < 3; j++) {
+    const double* e = &em[j * 4];
+    r0 -= e[0] * u[j][0] + e[1] * u[j][1];
+    r1 -= e[2] * u[j][0] + e[3] * u[j][1];
+  }
+  er[0] = r0;
+  er[1] = r1;
+}
+
+// src/femeqn/eqn_linear_solid3d.cpp:241-334. p = (lambda, myu, rho, gx, gy, gz)
+__device__ inline void row_solid3d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
+                                   double* er) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double vol = tet_volume(co);
+  double d[4][3];
+  tet_dldx(d, co);
+  for (int j = 0; j < 4; j++) {
+    double* e = &em[j * 9];
+    for (int r = 0; r < 3; r++)
+      for (int s = 0; s < 3; s++) e[r * 3 + s] = vol * (lambda * d[a][r] * d[j][s] + myu * d[j][r] * d[a][s]);
+    const double t = d[a][0] * d[j][0] + d[a][1] * d[j][1] + d[a][2] * d[j][2];
+    e[0] += vol * myu * t;
+    e[4] += vol * myu * t;
+    e[8] += vol * myu * t;
+  }
+  for (int r = 0; r < 3; r++) {
+    double x = vol * rho * p[3 + r] * 0.25;
+    for (int j = 0; j < 4; j++) {
+      const double* e = &em[j * 9];
+      x -= e[r * 3 + 0] * u[j][0] + e[r * 3 + 1] * u[j][1] + e[r * 3 + 2] * u[j][2];
+    }
+    er[r] = x;
+  }
+}
+
+// src/femeqn/eqn_st_venant.cpp:47-156 (AddElemMatFin_StVenant2D) and :157-272. p = (lambda, myu, rho, gx, gy)
+__device__ inline void row_stvk2d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
+                                  double* er) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double dn[3][2];
+  tri_dldx(dn, co);
+  double du[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int n = 0; n < 3; n++)
+    for (int r = 0; r < 2; r++)
+      for (int s = 0; s < 2; s++) du[r][s] += u[n][r] * dn[n][s];
+  const double w = area;
+  double st[2][2];
+  for (int r = 0; r < 2; r++)
+    for (int s = 0; s < 2; s++) {
+      st[r][s] = 0.5 * (du[r][s] + du[s][r]);
+      for (int k = 0; k < 2; k++) st[r][s] += 0.5 * du[k][r] * du[k][s];
+    }
+  double z[2][2];
+  for (int r = 0; r < 2; r++) {
+    for (int s = 0; s < 2; s++) z[r][s] = du[r][s];
+    z[r][r] += 1.0;
+  }
+  // d2s[k][kd][id][jd] = dn[k][jd] * z[kd][id]
+  double sg[2][2];
+  {
+    double tr = 0.0;
+    for (int r = 0; r < 2; r++) {
+      for (int s = 0; s < 2; s++) sg[r][s] = 2.0 * myu * st[r][s];
+      tr += st[r][r];
+    }
+    for (int r = 0; r < 2; r++) sg[r][r] += lambda * tr;
+  }
+  double fin[2] = {0.0, 0.0};
+  for (int kd = 0; kd < 2; kd++)
+    for (int r = 0; r < 2; r++)
+      for (int s = 0; s < 2; s++) fin[kd] += w * sg[r][s] * (dn[a][s] * z[kd][r]);
+  for (int j = 0; j < 3; j++) {
+    double* e = &em[j * 4];
+    for (int k = 0; k < 4; k++) e[k] = 0.0;
+    for (int id = 0; id < 2; id++)
+      for (int jd = 0; jd < 2; jd++) {
+        {
+          double t = 0.0;
+          for (int gd = 0; gd < 2; gd++)
+            for (int hd = 0; hd < 2; hd++) {
+              const double di = dn[a][hd] * z[id][gd];
+              t += di * (dn[j][hd] * z[jd][gd]) + di * (dn[j][gd] * z[jd][hd]);
+            }
+          e[id * 2 + jd] += w * myu * t;
+        }
+        {
+          double t1 = 0.0, t2 = 0.0;
+          for (int gd = 0; gd < 2; gd++) {
+            t1 += dn[a][gd] * z[id][gd];
+            t2 += dn[j][gd] * z[jd][gd];
+          }
+          e[id * 2 + jd] += w * lambda * t1 * t2;
+        }
+      }
+    {
+      double t = 0.0;
+      for (int gd = 0; gd < 2; gd++)
+        for (int hd = 0; hd < 2; hd++) t += sg[gd][hd] * dn[a][hd] * dn[j][gd];
+      for (int id = 0; id < 2; id++) e[id * 2 + id] += w * t;
+    }
+  }
+  const double g[2] = {p[3], p[4]};
+  for (int r = 0; r < 2; r++) {
+    double fex = 0.0;
+    fex += area * rho * g[r] / 3.0;
+    er[r] = fex - fin[r];
+  }
+}
+
+// src/femeqn/eqn_st_venant.cpp:724-834 (SetElemMatFin_StVenant3D) and :838-939. p = (lambda, myu, rho, gx,gy,gz)
+__device__ inline void row_stvk3d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
+                                  double* er) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double vol = tet_volume(co);
+  double dn[4][3];
+  tet_dldx(dn, co);
+  double du[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int n = 0; n < 4; n++)
+    for (int r = 0; r < 3; r++)
+      for (int s = 0; s < 3; s++) du[r][s] += u[n][r] * dn[n][s];
+  const double w = vol;
+  double sg[6];
+  {
+    double e[6];
+    e[0] = du[0][0] + 0.5 * (du[0][0] * du[0][0] + du[1][0] * du[1][0] + du[2][0] * du[2][0]);
+    e[1] = du[1][1] + 0.5 * (du[0][1] * du[0][1] + du[1][1] * du[1][1] + du[2][1] * du[2][1]);
+    e[2] = du[2][2] + 0.5 * (du[0][2] * du[0][2] + du[1][2] * du[1][2] + du[2][2] * du[2][2]);
+    e[3] = 0.5 * (du[0][1] + du[1][0] + du[0][0] * du[0][1] + du[1][0] * du[1][1] + du[2][0] * du[2][1]);
+    e[4] = 0.5 * (du[1][2] + du[2][1] + du[0][1] * du[0][2] + du[1][1] * du[1][2] + du[2][1] * du[2][2]);
+    e[5] = 0.5 * (du[2][0] + du[0][2] + du[0][2] * du[0][0] + du[1][2] * du[1][0] + du[2][2] * du[2][0]);
+    const double tr = e[0] + e[1] + e[2];
+    sg[0] = 2.0 * myu * e[0] + lambda * tr;
+    sg[1] = 2.0 * myu * e[1] + lambda * tr;
+    sg[2] = 2.0 * myu * e[2] + lambda * tr;
+    sg[3] = 2.0 * myu * e[3];
+    sg[4] = 2.0 * myu * e[4];
+    sg[5] = 2.0 * myu * e[5];
+  }
+  const double z[3][3] = {{du[0][0] + 1, du[0][1], du[0][2]},
+                          {du[1][0], du[1][1] + 1, du[1][2]},
+                          {du[2][0], du[2][1], du[2][2] + 1}};
+  double d2s[4][3][6];
+  for (int k = 0; k < 4; k++)
+    for (int kd = 0; kd < 3; kd++) {
+      d2s[k][kd][0] = dn[k][0] * z[kd][0];
+      d2s[k][kd][1] = dn[k][1] * z[kd][1];
+      d2s[k][kd][2] = dn[k][2] * z[kd][2];
+      d2s[k][kd][3] = (dn[k][0] * z[kd][1] + dn[k][1] * z[kd][0]);
+      d2s[k][kd][4] = (dn[k][1] * z[kd][2] + dn[k][2] * z[kd][1]);
+      d2s[k][kd][5] = (dn[k][2] * z[kd][0] + dn[k][0] * z[kd][2]);
+    }
+  for (int kd = 0; kd < 3; kd++) {
+    double t = 0;
+    t += sg[0] * d2s[a][kd][0];
+    t += sg[1] * d2s[a][kd][1];
+    t += sg[2] * d2s[a][kd][2];
+    t += sg[3] * d2s[a][kd][3];
+    t += sg[4] * d2s[a][kd][4];
+    t += sg[5] * d2s[a][kd][5];
+    er[kd] = t * w;  // internal force; the residual is formed below
+  }
+  // material part: the reference fills the pairs (ino > jno) and mirrors them, then the diagonal blocks
+  for (int j = 0; j < 4; j++) {
+    // (ino, jno) = (a, j) for j < a; (j, a) mirrored for j > a; diagonal formula for j == a
+    const int ino = (j <= a) ? a : j;
+    const int jno = (j <= a) ? j : a;
+    for (int r = 0; r < 3; r++)
+      for (int s = 0; s < 3; s++) {
+        const int id = (j <= a) ? r : s;
+        const int jd = (j <= a) ? s : r;
+        double t = 0.0;
+        t += d2s[ino][id][0] * d2s[jno][jd][0];
+        t += d2s[ino][id][1] * d2s[jno][jd][1];
+        t += d2s[ino][id][2] * d2s[jno][jd][2];
+        t += d2s[ino][id][3] * d2s[jno][jd][3] * 0.5;
+        t += d2s[ino][id][4] * d2s[jno][jd][4] * 0.5;
+        t += d2s[ino][id][5] * d2s[jno][jd][5] * 0.5;
+        const double t2 = d2s[ino][id][0] + d2s[ino][id][1] + d2s[ino][id][2];
+        const double t3 = d2s[jno][jd][0] + d2s[jno][jd][1] + d2s[jno][jd][2];
+        em[j * 9 + r * 3 + s] = w * (myu * 2 * t + lambda * t2 * t3);
+      }
+  }
+  // geometric part
+  for (int j = 0; j < 4; j++) {
+    double t = 0.0;
+    t += sg[0] * dn[a][0] * dn[j][0];
+    t += sg[1] * dn[a][1] * dn[j][1];
+    t += sg[2] * dn[a][2] * dn[j][2];
+    t += sg[3] * (dn[a][0] * dn[j][1] + dn[a][1] * dn[j][0]);
+    t += sg[4] * (dn[a][1] * dn[j][2] + dn[a][2] * dn[j][1]);
+    t += sg[5] * (dn[a][2] * dn[j][0] + dn[a][0] * dn[j][2]);
+    em[j * 9 + 0] += w * t;
+    em[j * 9 + 4] += w * t;
+    em[j * 9 + 8] += w * t;
+  }
+  for (int r = 0; r < 3; r++) {
+    double fex = 0.0;
+    fex += vol * rho * p[3 + r] / 4.0;
+    er[r] = fex - er[r];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+
+template <int KIND, int NDIM, int B>
+__global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 >= A.n) return;
+  constexpr int NNO = NDIM + 1;
+  constexpr int BS = B * B;
+  const int i1 = A.order ? A.order[i0] : i0;
+  const unsigned q0 = A.rowptr[i0], q1 = A.rowptr[i0 + 1];
+  double* dia = &A.dia[(long long)i0 * BS];
+  double* res = &A.res[(long long)i0 * B];
+  double racc[B];
+  for (int d = 0; d < B; d++) racc[d] = res[d];
+  for (int k = A.elsup_ptr[i1]; k < A.elsup_ptr[i1 + 1]; k++) {
+    const int e = A.elsup[k];
+    int no[NNO];
+    for (int c = 0; c < NNO; c++) no[c] = A.lnods[(long long)e * NNO + c];
+    int a = 0;
+    for (int c = 0; c < NNO; c++)
+      if (no[c] == i1) a = c;
+    double co[4][3], u[4][3], val[4], vval[4];
+    for (int c = 0; c < NNO; c++) {
+      for (int d = 0; d < 3; d++) co[c][d] = (d < NDIM) ? A.coords[(long long)no[c] * NDIM + d] : 0.0;
+      for (int d = 0; d < 3; d++) u[c][d] = (A.u && d < A.len) ? A.u[(long long)no[c] * A.len + d] : 0.0;
+      val[c] = A.u ? A.u[(long long)no[c] * A.len] : 0.0;
+      vval[c] = A.v ? A.v[(long long)no[c] * A.len] : 0.0;
+    }
+    const double* p = &A.params[(long long)e * A.pstride];
+    double em[NNO * BS], er[B];
+    if (KIND == DFM_POISSON) row_poisson<NDIM>(a, co, val, p, em, er);
+    if (KIND == DFM_DIFFUSION2D) row_diffusion2d(a, co, val, vval, p, A.glob0, A.glob1, em, er);
+    if (KIND == DFM_SOLID2D) row_solid2d(a, co, u, p, em, er);
+    if (KIND == DFM_SOLID3D) row_solid3d(a, co, u, p, em, er);
+    if (KIND == DFM_STVK2D) row_stvk2d(a, co, u, p, em, er);
+    if (KIND == DFM_STVK3D) row_stvk3d(a, co, u, p, em, er);
+    for (int c = 0; c < NNO; c++) {
+      const double* blk = &em[c * BS];
+      if (c == a) {
+        for (int t = 0; t < BS; t++) dia[t] += blk[t];
+      } else {
+        const unsigned j0 = (unsigned)(A.inv ? A.inv[no[c]] : no[c]);
+        for (unsigned q = q0; q < q1; q++) {
+          if (A.colind[q] == j0) {
+            double* o = &A.crs[(long long)q * BS];
+            for (int t = 0; t < BS; t++) o[t] += blk[t];
+            break;
+          }
+        }
+      }
+    }
+    for (int d = 0; d < B; d++) racc[d] += er[d];
+  }
+  // surface loads (a reference extension): added after the element contributions, in face order
+  if (A.load_ptr) {
+    for (int k = A.load_ptr[i1]; k < A.load_ptr[i1 + 1]; k++)
+      for (int d = 0; d < B; d++) racc[d] += A.loads[(long long)k * B + d];
+  }
+  for (int d = 0; d < B; d++) res[d] = racc[d];
+}
+
+hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
+  const int nt = 128;
+  const int nb = (a.n + nt - 1) / nt;
+  if (a.n == 0) return hipSuccess;
+#define DFM_ASM(K, D, B) \
+  if (a.kind == K && a.ndim == D && a.len == B) { hipLaunchKernelGGL((k_assemble<K, D, B>), dim3(nb), dim3(nt), 0, st, a); return hipGetLastError(); }
+  DFM_ASM(DFM_POISSON, 2, 1)
+  DFM_ASM(DFM_POISSON, 3, 1)
+  DFM_ASM(DFM_DIFFUSION2D, 2, 1)
+  DFM_ASM(DFM_SOLID2D, 2, 2)
+  DFM_ASM(DFM_SOLID3D, 3, 3)
+  DFM_ASM(DFM_STVK2D, 2, 2)
+  DFM_ASM(DFM_STVK3D, 3, 3)
+#undef DFM_ASM
+  return hipErrorInvalidValue;
+}
+
+// CMatDia_BlkCrs::SetBoundaryCondition (src/matvec/matdia_blkcrs.cpp:313-342) and CBCFlag::SetZeroToBCDof
+// (include/delfem/matvec/bcflag_blk.h:102-129): rows and columns of fixed dofs are zeroed, their diagonal
+// set to one, their residual zeroed. Only assignments of 0 and 1, so any traversal order gives the same
+// matrix; one thread per row.
+template <int B>
+__global__ void k_set_bc(int n, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res,
+                         const int* bc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int BS = B * B;
+  int fi[B];
+  for (int d = 0; d < B; d++) fi[d] = bc[(long long)i * B + d];
+  double* D = &dia[(long long)i * BS];
+  for (int id = 0; id < B; id++) {
+    if (!fi[id]) continue;
+    for (int k = 0; k < B; k++) D[k * B + id] = 0.0;
+    for (int j = 0; j < B; j++) D[id * B + j] = 0.0;
+    D[id * B + id] = 1.0;
+    res[(long long)i * B + id] = 0.0;
+  }
+  for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+    double* O = &crs[(long long)q * BS];
+    const unsigned j = colind[q];
+    for (int jd = 0; jd < B; jd++) {
+      const bool fj = bc[(long long)j * B + jd] != 0;
+      for (int id = 0; id < B; id++)
+        if (fi[id] || fj) O[id * B + jd] = 0.0;
+    }
+  }
+}
+
+hipError_t launch_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
+                         double* res, const int* bc, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const int nt = 256, nb = (n + nt - 1) / nt;
+  if (len == 1) hipLaunchKernelGGL(k_set_bc<1>, dim3(nb), dim3(nt), 0, st, n, rowptr, colind, crs, dia, res, bc);
+  else if (len == 2) hipLaunchKernelGGL(k_set_bc<2>, dim3(nb), dim3(nt), 0, st, n, rowptr, colind, crs, dia, res, bc);
+  else if (len == 3) hipLaunchKernelGGL(k_set_bc<3>, dim3(nb), dim3(nt), 0, st, n, rowptr, colind, crs, dia, res, bc);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// u[old] += update[new] (CNodeAry::AddValueToNodeSegment with factor 1.0, src/femls/linearsystem_field.cpp:740)
+__global__ void k_update_field(long long m, int len, const int* order, const double* upd, double* u) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long long i0 = k / len;
+  const int d = (int)(k - i0 * len);
+  const long long i1 = order ? order[i0] : i0;
+  u[i1 * len + d] += 1.0 * upd[k];
+}
+
+hipError_t launch_update_field(int n, int len, const int* order, const double* upd, double* u, hipStream_t st) {
+  const long long m = (long long)n * len;
+  if (m == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_update_field, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, upd, u);
+  return hipGetLastError();
+}
+
+}  // namespace dfm

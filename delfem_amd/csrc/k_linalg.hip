@@ -1,0 +1,290 @@
+// Vector and matrix kernels behind the handle operations of the C-ABI (the reference's CLinearSystem
+// MATVEC/DOT/AXPY/COPY/SCAL, src/ls/linearsystem.cpp:250-415, and CVector_Blk, src/matvec/vector_blk.cpp),
+// the block-CRS SpMV (CMatDia_BlkCrs::MatVec, src/matvec/matdia_blkcrs.cpp:921-1094) and the ILU(0) numeric
+// factorisation (CMatDiaFrac_BlkCrs::DoILUDecomp, src/matvec/matdiafrac_blkcrs.cpp:774-1107).
+#include "dev_common.hpp"
+#include "dfm_internal.hpp"
+#include "kernels.hpp"
+
+namespace dfm {
+
+__global__ void k_fill(double* x, double v, long long m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void k_fill_f(float* x, float v, long long m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void k_copy(const double* x, double* y, long long m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+__global__ void k_scal(double* x, double a, long long m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+__global__ void k_axpy(double a, const double* x, double* y, long long m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+__global__ void k_to_f32(const double* x, float* y, long long m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
+static dim3 stream_grid(long long m) {
+  long long b = (m + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+hipError_t launch_fill(double* x, double v, long long m, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, stream_grid(m), dim3(256), 0, st, x, v, m);
+  return hipGetLastError();
+}
+hipError_t launch_fill_f(float* x, float v, long long m, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f, stream_grid(m), dim3(256), 0, st, x, v, m);
+  return hipGetLastError();
+}
+hipError_t launch_copy(const double* x, double* y, long long m, hipStream_t st) {
+  if (m <= 0 || x == y) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, stream_grid(m), dim3(256), 0, st, x, y, m);
+  return hipGetLastError();
+}
+hipError_t launch_scal(double* x, double a, long long m, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scal, stream_grid(m), dim3(256), 0, st, x, a, m);
+  return hipGetLastError();
+}
+hipError_t launch_axpy(double a, const double* x, double* y, long long m, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy, stream_grid(m), dim3(256), 0, st, a, x, y, m);
+  return hipGetLastError();
+}
+hipError_t launch_to_f32(const double* x, float* y, long long m, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_to_f32, stream_grid(m), dim3(256), 0, st, x, y, m);
+  return hipGetLastError();
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_dot_part(const double* x, const double* y, long long m, double* partial) {
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) v[0] += x[i] * y[i];
+  produce<NT, 1>(partial, v);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_dot_finish(const double* partial, int G, double* out) {
+  double t[1];
+  consume<NT, 1>(partial, G, t);
+  if (threadIdx.x == 0) out[0] = t[0];
+}
+
+// deterministic dot product: fixed grid of partials, then one block sums them in a fixed order
+hipError_t launch_dot(const double* x, const double* y, long long m, double* partial, unsigned* counter, double* out,
+                      hipStream_t st) {
+  (void)counter;
+  const unsigned nb = vec_blocks(m, 256);
+  hipLaunchKernelGGL(k_dot_part<256>, dim3(nb), dim3(256), 0, st, x, y, m, partial);
+  hipLaunchKernelGGL(k_dot_finish<256>, dim3(1), dim3(256), 0, st, partial, (int)nb, out);
+  return hipGetLastError();
+}
+
+// y = alpha*A*x + beta*y (beta == 0 overwrites y)
+template <int B, int TPR>
+__global__ void __launch_bounds__(256) k_spmv(int n, const unsigned* __restrict__ rp, const unsigned* __restrict__ ci,
+                                              const double* __restrict__ crs, const double* __restrict__ dia,
+                                              double alpha, const double* __restrict__ x, double beta, double* y) {
+  const int row = (blockIdx.x * 256 + threadIdx.x) / TPR;
+  const int lane = threadIdx.x % TPR;
+  if (row >= n) return;
+  double acc[B];
+  spmv_row<B, TPR, double, double, double>(rp[row], rp[row + 1], ci, crs, x, lane, acc);
+  if (lane == 0) {
+    dia_row<B, double, double, double>(dia, x, row, acc);
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+      double* yr = &y[(size_t)row * B + r];
+      *yr = (beta == 0.0) ? alpha * acc[r] : beta * (*yr) + alpha * acc[r];
+    }
+  }
+}
+
+int spmv_tpr(long long ncrs, int n) {
+  const double avg = n > 0 ? (double)ncrs / n : 0.0;
+  if (avg <= 4.0) return 4;
+  if (avg <= 8.0) return 8;
+  if (avg <= 16.0) return 16;
+  return 32;
+}
+
+hipError_t launch_spmv(int n, int len, int tpr, const unsigned* rowptr, const unsigned* colind,
+                           const double* crs, const double* dia, double alpha, const double* x, double beta,
+                           double* y, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const long long threads = (long long)n * tpr;
+  const dim3 grid((unsigned)((threads + 255) / 256));
+#define DFM_SPMV(B, T) \
+  if (len == B && tpr == T) { hipLaunchKernelGGL((k_spmv<B, T>), grid, dim3(256), 0, st, n, rowptr, colind, crs, dia, alpha, x, beta, y); return hipGetLastError(); }
+  DFM_SPMV(1, 4) DFM_SPMV(1, 8) DFM_SPMV(1, 16) DFM_SPMV(1, 32)
+  DFM_SPMV(2, 4) DFM_SPMV(2, 8) DFM_SPMV(2, 16) DFM_SPMV(2, 32)
+  DFM_SPMV(3, 4) DFM_SPMV(3, 8) DFM_SPMV(3, 16) DFM_SPMV(3, 32)
+#undef DFM_SPMV
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// ILU(0) numeric factorisation, row-wise IKJ exactly as DoILUDecomp: for row i and each k < i of its lower
+// part in ascending order, A_ij -= A_ik * U_kj for every j > k of row k that is in row i's pattern (or is
+// i itself: the diagonal block); then D_i <- D_i^-1 (closed forms for 1x1, 2x2, 3x3; pivots with
+// |det| <= 1e-30 are left uninverted and counted); then U_ij <- D_i^-1 U_ij. Rows run in the forward
+// schedule: a row starts when the rows it reads are final.
+
+template <int B>
+__device__ inline void sub_prod(double* pij, const double* pik, const double* pkj) {
+  if (B == 1) {
+    pij[0] -= pik[0] * pkj[0];
+  } else if (B == 2) {
+    pij[0] -= pik[0] * pkj[0] + pik[1] * pkj[2];
+    pij[1] -= pik[0] * pkj[1] + pik[1] * pkj[3];
+    pij[2] -= pik[2] * pkj[0] + pik[3] * pkj[2];
+    pij[3] -= pik[2] * pkj[1] + pik[3] * pkj[3];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      pij[a * 3 + 0] -= pik[a * 3 + 0] * pkj[0] + pik[a * 3 + 1] * pkj[3] + pik[a * 3 + 2] * pkj[6];
+      pij[a * 3 + 1] -= pik[a * 3 + 0] * pkj[1] + pik[a * 3 + 1] * pkj[4] + pik[a * 3 + 2] * pkj[7];
+      pij[a * 3 + 2] -= pik[a * 3 + 0] * pkj[2] + pik[a * 3 + 1] * pkj[5] + pik[a * 3 + 2] * pkj[8];
+    }
+  }
+}
+
+template <int B>
+__device__ inline bool invert_dia(double* p) {
+  if (B == 1) {
+    const double v = p[0];
+    if (fabs(v) > 1.0e-30) { p[0] = 1.0 / v; return true; }
+    return false;
+  } else if (B == 2) {
+    const double det = p[0] * p[3] - p[1] * p[2];
+    if (fabs(det) > 1.0e-30) {
+      const double inv_det = 1.0 / det;
+      const double t = p[0];
+      p[0] = inv_det * p[3];
+      p[1] = -inv_det * p[1];
+      p[2] = -inv_det * p[2];
+      p[3] = inv_det * t;
+      return true;
+    }
+    return false;
+  } else {
+    const double* a = p;
+    const double det = a[0] * a[4] * a[8] + a[3] * a[7] * a[2] + a[6] * a[1] * a[5] - a[0] * a[7] * a[5] -
+                       a[6] * a[4] * a[2] - a[3] * a[1] * a[8];
+    if (!(fabs(det) > 1.0e-30)) return false;
+    const double inv_det = 1.0 / det;
+    double t[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) t[k] = p[k];
+    p[0] = inv_det * (t[4] * t[8] - t[5] * t[7]);
+    p[1] = inv_det * (t[2] * t[7] - t[1] * t[8]);
+    p[2] = inv_det * (t[1] * t[5] - t[2] * t[4]);
+    p[3] = inv_det * (t[5] * t[6] - t[3] * t[8]);
+    p[4] = inv_det * (t[0] * t[8] - t[2] * t[6]);
+    p[5] = inv_det * (t[2] * t[3] - t[0] * t[5]);
+    p[6] = inv_det * (t[3] * t[7] - t[4] * t[6]);
+    p[7] = inv_det * (t[1] * t[6] - t[0] * t[7]);
+    p[8] = inv_det * (t[0] * t[4] - t[1] * t[3]);
+    return true;
+  }
+}
+
+template <int B>
+__device__ inline void scale_u(double* pij, const double* pii) {
+  double t[B * B];
+#pragma unroll
+  for (int k = 0; k < B * B; k++) t[k] = pij[k];
+  if (B == 1) {
+    pij[0] = t[0] * pii[0];
+  } else if (B == 2) {
+    pij[0] = pii[0] * t[0] + pii[1] * t[2];
+    pij[1] = pii[0] * t[1] + pii[1] * t[3];
+    pij[2] = pii[2] * t[0] + pii[3] * t[2];
+    pij[3] = pii[2] * t[1] + pii[3] * t[3];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      pij[a * 3 + 0] = pii[a * 3 + 0] * t[0] + pii[a * 3 + 1] * t[3] + pii[a * 3 + 2] * t[6];
+      pij[a * 3 + 1] = pii[a * 3 + 0] * t[1] + pii[a * 3 + 1] * t[4] + pii[a * 3 + 2] * t[7];
+      pij[a * 3 + 2] = pii[a * 3 + 0] * t[2] + pii[a * 3 + 1] * t[5] + pii[a * 3 + 2] * t[8];
+    }
+  }
+}
+
+template <int B>
+__global__ void __launch_bounds__(64) k_ilu_factor(int gbase, const int* __restrict__ grp, const int* __restrict__ rowlist,
+                                                   const uint8_t* __restrict__ lev, int nlev,
+                                                   const unsigned* __restrict__ rp, const unsigned* __restrict__ di,
+                                                   const unsigned* __restrict__ ci, double* fcrs, double* fdia,
+                                                   int* nsing) {
+  constexpr int BS = B * B;
+  const int g = gbase + blockIdx.x;
+  const int start = grp[2 * g], end = grp[2 * g + 1];
+  const int lane = threadIdx.x;
+  const bool active = lane < end - start;
+  const int i = active ? (rowlist ? rowlist[start + lane] : start + lane) : 0;
+  const int my = active ? (int)lev[i] : -1;
+  for (int l = 0; l < nlev; l++) {
+    if (my == l) {
+      const unsigned r0 = rp[i], r1 = rp[i + 1], d = di[i];
+      double* pii = &fdia[(size_t)i * BS];
+      for (unsigned ik = r0; ik < d; ik++) {
+        const unsigned k = ci[ik];
+        const double* pik = &fcrs[(size_t)ik * BS];
+        unsigned qi = r0;
+        for (unsigned kj = di[k]; kj < rp[k + 1]; kj++) {
+          const unsigned j = ci[kj];
+          const double* pkj = &fcrs[(size_t)kj * BS];
+          if (j == (unsigned)i) {
+            sub_prod<B>(pii, pik, pkj);
+          } else {
+            while (qi < r1 && ci[qi] < j) qi++;
+            if (qi < r1 && ci[qi] == j) sub_prod<B>(&fcrs[(size_t)qi * BS], pik, pkj);
+          }
+        }
+      }
+      if (!invert_dia<B>(pii)) atomicAdd(nsing, 1);
+      for (unsigned ij = d; ij < r1; ij++) scale_u<B>(&fcrs[(size_t)ij * BS], pii);
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_ilu_factor(int n, int len, const unsigned* rowptr, const unsigned* colind, const unsigned* diaind,
+                             double* fcrs, double* fdia, const SweepDev& fw, int* nsing, hipStream_t st) {
+  (void)n;
+  for (int p = 0; p < fw.nphase; p++) {
+    const int g0 = fw.phase_gptr[p], ng = fw.phase_gptr[p + 1] - g0;
+    if (ng == 0) continue;
+    const int nl = fw.phase_nlev[p];
+    if (len == 1)
+      hipLaunchKernelGGL(k_ilu_factor<1>, dim3(ng), dim3(64), 0, st, g0, fw.grp, fw.rowlist, fw.lev, nl, rowptr, diaind,
+                         colind, fcrs, fdia, nsing);
+    else if (len == 2)
+      hipLaunchKernelGGL(k_ilu_factor<2>, dim3(ng), dim3(64), 0, st, g0, fw.grp, fw.rowlist, fw.lev, nl, rowptr, diaind,
+                         colind, fcrs, fdia, nsing);
+    else if (len == 3)
+      hipLaunchKernelGGL(k_ilu_factor<3>, dim3(ng), dim3(64), 0, st, g0, fw.grp, fw.rowlist, fw.lev, nl, rowptr, diaind,
+                         colind, fcrs, fdia, nsing);
+    else
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace dfm

@@ -1,0 +1,67 @@
+// Launchers of the device kernels (k_assemble.hip, k_linalg.hip). All run on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dfm {
+
+// ---- assembly (k_assemble.hip) ----
+struct AsmArgs {
+  int kind, ndim, npoel, len, n;
+  const double* coords;  // original node numbering
+  const int* lnods;
+  const int* elsup_ptr;  // node -> elements (ascending element index), original numbering
+  const int* elsup;
+  const int* order;      // device row -> original node
+  const int* inv;        // original node -> device row
+  const double* params;
+  int npar, pstride;
+  double glob0, glob1;
+  const double* u;  // field values (original numbering), may be null
+  const double* v;
+  const int* load_ptr;   // node -> surface-load entries (original numbering), may be null
+  const double* loads;   // [nentries*len]
+  const unsigned* rowptr;  // device numbering pattern
+  const unsigned* colind;
+  double* crs;
+  double* dia;
+  double* res;
+};
+hipError_t launch_assemble(const AsmArgs& a, hipStream_t st);
+hipError_t launch_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
+                         double* res, const int* bc, hipStream_t st);
+hipError_t launch_update_field(int n, int len, const int* order, const double* upd, double* u, hipStream_t st);
+
+// ---- linear algebra (k_linalg.hip) ----
+hipError_t launch_fill(double* x, double v, long long m, hipStream_t st);
+hipError_t launch_fill_f(float* x, float v, long long m, hipStream_t st);
+hipError_t launch_copy(const double* x, double* y, long long m, hipStream_t st);
+hipError_t launch_scal(double* x, double a, long long m, hipStream_t st);
+hipError_t launch_axpy(double a, const double* x, double* y, long long m, hipStream_t st);
+hipError_t launch_to_f32(const double* x, float* y, long long m, hipStream_t st);
+
+// deterministic reductions: result written to out[0] (device)
+hipError_t launch_dot(const double* x, const double* y, long long m, double* partial, unsigned* counter,
+                      double* out, hipStream_t st);
+
+// y = alpha*A*x + beta*y, device numbering, FP64 values
+int spmv_tpr(long long ncrs, int n);
+hipError_t launch_spmv(int n, int len, int tpr, const unsigned* rowptr, const unsigned* colind, const double* crs,
+                       const double* dia, double alpha, const double* x, double beta, double* y, hipStream_t st);
+
+// ILU(0)
+struct SweepDev {
+  int nphase;
+  const int* phase_gptr;  // host array
+  const int* phase_nlev;  // host array
+  const int* grp;         // device (start,end) pairs
+  const int* rowlist;     // device or null
+  const uint8_t* lev;     // device
+};
+hipError_t launch_ilu_factor(int n, int len, const unsigned* rowptr, const unsigned* colind,
+                             const unsigned* diaind, double* fcrs, double* fdia, const SweepDev& fwd,
+                             int* nsing, hipStream_t st);
+// Krylov device loops (solver.hip) use the kernels below through solver-specific launchers.
+}  // namespace dfm

@@ -1,0 +1,783 @@
+// Device-resident Krylov loops: LsSol::Solve_PCG and Solve_PBiCGSTAB (src/ls/solver_ls_iter.cpp:100-172,
+// :291-408) with every scalar kept in device memory, and the mixed-precision PCG with reliable updates.
+//
+// Every kernel first reads the done flag, so the host enqueues iterations in chunks and polls one flag per
+// chunk. Dot products are produced as per-block partials and reduced by the kernel that consumes them
+// (dev_common.hpp): deterministic, no fences, no atomics. A scalar written by block 0 of a kernel is never
+// read by that kernel (values that change per iteration live in slots indexed by the iteration's parity).
+// Vector updates keep the reference's operation order (SCAL then AXPY with 1.0, COPY then AXPY).
+#include <cmath>
+#include <cstdlib>
+
+#include "dev_common.hpp"
+#include "dfm_internal.hpp"
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace dfm {
+
+// scalar slots (doubles)
+enum {
+  S_RZ = 0,     // rho = (r, z), two parity slots 0/1
+  S_INV0 = 2,   // 1 / ||r0||^2
+  S_TOL2 = 3,   // tolerance^2
+  S_CONV = 4,   // achieved ratio
+  S_RMAX = 5,   // reliable updates: max ||r||^2 since the last update, two parity slots 5/6
+  S_DELTA2 = 7, // reliable updates: trigger ratio^2
+  S_RR = 8,     // last ||r||^2 seen by the convergence test
+  S_TMP = 15
+};
+// int flags
+enum { F_DONE = 0, F_IT = 1, F_ITOUT = 2, F_MAXIT = 3, F_NREL = 5 };
+// partial slots, kMaxPartial doubles per value
+enum { P_A = 0, P_B = 2, P_C = 4, P_D = 6, P_NSLOT = 8 };
+
+constexpr int NT = 256;
+
+__global__ void k_solver_init(double* sc, int* fl, double tol2, int maxit, double delta2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int k = 0; k < 16; k++) sc[k] = 0.0;
+    sc[S_TOL2] = tol2;
+    sc[S_DELTA2] = delta2;
+    fl[F_DONE] = 0;
+    fl[F_IT] = -1;
+    fl[F_ITOUT] = maxit;
+    fl[F_MAXIT] = maxit;
+    fl[F_NREL] = 0;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_vfill(T* x, T v, long long m, const int* done) {
+  if (done && *done) return;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) x[i] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_vcopy(const T* x, T* y, long long m, const int* done) {
+  if (done && *done) return;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) y[i] = x[i];
+}
+
+// partials of (a, b)
+template <typename T>
+__global__ void __launch_bounds__(NT) k_vdot(const T* a, const T* b, long long m, double* slot, const int* done) {
+  if (done && *done) return;
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT)
+    v[0] += (double)a[i] * (double)b[i];
+  produce<NT, 1>(slot, v);
+}
+
+// PCG start (solver_ls_iter.cpp:115-133): ||r0||^2 from partials; guard -> converged with ratio 0 at 0.
+__global__ void __launch_bounds__(NT) k_pcg_start(const double* slot, int G, double guard, double* sc, int* fl) {
+  double t[1];
+  consume<NT, 1>(slot, G, t);
+  if (threadIdx.x == 0) {
+    sc[S_RR] = t[0];
+    if (t[0] < guard) {
+      sc[S_CONV] = 0.0;
+      fl[F_ITOUT] = 0;
+      fl[F_DONE] = 1;
+    } else {
+      sc[S_INV0] = 1.0 / t[0];
+      sc[S_RMAX] = t[0];
+    }
+  }
+}
+
+// rho_0 = (r, z) from partials
+__global__ void __launch_bounds__(NT) k_pcg_rho0(const double* slot, int G, double* sc, const int* fl) {
+  if (fl[F_DONE]) return;
+  double t[1];
+  consume<NT, 1>(slot, G, t);
+  if (threadIdx.x == 0) sc[S_RZ] = t[0];
+}
+
+// q = A p, partials of (p, q). Block 0 advances the iteration counter (read only by later kernels).
+template <int B, int TPR, typename T>
+__global__ void __launch_bounds__(NT) k_pcg_spmv(int n, const unsigned* __restrict__ rp, const unsigned* __restrict__ ci,
+                                                 const T* __restrict__ crs, const T* __restrict__ dia,
+                                                 const T* __restrict__ p, T* __restrict__ q, double* slot, int* fl) {
+  if (fl[F_DONE]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) fl[F_IT] = fl[F_IT] + 1;
+  constexpr int RPB = NT / TPR;
+  const int lane = threadIdx.x % TPR;
+  double v[1] = {0.0};
+  for (int row = blockIdx.x * RPB + threadIdx.x / TPR; row - (int)(threadIdx.x / TPR) < n; row += gridDim.x * RPB) {
+    if (row < n) {
+      T acc[B];
+      spmv_row<B, TPR, T, T, T>(rp[row], rp[row + 1], ci, crs, p, lane, acc);
+      if (lane == 0) {
+        dia_row<B, T, T, T>(dia, p, row, acc);
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+          q[(size_t)row * B + r] = acc[r];
+          v[0] += (double)p[(size_t)row * B + r] * (double)acc[r];
+        }
+      }
+    }
+  }
+  produce<NT, 1>(slot, v);
+}
+
+// alpha = rho / (p, q); r -= alpha q; x += alpha p; partials of (r, r)   (solver_ls_iter.cpp:139-144)
+template <typename T, typename TX>
+__global__ void __launch_bounds__(NT) k_pcg_update(long long m, T* r, TX* x, const T* q, const T* p,
+                                                   const double* slot_pq, int Gpq, double* slot_rr, const double* sc,
+                                                   const int* fl) {
+  if (fl[F_DONE]) return;
+  double pq[1];
+  consume<NT, 1>(slot_pq, Gpq, pq);
+  const int it = fl[F_IT];
+  const double alpha = sc[S_RZ + (it & 1)] / pq[0];
+  const T ma = (T)(-alpha), pa = (T)alpha;
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
+    const T ri = r[i] + ma * q[i];
+    r[i] = ri;
+    x[i] = x[i] + (TX)(pa * p[i]);
+    v[0] += (double)ri * (double)ri;
+  }
+  produce<NT, 1>(slot_rr, v);
+}
+
+// Convergence judgement on (r, r) (solver_ls_iter.cpp:146-155), then partials of (r, z) for beta.
+__global__ void __launch_bounds__(NT) k_pcg_judge_rz(long long m, const double* r, const double* z,
+                                                     const double* slot_rr, int G, double* slot_rz, double* sc, int* fl) {
+  if (fl[F_DONE]) return;
+  double rr[1];
+  consume<NT, 1>(slot_rr, G, rr);
+  const int it = fl[F_IT];
+  const double ratio2 = rr[0] * sc[S_INV0];
+  if (ratio2 < sc[S_TOL2] || it + 1 >= fl[F_MAXIT]) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[S_CONV] = sqrt(ratio2);
+      sc[S_RR] = rr[0];
+      if (ratio2 < sc[S_TOL2]) {
+        fl[F_ITOUT] = it;
+        fl[F_DONE] = 1;
+      } else {
+        fl[F_DONE] = 2;
+      }
+    }
+    return;
+  }
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) v[0] += r[i] * z[i];
+  produce<NT, 1>(slot_rz, v);
+}
+
+// beta = rho_new / rho; p = beta p; p += 1.0 z   (solver_ls_iter.cpp:158-166)
+template <typename T>
+__global__ void __launch_bounds__(NT) k_pcg_pupdate(long long m, T* p, const T* z, const double* slot_rz, int G,
+                                                    double* sc, const int* fl) {
+  if (fl[F_DONE]) return;
+  double rz[1];
+  consume<NT, 1>(slot_rz, G, rz);
+  const int it = fl[F_IT];
+  const double beta = rz[0] / sc[S_RZ + (it & 1)];
+  const T bt = (T)beta;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
+    const T t = p[i] * bt;
+    p[i] = t + z[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_RZ + ((it + 1) & 1)] = rz[0];
+}
+
+// ---- mixed precision: FP32 PCG with reliable updates ------------------------------------------------------
+// The Krylov recurrence runs in FP32 (matrix, ILU factor, r, z, p, q, and the partial solution xlo); dot
+// products are accumulated in FP64. When the recursive residual has dropped by delta since the last update,
+// or looks converged, the FP64 solution takes xlo (x += xlo, xlo = 0) and the residual is replaced by the
+// FP64 true residual b - A x computed with the FP64 matrix; convergence is only declared on that true residual.
+
+__device__ inline bool reliable_due(double rr, const double* sc, int it) {
+  return rr * sc[S_INV0] < sc[S_TOL2] || rr < sc[S_DELTA2] * sc[S_RMAX + (it & 1)];
+}
+
+__global__ void __launch_bounds__(NT) k_mx_fold(long long m, float* xlo, double* x, const double* slot_rr, int G,
+                                                const double* sc, const int* fl) {
+  if (fl[F_DONE]) return;
+  double rr[1];
+  consume<NT, 1>(slot_rr, G, rr);
+  if (!reliable_due(rr[0], sc, fl[F_IT])) return;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
+    x[i] += (double)xlo[i];
+    xlo[i] = 0.0f;
+  }
+}
+
+// r_true = b - A x (FP64), r32 = (float) r_true, partials of ||r_true||^2
+template <int B, int TPR>
+__global__ void __launch_bounds__(NT) k_mx_true_res(int n, const unsigned* __restrict__ rp, const unsigned* __restrict__ ci,
+                                                    const double* __restrict__ crs, const double* __restrict__ dia,
+                                                    const double* __restrict__ x, const double* __restrict__ b,
+                                                    float* __restrict__ r32, const double* slot_rr, int G,
+                                                    double* slot_true, const double* sc, const int* fl) {
+  if (fl[F_DONE]) return;
+  double rr[1];
+  consume<NT, 1>(slot_rr, G, rr);
+  if (!reliable_due(rr[0], sc, fl[F_IT])) return;
+  constexpr int RPB = NT / TPR;
+  const int lane = threadIdx.x % TPR;
+  double v[1] = {0.0};
+  for (int row = blockIdx.x * RPB + threadIdx.x / TPR; row - (int)(threadIdx.x / TPR) < n; row += gridDim.x * RPB) {
+    if (row < n) {
+      double acc[B];
+      spmv_row<B, TPR, double, double, double>(rp[row], rp[row + 1], ci, crs, x, lane, acc);
+      if (lane == 0) {
+        dia_row<B, double, double, double>(dia, x, row, acc);
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+          const double t = b[(size_t)row * B + r] - acc[r];
+          r32[(size_t)row * B + r] = (float)t;
+          v[0] += t * t;
+        }
+      }
+    }
+  }
+  produce<NT, 1>(slot_true, v);
+}
+
+// Judgement: on the true residual after a reliable update (convergence only there), else track the max of the
+// recursive residual; then partials of (r, z).
+__global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r, const float* z, const double* slot_rr,
+                                                    int G, const double* slot_true, int Gt, double* slot_rz, double* sc,
+                                                    int* fl) {
+  if (fl[F_DONE]) return;
+  double rr[1];
+  consume<NT, 1>(slot_rr, G, rr);
+  const int it = fl[F_IT];
+  const bool rel = reliable_due(rr[0], sc, it);
+  double rmax_next = fmax(sc[S_RMAX + (it & 1)], rr[0]);
+  if (rel) {
+    double rt[1];
+    consume<NT, 1>(slot_true, Gt, rt);
+    const double ratio2 = rt[0] * sc[S_INV0];
+    if (ratio2 < sc[S_TOL2]) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc[S_CONV] = sqrt(ratio2);
+        sc[S_RR] = rt[0];
+        fl[F_ITOUT] = it;
+        fl[F_DONE] = 1;
+      }
+      return;
+    }
+    rmax_next = rt[0];
+  }
+  if (it + 1 >= fl[F_MAXIT]) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[S_CONV] = sqrt(rr[0] * sc[S_INV0]);
+      sc[S_RR] = rr[0];
+      fl[F_DONE] = 2;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_RMAX + ((it + 1) & 1)] = rmax_next;
+    if (rel) fl[F_NREL] = fl[F_NREL] + 1;
+  }
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT)
+    v[0] += (double)r[i] * (double)z[i];
+  produce<NT, 1>(slot_rz, v);
+}
+
+__global__ void __launch_bounds__(NT) k_mx_to32(long long m, const double* a, float* b) {
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) b[i] = (float)a[i];
+}
+__global__ void __launch_bounds__(NT) k_mx_final_fold(long long m, float* xlo, double* x) {
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
+    x[i] += (double)xlo[i];
+    xlo[i] = 0.0f;
+  }
+}
+
+// ---- PBiCGSTAB ------------------------------------------------------------------------------------------
+// y = A x with partials of (y, w) and (u, w) [mode 0: w = r2, u = r] or (y, y) and (w, y) [mode 1: w = s].
+template <int B, int TPR>
+__global__ void __launch_bounds__(NT) k_bi_spmv(int n, const unsigned* __restrict__ rp, const unsigned* __restrict__ ci,
+                                                const double* __restrict__ crs, const double* __restrict__ dia,
+                                                const double* __restrict__ x, double* __restrict__ y,
+                                                const double* __restrict__ w, const double* __restrict__ u, int mode,
+                                                double* slot, int* fl) {
+  if (fl[F_DONE]) return;
+  if (mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) fl[F_IT] = fl[F_IT] + 1;
+  constexpr int RPB = NT / TPR;
+  const int lane = threadIdx.x % TPR;
+  double v[2] = {0.0, 0.0};
+  for (int row = blockIdx.x * RPB + threadIdx.x / TPR; row - (int)(threadIdx.x / TPR) < n; row += gridDim.x * RPB) {
+    if (row < n) {
+      double acc[B];
+      spmv_row<B, TPR, double, double, double>(rp[row], rp[row + 1], ci, crs, x, lane, acc);
+      if (lane == 0) {
+        dia_row<B, double, double, double>(dia, x, row, acc);
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+          const size_t k = (size_t)row * B + r;
+          y[k] = acc[r];
+          if (mode == 0) {
+            v[0] += acc[r] * w[k];  // (AMp, r2)
+            v[1] += u[k] * w[k];    // (r, r2)
+          } else {
+            v[0] += acc[r] * acc[r];  // (AMs, AMs)
+            v[1] += w[k] * acc[r];    // (s, AMs)
+          }
+        }
+      }
+    }
+  }
+  produce<NT, 2>(slot, v);
+}
+
+// alpha = (r, r2) / (AMp, r2); s = r; s += (-alpha) AMp
+__global__ void __launch_bounds__(NT) k_bi_s(long long m, const double* r, const double* amp, double* s,
+                                             const double* slot2, int G2, const int* fl) {
+  if (fl[F_DONE]) return;
+  double t[2];
+  consume<NT, 2>(slot2, G2, t);
+  const double alpha = t[1] / t[0];
+  const double ma = -alpha;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) s[i] = r[i] + ma * amp[i];
+}
+
+// omega = (s, AMs) / (AMs, AMs); x += alpha Mp; x += omega Ms; r = s; r += (-omega) AMs; partials (r,r), (r,r2)
+__global__ void __launch_bounds__(NT) k_bi_update(long long m, double* x, double* r, const double* mp, const double* ms,
+                                                  const double* s, const double* ams, const double* r2,
+                                                  const double* slot2, int G2, const double* slot5, int G5,
+                                                  double* slot6, const int* fl) {
+  if (fl[F_DONE]) return;
+  double t2[2], t5[2];
+  consume<NT, 2>(slot2, G2, t2);
+  consume<NT, 2>(slot5, G5, t5);
+  const double alpha = t2[1] / t2[0];
+  const double omega = t5[1] / t5[0];
+  double v[2] = {0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
+    double xi = x[i] + alpha * mp[i];
+    xi = xi + omega * ms[i];
+    x[i] = xi;
+    const double ri = s[i] + (-omega) * ams[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+    v[1] += ri * r2[i];
+  }
+  produce<NT, 2>(slot6, v);
+}
+
+// judgement on (r, r) (:360-368); beta = (r,r2)_new * alpha / ((r,r2) * omega); p = beta p; p += r;
+// p += (-beta*omega) AMp
+__global__ void __launch_bounds__(NT) k_bi_pupdate(long long m, double* p, const double* r, const double* amp,
+                                                   const double* slot2, int G2, const double* slot5, int G5,
+                                                   const double* slot6, int G6, double* sc, int* fl) {
+  if (fl[F_DONE]) return;
+  double t2[2], t5[2], t6[2];
+  consume<NT, 2>(slot6, G6, t6);
+  const int it = fl[F_IT];
+  const double sqc = t6[0] * sc[S_INV0];
+  if (sqc < sc[S_TOL2] || it + 2 >= fl[F_MAXIT]) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (sqc < sc[S_TOL2]) {
+        sc[S_CONV] = sqrt(t6[0] * sc[S_INV0]);
+        fl[F_ITOUT] = it;
+        fl[F_DONE] = 1;
+      } else {
+        fl[F_DONE] = 2;  // iitr = it + 1 was the last one below max_iter
+      }
+    }
+    return;
+  }
+  consume<NT, 2>(slot2, G2, t2);
+  consume<NT, 2>(slot5, G5, t5);
+  const double alpha = t2[1] / t2[0];
+  const double omega = t5[1] / t5[0];
+  const double beta = t6[1] * alpha / (t2[1] * omega);
+  const double bo = -beta * omega;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
+    double t = p[i] * beta;
+    t = t + r[i];
+    p[i] = t + bo * amp[i];
+  }
+}
+
+// =========================================================================================================
+// host side
+
+static dim3 vgrid(long long m) { return dim3(vec_blocks(m, NT)); }
+
+static unsigned spmv_grid(const Sys& s, int tpr) {
+  const long long rows_per_block = NT / tpr;
+  return vec_blocks((s.n + rows_per_block - 1) / rows_per_block, 1);
+}
+
+template <int B, typename T, typename A>
+static hipError_t precond_T(const Sys& s, const T* fcrs, const T* fdia, const T* src, T* z, const int* done,
+                            hipStream_t st) {
+  const Sweep& f = s.fwd;
+  for (int p = 0; p < f.nphase(); p++) {
+    const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0;
+    if (!ng) continue;
+    hipLaunchKernelGGL((k_ilu_fwd<B, T, A, T, T>), dim3(ng), dim3(64), 0, st, g0, f.d_grp.p,
+                       f.rowlist.empty() ? nullptr : f.d_rowlist.p, f.d_lev.p, f.phase_nlev[p], s.d_rowptr.p,
+                       s.d_lsplit.p, s.d_diaind.p, s.d_colind.p, fcrs, fdia, src, z, done);
+  }
+  const Sweep& b = s.bwd;
+  for (int p = 0; p < b.nphase(); p++) {
+    const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0;
+    if (!ng) continue;
+    hipLaunchKernelGGL((k_ilu_bwd<B, T, A, T>), dim3(ng), dim3(64), 0, st, g0, b.d_grp.p,
+                       b.rowlist.empty() ? nullptr : b.d_rowlist.p, b.d_lev.p, b.phase_nlev[p], s.d_rowptr.p,
+                       s.d_usplit.p, s.d_diaind.p, s.d_colind.p, fcrs, z, done);
+  }
+  return hipGetLastError();
+}
+
+hipError_t precond_apply64(const Sys& s, const double* src, double* z, const int* done, hipStream_t st) {
+  if (s.len == 1) return precond_T<1, double, double>(s, s.d_fcrs.p, s.d_fdia.p, src, z, done, st);
+  if (s.len == 2) return precond_T<2, double, double>(s, s.d_fcrs.p, s.d_fdia.p, src, z, done, st);
+  if (s.len == 3) return precond_T<3, double, double>(s, s.d_fcrs.p, s.d_fdia.p, src, z, done, st);
+  return hipErrorInvalidValue;
+}
+
+static hipError_t precond_apply32(const Sys& s, const float* src, float* z, const int* done, hipStream_t st) {
+  if (s.len == 1) return precond_T<1, float, float>(s, s.d_fcrs32.p, s.d_fdia32.p, src, z, done, st);
+  if (s.len == 2) return precond_T<2, float, float>(s, s.d_fcrs32.p, s.d_fdia32.p, src, z, done, st);
+  if (s.len == 3) return precond_T<3, float, float>(s, s.d_fcrs32.p, s.d_fdia32.p, src, z, done, st);
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+static hipError_t pcg_spmv(const Sys& s, const T* crs, const T* dia, const T* p, T* q, double* slot, unsigned& G,
+                           hipStream_t st) {
+  const int tpr = spmv_tpr(s.ncrs, s.n);
+  G = spmv_grid(s, tpr);
+#define DFM_SD(B, TP)                                                                                     \
+  if (s.len == B && tpr == TP) {                                                                          \
+    hipLaunchKernelGGL((k_pcg_spmv<B, TP, T>), dim3(G), dim3(NT), 0, st, s.n, s.d_rowptr.p, s.d_colind.p, \
+                       crs, dia, p, q, slot, s.d_iflag.p);                                                \
+    return hipGetLastError();                                                                             \
+  }
+  DFM_SD(1, 4) DFM_SD(1, 8) DFM_SD(1, 16) DFM_SD(1, 32)
+  DFM_SD(2, 4) DFM_SD(2, 8) DFM_SD(2, 16) DFM_SD(2, 32)
+  DFM_SD(3, 4) DFM_SD(3, 8) DFM_SD(3, 16) DFM_SD(3, 32)
+#undef DFM_SD
+  return hipErrorInvalidValue;
+}
+
+static hipError_t mx_true_res(const Sys& s, const double* x, const double* b, float* r32, const double* slot_rr,
+                              int Grr, double* slot_true, unsigned& G, hipStream_t st) {
+  const int tpr = spmv_tpr(s.ncrs, s.n);
+  G = spmv_grid(s, tpr);
+#define DFM_TR(B, TP)                                                                                            \
+  if (s.len == B && tpr == TP) {                                                                                 \
+    hipLaunchKernelGGL((k_mx_true_res<B, TP>), dim3(G), dim3(NT), 0, st, s.n, s.d_rowptr.p, s.d_colind.p,       \
+                       s.d_crs.p, s.d_dia.p, x, b, r32, slot_rr, Grr, slot_true, s.d_scal.p, s.d_iflag.p);       \
+    return hipGetLastError();                                                                                    \
+  }
+  DFM_TR(1, 4) DFM_TR(1, 8) DFM_TR(1, 16) DFM_TR(1, 32)
+  DFM_TR(2, 4) DFM_TR(2, 8) DFM_TR(2, 16) DFM_TR(2, 32)
+  DFM_TR(3, 4) DFM_TR(3, 8) DFM_TR(3, 16) DFM_TR(3, 32)
+#undef DFM_TR
+  return hipErrorInvalidValue;
+}
+
+static hipError_t bi_spmv(const Sys& s, const double* x, double* y, const double* w, const double* u, int mode,
+                          double* slot, unsigned& G, hipStream_t st) {
+  const int tpr = spmv_tpr(s.ncrs, s.n);
+  G = spmv_grid(s, tpr);
+#define DFM_BS(B, TP)                                                                                         \
+  if (s.len == B && tpr == TP) {                                                                              \
+    hipLaunchKernelGGL((k_bi_spmv<B, TP>), dim3(G), dim3(NT), 0, st, s.n, s.d_rowptr.p, s.d_colind.p,        \
+                       s.d_crs.p, s.d_dia.p, x, y, w, u, mode, slot, s.d_iflag.p);                            \
+    return hipGetLastError();                                                                                 \
+  }
+  DFM_BS(1, 4) DFM_BS(1, 8) DFM_BS(1, 16) DFM_BS(1, 32)
+  DFM_BS(2, 4) DFM_BS(2, 8) DFM_BS(2, 16) DFM_BS(2, 32)
+  DFM_BS(3, 4) DFM_BS(3, 8) DFM_BS(3, 16) DFM_BS(3, 32)
+#undef DFM_BS
+  return hipErrorInvalidValue;
+}
+
+static double* slot(Sys& s, int k) { return s.d_partial.p + (size_t)k * kMaxPartial; }
+
+static int poll(Sys& s, hipStream_t st) {
+  DFM_HIP(hipMemcpyAsync(s.h_pinned, s.d_iflag.p, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p, sizeof(double) * 16, hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipStreamSynchronize(st));
+  return DFM_OK;
+}
+
+// Buffers of one PCG run: FP64 loop (x = the FP64 solution), or the FP32 loop with reliable updates
+// (x = the FP32 partial solution, x64/b64 the FP64 solution and right-hand side).
+struct PcgBufs {
+  const void* crs;
+  const void* dia;
+  void* r;
+  void* z;
+  void* p;
+  void* q;
+  void* x;
+  double* x64;
+  const double* b64;
+};
+
+template <typename T>
+static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
+  const long long m = (long long)s.n * s.len;
+  const dim3 vg = vgrid(m);
+  const int G = (int)vg.x;
+  unsigned Gs = 0, Gt = 0;
+  int* fl = s.d_iflag.p;
+  double* sc = s.d_scal.p;
+  T* r = (T*)v.r;
+  T* z = (T*)v.z;
+  T* p = (T*)v.p;
+  T* q = (T*)v.q;
+  if (pcg_spmv<T>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, st) != hipSuccess) return DFM_ERR_HIP;
+  if (mixed) {
+    hipLaunchKernelGGL((k_pcg_update<float, float>), vg, dim3(NT), 0, st, m, (float*)r, (float*)v.x, (const float*)q,
+                       (const float*)p, slot(s, P_A), (int)Gs, slot(s, P_B), sc, fl);
+    hipLaunchKernelGGL(k_mx_fold, vg, dim3(NT), 0, st, m, (float*)v.x, v.x64, slot(s, P_B), G, sc, fl);
+    if (mx_true_res(s, v.x64, v.b64, (float*)r, slot(s, P_B), G, slot(s, P_C), Gt, st) != hipSuccess) return DFM_ERR_HIP;
+    if (precond_apply32(s, (const float*)r, (float*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
+    hipLaunchKernelGGL(k_mx_judge_rz, vg, dim3(NT), 0, st, m, (const float*)r, (const float*)z, slot(s, P_B), G,
+                       slot(s, P_C), (int)Gt, slot(s, P_D), sc, fl);
+  } else {
+    hipLaunchKernelGGL((k_pcg_update<double, double>), vg, dim3(NT), 0, st, m, (double*)r, (double*)v.x,
+                       (const double*)q, (const double*)p, slot(s, P_A), (int)Gs, slot(s, P_B), sc, fl);
+    if (precond_apply64(s, (const double*)r, (double*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
+    hipLaunchKernelGGL(k_pcg_judge_rz, vg, dim3(NT), 0, st, m, (const double*)r, (const double*)z, slot(s, P_B), G,
+                       slot(s, P_D), sc, fl);
+  }
+  hipLaunchKernelGGL(k_pcg_pupdate<T>, vg, dim3(NT), 0, st, m, p, (const T*)z, slot(s, P_D), G, sc, fl);
+  return DFM_OK;
+}
+
+template <typename T>
+static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double guard, double& conv,
+                   unsigned& iter, bool& converged) {
+  hipStream_t st = s.ctx->stream;
+  const long long m = (long long)s.n * s.len;
+  const dim3 vg = vgrid(m);
+  const int G = (int)vg.x;
+  int* fl = s.d_iflag.p;
+  double* sc = s.d_scal.p;
+  T* r = (T*)v.r;
+  T* z = (T*)v.z;
+  T* p = (T*)v.p;
+  const char* dl = getenv("DFM_RELIABLE_DELTA");
+  const double delta = dl ? atof(dl) : 0.1;
+  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, st, sc, fl, tol * tol, (int)maxit, delta * delta);
+  hipLaunchKernelGGL(k_vfill<T>, vg, dim3(NT), 0, st, (T*)v.x, (T)0, m, (const int*)nullptr);
+  if (mixed) hipLaunchKernelGGL(k_vfill<double>, vg, dim3(NT), 0, st, v.x64, 0.0, m, (const int*)nullptr);
+  // x = 0; ||r0||^2 (guard); z = M^-1 r; p = z; rho = (r, z)
+  hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)r, m, slot(s, P_A), (const int*)nullptr);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, guard, sc, fl);
+  hipError_t e = mixed ? precond_apply32(s, (const float*)r, (float*)z, fl, st)
+                       : precond_apply64(s, (const double*)r, (double*)z, fl, st);
+  if (e != hipSuccess) return DFM_ERR_HIP;
+  hipLaunchKernelGGL(k_vcopy<T>, vg, dim3(NT), 0, st, (const T*)z, p, m, fl);
+  hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)z, m, slot(s, P_A), fl);
+  hipLaunchKernelGGL(k_pcg_rho0, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, sc, fl);
+  unsigned launched = 0, chunk = 8;
+  DFM_HIP(hipGetLastError());
+  while (launched < maxit) {
+    const unsigned k = std::min(chunk, maxit - launched);
+    for (unsigned t = 0; t < k; t++) {
+      const int rc = pcg_iteration<T>(s, v, mixed, st);
+      if (rc != DFM_OK) return rc;
+    }
+    launched += k;
+    const int rc = poll(s, st);
+    if (rc != DFM_OK) return rc;
+    if (s.h_pinned[F_DONE]) break;
+    chunk = std::min(chunk * 2, 128u);
+  }
+  if (launched == 0) {
+    const int rc = poll(s, st);
+    if (rc != DFM_OK) return rc;
+  }
+  if (mixed) {
+    // the FP64 solution takes any partial solution not yet folded (a run stopped by max_iter)
+    hipLaunchKernelGGL(k_mx_final_fold, vg, dim3(NT), 0, st, m, (float*)v.x, v.x64);
+  }
+  DFM_HIP(hipGetLastError());
+  const int done = s.h_pinned[F_DONE];
+  converged = done == 1;
+  // not converged within max_iter: the ratio of the last residual seen (solver_ls_iter.cpp:168-171)
+  conv = done ? s.h_pinned_d[S_CONV] : std::sqrt(s.h_pinned_d[S_RR] * s.h_pinned_d[S_INV0]);
+  iter = converged ? (unsigned)s.h_pinned[F_ITOUT] : maxit;
+  return DFM_OK;
+}
+
+int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
+  const long long m = (long long)s.n * s.len;
+  const double tol = *conv_ratio;
+  const unsigned maxit = *iteration;
+  hipStream_t st = s.ctx->stream;
+  hipEvent_t e0, e1;
+  DFM_HIP(hipEventCreate(&e0));
+  DFM_HIP(hipEventCreate(&e1));
+  DFM_HIP(hipEventRecord(e0, st));
+  double conv = 0;
+  unsigned it = 0;
+  bool ok = false;
+  int rc;
+  if (precision == DFM_FP64) {
+    PcgBufs v{s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_work.p + m, s.d_work.p, s.d_work.p + m, s.d_upd.p, nullptr,
+              nullptr};
+    rc = pcg_run<double>(s, v, false, tol, maxit, 1.0e-30, conv, it, ok);
+  } else {
+    // FP32: r, z, p, q, xlo; FP64: the right-hand side (d_rhs) and the solution (the update handle)
+    float* w = s.d_work32.p;
+    PcgBufs v{s.d_crs32.p, s.d_dia32.p, w, w + m, w + 2 * m, w + 3 * m, w + 4 * m, s.d_upd.p, s.d_rhs.p};
+    hipLaunchKernelGGL(k_mx_to32, vgrid(m), dim3(NT), 0, st, m, s.d_rhs.p, w);
+    rc = pcg_run<float>(s, v, true, tol, maxit, 1.0e-30, conv, it, ok);
+    // the residual handle receives the FP64 true residual b - A x (the reference leaves its final residual)
+    if (rc == DFM_OK) {
+      DFM_HIP(launch_copy(s.d_rhs.p, s.d_res.p, m, st));
+      DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
+                          s.d_upd.p, 1.0, s.d_res.p, st));
+    }
+  }
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(hipEventRecord(e1, st));
+  DFM_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *conv_ratio = conv;
+  if (ok) *iteration = it;
+  const unsigned its = ok ? it + 1 : maxit;
+  s.last_ms = ms;
+  s.last_inner = its;
+  s.last_ms_iter = its ? ms / its : 0.0;
+  s.last_nrel = s.h_pinned[F_NREL];
+  return ok ? 1 : 0;
+}
+
+int solve_pbicgstab(Sys& s, double* conv_ratio, unsigned* iteration) {
+  const long long m = (long long)s.n * s.len;
+  const double tol = *conv_ratio;
+  const unsigned maxit = *iteration;
+  hipStream_t st = s.ctx->stream;
+  const dim3 vg = vgrid(m);
+  const int G = (int)vg.x;
+  int* fl = s.d_iflag.p;
+  double* sc = s.d_scal.p;
+  double* r = s.d_res.p;
+  double* x = s.d_upd.p;
+  double* vs = s.d_work.p;
+  double* vMs = vs + m;
+  double* vAMs = vs + 2 * m;
+  double* vp = vs + 3 * m;
+  double* vMp = vs + 4 * m;
+  double* vAMp = vs + 5 * m;
+  double* vr2 = vs + 6 * m;
+  hipEvent_t e0, e1;
+  DFM_HIP(hipEventCreate(&e0));
+  DFM_HIP(hipEventCreate(&e1));
+  DFM_HIP(hipEventRecord(e0, st));
+  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, st, sc, fl, tol * tol, (int)maxit, 0.0);
+  hipLaunchKernelGGL(k_vdot<double>, vg, dim3(NT), 0, st, r, r, m, slot(s, P_A), (const int*)nullptr);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, 1.0e-60, sc, fl);
+  int rc = poll(s, st);
+  if (rc != DFM_OK) return rc;
+  if (s.h_pinned[F_DONE]) {
+    // guard: conv = sqrt(sq0), iteration 0, the update untouched (solver_ls_iter.cpp:305-313)
+    *conv_ratio = std::sqrt(s.h_pinned_d[S_RR]);
+    *iteration = 0;
+    return 1;
+  }
+  hipLaunchKernelGGL(k_vfill<double>, vg, dim3(NT), 0, st, x, 0.0, m, fl);
+  hipLaunchKernelGGL(k_vcopy<double>, vg, dim3(NT), 0, st, r, vr2, m, fl);
+  hipLaunchKernelGGL(k_vcopy<double>, vg, dim3(NT), 0, st, r, vp, m, fl);
+  // the reference loop counts iitr from 1 (:328); F_IT counts from 0, iitr = F_IT + 1
+  unsigned launched = 1, chunk = 8;
+  while (launched < maxit) {
+    const unsigned k = std::min(chunk, maxit - launched);
+    for (unsigned t = 0; t < k; t++) {
+      unsigned G2 = 0, G5 = 0;
+      if (precond_apply64(s, vp, vMp, fl, st) != hipSuccess) return DFM_ERR_HIP;
+      if (bi_spmv(s, vMp, vAMp, vr2, r, 0, slot(s, P_A), G2, st) != hipSuccess) return DFM_ERR_HIP;
+      hipLaunchKernelGGL(k_bi_s, vg, dim3(NT), 0, st, m, r, vAMp, vs, slot(s, P_A), (int)G2, fl);
+      if (precond_apply64(s, vs, vMs, fl, st) != hipSuccess) return DFM_ERR_HIP;
+      if (bi_spmv(s, vMs, vAMs, vs, nullptr, 1, slot(s, P_B), G5, st) != hipSuccess) return DFM_ERR_HIP;
+      hipLaunchKernelGGL(k_bi_update, vg, dim3(NT), 0, st, m, x, r, vMp, vMs, vs, vAMs, vr2, slot(s, P_A), (int)G2,
+                         slot(s, P_B), (int)G5, slot(s, P_C), fl);
+      hipLaunchKernelGGL(k_bi_pupdate, vg, dim3(NT), 0, st, m, vp, r, vAMp, slot(s, P_A), (int)G2, slot(s, P_B),
+                         (int)G5, slot(s, P_C), G, sc, fl);
+    }
+    launched += k;
+    rc = poll(s, st);
+    if (rc != DFM_OK) return rc;
+    if (s.h_pinned[F_DONE]) break;
+    chunk = std::min(chunk * 2, 128u);
+  }
+  DFM_HIP(hipGetLastError());
+  const bool converged = s.h_pinned[F_DONE] == 1;
+  if (converged) {
+    *conv_ratio = s.h_pinned_d[S_CONV];
+    *iteration = (unsigned)s.h_pinned[F_ITOUT] + 1;
+  } else {
+    *iteration = maxit;  // the reference returns true with num_iter = max_iter and the ratio untouched
+  }
+  DFM_HIP(hipEventRecord(e1, st));
+  DFM_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const unsigned its = converged ? (unsigned)s.h_pinned[F_ITOUT] + 1 : maxit;
+  s.last_ms = ms;
+  s.last_inner = its;
+  s.last_ms_iter = its ? ms / its : 0.0;
+  return 1;
+}
+
+// Average time of one launch of a hot kernel, timed with HIP events on the system's stream (the stream the
+// kernels run on). Algorithmic bytes: SpMV (ncrs + n) b^2 s_v + 4 ncrs + 4 (n+1) + 2 n b s_x; ILU apply
+// (forward + backward) (ncrs + n) b^2 s_v + 4 ncrs + 4 (n+1) + 4 n b s_x.
+int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes) {
+  hipStream_t st = s.ctx->stream;
+  const long long m = (long long)s.n * s.len;
+  int* fl = s.d_iflag.p;
+  double* sc = s.d_scal.p;
+  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, st, sc, fl, 0.0, 1 << 30, 0.0);
+  float* x32 = s.d_work32.p;
+  float* y32 = s.d_work32.p + m;
+  hipLaunchKernelGGL(k_mx_to32, vgrid(m), dim3(NT), 0, st, m, s.d_rhs.p, x32);
+  unsigned G = 0;
+  auto launch = [&]() -> hipError_t {
+    switch (which) {
+      case 0: return pcg_spmv<double>(s, s.d_crs.p, s.d_dia.p, s.d_rhs.p, s.d_work.p, slot(s, P_A), G, st);
+      case 1: return pcg_spmv<float>(s, s.d_crs32.p, s.d_dia32.p, x32, y32, slot(s, P_A), G, st);
+      case 2: return precond_apply64(s, s.d_rhs.p, s.d_work.p, fl, st);
+      default: return precond_apply32(s, x32, y32, fl, st);
+    }
+  };
+  for (int w = 0; w < 3; w++)
+    if (launch() != hipSuccess) return DFM_ERR_HIP;
+  hipEvent_t e0, e1;
+  DFM_HIP(hipEventCreate(&e0));
+  DFM_HIP(hipEventCreate(&e1));
+  DFM_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; r++)
+    if (launch() != hipSuccess) return DFM_ERR_HIP;
+  DFM_HIP(hipEventRecord(e1, st));
+  DFM_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (avg_ms) *avg_ms = ms / reps;
+  const double sv = (which == 0 || which == 2) ? 8.0 : 4.0;
+  const double sx = sv;
+  const double b2 = (double)s.bs;
+  const double mat = ((double)s.ncrs + s.n) * b2 * sv + 4.0 * s.ncrs + 4.0 * (s.n + 1);
+  if (bytes) *bytes = mat + ((which < 2) ? 2.0 : 4.0) * (double)m * sx;
+  return DFM_OK;
+}
+
+}  // namespace dfm

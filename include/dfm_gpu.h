@@ -1,0 +1,162 @@
+/* dfm_gpu.h — C-ABI of the MI355X-native DelFEM FEM hot path (libdfm_gpu.so).
+ *
+ * Plain pointers and sizes only. Every entry point returns an int status (DFM_OK = 0, negative on error;
+ * dfm_last_error() gives the message). Device state lives in opaque handles; host arrays passed in are
+ * copied, host arrays passed out are filled before the call returns.
+ *
+ * Numbering. Vectors and matrices are exchanged in the caller's node numbering ("original"). Internally a
+ * system may run in a permuted "device" numbering chosen by dfm_sys_set_ordering (the ILU ordering of
+ * LsSol::CPreconditioner_ILU::SetOrdering, include/delfem/ls/preconditioner.h:93-97 in the reference).
+ * Handle operations (dot/copy/scal/axpy/matvec/precond) are permutation-invariant, so their meaning is
+ * exactly the reference's.
+ *
+ * Vector handles follow the reference (src/ls/linearsystem.cpp:256-265): -1 = residual (right-hand side,
+ * overwritten by the solvers), -2 = update (solution), 0.. = solver temporaries.
+ */
+#ifndef DFM_GPU_H
+#define DFM_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFM_OK 0
+#define DFM_ERR_ARG -1
+#define DFM_ERR_HIP -2
+#define DFM_ERR_STATE -3
+#define DFM_ERR_NOMEM -4
+#define DFM_ERR_SINGULAR -5
+
+typedef struct dfm_ctx_s* dfm_ctx;
+typedef struct dfm_sys_s* dfm_sys;
+
+/* element kinds (Fem::Eqn::AddLinSys_* on P1 TRI/TET) and their per-element parameter tuples */
+enum {
+  DFM_POISSON = 0,     /* (alpha, source)                  src/femeqn/eqn_poisson.cpp:49-125, :249-323 */
+  DFM_SOLID2D = 1,     /* (lambda, myu, rho, gx, gy)       src/femeqn/eqn_linear_solid2d.cpp:505-598   */
+  DFM_SOLID3D = 2,     /* (lambda, myu, rho, gx, gy, gz)   src/femeqn/eqn_linear_solid3d.cpp:241-334   */
+  DFM_STVK2D = 3,      /* (lambda, myu, rho, gx, gy)       src/femeqn/eqn_st_venant.cpp:47-272         */
+  DFM_STVK3D = 4,      /* (lambda, myu, rho, gx, gy, gz)   src/femeqn/eqn_st_venant.cpp:724-939        */
+  DFM_DIFFUSION2D = 5  /* (rho, alpha, source), glob=(dt, gamma)  src/femeqn/eqn_diffusion.cpp:240-345 */
+};
+
+/* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering) */
+enum {
+  DFM_ORDER_NATURAL = 0,     /* identity: the reference's default CPreconditioner_ILU                    */
+  DFM_ORDER_MULTICOLOR = 1,  /* greedy node colouring, colour-major                                      */
+  DFM_ORDER_BLOCKCOLOR = 2   /* block multicolour: geometric tiles of <= 64 nodes, tiles coloured        */
+};
+
+/* precision of the device Krylov solvers */
+enum { DFM_FP64 = 0, DFM_MIXED = 1 };
+
+const char* dfm_last_error(void);
+int dfm_device_count(int* count);
+
+/* ---- context: one device, one stream ---------------------------------------------------------------- */
+int dfm_ctx_create(int device, dfm_ctx* out);
+int dfm_ctx_destroy(dfm_ctx ctx);
+int dfm_ctx_synchronize(dfm_ctx ctx);
+
+/* ---- seeded structured meshes (synthetic inputs; host-side) ------------------------------------------ */
+/* 2D rectangle lx*ly, nx*ny quads split into 2 P1 triangles; interior nodes jittered by +-jitter*h.
+ * coords[(nx+1)(ny+1)*2], tri[nx*ny*2*3]. Node order x-fastest (include/delfem/mesh_primitive.h:77). */
+int dfm_mesh_rect_tri(int nx, int ny, double lx, double ly, unsigned long long seed, double jitter,
+                      double* coords, int* tri);
+/* 3D box, nx*ny*nz hexes split into 6 Kuhn tets each. coords[(nx+1)(ny+1)(nz+1)*3], tet[nx*ny*nz*6*4]. */
+int dfm_mesh_box_tet(int nx, int ny, int nz, double lx, double ly, double lz, unsigned long long seed,
+                     double jitter, double* coords, int* tet);
+/* Counter-based SplitMix64 draws (uniform [0,1) / standard normal) for seeded materials and loads. */
+int dfm_rng_uniform(unsigned long long seed, unsigned long long stream, size_t n, double* out);
+int dfm_rng_normal(unsigned long long seed, unsigned long long stream, size_t n, double* out);
+
+/* ---- system: CLinearSystem_Field + CMatDia_BlkCrs + CVector_Blk + CBCFlag on the device ------------- */
+/* Pattern from the mesh (CLinearSystem_Field::AddPattern_Field, src/femls/linearsystem_field.cpp:196-280
+ * -> CElemAry::MakePattern_FEM, src/femfield/elem_ary.cpp:225-245). len = dofs per node (1, 2 or 3). */
+int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                        int len, dfm_sys* out);
+int dfm_sys_destroy(dfm_sys sys);
+int dfm_sys_info(dfm_sys sys, int* nblk, int* len, long long* ncrs);
+/* The CRS pattern in the original numbering (rowptr[n+1], colind[ncrs]; CMat_BlkCrs::GetPtrIndPSuP). */
+int dfm_sys_get_pattern(dfm_sys sys, unsigned* rowptr, unsigned* colind);
+
+/* ILU ordering (CPreconditioner_ILU::SetOrdering): order[new] = old, NULL = natural. Call before assembly;
+ * it re-labels the device numbering and rebuilds the schedule. */
+int dfm_sys_set_ordering(dfm_sys sys, const int* order);
+/* Build an ordering of the given kind (tile = nodes per tile edge for BLOCKCOLOR, 0 = default). */
+int dfm_sys_make_ordering(dfm_sys sys, int kind, int tile, int* order_out, int* ncolors_out);
+/* Number of sequential phases of the forward triangular sweep under the current ordering. */
+int dfm_sys_schedule_info(dfm_sys sys, int* nphase_fwd, int* nphase_bwd, int* max_group_levels);
+
+/* Dirichlet flags, n*len ints (CLinearSystem_Field::SetFixedBoundaryCondition_Field -> CBCFlag::SetBC,
+ * src/femls/linearsystem_field.cpp:618-710). */
+int dfm_sys_set_bc(dfm_sys sys, const int* flags);
+/* Nodal field values used by assembly (VALUE u and VELOCITY v; n*len each, v may be NULL). */
+int dfm_sys_set_field(dfm_sys sys, const double* u, const double* v);
+int dfm_sys_get_field(dfm_sys sys, double* u);
+/* u_field += update (CLinearSystem_Field::UpdateValueOfField, src/femls/linearsystem_field.cpp:716-794) */
+int dfm_sys_update_field(dfm_sys sys);
+
+/* InitializeMarge + Fem::Eqn::AddLinSys_<kind> + [surface loads] + FinalizeMarge in one call
+ * (src/ls/linearsystem.cpp:74-128). params: npar doubles per element with stride pstride (0 = shared).
+ * faces/trac: optional P1 surface loads (3D: triangles + traction vector each; a reference extension),
+ * nface may be 0. Returns ||residual|| after Dirichlet zeroing in *res_norm (may be NULL). */
+int dfm_sys_assemble(dfm_sys sys, int kind, const double* params, int npar, int pstride, const double* glob,
+                     int nface, const int* faces, const double* trac, double* res_norm);
+/* Resident inputs: upload element parameters (and optional surface loads) once, then re-assemble from HBM
+ * (Newton steps, repeated solves). dfm_sys_assemble_resident = dfm_sys_assemble on the stored inputs. */
+int dfm_sys_set_params(dfm_sys sys, const double* params, int npar, int pstride);
+int dfm_sys_set_loads(dfm_sys sys, int nface, const int* faces, const double* trac);
+int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double* res_norm);
+/* Matrix values in the original numbering: crs[ncrs*len*len], dia[n*len*len]. */
+int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia);
+
+/* ---- ILinearSystemPreconditioner_Sol handle ops (include/delfem/ls/linearsystem_interface_solver.h:63-82) */
+int dfm_sys_resize_tmp(dfm_sys sys, int ntmp);
+int dfm_sys_ntmp(dfm_sys sys, int* ntmp);
+int dfm_sys_dot(dfm_sys sys, int iv1, int iv2, double* out);
+int dfm_sys_copy(dfm_sys sys, int iv1, int iv2);
+int dfm_sys_scal(dfm_sys sys, double alpha, int iv);
+int dfm_sys_axpy(dfm_sys sys, double alpha, int iv1, int iv2);
+int dfm_sys_matvec(dfm_sys sys, double alpha, int iv1, double beta, int iv2);
+int dfm_sys_get_vector(dfm_sys sys, int iv, double* out);
+int dfm_sys_set_vector(dfm_sys sys, int iv, const double* in);
+
+/* ---- ILU(0) preconditioner (CPreconditioner_ILU, src/ls/preconditioner.cpp:57-335) ------------------- */
+/* Numeric factorisation of the assembled matrix (SetValue -> DoILUDecomp, matdiafrac_blkcrs.cpp:774-1107).
+ * Returns DFM_ERR_SINGULAR when more than 10 pivots were singular (the reference returns false). */
+int dfm_prec_factor(dfm_sys sys);
+/* In-place z = M^-1 z on handle iv (SolvePrecond). */
+int dfm_prec_apply(dfm_sys sys, int iv);
+/* The factor in the device numbering: frowptr[n+1], fcolind[ncrs], diaind[n], fcrs, fdia (L unscaled,
+ * D^-1, D^-1 U), for comparison with CMatDiaFrac_BlkCrs under the same ordering. */
+int dfm_prec_get_factor(dfm_sys sys, unsigned* frowptr, unsigned* fcolind, unsigned* diaind, double* fcrs,
+                        double* fdia);
+
+/* ---- device-resident Krylov loops (LsSol::Solve_PCG / Solve_PBiCGSTAB, src/ls/solver_ls_iter.cpp) ----
+ * Same contract as the reference: conv_ratio in = tolerance, out = achieved ratio; iteration in = max,
+ * out = the 0-based index at convergence for PCG (1-based loop index for PBiCGSTAB). Returns 1 when
+ * converged, 0 when not, negative on error. Scalars stay on the device; the host polls a done flag.
+ * DFM_FP64 runs the reference algorithm in FP64 on the recursive residual. DFM_MIXED keeps FP32 copies of
+ * the matrix and the ILU factor and runs FP32 SpMV/ILU inside FP64 iterative refinement on the true
+ * residual r = b - A x (FP64 matrix, FP64 dots and update); its conv_ratio is the true relative residual. */
+int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration);
+int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+/* ||b - A x|| / ||b|| in FP64 for b = the residual as assembled and x = the update (after a solve). */
+int dfm_sys_true_residual(dfm_sys sys, double* rel);
+
+/* ---- timing of the last device solve (HIP events on the system's stream) ----------------------------- */
+int dfm_sys_last_solve_stats(dfm_sys sys, double* ms_total, double* ms_per_iter, unsigned* inner_iters);
+/* Average device time of one launch of a hot kernel over `reps` back-to-back launches, timed with HIP events
+ * on the system's own stream: which = 0 SpMV FP64 (y = A x), 1 SpMV FP32, 2 ILU(0) apply FP64 (all phases of
+ * the forward and backward sweeps), 3 ILU(0) apply FP32. Also reports the algorithmic bytes of one launch. */
+int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes);
+/* Device memory currently allocated by the library (bytes). */
+int dfm_device_bytes(long long* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFM_GPU_H */

@@ -1,0 +1,215 @@
+"""HIP product (through the C-ABI) vs the oracle restatement, on the same seeded inputs.
+
+Bit-exact: pattern, assembled matrix and residual (every element kind, per-element materials, natural and
+permuted device numbering), ILU(0) factor pattern and values under the same ordering, forward/backward
+substitution for orderings without intra-group dependencies. Tolerance-based: SpMV and dot products
+(parallel summation order; rel 1e-13), Krylov solutions (iteration count within 2 of the oracle's, solution
+error bounded by the residual tolerance)."""
+import numpy as np
+import pytest
+
+import meshgen as M
+from oracle import pyref as P
+
+pytestmark = pytest.mark.gpu
+
+N = None
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    global N
+    from delfem_amd import _native
+
+    N = _native
+    c = _native.Context(0)
+    yield c
+    c.close()
+
+
+LAM2, MU2 = 0.3 / (1 - 0.09), 1.0 / 2.6
+LAM3, MU3 = 0.3 / (1.3 * 0.4), 1.0 / 2.6
+
+
+def _mesh(ndim, small=True):
+    if ndim == 2:
+        return M.rect_tri(12, 7, 1.0, 0.4, 31) if small else M.rect_tri(40, 24, 1.0, 0.6, 31)
+    return M.box_tet(4, 3, 3, 1.0, 0.8, 0.7, 32) if small else M.box_tet(8, 6, 6, 1.0, 0.8, 0.7, 32)
+
+
+def _flags(coords, ndim, length):
+    x = coords.reshape(-1, ndim)[:, 0]
+    f = np.zeros((x.size, length), np.int32)
+    f[x == x.min()] = 1
+    return f.ravel()
+
+
+KINDS = [
+    ("poisson2d", 2, 1, 0, lambda r, n: np.stack([r.uniform(0.5, 2, n), np.full(n, 1.0)], 1), True, False),
+    ("poisson3d", 3, 1, 0, lambda r, n: np.stack([r.uniform(0.5, 2, n), np.full(n, 0.7)], 1), True, False),
+    ("diffusion2d", 2, 1, 5, lambda r, n: np.stack([np.full(n, 1.1), r.uniform(0.5, 2, n), np.full(n, 0.3)], 1),
+     True, True),
+    ("solid2d", 2, 2, 1, lambda r, n: np.stack([LAM2 * r.uniform(0.5, 2, n), np.full(n, MU2), np.ones(n),
+                                                np.zeros(n), np.full(n, -0.01)], 1), True, False),
+    ("solid3d", 3, 3, 2, lambda r, n: np.stack([np.full(n, LAM3), MU3 * r.uniform(0.5, 2, n), np.ones(n),
+                                                np.full(n, 0.01), np.zeros(n), np.full(n, -0.02)], 1), True, False),
+    ("stvk2d", 2, 2, 3, lambda r, n: np.stack([np.full(n, LAM2), MU2 * r.uniform(0.5, 2, n), np.ones(n),
+                                               np.zeros(n), np.full(n, -0.05)], 1), True, False),
+    ("stvk3d", 3, 3, 4, lambda r, n: np.stack([LAM3 * r.uniform(0.5, 2, n), np.full(n, MU3), np.ones(n),
+                                               np.zeros(n), np.zeros(n), np.full(n, -0.05)], 1), True, False),
+]
+
+
+def _setup(ctx, case, ordering, small=True):
+    name, ndim, length, kind, pfun, use_u, use_v = case
+    co, ln = _mesh(ndim, small)
+    n = co.size // ndim
+    nelem = ln.size // (ndim + 1)
+    rng = np.random.default_rng(17)
+    params = pfun(rng, nelem)
+    u = rng.uniform(-0.05, 0.05, n * length) if use_u else None
+    v = rng.uniform(-0.05, 0.05, n * length) if use_v else None
+    flags = _flags(co, ndim, length)
+    glob = (0.05, 0.6)
+    cpu = P.CpuSystem(ndim, co, ln, length)
+    cpu.assemble(kind, params, u, v, glob)
+    cpu.set_bc(flags)
+    gs = N.System(ctx, ndim, co, ln, length)
+    order = None
+    if ordering != N.ORDER_NATURAL:
+        order, _ = gs.make_ordering(ordering)
+        gs.set_ordering(order)
+    gs.set_bc(flags)
+    gs.set_field(u, v)
+    rn = gs.assemble(kind, params, glob=glob)
+    return cpu, gs, order, rn
+
+
+ORDERS = [("natural", 0), ("multicolor", 1), ("blockcolor", 2)]
+
+
+@pytest.mark.parametrize("case", KINDS, ids=[k[0] for k in KINDS])
+@pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
+def test_assembly_bit_exact(ctx, case, oname, ordering):
+    cpu, gs, order, rn = _setup(ctx, case, ordering)
+    rp, ci = gs.pattern()
+    assert np.array_equal(rp, cpu.rowptr) and np.array_equal(ci, cpu.colind)
+    crs, dia = gs.get_matrix()
+    assert np.array_equal(crs, cpu.crs), np.abs(crs - cpu.crs).max()
+    assert np.array_equal(dia, cpu.dia), np.abs(dia - cpu.dia).max()
+    assert np.array_equal(gs.get_vector(-1), cpu.res)
+    assert abs(rn - np.sqrt(np.sum(cpu.res ** 2))) <= 1e-14 * max(rn, 1e-300)
+
+
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+@pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
+def test_ilu0_factor_bit_exact(ctx, case, oname, ordering):
+    cpu, gs, order, _ = _setup(ctx, case, ordering)
+    gs.factor()
+    f = cpu.ilu0(None if ordering == N.ORDER_NATURAL else order)
+    g = gs.get_factor()
+    for k in ["frowptr", "fcolind", "diaind"]:
+        assert np.array_equal(g[k], f[k]), k
+    assert np.array_equal(g["fcrs"], f["fcrs"]), np.abs(g["fcrs"] - f["fcrs"]).max()
+    assert np.array_equal(g["fdia"], f["fdia"])
+    # preconditioner apply, handle semantics (in place on a temp handle)
+    r = np.random.default_rng(2).normal(size=cpu.nnode * cpu.len)
+    gs.resize_tmp(1)
+    gs.set_vector(0, r)
+    gs.prec_apply(0)
+    z = gs.get_vector(0)
+    zc = cpu.ilu_apply(f, r)
+    if ordering == N.ORDER_BLOCKCOLOR:
+        assert np.allclose(z, zc, rtol=1e-12, atol=1e-12 * np.abs(zc).max())
+    else:
+        assert np.array_equal(z, zc)
+
+
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+def test_handle_ops(ctx, case):
+    cpu, gs, order, _ = _setup(ctx, case, N.ORDER_BLOCKCOLOR)
+    m = cpu.nnode * cpu.len
+    rng = np.random.default_rng(4)
+    x, y = rng.normal(size=m), rng.normal(size=m)
+    gs.resize_tmp(2)
+    gs.set_vector(0, x)
+    gs.set_vector(1, y)
+    assert np.isclose(gs.dot(0, 1), x @ y, rtol=1e-13)
+    gs.matvec(-0.7, 0, 1.3, 1)
+    yc = cpu.matvec(-0.7, x, 1.3, y)
+    assert np.allclose(gs.get_vector(1), yc, rtol=1e-13, atol=1e-13 * np.abs(yc).max())
+    gs.axpy(0.25, 0, 1)
+    assert np.allclose(gs.get_vector(1), yc + 0.25 * x, rtol=1e-13, atol=1e-13 * np.abs(yc).max())
+    gs.scal(2.0, 0)
+    assert np.array_equal(gs.get_vector(0), 2.0 * x)
+    gs.copy(0, 1)
+    assert np.array_equal(gs.get_vector(1), 2.0 * x)
+
+
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+@pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
+@pytest.mark.parametrize("method", ["pcg", "pbicgstab"])
+def test_krylov_vs_oracle(ctx, case, oname, ordering, method):
+    cpu, gs, order, _ = _setup(ctx, case, ordering, small=False)
+    f = cpu.ilu0(None if ordering == N.ORDER_NATURAL else order)
+    ok, conv, it, x, _ = cpu.solve(f, method, 1e-10, 2000)
+    gs.factor()
+    if method == "pcg":
+        gok, gconv, git = gs.solve_pcg(1e-10, 2000)
+    else:
+        gok, gconv, git = gs.solve_pbicgstab(1e-10, 2000)
+    assert ok and gok
+    assert abs(git - it) <= 2, (git, it)
+    assert gconv < 1e-10
+    xg = gs.get_vector(-2)
+    # the GPU solution's true residual, evaluated with the oracle's matrix (size- and rounding-independent)
+    b = cpu.res
+    r = b - cpu.matvec(1.0, xg, 0.0, np.zeros_like(b))
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(b)
+    # both solutions are within cond(A) * tol of the exact one (cond ~ 1e4-1e5 for these elasticity meshes)
+    assert np.linalg.norm(xg - x) <= 1e-5 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+def test_mixed_precision_true_residual(ctx, case):
+    cpu, gs, order, _ = _setup(ctx, case, N.ORDER_BLOCKCOLOR, small=False)
+    gs.factor()
+    ok, conv, it = gs.solve_pcg(1e-8, 5000, N.FP64)
+    x64 = gs.get_vector(-2)
+    assert ok
+    gs2 = gs
+    gs2.set_vector(-1, cpu.res)
+    ok, conv, it = gs2.solve_pcg(1e-8, 5000, N.MIXED)
+    assert ok and conv < 1e-8
+    assert gs2.true_residual() < 1e-8
+    xm = gs2.get_vector(-2)
+    # independent check with the oracle's FP64 matrix: b - A xm
+    b = cpu.res
+    r = b - cpu.matvec(1.0, xm, 0.0, np.zeros_like(b))
+    assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(b)
+    # the two solutions agree to within cond(A) * tol (cond ~ 1e5 here); both residuals are below 1e-8
+    assert np.linalg.norm(xm - x64) <= 1e-3 * np.linalg.norm(x64)
+    # mixed precision must not cost many more iterations than FP64 (reliable updates keep the Krylov space)
+    _, _, it64 = gs.solve_pcg(1e-8, 5000, N.FP64)
+    assert it <= 1.5 * it64 + 10, (it, it64)
+
+
+def test_pcg_not_converged_contract(ctx):
+    cpu, gs, order, _ = _setup(ctx, KINDS[0], N.ORDER_NATURAL, small=False)
+    gs.factor()
+    ok, conv, it = gs.solve_pcg(1e-14, 3)
+    f = cpu.ilu0()
+    cok, cconv, cit, _, _ = cpu.solve(f, "pcg", 1e-14, 3)
+    assert not ok and not cok and it == cit == 3
+    assert np.isclose(conv, cconv, rtol=1e-10)
+
+
+def test_singular_factor_reports(ctx):
+    """An unconstrained Poisson matrix with zero conductivity: every pivot is singular (DoILUDecomp false)."""
+    co, ln = _mesh(2)
+    gs = N.System(ctx, 2, co, ln, 1)
+    gs.set_bc(np.zeros(co.size // 2, np.int32))
+    gs.assemble(N.POISSON, [0.0, 1.0])
+    with pytest.raises(N.DfmError) as e:
+        gs.factor()
+    assert e.value.code == N.ERR_SINGULAR

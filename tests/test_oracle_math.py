@@ -1,0 +1,225 @@
+"""What the mathematics fixes about the oracle, independently of the reference: closed-form element matrices,
+rigid-body null spaces, zero row sums, symmetry, patch tests, the StVK tangent as the derivative of the
+residual, exact factorisation of matrices whose ILU(0) has no dropped fill, PCG with an exact preconditioner,
+and the consistent surface loads (the one oracle function with no reference counterpart)."""
+import numpy as np
+import pytest
+
+import meshgen as M
+from oracle import pyref as P
+
+
+def dense(cpu):
+    n, b = cpu.nnode, cpu.len
+    A = np.zeros((n * b, n * b))
+    for i in range(n):
+        A[i * b:(i + 1) * b, i * b:(i + 1) * b] = cpu.dia[i * b * b:(i + 1) * b * b].reshape(b, b)
+        for q in range(cpu.rowptr[i], cpu.rowptr[i + 1]):
+            j = cpu.colind[q]
+            A[i * b:(i + 1) * b, j * b:(j + 1) * b] = cpu.crs[q * b * b:(q + 1) * b * b].reshape(b, b)
+    return A
+
+
+def test_poisson_row_sums_zero_and_symmetry():
+    co, tri = M.rect_tri(6, 5, 1.0, 1.0, 3)
+    cpu = P.CpuSystem(2, co, tri, 1)
+    cpu.assemble(P.POISSON, [1.7, 0.0])
+    A = dense(cpu)
+    assert np.allclose(A.sum(axis=1), 0.0, atol=1e-13)
+    assert np.allclose(A, A.T, atol=1e-14)
+
+
+def test_poisson_closed_form_single_triangle():
+    co = np.array([0.0, 0.0, 2.0, 0.0, 0.0, 1.0])
+    cpu = P.CpuSystem(2, co, np.array([0, 1, 2]), 1)
+    cpu.assemble(P.POISSON, [3.0, 6.0])
+    A = dense(cpu)
+    # grad N: N0 = 1 - x/2 - y, N1 = x/2, N2 = y; area 1; K = alpha*area*G G^T
+    G = np.array([[-0.5, -1.0], [0.5, 0.0], [0.0, 1.0]])
+    assert np.allclose(A, 3.0 * 1.0 * G @ G.T, rtol=1e-15)
+    assert np.allclose(cpu.res, 6.0 * 1.0 / 3.0)
+
+
+def _rigid_modes_2d(co):
+    x, y = co[0::2], co[1::2]
+    t1 = np.stack([np.ones_like(x), np.zeros_like(x)], 1).ravel()
+    t2 = np.stack([np.zeros_like(x), np.ones_like(x)], 1).ravel()
+    r = np.stack([-y, x], 1).ravel()
+    return [t1, t2, r]
+
+
+def _rigid_modes_3d(co):
+    p = co.reshape(-1, 3)
+    modes = []
+    for d in range(3):
+        t = np.zeros_like(p)
+        t[:, d] = 1
+        modes.append(t.ravel())
+    for (a, b) in [(0, 1), (1, 2), (2, 0)]:
+        r = np.zeros_like(p)
+        r[:, a] = -p[:, b]
+        r[:, b] = p[:, a]
+        modes.append(r.ravel())
+    return modes
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_linear_elasticity_rigid_null_space(ndim):
+    if ndim == 2:
+        co, ln = M.rect_tri(5, 4, 1.0, 0.5, 9)
+        cpu = P.CpuSystem(2, co, ln, 2)
+        cpu.assemble(P.SOLID2D, [0.4, 0.3, 0.0, 0.0, 0.0])
+        modes = _rigid_modes_2d(co)
+    else:
+        co, ln = M.box_tet(3, 2, 2, 1.0, 0.8, 0.6, 9)
+        cpu = P.CpuSystem(3, co, ln, 3)
+        cpu.assemble(P.SOLID3D, [0.4, 0.3, 0.0, 0.0, 0.0, 0.0])
+        modes = _rigid_modes_3d(co)
+    A = dense(cpu)
+    assert np.allclose(A, A.T, atol=1e-13)
+    for mvec in modes:
+        assert np.abs(A @ mvec).max() < 1e-12
+    w = np.linalg.eigvalsh(A)
+    assert np.sum(np.abs(w) < 1e-10) == len(modes)  # exactly the rigid modes
+
+
+def test_tet_element_matrix_closed_form():
+    """Ke = V B^T D B for a single tetrahedron (Voigt form), against the restatement."""
+    co = np.array([0, 0, 0, 1.0, 0, 0, 0, 2.0, 0, 0, 0, 3.0])
+    lam, mu = 0.7, 0.4
+    cpu = P.CpuSystem(3, co, np.array([0, 1, 2, 3]), 3)
+    cpu.assemble(P.SOLID3D, [lam, mu, 0.0, 0.0, 0.0, 0.0])
+    A = dense(cpu)
+    p = co.reshape(4, 3)
+    T = np.hstack([np.ones((4, 1)), p])
+    C = np.linalg.inv(T)
+    dN = C[1:, :].T  # gradients of the barycentric functions, rows per node
+    V = abs(np.linalg.det(T)) / 6
+    B = np.zeros((6, 12))
+    for a in range(4):
+        gx, gy, gz = dN[a]
+        B[:, 3 * a:3 * a + 3] = [[gx, 0, 0], [0, gy, 0], [0, 0, gz], [gy, gx, 0], [0, gz, gy], [gz, 0, gx]]
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] += 2 * mu
+    D[3:, 3:] = np.eye(3) * mu
+    # the reference's TetVolume uses 0.1666666666666 instead of 1/6: a 4e-13 relative bias
+    assert np.allclose(A, V * B.T @ D @ B, rtol=1e-11, atol=1e-12)
+
+
+def test_patch_test_linear_field_zero_interior_residual():
+    """A linear displacement field solves the homogeneous equations: interior residuals vanish."""
+    co, ln = M.rect_tri(6, 6, 1.0, 1.0, 21)
+    cpu = P.CpuSystem(2, co, ln, 2)
+    x, y = co[0::2], co[1::2]
+    u = np.stack([0.01 * x + 0.02 * y, -0.03 * x + 0.005 * y], 1).ravel()
+    cpu.assemble(P.SOLID2D, [0.5, 0.4, 0.0, 0.0, 0.0], u=u)
+    ix, iy = np.arange(49) % 7, np.arange(49) // 7
+    inner = (ix > 0) & (ix < 6) & (iy > 0) & (iy < 6)
+    assert np.abs(cpu.res.reshape(-1, 2)[inner]).max() < 1e-13
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_stvk_tangent_is_residual_derivative(ndim):
+    """K(u) = -d r(u) / d u (central differences, O(h^2)); at u = 0 StVK equals linear elasticity."""
+    if ndim == 2:
+        co, ln = M.rect_tri(3, 2, 1.0, 0.6, 5)
+        kind, lin, par = P.STVK2D, P.SOLID2D, [0.5, 0.4, 1.0, 0.0, -0.1]
+    else:
+        co, ln = M.box_tet(2, 1, 1, 1.0, 0.6, 0.5, 5)
+        kind, lin, par = P.STVK3D, P.SOLID3D, [0.5, 0.4, 1.0, 0.0, 0.0, -0.1]
+    length = ndim
+    n = co.size // ndim
+    rng = np.random.default_rng(3)
+    u = rng.uniform(-0.05, 0.05, n * length)
+    cpu = P.CpuSystem(ndim, co, ln, length)
+    cpu.assemble(kind, par, u=u)
+    K = dense(cpu)
+    h = 1e-6
+    J = np.zeros_like(K)
+    for k in range(n * length):
+        e = np.zeros(n * length)
+        e[k] = h
+        cpu.zero()
+        cpu.assemble(kind, par, u=u + e)
+        rp = cpu.res.copy()
+        cpu.zero()
+        cpu.assemble(kind, par, u=u - e)
+        J[:, k] = -(rp - cpu.res) / (2 * h)
+    assert np.abs(K - J).max() < 1e-6 * np.abs(K).max()
+    cpu.zero()
+    cpu.assemble(kind, par, u=np.zeros(n * length))
+    c2 = P.CpuSystem(ndim, co, ln, length)
+    c2.assemble(lin, par, u=np.zeros(n * length))
+    assert np.allclose(dense(cpu), dense(c2), rtol=1e-13, atol=1e-14)
+
+
+def test_ilu0_exact_when_no_fill_is_dropped():
+    """A single tetrahedron has a complete node graph, so ILU(0) drops no fill and equals the exact block LU:
+    one application solves the system, and PCG converges at its first iteration (reported 0-based)."""
+    co = np.array([0, 0, 0, 1.0, 0.1, 0, 0.2, 1.1, 0, 0.1, 0.3, 0.9])
+    cpu = P.CpuSystem(3, co, np.array([0, 1, 2, 3]), 1)
+    cpu.assemble(P.POISSON, [1.3, 1.0])
+    flags = np.zeros(4, np.int32)
+    flags[0] = 1
+    cpu.set_bc(flags)
+    A = dense(cpu)
+    assert np.linalg.cond(A) < 1e3
+    f = cpu.ilu0()
+    b = np.random.default_rng(1).normal(size=4)
+    z = cpu.ilu_apply(f, b)
+    assert np.allclose(A @ z, b, rtol=1e-12, atol=1e-12)
+    ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-12, 10, res=b)
+    assert ok and it == 0
+    assert np.allclose(A @ x, b, atol=1e-10)
+
+
+def test_pcg_terminates_within_n_on_tiny_spd():
+    co, tri = M.rect_tri(3, 3, 1.0, 1.0, 2)
+    cpu = P.CpuSystem(2, co, tri, 1)
+    cpu.assemble(P.POISSON, [1.0, 1.0])
+    flags = np.zeros(16, np.int32)
+    flags[[0, 3, 12, 15]] = 1
+    cpu.set_bc(flags)
+    order = np.random.default_rng(0).permutation(16).astype(np.int32)
+    f = cpu.ilu0(order)
+    ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-12, 16)
+    A = dense(cpu)
+    assert ok and it < 16
+    assert np.allclose(A @ x, cpu.res, atol=1e-10)
+    xs = np.linalg.solve(A, cpu.res)
+    assert np.allclose(x, xs, rtol=1e-9, atol=1e-12)
+    ok, conv, it, x2, _ = cpu.solve(f, "pbicgstab", 1e-12, 40)
+    assert np.allclose(x2, xs, rtol=1e-8, atol=1e-11)
+
+
+def test_dirichlet_rows_and_columns():
+    co, tri = M.rect_tri(4, 3, 1.0, 1.0, 2)
+    cpu = P.CpuSystem(2, co, tri, 2)
+    cpu.assemble(P.SOLID2D, [0.5, 0.3, 1.0, 0.0, -1.0])
+    flags = np.zeros((20, 2), np.int32)
+    flags[0, :] = 1
+    flags[5, 1] = 1
+    cpu.set_bc(flags.ravel())
+    A = dense(cpu)
+    for k in np.flatnonzero(flags.ravel()):
+        e = np.zeros(40)
+        e[k] = 1
+        assert np.array_equal(A[k], e) and np.array_equal(A[:, k], e)
+        assert cpu.res[k] == 0.0
+
+
+def test_traction_loads_closed_form():
+    """Consistent P1 loads t*A/3 per node: they sum to t times the loaded area and split equally."""
+    nx = 3
+    co, tet = M.box_tet(nx, nx, nx, 1.0, 1.0, 1.0, 4, 0.0)
+    faces = M.box_face_tris(nx, nx, nx, 2, 1)
+    t = np.tile([0.1, -0.2, 0.3], faces.shape[0])
+    cpu = P.CpuSystem(3, co, tet, 3)
+    cpu.traction(faces.ravel(), t)
+    tot = cpu.res.reshape(-1, 3).sum(axis=0)
+    assert np.allclose(tot, [0.1, -0.2, 0.3], rtol=1e-14)
+    # a corner node of the diagonal (shared by 2 triangles of one quad) gets 2 * A/3 * t
+    a = 0.5 / nx / nx
+    top_corner = (nx * (nx + 1) + 0) * (nx + 1) + 0 + 0  # node (0,0,nx)
+    assert np.allclose(cpu.res.reshape(-1, 3)[top_corner], np.array([0.1, -0.2, 0.3]) * 2 * a / 3)
