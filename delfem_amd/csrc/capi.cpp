@@ -44,6 +44,8 @@ Sys::~Sys() {
   d_work.release(); d_work32.release(); d_partial.release(); d_counter.release(); d_scal.release(); d_iflag.release();
   for (Sweep* w : {&fwd, &bwd}) { w->d_grp.release(); w->d_rowlist.release(); w->d_lev.release(); }
   d_par.release(); d_lptr.release(); d_loads.release();
+  for (auto& g : gexec)
+    if (g) (void)hipGraphExecDestroy(g);
   if (h_pinned) hipHostFree(h_pinned);
   if (h_pinned_d) hipHostFree(h_pinned_d);
 }

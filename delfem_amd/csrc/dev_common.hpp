@@ -146,8 +146,12 @@ struct InGroup {
   static constexpr int MAXIN = (B == 2) ? 4 : (B == 3 ? 7 : 8);
 };
 
+// GPB groups per workgroup, one wavefront per group: few, larger workgroups keep every group of a phase
+// resident at once (the per-CU workgroup limit, not registers, bounds residency of one-wave workgroups).
+constexpr int kGPB = 4;
+
 template <int B, typename T, typename A, typename VI, typename VO>
-__global__ void __launch_bounds__(64) k_ilu_fwd(int gbase, const int* __restrict__ grp, const int* __restrict__ rowlist,
+__global__ void __launch_bounds__(64 * kGPB) k_ilu_fwd(int gbase, int gend, const int* __restrict__ grp, const int* __restrict__ rowlist,
                                                 const uint8_t* __restrict__ lev, int nlev,
                                                 const unsigned* __restrict__ rp, const unsigned* __restrict__ lsplit,
                                                 const unsigned* __restrict__ di, const unsigned* __restrict__ ci,
@@ -156,12 +160,15 @@ __global__ void __launch_bounds__(64) k_ilu_fwd(int gbase, const int* __restrict
   constexpr int BS = B * B;
   constexpr int MAXIN = InGroup<B>::MAXIN;
   if (done && *done) return;
-  const int g = gbase + blockIdx.x;
-  const int start = grp[2 * g], end = grp[2 * g + 1];
-  const int lane = threadIdx.x;
+  const int wv = threadIdx.x / 64;
+  const int g = gbase + blockIdx.x * kGPB + wv;
+  const bool gok = g < gend;
+  const int start = gok ? grp[2 * g] : 0, end = gok ? grp[2 * g + 1] : 0;
+  const int lane = threadIdx.x % 64;
   const bool active = lane < end - start;
   const int row = active ? (rowlist ? rowlist[start + lane] : start + lane) : 0;
-  __shared__ A xs[64 * B];
+  __shared__ A xsb[kGPB][64 * B];
+  A* xs = xsb[wv];
   A t[B], D[BS];
   int jl[MAXIN];
   A lv[MAXIN][BS];
@@ -182,7 +189,22 @@ __global__ void __launch_bounds__(64) k_ilu_fwd(int gbase, const int* __restrict
     }
 #pragma unroll
     for (int e = 0; e < BS; e++) D[e] = (A)fdia[(size_t)row * BS + e];
-    for (unsigned q = rp[row]; q < ls; q++) {
+    // rows of earlier phases: all column indices, then all gathers, then the products in ascending order
+    const unsigned q0 = rp[row];
+    unsigned jo[MAXIN];
+#pragma unroll
+    for (int k = 0; k < MAXIN; k++)
+      if (q0 + k < ls) jo[k] = ci[q0 + k];
+    A xo[MAXIN][B];
+#pragma unroll
+    for (int k = 0; k < MAXIN; k++)
+      if (q0 + k < ls)
+#pragma unroll
+        for (int c = 0; c < B; c++) xo[k][c] = (A)x[(size_t)jo[k] * B + c];
+#pragma unroll
+    for (int k = 0; k < MAXIN; k++)
+      if (q0 + k < ls) blk_sub<B>(t, fcrs + (size_t)(q0 + k) * BS, xo[k]);
+    for (unsigned q = q0 + MAXIN; q < ls; q++) {
       const unsigned j = ci[q];
       A v[B];
 #pragma unroll
@@ -215,21 +237,30 @@ __global__ void __launch_bounds__(64) k_ilu_fwd(int gbase, const int* __restrict
         for (int c = 0; c < B; c++) v[c] = xs[j * B + c];
         blk_sub<B>(t, fcrs + (size_t)q * BS, v);
       }
+      A y[B];
 #pragma unroll
       for (int r = 0; r < B; r++) {
         A s = D[r * B] * t[0];
 #pragma unroll
         for (int c = 1; c < B; c++) s += D[r * B + c] * t[c];
-        xs[lane * B + r] = s;
-        x[(size_t)row * B + r] = (VO)s;
+        y[r] = s;
+      }
+#pragma unroll
+      for (int r = 0; r < B; r++) {
+        xs[lane * B + r] = y[r];
+        t[r] = y[r];
       }
     }
+    // only LDS traffic inside the level loop: the barrier's fence then has no global store to wait for
     if (nlev > 1) __syncthreads();
   }
+  if (active)
+#pragma unroll
+    for (int r = 0; r < B; r++) x[(size_t)row * B + r] = (VO)t[r];
 }
 
 template <int B, typename T, typename A, typename VX>
-__global__ void __launch_bounds__(64) k_ilu_bwd(int gbase, const int* __restrict__ grp, const int* __restrict__ rowlist,
+__global__ void __launch_bounds__(64 * kGPB) k_ilu_bwd(int gbase, int gend, const int* __restrict__ grp, const int* __restrict__ rowlist,
                                                 const uint8_t* __restrict__ lev, int nlev,
                                                 const unsigned* __restrict__ rp, const unsigned* __restrict__ usplit,
                                                 const unsigned* __restrict__ di, const unsigned* __restrict__ ci,
@@ -237,12 +268,15 @@ __global__ void __launch_bounds__(64) k_ilu_bwd(int gbase, const int* __restrict
   constexpr int BS = B * B;
   constexpr int MAXIN = InGroup<B>::MAXIN;
   if (done && *done) return;
-  const int g = gbase + blockIdx.x;
-  const int start = grp[2 * g], end = grp[2 * g + 1];
-  const int lane = threadIdx.x;
+  const int wv = threadIdx.x / 64;
+  const int g = gbase + blockIdx.x * kGPB + wv;
+  const bool gok = g < gend;
+  const int start = gok ? grp[2 * g] : 0, end = gok ? grp[2 * g + 1] : 0;
+  const int lane = threadIdx.x % 64;
   const bool active = lane < end - start;
   const int row = active ? (rowlist ? rowlist[start + lane] : start + lane) : 0;
-  __shared__ A xs[64 * B];
+  __shared__ A xsb[kGPB][64 * B];
+  A* xs = xsb[wv];
   A t[B];
   int jl[MAXIN];
   A uv[MAXIN][BS];
@@ -261,8 +295,22 @@ __global__ void __launch_bounds__(64) k_ilu_bwd(int gbase, const int* __restrict
         for (int e = 0; e < BS; e++) uv[k][e] = (A)fcrs[(size_t)(d + k) * BS + e];
       }
     }
-    // rows of later phases first; with no intra-group entries this is the reference's ascending order
-    for (unsigned q = us; q < qe; q++) {
+    // rows of later phases first (with no intra-group entries this is the reference's ascending order):
+    // all column indices, then all gathers, then the products
+    unsigned jo[MAXIN];
+#pragma unroll
+    for (int k = 0; k < MAXIN; k++)
+      if (us + k < qe) jo[k] = ci[us + k];
+    A xo[MAXIN][B];
+#pragma unroll
+    for (int k = 0; k < MAXIN; k++)
+      if (us + k < qe)
+#pragma unroll
+        for (int c = 0; c < B; c++) xo[k][c] = (A)x[(size_t)jo[k] * B + c];
+#pragma unroll
+    for (int k = 0; k < MAXIN; k++)
+      if (us + k < qe) blk_sub<B>(t, fcrs + (size_t)(us + k) * BS, xo[k]);
+    for (unsigned q = us + MAXIN; q < qe; q++) {
       const unsigned j = ci[q];
       A v[B];
 #pragma unroll
@@ -296,13 +344,13 @@ __global__ void __launch_bounds__(64) k_ilu_bwd(int gbase, const int* __restrict
         blk_sub<B>(t, fcrs + (size_t)q * BS, v);
       }
 #pragma unroll
-      for (int r = 0; r < B; r++) {
-        xs[lane * B + r] = t[r];
-        x[(size_t)row * B + r] = (VX)t[r];
-      }
+      for (int r = 0; r < B; r++) xs[lane * B + r] = t[r];
     }
     if (nlev > 1) __syncthreads();
   }
+  if (active)
+#pragma unroll
+    for (int r = 0; r < B; r++) x[(size_t)row * B + r] = (VX)t[r];
 }
 
 }  // namespace dfm

@@ -126,6 +126,11 @@ struct Sys {
   DBuf<double> d_loads;
   bool has_loads = false;
 
+  // captured chunks of Krylov iterations (hipGraph), per loop kind: 0 PCG FP64, 1 PCG mixed, 2 PBiCGSTAB
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  const void* gkey[3] = {nullptr, nullptr, nullptr};  // work buffer the graph was captured with
+  int gchunk = 16;
+
   bool assembled = false, factored = false, factored32 = false;
   double last_ms = 0.0, last_ms_iter = 0.0;
   unsigned last_inner = 0;

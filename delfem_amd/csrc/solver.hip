@@ -28,7 +28,7 @@ enum {
   S_TMP = 15
 };
 // int flags
-enum { F_DONE = 0, F_IT = 1, F_ITOUT = 2, F_MAXIT = 3, F_NREL = 5 };
+enum { F_DONE = 0, F_IT = 1, F_ITOUT = 2, F_MAXIT = 3, F_NREL = 5, F_PCUR = 6, F_PNEXT = 7 };
 // partial slots, kMaxPartial doubles per value
 enum { P_A = 0, P_B = 2, P_C = 4, P_D = 6, P_NSLOT = 8 };
 
@@ -44,6 +44,8 @@ __global__ void k_solver_init(double* sc, int* fl, double tol2, int maxit, doubl
     fl[F_ITOUT] = maxit;
     fl[F_MAXIT] = maxit;
     fl[F_NREL] = 0;
+    fl[F_PCUR] = 0;
+    fl[F_PNEXT] = 0;
   }
 }
 
@@ -95,15 +97,22 @@ __global__ void __launch_bounds__(NT) k_pcg_rho0(const double* slot, int G, doub
 }
 
 // q = A p, partials of (p, q). Block 0 advances the iteration counter (read only by later kernels).
-template <int B, int TPR, typename T>
+// Mixed precision (REL): when a reliable update is pending, the same row sweep also forms the FP64 true
+// residual b - A x with the FP64 matrix, stores it (rounded) as the new FP32 residual and produces the
+// partials of its squared norm.
+template <int B, int TPR, typename T, bool REL>
 __global__ void __launch_bounds__(NT) k_pcg_spmv(int n, const unsigned* __restrict__ rp, const unsigned* __restrict__ ci,
                                                  const T* __restrict__ crs, const T* __restrict__ dia,
-                                                 const T* __restrict__ p, T* __restrict__ q, double* slot, int* fl) {
+                                                 const T* __restrict__ p, T* __restrict__ q, double* slot,
+                                                 const double* __restrict__ crs64, const double* __restrict__ dia64,
+                                                 const double* __restrict__ x64, const double* __restrict__ b64,
+                                                 T* __restrict__ r, double* slot_true, int* fl) {
   if (fl[F_DONE]) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) fl[F_IT] = fl[F_IT] + 1;
+  const bool rel = REL && fl[F_PCUR];
   constexpr int RPB = NT / TPR;
   const int lane = threadIdx.x % TPR;
-  double v[1] = {0.0};
+  double v[1] = {0.0}, vt[1] = {0.0};
   for (int row = blockIdx.x * RPB + threadIdx.x / TPR; row - (int)(threadIdx.x / TPR) < n; row += gridDim.x * RPB) {
     if (row < n) {
       T acc[B];
@@ -111,14 +120,28 @@ __global__ void __launch_bounds__(NT) k_pcg_spmv(int n, const unsigned* __restri
       if (lane == 0) {
         dia_row<B, T, T, T>(dia, p, row, acc);
 #pragma unroll
-        for (int r = 0; r < B; r++) {
-          q[(size_t)row * B + r] = acc[r];
-          v[0] += (double)p[(size_t)row * B + r] * (double)acc[r];
+        for (int k = 0; k < B; k++) {
+          q[(size_t)row * B + k] = acc[k];
+          v[0] += (double)p[(size_t)row * B + k] * (double)acc[k];
+        }
+      }
+      if (REL && rel) {
+        double a64[B];
+        spmv_row<B, TPR, double, double, double>(rp[row], rp[row + 1], ci, crs64, x64, lane, a64);
+        if (lane == 0) {
+          dia_row<B, double, double, double>(dia64, x64, row, a64);
+#pragma unroll
+          for (int k = 0; k < B; k++) {
+            const double t = b64[(size_t)row * B + k] - a64[k];
+            r[(size_t)row * B + k] = (T)t;
+            vt[0] += t * t;
+          }
         }
       }
     }
   }
   produce<NT, 1>(slot, v);
+  if (REL && rel) produce<NT, 1>(slot_true, vt);
 }
 
 // alpha = rho / (p, q); r -= alpha q; x += alpha p; partials of (r, r)   (solver_ls_iter.cpp:139-144)
@@ -150,6 +173,7 @@ __global__ void __launch_bounds__(NT) k_pcg_judge_rz(long long m, const double* 
   consume<NT, 1>(slot_rr, G, rr);
   const int it = fl[F_IT];
   const double ratio2 = rr[0] * sc[S_INV0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_RR] = rr[0];
   if (ratio2 < sc[S_TOL2] || it + 1 >= fl[F_MAXIT]) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc[S_CONV] = sqrt(ratio2);
@@ -169,20 +193,30 @@ __global__ void __launch_bounds__(NT) k_pcg_judge_rz(long long m, const double* 
 }
 
 // beta = rho_new / rho; p = beta p; p += 1.0 z   (solver_ls_iter.cpp:158-166)
-template <typename T>
+// Mixed precision (REL): when the judgement has scheduled a reliable update, the FP64 solution also takes the
+// FP32 partial solution here (x += xlo, xlo = 0), and block 0 hands the decision on to the next iteration.
+template <typename T, bool REL>
 __global__ void __launch_bounds__(NT) k_pcg_pupdate(long long m, T* p, const T* z, const double* slot_rz, int G,
-                                                    double* sc, const int* fl) {
+                                                    double* sc, int* fl, float* xlo, double* x64) {
   if (fl[F_DONE]) return;
   double rz[1];
   consume<NT, 1>(slot_rz, G, rz);
   const int it = fl[F_IT];
   const double beta = rz[0] / sc[S_RZ + (it & 1)];
   const T bt = (T)beta;
+  const bool fold = REL && fl[F_PNEXT];
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
     const T t = p[i] * bt;
     p[i] = t + z[i];
+    if (REL && fold) {
+      x64[i] += (double)xlo[i];
+      xlo[i] = 0.0f;
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_RZ + ((it + 1) & 1)] = rz[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_RZ + ((it + 1) & 1)] = rz[0];
+    if (REL) fl[F_PCUR] = fl[F_PNEXT];
+  }
 }
 
 // ---- mixed precision: FP32 PCG with reliable updates ------------------------------------------------------
@@ -195,52 +229,10 @@ __device__ inline bool reliable_due(double rr, const double* sc, int it) {
   return rr * sc[S_INV0] < sc[S_TOL2] || rr < sc[S_DELTA2] * sc[S_RMAX + (it & 1)];
 }
 
-__global__ void __launch_bounds__(NT) k_mx_fold(long long m, float* xlo, double* x, const double* slot_rr, int G,
-                                                const double* sc, const int* fl) {
-  if (fl[F_DONE]) return;
-  double rr[1];
-  consume<NT, 1>(slot_rr, G, rr);
-  if (!reliable_due(rr[0], sc, fl[F_IT])) return;
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
-    x[i] += (double)xlo[i];
-    xlo[i] = 0.0f;
-  }
-}
-
-// r_true = b - A x (FP64), r32 = (float) r_true, partials of ||r_true||^2
-template <int B, int TPR>
-__global__ void __launch_bounds__(NT) k_mx_true_res(int n, const unsigned* __restrict__ rp, const unsigned* __restrict__ ci,
-                                                    const double* __restrict__ crs, const double* __restrict__ dia,
-                                                    const double* __restrict__ x, const double* __restrict__ b,
-                                                    float* __restrict__ r32, const double* slot_rr, int G,
-                                                    double* slot_true, const double* sc, const int* fl) {
-  if (fl[F_DONE]) return;
-  double rr[1];
-  consume<NT, 1>(slot_rr, G, rr);
-  if (!reliable_due(rr[0], sc, fl[F_IT])) return;
-  constexpr int RPB = NT / TPR;
-  const int lane = threadIdx.x % TPR;
-  double v[1] = {0.0};
-  for (int row = blockIdx.x * RPB + threadIdx.x / TPR; row - (int)(threadIdx.x / TPR) < n; row += gridDim.x * RPB) {
-    if (row < n) {
-      double acc[B];
-      spmv_row<B, TPR, double, double, double>(rp[row], rp[row + 1], ci, crs, x, lane, acc);
-      if (lane == 0) {
-        dia_row<B, double, double, double>(dia, x, row, acc);
-#pragma unroll
-        for (int r = 0; r < B; r++) {
-          const double t = b[(size_t)row * B + r] - acc[r];
-          r32[(size_t)row * B + r] = (float)t;
-          v[0] += t * t;
-        }
-      }
-    }
-  }
-  produce<NT, 1>(slot_true, v);
-}
-
-// Judgement: on the true residual after a reliable update (convergence only there), else track the max of the
-// recursive residual; then partials of (r, z).
+// Judgement. If this iteration's SpMV formed a true residual (a reliable update scheduled one iteration
+// earlier), convergence is decided on it: the FP64 solution then is the one of the previous iteration.
+// Otherwise the recursive residual only schedules the next reliable update (it dropped by delta, or looks
+// converged). Then partials of (r, z).
 __global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r, const float* z, const double* slot_rr,
                                                     int G, const double* slot_true, int Gt, double* slot_rz, double* sc,
                                                     int* fl) {
@@ -248,9 +240,8 @@ __global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r,
   double rr[1];
   consume<NT, 1>(slot_rr, G, rr);
   const int it = fl[F_IT];
-  const bool rel = reliable_due(rr[0], sc, it);
-  double rmax_next = fmax(sc[S_RMAX + (it & 1)], rr[0]);
-  if (rel) {
+  double rmax = sc[S_RMAX + (it & 1)];
+  if (fl[F_PCUR]) {
     double rt[1];
     consume<NT, 1>(slot_true, Gt, rt);
     const double ratio2 = rt[0] * sc[S_INV0];
@@ -258,12 +249,12 @@ __global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r,
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc[S_CONV] = sqrt(ratio2);
         sc[S_RR] = rt[0];
-        fl[F_ITOUT] = it;
+        fl[F_ITOUT] = it - 1;
         fl[F_DONE] = 1;
       }
       return;
     }
-    rmax_next = rt[0];
+    rmax = rt[0];
   }
   if (it + 1 >= fl[F_MAXIT]) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -273,9 +264,12 @@ __global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r,
     }
     return;
   }
+  const bool due = rr[0] * sc[S_INV0] < sc[S_TOL2] || rr[0] < sc[S_DELTA2] * rmax;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc[S_RMAX + ((it + 1) & 1)] = rmax_next;
-    if (rel) fl[F_NREL] = fl[F_NREL] + 1;
+    sc[S_RR] = rr[0];
+    sc[S_RMAX + ((it + 1) & 1)] = due ? rr[0] : fmax(rmax, rr[0]);
+    fl[F_PNEXT] = due ? 1 : 0;
+    if (due) fl[F_NREL] = fl[F_NREL] + 1;
   }
   double v[1] = {0.0};
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT)
@@ -417,7 +411,7 @@ static hipError_t precond_T(const Sys& s, const T* fcrs, const T* fdia, const T*
   for (int p = 0; p < f.nphase(); p++) {
     const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_ilu_fwd<B, T, A, T, T>), dim3(ng), dim3(64), 0, st, g0, f.d_grp.p,
+    hipLaunchKernelGGL((k_ilu_fwd<B, T, A, T, T>), dim3((ng + kGPB - 1) / kGPB), dim3(64 * kGPB), 0, st, g0, g0 + ng, f.d_grp.p,
                        f.rowlist.empty() ? nullptr : f.d_rowlist.p, f.d_lev.p, f.phase_nlev[p], s.d_rowptr.p,
                        s.d_lsplit.p, s.d_diaind.p, s.d_colind.p, fcrs, fdia, src, z, done);
   }
@@ -425,7 +419,7 @@ static hipError_t precond_T(const Sys& s, const T* fcrs, const T* fdia, const T*
   for (int p = 0; p < b.nphase(); p++) {
     const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_ilu_bwd<B, T, A, T>), dim3(ng), dim3(64), 0, st, g0, b.d_grp.p,
+    hipLaunchKernelGGL((k_ilu_bwd<B, T, A, T>), dim3((ng + kGPB - 1) / kGPB), dim3(64 * kGPB), 0, st, g0, g0 + ng, b.d_grp.p,
                        b.rowlist.empty() ? nullptr : b.d_rowlist.p, b.d_lev.p, b.phase_nlev[p], s.d_rowptr.p,
                        s.d_usplit.p, s.d_diaind.p, s.d_colind.p, fcrs, z, done);
   }
@@ -446,38 +440,21 @@ static hipError_t precond_apply32(const Sys& s, const float* src, float* z, cons
   return hipErrorInvalidValue;
 }
 
-template <typename T>
+template <typename T, bool REL>
 static hipError_t pcg_spmv(const Sys& s, const T* crs, const T* dia, const T* p, T* q, double* slot, unsigned& G,
-                           hipStream_t st) {
+                           const double* x64, const double* b64, T* r, double* slot_true, hipStream_t st) {
   const int tpr = spmv_tpr(s.ncrs, s.n);
   G = spmv_grid(s, tpr);
-#define DFM_SD(B, TP)                                                                                     \
-  if (s.len == B && tpr == TP) {                                                                          \
-    hipLaunchKernelGGL((k_pcg_spmv<B, TP, T>), dim3(G), dim3(NT), 0, st, s.n, s.d_rowptr.p, s.d_colind.p, \
-                       crs, dia, p, q, slot, s.d_iflag.p);                                                \
-    return hipGetLastError();                                                                             \
+#define DFM_SD(B, TP)                                                                                          \
+  if (s.len == B && tpr == TP) {                                                                               \
+    hipLaunchKernelGGL((k_pcg_spmv<B, TP, T, REL>), dim3(G), dim3(NT), 0, st, s.n, s.d_rowptr.p, s.d_colind.p, \
+                       crs, dia, p, q, slot, s.d_crs.p, s.d_dia.p, x64, b64, r, slot_true, s.d_iflag.p);       \
+    return hipGetLastError();                                                                                  \
   }
   DFM_SD(1, 4) DFM_SD(1, 8) DFM_SD(1, 16) DFM_SD(1, 32)
   DFM_SD(2, 4) DFM_SD(2, 8) DFM_SD(2, 16) DFM_SD(2, 32)
   DFM_SD(3, 4) DFM_SD(3, 8) DFM_SD(3, 16) DFM_SD(3, 32)
 #undef DFM_SD
-  return hipErrorInvalidValue;
-}
-
-static hipError_t mx_true_res(const Sys& s, const double* x, const double* b, float* r32, const double* slot_rr,
-                              int Grr, double* slot_true, unsigned& G, hipStream_t st) {
-  const int tpr = spmv_tpr(s.ncrs, s.n);
-  G = spmv_grid(s, tpr);
-#define DFM_TR(B, TP)                                                                                            \
-  if (s.len == B && tpr == TP) {                                                                                 \
-    hipLaunchKernelGGL((k_mx_true_res<B, TP>), dim3(G), dim3(NT), 0, st, s.n, s.d_rowptr.p, s.d_colind.p,       \
-                       s.d_crs.p, s.d_dia.p, x, b, r32, slot_rr, Grr, slot_true, s.d_scal.p, s.d_iflag.p);       \
-    return hipGetLastError();                                                                                    \
-  }
-  DFM_TR(1, 4) DFM_TR(1, 8) DFM_TR(1, 16) DFM_TR(1, 32)
-  DFM_TR(2, 4) DFM_TR(2, 8) DFM_TR(2, 16) DFM_TR(2, 32)
-  DFM_TR(3, 4) DFM_TR(3, 8) DFM_TR(3, 16) DFM_TR(3, 32)
-#undef DFM_TR
   return hipErrorInvalidValue;
 }
 
@@ -526,31 +503,72 @@ static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
   const long long m = (long long)s.n * s.len;
   const dim3 vg = vgrid(m);
   const int G = (int)vg.x;
-  unsigned Gs = 0, Gt = 0;
+  unsigned Gs = 0;
   int* fl = s.d_iflag.p;
   double* sc = s.d_scal.p;
   T* r = (T*)v.r;
   T* z = (T*)v.z;
   T* p = (T*)v.p;
   T* q = (T*)v.q;
-  if (pcg_spmv<T>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, st) != hipSuccess) return DFM_ERR_HIP;
   if (mixed) {
+    if (pcg_spmv<T, true>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, v.x64, v.b64, r, slot(s, P_C),
+                          st) != hipSuccess)
+      return DFM_ERR_HIP;
     hipLaunchKernelGGL((k_pcg_update<float, float>), vg, dim3(NT), 0, st, m, (float*)r, (float*)v.x, (const float*)q,
                        (const float*)p, slot(s, P_A), (int)Gs, slot(s, P_B), sc, fl);
-    hipLaunchKernelGGL(k_mx_fold, vg, dim3(NT), 0, st, m, (float*)v.x, v.x64, slot(s, P_B), G, sc, fl);
-    if (mx_true_res(s, v.x64, v.b64, (float*)r, slot(s, P_B), G, slot(s, P_C), Gt, st) != hipSuccess) return DFM_ERR_HIP;
     if (precond_apply32(s, (const float*)r, (float*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
     hipLaunchKernelGGL(k_mx_judge_rz, vg, dim3(NT), 0, st, m, (const float*)r, (const float*)z, slot(s, P_B), G,
-                       slot(s, P_C), (int)Gt, slot(s, P_D), sc, fl);
+                       slot(s, P_C), (int)Gs, slot(s, P_D), sc, fl);
+    hipLaunchKernelGGL((k_pcg_pupdate<T, true>), vg, dim3(NT), 0, st, m, p, (const T*)z, slot(s, P_D), G, sc, fl,
+                       (float*)v.x, v.x64);
   } else {
+    if (pcg_spmv<T, false>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, nullptr, nullptr, nullptr,
+                           nullptr, st) != hipSuccess)
+      return DFM_ERR_HIP;
     hipLaunchKernelGGL((k_pcg_update<double, double>), vg, dim3(NT), 0, st, m, (double*)r, (double*)v.x,
                        (const double*)q, (const double*)p, slot(s, P_A), (int)Gs, slot(s, P_B), sc, fl);
     if (precond_apply64(s, (const double*)r, (double*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
     hipLaunchKernelGGL(k_pcg_judge_rz, vg, dim3(NT), 0, st, m, (const double*)r, (const double*)z, slot(s, P_B), G,
                        slot(s, P_D), sc, fl);
+    hipLaunchKernelGGL((k_pcg_pupdate<T, false>), vg, dim3(NT), 0, st, m, p, (const T*)z, slot(s, P_D), G, sc, fl,
+                       (float*)nullptr, (double*)nullptr);
   }
-  hipLaunchKernelGGL(k_pcg_pupdate<T>, vg, dim3(NT), 0, st, m, p, (const T*)z, slot(s, P_D), G, sc, fl);
   return DFM_OK;
+}
+
+// A chunk of `k` iterations captured once as a hipGraph and replayed (the kernels take every scalar from
+// device memory, so a replay is exactly the same launches). Rebuilt when the work buffers change.
+template <typename F>
+static int graph_chunk(Sys& s, int kind, const void* key, F enqueue_iteration, hipStream_t st) {
+  if (s.gexec[kind] && s.gkey[kind] == key) return DFM_OK;
+  if (s.gexec[kind]) {
+    (void)hipGraphExecDestroy(s.gexec[kind]);
+    s.gexec[kind] = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  DFM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = DFM_OK;
+  for (int t = 0; t < s.gchunk && rc == DFM_OK; t++) rc = enqueue_iteration();
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(e);
+  e = hipGraphInstantiate(&s.gexec[kind], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  DFM_HIP(e);
+  s.gkey[kind] = key;
+  return DFM_OK;
+}
+
+// Iterations to enqueue before the next poll: from the decay rate of the residual between the last two
+// polls, the estimated number left to reach the tolerance (plus a margin), within [lo, hi].
+static unsigned next_chunk(double ratio2, double prev_ratio2, unsigned since, double tol2, unsigned lo, unsigned hi) {
+  if (!(ratio2 > 0.0) || !(prev_ratio2 > ratio2) || since == 0) return lo;
+  const double per = std::log(ratio2 / prev_ratio2) / since;  // < 0
+  const double need = std::log(tol2 / ratio2) / per;
+  double k = need * 1.05 + 4.0;
+  if (!(k > lo)) k = lo;
+  if (k > hi) k = hi;
+  return (unsigned)k;
 }
 
 template <typename T>
@@ -579,26 +597,43 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
   hipLaunchKernelGGL(k_vcopy<T>, vg, dim3(NT), 0, st, (const T*)z, p, m, fl);
   hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)z, m, slot(s, P_A), fl);
   hipLaunchKernelGGL(k_pcg_rho0, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, sc, fl);
-  unsigned launched = 0, chunk = 8;
   DFM_HIP(hipGetLastError());
+  const int gk = mixed ? 1 : 0;
+  const bool use_graph = getenv("DFM_NO_GRAPH") == nullptr;
+  if (use_graph) {
+    const int rc = graph_chunk(s, gk, v.r, [&]() { return pcg_iteration<T>(s, v, mixed, st); }, st);
+    if (rc != DFM_OK) return rc;
+  }
+  unsigned launched = 0, chunk = 32, since = 0;
+  double prev_ratio2 = 1.0;
   while (launched < maxit) {
-    const unsigned k = std::min(chunk, maxit - launched);
-    for (unsigned t = 0; t < k; t++) {
+    unsigned k = std::min(chunk, maxit - launched);
+    unsigned t = 0;
+    if (use_graph)
+      for (; t + (unsigned)s.gchunk <= k; t += s.gchunk) DFM_HIP(hipGraphLaunch(s.gexec[gk], st));
+    for (; t < k; t++) {
       const int rc = pcg_iteration<T>(s, v, mixed, st);
       if (rc != DFM_OK) return rc;
     }
     launched += k;
+    since += k;
     const int rc = poll(s, st);
     if (rc != DFM_OK) return rc;
     if (s.h_pinned[F_DONE]) break;
-    chunk = std::min(chunk * 2, 128u);
+    const double ratio2 = s.h_pinned_d[S_RR] * s.h_pinned_d[S_INV0];
+    chunk = next_chunk(ratio2, prev_ratio2, since, tol * tol, 32, 1024);
+    if (ratio2 < prev_ratio2) {
+      prev_ratio2 = ratio2;
+      since = 0;
+    }
   }
   if (launched == 0) {
     const int rc = poll(s, st);
     if (rc != DFM_OK) return rc;
   }
-  if (mixed) {
-    // the FP64 solution takes any partial solution not yet folded (a run stopped by max_iter)
+  if (mixed && s.h_pinned[F_DONE] != 1) {
+    // stopped by max_iter: the FP64 solution takes the partial solution not yet folded (when converged, the
+    // FP64 solution is the one whose true residual was confirmed)
     hipLaunchKernelGGL(k_mx_final_fold, vg, dim3(NT), 0, st, m, (float*)v.x, v.x64);
   }
   DFM_HIP(hipGetLastError());
@@ -751,8 +786,8 @@ int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes) {
   unsigned G = 0;
   auto launch = [&]() -> hipError_t {
     switch (which) {
-      case 0: return pcg_spmv<double>(s, s.d_crs.p, s.d_dia.p, s.d_rhs.p, s.d_work.p, slot(s, P_A), G, st);
-      case 1: return pcg_spmv<float>(s, s.d_crs32.p, s.d_dia32.p, x32, y32, slot(s, P_A), G, st);
+      case 0: return pcg_spmv<double, false>(s, s.d_crs.p, s.d_dia.p, s.d_rhs.p, s.d_work.p, slot(s, P_A), G, nullptr, nullptr, nullptr, nullptr, st);
+      case 1: return pcg_spmv<float, false>(s, s.d_crs32.p, s.d_dia32.p, x32, y32, slot(s, P_A), G, nullptr, nullptr, nullptr, nullptr, st);
       case 2: return precond_apply64(s, s.d_rhs.p, s.d_work.p, fl, st);
       default: return precond_apply32(s, x32, y32, fl, st);
     }
