@@ -38,11 +38,11 @@ Sys::~Sys() {
   d_order.release(); d_inv.release(); d_elsup_ptr.release(); d_elsup.release(); d_lnods.release(); d_bc.release();
   d_coords.release(); d_u.release(); d_v.release();
   d_crs.release(); d_dia.release(); d_crs32.release(); d_dia32.release();
-  d_fcrs.release(); d_fdia.release(); d_fcrs32.release(); d_fdia32.release();
+  d_fcrs.release(); d_fdia.release();
   d_res.release(); d_upd.release(); d_rhs.release();
   for (auto* t : d_tmp) { t->release(); delete t; }
   d_work.release(); d_work32.release(); d_partial.release(); d_counter.release(); d_scal.release(); d_iflag.release();
-  for (Sweep* w : {&fwd, &bwd}) { w->d_grp.release(); w->d_rowlist.release(); w->d_lev.release(); }
+  for (Sweep* w : {&fwd, &bwd}) w->release_dev();
   d_par.release(); d_lptr.release(); d_loads.release();
   for (auto& g : gexec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -70,9 +70,15 @@ static int upload_structure(Sys& s) {
   DFM_HIP(upload(s.d_order, s.order, st));
   DFM_HIP(upload(s.d_inv, s.inv, st));
   for (Sweep* w : {&s.fwd, &s.bwd}) {
+    w->release_dev();
     DFM_HIP(upload(w->d_grp, w->grp, st));
     DFM_HIP(upload(w->d_rowlist, w->rowlist, st));
     DFM_HIP(upload(w->d_lev, w->lev, st));
+    DFM_HIP(upload(w->d_gmeta, w->gmeta, st));
+    DFM_HIP(upload(w->d_srow, w->srow, st));
+    DFM_HIP(upload(w->d_scnt, w->scnt, st));
+    DFM_HIP(upload(w->d_ecol, w->ecol, st));
+    DFM_HIP(upload(w->d_esrc, w->esrc, st));
   }
   return DFM_OK;
 }
@@ -628,6 +634,18 @@ int dfm_prec_factor(dfm_sys sys) {
   DFM_HIP(hipStreamSynchronize(st));
   nsing = s.h_pinned[8];
   s.factored32 = false;
+  // slab copies of the factor for the sweep kernels
+  for (int dir = 0; dir < 2; dir++) {
+    Sweep& w = dir == 0 ? s.fwd : s.bwd;
+    const long long ne = w.nslab * 64;
+    if (w.d_eval.n < (size_t)(ne * s.bs)) DFM_HIP(w.d_eval.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
+    DFM_HIP(launch_slab_fill(s.len, ne, w.d_esrc.p, s.d_fcrs.p, w.d_eval.p, st));
+    if (dir == 0) {
+      const long long ns = (long long)w.srow.size();
+      if (w.d_dval.n < (size_t)(ns * s.bs)) DFM_HIP(w.d_dval.alloc((size_t)ns * s.bs));
+      DFM_HIP(launch_slab_fill(s.len, ns, (const unsigned*)w.d_srow.p, s.d_fdia.p, w.d_dval.p, st));
+    }
+  }
   if (nsing > 10) {
     s.factored = false;
     set_error("dfm_prec_factor: more than 10 singular pivots (DoILUDecomp returns false)");
@@ -673,12 +691,19 @@ static int ensure_fp32(Sys& s) {
   hipStream_t st = s.ctx->stream;
   if (!s.d_crs32.p) DFM_HIP(s.d_crs32.alloc((size_t)s.ncrs * s.bs));
   if (!s.d_dia32.p) DFM_HIP(s.d_dia32.alloc((size_t)s.n * s.bs));
-  if (!s.d_fcrs32.p) DFM_HIP(s.d_fcrs32.alloc((size_t)s.ncrs * s.bs));
-  if (!s.d_fdia32.p) DFM_HIP(s.d_fdia32.alloc((size_t)s.n * s.bs));
   DFM_HIP(launch_to_f32(s.d_crs.p, s.d_crs32.p, (long long)s.ncrs * s.bs, st));
   DFM_HIP(launch_to_f32(s.d_dia.p, s.d_dia32.p, (long long)s.n * s.bs, st));
-  DFM_HIP(launch_to_f32(s.d_fcrs.p, s.d_fcrs32.p, (long long)s.ncrs * s.bs, st));
-  DFM_HIP(launch_to_f32(s.d_fdia.p, s.d_fdia32.p, (long long)s.n * s.bs, st));
+  for (int dir = 0; dir < 2; dir++) {
+    Sweep& w = dir == 0 ? s.fwd : s.bwd;
+    const long long ne = w.nslab * 64;
+    if (w.d_eval32.n < (size_t)(ne * s.bs)) DFM_HIP(w.d_eval32.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
+    DFM_HIP(launch_slab_fill(s.len, ne, w.d_esrc.p, s.d_fcrs.p, w.d_eval32.p, st));
+    if (dir == 0) {
+      const long long ns = (long long)w.srow.size();
+      if (w.d_dval32.n < (size_t)(ns * s.bs)) DFM_HIP(w.d_dval32.alloc((size_t)ns * s.bs));
+      DFM_HIP(launch_slab_fill(s.len, ns, (const unsigned*)w.d_srow.p, s.d_fdia.p, w.d_dval32.p, st));
+    }
+  }
   s.factored32 = true;
   return DFM_OK;
 }

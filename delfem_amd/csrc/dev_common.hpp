@@ -121,12 +121,15 @@ __device__ inline void dia_row(const T* __restrict__ dia, const VX* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// ILU(0) sweeps. Forward (ForwardSubstitution, src/matvec/matdiafrac_blkcrs.cpp:332-507):
+// ILU(0) sweeps over the slab layout. Forward (ForwardSubstitution, src/matvec/matdiafrac_blkcrs.cpp:332-507):
 //   t = b_i - sum_{j<i} L_ij x_j (ascending j), x_i = D_i^-1 t.
-// Backward (BackwardSubstitution, :509-669): x_i -= sum_{j>i} U_ij x_j (D^-1 already folded into U).
-// One block of 64 threads per group; lane = row. Entries [rp, lsplit) / [usplit, rp+1) reference rows of
-// earlier phases (final in global memory); [lsplit, di) / [di, usplit) reference rows of the same group,
-// resolved level by level through LDS.
+// Backward (BackwardSubstitution, :509-669): x_i -= sum_{j>i} U_ij x_j (ascending j; D^-1 folded into U).
+// One wavefront per group, lane = row, WPB independent wavefronts per workgroup. Entries referencing rows of
+// earlier phases are final in global memory: their columns, values and gathered x are loaded before the level
+// loop. In-group entries are resolved level by level through the wavefront's own LDS row, with only a
+// wavefront-scope fence between levels. Colour-major orderings put the outside entries of a forward row
+// below its in-group ones and those of a backward row above them, so each class is applied in the position
+// the ascending order gives it: the sweeps reproduce the reference's floating-point order exactly.
 
 template <int B, typename T, typename A>
 __device__ inline void blk_sub(A (&t)[B], const T* __restrict__ a, const A (&v)[B]) {
@@ -139,104 +142,182 @@ __device__ inline void blk_sub(A (&t)[B], const T* __restrict__ a, const A (&v)[
   }
 }
 
-// MAXIN in-group entries per row are loaded into registers before the level loop, so that a level costs
-// LDS reads and arithmetic only; rows with more in-group entries read the rest from global memory.
-template <int B>
-struct InGroup {
-  static constexpr int MAXIN = (B == 2) ? 4 : (B == 3 ? 7 : 8);
+// t -= a v for a block held in registers (row-major b x b)
+template <int B, typename A>
+__device__ inline void blk_sub_reg(A (&t)[B], const A (&a)[B * B], const A (&v)[B]) {
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    A s = a[r * B] * v[0];
+#pragma unroll
+    for (int c = 1; c < B; c++) s += a[r * B + c] * v[c];
+    t[r] -= s;
+  }
+}
+
+// Entries per row and class held in registers from the start of the kernel; further entries (rare) are read
+// from the slab when they are applied.
+template <int B, typename A>
+struct SlabCap {
+  static constexpr int OUT = B == 1 ? 4 : (B == 2 ? 6 : (sizeof(A) == 8 ? 4 : 6));
+  static constexpr int IN = B == 1 ? 4 : (B == 2 ? 6 : (sizeof(A) == 8 ? 5 : 7));
 };
 
-// GPB groups per workgroup, one wavefront per group: few, larger workgroups keep every group of a phase
-// resident at once (the per-CU workgroup limit, not registers, bounds residency of one-wave workgroups).
-constexpr int kGPB = 4;
+constexpr int kWPB = 4;  // wavefronts (groups) per workgroup of the sweep kernels
 
-template <int B, typename T, typename A, typename VI, typename VO>
-__global__ void __launch_bounds__(64 * kGPB) k_ilu_fwd(int gbase, int gend, const int* __restrict__ grp, const int* __restrict__ rowlist,
-                                                const uint8_t* __restrict__ lev, int nlev,
-                                                const unsigned* __restrict__ rp, const unsigned* __restrict__ lsplit,
-                                                const unsigned* __restrict__ di, const unsigned* __restrict__ ci,
-                                                const T* __restrict__ fcrs, const T* __restrict__ fdia,
-                                                const VI* b, VO* x, const int* __restrict__ done) {
-  constexpr int BS = B * B;
-  constexpr int MAXIN = InGroup<B>::MAXIN;
-  if (done && *done) return;
-  const int wv = threadIdx.x / 64;
-  const int g = gbase + blockIdx.x * kGPB + wv;
-  const bool gok = g < gend;
-  const int start = gok ? grp[2 * g] : 0, end = gok ? grp[2 * g + 1] : 0;
-  const int lane = threadIdx.x % 64;
-  const bool active = lane < end - start;
-  const int row = active ? (rowlist ? rowlist[start + lane] : start + lane) : 0;
-  __shared__ A xsb[kGPB][64 * B];
-  A* xs = xsb[wv];
-  A t[B], D[BS];
-  int jl[MAXIN];
-  A lv[MAXIN][BS];
-  unsigned ls = 0, d = 0;
-  if (active) {
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct SlabArgs {
+  const int* gmeta;        // 4 ints per group
+  const int* srow;         // row per slot (indirect groups)
+  const unsigned* scnt;    // nout | nin << 8 | level << 16 per slot
+  const unsigned* ecol;    // per entry
+};
+
+// Per-slot state common to both sweeps. The group descriptor is wave-uniform (scalar loads); every vector load
+// below depends only on the slot, so all of them are in flight together. Padding entries of a slab row hold
+// column 0 and value 0 and empty lanes map to row 0: loads are unconditional under the wave-uniform widths,
+// only the arithmetic is predicated by the lane's own counts.
+struct SlabSlot {
+  int g, lane, off, wo, wi, nlev, row, no, ni, lv;
+  bool active;
+};
+
+__device__ inline bool slab_slot(const SlabArgs& sa, int gbase, int gend, const int* done, SlabSlot& ss) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  ss.g = gbase + (int)blockIdx.x * kWPB + wave;
+  if (ss.g >= gend) return false;
+  const int* m = sa.gmeta + 4 * (size_t)ss.g;
+  const int m0 = m[0], m1 = m[1], row0 = m[2], nrows = m[3];
+  if (done && *done) return false;
+  ss.off = m0;
+  ss.wo = m1 & 255;
+  ss.wi = (m1 >> 8) & 255;
+  ss.nlev = m1 >> 16;
+  ss.lane = (int)(threadIdx.x % 64);
+  ss.active = ss.lane < nrows;
+  const size_t slot = (size_t)ss.g * 64 + ss.lane;
+  ss.row = row0 >= 0 ? (ss.active ? row0 + ss.lane : row0) : (ss.active ? sa.srow[slot] : 0);
+  const unsigned c = sa.scnt[slot];
+  ss.no = (int)(c & 255u);
+  ss.ni = (int)((c >> 8) & 255u);
+  ss.lv = ss.active ? (int)(c >> 16) : -1;
+  return true;
+}
+
+template <int B, typename T, typename A>
+__device__ inline void slab_load_blk(const T* __restrict__ val, size_t srow_, int lane, A (&a)[B * B]) {
 #pragma unroll
-    for (int r = 0; r < B; r++) t[r] = (A)b[(size_t)row * B + r];
-    ls = lsplit[row];
-    d = di[row];
-    // in-group entries and the diagonal block first (independent of the sweep), then rows of earlier phases
+  for (int e = 0; e < B * B; e++) a[e] = (A)val[(srow_ * (B * B) + e) * 64 + lane];
+}
+
+// Loads of the register-held entries of one class: MX slab rows starting at r0, wave-uniform count w.
+template <int B, int MX, typename T, typename A>
+__device__ inline void slab_load_class(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int w,
+                                       int lane, unsigned (&col)[MX], A (&v)[MX][B * B]) {
 #pragma unroll
-    for (int k = 0; k < MAXIN; k++) {
-      if (ls + k < d) {
-        jl[k] = (int)ci[ls + k] - start;
-#pragma unroll
-        for (int e = 0; e < BS; e++) lv[k][e] = (A)fcrs[(size_t)(ls + k) * BS + e];
-      }
+  for (int k = 0; k < MX; k++)
+    if (k < w) {
+      col[k] = ecol[(size_t)(r0 + k) * 64 + lane];
+      slab_load_blk<B, T, A>(val, (size_t)(r0 + k), lane, v[k]);
     }
+}
+
+template <int B, int MX, typename A, typename VX>
+__device__ inline void slab_gather(const VX* x, int w, const unsigned (&col)[MX], A (&xo)[MX][B]) {
 #pragma unroll
-    for (int e = 0; e < BS; e++) D[e] = (A)fdia[(size_t)row * BS + e];
-    // rows of earlier phases: all column indices, then all gathers, then the products in ascending order
-    const unsigned q0 = rp[row];
-    unsigned jo[MAXIN];
+  for (int k = 0; k < MX; k++)
+    if (k < w)
 #pragma unroll
-    for (int k = 0; k < MAXIN; k++)
-      if (q0 + k < ls) jo[k] = ci[q0 + k];
-    A xo[MAXIN][B];
+      for (int c = 0; c < B; c++) xo[k][c] = (A)x[(size_t)col[k] * B + c];
+}
+
+// t -= sum over entries [MX, n) of the class starting at slab row r0, values x from global memory (final)
+template <int B, int MX, typename T, typename A, typename VX>
+__device__ inline void slab_tail_global(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int n,
+                                        int lane, const VX* x, A (&t)[B]) {
+  for (int k = MX; k < n; k++) {
+    const size_t sr = (size_t)(r0 + k);
+    const unsigned j = ecol[sr * 64 + lane];
+    A a[B * B], v[B];
+    slab_load_blk<B, T, A>(val, sr, lane, a);
 #pragma unroll
-    for (int k = 0; k < MAXIN; k++)
-      if (q0 + k < ls)
-#pragma unroll
-        for (int c = 0; c < B; c++) xo[k][c] = (A)x[(size_t)jo[k] * B + c];
-#pragma unroll
-    for (int k = 0; k < MAXIN; k++)
-      if (q0 + k < ls) blk_sub<B>(t, fcrs + (size_t)(q0 + k) * BS, xo[k]);
-    for (unsigned q = q0 + MAXIN; q < ls; q++) {
-      const unsigned j = ci[q];
-      A v[B];
-#pragma unroll
-      for (int c = 0; c < B; c++) v[c] = (A)x[(size_t)j * B + c];
-      blk_sub<B>(t, fcrs + (size_t)q * BS, v);
-    }
+    for (int c = 0; c < B; c++) v[c] = (A)x[(size_t)j * B + c];
+    blk_sub_reg<B, A>(t, a, v);
   }
-  const int my = active ? (int)lev[row] : -1;
-  for (int l = 0; l < nlev; l++) {
-    if (my == l) {
+}
+
+// t -= v y where the lane's own count says the entry exists (k < n); a select, not a branch, so the
+// wavefront never splits around the loads that feed it.
+template <int B, typename A>
+__device__ inline void blk_sub_if(bool on, A (&t)[B], const A (&a)[B * B], const A (&v)[B]) {
 #pragma unroll
-      for (int k = 0; k < MAXIN; k++) {
-        if (ls + k < d) {
-          A v[B];
+  for (int r = 0; r < B; r++) {
+    A s = a[r * B] * v[0];
 #pragma unroll
-          for (int c = 0; c < B; c++) v[c] = xs[jl[k] * B + c];
+    for (int c = 1; c < B; c++) s += a[r * B + c] * v[c];
+    const A u = t[r] - s;
+    t[r] = on ? u : t[r];
+  }
+}
+
+// t -= sum over the in-group entries of the lane (values of this wavefront's rows from LDS), ascending. All
+// reads of the wave-uniform width w are issued together (padding columns point at lane 0).
+template <int B, int MX, typename T, typename A>
+__device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int w, int n,
+                                     int lane, const unsigned (&col)[MX], const A (&v)[MX][B * B], const A* xs,
+                                     A (&t)[B]) {
+  A y[MX][B];
 #pragma unroll
-          for (int r = 0; r < B; r++) {
-            A s = lv[k][r * B] * v[0];
+  for (int k = 0; k < MX; k++)
+    if (k < w)
 #pragma unroll
-            for (int c = 1; c < B; c++) s += lv[k][r * B + c] * v[c];
-            t[r] -= s;
-          }
-        }
-      }
-      for (unsigned q = ls + MAXIN; q < d; q++) {
-        const int j = (int)ci[q] - start;
-        A v[B];
+      for (int c = 0; c < B; c++) y[k][c] = xs[col[k] * B + c];
 #pragma unroll
-        for (int c = 0; c < B; c++) v[c] = xs[j * B + c];
-        blk_sub<B>(t, fcrs + (size_t)q * BS, v);
-      }
+  for (int k = 0; k < MX; k++)
+    if (k < w) blk_sub_if<B, A>(k < n, t, v[k], y[k]);
+  for (int k = MX; k < n; k++) {
+    const size_t sr = (size_t)(r0 + k);
+    const unsigned j = ecol[sr * 64 + lane];
+    A a[B * B], z[B];
+    slab_load_blk<B, T, A>(val, sr, lane, a);
+#pragma unroll
+    for (int c = 0; c < B; c++) z[c] = xs[j * B + c];
+    blk_sub_reg<B, A>(t, a, z);
+  }
+}
+
+// Forward sweep of one phase: x = D^-1 (b - L x) for the phase's rows. b and x may alias.
+template <int B, typename T, typename A, typename VI, typename VO>
+__global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
+                                                      const T* __restrict__ dval, const VI* b, VO* x,
+                                                      const int* __restrict__ done) {
+  constexpr int BS = B * B;
+  constexpr int MO = SlabCap<B, A>::OUT, MI = SlabCap<B, A>::IN;
+  __shared__ A xsb[kWPB][64 * B];
+  SlabSlot ss;
+  if (!slab_slot(sa, gbase, gend, done, ss)) return;
+  A* xs = xsb[threadIdx.x / 64];
+  A t[B], D[BS];
+  A vo[MO][BS], xo[MO][B], vi[MI][BS];
+  unsigned jo[MO], li[MI];
+#pragma unroll
+  for (int r = 0; r < B; r++) t[r] = (A)b[(size_t)ss.row * B + r];
+#pragma unroll
+  for (int e = 0; e < BS; e++) D[e] = (A)dval[((size_t)ss.g * BS + e) * 64 + ss.lane];
+  slab_load_class<B, MO, T, A>(sa.ecol, val, ss.off, ss.wo, ss.lane, jo, vo);
+  slab_load_class<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.lane, li, vi);
+  slab_gather<B, MO, A, VO>(x, ss.wo, jo, xo);
+#pragma unroll
+  for (int k = 0; k < MO; k++)
+    if (k < ss.wo) blk_sub_if<B, A>(k < ss.no, t, vo[k], xo[k]);
+  slab_tail_global<B, MO, T, A, VO>(sa.ecol, val, ss.off, ss.no, ss.lane, x, t);
+  for (int l = 0; l < ss.nlev; l++) {
+    if (ss.lv == l) {
+      slab_in_group<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.ni, ss.lane, li, vi, xs, t);
       A y[B];
 #pragma unroll
       for (int r = 0; r < B; r++) {
@@ -247,110 +328,50 @@ __global__ void __launch_bounds__(64 * kGPB) k_ilu_fwd(int gbase, int gend, cons
       }
 #pragma unroll
       for (int r = 0; r < B; r++) {
-        xs[lane * B + r] = y[r];
         t[r] = y[r];
+        xs[ss.lane * B + r] = y[r];
       }
     }
-    // only LDS traffic inside the level loop: the barrier's fence then has no global store to wait for
-    if (nlev > 1) __syncthreads();
+    if (ss.nlev > 1) wave_lds_sync();
   }
-  if (active)
+  if (ss.active)
 #pragma unroll
-    for (int r = 0; r < B; r++) x[(size_t)row * B + r] = (VO)t[r];
+    for (int r = 0; r < B; r++) x[(size_t)ss.row * B + r] = (VO)t[r];
 }
 
+// Backward sweep of one phase, in place on x: x_i -= U_in x_in (in-group, ascending) then U_out x_out.
 template <int B, typename T, typename A, typename VX>
-__global__ void __launch_bounds__(64 * kGPB) k_ilu_bwd(int gbase, int gend, const int* __restrict__ grp, const int* __restrict__ rowlist,
-                                                const uint8_t* __restrict__ lev, int nlev,
-                                                const unsigned* __restrict__ rp, const unsigned* __restrict__ usplit,
-                                                const unsigned* __restrict__ di, const unsigned* __restrict__ ci,
-                                                const T* __restrict__ fcrs, VX* x, const int* __restrict__ done) {
+__global__ void __launch_bounds__(64 * kWPB) k_sweep_bwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
+                                                      VX* x, const int* __restrict__ done) {
   constexpr int BS = B * B;
-  constexpr int MAXIN = InGroup<B>::MAXIN;
-  if (done && *done) return;
-  const int wv = threadIdx.x / 64;
-  const int g = gbase + blockIdx.x * kGPB + wv;
-  const bool gok = g < gend;
-  const int start = gok ? grp[2 * g] : 0, end = gok ? grp[2 * g + 1] : 0;
-  const int lane = threadIdx.x % 64;
-  const bool active = lane < end - start;
-  const int row = active ? (rowlist ? rowlist[start + lane] : start + lane) : 0;
-  __shared__ A xsb[kGPB][64 * B];
-  A* xs = xsb[wv];
+  constexpr int MO = SlabCap<B, A>::OUT, MI = SlabCap<B, A>::IN;
+  __shared__ A xsb[kWPB][64 * B];
+  SlabSlot ss;
+  if (!slab_slot(sa, gbase, gend, done, ss)) return;
+  A* xs = xsb[threadIdx.x / 64];
   A t[B];
-  int jl[MAXIN];
-  A uv[MAXIN][BS];
-  unsigned d = 0, us = 0;
-  if (active) {
+  A vo[MO][BS], xo[MO][B], vi[MI][BS];
+  unsigned jo[MO], li[MI];
 #pragma unroll
-    for (int r = 0; r < B; r++) t[r] = (A)x[(size_t)row * B + r];
-    d = di[row];
-    us = usplit[row];
-    const unsigned qe = rp[row + 1];
+  for (int r = 0; r < B; r++) t[r] = (A)x[(size_t)ss.row * B + r];
+  slab_load_class<B, MO, T, A>(sa.ecol, val, ss.off, ss.wo, ss.lane, jo, vo);
+  slab_load_class<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.lane, li, vi);
+  slab_gather<B, MO, A, VX>(x, ss.wo, jo, xo);
+  for (int l = 0; l < ss.nlev; l++) {
+    if (ss.lv == l) {
+      slab_in_group<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.ni, ss.lane, li, vi, xs, t);
 #pragma unroll
-    for (int k = 0; k < MAXIN; k++) {
-      if (d + k < us) {
-        jl[k] = (int)ci[d + k] - start;
+      for (int k = 0; k < MO; k++)
+        if (k < ss.wo) blk_sub_if<B, A>(k < ss.no, t, vo[k], xo[k]);
+      slab_tail_global<B, MO, T, A, VX>(sa.ecol, val, ss.off, ss.no, ss.lane, x, t);
 #pragma unroll
-        for (int e = 0; e < BS; e++) uv[k][e] = (A)fcrs[(size_t)(d + k) * BS + e];
-      }
+      for (int r = 0; r < B; r++) xs[ss.lane * B + r] = t[r];
     }
-    // rows of later phases first (with no intra-group entries this is the reference's ascending order):
-    // all column indices, then all gathers, then the products
-    unsigned jo[MAXIN];
-#pragma unroll
-    for (int k = 0; k < MAXIN; k++)
-      if (us + k < qe) jo[k] = ci[us + k];
-    A xo[MAXIN][B];
-#pragma unroll
-    for (int k = 0; k < MAXIN; k++)
-      if (us + k < qe)
-#pragma unroll
-        for (int c = 0; c < B; c++) xo[k][c] = (A)x[(size_t)jo[k] * B + c];
-#pragma unroll
-    for (int k = 0; k < MAXIN; k++)
-      if (us + k < qe) blk_sub<B>(t, fcrs + (size_t)(us + k) * BS, xo[k]);
-    for (unsigned q = us + MAXIN; q < qe; q++) {
-      const unsigned j = ci[q];
-      A v[B];
-#pragma unroll
-      for (int c = 0; c < B; c++) v[c] = (A)x[(size_t)j * B + c];
-      blk_sub<B>(t, fcrs + (size_t)q * BS, v);
-    }
+    if (ss.nlev > 1) wave_lds_sync();
   }
-  const int my = active ? (int)lev[row] : -1;
-  for (int l = 0; l < nlev; l++) {
-    if (my == l) {
+  if (ss.active)
 #pragma unroll
-      for (int k = 0; k < MAXIN; k++) {
-        if (d + k < us) {
-          A v[B];
-#pragma unroll
-          for (int c = 0; c < B; c++) v[c] = xs[jl[k] * B + c];
-#pragma unroll
-          for (int r = 0; r < B; r++) {
-            A s = uv[k][r * B] * v[0];
-#pragma unroll
-            for (int c = 1; c < B; c++) s += uv[k][r * B + c] * v[c];
-            t[r] -= s;
-          }
-        }
-      }
-      for (unsigned q = d + MAXIN; q < us; q++) {
-        const int j = (int)ci[q] - start;
-        A v[B];
-#pragma unroll
-        for (int c = 0; c < B; c++) v[c] = xs[j * B + c];
-        blk_sub<B>(t, fcrs + (size_t)q * BS, v);
-      }
-#pragma unroll
-      for (int r = 0; r < B; r++) xs[lane * B + r] = t[r];
-    }
-    if (nlev > 1) __syncthreads();
-  }
-  if (active)
-#pragma unroll
-    for (int r = 0; r < B; r++) x[(size_t)row * B + r] = (VX)t[r];
+    for (int r = 0; r < B; r++) x[(size_t)ss.row * B + r] = (VX)t[r];
 }
 
 }  // namespace dfm

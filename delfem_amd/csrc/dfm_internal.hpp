@@ -75,6 +75,28 @@ struct Sweep {
   DBuf<int> d_grp, d_rowlist;
   DBuf<uint8_t> d_lev;
   int nphase() const { return (int)phase_nlev.size(); }
+
+  // Slab layout of the factor for the sweep kernels (one 64-lane slot per row, lane = position in its group).
+  // Group k (execution order) owns slab rows [off, off + wo + wi): first wo rows of entries referencing rows of
+  // earlier phases ("outside", in ascending column order per lane), then wi rows of in-group entries (ascending;
+  // the column is stored as the source lane). Entry (slab row r, lane l) sits at index r*64 + l; its b*b values
+  // are stored element-major: value e at (r*bs + e)*64 + l, so every load instruction of a wavefront is one
+  // contiguous 64-element segment.
+  std::vector<int> gmeta;        // [4*ngroup] off, wo | wi << 8 | nlev << 16, first row (contiguous) or -1, nrows
+  std::vector<int> srow;         // [64*ngroup] device row of each slot, -1 for an empty lane
+  std::vector<unsigned> scnt;    // [64*ngroup] nout | nin << 8 | level << 16
+  std::vector<unsigned> ecol;    // [64*nslab] column (outside) or source lane (in-group)
+  std::vector<unsigned> esrc;    // [64*nslab] factor block index of the entry, ~0u for padding
+  long long nslab = 0;
+  DBuf<int> d_gmeta, d_srow;
+  DBuf<unsigned> d_scnt, d_ecol, d_esrc;
+  DBuf<double> d_eval, d_dval;   // FP64 values (d_dval: forward sweep only, D^-1 per slot)
+  DBuf<float> d_eval32, d_dval32;
+  void release_dev() {
+    d_grp.release(); d_rowlist.release(); d_lev.release();
+    d_gmeta.release(); d_srow.release(); d_scnt.release(); d_ecol.release(); d_esrc.release();
+    d_eval.release(); d_dval.release(); d_eval32.release(); d_dval32.release();
+  }
 };
 
 struct Sys {
@@ -107,7 +129,6 @@ struct Sys {
   DBuf<double> d_crs, d_dia;         // matrix, device numbering, FP64
   DBuf<float> d_crs32, d_dia32;      // FP32 copy (mixed precision)
   DBuf<double> d_fcrs, d_fdia;       // ILU(0) factor FP64
-  DBuf<float> d_fcrs32, d_fdia32;    // ILU(0) factor FP32
   DBuf<double> d_res, d_upd, d_rhs;  // handles -1, -2 and a copy of the assembled right-hand side
   std::vector<DBuf<double>*> d_tmp;  // handles 0..
   DBuf<double> d_work;               // solver work vectors
@@ -143,7 +164,7 @@ struct Sys {
 void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vector<unsigned>& rowptr,
                    std::vector<unsigned>& colind, std::vector<int>& elsup_ptr, std::vector<int>& elsup);
 void build_device_pattern(Sys& s);  // from order/inv: rowptr, colind, diaind, perm_crs
-void build_schedule(Sys& s);        // fwd/bwd sweeps, lsplit/usplit
+void build_schedule(Sys& s);        // fwd/bwd sweeps, lsplit/usplit, slabs
 void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors);
 void make_blockcolor(const Sys& s, int tile, std::vector<int>& order, std::vector<int>& color_of_new,
                      std::vector<int>& tile_of_new, int& ncolors);

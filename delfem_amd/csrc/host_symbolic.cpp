@@ -214,9 +214,70 @@ static void group_sweeps(Sys& s, const std::vector<int>& gstart, const std::vect
   }
 }
 
+// Slab layout of one sweep (see Sweep): per group, the entries of each row split into "outside" (rows of
+// earlier phases, final before the phase starts) and "in-group" (resolved level by level inside the wavefront),
+// each class in ascending column order as the reference's substitution loops visit them.
+static void build_slabs(const Sys& s, Sweep& w, bool forward) {
+  const int ngroup = (int)w.grp.size() / 2;
+  const bool indirect = !w.rowlist.empty();
+  w.gmeta.assign((size_t)4 * ngroup, 0);
+  w.srow.assign((size_t)kGroup * ngroup, -1);
+  w.scnt.assign((size_t)kGroup * ngroup, 0u);
+  w.ecol.clear();
+  w.esrc.clear();
+  long long off = 0;
+  for (int g = 0; g < ngroup; g++) {
+    const int a = w.grp[2 * g], e = w.grp[2 * g + 1], nrows = e - a;
+    if (nrows < 1 || nrows > kGroup) throw std::runtime_error("schedule: bad group size");
+    int wo = 0, wi = 0, nlev = 1;
+    // per lane: [o0, o1) outside entries, [i0, i1) in-group entries
+    unsigned o0[kGroup], o1[kGroup], i0[kGroup], i1[kGroup];
+    for (int l = 0; l < nrows; l++) {
+      const int row = indirect ? w.rowlist[a + l] : a + l;
+      if (forward) {
+        o0[l] = s.rowptr[row]; o1[l] = s.lsplit[row]; i0[l] = s.lsplit[row]; i1[l] = s.diaind[row];
+      } else {
+        i0[l] = s.diaind[row]; i1[l] = s.usplit[row]; o0[l] = s.usplit[row]; o1[l] = s.rowptr[row + 1];
+      }
+      const int no = (int)(o1[l] - o0[l]), ni = (int)(i1[l] - i0[l]);
+      if (no > 255 || ni > 255) throw std::runtime_error("schedule: more than 255 entries per row and class");
+      if (ni && indirect) throw std::runtime_error("schedule: in-group entries in an indirect group");
+      wo = std::max(wo, no);
+      wi = std::max(wi, ni);
+      nlev = std::max(nlev, (int)w.lev[row] + 1);
+      w.srow[(size_t)g * kGroup + l] = row;
+      w.scnt[(size_t)g * kGroup + l] = (unsigned)no | ((unsigned)ni << 8) | ((unsigned)w.lev[row] << 16);
+    }
+    if (off > INT32_MAX - 512) throw std::runtime_error("schedule: slab too large");
+    w.gmeta[4 * g + 0] = (int)off;
+    w.gmeta[4 * g + 1] = wo | (wi << 8) | (nlev << 16);
+    w.gmeta[4 * g + 2] = indirect ? -1 : a;
+    w.gmeta[4 * g + 3] = nrows;
+    const size_t base = w.ecol.size();
+    w.ecol.resize(base + (size_t)(wo + wi) * kGroup, 0u);
+    w.esrc.resize(base + (size_t)(wo + wi) * kGroup, ~0u);
+    for (int l = 0; l < nrows; l++) {
+      for (unsigned q = o0[l]; q < o1[l]; q++) {
+        const size_t k = base + (size_t)(q - o0[l]) * kGroup + l;
+        w.ecol[k] = s.colind[q];
+        w.esrc[k] = q;
+      }
+      for (unsigned q = i0[l]; q < i1[l]; q++) {
+        const size_t k = base + (size_t)(wo + (q - i0[l])) * kGroup + l;
+        w.ecol[k] = s.colind[q] - (unsigned)a;  // contiguous group: lane of the source row
+        w.esrc[k] = q;
+      }
+    }
+    off += wo + wi;
+  }
+  w.nslab = off;
+}
+
 void build_schedule(Sys& s) {
   if (s.color_of.empty()) {
     level_sweeps(s);
+    build_slabs(s, s.fwd, true);
+    build_slabs(s, s.bwd, false);
     return;
   }
   // contiguous groups: tiles (BMC) or 64-row chunks of each colour (MC)
@@ -234,6 +295,8 @@ void build_schedule(Sys& s) {
     i = j;
   }
   group_sweeps(s, gstart, gcolor, s.ncolors);
+  build_slabs(s, s.fwd, true);
+  build_slabs(s, s.bwd, false);
 }
 
 // Greedy node colouring in natural order; order = colour-major, natural inside a colour.

@@ -265,6 +265,35 @@ __global__ void __launch_bounds__(64) k_ilu_factor(int gbase, const int* __restr
   }
 }
 
+template <int B, typename T>
+__global__ void __launch_bounds__(256) k_slab_fill(long long nent, const unsigned* __restrict__ src,
+                                                   const double* __restrict__ f, T* __restrict__ val) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nent) return;
+  const long long r = i / 64;
+  const int lane = (int)(i % 64);
+  const unsigned q = src[i];
+#pragma unroll
+  for (int e = 0; e < B * B; e++) val[(r * (B * B) + e) * 64 + lane] = q != ~0u ? (T)f[(size_t)q * (B * B) + e] : T(0);
+}
+
+template <typename T>
+static hipError_t slab_fill_T(int len, long long nent, const unsigned* src, const double* f, T* val, hipStream_t st) {
+  if (nent <= 0) return hipSuccess;
+  const dim3 g((unsigned)((nent + 255) / 256));
+  if (len == 1) hipLaunchKernelGGL((k_slab_fill<1, T>), g, dim3(256), 0, st, nent, src, f, val);
+  else if (len == 2) hipLaunchKernelGGL((k_slab_fill<2, T>), g, dim3(256), 0, st, nent, src, f, val);
+  else if (len == 3) hipLaunchKernelGGL((k_slab_fill<3, T>), g, dim3(256), 0, st, nent, src, f, val);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t launch_slab_fill(int len, long long nent, const unsigned* src, const double* f, double* val, hipStream_t st) {
+  return slab_fill_T<double>(len, nent, src, f, val, st);
+}
+hipError_t launch_slab_fill(int len, long long nent, const unsigned* src, const double* f, float* val, hipStream_t st) {
+  return slab_fill_T<float>(len, nent, src, f, val, st);
+}
+
 hipError_t launch_ilu_factor(int n, int len, const unsigned* rowptr, const unsigned* colind, const unsigned* diaind,
                              double* fcrs, double* fdia, const SweepDev& fw, int* nsing, hipStream_t st) {
   (void)n;

@@ -63,5 +63,9 @@ struct SweepDev {
 hipError_t launch_ilu_factor(int n, int len, const unsigned* rowptr, const unsigned* colind,
                              const unsigned* diaind, double* fcrs, double* fdia, const SweepDev& fwd,
                              int* nsing, hipStream_t st);
+// Slab values of a sweep from the FP64 factor: val[(r*bs + e)*64 + l] = f[src[r*64 + l]*bs + e], 0 where
+// src = ~0u. nent = 64 * slab rows. T = double or float.
+hipError_t launch_slab_fill(int len, long long nent, const unsigned* src, const double* f, double* val, hipStream_t st);
+hipError_t launch_slab_fill(int len, long long nent, const unsigned* src, const double* f, float* val, hipStream_t st);
 // Krylov device loops (solver.hip) use the kernels below through solver-specific launchers.
 }  // namespace dfm

@@ -405,38 +405,40 @@ static unsigned spmv_grid(const Sys& s, int tpr) {
 }
 
 template <int B, typename T, typename A>
-static hipError_t precond_T(const Sys& s, const T* fcrs, const T* fdia, const T* src, T* z, const int* done,
-                            hipStream_t st) {
+static hipError_t precond_T(const Sys& s, bool fp32, const T* src, T* z, const int* done, hipStream_t st) {
   const Sweep& f = s.fwd;
+  const SlabArgs fa{f.d_gmeta.p, f.d_srow.p, f.d_scnt.p, f.d_ecol.p};
+  const T* fval = fp32 ? (const T*)f.d_eval32.p : (const T*)f.d_eval.p;
+  const T* fdval = fp32 ? (const T*)f.d_dval32.p : (const T*)f.d_dval.p;
   for (int p = 0; p < f.nphase(); p++) {
     const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_ilu_fwd<B, T, A, T, T>), dim3((ng + kGPB - 1) / kGPB), dim3(64 * kGPB), 0, st, g0, g0 + ng, f.d_grp.p,
-                       f.rowlist.empty() ? nullptr : f.d_rowlist.p, f.d_lev.p, f.phase_nlev[p], s.d_rowptr.p,
-                       s.d_lsplit.p, s.d_diaind.p, s.d_colind.p, fcrs, fdia, src, z, done);
+    hipLaunchKernelGGL((k_sweep_fwd<B, T, A, T, T>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
+                       fa, fval, fdval, src, z, done);
   }
   const Sweep& b = s.bwd;
+  const SlabArgs ba{b.d_gmeta.p, b.d_srow.p, b.d_scnt.p, b.d_ecol.p};
+  const T* bval = fp32 ? (const T*)b.d_eval32.p : (const T*)b.d_eval.p;
   for (int p = 0; p < b.nphase(); p++) {
     const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_ilu_bwd<B, T, A, T>), dim3((ng + kGPB - 1) / kGPB), dim3(64 * kGPB), 0, st, g0, g0 + ng, b.d_grp.p,
-                       b.rowlist.empty() ? nullptr : b.d_rowlist.p, b.d_lev.p, b.phase_nlev[p], s.d_rowptr.p,
-                       s.d_usplit.p, s.d_diaind.p, s.d_colind.p, fcrs, z, done);
+    hipLaunchKernelGGL((k_sweep_bwd<B, T, A, T>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
+                       ba, bval, z, done);
   }
   return hipGetLastError();
 }
 
 hipError_t precond_apply64(const Sys& s, const double* src, double* z, const int* done, hipStream_t st) {
-  if (s.len == 1) return precond_T<1, double, double>(s, s.d_fcrs.p, s.d_fdia.p, src, z, done, st);
-  if (s.len == 2) return precond_T<2, double, double>(s, s.d_fcrs.p, s.d_fdia.p, src, z, done, st);
-  if (s.len == 3) return precond_T<3, double, double>(s, s.d_fcrs.p, s.d_fdia.p, src, z, done, st);
+  if (s.len == 1) return precond_T<1, double, double>(s, false, src, z, done, st);
+  if (s.len == 2) return precond_T<2, double, double>(s, false, src, z, done, st);
+  if (s.len == 3) return precond_T<3, double, double>(s, false, src, z, done, st);
   return hipErrorInvalidValue;
 }
 
 static hipError_t precond_apply32(const Sys& s, const float* src, float* z, const int* done, hipStream_t st) {
-  if (s.len == 1) return precond_T<1, float, float>(s, s.d_fcrs32.p, s.d_fdia32.p, src, z, done, st);
-  if (s.len == 2) return precond_T<2, float, float>(s, s.d_fcrs32.p, s.d_fdia32.p, src, z, done, st);
-  if (s.len == 3) return precond_T<3, float, float>(s, s.d_fcrs32.p, s.d_fdia32.p, src, z, done, st);
+  if (s.len == 1) return precond_T<1, float, float>(s, true, src, z, done, st);
+  if (s.len == 2) return precond_T<2, float, float>(s, true, src, z, done, st);
+  if (s.len == 3) return precond_T<3, float, float>(s, true, src, z, done, st);
   return hipErrorInvalidValue;
 }
 
