@@ -25,6 +25,9 @@ _live = weakref.WeakSet()  # systems and contexts still open, closed in order at
 def _close_all():
     objs = list(_live)
     for o in objs:
+        if isinstance(o, Batch):
+            o.close()
+    for o in objs:
         if isinstance(o, System):
             o.close()
     for o in objs:
@@ -306,3 +309,41 @@ class System:
         a, b, c = C.c_double(), C.c_double(), C.c_uint()
         _check(lib().dfm_sys_last_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"ms": a.value, "ms_per_iter": b.value, "iters": c.value}
+
+
+class Batch:
+    """Many systems solved together: one launch per kernel for all running systems (dfm_batch_*)."""
+
+    def __init__(self, ctx, precision=MIXED, max_systems=1024):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().dfm_batch_create(ctx.h, precision, max_systems, C.byref(self.h)))
+        _live.add(self)
+
+    def add(self, sys, tol, max_iter):
+        _check(lib().dfm_batch_add(self.h, sys.h, C.c_double(tol), C.c_uint(max_iter)))
+
+    def step(self, iterations):
+        n = C.c_int()
+        _check(lib().dfm_batch_step(self.h, iterations, C.byref(n)))
+        return n.value
+
+    def status(self, sys):
+        """(state, conv_ratio, iteration, seconds); state 0 running, 1 converged, 2 stopped at max_iter."""
+        st, conv, it, sec = C.c_int(), C.c_double(), C.c_uint(), C.c_double()
+        _check(lib().dfm_batch_status(self.h, sys.h, C.byref(st), C.byref(conv), C.byref(it), C.byref(sec)))
+        return st.value, conv.value, it.value, sec.value
+
+    def remove(self, sys):
+        _check(lib().dfm_batch_remove(self.h, sys.h))
+
+    def close(self):
+        if self.h:
+            lib().dfm_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

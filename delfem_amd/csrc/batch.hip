@@ -1,0 +1,652 @@
+// Segmented PCG: many independent linear systems ("segments") advanced together, one launch per kernel for
+// all running segments of one block size. Each segment keeps its own Krylov scalars, done flag and iteration
+// count (LsSol::Solve_PCG semantics per segment, src/ls/solver_ls_iter.cpp:100-172; the mixed-precision form
+// with reliable updates as in solver.hip).
+//
+// Launch geometry. Row-parallel kernels walk a block table: SpMV blocks cover RS = 256/TPR rows of one
+// segment, vector blocks cover DV dofs of one segment. The ILU sweeps walk, per phase, a table of (segment,
+// group) pairs, one wavefront each. Reductions are per segment: every block writes one partial into its own
+// slot; a small kernel with one workgroup per segment then sums exactly that segment's partials in block order
+// and does the segment's scalar work (alpha; the convergence judgement; beta), so the row-parallel kernels read
+// one scalar instead of re-reducing. The partition of a segment does not depend on the other segments, so its
+// results are bit-identical alone or in any mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "dfm_internal.hpp"
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace dfm {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int VPT = 4;            // dofs per thread of the vector kernels
+constexpr int DV = NT * VPT;      // dofs per vector block
+
+struct SegDev {
+  int n, m;
+  const unsigned* rp;
+  const unsigned* ci;
+  const void* crs;  // matrix of the Krylov SpMV (FP32 in mixed mode)
+  const void* dia;
+  const double* crs64;  // FP64 matrix (true residual, mixed mode)
+  const double* dia64;
+  SlabArgs fa, ba;
+  const void* fval;
+  const void* fdval;
+  const void* bval;
+  void *r, *z, *p, *q, *x;  // x: partial solution (mixed) or the solution (FP64)
+  double* x64;
+  const double* b64;
+  double* sc;
+  int* fl;
+  int ps0, ps1;  // SpMV blocks of the segment
+  int pv0, pv1;  // vector blocks of the segment
+};
+
+template <int B>
+constexpr int tpr_of() {
+  return B == 3 ? 16 : 8;
+}
+
+// scalar slots of the batch loop beyond those of pcg_start (rho stays in S_RZ, the reliable-update maximum in
+// S_RMAX: only the per-segment reduction kernels write them, one workgroup per segment)
+enum { S_ALPHA = 9, S_BETA = 10 };
+
+// block-wide sum of the partials slot[b0..b1), identical in every thread
+__device__ inline double consume_range(const double* __restrict__ slot, int b0, int b1) {
+  double t[1] = {0.0};
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += NT) t[0] += slot[b];
+  block_allsum<NT, 1>(t);
+  return t[0];
+}
+
+__device__ inline void produce_at(double* slot, int idx, double v) {
+  double t[1] = {v};
+  block_allsum<NT, 1>(t);
+  if (threadIdx.x == 0) slot[idx] = t[0];
+}
+
+// q = A p, partials of (p, q); mixed, when a reliable update is due: r = b - A64 x64 with its partials
+template <int B, int TPR, typename T, bool REL>
+__global__ void __launch_bounds__(NT) k_b_spmv(const SegDev* __restrict__ segs, const int* __restrict__ bseg,
+                                               double* slot_pq, double* slot_true) {
+  const int b = blockIdx.x;
+  const SegDev& d = segs[bseg[b]];
+  const int* fl = d.fl;
+  if (fl[F_DONE]) return;
+  const bool rel = REL && fl[F_PCUR];
+  constexpr int RS = NT / TPR;
+  const int lane = threadIdx.x % TPR;
+  const int row = (b - d.ps0) * RS + (int)threadIdx.x / TPR;
+  const T* p = (const T*)d.p;
+  double v = 0.0, vt = 0.0;
+  if (row < d.n) {
+    T acc[B];
+    spmv_row<B, TPR, T, T, T>(d.rp[row], d.rp[row + 1], d.ci, (const T*)d.crs, p, lane, acc);
+    if (lane == 0) {
+      dia_row<B, T, T, T>((const T*)d.dia, p, row, acc);
+#pragma unroll
+      for (int k = 0; k < B; k++) {
+        ((T*)d.q)[(size_t)row * B + k] = acc[k];
+        v += (double)p[(size_t)row * B + k] * (double)acc[k];
+      }
+    }
+    if (REL && rel) {
+      double a64[B];
+      spmv_row<B, TPR, double, double, double>(d.rp[row], d.rp[row + 1], d.ci, d.crs64, d.x64, lane, a64);
+      if (lane == 0) {
+        dia_row<B, double, double, double>(d.dia64, d.x64, row, a64);
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          const double t = d.b64[(size_t)row * B + k] - a64[k];
+          ((T*)d.r)[(size_t)row * B + k] = (T)t;
+          vt += t * t;
+        }
+      }
+    }
+  }
+  produce_at(slot_pq, b, v);
+  if (REL && rel) produce_at(slot_true, b, vt);
+}
+
+// per segment: iteration counter, alpha = rho / (p, q)   (solver_ls_iter.cpp:137-138)
+__global__ void __launch_bounds__(NT) k_b_alpha(const SegDev* __restrict__ segs, const double* slot_pq) {
+  const SegDev& d = segs[blockIdx.x];
+  if (d.fl[F_DONE]) return;
+  const double pq = consume_range(slot_pq, d.ps0, d.ps1);
+  if (threadIdx.x == 0) {
+    d.fl[F_IT] = d.fl[F_IT] + 1;
+    d.sc[S_ALPHA] = d.sc[S_RZ] / pq;
+  }
+}
+
+// r -= alpha q; x += alpha p; partials of (r, r)   (:139-144)
+template <typename T>
+__global__ void __launch_bounds__(NT) k_b_update(const SegDev* __restrict__ segs, const int* __restrict__ bseg,
+                                                 double* slot_rr) {
+  const int b = blockIdx.x;
+  const SegDev& d = segs[bseg[b]];
+  if (d.fl[F_DONE]) return;
+  const double alpha = d.sc[S_ALPHA];
+  const T ma = (T)(-alpha), pa = (T)alpha;
+  T* r = (T*)d.r;
+  T* x = (T*)d.x;
+  const T* q = (const T*)d.q;
+  const T* p = (const T*)d.p;
+  double v = 0.0;
+  const long long i0 = (long long)(b - d.pv0) * DV + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < VPT; j++) {
+    const long long i = i0 + (long long)j * NT;
+    if (i < d.m) {
+      const T ri = r[i] + ma * q[i];
+      r[i] = ri;
+      x[i] = x[i] + pa * p[i];
+      v += (double)ri * (double)ri;
+    }
+  }
+  produce_at(slot_rr, b, v);
+}
+
+// per segment: convergence judgement. FP64: on the recursive residual (:146-155). Mixed: on the true residual
+// when one was formed this iteration (the solution is then the previous iteration's), else the recursive
+// residual only schedules the next reliable update (it dropped by delta, or looks converged).
+template <bool MX>
+__global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs, const double* slot_rr,
+                                                const double* slot_true) {
+  const SegDev& d = segs[blockIdx.x];
+  int* fl = d.fl;
+  double* sc = d.sc;
+  if (fl[F_DONE]) return;
+  const double rr = consume_range(slot_rr, d.pv0, d.pv1);
+  const double rt = (MX && fl[F_PCUR]) ? consume_range(slot_true, d.ps0, d.ps1) : 0.0;
+  if (threadIdx.x) return;
+  const int it = fl[F_IT];
+  sc[S_RR] = rr;
+  if (!MX) {
+    const double ratio2 = rr * sc[S_INV0];
+    if (ratio2 < sc[S_TOL2] || it + 1 >= fl[F_MAXIT]) {
+      sc[S_CONV] = sqrt(ratio2);
+      if (ratio2 < sc[S_TOL2]) {
+        fl[F_ITOUT] = it;
+        fl[F_DONE] = 1;
+      } else {
+        fl[F_DONE] = 2;
+      }
+    }
+    return;
+  }
+  double rmax = sc[S_RMAX];
+  if (fl[F_PCUR]) {
+    const double ratio2 = rt * sc[S_INV0];
+    if (ratio2 < sc[S_TOL2]) {
+      sc[S_CONV] = sqrt(ratio2);
+      sc[S_RR] = rt;
+      fl[F_ITOUT] = it - 1;
+      fl[F_DONE] = 1;
+      return;
+    }
+    rmax = rt;
+  }
+  if (it + 1 >= fl[F_MAXIT]) {
+    sc[S_CONV] = sqrt(rr * sc[S_INV0]);
+    fl[F_DONE] = 2;
+    return;
+  }
+  const bool due = rr * sc[S_INV0] < sc[S_TOL2] || rr < sc[S_DELTA2] * rmax;
+  sc[S_RMAX] = due ? rr : fmax(rmax, rr);
+  fl[F_PNEXT] = due ? 1 : 0;
+  if (due) fl[F_NREL] = fl[F_NREL] + 1;
+}
+
+template <int B, typename T, typename A>
+__global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
+                                                     int count) {
+  __shared__ A xsb[kWPB][64 * B];
+  const int w = wave_in_block();
+  const int e = (int)blockIdx.x * kWPB + w;
+  if (e >= count) return;
+  const int2 en = ent[e];
+  const SegDev& d = segs[en.x];
+  sweep_fwd_wave<B, T, A, T, T>(d.fa, en.y, (const T*)d.fval, (const T*)d.fdval, (const T*)d.r, (T*)d.z, d.fl, xsb[w]);
+}
+
+template <int B, typename T, typename A>
+__global__ void __launch_bounds__(64 * kWPB) k_b_bwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
+                                                     int count) {
+  __shared__ A xsb[kWPB][64 * B];
+  const int w = wave_in_block();
+  const int e = (int)blockIdx.x * kWPB + w;
+  if (e >= count) return;
+  const int2 en = ent[e];
+  const SegDev& d = segs[en.x];
+  sweep_bwd_wave<B, T, A, T>(d.ba, en.y, (const T*)d.bval, (T*)d.z, d.fl, xsb[w]);
+}
+
+// partials of (r, z)
+template <typename T>
+__global__ void __launch_bounds__(NT) k_b_rz(const SegDev* __restrict__ segs, const int* __restrict__ bseg,
+                                             double* slot_rz) {
+  const int b = blockIdx.x;
+  const SegDev& d = segs[bseg[b]];
+  if (d.fl[F_DONE]) return;
+  const T* r = (const T*)d.r;
+  const T* z = (const T*)d.z;
+  double v = 0.0;
+  const long long i0 = (long long)(b - d.pv0) * DV + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < VPT; j++) {
+    const long long i = i0 + (long long)j * NT;
+    if (i < d.m) v += (double)r[i] * (double)z[i];
+  }
+  produce_at(slot_rz, b, v);
+}
+
+// per segment: beta = rho_new / rho (:158-159); the scheduled reliable update becomes current
+template <bool MX>
+__global__ void __launch_bounds__(NT) k_b_beta(const SegDev* __restrict__ segs, const double* slot_rz) {
+  const SegDev& d = segs[blockIdx.x];
+  if (d.fl[F_DONE]) return;
+  const double rz = consume_range(slot_rz, d.pv0, d.pv1);
+  if (threadIdx.x == 0) {
+    d.sc[S_BETA] = rz / d.sc[S_RZ];
+    d.sc[S_RZ] = rz;
+    if (MX) d.fl[F_PCUR] = d.fl[F_PNEXT];
+  }
+}
+
+// p = beta p; p += z (:160-166); mixed: the FP64 solution takes the partial solution when an update is due
+template <typename T, bool MX>
+__global__ void __launch_bounds__(NT) k_b_pupdate(const SegDev* __restrict__ segs, const int* __restrict__ bseg) {
+  const int b = blockIdx.x;
+  const SegDev& d = segs[bseg[b]];
+  if (d.fl[F_DONE]) return;
+  const T bt = (T)d.sc[S_BETA];
+  const bool fold = MX && d.fl[F_PCUR];
+  T* p = (T*)d.p;
+  const T* z = (const T*)d.z;
+  const long long i0 = (long long)(b - d.pv0) * DV + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < VPT; j++) {
+    const long long i = i0 + (long long)j * NT;
+    if (i < d.m) {
+      const T t = p[i] * bt;
+      p[i] = t + z[i];
+      if (MX && fold) {
+        float* xlo = (float*)d.x;
+        d.x64[i] += (double)xlo[i];
+        xlo[i] = 0.0f;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------
+// host side
+
+struct BatchSeg {
+  Sys* s = nullptr;
+  int slot = -1;
+  PcgBufs v{};
+  double tol = 0;
+  unsigned maxit = 0;
+  int state = 0;  // 0 running, 1 converged, 2 stopped at max_iter
+  double conv = 0;
+  unsigned iters = 0;
+  int nrel = 0;
+  double t_add = 0, t_done = 0;  // host clock (s)
+};
+
+struct Batch {
+  Ctx* ctx = nullptr;
+  bool mixed = false;
+  int cap = 0;
+  DBuf<double> d_sc;  // 16 per slot
+  DBuf<int> d_fl;     // 8 per slot
+  int* h_fl = nullptr;
+  double* h_sc = nullptr;
+  std::vector<int> free_slots;
+  std::vector<BatchSeg> segs;
+  bool dirty = true;
+  struct Group {
+    int B = 0;
+    std::vector<int> members;  // indices into segs
+    DBuf<SegDev> d_segs;
+    DBuf<int> d_sblk, d_vblk;
+    DBuf<int2> d_fent, d_bent;
+    std::vector<int> fptr, bptr;  // per phase offsets into the entry tables
+    int nsb = 0, nvb = 0;
+    DBuf<double> d_pq, d_true, d_rr, d_rz;
+  } grp[3];
+  ~Batch() {
+    for (auto& g : grp) {
+      g.d_segs.release(); g.d_sblk.release(); g.d_vblk.release(); g.d_fent.release(); g.d_bent.release();
+      g.d_pq.release(); g.d_true.release(); g.d_rr.release(); g.d_rz.release();
+    }
+    d_sc.release();
+    d_fl.release();
+    if (h_fl) (void)hipHostFree(h_fl);
+    if (h_sc) (void)hipHostFree(h_sc);
+  }
+};
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+static void grow(DBuf<T>& b, size_t n, hipError_t& e) {
+  if (e != hipSuccess) return;
+  if (b.n < n) e = b.alloc(std::max<size_t>(n, 1));
+}
+
+static int rebuild(Batch& bt) {
+  hipStream_t st = bt.ctx->stream;
+  for (int gi = 0; gi < 3; gi++) bt.grp[gi].members.clear();
+  for (int k = 0; k < (int)bt.segs.size(); k++)
+    if (bt.segs[k].state == 0) bt.grp[bt.segs[k].s->len - 1].members.push_back(k);
+  for (int gi = 0; gi < 3; gi++) {
+    Batch::Group& g = bt.grp[gi];
+    g.B = gi + 1;
+    const int tpr = g.B == 3 ? tpr_of<3>() : tpr_of<1>();
+    const int rs = NT / tpr;
+    std::vector<SegDev> h;
+    std::vector<int> sblk, vblk;
+    std::vector<std::vector<int2>> fph, bph;
+    for (int k : g.members) {
+      BatchSeg& sg = bt.segs[k];
+      Sys& s = *sg.s;
+      SegDev d{};
+      const int si = (int)h.size();
+      d.n = s.n;
+      d.m = s.n * s.len;
+      d.rp = s.d_rowptr.p;
+      d.ci = s.d_colind.p;
+      d.crs = sg.v.crs;
+      d.dia = sg.v.dia;
+      d.crs64 = s.d_crs.p;
+      d.dia64 = s.d_dia.p;
+      d.fa = SlabArgs{s.fwd.d_gmeta.p, s.fwd.d_srow.p, s.fwd.d_scnt.p, s.fwd.d_ecol.p};
+      d.ba = SlabArgs{s.bwd.d_gmeta.p, s.bwd.d_srow.p, s.bwd.d_scnt.p, s.bwd.d_ecol.p};
+      d.fval = bt.mixed ? (const void*)s.fwd.d_eval32.p : (const void*)s.fwd.d_eval.p;
+      d.fdval = bt.mixed ? (const void*)s.fwd.d_dval32.p : (const void*)s.fwd.d_dval.p;
+      d.bval = bt.mixed ? (const void*)s.bwd.d_eval32.p : (const void*)s.bwd.d_eval.p;
+      d.r = sg.v.r;
+      d.z = sg.v.z;
+      d.p = sg.v.p;
+      d.q = sg.v.q;
+      d.x = sg.v.x;
+      d.x64 = sg.v.x64;
+      d.b64 = sg.v.b64;
+      d.sc = bt.d_sc.p + 16 * (size_t)sg.slot;
+      d.fl = bt.d_fl.p + 8 * (size_t)sg.slot;
+      d.ps0 = (int)sblk.size();
+      for (int r = 0; r < s.n; r += rs) sblk.push_back(si);
+      d.ps1 = (int)sblk.size();
+      d.pv0 = (int)vblk.size();
+      for (long long i = 0; i < (long long)d.m; i += DV) vblk.push_back(si);
+      d.pv1 = (int)vblk.size();
+      for (int dir = 0; dir < 2; dir++) {
+        const Sweep& w = dir == 0 ? s.fwd : s.bwd;
+        auto& ph = dir == 0 ? fph : bph;
+        if ((int)ph.size() < w.nphase()) ph.resize(w.nphase());
+        for (int p = 0; p < w.nphase(); p++)
+          for (int q = w.phase_gptr[p]; q < w.phase_gptr[p + 1]; q++) ph[p].push_back(make_int2(si, q));
+      }
+      h.push_back(d);
+    }
+    g.nsb = (int)sblk.size();
+    g.nvb = (int)vblk.size();
+    if (h.empty()) continue;
+    std::vector<int2> fe, be;
+    g.fptr.assign(1, 0);
+    g.bptr.assign(1, 0);
+    for (auto& v : fph) {
+      fe.insert(fe.end(), v.begin(), v.end());
+      g.fptr.push_back((int)fe.size());
+    }
+    for (auto& v : bph) {
+      be.insert(be.end(), v.begin(), v.end());
+      g.bptr.push_back((int)be.size());
+    }
+    hipError_t e = hipSuccess;
+    grow(g.d_segs, h.size(), e);
+    grow(g.d_sblk, sblk.size(), e);
+    grow(g.d_vblk, vblk.size(), e);
+    grow(g.d_fent, fe.size(), e);
+    grow(g.d_bent, be.size(), e);
+    grow(g.d_pq, sblk.size(), e);
+    grow(g.d_true, sblk.size(), e);
+    grow(g.d_rr, vblk.size(), e);
+    grow(g.d_rz, vblk.size(), e);
+    DFM_HIP(e);
+    DFM_HIP(hipMemcpyAsync(g.d_segs.p, h.data(), h.size() * sizeof(SegDev), hipMemcpyHostToDevice, st));
+    DFM_HIP(hipMemcpyAsync(g.d_sblk.p, sblk.data(), sblk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    DFM_HIP(hipMemcpyAsync(g.d_vblk.p, vblk.data(), vblk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (!fe.empty()) DFM_HIP(hipMemcpyAsync(g.d_fent.p, fe.data(), fe.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    if (!be.empty()) DFM_HIP(hipMemcpyAsync(g.d_bent.p, be.data(), be.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    // the tables are host temporaries: the copies must finish before they go out of scope
+    DFM_HIP(hipStreamSynchronize(st));
+  }
+  bt.dirty = false;
+  return DFM_OK;
+}
+
+template <int B, typename T, bool MX>
+static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
+  constexpr int TPR = tpr_of<B>();
+  const SegDev* segs = g.d_segs.p;
+  const dim3 nseg((unsigned)g.members.size());
+  hipLaunchKernelGGL((k_b_spmv<B, TPR, T, MX>), dim3(g.nsb), dim3(NT), 0, st, segs, g.d_sblk.p, g.d_pq.p, g.d_true.p);
+  hipLaunchKernelGGL(k_b_alpha, nseg, dim3(NT), 0, st, segs, (const double*)g.d_pq.p);
+  hipLaunchKernelGGL((k_b_update<T>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p, g.d_rr.p);
+  hipLaunchKernelGGL((k_b_judge<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p, (const double*)g.d_true.p);
+  for (size_t p = 0; p + 1 < g.fptr.size(); p++) {
+    const int c = g.fptr[p + 1] - g.fptr[p];
+    if (c) hipLaunchKernelGGL((k_b_fwd<B, T, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
+                              (const int2*)g.d_fent.p + g.fptr[p], c);
+  }
+  for (size_t p = 0; p + 1 < g.bptr.size(); p++) {
+    const int c = g.bptr[p + 1] - g.bptr[p];
+    if (c) hipLaunchKernelGGL((k_b_bwd<B, T, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
+                              (const int2*)g.d_bent.p + g.bptr[p], c);
+  }
+  hipLaunchKernelGGL((k_b_rz<T>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p, g.d_rz.p);
+  hipLaunchKernelGGL((k_b_beta<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rz.p);
+  hipLaunchKernelGGL((k_b_pupdate<T, MX>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p);
+  return hipGetLastError();
+}
+
+static int finish_seg(Batch& bt, BatchSeg& sg) {
+  const int* fl = bt.h_fl + 8 * sg.slot;
+  const double* sc = bt.h_sc + 16 * sg.slot;
+  const int done = fl[F_DONE];
+  sg.state = done == 1 ? 1 : 2;
+  sg.conv = done ? sc[S_CONV] : std::sqrt(sc[S_RR] * sc[S_INV0]);
+  sg.iters = done == 1 ? (unsigned)fl[F_ITOUT] : sg.maxit;
+  sg.nrel = fl[F_NREL];
+  sg.t_done = now_s();
+  const int rc = pcg_finish(*sg.s, sg.v, bt.mixed, done, bt.ctx->stream);
+  bt.dirty = true;
+  return rc;
+}
+
+static int poll(Batch& bt) {
+  hipStream_t st = bt.ctx->stream;
+  DFM_HIP(hipMemcpyAsync(bt.h_fl, bt.d_fl.p, sizeof(int) * 8 * bt.cap, hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipMemcpyAsync(bt.h_sc, bt.d_sc.p, sizeof(double) * 16 * bt.cap, hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipStreamSynchronize(st));
+  for (auto& sg : bt.segs)
+    if (sg.state == 0 && bt.h_fl[8 * sg.slot + F_DONE]) {
+      const int rc = finish_seg(bt, sg);
+      if (rc != DFM_OK) return rc;
+    }
+  return DFM_OK;
+}
+
+static int find_seg(Batch& bt, Sys* s) {
+  for (int k = 0; k < (int)bt.segs.size(); k++)
+    if (bt.segs[k].s == s) return k;
+  return -1;
+}
+
+void batch_detach(Batch* bt, Sys* s) {
+  if (!bt) return;
+  const int k = find_seg(*bt, s);
+  if (k < 0) return;
+  (void)hipStreamSynchronize(bt->ctx->stream);
+  bt->free_slots.push_back(bt->segs[k].slot);
+  bt->segs.erase(bt->segs.begin() + k);
+  s->batch = nullptr;
+  bt->dirty = true;
+}
+
+}  // namespace dfm
+
+using namespace dfm;
+
+struct dfm_batch_s {
+  Batch b;
+  dfm_ctx_s* owner = nullptr;
+};
+
+extern "C" {
+
+int dfm_batch_create(dfm_ctx ctx, int precision, int max_systems, dfm_batch* out) {
+  DFM_CHECK(ctx && out, DFM_ERR_ARG, "dfm_batch_create: null argument");
+  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED, DFM_ERR_ARG, "dfm_batch_create: bad precision");
+  DFM_CHECK(max_systems > 0 && max_systems <= (1 << 20), DFM_ERR_ARG, "dfm_batch_create: bad max_systems");
+  DFM_HIP(hipSetDevice(ctx->c.device));
+  auto* h = new dfm_batch_s;
+  Batch& bt = h->b;
+  bt.ctx = &ctx->c;
+  bt.mixed = precision == DFM_MIXED;
+  bt.cap = max_systems;
+  if (bt.d_sc.alloc((size_t)16 * bt.cap) != hipSuccess || bt.d_fl.alloc((size_t)8 * bt.cap) != hipSuccess ||
+      hipHostMalloc((void**)&bt.h_fl, sizeof(int) * 8 * bt.cap) != hipSuccess ||
+      hipHostMalloc((void**)&bt.h_sc, sizeof(double) * 16 * bt.cap) != hipSuccess) {
+    delete h;
+    set_error("dfm_batch_create: out of memory");
+    return DFM_ERR_NOMEM;
+  }
+  // flags of free slots read as done, so no stale slot is ever taken for a running one
+  std::vector<int> f((size_t)8 * bt.cap, 0);
+  for (int k = 0; k < bt.cap; k++) f[8 * (size_t)k + F_DONE] = 3;
+  if (hipMemcpy(bt.d_fl.p, f.data(), f.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    delete h;
+    set_error("dfm_batch_create: copy failed");
+    return DFM_ERR_HIP;
+  }
+  for (int k = bt.cap - 1; k >= 0; k--) bt.free_slots.push_back(k);
+  ctx->c.refs++;
+  h->owner = ctx;
+  *out = h;
+  return DFM_OK;
+}
+
+int dfm_batch_destroy(dfm_batch b) {
+  if (!b) return DFM_OK;
+  Batch& bt = b->b;
+  (void)hipStreamSynchronize(bt.ctx->stream);
+  for (auto& sg : bt.segs) sg.s->batch = nullptr;
+  dfm_ctx owner = b->owner;
+  delete b;
+  if (owner) ctx_release(owner);
+  return DFM_OK;
+}
+
+int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
+  DFM_CHECK(b && sys, DFM_ERR_ARG, "dfm_batch_add: null argument");
+  Batch& bt = b->b;
+  Sys& s = sys->s;
+  DFM_CHECK(s.ctx == bt.ctx, DFM_ERR_ARG, "dfm_batch_add: the system belongs to another context");
+  DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_batch_add: the system is already in a batch");
+  DFM_CHECK(!bt.free_slots.empty(), DFM_ERR_STATE, "dfm_batch_add: batch full");
+  DFM_CHECK(max_iter > 0, DFM_ERR_ARG, "dfm_batch_add: max_iter must be positive");
+  int rc = sys_solve_setup(s, bt.mixed ? DFM_MIXED : DFM_FP64);
+  if (rc != DFM_OK) return rc;
+  BatchSeg sg;
+  sg.s = &s;
+  sg.slot = bt.free_slots.back();
+  sg.v = pcg_bufs(s, bt.mixed);
+  sg.tol = tol;
+  sg.maxit = max_iter;
+  hipStream_t st = bt.ctx->stream;
+  rc = pcg_prepare(s, sg.v, bt.mixed, st);
+  if (rc == DFM_OK)
+    rc = pcg_start(s, sg.v, bt.mixed, tol, max_iter, bt.d_sc.p + 16 * (size_t)sg.slot, bt.d_fl.p + 8 * (size_t)sg.slot, st);
+  if (rc != DFM_OK) return rc;
+  bt.free_slots.pop_back();
+  sg.t_add = now_s();
+  bt.segs.push_back(sg);
+  s.batch = &bt;
+  bt.dirty = true;
+  return DFM_OK;
+}
+
+int dfm_batch_step(dfm_batch b, int iterations, int* n_running) {
+  DFM_CHECK(b && iterations >= 0, DFM_ERR_ARG, "dfm_batch_step: bad argument");
+  Batch& bt = b->b;
+  hipStream_t st = bt.ctx->stream;
+  int rc = DFM_OK;
+  if (bt.dirty) {
+    // segments whose start already decided them (zero right-hand side) leave before the first iteration
+    rc = poll(bt);
+    if (rc != DFM_OK) return rc;
+    rc = rebuild(bt);
+    if (rc != DFM_OK) return rc;
+  }
+  for (int t = 0; t < iterations; t++) {
+    for (auto& g : bt.grp) {
+      if (g.members.empty()) continue;
+      hipError_t e;
+      if (bt.mixed)
+        e = g.B == 1 ? iterate_group<1, float, true>(g, st)
+                     : g.B == 2 ? iterate_group<2, float, true>(g, st) : iterate_group<3, float, true>(g, st);
+      else
+        e = g.B == 1 ? iterate_group<1, double, false>(g, st)
+                     : g.B == 2 ? iterate_group<2, double, false>(g, st) : iterate_group<3, double, false>(g, st);
+      DFM_HIP(e);
+    }
+  }
+  rc = poll(bt);
+  if (rc != DFM_OK) return rc;
+  int nr = 0;
+  for (auto& sg : bt.segs) nr += sg.state == 0;
+  if (n_running) *n_running = nr;
+  return DFM_OK;
+}
+
+int dfm_batch_status(dfm_batch b, dfm_sys sys, int* state, double* conv_ratio, unsigned* iteration,
+                     double* seconds) {
+  DFM_CHECK(b && sys, DFM_ERR_ARG, "dfm_batch_status: null argument");
+  Batch& bt = b->b;
+  const int k = find_seg(bt, &sys->s);
+  DFM_CHECK(k >= 0, DFM_ERR_ARG, "dfm_batch_status: the system is not in this batch");
+  const BatchSeg& sg = bt.segs[k];
+  if (state) *state = sg.state;
+  if (conv_ratio) *conv_ratio = sg.conv;
+  if (iteration) *iteration = sg.iters;
+  if (seconds) *seconds = sg.state ? sg.t_done - sg.t_add : now_s() - sg.t_add;
+  return DFM_OK;
+}
+
+int dfm_batch_remove(dfm_batch b, dfm_sys sys) {
+  DFM_CHECK(b && sys, DFM_ERR_ARG, "dfm_batch_remove: null argument");
+  Batch& bt = b->b;
+  DFM_CHECK(find_seg(bt, &sys->s) >= 0, DFM_ERR_ARG, "dfm_batch_remove: the system is not in this batch");
+  batch_detach(&bt, &sys->s);
+  return DFM_OK;
+}
+
+}  // extern "C"

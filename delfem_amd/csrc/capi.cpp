@@ -148,12 +148,16 @@ int dfm_ctx_create(int device, dfm_ctx* out) {
   return DFM_OK;
 }
 
-static void ctx_release(dfm_ctx ctx) {
+}  // extern "C"
+
+void ctx_release(dfm_ctx ctx) {
   if (--ctx->c.refs > 0) return;
   (void)hipStreamSynchronize(ctx->c.stream);
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }
+
+extern "C" {
 
 // Systems keep their context alive: destroying a context with live systems defers the release.
 int dfm_ctx_destroy(dfm_ctx ctx) {
@@ -234,6 +238,7 @@ int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, 
 
 int dfm_sys_destroy(dfm_sys sys) {
   if (!sys) return DFM_OK;
+  if (sys->s.batch) batch_detach(sys->s.batch, &sys->s);
   (void)hipStreamSynchronize(sys->s.ctx->stream);
   dfm_ctx owner = sys->owner;
   delete sys;
@@ -708,13 +713,12 @@ static int ensure_fp32(Sys& s) {
   return DFM_OK;
 }
 
-int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration) {
-  NEED(sys, "dfm_solve_pcg");
-  NEED(conv_ratio, "dfm_solve_pcg");
-  NEED(iteration, "dfm_solve_pcg");
-  Sys& s = sys->s;
-  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED, DFM_ERR_ARG, "dfm_solve_pcg: bad precision");
-  DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_solve_pcg: assemble and factor first");
+}  // extern "C"
+
+namespace dfm {
+int sys_solve_setup(Sys& s, int precision) {
+  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED, DFM_ERR_ARG, "solve: bad precision");
+  DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "solve: assemble and factor first");
   int rc = ensure_work(s, 2, precision == DFM_FP64 ? 0 : 5);
   if (rc != DFM_OK) return rc;
   if (precision == DFM_MIXED) {
@@ -722,6 +726,20 @@ int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iter
     if (rc != DFM_OK) return rc;
   }
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, mdof(s), s.ctx->stream));
+  return DFM_OK;
+}
+}  // namespace dfm
+
+extern "C" {
+
+int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration) {
+  NEED(sys, "dfm_solve_pcg");
+  NEED(conv_ratio, "dfm_solve_pcg");
+  NEED(iteration, "dfm_solve_pcg");
+  Sys& s = sys->s;
+  DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_solve_pcg: the system is being solved in a batch");
+  const int rc = sys_solve_setup(s, precision);
+  if (rc != DFM_OK) return rc;
   return solve_pcg(s, precision, conv_ratio, iteration);
 }
 

@@ -186,11 +186,10 @@ struct SlabSlot {
   bool active;
 };
 
-__device__ inline bool slab_slot(const SlabArgs& sa, int gbase, int gend, const int* done, SlabSlot& ss) {
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
-  ss.g = gbase + (int)blockIdx.x * kWPB + wave;
-  if (ss.g >= gend) return false;
-  const int* m = sa.gmeta + 4 * (size_t)ss.g;
+// g: the group (wave-uniform); returns false when the system's done flag is set.
+__device__ inline bool slab_slot(const SlabArgs& sa, int g, const int* done, SlabSlot& ss) {
+  ss.g = g;
+  const int* m = sa.gmeta + 4 * (size_t)g;
   const int m0 = m[0], m1 = m[1], row0 = m[2], nrows = m[3];
   if (done && *done) return false;
   ss.off = m0;
@@ -199,7 +198,7 @@ __device__ inline bool slab_slot(const SlabArgs& sa, int gbase, int gend, const 
   ss.nlev = m1 >> 16;
   ss.lane = (int)(threadIdx.x % 64);
   ss.active = ss.lane < nrows;
-  const size_t slot = (size_t)ss.g * 64 + ss.lane;
+  const size_t slot = (size_t)g * 64 + ss.lane;
   ss.row = row0 >= 0 ? (ss.active ? row0 + ss.lane : row0) : (ss.active ? sa.srow[slot] : 0);
   const unsigned c = sa.scnt[slot];
   ss.no = (int)(c & 255u);
@@ -207,6 +206,8 @@ __device__ inline bool slab_slot(const SlabArgs& sa, int gbase, int gend, const 
   ss.lv = ss.active ? (int)(c >> 16) : -1;
   return true;
 }
+
+__device__ inline int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64)); }
 
 template <int B, typename T, typename A>
 __device__ inline void slab_load_blk(const T* __restrict__ val, size_t srow_, int lane, A (&a)[B * B]) {
@@ -290,17 +291,15 @@ __device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T*
   }
 }
 
-// Forward sweep of one phase: x = D^-1 (b - L x) for the phase's rows. b and x may alias.
+// Forward sweep of group g by the calling wavefront: x = D^-1 (b - L x) for the group's rows. b and x may alias.
+// xs: the wavefront's LDS row (64*B values).
 template <int B, typename T, typename A, typename VI, typename VO>
-__global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
-                                                      const T* __restrict__ dval, const VI* b, VO* x,
-                                                      const int* __restrict__ done) {
+__device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, const T* __restrict__ dval,
+                                      const VI* b, VO* x, const int* __restrict__ done, A* xs) {
   constexpr int BS = B * B;
   constexpr int MO = SlabCap<B, A>::OUT, MI = SlabCap<B, A>::IN;
-  __shared__ A xsb[kWPB][64 * B];
   SlabSlot ss;
-  if (!slab_slot(sa, gbase, gend, done, ss)) return;
-  A* xs = xsb[threadIdx.x / 64];
+  if (!slab_slot(sa, g, done, ss)) return;
   A t[B], D[BS];
   A vo[MO][BS], xo[MO][B], vi[MI][BS];
   unsigned jo[MO], li[MI];
@@ -339,16 +338,15 @@ __global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, Sl
     for (int r = 0; r < B; r++) x[(size_t)ss.row * B + r] = (VO)t[r];
 }
 
-// Backward sweep of one phase, in place on x: x_i -= U_in x_in (in-group, ascending) then U_out x_out.
+// Backward sweep of group g by the calling wavefront, in place on x: x_i -= U_in x_in (in-group, ascending) then
+// U_out x_out.
 template <int B, typename T, typename A, typename VX>
-__global__ void __launch_bounds__(64 * kWPB) k_sweep_bwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
-                                                      VX* x, const int* __restrict__ done) {
+__device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, VX* x,
+                                      const int* __restrict__ done, A* xs) {
   constexpr int BS = B * B;
   constexpr int MO = SlabCap<B, A>::OUT, MI = SlabCap<B, A>::IN;
-  __shared__ A xsb[kWPB][64 * B];
   SlabSlot ss;
-  if (!slab_slot(sa, gbase, gend, done, ss)) return;
-  A* xs = xsb[threadIdx.x / 64];
+  if (!slab_slot(sa, g, done, ss)) return;
   A t[B];
   A vo[MO][BS], xo[MO][B], vi[MI][BS];
   unsigned jo[MO], li[MI];
@@ -372,6 +370,28 @@ __global__ void __launch_bounds__(64 * kWPB) k_sweep_bwd(int gbase, int gend, Sl
   if (ss.active)
 #pragma unroll
     for (int r = 0; r < B; r++) x[(size_t)ss.row * B + r] = (VX)t[r];
+}
+
+// One phase of one system: groups [gbase, gend), kWPB groups per workgroup.
+template <int B, typename T, typename A, typename VI, typename VO>
+__global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
+                                                      const T* __restrict__ dval, const VI* b, VO* x,
+                                                      const int* __restrict__ done) {
+  __shared__ A xsb[kWPB][64 * B];
+  const int w = wave_in_block();
+  const int g = gbase + (int)blockIdx.x * kWPB + w;
+  if (g >= gend) return;
+  sweep_fwd_wave<B, T, A, VI, VO>(sa, g, val, dval, b, x, done, xsb[w]);
+}
+
+template <int B, typename T, typename A, typename VX>
+__global__ void __launch_bounds__(64 * kWPB) k_sweep_bwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
+                                                      VX* x, const int* __restrict__ done) {
+  __shared__ A xsb[kWPB][64 * B];
+  const int w = wave_in_block();
+  const int g = gbase + (int)blockIdx.x * kWPB + w;
+  if (g >= gend) return;
+  sweep_bwd_wave<B, T, A, VX>(sa, g, val, x, done, xsb[w]);
 }
 
 }  // namespace dfm

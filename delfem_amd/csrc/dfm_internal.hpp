@@ -156,9 +156,15 @@ struct Sys {
   double last_ms = 0.0, last_ms_iter = 0.0;
   unsigned last_inner = 0;
   int last_nrel = 0;  // reliable updates in the last mixed solve
+  struct Batch* batch = nullptr;  // the batch the system is being solved in, if any
 
   ~Sys();
 };
+
+// Buffers, FP32 copies and the right-hand side copy for a solve of the given precision (capi.cpp).
+int sys_solve_setup(Sys& s, int precision);
+// Take a system out of the batch it is solved in (batch.hip).
+void batch_detach(struct Batch* b, Sys* s);
 
 // host symbolic routines (host_symbolic.cpp)
 void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vector<unsigned>& rowptr,
@@ -183,3 +189,6 @@ struct dfm_sys_s {
   dfm::Sys s;
   dfm_ctx_s* owner = nullptr;
 };
+
+// Drop one reference of a context; the last one frees it (capi.cpp).
+void ctx_release(dfm_ctx ctx);

@@ -16,22 +16,6 @@
 
 namespace dfm {
 
-// scalar slots (doubles)
-enum {
-  S_RZ = 0,     // rho = (r, z), two parity slots 0/1
-  S_INV0 = 2,   // 1 / ||r0||^2
-  S_TOL2 = 3,   // tolerance^2
-  S_CONV = 4,   // achieved ratio
-  S_RMAX = 5,   // reliable updates: max ||r||^2 since the last update, two parity slots 5/6
-  S_DELTA2 = 7, // reliable updates: trigger ratio^2
-  S_RR = 8,     // last ||r||^2 seen by the convergence test
-  S_TMP = 15
-};
-// int flags
-enum { F_DONE = 0, F_IT = 1, F_ITOUT = 2, F_MAXIT = 3, F_NREL = 5, F_PCUR = 6, F_PNEXT = 7 };
-// partial slots, kMaxPartial doubles per value
-enum { P_A = 0, P_B = 2, P_C = 4, P_D = 6, P_NSLOT = 8 };
-
 constexpr int NT = 256;
 
 __global__ void k_solver_init(double* sc, int* fl, double tol2, int maxit, double delta2) {
@@ -486,20 +470,6 @@ static int poll(Sys& s, hipStream_t st) {
   return DFM_OK;
 }
 
-// Buffers of one PCG run: FP64 loop (x = the FP64 solution), or the FP32 loop with reliable updates
-// (x = the FP32 partial solution, x64/b64 the FP64 solution and right-hand side).
-struct PcgBufs {
-  const void* crs;
-  const void* dia;
-  void* r;
-  void* z;
-  void* p;
-  void* q;
-  void* x;
-  double* x64;
-  const double* b64;
-};
-
 template <typename T>
 static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
   const long long m = (long long)s.n * s.len;
@@ -573,15 +543,13 @@ static unsigned next_chunk(double ratio2, double prev_ratio2, unsigned since, do
   return (unsigned)k;
 }
 
+// x = 0; ||r0||^2 (guard); z = M^-1 r; p = z; rho = (r, z)   (solver_ls_iter.cpp:107-133)
 template <typename T>
-static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double guard, double& conv,
-                   unsigned& iter, bool& converged) {
-  hipStream_t st = s.ctx->stream;
+static int pcg_start_T(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double guard, double* sc,
+                       int* fl, hipStream_t st) {
   const long long m = (long long)s.n * s.len;
   const dim3 vg = vgrid(m);
   const int G = (int)vg.x;
-  int* fl = s.d_iflag.p;
-  double* sc = s.d_scal.p;
   T* r = (T*)v.r;
   T* z = (T*)v.z;
   T* p = (T*)v.p;
@@ -590,7 +558,6 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
   hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, st, sc, fl, tol * tol, (int)maxit, delta * delta);
   hipLaunchKernelGGL(k_vfill<T>, vg, dim3(NT), 0, st, (T*)v.x, (T)0, m, (const int*)nullptr);
   if (mixed) hipLaunchKernelGGL(k_vfill<double>, vg, dim3(NT), 0, st, v.x64, 0.0, m, (const int*)nullptr);
-  // x = 0; ||r0||^2 (guard); z = M^-1 r; p = z; rho = (r, z)
   hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)r, m, slot(s, P_A), (const int*)nullptr);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, guard, sc, fl);
   hipError_t e = mixed ? precond_apply32(s, (const float*)r, (float*)z, fl, st)
@@ -600,6 +567,54 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
   hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)z, m, slot(s, P_A), fl);
   hipLaunchKernelGGL(k_pcg_rho0, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, sc, fl);
   DFM_HIP(hipGetLastError());
+  return DFM_OK;
+}
+
+int pcg_start(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double* sc, int* fl, hipStream_t st) {
+  return mixed ? pcg_start_T<float>(s, v, true, tol, maxit, 1.0e-30, sc, fl, st)
+               : pcg_start_T<double>(s, v, false, tol, maxit, 1.0e-30, sc, fl, st);
+}
+
+PcgBufs pcg_bufs(Sys& s, bool mixed) {
+  const long long m = (long long)s.n * s.len;
+  if (!mixed) return PcgBufs{s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_work.p + m, s.d_work.p, s.d_work.p + m, s.d_upd.p,
+                             nullptr, nullptr};
+  // FP32: r, z, p, q, xlo; FP64: the right-hand side (d_rhs) and the solution (the update handle)
+  float* w = s.d_work32.p;
+  return PcgBufs{s.d_crs32.p, s.d_dia32.p, w, w + m, w + 2 * m, w + 3 * m, w + 4 * m, s.d_upd.p, s.d_rhs.p};
+}
+
+int pcg_prepare(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
+  const long long m = (long long)s.n * s.len;
+  if (mixed) hipLaunchKernelGGL(k_mx_to32, vgrid(m), dim3(NT), 0, st, m, s.d_rhs.p, (float*)v.r);
+  DFM_HIP(hipGetLastError());
+  return DFM_OK;
+}
+
+int pcg_finish(Sys& s, const PcgBufs& v, bool mixed, int done, hipStream_t st) {
+  const long long m = (long long)s.n * s.len;
+  if (!mixed) return DFM_OK;
+  // stopped by max_iter: the FP64 solution takes the partial solution not yet folded (when converged, the
+  // FP64 solution is the one whose true residual was confirmed)
+  if (done != 1) hipLaunchKernelGGL(k_mx_final_fold, vgrid(m), dim3(NT), 0, st, m, (float*)v.x, v.x64);
+  // the residual handle receives the FP64 true residual b - A x (the reference leaves its final residual)
+  DFM_HIP(launch_copy(s.d_rhs.p, s.d_res.p, m, st));
+  DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
+                      s.d_upd.p, 1.0, s.d_res.p, st));
+  DFM_HIP(hipGetLastError());
+  return DFM_OK;
+}
+
+template <typename T>
+static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double& conv, unsigned& iter,
+                   bool& converged) {
+  hipStream_t st = s.ctx->stream;
+  int* fl = s.d_iflag.p;
+  double* sc = s.d_scal.p;
+  {
+    const int rc = pcg_start(s, v, mixed, tol, maxit, sc, fl, st);
+    if (rc != DFM_OK) return rc;
+  }
   const int gk = mixed ? 1 : 0;
   const bool use_graph = getenv("DFM_NO_GRAPH") == nullptr;
   if (use_graph) {
@@ -633,13 +648,11 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
     const int rc = poll(s, st);
     if (rc != DFM_OK) return rc;
   }
-  if (mixed && s.h_pinned[F_DONE] != 1) {
-    // stopped by max_iter: the FP64 solution takes the partial solution not yet folded (when converged, the
-    // FP64 solution is the one whose true residual was confirmed)
-    hipLaunchKernelGGL(k_mx_final_fold, vg, dim3(NT), 0, st, m, (float*)v.x, v.x64);
-  }
-  DFM_HIP(hipGetLastError());
   const int done = s.h_pinned[F_DONE];
+  {
+    const int rc = pcg_finish(s, v, mixed, done, st);
+    if (rc != DFM_OK) return rc;
+  }
   converged = done == 1;
   // not converged within max_iter: the ratio of the last residual seen (solver_ls_iter.cpp:168-171)
   conv = done ? s.h_pinned_d[S_CONV] : std::sqrt(s.h_pinned_d[S_RR] * s.h_pinned_d[S_INV0]);
@@ -648,7 +661,6 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
 }
 
 int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
-  const long long m = (long long)s.n * s.len;
   const double tol = *conv_ratio;
   const unsigned maxit = *iteration;
   hipStream_t st = s.ctx->stream;
@@ -660,23 +672,11 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   unsigned it = 0;
   bool ok = false;
   int rc;
-  if (precision == DFM_FP64) {
-    PcgBufs v{s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_work.p + m, s.d_work.p, s.d_work.p + m, s.d_upd.p, nullptr,
-              nullptr};
-    rc = pcg_run<double>(s, v, false, tol, maxit, 1.0e-30, conv, it, ok);
-  } else {
-    // FP32: r, z, p, q, xlo; FP64: the right-hand side (d_rhs) and the solution (the update handle)
-    float* w = s.d_work32.p;
-    PcgBufs v{s.d_crs32.p, s.d_dia32.p, w, w + m, w + 2 * m, w + 3 * m, w + 4 * m, s.d_upd.p, s.d_rhs.p};
-    hipLaunchKernelGGL(k_mx_to32, vgrid(m), dim3(NT), 0, st, m, s.d_rhs.p, w);
-    rc = pcg_run<float>(s, v, true, tol, maxit, 1.0e-30, conv, it, ok);
-    // the residual handle receives the FP64 true residual b - A x (the reference leaves its final residual)
-    if (rc == DFM_OK) {
-      DFM_HIP(launch_copy(s.d_rhs.p, s.d_res.p, m, st));
-      DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
-                          s.d_upd.p, 1.0, s.d_res.p, st));
-    }
-  }
+  const bool mixed = precision == DFM_MIXED;
+  const PcgBufs v = pcg_bufs(s, mixed);
+  rc = pcg_prepare(s, v, mixed, st);
+  if (rc == DFM_OK)
+    rc = mixed ? pcg_run<float>(s, v, true, tol, maxit, conv, it, ok) : pcg_run<double>(s, v, false, tol, maxit, conv, it, ok);
   if (rc != DFM_OK) return rc;
   DFM_HIP(hipEventRecord(e1, st));
   DFM_HIP(hipEventSynchronize(e1));

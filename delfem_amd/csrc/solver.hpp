@@ -5,6 +5,44 @@
 
 namespace dfm {
 
+// scalar slots (doubles)
+enum {
+  S_RZ = 0,     // rho = (r, z), two parity slots 0/1
+  S_INV0 = 2,   // 1 / ||r0||^2
+  S_TOL2 = 3,   // tolerance^2
+  S_CONV = 4,   // achieved ratio
+  S_RMAX = 5,   // reliable updates: max ||r||^2 since the last update, two parity slots 5/6
+  S_DELTA2 = 7, // reliable updates: trigger ratio^2
+  S_RR = 8,     // last ||r||^2 seen by the convergence test
+  S_TMP = 15
+};
+// int flags
+enum { F_DONE = 0, F_IT = 1, F_ITOUT = 2, F_MAXIT = 3, F_NREL = 5, F_PCUR = 6, F_PNEXT = 7 };
+// partial slots, kMaxPartial doubles per value
+enum { P_A = 0, P_B = 2, P_C = 4, P_D = 6, P_NSLOT = 8 };
+
+// Buffers of one PCG run: FP64 loop (x = the FP64 solution), or the FP32 loop with reliable updates
+// (x = the FP32 partial solution, x64/b64 the FP64 solution and right-hand side).
+struct PcgBufs {
+  const void* crs;
+  const void* dia;
+  void* r;
+  void* z;
+  void* p;
+  void* q;
+  void* x;
+  double* x64;
+  const double* b64;
+};
+
+// Start of a PCG solve on the system's own buffers with scalars/flags at sc/fl (16 doubles, 8 ints): x = 0,
+// guard on ||r0||, z = M^-1 r, p = z, rho. pcg_prepare first (mixed: r = (float) rhs). pcg_finish after the
+// loop stopped with flag `done` (1 converged, 2 max_iter): final fold (mixed) and the residual handle.
+PcgBufs pcg_bufs(Sys& s, bool mixed);
+int pcg_prepare(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st);
+int pcg_start(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double* sc, int* fl, hipStream_t st);
+int pcg_finish(Sys& s, const PcgBufs& v, bool mixed, int done, hipStream_t st);
+
 int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration);
 int solve_pbicgstab(Sys& s, double* conv_ratio, unsigned* iteration);
 // z = M^-1 src with the FP64 factor (src may equal z); kernels skip when *done != 0 (done may be null)

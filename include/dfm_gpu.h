@@ -153,6 +153,26 @@ int dfm_sys_last_solve_stats(dfm_sys sys, double* ms_total, double* ms_per_iter,
  * on the system's own stream: which = 0 SpMV FP64 (y = A x), 1 SpMV FP32, 2 ILU(0) apply FP64 (all phases of
  * the forward and backward sweeps), 3 ILU(0) apply FP32. Also reports the algorithmic bytes of one launch. */
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes);
+/* ---- batches: many systems solved together (serving, instance batches) ------------------------------
+ * A batch advances every system added to it with one launch per kernel for all running systems of one block
+ * size; each system keeps its own Krylov scalars, done flag and iteration count, so its iterates are those of
+ * LsSol::Solve_PCG (src/ls/solver_ls_iter.cpp:100-172) for DFM_FP64, or of the mixed-precision form of
+ * dfm_solve_pcg for DFM_MIXED, and do not depend on which other systems share the batch. No counterpart in
+ * the reference (single-threaded, one system at a time). Systems must be assembled and factored, belong to
+ * the batch's context, and stay alive while in the batch (dfm_sys_destroy takes a system out first). */
+typedef struct dfm_batch_s* dfm_batch;
+int dfm_batch_create(dfm_ctx ctx, int precision, int max_systems, dfm_batch* out);
+int dfm_batch_destroy(dfm_batch b);
+/* Start solving sys (residual handle = right-hand side, solution into the update handle). */
+int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter);
+/* Enqueue `iterations` PCG iterations for all running systems, then collect the finished ones. */
+int dfm_batch_step(dfm_batch b, int iterations, int* n_running);
+/* state: 0 running, 1 converged, 2 stopped at max_iter; conv_ratio/iteration as dfm_solve_pcg; seconds from
+ * dfm_batch_add to the step that found it finished (host clock). */
+int dfm_batch_status(dfm_batch b, dfm_sys sys, int* state, double* conv_ratio, unsigned* iteration,
+                     double* seconds);
+int dfm_batch_remove(dfm_batch b, dfm_sys sys);
+
 /* Device memory currently allocated by the library (bytes). */
 int dfm_device_bytes(long long* bytes);
 

@@ -8,7 +8,8 @@ error bounded by the residual tolerance)."""
 import numpy as np
 import pytest
 
-import meshgen as M
+import meshgen as M  # noqa: F401
+from fem_cases import KINDS, mesh as _mesh, setup
 from oracle import pyref as P
 
 pytestmark = pytest.mark.gpu
@@ -27,71 +28,13 @@ def ctx():
     c.close()
 
 
-LAM2, MU2 = 0.3 / (1 - 0.09), 1.0 / 2.6
-LAM3, MU3 = 0.3 / (1.3 * 0.4), 1.0 / 2.6
-
-
-def _mesh(ndim, small=True):
-    if ndim == 2:
-        return M.rect_tri(12, 7, 1.0, 0.4, 31) if small else M.rect_tri(40, 24, 1.0, 0.6, 31)
-    return M.box_tet(4, 3, 3, 1.0, 0.8, 0.7, 32) if small else M.box_tet(8, 6, 6, 1.0, 0.8, 0.7, 32)
-
-
-def _flags(coords, ndim, length):
-    x = coords.reshape(-1, ndim)[:, 0]
-    f = np.zeros((x.size, length), np.int32)
-    f[x == x.min()] = 1
-    return f.ravel()
-
-
-KINDS = [
-    ("poisson2d", 2, 1, 0, lambda r, n: np.stack([r.uniform(0.5, 2, n), np.full(n, 1.0)], 1), True, False),
-    ("poisson3d", 3, 1, 0, lambda r, n: np.stack([r.uniform(0.5, 2, n), np.full(n, 0.7)], 1), True, False),
-    ("diffusion2d", 2, 1, 5, lambda r, n: np.stack([np.full(n, 1.1), r.uniform(0.5, 2, n), np.full(n, 0.3)], 1),
-     True, True),
-    ("solid2d", 2, 2, 1, lambda r, n: np.stack([LAM2 * r.uniform(0.5, 2, n), np.full(n, MU2), np.ones(n),
-                                                np.zeros(n), np.full(n, -0.01)], 1), True, False),
-    ("solid3d", 3, 3, 2, lambda r, n: np.stack([np.full(n, LAM3), MU3 * r.uniform(0.5, 2, n), np.ones(n),
-                                                np.full(n, 0.01), np.zeros(n), np.full(n, -0.02)], 1), True, False),
-    ("stvk2d", 2, 2, 3, lambda r, n: np.stack([np.full(n, LAM2), MU2 * r.uniform(0.5, 2, n), np.ones(n),
-                                               np.zeros(n), np.full(n, -0.05)], 1), True, False),
-    ("stvk3d", 3, 3, 4, lambda r, n: np.stack([LAM3 * r.uniform(0.5, 2, n), np.full(n, MU3), np.ones(n),
-                                               np.zeros(n), np.zeros(n), np.full(n, -0.05)], 1), True, False),
-]
-
-
-def _setup(ctx, case, ordering, small=True):
-    name, ndim, length, kind, pfun, use_u, use_v = case
-    co, ln = _mesh(ndim, small)
-    n = co.size // ndim
-    nelem = ln.size // (ndim + 1)
-    rng = np.random.default_rng(17)
-    params = pfun(rng, nelem)
-    u = rng.uniform(-0.05, 0.05, n * length) if use_u else None
-    v = rng.uniform(-0.05, 0.05, n * length) if use_v else None
-    flags = _flags(co, ndim, length)
-    glob = (0.05, 0.6)
-    cpu = P.CpuSystem(ndim, co, ln, length)
-    cpu.assemble(kind, params, u, v, glob)
-    cpu.set_bc(flags)
-    gs = N.System(ctx, ndim, co, ln, length)
-    order = None
-    if ordering != N.ORDER_NATURAL:
-        order, _ = gs.make_ordering(ordering)
-        gs.set_ordering(order)
-    gs.set_bc(flags)
-    gs.set_field(u, v)
-    rn = gs.assemble(kind, params, glob=glob)
-    return cpu, gs, order, rn
-
-
 ORDERS = [("natural", 0), ("multicolor", 1), ("blockcolor", 2)]
 
 
 @pytest.mark.parametrize("case", KINDS, ids=[k[0] for k in KINDS])
 @pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
 def test_assembly_bit_exact(ctx, case, oname, ordering):
-    cpu, gs, order, rn = _setup(ctx, case, ordering)
+    cpu, gs, order, rn = setup(N, ctx, case, ordering)
     rp, ci = gs.pattern()
     assert np.array_equal(rp, cpu.rowptr) and np.array_equal(ci, cpu.colind)
     crs, dia = gs.get_matrix()
@@ -104,7 +47,7 @@ def test_assembly_bit_exact(ctx, case, oname, ordering):
 @pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
 @pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
 def test_ilu0_factor_bit_exact(ctx, case, oname, ordering):
-    cpu, gs, order, _ = _setup(ctx, case, ordering)
+    cpu, gs, order, _ = setup(N, ctx, case, ordering)
     gs.factor()
     f = cpu.ilu0(None if ordering == N.ORDER_NATURAL else order)
     g = gs.get_factor()
@@ -125,7 +68,7 @@ def test_ilu0_factor_bit_exact(ctx, case, oname, ordering):
 
 @pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
 def test_handle_ops(ctx, case):
-    cpu, gs, order, _ = _setup(ctx, case, N.ORDER_BLOCKCOLOR)
+    cpu, gs, order, _ = setup(N, ctx, case, N.ORDER_BLOCKCOLOR)
     m = cpu.nnode * cpu.len
     rng = np.random.default_rng(4)
     x, y = rng.normal(size=m), rng.normal(size=m)
@@ -148,7 +91,7 @@ def test_handle_ops(ctx, case):
 @pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
 @pytest.mark.parametrize("method", ["pcg", "pbicgstab"])
 def test_krylov_vs_oracle(ctx, case, oname, ordering, method):
-    cpu, gs, order, _ = _setup(ctx, case, ordering, small=False)
+    cpu, gs, order, _ = setup(N, ctx, case, ordering, small=False)
     f = cpu.ilu0(None if ordering == N.ORDER_NATURAL else order)
     ok, conv, it, x, _ = cpu.solve(f, method, 1e-10, 2000)
     gs.factor()
@@ -170,7 +113,7 @@ def test_krylov_vs_oracle(ctx, case, oname, ordering, method):
 
 @pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
 def test_mixed_precision_true_residual(ctx, case):
-    cpu, gs, order, _ = _setup(ctx, case, N.ORDER_BLOCKCOLOR, small=False)
+    cpu, gs, order, _ = setup(N, ctx, case, N.ORDER_BLOCKCOLOR, small=False)
     gs.factor()
     ok, conv, it = gs.solve_pcg(1e-8, 5000, N.FP64)
     x64 = gs.get_vector(-2)
@@ -193,7 +136,7 @@ def test_mixed_precision_true_residual(ctx, case):
 
 
 def test_pcg_not_converged_contract(ctx):
-    cpu, gs, order, _ = _setup(ctx, KINDS[0], N.ORDER_NATURAL, small=False)
+    cpu, gs, order, _ = setup(N, ctx, KINDS[0], N.ORDER_NATURAL, small=False)
     gs.factor()
     ok, conv, it = gs.solve_pcg(1e-14, 3)
     f = cpu.ilu0()
