@@ -347,3 +347,18 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def newton_batch(ctx, systems, kind, load_scale, nsteps, max_newton=40, newton_tol=1e-6, precision=MIXED,
+                 lin_tol=1e-6, lin_maxit=1000, glob=None):
+    """dfm_newton_batch: returns (newton_iters [nsys, nsteps], pcg_iters [nsys], unconverged_pcg [nsys])."""
+    n = len(systems)
+    arr = (C.c_void_p * n)(*[s.h.value for s in systems])
+    ls = _f64(load_scale)
+    nit = np.zeros(n * nsteps, np.int32)
+    lit = np.zeros(n, np.uint32)
+    lun = np.zeros(n, np.int32)
+    g = None if glob is None else _f64(glob)
+    _check(lib().dfm_newton_batch(ctx.h, n, arr, kind, _p(g), nsteps, _p(ls), max_newton, C.c_double(newton_tol),
+                                  precision, C.c_double(lin_tol), C.c_uint(lin_maxit), _p(nit), _p(lit), _p(lun)))
+    return nit.reshape(n, nsteps), lit, lun

@@ -434,6 +434,12 @@ int dfm_sys_set_loads(dfm_sys sys, int nface, const int* faces, const double* tr
   return DFM_OK;
 }
 
+int dfm_sys_set_load_scale(dfm_sys sys, double scale) {
+  NEED(sys, "dfm_sys_set_load_scale");
+  sys->s.load_scale = scale;
+  return DFM_OK;
+}
+
 int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double* res_norm) {
   NEED(sys, "dfm_sys_assemble_resident");
   Sys& s = sys->s;
@@ -463,6 +469,7 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   a.u = s.d_u.p; a.v = s.d_v.p;
   a.load_ptr = s.has_loads ? s.d_lptr.p : nullptr;
   a.loads = s.has_loads ? s.d_loads.p : nullptr;
+  a.load_scale = s.load_scale;
   a.rowptr = s.d_rowptr.p; a.colind = s.d_colind.p; a.crs = s.d_crs.p; a.dia = s.d_dia.p; a.res = s.d_res.p;
   DFM_HIP(launch_assemble(a, st));
   // FinalizeMarge (:92-128): Dirichlet rows/columns, residual zeroed at fixed dofs, ||r|| returned

@@ -146,6 +146,7 @@ struct Sys {
   DBuf<int> d_lptr;
   DBuf<double> d_loads;
   bool has_loads = false;
+  double load_scale = 1.0;
 
   // captured chunks of Krylov iterations (hipGraph), per loop kind: 0 PCG FP64, 1 PCG mixed, 2 PBiCGSTAB
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};

@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
   // surface loads (a reference extension): added after the element contributions, in face order
   if (A.load_ptr) {
     for (int k = A.load_ptr[i1]; k < A.load_ptr[i1 + 1]; k++)
-      for (int d = 0; d < B; d++) racc[d] += A.loads[(long long)k * B + d];
+      for (int d = 0; d < B; d++) racc[d] += A.load_scale * A.loads[(long long)k * B + d];
   }
   for (int d = 0; d < B; d++) res[d] = racc[d];
 }

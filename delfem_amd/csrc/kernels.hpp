@@ -23,6 +23,7 @@ struct AsmArgs {
   const double* v;
   const int* load_ptr;   // node -> surface-load entries (original numbering), may be null
   const double* loads;   // [nentries*len]
+  double load_scale;     // multiplies the surface loads (load stepping); 1 = as given
   const unsigned* rowptr;  // device numbering pattern
   const unsigned* colind;
   double* crs;

@@ -110,6 +110,8 @@ int dfm_sys_assemble(dfm_sys sys, int kind, const double* params, int npar, int 
 int dfm_sys_set_params(dfm_sys sys, const double* params, int npar, int pstride);
 int dfm_sys_set_loads(dfm_sys sys, int nface, const int* faces, const double* trac);
 int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double* res_norm);
+/* Factor applied to the stored surface loads by later assemblies (load stepping; 1 = as given). */
+int dfm_sys_set_load_scale(dfm_sys sys, double scale);
 /* Matrix values in the original numbering: crs[ncrs*len*len], dia[n*len*len]. */
 int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia);
 
@@ -172,6 +174,19 @@ int dfm_batch_step(dfm_batch b, int iterations, int* n_running);
 int dfm_batch_status(dfm_batch b, dfm_sys sys, int* state, double* conv_ratio, unsigned* iteration,
                      double* seconds);
 int dfm_batch_remove(dfm_batch b, dfm_sys sys);
+
+/* ---- load-stepped Newton for many instances (CEqn_Solid3D_Linear::Solve, geometric-nonlinear branch,
+ * src/femeqn/eqnsys_solid.cpp:174-209) ----------------------------------------------------------------------
+ * Each system has its parameters, loads, Dirichlet flags and (zero) field set. For load step s = 1..nsteps
+ * the surface loads are scaled by load_scale[i] * s / nsteps; then, per instance, up to max_newton iterations
+ * of: assemble tangent + residual (kind), ||r|| (iteration 0: stop if < 1e-20, else ||r_0||), stop when
+ * ||r|| < newton_tol * ||r_0||, ILU(0) factor, PCG (lin_tol, lin_maxit; the result is used whether or not it
+ * converged, as the reference does), u += du. The PCG solves of one Newton iteration of all instances run as
+ * one batch. Outputs (may be NULL): newton_iters[i*nsteps + s-1], total PCG iterations and unconverged PCG
+ * solves per instance. */
+int dfm_newton_batch(dfm_ctx ctx, int nsys, const dfm_sys* sys, int kind, const double* glob, int nsteps,
+                     const double* load_scale, int max_newton, double newton_tol, int precision, double lin_tol,
+                     unsigned lin_maxit, int* newton_iters, unsigned* lin_iters, int* lin_unconverged);
 
 /* Device memory currently allocated by the library (bytes). */
 int dfm_device_bytes(long long* bytes);
