@@ -69,6 +69,60 @@ def cpu_baseline(w, gpu_iters, budget_s):
     }
 
 
+def run_c5_batch(args, N, W):
+    """StVK cantilever (64x16x16 hexes x 6 tets, 56,355 dofs) as a batch of served instances with seeded load
+    magnitudes U[0.001, 0.02], end traction ramped over the load steps, Newton per step to ||r|| < 1e-6 ||r_0||
+    (the reference's criterion), tangent + residual re-assembled on the GPU in FP64 every Newton iteration,
+    inner PCG to the reference's conv_ratio 1e-6; all instances' linear solves of a Newton iteration as one
+    batch. One step of the bench = the whole load path of every instance."""
+    import numpy as np
+
+    w = W.make("c5", load=1.0)
+    ctx = N.Context(0)
+    t0 = time.perf_counter()
+    base = W.build_system(ctx, w, N.ORDER_BLOCKCOLOR)
+    systems = [base] + [base.clone() for _ in range(args.instances - 1)]
+    scales = 0.001 + (0.02 - 0.001) * N.rng_uniform(W.SEEDS["c5"], 7, args.instances)
+    log(f"[bench] c5 x{args.instances} ndof={base.ndof} setup {time.perf_counter() - t0:.1f}s")
+    res = {}
+    for prec, name in [(N.MIXED, "mixed"), (N.FP64, "fp64")]:
+        for s in systems:
+            s.set_field(None)
+            s.set_params(w["params"])
+            s.set_loads(w["faces"], w["trac"])
+        ctx.synchronize()
+        a = time.perf_counter()
+        nit, lit, lun = N.newton_batch(ctx, systems, N.STVK3D, scales, args.load_steps, precision=prec,
+                                       lin_tol=1e-6, lin_maxit=1000)
+        ctx.synchronize()
+        dt = time.perf_counter() - a
+        tip = float(np.mean([s.get_field().reshape(-1, 3)[:, 2].min() for s in systems[:4]]))
+        res[name] = dict(t=dt, newton=int(nit.sum()), pcg=int(lit.sum()), unconv=int(lun.sum()), tip=tip)
+        log(f"[bench] c5 {name}: {dt:.2f}s, newton iterations {nit.sum()} (max/step {nit.max()}), "
+            f"pcg iterations {lit.sum()}, unconverged pcg {lun.sum()}")
+    ndof = base.ndof
+    work = ndof * args.instances * args.load_steps
+    out = {
+        "metric": "DOF-solves/s (converged Newton load steps x dofs), StVK batch",
+        "value": work / res["mixed"]["t"],
+        "unit": "DOF-solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": res["mixed"]["t"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 assembly, f32+f64 PCG", "data": "synthetic",
+        "config": {"workload": f"c5: 3D StVK cantilever 64x16x16 hexes x 6 tets, {ndof} dofs, "
+                               f"{args.instances} instances, {args.load_steps} load steps",
+                   "instances": args.instances, "load_steps": args.load_steps, "newton_tol": 1e-6,
+                   "pcg_tol": 1e-6},
+        "fp64_value": work / res["fp64"]["t"],
+        "mixed": res["mixed"], "fp64": res["fp64"],
+    }
+    for s in systems:
+        s.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +133,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--instances", type=int, default=256, help="c5: instances in the batch")
+    ap.add_argument("--load-steps", type=int, default=10, help="c5: load steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every GPU box of this build has one; --gpus {args.gpus} is not "
@@ -88,6 +144,8 @@ def main():
     from delfem_amd import _native as N
     from delfem_amd import workloads as W
 
+    if args.config == "c5":
+        return run_c5_batch(args, N, W)
     t0 = time.perf_counter()
     w = W.make(args.config)
     ctx = N.Context(0)

@@ -149,6 +149,15 @@ class System:
         self.ncrs = nc.value
         _live.add(self)
 
+    def clone(self):
+        """A new system on the same mesh, ordering and Dirichlet flags (dfm_sys_clone)."""
+        c = System.__new__(System)
+        c.ctx, c.ndim, c.len, c.nnode, c.nelem, c.ncrs = self.ctx, self.ndim, self.len, self.nnode, self.nelem, self.ncrs
+        c.h = C.c_void_p()
+        _check(lib().dfm_sys_clone(self.h, C.byref(c.h)))
+        _live.add(c)
+        return c
+
     def close(self):
         if self.h:
             lib().dfm_sys_destroy(self.h)

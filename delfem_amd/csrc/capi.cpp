@@ -173,6 +173,41 @@ int dfm_ctx_synchronize(dfm_ctx ctx) {
   return DFM_OK;
 }
 
+// Device copies of the mesh, value arrays, vectors and scalars of a system whose host structure is set.
+static hipError_t sys_alloc_device(Sys& s) {
+  hipStream_t st = s.ctx->stream;
+  const long long m = mdof(s);
+  const int nnode = s.n;
+  hipError_t e;
+  if ((e = upload(s.d_coords, s.coords, st)) != hipSuccess) return e;
+  if ((e = upload(s.d_lnods, s.lnods, st)) != hipSuccess) return e;
+  if ((e = upload(s.d_elsup_ptr, s.elsup_ptr, st)) != hipSuccess) return e;
+  if ((e = upload(s.d_elsup, s.elsup, st)) != hipSuccess) return e;
+  if ((e = s.d_crs.alloc((size_t)s.ncrs * s.bs)) != hipSuccess) return e;
+  if ((e = s.d_dia.alloc((size_t)nnode * s.bs)) != hipSuccess) return e;
+  if ((e = s.d_res.alloc(m)) != hipSuccess) return e;
+  if ((e = s.d_upd.alloc(m)) != hipSuccess) return e;
+  if ((e = s.d_rhs.alloc(m)) != hipSuccess) return e;
+  if ((e = s.d_u.alloc(m)) != hipSuccess) return e;
+  if ((e = s.d_v.alloc(m)) != hipSuccess) return e;
+  if ((e = s.d_bc.alloc(m)) != hipSuccess) return e;
+  // per-block partial sums: up to 2 values per block of the widest reduction grid (SpMV rows at 32 lanes)
+  if ((e = s.d_partial.alloc(8 * 2048)) != hipSuccess) return e;
+  if ((e = s.d_counter.alloc(8)) != hipSuccess) return e;
+  if ((e = s.d_scal.alloc(16)) != hipSuccess) return e;
+  if ((e = s.d_iflag.alloc(8)) != hipSuccess) return e;
+  if ((e = hipHostMalloc((void**)&s.h_pinned, 16 * sizeof(int))) != hipSuccess) return e;
+  if ((e = hipHostMalloc((void**)&s.h_pinned_d, 16 * sizeof(double))) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(s.d_counter.p, 0, 8 * sizeof(unsigned), st)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(s.d_bc.p, 0, m * sizeof(int), st)) != hipSuccess) return e;
+  if ((e = launch_fill(s.d_u.p, 0.0, m, st)) != hipSuccess) return e;
+  if ((e = launch_fill(s.d_v.p, 0.0, m, st)) != hipSuccess) return e;
+  if ((e = launch_fill(s.d_res.p, 0.0, m, st)) != hipSuccess) return e;
+  if ((e = launch_fill(s.d_upd.p, 0.0, m, st)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
 int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, int nelem, const int* lnods, int len,
                         dfm_sys* out) {
   NEED(ctx, "dfm_sys_create_mesh");
@@ -191,8 +226,6 @@ int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, 
   s.lnods.assign(lnods, lnods + (size_t)nelem * npoel);
   build_pattern(nnode, nelem, npoel, lnods, s.rowptr0, s.colind0, s.elsup_ptr, s.elsup);
   s.ncrs = s.rowptr0[nnode];
-  hipStream_t st = s.ctx->stream;
-  const long long m = mdof(s);
   std::vector<int> ident(nnode);
   std::iota(ident.begin(), ident.end(), 0);
   int rc = apply_order(s, ident, {}, {}, 0);
@@ -202,36 +235,52 @@ int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, 
     delete h;
     return e == hipErrorOutOfMemory ? DFM_ERR_NOMEM : DFM_ERR_HIP;
   };
-  hipError_t e;
-  if ((e = upload(s.d_coords, s.coords, st)) != hipSuccess) return fail(e);
-  if ((e = upload(s.d_lnods, s.lnods, st)) != hipSuccess) return fail(e);
-  if ((e = upload(s.d_elsup_ptr, s.elsup_ptr, st)) != hipSuccess) return fail(e);
-  if ((e = upload(s.d_elsup, s.elsup, st)) != hipSuccess) return fail(e);
-  if ((e = s.d_crs.alloc((size_t)s.ncrs * s.bs)) != hipSuccess) return fail(e);
-  if ((e = s.d_dia.alloc((size_t)nnode * s.bs)) != hipSuccess) return fail(e);
-  if ((e = s.d_res.alloc(m)) != hipSuccess) return fail(e);
-  if ((e = s.d_upd.alloc(m)) != hipSuccess) return fail(e);
-  if ((e = s.d_rhs.alloc(m)) != hipSuccess) return fail(e);
-  if ((e = s.d_u.alloc(m)) != hipSuccess) return fail(e);
-  if ((e = s.d_v.alloc(m)) != hipSuccess) return fail(e);
-  if ((e = s.d_bc.alloc(m)) != hipSuccess) return fail(e);
-  // per-block partial sums: up to 2 values per block of the widest reduction grid (SpMV rows at 32 lanes)
-  if ((e = s.d_partial.alloc(8 * 2048)) != hipSuccess)
-    return fail(e);
-  if ((e = s.d_counter.alloc(8)) != hipSuccess) return fail(e);
-  if ((e = s.d_scal.alloc(16)) != hipSuccess) return fail(e);
-  if ((e = s.d_iflag.alloc(8)) != hipSuccess) return fail(e);
-  if ((e = hipHostMalloc((void**)&s.h_pinned, 16 * sizeof(int))) != hipSuccess) return fail(e);
-  if ((e = hipHostMalloc((void**)&s.h_pinned_d, 16 * sizeof(double))) != hipSuccess) return fail(e);
-  if ((e = hipMemsetAsync(s.d_counter.p, 0, 8 * sizeof(unsigned), st)) != hipSuccess) return fail(e);
-  if ((e = hipMemsetAsync(s.d_bc.p, 0, m * sizeof(int), st)) != hipSuccess) return fail(e);
-  if ((e = launch_fill(s.d_u.p, 0.0, m, st)) != hipSuccess) return fail(e);
-  if ((e = launch_fill(s.d_v.p, 0.0, m, st)) != hipSuccess) return fail(e);
-  if ((e = launch_fill(s.d_res.p, 0.0, m, st)) != hipSuccess) return fail(e);
-  if ((e = launch_fill(s.d_upd.p, 0.0, m, st)) != hipSuccess) return fail(e);
-  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);
+  const hipError_t e = sys_alloc_device(s);
+  if (e != hipSuccess) return fail(e);
   h->owner = ctx;
   ctx->c.refs++;
+  *out = h;
+  return DFM_OK;
+}
+
+static void copy_sweep_host(Sweep& d, const Sweep& a) {
+  d.phase_gptr = a.phase_gptr; d.phase_nlev = a.phase_nlev; d.grp = a.grp; d.rowlist = a.rowlist; d.lev = a.lev;
+  d.gmeta = a.gmeta; d.srow = a.srow; d.scnt = a.scnt; d.ecol = a.ecol; d.esrc = a.esrc; d.nslab = a.nslab;
+}
+
+int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
+  NEED(src, "dfm_sys_clone");
+  NEED(out, "dfm_sys_clone");
+  const Sys& a = src->s;
+  dfm_sys h = new dfm_sys_s;
+  Sys& s = h->s;
+  s.ctx = a.ctx;
+  s.ndim = a.ndim; s.n = a.n; s.len = a.len; s.bs = a.bs; s.npoel = a.npoel; s.nelem = a.nelem; s.ncrs = a.ncrs;
+  s.coords = a.coords; s.lnods = a.lnods; s.rowptr0 = a.rowptr0; s.colind0 = a.colind0;
+  s.elsup_ptr = a.elsup_ptr; s.elsup = a.elsup;
+  s.order = a.order; s.inv = a.inv; s.natural = a.natural; s.ncolors = a.ncolors;
+  s.color_of = a.color_of; s.tile_of = a.tile_of;
+  s.rowptr = a.rowptr; s.colind = a.colind; s.diaind = a.diaind; s.lsplit = a.lsplit; s.usplit = a.usplit;
+  s.perm_crs = a.perm_crs;
+  copy_sweep_host(s.fwd, a.fwd);
+  copy_sweep_host(s.bwd, a.bwd);
+  auto fail = [&](hipError_t e) {
+    set_error(std::string("dfm_sys_clone: ") + hipGetErrorString(e));
+    delete h;
+    return e == hipErrorOutOfMemory ? DFM_ERR_NOMEM : DFM_ERR_HIP;
+  };
+  const int rc = upload_structure(s);
+  if (rc != DFM_OK) {
+    delete h;
+    return rc;
+  }
+  hipError_t e = sys_alloc_device(s);
+  if (e != hipSuccess) return fail(e);
+  e = hipMemcpyAsync(s.d_bc.p, a.d_bc.p, sizeof(int) * mdof(s), hipMemcpyDeviceToDevice, s.ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s.ctx->stream);
+  if (e != hipSuccess) return fail(e);
+  h->owner = src->owner;
+  if (h->owner) h->owner->c.refs++;
   *out = h;
   return DFM_OK;
 }

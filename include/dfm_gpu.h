@@ -78,6 +78,10 @@ int dfm_rng_normal(unsigned long long seed, unsigned long long stream, size_t n,
 int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, int nelem, const int* lnods,
                         int len, dfm_sys* out);
 int dfm_sys_destroy(dfm_sys sys);
+/* A new system on the same mesh, ordering, ILU schedule and Dirichlet flags as src (no symbolic work; field,
+ * parameters, loads and values are not copied). No counterpart in the reference: instances of one geometry
+ * (Newton batches, repeated serving geometries). */
+int dfm_sys_clone(dfm_sys src, dfm_sys* out);
 int dfm_sys_info(dfm_sys sys, int* nblk, int* len, long long* ncrs);
 /* The CRS pattern in the original numbering (rowptr[n+1], colind[ncrs]; CMat_BlkCrs::GetPtrIndPSuP). */
 int dfm_sys_get_pattern(dfm_sys sys, unsigned* rowptr, unsigned* colind);

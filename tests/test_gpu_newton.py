@@ -51,16 +51,16 @@ def _problem():
 
 
 def _gpu_systems(ctx, n):
+    """One system built from the mesh, the others cloned from it (same ordering and Dirichlet flags)."""
     co, tet, bc, faces, trac = _problem()
-    out = []
-    for _ in range(n):
-        s = N.System(ctx, 3, co, tet, 3)
-        order, _ = s.make_ordering(N.ORDER_BLOCKCOLOR)
-        s.set_ordering(order)
-        s.set_bc(bc)
-        s.set_params(PARAMS)
-        s.set_loads(faces, trac)
-        out.append(s)
+    s = N.System(ctx, 3, co, tet, 3)
+    order, _ = s.make_ordering(N.ORDER_BLOCKCOLOR)
+    s.set_ordering(order)
+    s.set_bc(bc)
+    out = [s] + [s.clone() for _ in range(n - 1)]
+    for t in out:
+        t.set_params(PARAMS)
+        t.set_loads(faces, trac)
     return out
 
 
