@@ -184,9 +184,12 @@ def main():
             ok_all &= ok
         true_rel = s.true_residual()
         st = s.last_solve_stats()
-        res[name] = dict(lat=lat, its=its, ok=ok_all, conv=conv, true_rel=true_rel, stats=st)
+        nrel, fell_back = s.last_solve_info()
+        res[name] = dict(lat=lat, its=its, ok=ok_all, conv=conv, true_rel=true_rel, stats=st, nrel=nrel,
+                         fell_back=fell_back)
         log(f"[bench] {name}: {statistics.median(lat) * 1e3:.1f} ms/solve, iters {its[-1]}, conv {conv:.2e}, "
-            f"true rel residual {true_rel:.2e}, solve {st['ms']:.1f} ms, {st['ms_per_iter'] * 1e3:.1f} us/iter")
+            f"true rel residual {true_rel:.2e}, solve {st['ms']:.1f} ms, {st['ms_per_iter'] * 1e3:.1f} us/iter"
+            + (f", reliable updates {nrel}" if name == "mixed" else "") + (", fell back to FP64" if fell_back else ""))
 
     # dominant kernel of the mixed path: fused SpMV + dot in FP32 (timed with HIP events on the lib's stream)
     spmv32_ms, spmv32_b = s.bench_kernel(1, 50)
@@ -205,6 +208,8 @@ def main():
     value = ndof * K / t_mixed
     value64 = ndof * K / t_fp64
     lat_sorted = sorted(mixed["lat"])
+    # the plain-FP64 configuration reports its FP64 path
+    head = fp64 if args.config == "c1" else mixed
 
     def pct(v, q):
         if len(v) == 1:
@@ -214,6 +219,8 @@ def main():
         hi = min(lo + 1, len(v) - 1)
         return v[lo] + (v[hi] - v[lo]) * (i - lo)
 
+    if head is fp64:
+        value, t_mixed, lat_sorted = value64, t_fp64, sorted(fp64["lat"])
     out = {
         "metric": METRIC,
         "value": value,
@@ -225,7 +232,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if head is fp64 else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": (f"{w['name']}: " + {"c1": "2D plane-stress elasticity P1, 128x32 quads",
@@ -251,6 +258,9 @@ def main():
         "true_rel_residual_mixed": mixed["true_rel"],
         "true_rel_residual_fp64": fp64["true_rel"],
         "converged": bool(mixed["ok"] and fp64["ok"]),
+        "mixed_reliable_updates": mixed["nrel"],
+        "mixed_fell_back_to_fp64": mixed["fell_back"],
+        "value_path": "fp64" if head is fp64 else "mixed",
         "ms_per_pcg_iter_mixed": mixed["stats"]["ms_per_iter"],
         "ms_per_pcg_iter_fp64": fp64["stats"]["ms_per_iter"],
         "solve_ms_mixed": mixed["stats"]["ms"],

@@ -25,7 +25,7 @@ _live = weakref.WeakSet()  # systems and contexts still open, closed in order at
 def _close_all():
     objs = list(_live)
     for o in objs:
-        if isinstance(o, Batch):
+        if isinstance(o, (Batch, Server)):
             o.close()
     for o in objs:
         if isinstance(o, System):
@@ -314,6 +314,12 @@ class System:
         _check(lib().dfm_sys_true_residual(self.h, C.byref(rel)))
         return rel.value
 
+    def last_solve_info(self):
+        """(reliable updates, fell back to FP64) of the last mixed-precision solve."""
+        a, b = C.c_int(), C.c_int()
+        _check(lib().dfm_sys_last_solve_info(self.h, C.byref(a), C.byref(b)))
+        return a.value, bool(b.value)
+
     def last_solve_stats(self):
         a, b, c = C.c_double(), C.c_double(), C.c_uint()
         _check(lib().dfm_sys_last_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
@@ -343,6 +349,12 @@ class Batch:
         _check(lib().dfm_batch_status(self.h, sys.h, C.byref(st), C.byref(conv), C.byref(it), C.byref(sec)))
         return st.value, conv.value, it.value, sec.value
 
+    def info(self, sys):
+        """(reliable updates, fell back to FP64)."""
+        a, b = C.c_int(), C.c_int()
+        _check(lib().dfm_batch_info(self.h, sys.h, C.byref(a), C.byref(b)))
+        return a.value, bool(b.value)
+
     def remove(self, sys):
         _check(lib().dfm_batch_remove(self.h, sys.h))
 
@@ -371,3 +383,74 @@ def newton_batch(ctx, systems, kind, load_scale, nsteps, max_newton=40, newton_t
     _check(lib().dfm_newton_batch(ctx.h, n, arr, kind, _p(g), nsteps, _p(ls), max_newton, C.c_double(newton_tol),
                                   precision, C.c_double(lin_tol), C.c_uint(lin_maxit), _p(nit), _p(lit), _p(lun)))
     return nit.reshape(n, nsteps), lit, lun
+
+
+class Request(C.Structure):
+    """dfm_request: one problem submitted to a Server (arrays must stay alive until submit returns)."""
+    _fields_ = [("ndim", C.c_int), ("len", C.c_int), ("kind", C.c_int), ("nnode", C.c_int), ("nelem", C.c_int),
+                ("coords", C.c_void_p), ("lnods", C.c_void_p), ("params", C.c_void_p), ("npar", C.c_int),
+                ("pstride", C.c_int), ("glob", C.c_void_p), ("bc", C.c_void_p), ("nface", C.c_int),
+                ("faces", C.c_void_p), ("trac", C.c_void_p), ("tol", C.c_double), ("max_iter", C.c_uint)]
+
+
+REQ_QUEUED, REQ_SETUP, REQ_SOLVING, REQ_CONVERGED, REQ_MAXITER, REQ_FAILED = range(6)
+
+
+class Server:
+    """Continuous-batching solve server (dfm_server_*)."""
+
+    def __init__(self, ctx, precision=MIXED, budget_bytes=64 << 30, max_active=64, nworkers=8):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().dfm_server_create(ctx.h, precision, C.c_longlong(budget_bytes), max_active, nworkers,
+                                       C.byref(self.h)))
+        _live.add(self)
+
+    def submit(self, ndim, length, kind, coords, lnods, params, bc, tol=1e-8, max_iter=20000, glob=None,
+               faces=None, trac=None):
+        coords, lnods, bc = _f64(coords), _i32(lnods), _i32(bc)
+        params = _f64(params)
+        npar, pstride = (params.size, 0) if params.ndim == 1 else (params.shape[1], params.shape[1])
+        g = None if glob is None else _f64(glob)
+        faces = None if faces is None else _i32(faces)
+        trac = None if trac is None else _f64(trac)
+        q = Request(ndim, length, kind, coords.size // ndim, lnods.size // (ndim + 1), coords.ctypes.data,
+                    lnods.ctypes.data, params.ctypes.data, npar, pstride, None if g is None else g.ctypes.data,
+                    bc.ctypes.data, 0 if faces is None else faces.size // 3,
+                    None if faces is None else faces.ctypes.data, None if trac is None else trac.ctypes.data,
+                    tol, max_iter)
+        t = C.c_longlong()
+        _check(lib().dfm_server_submit(self.h, C.byref(q), C.byref(t)))
+        return t.value
+
+    def progress(self, iterations=16, wait_ms=2):
+        a, q = C.c_int(), C.c_int()
+        _check(lib().dfm_server_progress(self.h, iterations, wait_ms, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def poll(self, ticket):
+        """dict(state, latency_s, solve_s, iters, conv, fell_back)."""
+        st, lat, sol, it, conv, fb = C.c_int(), C.c_double(), C.c_double(), C.c_uint(), C.c_double(), C.c_int()
+        _check(lib().dfm_server_poll(self.h, C.c_longlong(ticket), C.byref(st), C.byref(lat), C.byref(sol),
+                                     C.byref(it), C.byref(conv), C.byref(fb)))
+        return dict(state=st.value, latency_s=lat.value, solve_s=sol.value, iters=it.value, conv=conv.value,
+                    fell_back=bool(fb.value))
+
+    def result(self, ticket, ndof):
+        u = np.empty(ndof)
+        _check(lib().dfm_server_result(self.h, C.c_longlong(ticket), _p(u)))
+        return u
+
+    def release(self, ticket):
+        _check(lib().dfm_server_release(self.h, C.c_longlong(ticket)))
+
+    def close(self):
+        if self.h:
+            lib().dfm_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -194,6 +194,12 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
       fl[F_DONE] = 1;
       return;
     }
+    if (stall_update(rt, sc[S_TBEST], fl[F_STALL])) {
+      sc[S_CONV] = sqrt(ratio2);
+      sc[S_RR] = rt;
+      fl[F_DONE] = 3;
+      return;
+    }
     rmax = rt;
   }
   if (it + 1 >= fl[F_MAXIT]) {
@@ -201,10 +207,14 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
     fl[F_DONE] = 2;
     return;
   }
-  const bool due = rr * sc[S_INV0] < sc[S_TOL2] || rr < sc[S_DELTA2] * rmax;
+  const bool due = rr * sc[S_INV0] < sc[S_TOL2] || rr < sc[S_DELTA2] * rmax ||
+                   it - (int)sc[S_LASTUPD] >= kForceUpdate;
   sc[S_RMAX] = due ? rr : fmax(rmax, rr);
   fl[F_PNEXT] = due ? 1 : 0;
-  if (due) fl[F_NREL] = fl[F_NREL] + 1;
+  if (due) {
+    fl[F_NREL] = fl[F_NREL] + 1;
+    sc[S_LASTUPD] = (double)it;
+  }
 }
 
 template <int B, typename T, typename A>
@@ -297,6 +307,8 @@ __global__ void __launch_bounds__(NT) k_b_pupdate(const SegDev* __restrict__ seg
 struct BatchSeg {
   Sys* s = nullptr;
   int slot = -1;
+  bool mixed = false;  // FP32 Krylov loop with reliable updates; a stalled one restarts in FP64
+  bool fell_back = false;
   PcgBufs v{};
   double tol = 0;
   unsigned maxit = 0;
@@ -326,8 +338,9 @@ struct Batch {
     DBuf<int2> d_fent, d_bent;
     std::vector<int> fptr, bptr;  // per phase offsets into the entry tables
     int nsb = 0, nvb = 0;
+    bool mixed = false;
     DBuf<double> d_pq, d_true, d_rr, d_rz;
-  } grp[3];
+  } grp[6];  // (block size, precision): index (B - 1) + 3 * mixed
   ~Batch() {
     for (auto& g : grp) {
       g.d_segs.release(); g.d_sblk.release(); g.d_vblk.release(); g.d_fent.release(); g.d_bent.release();
@@ -352,12 +365,13 @@ static void grow(DBuf<T>& b, size_t n, hipError_t& e) {
 
 static int rebuild(Batch& bt) {
   hipStream_t st = bt.ctx->stream;
-  for (int gi = 0; gi < 3; gi++) bt.grp[gi].members.clear();
+  for (auto& g : bt.grp) g.members.clear();
   for (int k = 0; k < (int)bt.segs.size(); k++)
-    if (bt.segs[k].state == 0) bt.grp[bt.segs[k].s->len - 1].members.push_back(k);
-  for (int gi = 0; gi < 3; gi++) {
+    if (bt.segs[k].state == 0) bt.grp[bt.segs[k].s->len - 1 + (bt.segs[k].mixed ? 3 : 0)].members.push_back(k);
+  for (int gi = 0; gi < 6; gi++) {
     Batch::Group& g = bt.grp[gi];
-    g.B = gi + 1;
+    g.B = gi % 3 + 1;
+    g.mixed = gi >= 3;
     const int tpr = g.B == 3 ? tpr_of<3>() : tpr_of<1>();
     const int rs = NT / tpr;
     std::vector<SegDev> h;
@@ -378,9 +392,9 @@ static int rebuild(Batch& bt) {
       d.dia64 = s.d_dia.p;
       d.fa = SlabArgs{s.fwd.d_gmeta.p, s.fwd.d_srow.p, s.fwd.d_scnt.p, s.fwd.d_ecol.p};
       d.ba = SlabArgs{s.bwd.d_gmeta.p, s.bwd.d_srow.p, s.bwd.d_scnt.p, s.bwd.d_ecol.p};
-      d.fval = bt.mixed ? (const void*)s.fwd.d_eval32.p : (const void*)s.fwd.d_eval.p;
-      d.fdval = bt.mixed ? (const void*)s.fwd.d_dval32.p : (const void*)s.fwd.d_dval.p;
-      d.bval = bt.mixed ? (const void*)s.bwd.d_eval32.p : (const void*)s.bwd.d_eval.p;
+      d.fval = sg.mixed ? (const void*)s.fwd.d_eval32.p : (const void*)s.fwd.d_eval.p;
+      d.fdval = sg.mixed ? (const void*)s.fwd.d_dval32.p : (const void*)s.fwd.d_dval.p;
+      d.bval = sg.mixed ? (const void*)s.bwd.d_eval32.p : (const void*)s.bwd.d_eval.p;
       d.r = sg.v.r;
       d.z = sg.v.z;
       d.p = sg.v.p;
@@ -467,16 +481,35 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
   return hipGetLastError();
 }
 
+// A mixed-precision segment whose true residual stalled restarts as an FP64 solve of the same system (its
+// residual handle still holds the right-hand side).
+static int restart_fp64(Batch& bt, BatchSeg& sg) {
+  Sys& s = *sg.s;
+  sg.mixed = false;
+  sg.fell_back = true;
+  sg.nrel = bt.h_fl[8 * sg.slot + F_NREL];
+  int rc = sys_solve_setup(s, DFM_FP64);
+  if (rc != DFM_OK) return rc;
+  if (s.ctx != bt.ctx) DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  s.prepared = -1;
+  sg.v = pcg_bufs(s, false);
+  rc = pcg_start(s, sg.v, false, sg.tol, sg.maxit, bt.d_sc.p + 16 * (size_t)sg.slot, bt.d_fl.p + 8 * (size_t)sg.slot,
+                 bt.ctx->stream);
+  bt.dirty = true;
+  return rc;
+}
+
 static int finish_seg(Batch& bt, BatchSeg& sg) {
   const int* fl = bt.h_fl + 8 * sg.slot;
   const double* sc = bt.h_sc + 16 * sg.slot;
   const int done = fl[F_DONE];
+  if (done == 3) return restart_fp64(bt, sg);
   sg.state = done == 1 ? 1 : 2;
   sg.conv = done ? sc[S_CONV] : std::sqrt(sc[S_RR] * sc[S_INV0]);
   sg.iters = done == 1 ? (unsigned)fl[F_ITOUT] : sg.maxit;
-  sg.nrel = fl[F_NREL];
+  if (!sg.fell_back) sg.nrel = fl[F_NREL];
   sg.t_done = now_s();
-  const int rc = pcg_finish(*sg.s, sg.v, bt.mixed, done, bt.ctx->stream);
+  const int rc = pcg_finish(*sg.s, sg.v, sg.mixed, done, bt.ctx->stream);
   bt.dirty = true;
   return rc;
 }
@@ -486,11 +519,15 @@ static int poll(Batch& bt) {
   DFM_HIP(hipMemcpyAsync(bt.h_fl, bt.d_fl.p, sizeof(int) * 8 * bt.cap, hipMemcpyDeviceToHost, st));
   DFM_HIP(hipMemcpyAsync(bt.h_sc, bt.d_sc.p, sizeof(double) * 16 * bt.cap, hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
+  bool any = false;
   for (auto& sg : bt.segs)
     if (sg.state == 0 && bt.h_fl[8 * sg.slot + F_DONE]) {
       const int rc = finish_seg(bt, sg);
       if (rc != DFM_OK) return rc;
+      any = sg.state != 0 || any;
     }
+  // finished systems are read through their own context's stream: complete the batch stream's work on them
+  if (any) DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
 
@@ -569,15 +606,23 @@ int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
   DFM_CHECK(b && sys, DFM_ERR_ARG, "dfm_batch_add: null argument");
   Batch& bt = b->b;
   Sys& s = sys->s;
-  DFM_CHECK(s.ctx == bt.ctx, DFM_ERR_ARG, "dfm_batch_add: the system belongs to another context");
+  DFM_CHECK(s.ctx->device == bt.ctx->device, DFM_ERR_ARG, "dfm_batch_add: the system is on another device");
   DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_batch_add: the system is already in a batch");
   DFM_CHECK(!bt.free_slots.empty(), DFM_ERR_STATE, "dfm_batch_add: batch full");
   DFM_CHECK(max_iter > 0, DFM_ERR_ARG, "dfm_batch_add: max_iter must be positive");
-  int rc = sys_solve_setup(s, bt.mixed ? DFM_MIXED : DFM_FP64);
-  if (rc != DFM_OK) return rc;
+  const int prec = bt.mixed ? DFM_MIXED : DFM_FP64;
+  int rc = DFM_OK;
+  if (s.prepared != prec) {
+    rc = sys_solve_setup(s, prec);
+    if (rc != DFM_OK) return rc;
+    // prepared on another context's stream: the data must be complete before the batch stream reads it
+    if (s.ctx != bt.ctx) DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  }
+  s.prepared = -1;  // the right-hand side copy is consumed by this solve
   BatchSeg sg;
   sg.s = &s;
   sg.slot = bt.free_slots.back();
+  sg.mixed = bt.mixed;
   sg.v = pcg_bufs(s, bt.mixed);
   sg.tol = tol;
   sg.maxit = max_iter;
@@ -610,7 +655,7 @@ int dfm_batch_step(dfm_batch b, int iterations, int* n_running) {
     for (auto& g : bt.grp) {
       if (g.members.empty()) continue;
       hipError_t e;
-      if (bt.mixed)
+      if (g.mixed)
         e = g.B == 1 ? iterate_group<1, float, true>(g, st)
                      : g.B == 2 ? iterate_group<2, float, true>(g, st) : iterate_group<3, float, true>(g, st);
       else
@@ -638,6 +683,16 @@ int dfm_batch_status(dfm_batch b, dfm_sys sys, int* state, double* conv_ratio, u
   if (conv_ratio) *conv_ratio = sg.conv;
   if (iteration) *iteration = sg.iters;
   if (seconds) *seconds = sg.state ? sg.t_done - sg.t_add : now_s() - sg.t_add;
+  return DFM_OK;
+}
+
+int dfm_batch_info(dfm_batch b, dfm_sys sys, int* reliable_updates, int* fell_back) {
+  DFM_CHECK(b && sys, DFM_ERR_ARG, "dfm_batch_info: null argument");
+  Batch& bt = b->b;
+  const int k = find_seg(bt, &sys->s);
+  DFM_CHECK(k >= 0, DFM_ERR_ARG, "dfm_batch_info: the system is not in this batch");
+  if (reliable_updates) *reliable_updates = bt.segs[k].nrel;
+  if (fell_back) *fell_back = bt.segs[k].fell_back ? 1 : 0;
   return DFM_OK;
 }
 

@@ -526,6 +526,7 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
   s.assembled = true;
   s.factored = s.factored32 = false;
+  s.prepared = -1;
   if (res_norm) {
     DFM_HIP(launch_dot(s.d_res.p, s.d_res.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + 15, st));
     DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -671,6 +672,7 @@ int dfm_sys_set_vector(dfm_sys sys, int iv, const double* in) {
   for (int i0 = 0; i0 < s.n; i0++)
     for (int d = 0; d < s.len; d++) v[(size_t)i0 * s.len + d] = in[(size_t)s.order[i0] * s.len + d];
   DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  s.prepared = -1;
   DFM_HIP(hipMemcpy(a, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
   return DFM_OK;
 }
@@ -695,6 +697,7 @@ int dfm_prec_factor(dfm_sys sys) {
   DFM_HIP(hipStreamSynchronize(st));
   nsing = s.h_pinned[8];
   s.factored32 = false;
+  s.prepared = -1;
   // slab copies of the factor for the sweep kernels
   for (int dir = 0; dir < 2; dir++) {
     Sweep& w = dir == 0 ? s.fwd : s.bwd;
@@ -782,6 +785,7 @@ int sys_solve_setup(Sys& s, int precision) {
     if (rc != DFM_OK) return rc;
   }
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, mdof(s), s.ctx->stream));
+  s.prepared = precision;
   return DFM_OK;
 }
 }  // namespace dfm
@@ -796,6 +800,7 @@ int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iter
   DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_solve_pcg: the system is being solved in a batch");
   const int rc = sys_solve_setup(s, precision);
   if (rc != DFM_OK) return rc;
+  s.prepared = -1;  // consumed by this solve
   return solve_pcg(s, precision, conv_ratio, iteration);
 }
 
@@ -848,6 +853,13 @@ int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, doubl
 int dfm_device_bytes(long long* bytes) {
   NEED(bytes, "dfm_device_bytes");
   *bytes = device_bytes_in_use();
+  return DFM_OK;
+}
+
+int dfm_sys_last_solve_info(dfm_sys sys, int* reliable_updates, int* fell_back) {
+  NEED(sys, "dfm_sys_last_solve_info");
+  if (reliable_updates) *reliable_updates = sys->s.last_nrel;
+  if (fell_back) *fell_back = sys->s.last_fallback;
   return DFM_OK;
 }
 

@@ -70,6 +70,24 @@ __device__ inline void consume(const double* __restrict__ slot, int G, double (&
   block_allsum<NT, K>(tot);
 }
 
+// Mixed-precision stagnation test, called with each FP64 true residual rt (squared norm) that did not meet the
+// tolerance: progress means at least halving the norm against the best one so far. With an FP32 matrix the
+// recurrence solves A32 x = b; when cond(A) * eps32 is not small the true residual of A64 stops falling while
+// reliable updates keep coming. Returns true after kStallUpdates updates in a row without progress.
+constexpr int kStallUpdates = 6;
+// A reliable update is forced after this many iterations without one, so that a recursive residual that has
+// stopped falling (an FP32 plateau) still reaches the stagnation test.
+constexpr int kForceUpdate = 200;
+__device__ inline bool stall_update(double rt, double& tbest, int& stall) {
+  if (tbest == 0.0 || rt < 0.25 * tbest) {
+    tbest = rt;
+    stall = 0;
+    return false;
+  }
+  stall = stall + 1;
+  return stall >= kStallUpdates;
+}
+
 inline unsigned vec_blocks(long long m, int nt) {
   long long b = (m + nt - 1) / nt;
   if (b < 1) b = 1;

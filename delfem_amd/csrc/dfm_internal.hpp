@@ -157,7 +157,9 @@ struct Sys {
   double last_ms = 0.0, last_ms_iter = 0.0;
   unsigned last_inner = 0;
   int last_nrel = 0;  // reliable updates in the last mixed solve
+  int last_fallback = 0;  // the last mixed solve stalled and was repeated in FP64
   struct Batch* batch = nullptr;  // the batch the system is being solved in, if any
+  int prepared = -1;              // precision sys_solve_setup last prepared a solve for (-1: none pending)
 
   ~Sys();
 };

@@ -30,6 +30,7 @@ __global__ void k_solver_init(double* sc, int* fl, double tol2, int maxit, doubl
     fl[F_NREL] = 0;
     fl[F_PCUR] = 0;
     fl[F_PNEXT] = 0;
+    fl[F_STALL] = 0;
   }
 }
 
@@ -238,6 +239,12 @@ __global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r,
       }
       return;
     }
+    // block 0 alone keeps the stagnation state; the other blocks stop at the done flag of the next kernel
+    if (blockIdx.x == 0 && threadIdx.x == 0 && stall_update(rt[0], sc[S_TBEST], fl[F_STALL])) {
+      sc[S_CONV] = sqrt(ratio2);
+      sc[S_RR] = rt[0];
+      fl[F_DONE] = 3;
+    }
     rmax = rt[0];
   }
   if (it + 1 >= fl[F_MAXIT]) {
@@ -248,12 +255,16 @@ __global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r,
     }
     return;
   }
-  const bool due = rr[0] * sc[S_INV0] < sc[S_TOL2] || rr[0] < sc[S_DELTA2] * rmax;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const bool due = rr[0] * sc[S_INV0] < sc[S_TOL2] || rr[0] < sc[S_DELTA2] * rmax ||
+                     it - (int)sc[S_LASTUPD] >= kForceUpdate;
     sc[S_RR] = rr[0];
     sc[S_RMAX + ((it + 1) & 1)] = due ? rr[0] : fmax(rmax, rr[0]);
     fl[F_PNEXT] = due ? 1 : 0;
-    if (due) fl[F_NREL] = fl[F_NREL] + 1;
+    if (due) {
+      fl[F_NREL] = fl[F_NREL] + 1;
+      sc[S_LASTUPD] = (double)it;
+    }
   }
   double v[1] = {0.0};
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT)
@@ -593,7 +604,7 @@ int pcg_prepare(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
 
 int pcg_finish(Sys& s, const PcgBufs& v, bool mixed, int done, hipStream_t st) {
   const long long m = (long long)s.n * s.len;
-  if (!mixed) return DFM_OK;
+  if (!mixed || done == 3) return DFM_OK;  // stalled: the residual handle keeps the right-hand side for FP64
   // stopped by max_iter: the FP64 solution takes the partial solution not yet folded (when converged, the
   // FP64 solution is the one whose true residual was confirmed)
   if (done != 1) hipLaunchKernelGGL(k_mx_final_fold, vgrid(m), dim3(NT), 0, st, m, (float*)v.x, v.x64);
@@ -674,10 +685,19 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   int rc;
   const bool mixed = precision == DFM_MIXED;
   const PcgBufs v = pcg_bufs(s, mixed);
+  s.last_fallback = 0;
   rc = pcg_prepare(s, v, mixed, st);
   if (rc == DFM_OK)
     rc = mixed ? pcg_run<float>(s, v, true, tol, maxit, conv, it, ok) : pcg_run<double>(s, v, false, tol, maxit, conv, it, ok);
   if (rc != DFM_OK) return rc;
+  int nrel = s.h_pinned[F_NREL];
+  if (mixed && s.h_pinned[F_DONE] == 3) {
+    // the FP32 matrix cannot reach the tolerance (cond(A) eps32 not small): solve again in FP64
+    s.last_fallback = 1;
+    const PcgBufs v64 = pcg_bufs(s, false);
+    rc = pcg_run<double>(s, v64, false, tol, maxit, conv, it, ok);
+    if (rc != DFM_OK) return rc;
+  }
   DFM_HIP(hipEventRecord(e1, st));
   DFM_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -690,7 +710,7 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   s.last_ms = ms;
   s.last_inner = its;
   s.last_ms_iter = its ? ms / its : 0.0;
-  s.last_nrel = s.h_pinned[F_NREL];
+  s.last_nrel = nrel;
   return ok ? 1 : 0;
 }
 

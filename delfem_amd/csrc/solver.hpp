@@ -14,10 +14,14 @@ enum {
   S_RMAX = 5,   // reliable updates: max ||r||^2 since the last update, two parity slots 5/6
   S_DELTA2 = 7, // reliable updates: trigger ratio^2
   S_RR = 8,     // last ||r||^2 seen by the convergence test
+  S_TBEST = 11, // mixed precision: smallest true ||r||^2 that counted as progress
+  S_LASTUPD = 12, // mixed precision: iteration of the last reliable update
   S_TMP = 15
 };
 // int flags
-enum { F_DONE = 0, F_IT = 1, F_ITOUT = 2, F_MAXIT = 3, F_NREL = 5, F_PCUR = 6, F_PNEXT = 7 };
+// F_DONE: 0 running, 1 converged, 2 max_iter, 3 mixed precision stalled (the FP32 matrix cannot reach the
+// tolerance: the solve is repeated in FP64); F_STALL: reliable updates in a row without progress
+enum { F_DONE = 0, F_IT = 1, F_ITOUT = 2, F_MAXIT = 3, F_STALL = 4, F_NREL = 5, F_PCUR = 6, F_PNEXT = 7 };
 // partial slots, kMaxPartial doubles per value
 enum { P_A = 0, P_B = 2, P_C = 4, P_D = 6, P_NSLOT = 8 };
 

@@ -7,6 +7,10 @@ draws (SplitMix64, include/dfm_gpu.h) so that every consumer sees bit-identical 
       per-element conductivity lognormal(0, 1), f = 1, u = 0 on the boundary, 1e-8.
   c3  3D elasticity, unit cube 96^3 hexes -> 6 Kuhn tets each, jitter +-10 %, per-element E lognormal(0, 0.5),
       nu = 0.3, bottom face clamped, seeded traction U(-1,1)^3 * 1e-2 per top quad, 1e-8.
+  c4  serving stream: requests arriving as a seeded Poisson process, each a 2D Poisson problem (unit square,
+      conductivity lognormal(0, 1), f = 1, boundary u = 0) or a 2D plane-stress elasticity problem (rectangle
+      5:1, E lognormal(0, 0.25) per element, nu = 0.3, left edge clamped, body force (0, -g), g ~ U[0.005, 0.02])
+      with equal probability, dofs log-uniform in [2k, 500k], jitter +-10 %, 1e-8.
   c5  3D St. Venant-Kirchhoff cantilever 4 x 1 x 1, 64 x 16 x 16 hexes -> tets, E = 1, nu = 0.3, clamped at x = 0,
       end traction (0, 0, -P) on x = 4.
 `scale` shrinks the grids (tests use small versions of the same problems).
@@ -108,6 +112,44 @@ def c5(scale=1.0, load=0.01, kind=None):
     return dict(name="c5", ndim=3, len=3, coords=coords, lnods=tet, kind=N.STVK3D if kind is None else kind,
                 params=np.array([lam, mu, 0.0, 0.0, 0.0, 0.0]), bc=bc.ravel(), faces=faces.ravel(), trac=trac,
                 tol=1e-8, grid=(nx, ny, ny))
+
+
+def c4_request(i, dof_range=(2000, 500000)):
+    """Request i of the serving stream (deterministic in i)."""
+    base = SEEDS["c4"]
+    u = N.rng_uniform(base, 1000 + i, 3)
+    lo, hi = np.log(dof_range[0]), np.log(dof_range[1])
+    dofs = float(np.exp(lo + u[1] * (hi - lo)))
+    seed = base + 7919 * (i + 1)
+    if u[0] < 0.5:
+        k = max(2, int(round(np.sqrt(dofs))) - 1)
+        coords, tri = N.rect_tri(k, k, 1.0, 1.0, seed, 0.1)
+        nelem = tri.size // 3
+        params = np.stack([np.exp(N.rng_normal(seed, 4, nelem)), np.ones(nelem)], axis=1)
+        nn = (k + 1) * (k + 1)
+        ix, iy = np.arange(nn) % (k + 1), np.arange(nn) // (k + 1)
+        bc = ((ix == 0) | (ix == k) | (iy == 0) | (iy == k)).astype(np.int32)
+        return dict(kind=N.POISSON, ndim=2, len=1, coords=coords, lnods=tri, params=params, bc=bc, ndof=nn,
+                    grid=(k, k))
+    ny = max(2, int(round(np.sqrt(dofs / 10.0))) - 1)
+    nx = 5 * ny
+    coords, tri = N.rect_tri(nx, ny, 1.0, 0.2, seed, 0.1)
+    nelem = tri.size // 3
+    E = np.exp(0.25 * N.rng_normal(seed, 4, nelem))
+    lam, mu = plane_stress(E, 0.3)
+    g = 0.005 + 0.015 * u[2]
+    params = np.stack([lam, mu, np.ones(nelem), np.zeros(nelem), np.full(nelem, -g)], axis=1)
+    nn = (nx + 1) * (ny + 1)
+    bc = np.zeros((nn, 2), np.int32)
+    bc[np.arange(nn) % (nx + 1) == 0] = 1
+    return dict(kind=N.SOLID2D, ndim=2, len=2, coords=coords, lnods=tri, params=params, bc=bc.ravel(), ndof=2 * nn,
+                grid=(nx, ny))
+
+
+def c4_arrivals(n, rate):
+    """Arrival times (s) of a seeded Poisson process with the given rate (requests/s)."""
+    u = N.rng_uniform(SEEDS["c4"], 1, n)
+    return np.cumsum(-np.log1p(-u) / rate)
 
 
 def make(name, **kw):

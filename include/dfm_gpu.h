@@ -155,6 +155,10 @@ int dfm_sys_true_residual(dfm_sys sys, double* rel);
 
 /* ---- timing of the last device solve (HIP events on the system's stream) ----------------------------- */
 int dfm_sys_last_solve_stats(dfm_sys sys, double* ms_total, double* ms_per_iter, unsigned* inner_iters);
+/* Mixed precision: reliable updates of the last solve, and whether its FP64 true residual stalled (six
+ * reliable updates in a row without halving it: cond(A) * eps32 too large for an FP32 matrix) so that the
+ * solve was repeated in FP64. */
+int dfm_sys_last_solve_info(dfm_sys sys, int* reliable_updates, int* fell_back);
 /* Average device time of one launch of a hot kernel over `reps` back-to-back launches, timed with HIP events
  * on the system's own stream: which = 0 SpMV FP64 (y = A x), 1 SpMV FP32, 2 ILU(0) apply FP64 (all phases of
  * the forward and backward sweeps), 3 ILU(0) apply FP32. Also reports the algorithmic bytes of one launch. */
@@ -164,8 +168,9 @@ int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, doubl
  * size; each system keeps its own Krylov scalars, done flag and iteration count, so its iterates are those of
  * LsSol::Solve_PCG (src/ls/solver_ls_iter.cpp:100-172) for DFM_FP64, or of the mixed-precision form of
  * dfm_solve_pcg for DFM_MIXED, and do not depend on which other systems share the batch. No counterpart in
- * the reference (single-threaded, one system at a time). Systems must be assembled and factored, belong to
- * the batch's context, and stay alive while in the batch (dfm_sys_destroy takes a system out first). */
+ * the reference (single-threaded, one system at a time). Systems must be assembled and factored, live on the
+ * batch's device (their own context may differ: a system prepared on another stream is synchronized when
+ * added), and stay alive while in the batch (dfm_sys_destroy takes a system out first). */
 typedef struct dfm_batch_s* dfm_batch;
 int dfm_batch_create(dfm_ctx ctx, int precision, int max_systems, dfm_batch* out);
 int dfm_batch_destroy(dfm_batch b);
@@ -177,7 +182,50 @@ int dfm_batch_step(dfm_batch b, int iterations, int* n_running);
  * dfm_batch_add to the step that found it finished (host clock). */
 int dfm_batch_status(dfm_batch b, dfm_sys sys, int* state, double* conv_ratio, unsigned* iteration,
                      double* seconds);
+/* Reliable updates and FP64 fallback of a system's solve (as dfm_sys_last_solve_info). */
+int dfm_batch_info(dfm_batch b, dfm_sys sys, int* reliable_updates, int* fell_back);
 int dfm_batch_remove(dfm_batch b, dfm_sys sys);
+
+/* ---- continuous-batching solve server ------------------------------------------------------------------
+ * No counterpart in the reference (single-threaded, one system at a time). Requests are admitted in arrival
+ * order while fewer than max_active are in flight and their estimated device bytes fit budget_bytes; nworkers
+ * threads (each with its own stream) build, order (block multicolour), assemble and factor admitted requests
+ * while the running batch keeps iterating; a prepared request joins the batch at the next progress call and
+ * leaves it when converged (PCG to tol on the true residual in DFM_MIXED). Each request's result is that of
+ * the same system solved alone. Results stay on the device until dfm_server_release. */
+typedef struct dfm_server_s* dfm_server;
+typedef struct {
+  int ndim, len, kind;          /* kind: DFM_POISSON, DFM_SOLID2D, ... (dfm_sys_assemble)            */
+  int nnode, nelem;
+  const double* coords;         /* [nnode*ndim]                                                        */
+  const int* lnods;             /* [nelem*(ndim+1)]                                                    */
+  const double* params;         /* npar per element (pstride) or shared (pstride = 0)                  */
+  int npar, pstride;
+  const double* glob;           /* 2 values or NULL                                                    */
+  const int* bc;                /* [nnode*len] Dirichlet flags                                         */
+  int nface;                    /* surface loads (3D), may be 0                                        */
+  const int* faces;
+  const double* trac;
+  double tol;                   /* relative residual                                                   */
+  unsigned max_iter;
+} dfm_request;
+enum { DFM_REQ_QUEUED = 0, DFM_REQ_SETUP = 1, DFM_REQ_SOLVING = 2, DFM_REQ_CONVERGED = 3, DFM_REQ_MAXITER = 4,
+       DFM_REQ_FAILED = 5 };
+int dfm_server_create(dfm_ctx ctx, int precision, long long budget_bytes, int max_active, int nworkers,
+                      dfm_server* out);
+int dfm_server_destroy(dfm_server s);
+/* Copies the request; *ticket identifies it. */
+int dfm_server_submit(dfm_server s, const dfm_request* req, long long* ticket);
+/* Admit, join prepared requests, run `iterations` batch iterations, retire finished ones. With nothing to
+ * iterate, waits up to wait_ms for a request to become ready. */
+int dfm_server_progress(dfm_server s, int iterations, int wait_ms, int* n_active, int* n_queued);
+/* latency_s: submission to completion; solve_s: admission to completion; fell_back: the mixed-precision solve
+ * stalled and was repeated in FP64 (dfm_sys_last_solve_info). */
+int dfm_server_poll(dfm_server s, long long ticket, int* state, double* latency_s, double* solve_s,
+                    unsigned* iteration, double* conv_ratio, int* fell_back);
+/* The solution (update handle, caller's node numbering) of a finished request. */
+int dfm_server_result(dfm_server s, long long ticket, double* u);
+int dfm_server_release(dfm_server s, long long ticket);
 
 /* ---- load-stepped Newton for many instances (CEqn_Solid3D_Linear::Solve, geometric-nonlinear branch,
  * src/femeqn/eqnsys_solid.cpp:174-209) ----------------------------------------------------------------------
