@@ -123,18 +123,112 @@ def run_c5_batch(args, N, W):
     print(json.dumps(out), flush=True)
 
 
+def run_c4_serving(args, N, W):
+    """Serving stream: requests (2D Poisson or 2D elasticity, 50/50, dofs log-uniform in [2k, 500k]) arriving as
+    a seeded Poisson process at --rate requests/s; continuous batching under a 64 GB admission budget, mixed
+    precision to a 1e-8 true relative residual. Reports aggregate DOF-solves/s (sum of solved dofs over the time
+    from the first arrival to the last completion) and p50/p99 latency (arrival to completion)."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    n = args.requests
+    arrivals = W.c4_arrivals(n, args.rate)
+    q = queue.Queue(maxsize=256)
+
+    def gen():
+        for i in range(n):
+            q.put((i, W.c4_request(i)))
+
+    threading.Thread(target=gen, daemon=True).start()
+    ctx = N.Context(0)
+    nworkers = max(1, min(16, (os.cpu_count() or 8) - 2))
+    srv = N.Server(ctx, N.MIXED, 64 << 30, max_active=args.slots, nworkers=nworkers)
+    log(f"[bench] c4: {n} requests at {args.rate}/s, {args.slots} slots, {nworkers} setup workers")
+    lat, solve, iters, dofs, fb, kinds = [], [], [], [], [], []
+    inflight = {}
+    nxt = 0
+    t0 = time.perf_counter()
+    last_done = t0
+    max_active = max_queue = 0
+    while nxt < n or inflight:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            i, r = q.get()
+            ts = time.perf_counter() - t0
+            t = srv.submit(r["ndim"], r["len"], r["kind"], r["coords"], r["lnods"], r["params"], r["bc"], tol=1e-8)
+            inflight[t] = (arrivals[i], ts, r["ndof"], r["kind"])
+            nxt += 1
+        active, queued = srv.progress(16, 1)
+        max_active, max_queue = max(max_active, active), max(max_queue, queued)
+        for t in list(inflight):
+            info = srv.poll(t)
+            if info["state"] < N.REQ_CONVERGED:
+                continue
+            arr, ts, nd, kind = inflight.pop(t)
+            if info["state"] != N.REQ_CONVERGED:
+                raise RuntimeError(f"request failed: {info}")
+            lat.append(ts - arr + info["latency_s"])
+            solve.append(info["solve_s"])
+            iters.append(info["iters"])
+            dofs.append(nd)
+            fb.append(info["fell_back"])
+            kinds.append(kind)
+            srv.release(t)
+            last_done = time.perf_counter()
+        if nxt < n and not inflight:
+            time.sleep(max(0.0, min(0.001, arrivals[nxt] - (time.perf_counter() - t0))))
+    wall = last_done - t0 - arrivals[0]
+    srv.close()
+    ctx.close()
+    lat_s = sorted(lat)
+
+    def pct(v, qq):
+        return float(np.percentile(v, qq))
+
+    total = float(np.sum(dofs))
+    out = {
+        "metric": "DOF-solves/s at 1e-8 rel. residual (serving, mixed precision); p50/p99 request latency",
+        "value": total / wall,
+        "unit": "DOF-solves/s",
+        "n_gpus": 1, "steps": n, "warmup": 0,
+        "ms_per_step": wall / n * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"c4: {n} requests, Poisson arrivals {args.rate}/s, 2D Poisson or 2D elasticity "
+                               f"50/50, dofs log-uniform [2k, 500k], 64 GB budget, {args.slots} slots",
+                   "requests": n, "rate_per_s": args.rate, "slots": args.slots, "setup_workers": nworkers},
+        "p50_ms": pct(lat_s, 50) * 1e3,
+        "p99_ms": pct(lat_s, 99) * 1e3,
+        "p50_solve_ms": pct(solve, 50) * 1e3,
+        "wall_s": wall,
+        "offered_dofs_per_s": total / (arrivals[-1] - arrivals[0]) if n > 1 else None,
+        "total_dofs": total,
+        "mean_pcg_iters": float(np.mean(iters)),
+        "fell_back_to_fp64": int(np.sum(fb)),
+        "max_active": max_active, "max_queued": max_queue,
+    }
+    log(f"[bench] c4: {total / wall / 1e6:.2f} M DOF-solves/s over {wall:.1f}s, p50 {out['p50_ms']:.0f} ms, "
+        f"p99 {out['p99_ms']:.0f} ms, max active {max_active}, max queued {max_queue}, fell back {int(np.sum(fb))}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--ordering", default="blockcolor", choices=["natural", "multicolor", "blockcolor"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--max-iter", type=int, default=20000)
     ap.add_argument("--instances", type=int, default=256, help="c5: instances in the batch")
     ap.add_argument("--load-steps", type=int, default=10, help="c5: load steps")
+    ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
+    ap.add_argument("--rate", type=float, default=100.0, help="c4: arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=64, help="c4: systems solved at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every GPU box of this build has one; --gpus {args.gpus} is not "
@@ -146,6 +240,8 @@ def main():
 
     if args.config == "c5":
         return run_c5_batch(args, N, W)
+    if args.config == "c4":
+        return run_c4_serving(args, N, W)
     t0 = time.perf_counter()
     w = W.make(args.config)
     ctx = N.Context(0)

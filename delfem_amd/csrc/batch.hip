@@ -217,28 +217,30 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
   }
 }
 
-template <int B, typename T, typename A>
+// TF: factor values (FP64 or the FP32 copy); T: vectors and arithmetic
+template <int B, typename TF, typename T>
 __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
                                                      int count) {
-  __shared__ A xsb[kWPB][64 * B];
+  __shared__ T xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int e = (int)blockIdx.x * kWPB + w;
   if (e >= count) return;
   const int2 en = ent[e];
   const SegDev& d = segs[en.x];
-  sweep_fwd_wave<B, T, A, T, T>(d.fa, en.y, (const T*)d.fval, (const T*)d.fdval, (const T*)d.r, (T*)d.z, d.fl, xsb[w]);
+  sweep_fwd_wave<B, TF, T, T, T>(d.fa, en.y, (const TF*)d.fval, (const TF*)d.fdval, (const T*)d.r, (T*)d.z, d.fl,
+                                 xsb[w]);
 }
 
-template <int B, typename T, typename A>
+template <int B, typename TF, typename T>
 __global__ void __launch_bounds__(64 * kWPB) k_b_bwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
                                                      int count) {
-  __shared__ A xsb[kWPB][64 * B];
+  __shared__ T xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int e = (int)blockIdx.x * kWPB + w;
   if (e >= count) return;
   const int2 en = ent[e];
   const SegDev& d = segs[en.x];
-  sweep_bwd_wave<B, T, A, T>(d.ba, en.y, (const T*)d.bval, (T*)d.z, d.fl, xsb[w]);
+  sweep_bwd_wave<B, TF, T, T>(d.ba, en.y, (const TF*)d.bval, (T*)d.z, d.fl, xsb[w]);
 }
 
 // partials of (r, z)
@@ -307,7 +309,7 @@ __global__ void __launch_bounds__(NT) k_b_pupdate(const SegDev* __restrict__ seg
 struct BatchSeg {
   Sys* s = nullptr;
   int slot = -1;
-  bool mixed = false;  // FP32 Krylov loop with reliable updates; a stalled one restarts in FP64
+  int mode = DFM_MIXED;  // DFM_FP64, DFM_MIXED (a stalled one restarts as DFM_FP32_ILU), DFM_FP32_ILU
   bool fell_back = false;
   PcgBufs v{};
   double tol = 0;
@@ -321,7 +323,7 @@ struct BatchSeg {
 
 struct Batch {
   Ctx* ctx = nullptr;
-  bool mixed = false;
+  int prec = DFM_MIXED;  // mode of added systems
   int cap = 0;
   DBuf<double> d_sc;  // 16 per slot
   DBuf<int> d_fl;     // 8 per slot
@@ -338,9 +340,9 @@ struct Batch {
     DBuf<int2> d_fent, d_bent;
     std::vector<int> fptr, bptr;  // per phase offsets into the entry tables
     int nsb = 0, nvb = 0;
-    bool mixed = false;
+    int mode = 0;
     DBuf<double> d_pq, d_true, d_rr, d_rz;
-  } grp[6];  // (block size, precision): index (B - 1) + 3 * mixed
+  } grp[9];  // (block size, mode): index (B - 1) + 3 * mode
   ~Batch() {
     for (auto& g : grp) {
       g.d_segs.release(); g.d_sblk.release(); g.d_vblk.release(); g.d_fent.release(); g.d_bent.release();
@@ -367,11 +369,11 @@ static int rebuild(Batch& bt) {
   hipStream_t st = bt.ctx->stream;
   for (auto& g : bt.grp) g.members.clear();
   for (int k = 0; k < (int)bt.segs.size(); k++)
-    if (bt.segs[k].state == 0) bt.grp[bt.segs[k].s->len - 1 + (bt.segs[k].mixed ? 3 : 0)].members.push_back(k);
-  for (int gi = 0; gi < 6; gi++) {
+    if (bt.segs[k].state == 0) bt.grp[bt.segs[k].s->len - 1 + 3 * bt.segs[k].mode].members.push_back(k);
+  for (int gi = 0; gi < 9; gi++) {
     Batch::Group& g = bt.grp[gi];
     g.B = gi % 3 + 1;
-    g.mixed = gi >= 3;
+    g.mode = gi / 3;
     const int tpr = g.B == 3 ? tpr_of<3>() : tpr_of<1>();
     const int rs = NT / tpr;
     std::vector<SegDev> h;
@@ -392,9 +394,10 @@ static int rebuild(Batch& bt) {
       d.dia64 = s.d_dia.p;
       d.fa = SlabArgs{s.fwd.d_gmeta.p, s.fwd.d_srow.p, s.fwd.d_scnt.p, s.fwd.d_ecol.p};
       d.ba = SlabArgs{s.bwd.d_gmeta.p, s.bwd.d_srow.p, s.bwd.d_scnt.p, s.bwd.d_ecol.p};
-      d.fval = sg.mixed ? (const void*)s.fwd.d_eval32.p : (const void*)s.fwd.d_eval.p;
-      d.fdval = sg.mixed ? (const void*)s.fwd.d_dval32.p : (const void*)s.fwd.d_dval.p;
-      d.bval = sg.mixed ? (const void*)s.bwd.d_eval32.p : (const void*)s.bwd.d_eval.p;
+      const bool f32 = sg.mode != DFM_FP64;
+      d.fval = f32 ? (const void*)s.fwd.d_eval32.p : (const void*)s.fwd.d_eval.p;
+      d.fdval = f32 ? (const void*)s.fwd.d_dval32.p : (const void*)s.fwd.d_dval.p;
+      d.bval = f32 ? (const void*)s.bwd.d_eval32.p : (const void*)s.bwd.d_eval.p;
       d.r = sg.v.r;
       d.z = sg.v.z;
       d.p = sg.v.p;
@@ -456,7 +459,8 @@ static int rebuild(Batch& bt) {
   return DFM_OK;
 }
 
-template <int B, typename T, bool MX>
+// T: vectors; MX: reliable-update loop; TF: factor values
+template <int B, typename T, bool MX, typename TF>
 static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
   constexpr int TPR = tpr_of<B>();
   const SegDev* segs = g.d_segs.p;
@@ -467,12 +471,12 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
   hipLaunchKernelGGL((k_b_judge<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p, (const double*)g.d_true.p);
   for (size_t p = 0; p + 1 < g.fptr.size(); p++) {
     const int c = g.fptr[p + 1] - g.fptr[p];
-    if (c) hipLaunchKernelGGL((k_b_fwd<B, T, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
+    if (c) hipLaunchKernelGGL((k_b_fwd<B, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
                               (const int2*)g.d_fent.p + g.fptr[p], c);
   }
   for (size_t p = 0; p + 1 < g.bptr.size(); p++) {
     const int c = g.bptr[p + 1] - g.bptr[p];
-    if (c) hipLaunchKernelGGL((k_b_bwd<B, T, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
+    if (c) hipLaunchKernelGGL((k_b_bwd<B, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
                               (const int2*)g.d_bent.p + g.bptr[p], c);
   }
   hipLaunchKernelGGL((k_b_rz<T>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p, g.d_rz.p);
@@ -481,18 +485,19 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
   return hipGetLastError();
 }
 
-// A mixed-precision segment whose true residual stalled restarts as an FP64 solve of the same system (its
-// residual handle still holds the right-hand side).
+// A mixed-precision segment whose true residual stalled restarts as an FP64 solve of the same system with the
+// FP32 factor (its residual handle still holds the right-hand side).
 static int restart_fp64(Batch& bt, BatchSeg& sg) {
   Sys& s = *sg.s;
-  sg.mixed = false;
+  sg.mode = DFM_FP32_ILU;
   sg.fell_back = true;
   sg.nrel = bt.h_fl[8 * sg.slot + F_NREL];
-  int rc = sys_solve_setup(s, DFM_FP64);
+  int rc = sys_solve_setup(s, DFM_FP32_ILU);
   if (rc != DFM_OK) return rc;
   if (s.ctx != bt.ctx) DFM_HIP(hipStreamSynchronize(s.ctx->stream));
   s.prepared = -1;
   sg.v = pcg_bufs(s, false);
+  sg.v.f32ilu = true;
   rc = pcg_start(s, sg.v, false, sg.tol, sg.maxit, bt.d_sc.p + 16 * (size_t)sg.slot, bt.d_fl.p + 8 * (size_t)sg.slot,
                  bt.ctx->stream);
   bt.dirty = true;
@@ -509,7 +514,7 @@ static int finish_seg(Batch& bt, BatchSeg& sg) {
   sg.iters = done == 1 ? (unsigned)fl[F_ITOUT] : sg.maxit;
   if (!sg.fell_back) sg.nrel = fl[F_NREL];
   sg.t_done = now_s();
-  const int rc = pcg_finish(*sg.s, sg.v, sg.mixed, done, bt.ctx->stream);
+  const int rc = pcg_finish(*sg.s, sg.v, sg.mode == DFM_MIXED, done, bt.ctx->stream);
   bt.dirty = true;
   return rc;
 }
@@ -561,13 +566,14 @@ extern "C" {
 
 int dfm_batch_create(dfm_ctx ctx, int precision, int max_systems, dfm_batch* out) {
   DFM_CHECK(ctx && out, DFM_ERR_ARG, "dfm_batch_create: null argument");
-  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED, DFM_ERR_ARG, "dfm_batch_create: bad precision");
+  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU, DFM_ERR_ARG,
+            "dfm_batch_create: bad precision");
   DFM_CHECK(max_systems > 0 && max_systems <= (1 << 20), DFM_ERR_ARG, "dfm_batch_create: bad max_systems");
   DFM_HIP(hipSetDevice(ctx->c.device));
   auto* h = new dfm_batch_s;
   Batch& bt = h->b;
   bt.ctx = &ctx->c;
-  bt.mixed = precision == DFM_MIXED;
+  bt.prec = precision;
   bt.cap = max_systems;
   if (bt.d_sc.alloc((size_t)16 * bt.cap) != hipSuccess || bt.d_fl.alloc((size_t)8 * bt.cap) != hipSuccess ||
       hipHostMalloc((void**)&bt.h_fl, sizeof(int) * 8 * bt.cap) != hipSuccess ||
@@ -610,7 +616,7 @@ int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
   DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_batch_add: the system is already in a batch");
   DFM_CHECK(!bt.free_slots.empty(), DFM_ERR_STATE, "dfm_batch_add: batch full");
   DFM_CHECK(max_iter > 0, DFM_ERR_ARG, "dfm_batch_add: max_iter must be positive");
-  const int prec = bt.mixed ? DFM_MIXED : DFM_FP64;
+  const int prec = bt.prec;
   int rc = DFM_OK;
   if (s.prepared != prec) {
     rc = sys_solve_setup(s, prec);
@@ -622,14 +628,16 @@ int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
   BatchSeg sg;
   sg.s = &s;
   sg.slot = bt.free_slots.back();
-  sg.mixed = bt.mixed;
-  sg.v = pcg_bufs(s, bt.mixed);
+  sg.mode = bt.prec;
+  sg.v = pcg_bufs(s, prec == DFM_MIXED);
+  sg.v.f32ilu = prec == DFM_FP32_ILU;
   sg.tol = tol;
   sg.maxit = max_iter;
   hipStream_t st = bt.ctx->stream;
-  rc = pcg_prepare(s, sg.v, bt.mixed, st);
+  rc = pcg_prepare(s, sg.v, prec == DFM_MIXED, st);
   if (rc == DFM_OK)
-    rc = pcg_start(s, sg.v, bt.mixed, tol, max_iter, bt.d_sc.p + 16 * (size_t)sg.slot, bt.d_fl.p + 8 * (size_t)sg.slot, st);
+    rc = pcg_start(s, sg.v, prec == DFM_MIXED, tol, max_iter, bt.d_sc.p + 16 * (size_t)sg.slot,
+                   bt.d_fl.p + 8 * (size_t)sg.slot, st);
   if (rc != DFM_OK) return rc;
   bt.free_slots.pop_back();
   sg.t_add = now_s();
@@ -655,12 +663,17 @@ int dfm_batch_step(dfm_batch b, int iterations, int* n_running) {
     for (auto& g : bt.grp) {
       if (g.members.empty()) continue;
       hipError_t e;
-      if (g.mixed)
-        e = g.B == 1 ? iterate_group<1, float, true>(g, st)
-                     : g.B == 2 ? iterate_group<2, float, true>(g, st) : iterate_group<3, float, true>(g, st);
+      if (g.mode == DFM_MIXED)
+        e = g.B == 1 ? iterate_group<1, float, true, float>(g, st)
+                     : g.B == 2 ? iterate_group<2, float, true, float>(g, st) : iterate_group<3, float, true, float>(g, st);
+      else if (g.mode == DFM_FP32_ILU)
+        e = g.B == 1 ? iterate_group<1, double, false, float>(g, st)
+                     : g.B == 2 ? iterate_group<2, double, false, float>(g, st)
+                                : iterate_group<3, double, false, float>(g, st);
       else
-        e = g.B == 1 ? iterate_group<1, double, false>(g, st)
-                     : g.B == 2 ? iterate_group<2, double, false>(g, st) : iterate_group<3, double, false>(g, st);
+        e = g.B == 1 ? iterate_group<1, double, false, double>(g, st)
+                     : g.B == 2 ? iterate_group<2, double, false, double>(g, st)
+                                : iterate_group<3, double, false, double>(g, st);
       DFM_HIP(e);
     }
   }

@@ -776,11 +776,12 @@ static int ensure_fp32(Sys& s) {
 
 namespace dfm {
 int sys_solve_setup(Sys& s, int precision) {
-  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED, DFM_ERR_ARG, "solve: bad precision");
+  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU, DFM_ERR_ARG,
+            "solve: bad precision");
   DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "solve: assemble and factor first");
-  int rc = ensure_work(s, 2, precision == DFM_FP64 ? 0 : 5);
+  int rc = ensure_work(s, 2, precision == DFM_MIXED ? 5 : 0);
   if (rc != DFM_OK) return rc;
-  if (precision == DFM_MIXED) {
+  if (precision != DFM_FP64) {
     rc = ensure_fp32(s);
     if (rc != DFM_OK) return rc;
   }

@@ -148,9 +148,10 @@ struct Sys {
   bool has_loads = false;
   double load_scale = 1.0;
 
-  // captured chunks of Krylov iterations (hipGraph), per loop kind: 0 PCG FP64, 1 PCG mixed, 2 PBiCGSTAB
-  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-  const void* gkey[3] = {nullptr, nullptr, nullptr};  // work buffer the graph was captured with
+  // captured chunks of Krylov iterations (hipGraph), per loop kind: 0 PCG FP64, 1 PCG mixed, 2 PBiCGSTAB,
+  // 3 PCG FP64 with the FP32 factor
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  const void* gkey[4] = {nullptr, nullptr, nullptr, nullptr};  // work buffer the graph was captured with
   int gchunk = 16;
 
   bool assembled = false, factored = false, factored32 = false;

@@ -399,42 +399,58 @@ static unsigned spmv_grid(const Sys& s, int tpr) {
   return vec_blocks((s.n + rows_per_block - 1) / rows_per_block, 1);
 }
 
-template <int B, typename T, typename A>
-static hipError_t precond_T(const Sys& s, bool fp32, const T* src, T* z, const int* done, hipStream_t st) {
+// z = M^-1 src over the sweep slabs: TV the factor values (FP64 or the FP32 copy), V the vectors and the
+// arithmetic.
+template <int B, typename TV, typename V>
+static hipError_t precond_T(const Sys& s, const V* src, V* z, const int* done, hipStream_t st) {
+  constexpr bool f32 = sizeof(TV) == 4;
   const Sweep& f = s.fwd;
   const SlabArgs fa{f.d_gmeta.p, f.d_srow.p, f.d_scnt.p, f.d_ecol.p};
-  const T* fval = fp32 ? (const T*)f.d_eval32.p : (const T*)f.d_eval.p;
-  const T* fdval = fp32 ? (const T*)f.d_dval32.p : (const T*)f.d_dval.p;
+  const TV* fval = f32 ? (const TV*)f.d_eval32.p : (const TV*)f.d_eval.p;
+  const TV* fdval = f32 ? (const TV*)f.d_dval32.p : (const TV*)f.d_dval.p;
   for (int p = 0; p < f.nphase(); p++) {
     const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_sweep_fwd<B, T, A, T, T>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
+    hipLaunchKernelGGL((k_sweep_fwd<B, TV, V, V, V>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
                        fa, fval, fdval, src, z, done);
   }
   const Sweep& b = s.bwd;
   const SlabArgs ba{b.d_gmeta.p, b.d_srow.p, b.d_scnt.p, b.d_ecol.p};
-  const T* bval = fp32 ? (const T*)b.d_eval32.p : (const T*)b.d_eval.p;
+  const TV* bval = f32 ? (const TV*)b.d_eval32.p : (const TV*)b.d_eval.p;
   for (int p = 0; p < b.nphase(); p++) {
     const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_sweep_bwd<B, T, A, T>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
+    hipLaunchKernelGGL((k_sweep_bwd<B, TV, V, V>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
                        ba, bval, z, done);
   }
   return hipGetLastError();
 }
 
 hipError_t precond_apply64(const Sys& s, const double* src, double* z, const int* done, hipStream_t st) {
-  if (s.len == 1) return precond_T<1, double, double>(s, false, src, z, done, st);
-  if (s.len == 2) return precond_T<2, double, double>(s, false, src, z, done, st);
-  if (s.len == 3) return precond_T<3, double, double>(s, false, src, z, done, st);
+  if (s.len == 1) return precond_T<1, double, double>(s, src, z, done, st);
+  if (s.len == 2) return precond_T<2, double, double>(s, src, z, done, st);
+  if (s.len == 3) return precond_T<3, double, double>(s, src, z, done, st);
+  return hipErrorInvalidValue;
+}
+
+// FP64 vectors and arithmetic, FP32 factor values
+static hipError_t precond_apply_f32ilu(const Sys& s, const double* src, double* z, const int* done, hipStream_t st) {
+  if (s.len == 1) return precond_T<1, float, double>(s, src, z, done, st);
+  if (s.len == 2) return precond_T<2, float, double>(s, src, z, done, st);
+  if (s.len == 3) return precond_T<3, float, double>(s, src, z, done, st);
   return hipErrorInvalidValue;
 }
 
 static hipError_t precond_apply32(const Sys& s, const float* src, float* z, const int* done, hipStream_t st) {
-  if (s.len == 1) return precond_T<1, float, float>(s, true, src, z, done, st);
-  if (s.len == 2) return precond_T<2, float, float>(s, true, src, z, done, st);
-  if (s.len == 3) return precond_T<3, float, float>(s, true, src, z, done, st);
+  if (s.len == 1) return precond_T<1, float, float>(s, src, z, done, st);
+  if (s.len == 2) return precond_T<2, float, float>(s, src, z, done, st);
+  if (s.len == 3) return precond_T<3, float, float>(s, src, z, done, st);
   return hipErrorInvalidValue;
+}
+
+static hipError_t precond_apply_d(const Sys& s, const PcgBufs& v, const double* src, double* z, const int* done,
+                                  hipStream_t st) {
+  return v.f32ilu ? precond_apply_f32ilu(s, src, z, done, st) : precond_apply64(s, src, z, done, st);
 }
 
 template <typename T, bool REL>
@@ -510,7 +526,7 @@ static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
       return DFM_ERR_HIP;
     hipLaunchKernelGGL((k_pcg_update<double, double>), vg, dim3(NT), 0, st, m, (double*)r, (double*)v.x,
                        (const double*)q, (const double*)p, slot(s, P_A), (int)Gs, slot(s, P_B), sc, fl);
-    if (precond_apply64(s, (const double*)r, (double*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
+    if (precond_apply_d(s, v, (const double*)r, (double*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
     hipLaunchKernelGGL(k_pcg_judge_rz, vg, dim3(NT), 0, st, m, (const double*)r, (const double*)z, slot(s, P_B), G,
                        slot(s, P_D), sc, fl);
     hipLaunchKernelGGL((k_pcg_pupdate<T, false>), vg, dim3(NT), 0, st, m, p, (const T*)z, slot(s, P_D), G, sc, fl,
@@ -572,7 +588,7 @@ static int pcg_start_T(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigne
   hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)r, m, slot(s, P_A), (const int*)nullptr);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, guard, sc, fl);
   hipError_t e = mixed ? precond_apply32(s, (const float*)r, (float*)z, fl, st)
-                       : precond_apply64(s, (const double*)r, (double*)z, fl, st);
+                       : precond_apply_d(s, v, (const double*)r, (double*)z, fl, st);
   if (e != hipSuccess) return DFM_ERR_HIP;
   hipLaunchKernelGGL(k_vcopy<T>, vg, dim3(NT), 0, st, (const T*)z, p, m, fl);
   hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)z, m, slot(s, P_A), fl);
@@ -626,7 +642,7 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
     const int rc = pcg_start(s, v, mixed, tol, maxit, sc, fl, st);
     if (rc != DFM_OK) return rc;
   }
-  const int gk = mixed ? 1 : 0;
+  const int gk = mixed ? 1 : (v.f32ilu ? 3 : 0);
   const bool use_graph = getenv("DFM_NO_GRAPH") == nullptr;
   if (use_graph) {
     const int rc = graph_chunk(s, gk, v.r, [&]() { return pcg_iteration<T>(s, v, mixed, st); }, st);
@@ -684,7 +700,8 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   bool ok = false;
   int rc;
   const bool mixed = precision == DFM_MIXED;
-  const PcgBufs v = pcg_bufs(s, mixed);
+  PcgBufs v = pcg_bufs(s, mixed);
+  v.f32ilu = precision == DFM_FP32_ILU;
   s.last_fallback = 0;
   rc = pcg_prepare(s, v, mixed, st);
   if (rc == DFM_OK)
@@ -692,9 +709,11 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   if (rc != DFM_OK) return rc;
   int nrel = s.h_pinned[F_NREL];
   if (mixed && s.h_pinned[F_DONE] == 3) {
-    // the FP32 matrix cannot reach the tolerance (cond(A) eps32 not small): solve again in FP64
+    // the FP32 matrix cannot reach the tolerance (cond(A) eps32 not small): solve again with FP64 vectors and
+    // matrix, keeping the FP32 factor (the preconditioner's precision does not bound the attainable accuracy)
     s.last_fallback = 1;
-    const PcgBufs v64 = pcg_bufs(s, false);
+    PcgBufs v64 = pcg_bufs(s, false);
+    v64.f32ilu = true;
     rc = pcg_run<double>(s, v64, false, tol, maxit, conv, it, ok);
     if (rc != DFM_OK) return rc;
   }

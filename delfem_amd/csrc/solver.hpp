@@ -37,6 +37,7 @@ struct PcgBufs {
   void* x;
   double* x64;
   const double* b64;
+  bool f32ilu = false;  // FP64 loop with the FP32 factor copy as preconditioner
 };
 
 // Start of a PCG solve on the system's own buffers with scalars/flags at sc/fl (16 doubles, 8 ints): x = 0,

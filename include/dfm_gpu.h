@@ -49,8 +49,11 @@ enum {
   DFM_ORDER_BLOCKCOLOR = 2   /* block multicolour: geometric tiles of <= 64 nodes, tiles coloured        */
 };
 
-/* precision of the device Krylov solvers */
-enum { DFM_FP64 = 0, DFM_MIXED = 1 };
+/* precision of the device Krylov solvers: DFM_FP64 the reference algorithm in FP64; DFM_MIXED FP32 matrix,
+ * factor and Krylov vectors with FP64 dots, solution and true residual (reliable updates), falling back to
+ * DFM_FP32_ILU when its true residual stalls; DFM_FP32_ILU the FP64 loop with the FP32 copy of the factor as
+ * preconditioner (attainable accuracy of FP64, half the factor bytes). */
+enum { DFM_FP64 = 0, DFM_MIXED = 1, DFM_FP32_ILU = 2 };
 
 const char* dfm_last_error(void);
 int dfm_device_count(int* count);

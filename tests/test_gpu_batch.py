@@ -61,9 +61,9 @@ def _run(batch, systems, schedule, chunk=7, max_steps=2000):
     return res
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "fp32_ilu"])
 def test_alone_equals_mixed_in(ctx, precision):
-    prec = N.MIXED if precision == "mixed" else N.FP64
+    prec = {"mixed": N.MIXED, "fp64": N.FP64, "fp32_ilu": N.FP32_ILU}[precision]
     systems = _systems(ctx)
     alone = []
     for s in systems:
