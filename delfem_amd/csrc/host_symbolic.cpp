@@ -4,6 +4,8 @@
 #include <cmath>
 #include <deque>
 #include <numeric>
+#include <set>
+#include <tuple>
 #include <stdexcept>
 
 #include "dfm_internal.hpp"
@@ -382,6 +384,12 @@ void make_blockcolor(const Sys& s, int tedge, std::vector<int>& order, std::vect
       const int a = tile[i], c = tile[s.colind0[q]];
       if (a != c) tadj[a].push_back(c);
     }
+  for (auto& a : tadj) {
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  // greedy in tile order, then DSATUR (the uncoloured tile with the most distinct neighbour colours next,
+  // ties to the larger degree, then the lower index); the one with fewer colours is kept (fewer sweep phases)
   std::vector<int> tcol((size_t)ntile, -1);
   ncolors = 0;
   std::vector<char> used;
@@ -393,6 +401,38 @@ void make_blockcolor(const Sys& s, int tedge, std::vector<int>& order, std::vect
     while (used[c]) c++;
     tcol[t] = c;
     ncolors = std::max(ncolors, c + 1);
+  }
+  {
+    std::vector<int> dcol((size_t)ntile, -1), sat((size_t)ntile, 0);
+    std::vector<std::vector<char>> seen((size_t)ntile);
+    std::set<std::tuple<int, int, int>> q;  // (-saturation, -degree, tile)
+    for (int t = 0; t < ntile; t++) q.insert(std::make_tuple(0, -(int)tadj[t].size(), t));
+    int nc = 0;
+    while (!q.empty()) {
+      const int t = std::get<2>(*q.begin());
+      q.erase(q.begin());
+      used.assign((size_t)nc + 1, 0);
+      for (int c : tadj[t])
+        if (dcol[c] >= 0) used[dcol[c]] = 1;
+      int c = 0;
+      while (used[c]) c++;
+      dcol[t] = c;
+      nc = std::max(nc, c + 1);
+      for (int u : tadj[t]) {
+        if (dcol[u] >= 0) continue;
+        auto& sv = seen[u];
+        if ((int)sv.size() <= c) sv.resize((size_t)c + 1, 0);
+        if (sv[c]) continue;
+        sv[c] = 1;
+        q.erase(std::make_tuple(-sat[u], -(int)tadj[u].size(), u));
+        sat[u]++;
+        q.insert(std::make_tuple(-sat[u], -(int)tadj[u].size(), u));
+      }
+    }
+    if (nc < ncolors) {
+      tcol.swap(dcol);
+      ncolors = nc;
+    }
   }
   order = byk;  // already tile-major (tiles in creation order), natural inside a tile
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tcol[tile[a]] < tcol[tile[b]]; });
