@@ -43,6 +43,7 @@ Sys::~Sys() {
   for (auto* t : d_tmp) { t->release(); delete t; }
   d_work.release(); d_work32.release(); d_partial.release(); d_counter.release(); d_scal.release(); d_iflag.release();
   for (Sweep* w : {&fwd, &bwd}) w->release_dev();
+  sell.release_dev();
   d_par.release(); d_lptr.release(); d_loads.release();
   for (auto& g : gexec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -69,6 +70,10 @@ static int upload_structure(Sys& s) {
   DFM_HIP(upload(s.d_usplit, s.usplit, st));
   DFM_HIP(upload(s.d_order, s.order, st));
   DFM_HIP(upload(s.d_inv, s.inv, st));
+  s.sell.release_dev();
+  DFM_HIP(upload(s.sell.d_sptr, s.sell.sptr, st));
+  DFM_HIP(upload(s.sell.d_col, s.sell.col, st));
+  DFM_HIP(upload(s.sell.d_src, s.sell.src, st));
   for (Sweep* w : {&s.fwd, &s.bwd}) {
     w->release_dev();
     DFM_HIP(upload(w->d_grp, w->grp, st));
@@ -264,6 +269,7 @@ int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
   s.perm_crs = a.perm_crs;
   copy_sweep_host(s.fwd, a.fwd);
   copy_sweep_host(s.bwd, a.bwd);
+  s.sell.sptr = a.sell.sptr; s.sell.col = a.sell.col; s.sell.src = a.sell.src; s.sell.nslab = a.sell.nslab;
   auto fail = [&](hipError_t e) {
     set_error(std::string("dfm_sys_clone: ") + hipGetErrorString(e));
     delete h;
@@ -524,6 +530,12 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   // FinalizeMarge (:92-128): Dirichlet rows/columns, residual zeroed at fixed dofs, ||r|| returned
   DFM_HIP(launch_set_bc(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_bc.p, st));
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
+  // SELL copy of the matrix for the Krylov SpMV
+  {
+    const long long ne = s.sell.nslab * 64;
+    if (s.sell.d_val.n < (size_t)(ne * s.bs)) DFM_HIP(s.sell.d_val.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
+    DFM_HIP(launch_slab_fill(s.len, ne, s.sell.d_src.p, s.d_crs.p, s.sell.d_val.p, st));
+  }
   s.assembled = true;
   s.factored = s.factored32 = false;
   s.prepared = -1;
@@ -757,6 +769,11 @@ static int ensure_fp32(Sys& s) {
   if (!s.d_dia32.p) DFM_HIP(s.d_dia32.alloc((size_t)s.n * s.bs));
   DFM_HIP(launch_to_f32(s.d_crs.p, s.d_crs32.p, (long long)s.ncrs * s.bs, st));
   DFM_HIP(launch_to_f32(s.d_dia.p, s.d_dia32.p, (long long)s.n * s.bs, st));
+  {
+    const long long ne = s.sell.nslab * 64;
+    if (s.sell.d_val32.n < (size_t)(ne * s.bs)) DFM_HIP(s.sell.d_val32.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
+    DFM_HIP(launch_slab_fill(s.len, ne, s.sell.d_src.p, s.d_crs.p, s.sell.d_val32.p, st));
+  }
   for (int dir = 0; dir < 2; dir++) {
     Sweep& w = dir == 0 ? s.fwd : s.bwd;
     const long long ne = w.nslab * 64;

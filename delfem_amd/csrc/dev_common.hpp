@@ -138,6 +138,49 @@ __device__ inline void dia_row(const T* __restrict__ dia, const VX* __restrict__
   }
 }
 
+// SELL-64 SpMV row (one row per lane): acc = sum over the slice's w entries of the lane, in CRS order; CH
+// entries' columns and values are loaded together, then their x gathered, then accumulated. Padding entries
+// (value 0, column 0) add exact zeros. The caller adds the diagonal block last, as MatVec does
+// (src/matvec/matdia_blkcrs.cpp:1008-1025).
+template <int B, int CH, typename T, typename A, typename VX>
+__device__ inline void sell_row(const unsigned* __restrict__ col, const T* __restrict__ val, int r0, int w, int lane,
+                                const VX* __restrict__ x, A (&acc)[B]) {
+#pragma unroll
+  for (int r = 0; r < B; r++) acc[r] = A(0);
+  for (int k0 = 0; k0 < w; k0 += CH) {
+    unsigned j[CH];
+    A a[CH][B * B], xv[CH][B];
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (k0 + k < w) {
+        const size_t sr = (size_t)(r0 + k0 + k);
+        j[k] = col[sr * 64 + lane];
+#pragma unroll
+        for (int e = 0; e < B * B; e++) a[k][e] = (A)val[(sr * (B * B) + e) * 64 + lane];
+      }
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (k0 + k < w)
+#pragma unroll
+        for (int c = 0; c < B; c++) xv[k][c] = (A)x[(size_t)j[k] * B + c];
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (k0 + k < w)
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+          A t = a[k][r * B] * xv[k][0];
+#pragma unroll
+          for (int c = 1; c < B; c++) t += a[k][r * B + c] * xv[k][c];
+          acc[r] += t;
+        }
+  }
+}
+
+template <int B>
+struct SellChunk {
+  static constexpr int CH = B == 1 ? 8 : 4;
+};
+
 // ---------------------------------------------------------------------------------------------------------
 // ILU(0) sweeps over the slab layout. Forward (ForwardSubstitution, src/matvec/matdiafrac_blkcrs.cpp:332-507):
 //   t = b_i - sum_{j<i} L_ij x_j (ascending j), x_i = D_i^-1 t.

@@ -99,6 +99,23 @@ struct Sweep {
   }
 };
 
+// SELL-64 copy of the off-diagonal blocks for the SpMV of the Krylov loops: slice s = device rows
+// [64 s, 64 s + 64), one row per lane; the slice's entries are slab rows [sptr[s], sptr[s+1]) in CRS order per
+// lane (padding: column 0, no source), value e of entry (slab row r, lane l) at (r*bs + e)*64 + l.
+struct Sell {
+  std::vector<int> sptr;          // [nslice+1]
+  std::vector<unsigned> col, src; // [64*nslab]; src = crs block index or ~0u
+  long long nslab = 0;
+  DBuf<int> d_sptr;
+  DBuf<unsigned> d_col, d_src;
+  DBuf<double> d_val;
+  DBuf<float> d_val32;
+  int nslice() const { return sptr.empty() ? 0 : (int)sptr.size() - 1; }
+  void release_dev() {
+    d_sptr.release(); d_col.release(); d_src.release(); d_val.release(); d_val32.release();
+  }
+};
+
 struct Sys {
   Ctx* ctx = nullptr;
   int ndim = 0, n = 0, len = 0, bs = 0, npoel = 0, nelem = 0;
@@ -121,6 +138,7 @@ struct Sys {
   std::vector<unsigned> rowptr, colind, diaind, lsplit, usplit;
   std::vector<long long> perm_crs;  // original crs index -> device crs index
   Sweep fwd, bwd;
+  Sell sell;
 
   // device arrays
   DBuf<unsigned> d_rowptr, d_colind, d_diaind, d_lsplit, d_usplit;

@@ -275,7 +275,31 @@ static void build_slabs(const Sys& s, Sweep& w, bool forward) {
   w.nslab = off;
 }
 
+// SELL-64 layout of the device-numbering pattern (see Sell).
+static void build_sell(Sys& s) {
+  Sell& c = s.sell;
+  const int n = s.n, ns = (n + kGroup - 1) / kGroup;
+  c.sptr.assign((size_t)ns + 1, 0);
+  for (int sl = 0; sl < ns; sl++) {
+    unsigned w = 0;
+    for (int r = sl * kGroup; r < std::min(n, (sl + 1) * kGroup); r++) w = std::max(w, s.rowptr[r + 1] - s.rowptr[r]);
+    c.sptr[sl + 1] = c.sptr[sl] + (int)w;
+  }
+  c.nslab = c.sptr[ns];
+  c.col.assign((size_t)c.nslab * kGroup, 0u);
+  c.src.assign((size_t)c.nslab * kGroup, ~0u);
+  for (int r = 0; r < n; r++) {
+    const int sl = r / kGroup, l = r % kGroup;
+    for (unsigned q = s.rowptr[r]; q < s.rowptr[r + 1]; q++) {
+      const size_t k = ((size_t)c.sptr[sl] + (q - s.rowptr[r])) * kGroup + l;
+      c.col[k] = s.colind[q];
+      c.src[k] = q;
+    }
+  }
+}
+
 void build_schedule(Sys& s) {
+  build_sell(s);
   if (s.color_of.empty()) {
     level_sweeps(s);
     build_slabs(s, s.fwd, true);

@@ -129,6 +129,54 @@ __global__ void __launch_bounds__(NT) k_pcg_spmv(int n, const unsigned* __restri
   if (REL && rel) produce<NT, 1>(slot_true, vt);
 }
 
+// The same over the SELL-64 copy of the matrix: one slice (64 rows, one per lane) per wavefront, kSellWaves
+// wavefronts per block; the row sums follow CRS order with the diagonal block last, as MatVec.
+constexpr int kSellWaves = 16;
+
+template <int B, typename T, bool REL>
+__global__ void __launch_bounds__(64 * kSellWaves)
+    k_pcg_spmv_sell(int n, int nslice, const int* __restrict__ sptr, const unsigned* __restrict__ col,
+                    const T* __restrict__ val, const T* __restrict__ dia, const T* __restrict__ p, T* __restrict__ q,
+                    double* slot, const double* __restrict__ val64, const double* __restrict__ dia64,
+                    const double* __restrict__ x64, const double* __restrict__ b64, T* __restrict__ r, double* slot_true,
+                    int* fl) {
+  constexpr int CH = SellChunk<B>::CH;
+  if (fl[F_DONE]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) fl[F_IT] = fl[F_IT] + 1;
+  const bool rel = REL && fl[F_PCUR];
+  const int wave = wave_in_block(), lane = threadIdx.x % 64;
+  double v[1] = {0.0}, vt[1] = {0.0};
+  for (int sl = (int)blockIdx.x * kSellWaves + wave; sl < nslice; sl += (int)gridDim.x * kSellWaves) {
+    const int r0 = sptr[sl], w = sptr[sl + 1] - r0;
+    const int row = sl * 64 + lane;
+    T acc[B];
+    sell_row<B, CH, T, T, T>(col, val, r0, w, lane, p, acc);
+    if (row < n) {
+      dia_row<B, T, T, T>(dia, p, row, acc);
+#pragma unroll
+      for (int k = 0; k < B; k++) {
+        q[(size_t)row * B + k] = acc[k];
+        v[0] += (double)p[(size_t)row * B + k] * (double)acc[k];
+      }
+    }
+    if (REL && rel) {
+      double a64[B];
+      sell_row<B, CH, double, double, double>(col, val64, r0, w, lane, x64, a64);
+      if (row < n) {
+        dia_row<B, double, double, double>(dia64, x64, row, a64);
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          const double t = b64[(size_t)row * B + k] - a64[k];
+          r[(size_t)row * B + k] = (T)t;
+          vt[0] += t * t;
+        }
+      }
+    }
+  }
+  produce<64 * kSellWaves, 1>(slot, v);
+  if (REL && rel) produce<64 * kSellWaves, 1>(slot_true, vt);
+}
+
 // alpha = rho / (p, q); r -= alpha q; x += alpha p; partials of (r, r)   (solver_ls_iter.cpp:139-144)
 template <typename T, typename TX>
 __global__ void __launch_bounds__(NT) k_pcg_update(long long m, T* r, TX* x, const T* q, const T* p,
@@ -456,6 +504,21 @@ static hipError_t precond_apply_d(const Sys& s, const PcgBufs& v, const double* 
 template <typename T, bool REL>
 static hipError_t pcg_spmv(const Sys& s, const T* crs, const T* dia, const T* p, T* q, double* slot, unsigned& G,
                            const double* x64, const double* b64, T* r, double* slot_true, hipStream_t st) {
+  if (s.sell.nslice() > 0) {
+    const int ns = s.sell.nslice();
+    G = (unsigned)std::min<long long>(kMaxPartial, (ns + kSellWaves - 1) / kSellWaves);
+    const T* sval = sizeof(T) == 4 ? (const T*)s.sell.d_val32.p : (const T*)s.sell.d_val.p;
+#define DFM_SS(B)                                                                                              \
+  if (s.len == B) {                                                                                            \
+    hipLaunchKernelGGL((k_pcg_spmv_sell<B, T, REL>), dim3(G), dim3(64 * kSellWaves), 0, st, s.n, ns,           \
+                       s.sell.d_sptr.p, s.sell.d_col.p, sval, dia, p, q, slot, s.sell.d_val.p, s.d_dia.p, x64, b64, \
+                       r, slot_true, s.d_iflag.p);                                                             \
+    return hipGetLastError();                                                                                  \
+  }
+    DFM_SS(1) DFM_SS(2) DFM_SS(3)
+#undef DFM_SS
+    return hipErrorInvalidValue;
+  }
   const int tpr = spmv_tpr(s.ncrs, s.n);
   G = spmv_grid(s, tpr);
 #define DFM_SD(B, TP)                                                                                          \
