@@ -8,8 +8,9 @@ assembly -> Dirichlet rows/columns -> ILU(0) numeric factorisation -> PCG to a 1
 `value` is the mixed-precision path (FP32 matrix/factor/SpMV/ILU inside FP64 iterative refinement on the
 true residual); the plain FP64 path is timed beside it on the same system.
 
-The roofline object is for the dominant kernel of the mixed path's inner loop, the fused block-CRS SpMV +
-dot kernel, timed with HIP events on the library's stream. The CPU baseline is the oracle's single-threaded C
+The roofline object is for the dominant kernel family of the mixed path's inner loop, the ILU(0) phase
+sweeps, and `roofline_spmv` for the SELL SpMV, both timed with HIP events on the library's stream; `traffic` is
+the HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/c2_hbm_traffic.md). The CPU baseline is the oracle's single-threaded C
 restatement of the reference, timed on a bounded sample of the same workload (see its `sample` field).
 """
 import argparse
@@ -28,6 +29,27 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel_prefixes):
+    """HBM bytes per dispatch of the named kernels from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    of the C2 workload (profiles/c2_hbm_traffic.md, last column in MB), averaged over the kernels; None if the
+    summary or a kernel is missing."""
+    path = os.path.join(ROOT, "profiles", "c2_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("| `"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            vals[cells[0].strip("`")] = float(cells[-1]) * 1e6
+    got = []
+    for k in kernel_prefixes:
+        m = [v for name, v in vals.items() if name.startswith(k)]
+        if not m:
+            return None
+        got.append(m[0])
+    return sum(got) / len(got)
 
 
 def cpu_baseline(w, gpu_iters, budget_s):
@@ -287,7 +309,7 @@ def main():
             f"true rel residual {true_rel:.2e}, solve {st['ms']:.1f} ms, {st['ms_per_iter'] * 1e3:.1f} us/iter"
             + (f", reliable updates {nrel}" if name == "mixed" else "") + (", fell back to FP64" if fell_back else ""))
 
-    # dominant kernel of the mixed path: fused SpMV + dot in FP32 (timed with HIP events on the lib's stream)
+    # kernels of the mixed path's inner loop, timed with HIP events on the library's stream
     spmv32_ms, spmv32_b = s.bench_kernel(1, 50)
     spmv64_ms, spmv64_b = s.bench_kernel(0, 50)
     ilu32_ms, ilu32_b = s.bench_kernel(3, 20)
@@ -297,6 +319,11 @@ def main():
         f"({spmv64_b / spmv64_ms / 1e6:.0f} GB/s), ilu fp32 {ilu32_ms * 1e3:.1f} us "
         f"({ilu32_b / ilu32_ms / 1e6:.0f} GB/s), ilu fp64 {ilu64_ms * 1e3:.1f} us ({ilu64_b / ilu64_ms / 1e6:.0f} GB/s)")
 
+    nlaunch = nph[0] + nph[1]
+    sweep_traffic = spmv_traffic = None
+    if args.config == "c2" and args.ordering == "blockcolor":
+        sweep_traffic = measured_traffic(["dfm::k_sweep_fwd<1, float", "dfm::k_sweep_bwd<1, float"])
+        spmv_traffic = measured_traffic(["dfm::k_pcg_spmv_sell<1, float, false>"])
     mixed, fp64 = res["mixed"], res["fp64"]
     K = args.steps
     t_mixed = sum(mixed["lat"])
@@ -361,14 +388,28 @@ def main():
         "ms_per_pcg_iter_fp64": fp64["stats"]["ms_per_iter"],
         "solve_ms_mixed": mixed["stats"]["ms"],
         "solve_ms_fp64": fp64["stats"]["ms"],
+        # dominant kernel family of the mixed inner loop: the ILU(0) sweep phases (one launch per colour and
+        # direction); per launch = one apply's algorithmic bytes and HIP-event time over its launches
         "roofline": {
-            "kernel": "k_pcg_spmv_dot<B=%d,FP32> (fused block-CRS SpMV + dot)" % w["len"],
+            "kernel": "k_sweep_fwd/k_sweep_bwd<B=%d, FP32> (ILU(0) phase sweeps, %d launches per apply)"
+                      % (w["len"], nlaunch),
+            "bound": "hbm",
+            "achieved": ilu32_b / (ilu32_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": ilu32_b / (ilu32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": sweep_traffic,
+            "avg_launch_us": ilu32_ms * 1e3 / nlaunch,
+            "algorithmic_bytes": ilu32_b / nlaunch,
+        },
+        "roofline_spmv": {
+            "kernel": "k_pcg_spmv_sell<B=%d, FP32> (SELL-64 SpMV + dot)" % w["len"],
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": spmv_traffic,
             "avg_launch_us": spmv32_ms * 1e3,
             "algorithmic_bytes": spmv32_b,
         },
