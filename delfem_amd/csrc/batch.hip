@@ -217,8 +217,8 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
   }
 }
 
-// TF: factor values (FP64 or the FP32 copy); T: vectors and arithmetic
-template <int B, typename TF, typename T>
+// TF: factor values (FP64 or the FP32 copy); T: vectors and arithmetic; MO/MI: register-held entries per class
+template <int B, int MO, int MI, typename TF, typename T>
 __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
                                                      int count) {
   __shared__ T xsb[kWPB][64 * B];
@@ -227,11 +227,11 @@ __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ 
   if (e >= count) return;
   const int2 en = ent[e];
   const SegDev& d = segs[en.x];
-  sweep_fwd_wave<B, TF, T, T, T>(d.fa, en.y, (const TF*)d.fval, (const TF*)d.fdval, (const T*)d.r, (T*)d.z, d.fl,
+  sweep_fwd_wave<B, MO, MI, TF, T, T, T>(d.fa, en.y, (const TF*)d.fval, (const TF*)d.fdval, (const T*)d.r, (T*)d.z, d.fl,
                                  xsb[w]);
 }
 
-template <int B, typename TF, typename T>
+template <int B, int MO, int MI, typename TF, typename T>
 __global__ void __launch_bounds__(64 * kWPB) k_b_bwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
                                                      int count) {
   __shared__ T xsb[kWPB][64 * B];
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(64 * kWPB) k_b_bwd(const SegDev* __restrict__ 
   if (e >= count) return;
   const int2 en = ent[e];
   const SegDev& d = segs[en.x];
-  sweep_bwd_wave<B, TF, T, T>(d.ba, en.y, (const TF*)d.bval, (T*)d.z, d.fl, xsb[w]);
+  sweep_bwd_wave<B, MO, MI, TF, T, T>(d.ba, en.y, (const TF*)d.bval, (T*)d.z, d.fl, xsb[w]);
 }
 
 // partials of (r, z)
@@ -339,6 +339,7 @@ struct Batch {
     DBuf<int> d_sblk, d_vblk;
     DBuf<int2> d_fent, d_bent;
     std::vector<int> fptr, bptr;  // per phase offsets into the entry tables
+    std::vector<int> fwo, fwi, bwo, bwi;  // per phase widest classes over the segments
     int nsb = 0, nvb = 0;
     int mode = 0;
     DBuf<double> d_pq, d_true, d_rr, d_rz;
@@ -379,6 +380,7 @@ static int rebuild(Batch& bt) {
     std::vector<SegDev> h;
     std::vector<int> sblk, vblk;
     std::vector<std::vector<int2>> fph, bph;
+    std::vector<int> fwo, fwi, bwo, bwi;
     for (int k : g.members) {
       BatchSeg& sg = bt.segs[k];
       Sys& s = *sg.s;
@@ -416,7 +418,17 @@ static int rebuild(Batch& bt) {
       for (int dir = 0; dir < 2; dir++) {
         const Sweep& w = dir == 0 ? s.fwd : s.bwd;
         auto& ph = dir == 0 ? fph : bph;
+        auto& po = dir == 0 ? fwo : bwo;
+        auto& pi = dir == 0 ? fwi : bwi;
         if ((int)ph.size() < w.nphase()) ph.resize(w.nphase());
+        if ((int)po.size() < w.nphase()) {
+          po.resize(w.nphase(), 0);
+          pi.resize(w.nphase(), 0);
+        }
+        for (int p = 0; p < w.nphase(); p++) {
+          po[p] = std::max(po[p], w.phase_wo[p]);
+          pi[p] = std::max(pi[p], w.phase_wi[p]);
+        }
         for (int p = 0; p < w.nphase(); p++)
           for (int q = w.phase_gptr[p]; q < w.phase_gptr[p + 1]; q++) ph[p].push_back(make_int2(si, q));
       }
@@ -424,6 +436,10 @@ static int rebuild(Batch& bt) {
     }
     g.nsb = (int)sblk.size();
     g.nvb = (int)vblk.size();
+    g.fwo = fwo;
+    g.fwi = fwi;
+    g.bwo = bwo;
+    g.bwi = bwi;
     if (h.empty()) continue;
     std::vector<int2> fe, be;
     g.fptr.assign(1, 0);
@@ -471,13 +487,19 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
   hipLaunchKernelGGL((k_b_judge<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p, (const double*)g.d_true.p);
   for (size_t p = 0; p + 1 < g.fptr.size(); p++) {
     const int c = g.fptr[p + 1] - g.fptr[p];
-    if (c) hipLaunchKernelGGL((k_b_fwd<B, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
-                              (const int2*)g.d_fent.p + g.fptr[p], c);
+    if (c)
+      with_slab_caps(g.fwo[p], g.fwi[p], B == 3 && sizeof(T) == 8 && sizeof(TF) == 8, [&]<int MO, int MI>() {
+        hipLaunchKernelGGL((k_b_fwd<B, MO, MI, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
+                           (const int2*)g.d_fent.p + g.fptr[p], c);
+      });
   }
   for (size_t p = 0; p + 1 < g.bptr.size(); p++) {
     const int c = g.bptr[p + 1] - g.bptr[p];
-    if (c) hipLaunchKernelGGL((k_b_bwd<B, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
-                              (const int2*)g.d_bent.p + g.bptr[p], c);
+    if (c)
+      with_slab_caps(g.bwo[p], g.bwi[p], B == 3 && sizeof(T) == 8 && sizeof(TF) == 8, [&]<int MO, int MI>() {
+        hipLaunchKernelGGL((k_b_bwd<B, MO, MI, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
+                           (const int2*)g.d_bent.p + g.bptr[p], c);
+      });
   }
   hipLaunchKernelGGL((k_b_rz<T>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p, g.d_rz.p);
   hipLaunchKernelGGL((k_b_beta<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rz.p);

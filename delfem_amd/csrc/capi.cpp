@@ -250,6 +250,7 @@ int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, 
 
 static void copy_sweep_host(Sweep& d, const Sweep& a) {
   d.phase_gptr = a.phase_gptr; d.phase_nlev = a.phase_nlev; d.grp = a.grp; d.rowlist = a.rowlist; d.lev = a.lev;
+  d.phase_wo = a.phase_wo; d.phase_wi = a.phase_wi;
   d.gmeta = a.gmeta; d.srow = a.srow; d.scnt = a.scnt; d.ecol = a.ecol; d.esrc = a.esrc; d.nslab = a.nslab;
 }
 

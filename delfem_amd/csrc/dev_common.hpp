@@ -215,13 +215,10 @@ __device__ inline void blk_sub_reg(A (&t)[B], const A (&a)[B * B], const A (&v)[
   }
 }
 
-// Entries per row and class held in registers from the start of the kernel; further entries (rare) are read
-// from the slab when they are applied.
-template <int B, typename A>
-struct SlabCap {
-  static constexpr int OUT = B == 1 ? 4 : (B == 2 ? 6 : (sizeof(A) == 8 ? 4 : 6));
-  static constexpr int IN = B == 1 ? 4 : (B == 2 ? 6 : (sizeof(A) == 8 ? 5 : 7));
-};
+// MO / MI: entries per row of the two classes held in registers from the start of the kernel (template
+// parameters chosen per phase from its widest group: the level loop costs in proportion to them); further
+// entries are read from the slab when they are applied.
+inline int slab_cap(int w) { return w <= 4 ? 4 : 8; }
 
 constexpr int kWPB = 4;  // wavefronts (groups) per workgroup of the sweep kernels
 
@@ -279,12 +276,12 @@ __device__ inline void slab_load_blk(const T* __restrict__ val, size_t srow_, in
 // Loads of the register-held entries of one class: MX slab rows starting at r0, wave-uniform count w.
 template <int B, int MX, typename T, typename A>
 __device__ inline void slab_load_class(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int w,
-                                       int lane, unsigned (&col)[MX], A (&v)[MX][B * B]) {
+                                       int lane, unsigned (&col)[MX], T (&v)[MX][B * B]) {
 #pragma unroll
   for (int k = 0; k < MX; k++)
     if (k < w) {
       col[k] = ecol[(size_t)(r0 + k) * 64 + lane];
-      slab_load_blk<B, T, A>(val, (size_t)(r0 + k), lane, v[k]);
+      slab_load_blk<B, T, T>(val, (size_t)(r0 + k), lane, v[k]);
     }
 }
 
@@ -314,13 +311,13 @@ __device__ inline void slab_tail_global(const unsigned* __restrict__ ecol, const
 
 // t -= v y where the lane's own count says the entry exists (k < n); a select, not a branch, so the
 // wavefront never splits around the loads that feed it.
-template <int B, typename A>
-__device__ inline void blk_sub_if(bool on, A (&t)[B], const A (&a)[B * B], const A (&v)[B]) {
+template <int B, typename A, typename TV>
+__device__ inline void blk_sub_if(bool on, A (&t)[B], const TV (&a)[B * B], const A (&v)[B]) {
 #pragma unroll
   for (int r = 0; r < B; r++) {
-    A s = a[r * B] * v[0];
+    A s = (A)a[r * B] * v[0];
 #pragma unroll
-    for (int c = 1; c < B; c++) s += a[r * B + c] * v[c];
+    for (int c = 1; c < B; c++) s += (A)a[r * B + c] * v[c];
     const A u = t[r] - s;
     t[r] = on ? u : t[r];
   }
@@ -330,7 +327,7 @@ __device__ inline void blk_sub_if(bool on, A (&t)[B], const A (&a)[B * B], const
 // reads of the wave-uniform width w are issued together (padding columns point at lane 0).
 template <int B, int MX, typename T, typename A>
 __device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int w, int n,
-                                     int lane, const unsigned (&col)[MX], const A (&v)[MX][B * B], const A* xs,
+                                     int lane, const unsigned (&col)[MX], const T (&v)[MX][B * B], const A* xs,
                                      A (&t)[B]) {
   A y[MX][B];
 #pragma unroll
@@ -340,7 +337,7 @@ __device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T*
       for (int c = 0; c < B; c++) y[k][c] = xs[col[k] * B + c];
 #pragma unroll
   for (int k = 0; k < MX; k++)
-    if (k < w) blk_sub_if<B, A>(k < n, t, v[k], y[k]);
+    if (k < w) blk_sub_if<B, A, T>(k < n, t, v[k], y[k]);
   for (int k = MX; k < n; k++) {
     const size_t sr = (size_t)(r0 + k);
     const unsigned j = ecol[sr * 64 + lane];
@@ -354,15 +351,15 @@ __device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T*
 
 // Forward sweep of group g by the calling wavefront: x = D^-1 (b - L x) for the group's rows. b and x may alias.
 // xs: the wavefront's LDS row (64*B values).
-template <int B, typename T, typename A, typename VI, typename VO>
+template <int B, int MO, int MI, typename T, typename A, typename VI, typename VO>
 __device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, const T* __restrict__ dval,
                                       const VI* b, VO* x, const int* __restrict__ done, A* xs) {
   constexpr int BS = B * B;
-  constexpr int MO = SlabCap<B, A>::OUT, MI = SlabCap<B, A>::IN;
   SlabSlot ss;
   if (!slab_slot(sa, g, done, ss)) return;
   A t[B], D[BS];
-  A vo[MO][BS], xo[MO][B], vi[MI][BS];
+  T vo[MO][BS], vi[MI][BS];
+  A xo[MO][B];
   unsigned jo[MO], li[MI];
 #pragma unroll
   for (int r = 0; r < B; r++) t[r] = (A)b[(size_t)ss.row * B + r];
@@ -373,7 +370,7 @@ __device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __rest
   slab_gather<B, MO, A, VO>(x, ss.wo, jo, xo);
 #pragma unroll
   for (int k = 0; k < MO; k++)
-    if (k < ss.wo) blk_sub_if<B, A>(k < ss.no, t, vo[k], xo[k]);
+    if (k < ss.wo) blk_sub_if<B, A, T>(k < ss.no, t, vo[k], xo[k]);
   slab_tail_global<B, MO, T, A, VO>(sa.ecol, val, ss.off, ss.no, ss.lane, x, t);
   for (int l = 0; l < ss.nlev; l++) {
     if (ss.lv == l) {
@@ -401,15 +398,15 @@ __device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __rest
 
 // Backward sweep of group g by the calling wavefront, in place on x: x_i -= U_in x_in (in-group, ascending) then
 // U_out x_out.
-template <int B, typename T, typename A, typename VX>
+template <int B, int MO, int MI, typename T, typename A, typename VX>
 __device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, VX* x,
                                       const int* __restrict__ done, A* xs) {
   constexpr int BS = B * B;
-  constexpr int MO = SlabCap<B, A>::OUT, MI = SlabCap<B, A>::IN;
   SlabSlot ss;
   if (!slab_slot(sa, g, done, ss)) return;
   A t[B];
-  A vo[MO][BS], xo[MO][B], vi[MI][BS];
+  T vo[MO][BS], vi[MI][BS];
+  A xo[MO][B];
   unsigned jo[MO], li[MI];
 #pragma unroll
   for (int r = 0; r < B; r++) t[r] = (A)x[(size_t)ss.row * B + r];
@@ -421,7 +418,7 @@ __device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __rest
       slab_in_group<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.ni, ss.lane, li, vi, xs, t);
 #pragma unroll
       for (int k = 0; k < MO; k++)
-        if (k < ss.wo) blk_sub_if<B, A>(k < ss.no, t, vo[k], xo[k]);
+        if (k < ss.wo) blk_sub_if<B, A, T>(k < ss.no, t, vo[k], xo[k]);
       slab_tail_global<B, MO, T, A, VX>(sa.ecol, val, ss.off, ss.no, ss.lane, x, t);
 #pragma unroll
       for (int r = 0; r < B; r++) xs[ss.lane * B + r] = t[r];
@@ -434,7 +431,7 @@ __device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __rest
 }
 
 // One phase of one system: groups [gbase, gend), kWPB groups per workgroup.
-template <int B, typename T, typename A, typename VI, typename VO>
+template <int B, int MO, int MI, typename T, typename A, typename VI, typename VO>
 __global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
                                                       const T* __restrict__ dval, const VI* b, VO* x,
                                                       const int* __restrict__ done) {
@@ -442,17 +439,28 @@ __global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, Sl
   const int w = wave_in_block();
   const int g = gbase + (int)blockIdx.x * kWPB + w;
   if (g >= gend) return;
-  sweep_fwd_wave<B, T, A, VI, VO>(sa, g, val, dval, b, x, done, xsb[w]);
+  sweep_fwd_wave<B, MO, MI, T, A, VI, VO>(sa, g, val, dval, b, x, done, xsb[w]);
 }
 
-template <int B, typename T, typename A, typename VX>
+template <int B, int MO, int MI, typename T, typename A, typename VX>
 __global__ void __launch_bounds__(64 * kWPB) k_sweep_bwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
                                                       VX* x, const int* __restrict__ done) {
   __shared__ A xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int g = gbase + (int)blockIdx.x * kWPB + w;
   if (g >= gend) return;
-  sweep_bwd_wave<B, T, A, VX>(sa, g, val, x, done, xsb[w]);
+  sweep_bwd_wave<B, MO, MI, T, A, VX>(sa, g, val, x, done, xsb[w]);
+}
+
+// Launch helper: calls f.template operator()<MO, MI>() for the caps of a phase's widths (wo, wi). FP64
+// arithmetic on 3x3 blocks keeps caps of 4 (registers: wider caps drop occupancy to one wave per SIMD).
+template <typename F>
+inline void with_slab_caps(int wo, int wi, bool narrow, F&& f) {
+  const int co = narrow ? 4 : slab_cap(wo), ci = narrow ? 4 : slab_cap(wi);
+  if (co == 4 && ci == 4) f.template operator()<4, 4>();
+  else if (co == 4) f.template operator()<4, 8>();
+  else if (ci == 4) f.template operator()<8, 4>();
+  else f.template operator()<8, 8>();
 }
 
 }  // namespace dfm

@@ -88,6 +88,7 @@ struct Sweep {
   std::vector<unsigned> ecol;    // [64*nslab] column (outside) or source lane (in-group)
   std::vector<unsigned> esrc;    // [64*nslab] factor block index of the entry, ~0u for padding
   long long nslab = 0;
+  std::vector<int> phase_wo, phase_wi;  // [nphase] widest classes of the phase's groups
   DBuf<int> d_gmeta, d_srow;
   DBuf<unsigned> d_scnt, d_ecol, d_esrc;
   DBuf<double> d_eval, d_dval;   // FP64 values (d_dval: forward sweep only, D^-1 per slot)

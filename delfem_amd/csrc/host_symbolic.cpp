@@ -273,6 +273,13 @@ static void build_slabs(const Sys& s, Sweep& w, bool forward) {
     off += wo + wi;
   }
   w.nslab = off;
+  w.phase_wo.assign(w.phase_nlev.size(), 0);
+  w.phase_wi.assign(w.phase_nlev.size(), 0);
+  for (size_t p = 0; p < w.phase_wo.size(); p++)
+    for (int g = w.phase_gptr[p]; g < w.phase_gptr[p + 1]; g++) {
+      w.phase_wo[p] = std::max(w.phase_wo[p], w.gmeta[4 * g + 1] & 255);
+      w.phase_wi[p] = std::max(w.phase_wi[p], (w.gmeta[4 * g + 1] >> 8) & 255);
+    }
 }
 
 // SELL-64 layout of the device-numbering pattern (see Sell).

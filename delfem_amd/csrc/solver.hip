@@ -459,8 +459,10 @@ static hipError_t precond_T(const Sys& s, const V* src, V* z, const int* done, h
   for (int p = 0; p < f.nphase(); p++) {
     const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_sweep_fwd<B, TV, V, V, V>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
-                       fa, fval, fdval, src, z, done);
+    with_slab_caps(f.phase_wo[p], f.phase_wi[p], B == 3 && sizeof(V) == 8 && sizeof(TV) == 8, [&]<int MO, int MI>() {
+      hipLaunchKernelGGL((k_sweep_fwd<B, MO, MI, TV, V, V, V>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st,
+                         g0, g0 + ng, fa, fval, fdval, src, z, done);
+    });
   }
   const Sweep& b = s.bwd;
   const SlabArgs ba{b.d_gmeta.p, b.d_srow.p, b.d_scnt.p, b.d_ecol.p};
@@ -468,8 +470,10 @@ static hipError_t precond_T(const Sys& s, const V* src, V* z, const int* done, h
   for (int p = 0; p < b.nphase(); p++) {
     const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0;
     if (!ng) continue;
-    hipLaunchKernelGGL((k_sweep_bwd<B, TV, V, V>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0, g0 + ng,
-                       ba, bval, z, done);
+    with_slab_caps(b.phase_wo[p], b.phase_wi[p], B == 3 && sizeof(V) == 8 && sizeof(TV) == 8, [&]<int MO, int MI>() {
+      hipLaunchKernelGGL((k_sweep_bwd<B, MO, MI, TV, V, V>), dim3((ng + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, g0,
+                         g0 + ng, ba, bval, z, done);
+    });
   }
   return hipGetLastError();
 }
