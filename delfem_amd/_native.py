@@ -309,6 +309,27 @@ class System:
         ok = _check(lib().dfm_solve_pbicgstab(self.h, C.byref(conv), C.byref(it)))
         return bool(ok), conv.value, it.value
 
+    def _solve_fn(self, fn, tol, max_iter):
+        conv = C.c_double(tol)
+        it = C.c_uint(max_iter)
+        ok = _check(fn(self.h, C.byref(conv), C.byref(it)))
+        return bool(ok), conv.value, it.value
+
+    def solve_cg(self, tol, max_iter):
+        """LsSol::Solve_CG (no preconditioner), host-driven over the device handle operations."""
+        return self._solve_fn(lib().dfm_solve_cg, tol, max_iter)
+
+    def solve_bicgstab(self, tol, max_iter):
+        """LsSol::Solve_BiCGSTAB (no preconditioner), host-driven."""
+        return self._solve_fn(lib().dfm_solve_bicgstab, tol, max_iter)
+
+    def solve_pcg_host(self, tol, max_iter):
+        """LsSol::Solve_PCG driven from the host over the seam (C++ layer)."""
+        return self._solve_fn(lib().dfm_solve_pcg_host, tol, max_iter)
+
+    def solve_pbicgstab_host(self, tol, max_iter):
+        return self._solve_fn(lib().dfm_solve_pbicgstab_host, tol, max_iter)
+
     def true_residual(self):
         rel = C.c_double()
         _check(lib().dfm_sys_true_residual(self.h, C.byref(rel)))

@@ -153,6 +153,14 @@ int dfm_prec_get_factor(dfm_sys sys, unsigned* frowptr, unsigned* fcolind, unsig
  * residual r = b - A x (FP64 matrix, FP64 dots and update); its conv_ratio is the true relative residual. */
 int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration);
 int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+/* Host-driven loops over the handle operations (the C++ layer include/dfm_lssol.hpp), FP64, same contract as
+ * the reference functions they follow: LsSol::Solve_CG and Solve_BiCGSTAB without preconditioner
+ * (src/ls/solver_ls_iter.cpp:31-97, :177-286; both return 1 also when max_iter is reached), and Solve_PCG /
+ * Solve_PBiCGSTAB with the ILU(0) preconditioner (:100-172, :291-408) as the reference drives them. */
+int dfm_solve_cg(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+int dfm_solve_bicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+int dfm_solve_pcg_host(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+int dfm_solve_pbicgstab_host(dfm_sys sys, double* conv_ratio, unsigned* iteration);
 /* ||b - A x|| / ||b|| in FP64 for b = the residual as assembled and x = the update (after a solve). */
 int dfm_sys_true_residual(dfm_sys sys, double* rel);
 

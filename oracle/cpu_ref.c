@@ -1053,3 +1053,105 @@ done:
   free(buf);
   return ret;
 }
+
+/* Solve_CG (src/ls/solver_ls_iter.cpp:31-97): no preconditioner; guard 1e-60 returns ||r0|| as conv_ratio;
+ * x updated before r; returns 1 also when max_iter is reached (conv_ratio then keeps its input value). */
+int cr_solve_cg(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const unsigned max_iter = *iter;
+  const double tol = *conv_ratio;
+  const long m = (long)s->n * s->len;
+  double* p = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* ap = (double*)calloc((size_t)m + 1, sizeof(double));
+  v_scal(m, 0.0, x);
+  double sq = v_dot(m, r, r);
+  if (sq < 1.0e-60) {
+    *iter = 0;
+    *conv_ratio = sqrt(sq);
+    free(p);
+    free(ap);
+    return 1;
+  }
+  const double sq_inv0 = 1.0 / sq;
+  memcpy(p, r, sizeof(double) * m);
+  unsigned it;
+  for (it = 0; it < max_iter; it++) {
+    double alpha;
+    {
+      matvec(s, 1.0, p, 0.0, ap);
+      const double pap = v_dot(m, p, ap);
+      alpha = sq / pap;
+    }
+    v_axpy(m, alpha, p, x);
+    v_axpy(m, -alpha, ap, r);
+    const double sq_new = v_dot(m, r, r);
+    if (sq_new * sq_inv0 < tol * tol) {
+      *conv_ratio = sqrt(sq_new * sq_inv0);
+      break;
+    }
+    const double beta = sq_new / sq;
+    sq = sq_new;
+    v_scal(m, beta, p);
+    v_axpy(m, 1.0, r, p);
+  }
+  *iter = it;
+  free(p);
+  free(ap);
+  return 1;
+}
+
+/* Solve_BiCGSTAB (src/ls/solver_ls_iter.cpp:177-286): no preconditioner; loop from 1; iteration = max_iter
+ * unless converged; returns 1 also when max_iter is reached. */
+int cr_solve_bicgstab(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const unsigned max_iter = *iter;
+  const double tol = *conv_ratio;
+  const long m = (long)s->n * s->len;
+  double* buf = (double*)calloc((size_t)5 * m + 1, sizeof(double));
+  double *vs = buf, *vas = buf + m, *vp = buf + 2 * m, *vap = buf + 3 * m, *vr2 = buf + 4 * m;
+  v_scal(m, 0.0, x);
+  double sq_inv0;
+  {
+    const double sq0 = v_dot(m, r, r);
+    if (sq0 < 1.0e-60) {
+      *conv_ratio = sqrt(sq0);
+      *iter = 0;
+      free(buf);
+      return 1;
+    }
+    sq_inv0 = 1.0 / sq0;
+  }
+  memcpy(vr2, r, sizeof(double) * m);
+  memcpy(vp, r, sizeof(double) * m);
+  double r_r2 = v_dot(m, r, vr2);
+  *iter = max_iter;
+  for (unsigned it = 1; it < max_iter; it++) {
+    matvec(s, 1.0, vp, 0.0, vap);
+    const double alpha = r_r2 / v_dot(m, vap, vr2);
+    memcpy(vs, r, sizeof(double) * m);
+    v_axpy(m, -alpha, vap, vs);
+    matvec(s, 1.0, vs, 0.0, vas);
+    const double den = v_dot(m, vas, vas);
+    const double num = v_dot(m, vas, vs);
+    const double omega = num / den;
+    v_axpy(m, alpha, vp, x);
+    v_axpy(m, omega, vs, x);
+    memcpy(r, vs, sizeof(double) * m);
+    v_axpy(m, -omega, vas, r);
+    {
+      const double sq = v_dot(m, r, r);
+      const double sqc = sq * sq_inv0;
+      if (sqc < tol * tol) {
+        *conv_ratio = sqrt(sqc);
+        *iter = it;
+        break;
+      }
+    }
+    const double tmp1 = v_dot(m, r, vr2);
+    const double beta = (tmp1 * alpha) / (r_r2 * omega);
+    r_r2 = tmp1;
+    v_scal(m, beta, vp);
+    v_axpy(m, 1.0, r, vp);
+    v_axpy(m, -beta * omega, vap, vp);
+  }
+  free(buf);
+  return 1;
+}

@@ -77,6 +77,9 @@ typedef struct {
  * conv_ratio and iter are in/out as in the reference. Returns the reference's bool. */
 int cr_solve_pcg(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
 int cr_solve_pbicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+/* unpreconditioned Solve_CG / Solve_BiCGSTAB (the factor fields of s are not used) */
+int cr_solve_cg(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+int cr_solve_bicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,8 @@ class CpuSystem:
         return v
 
     def _cr_system(self, f):
+        if f is None:  # unpreconditioned solvers: no factor
+            f = {"frowptr": None, "fcolind": None, "diaind": None, "fcrs": None, "fdia": None, "order": None}
         class S(C.Structure):
             _fields_ = [("n", C.c_int), ("len", C.c_int), ("rowptr", C.c_void_p), ("colind", C.c_void_p),
                         ("crs", C.c_void_p), ("dia", C.c_void_p), ("frowptr", C.c_void_p),
@@ -143,7 +145,8 @@ class CpuSystem:
         x = np.zeros_like(r)
         conv = C.c_double(tol)
         it = C.c_uint(max_iter)
-        fn = cpu_ref().cr_solve_pcg if method == "pcg" else cpu_ref().cr_solve_pbicgstab
+        fn = {"pcg": cpu_ref().cr_solve_pcg, "pbicgstab": cpu_ref().cr_solve_pbicgstab, "cg": cpu_ref().cr_solve_cg,
+              "bicgstab": cpu_ref().cr_solve_bicgstab}[method]
         ok = fn(C.byref(s), r.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p), C.byref(conv),
                 C.byref(it))
         return bool(ok), conv.value, it.value, x, r
@@ -277,8 +280,8 @@ class RefSystem:
         it = C.c_uint(max_iter)
         order = None if order is None else _i32(order)
         p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
-        ok = reference().ref_sys_solve(self.h, C.c_int(0 if method == "pcg" else 1), C.c_int(lev), p(order),
-                                       C.byref(conv), C.byref(it))
+        code = {"pcg": 0, "pbicgstab": 1, "cg": 2, "bicgstab": 3}[method]
+        ok = reference().ref_sys_solve(self.h, C.c_int(code), C.c_int(lev), p(order), C.byref(conv), C.byref(it))
         return ok, conv.value, it.value
 
     def update_field(self):

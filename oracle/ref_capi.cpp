@@ -302,8 +302,11 @@ int ref_ilu_apply(void* h, int lev, const int* order, const double* r, double* z
 // Solve with the reference's Krylov loops on the assembled system: method 0 Solve_PCG, 1 Solve_PBiCGSTAB.
 // lev = ILU fill level (ignored with an ordering). Returns the solver's bool; conv/iter are in/out as
 // in the reference. The update (handle -2) is left in the system (read it with ref_sys_get_vector).
+// method: 0 Solve_PCG, 1 Solve_PBiCGSTAB (ILU(lev) under `order`), 2 Solve_CG, 3 Solve_BiCGSTAB (no preconditioner)
 int ref_sys_solve(void* h, int method, int lev, const int* order, double* conv_ratio, unsigned int* iter) {
   RefSys* s = (RefSys*)h;
+  if (method == 2) return LsSol::Solve_CG(*conv_ratio, *iter, s->ls.m_ls) ? 1 : 0;
+  if (method == 3) return LsSol::Solve_BiCGSTAB(*conv_ratio, *iter, s->ls.m_ls) ? 1 : 0;
   LsSol::CPreconditioner_ILU prec;
   prec.SetFillInLevel(lev);
   if (order) prec.SetOrdering(std::vector<int>(order, order + s->nnode));

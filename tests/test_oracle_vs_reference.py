@@ -163,3 +163,25 @@ def test_pcg_nonconvergence_matches():
     ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-12, 2)
     rok, rconv, rit = ref.solve("pcg", 1e-12, 2, 0, None)
     assert not ok and rok == 0 and it == rit == 2 and conv == rconv
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_unpreconditioned_krylov_bit_exact(ndim, length, nxyz, method):
+    """Solve_CG / Solve_BiCGSTAB (no preconditioner): iterates, iteration count and conv_ratio."""
+    ref, cpu = _assembled(ndim, length, nxyz)
+    ok, conv, it, x, _ = cpu.solve(None, method, 1e-10, 2000)
+    rok, rconv, rit = ref.solve(method, 1e-10, 2000)
+    assert (ok, conv, it) == (bool(rok), rconv, rit)
+    assert np.array_equal(ref.vector(-2), x)
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_unpreconditioned_max_iter_quirks(method):
+    """Not converged: both return true; CG leaves conv_ratio at its input and iteration at max_iter;
+    BiCGSTAB (loop from 1) leaves iteration at max_iter."""
+    ref, cpu = _assembled(2, 2, (9, 4))
+    ok, conv, it, x, _ = cpu.solve(None, method, 1e-14, 3)
+    rok, rconv, rit = ref.solve(method, 1e-14, 3)
+    assert ok and rok == 1 and it == rit == 3 and conv == rconv == 1e-14
+    assert np.array_equal(ref.vector(-2), x)
