@@ -58,7 +58,6 @@ constexpr int tpr_of() {
 
 // scalar slots of the batch loop beyond those of pcg_start (rho stays in S_RZ, the reliable-update maximum in
 // S_RMAX: only the per-segment reduction kernels write them, one workgroup per segment)
-enum { S_ALPHA = 9, S_BETA = 10 };
 
 // block-wide sum of the partials slot[b0..b1), identical in every thread
 __device__ inline double consume_range(const double* __restrict__ slot, int b0, int b1) {

@@ -28,7 +28,7 @@ void DevAlloc::release(void* p) {
   size_t sz = 0;
   // the size is recovered from the allocation itself for the accounting
   if (hipMemPtrGetInfo(p, &sz) == hipSuccess) g_dev_bytes -= (long long)sz;
-  hipFree(p);
+  (void)hipFree(p);
 }
 
 long long device_bytes_in_use() { return g_dev_bytes.load(); }
@@ -41,14 +41,14 @@ Sys::~Sys() {
   d_fcrs.release(); d_fdia.release();
   d_res.release(); d_upd.release(); d_rhs.release();
   for (auto* t : d_tmp) { t->release(); delete t; }
-  d_work.release(); d_work32.release(); d_partial.release(); d_counter.release(); d_scal.release(); d_iflag.release();
+  d_work.release(); d_work32.release(); d_partial.release(); d_part_sw.release(); d_counter.release(); d_scal.release(); d_iflag.release();
   for (Sweep* w : {&fwd, &bwd}) w->release_dev();
   sell.release_dev();
   d_par.release(); d_lptr.release(); d_loads.release();
   for (auto& g : gexec)
     if (g) (void)hipGraphExecDestroy(g);
-  if (h_pinned) hipHostFree(h_pinned);
-  if (h_pinned_d) hipHostFree(h_pinned_d);
+  if (h_pinned) (void)hipHostFree(h_pinned);
+  if (h_pinned_d) (void)hipHostFree(h_pinned_d);
 }
 
 template <typename T>

@@ -349,20 +349,21 @@ __device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T*
   }
 }
 
-// Forward sweep of group g by the calling wavefront: x = D^-1 (b - L x) for the group's rows. b and x may alias.
-// xs: the wavefront's LDS row (64*B values).
-template <int B, int MO, int MI, typename T, typename A, typename VI, typename VO>
-__device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, const T* __restrict__ dval,
-                                      const VI* b, VO* x, const int* __restrict__ done, A* xs) {
+// Forward sweep of group g by the calling wavefront: x = D^-1 (b - L x) for the group's rows. `rhs(ss, t)`
+// fills t with the lane's right-hand side (ss.row; also called for lanes without a row, whose row index is then
+// a valid one); on return t holds the lane's solution, which x has received. xs: the wavefront's LDS row
+// (64*B values). Returns false when the done flag is set.
+template <int B, int MO, int MI, typename T, typename A, typename VO, typename Rhs>
+__device__ inline bool sweep_fwd_core(const SlabArgs& sa, int g, const T* __restrict__ val, const T* __restrict__ dval,
+                                      Rhs&& rhs, VO* x, const int* __restrict__ done, A* xs, SlabSlot& ss,
+                                      A (&t)[B]) {
   constexpr int BS = B * B;
-  SlabSlot ss;
-  if (!slab_slot(sa, g, done, ss)) return;
-  A t[B], D[BS];
+  if (!slab_slot(sa, g, done, ss)) return false;
+  A D[BS];
   T vo[MO][BS], vi[MI][BS];
   A xo[MO][B];
   unsigned jo[MO], li[MI];
-#pragma unroll
-  for (int r = 0; r < B; r++) t[r] = (A)b[(size_t)ss.row * B + r];
+  rhs(ss, t);
 #pragma unroll
   for (int e = 0; e < BS; e++) D[e] = (A)dval[((size_t)ss.g * BS + e) * 64 + ss.lane];
   slab_load_class<B, MO, T, A>(sa.ecol, val, ss.off, ss.wo, ss.lane, jo, vo);
@@ -394,17 +395,31 @@ __device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __rest
   if (ss.active)
 #pragma unroll
     for (int r = 0; r < B; r++) x[(size_t)ss.row * B + r] = (VO)t[r];
+  return true;
+}
+
+// The forward sweep with the right-hand side b (b and x may alias).
+template <int B, int MO, int MI, typename T, typename A, typename VI, typename VO>
+__device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, const T* __restrict__ dval,
+                                      const VI* b, VO* x, const int* __restrict__ done, A* xs) {
+  SlabSlot ss;
+  A t[B];
+  sweep_fwd_core<B, MO, MI, T, A, VO>(
+      sa, g, val, dval,
+      [&](const SlabSlot& s, A(&u)[B]) {
+#pragma unroll
+        for (int r = 0; r < B; r++) u[r] = (A)b[(size_t)s.row * B + r];
+      },
+      x, done, xs, ss, t);
 }
 
 // Backward sweep of group g by the calling wavefront, in place on x: x_i -= U_in x_in (in-group, ascending) then
-// U_out x_out.
+// U_out x_out. On return t holds the lane's result (for lanes with a row). Returns false when the done flag is set.
 template <int B, int MO, int MI, typename T, typename A, typename VX>
-__device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, VX* x,
-                                      const int* __restrict__ done, A* xs) {
+__device__ inline bool sweep_bwd_core(const SlabArgs& sa, int g, const T* __restrict__ val, VX* x,
+                                      const int* __restrict__ done, A* xs, SlabSlot& ss, A (&t)[B]) {
   constexpr int BS = B * B;
-  SlabSlot ss;
-  if (!slab_slot(sa, g, done, ss)) return;
-  A t[B];
+  if (!slab_slot(sa, g, done, ss)) return false;
   T vo[MO][BS], vi[MI][BS];
   A xo[MO][B];
   unsigned jo[MO], li[MI];
@@ -428,6 +443,15 @@ __device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __rest
   if (ss.active)
 #pragma unroll
     for (int r = 0; r < B; r++) x[(size_t)ss.row * B + r] = (VX)t[r];
+  return true;
+}
+
+template <int B, int MO, int MI, typename T, typename A, typename VX>
+__device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __restrict__ val, VX* x,
+                                      const int* __restrict__ done, A* xs) {
+  SlabSlot ss;
+  A t[B];
+  sweep_bwd_core<B, MO, MI, T, A, VX>(sa, g, val, x, done, xs, ss, t);
 }
 
 // One phase of one system: groups [gbase, gend), kWPB groups per workgroup.

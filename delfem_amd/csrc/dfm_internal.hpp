@@ -153,6 +153,7 @@ struct Sys {
   DBuf<double> d_work;               // solver work vectors
   DBuf<float> d_work32;
   DBuf<double> d_partial;            // per-block partial sums
+  DBuf<double> d_part_sw;            // per-workgroup partial sums of the sweep phases (forward, then backward)
   DBuf<unsigned> d_counter;          // last-block counters
   DBuf<double> d_scal;               // device scalars of the Krylov loops
   DBuf<int> d_iflag;                 // device int scalars (done flag, iteration)
@@ -170,7 +171,7 @@ struct Sys {
   // captured chunks of Krylov iterations (hipGraph), per loop kind: 0 PCG FP64, 1 PCG mixed, 2 PBiCGSTAB,
   // 3 PCG FP64 with the FP32 factor
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
-  const void* gkey[4] = {nullptr, nullptr, nullptr, nullptr};  // work buffer the graph was captured with
+  std::vector<const void*> gkey[4];  // every device pointer the graph was captured with
   int gchunk = 16;
 
   bool assembled = false, factored = false, factored32 = false;

@@ -177,136 +177,166 @@ __global__ void __launch_bounds__(64 * kSellWaves)
   if (REL && rel) produce<64 * kSellWaves, 1>(slot_true, vt);
 }
 
-// alpha = rho / (p, q); r -= alpha q; x += alpha p; partials of (r, r)   (solver_ls_iter.cpp:139-144)
-template <typename T, typename TX>
-__global__ void __launch_bounds__(NT) k_pcg_update(long long m, T* r, TX* x, const T* q, const T* p,
-                                                   const double* slot_pq, int Gpq, double* slot_rr, const double* sc,
-                                                   const int* fl) {
-  if (fl[F_DONE]) return;
-  double pq[1];
-  consume<NT, 1>(slot_pq, Gpq, pq);
-  const int it = fl[F_IT];
-  const double alpha = sc[S_RZ + (it & 1)] / pq[0];
-  const T ma = (T)(-alpha), pa = (T)alpha;
-  double v[1] = {0.0};
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
-    const T ri = r[i] + ma * q[i];
-    r[i] = ri;
-    x[i] = x[i] + (TX)(pa * p[i]);
-    v[0] += (double)ri * (double)ri;
+// ---- PCG iteration with the vector work folded into the sweeps ---------------------------------------------
+// The update r -= alpha q, x += alpha p touches each row once, so every forward phase does it for its own rows
+// (solver_ls_iter.cpp:139-144) and yields the partials of (r, r); every backward phase yields the partials
+// of (r, z) for the rows it finalises. One-workgroup kernels form alpha, and the judgement with beta, from the
+// partials. Sweep partials: one per workgroup, phase after phase.
+
+// One-workgroup reductions of per-workgroup partials: every thread sums a fixed strided subset of each list with
+// four independent accumulators (loads in flight together), then the workgroup sums; the order is fixed, so the
+// totals are reproducible. Lists with G = 0 give 0.
+constexpr int kRedNT = 1024;
+
+template <int K>
+__device__ inline void sum_partials(const double* const (&slot)[K], const int (&G)[K], double (&tot)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = (int)threadIdx.x;
+    for (; b + 3 * kRedNT < G[k]; b += 4 * kRedNT) {
+      s0 += slot[k][b];
+      s1 += slot[k][b + kRedNT];
+      s2 += slot[k][b + 2 * kRedNT];
+      s3 += slot[k][b + 3 * kRedNT];
+    }
+    for (; b < G[k]; b += kRedNT) s0 += slot[k][b];
+    tot[k] = (s0 + s1) + (s2 + s3);
   }
-  produce<NT, 1>(slot_rr, v);
+  block_allsum<kRedNT, K>(tot);
 }
 
-// Convergence judgement on (r, r) (solver_ls_iter.cpp:146-155), then partials of (r, z) for beta.
-__global__ void __launch_bounds__(NT) k_pcg_judge_rz(long long m, const double* r, const double* z,
-                                                     const double* slot_rr, int G, double* slot_rz, double* sc, int* fl) {
+// alpha = rho / (p, q)   (solver_ls_iter.cpp:137-138)
+__global__ void __launch_bounds__(kRedNT) k_pcg_alpha(const double* slot_pq, int G, double* sc, const int* fl) {
   if (fl[F_DONE]) return;
-  double rr[1];
-  consume<NT, 1>(slot_rr, G, rr);
+  const double* const sl[1] = {slot_pq};
+  const int g[1] = {G};
+  double pq[1];
+  sum_partials<1>(sl, g, pq);
+  if (threadIdx.x == 0) sc[S_ALPHA] = sc[S_RZ + (fl[F_IT] & 1)] / pq[0];
+}
+
+// forward phase with the update of its rows: r_i -= alpha q_i, x_i += alpha p_i, then z = L-sweep of r; the new
+// r and x are kept in registers and stored at the end, so every load of the phase is issued up front.
+// q and z may be one buffer: a lane reads q of its rows before the sweep writes z there, and the sweep reads z
+// only of rows finished in earlier phases or in LDS.
+template <int B, int MO, int MI, typename TV, typename T>
+__global__ void __launch_bounds__(64 * kWPB)
+    k_fwd_update(int gbase, int gend, SlabArgs sa, const TV* __restrict__ val, const TV* __restrict__ dval, T* r,
+                 const T* q, const T* p, T* x, T* z, const double* sc, const int* fl, double* slot_rr) {
+  __shared__ T xsb[kWPB][64 * B];
+  if (fl[F_DONE]) return;
+  const int w = wave_in_block();
+  const int g = gbase + (int)blockIdx.x * kWPB + w;
+  double rr = 0.0;
+  if (g < gend) {
+    const double alpha = sc[S_ALPHA];
+    const T ma = (T)(-alpha), pa = (T)alpha;
+    T rn[B], xn[B];
+    SlabSlot ss;
+    T t[B];
+    sweep_fwd_core<B, MO, MI, TV, T, T>(
+        sa, g, val, dval,
+        [&](const SlabSlot& s, T(&u)[B]) {
+#pragma unroll
+          for (int k = 0; k < B; k++) {
+            const size_t i = (size_t)s.row * B + k;
+            rn[k] = r[i] + ma * q[i];
+            xn[k] = x[i] + pa * p[i];
+            u[k] = rn[k];
+          }
+        },
+        z, nullptr, xsb[w], ss, t);
+    if (ss.active)
+#pragma unroll
+      for (int k = 0; k < B; k++) {
+        const size_t i = (size_t)ss.row * B + k;
+        r[i] = rn[k];
+        x[i] = xn[k];
+        rr += (double)rn[k] * (double)rn[k];
+      }
+  }
+  double v[1] = {rr};
+  block_allsum<64 * kWPB, 1>(v);
+  if (threadIdx.x == 0) slot_rr[blockIdx.x] = v[0];
+}
+
+// backward phase, then the partials of (r, z) of its rows
+template <int B, int MO, int MI, typename TV, typename T>
+__global__ void __launch_bounds__(64 * kWPB)
+    k_bwd_rz(int gbase, int gend, SlabArgs sa, const TV* __restrict__ val, const T* __restrict__ r, T* z,
+             const int* fl, double* slot_rz) {
+  __shared__ T xsb[kWPB][64 * B];
+  if (fl[F_DONE]) return;
+  const int w = wave_in_block();
+  const int g = gbase + (int)blockIdx.x * kWPB + w;
+  double rz = 0.0;
+  if (g < gend) {
+    SlabSlot ss;
+    T t[B];
+    sweep_bwd_core<B, MO, MI, TV, T, T>(sa, g, val, z, nullptr, xsb[w], ss, t);
+    if (ss.active)
+#pragma unroll
+      for (int k = 0; k < B; k++) rz += (double)r[(size_t)ss.row * B + k] * (double)t[k];
+  }
+  double v[1] = {rz};
+  block_allsum<64 * kWPB, 1>(v);
+  if (threadIdx.x == 0) slot_rz[blockIdx.x] = v[0];
+}
+
+// Judgement and beta in one workgroup. FP64 (solver_ls_iter.cpp:146-166): on the recursive residual. Mixed: on
+// the true residual when one was formed this iteration, else the recursive residual only schedules the next
+// reliable update; the scheduled update becomes current for the next SpMV.
+template <bool MX>
+__global__ void __launch_bounds__(kRedNT) k_pcg_judge_beta(const double* slot_rr, int Grr, const double* slot_true,
+                                                           int Gt, const double* slot_rz, int Grz, double* sc, int* fl) {
+  if (fl[F_DONE]) return;
+  const double* const sl[3] = {slot_rr, slot_rz, slot_true};
+  const int g[3] = {Grr, Grz, MX && fl[F_PCUR] ? Gt : 0};
+  double tot[3];
+  sum_partials<3>(sl, g, tot);
+  const double rr[1] = {tot[0]}, rz[1] = {tot[1]}, rt[1] = {tot[2]};
+  if (threadIdx.x) return;
   const int it = fl[F_IT];
-  const double ratio2 = rr[0] * sc[S_INV0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_RR] = rr[0];
-  if (ratio2 < sc[S_TOL2] || it + 1 >= fl[F_MAXIT]) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+  sc[S_RR] = rr[0];
+  if (!MX) {
+    const double ratio2 = rr[0] * sc[S_INV0];
+    if (ratio2 < sc[S_TOL2] || it + 1 >= fl[F_MAXIT]) {
       sc[S_CONV] = sqrt(ratio2);
-      sc[S_RR] = rr[0];
       if (ratio2 < sc[S_TOL2]) {
         fl[F_ITOUT] = it;
         fl[F_DONE] = 1;
       } else {
         fl[F_DONE] = 2;
       }
+      return;
     }
-    return;
-  }
-  double v[1] = {0.0};
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) v[0] += r[i] * z[i];
-  produce<NT, 1>(slot_rz, v);
-}
-
-// beta = rho_new / rho; p = beta p; p += 1.0 z   (solver_ls_iter.cpp:158-166)
-// Mixed precision (REL): when the judgement has scheduled a reliable update, the FP64 solution also takes the
-// FP32 partial solution here (x += xlo, xlo = 0), and block 0 hands the decision on to the next iteration.
-template <typename T, bool REL>
-__global__ void __launch_bounds__(NT) k_pcg_pupdate(long long m, T* p, const T* z, const double* slot_rz, int G,
-                                                    double* sc, int* fl, float* xlo, double* x64) {
-  if (fl[F_DONE]) return;
-  double rz[1];
-  consume<NT, 1>(slot_rz, G, rz);
-  const int it = fl[F_IT];
-  const double beta = rz[0] / sc[S_RZ + (it & 1)];
-  const T bt = (T)beta;
-  const bool fold = REL && fl[F_PNEXT];
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
-    const T t = p[i] * bt;
-    p[i] = t + z[i];
-    if (REL && fold) {
-      x64[i] += (double)xlo[i];
-      xlo[i] = 0.0f;
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc[S_RZ + ((it + 1) & 1)] = rz[0];
-    if (REL) fl[F_PCUR] = fl[F_PNEXT];
-  }
-}
-
-// ---- mixed precision: FP32 PCG with reliable updates ------------------------------------------------------
-// The Krylov recurrence runs in FP32 (matrix, ILU factor, r, z, p, q, and the partial solution xlo); dot
-// products are accumulated in FP64. When the recursive residual has dropped by delta since the last update,
-// or looks converged, the FP64 solution takes xlo (x += xlo, xlo = 0) and the residual is replaced by the
-// FP64 true residual b - A x computed with the FP64 matrix; convergence is only declared on that true residual.
-
-__device__ inline bool reliable_due(double rr, const double* sc, int it) {
-  return rr * sc[S_INV0] < sc[S_TOL2] || rr < sc[S_DELTA2] * sc[S_RMAX + (it & 1)];
-}
-
-// Judgement. If this iteration's SpMV formed a true residual (a reliable update scheduled one iteration
-// earlier), convergence is decided on it: the FP64 solution then is the one of the previous iteration.
-// Otherwise the recursive residual only schedules the next reliable update (it dropped by delta, or looks
-// converged). Then partials of (r, z).
-__global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r, const float* z, const double* slot_rr,
-                                                    int G, const double* slot_true, int Gt, double* slot_rz, double* sc,
-                                                    int* fl) {
-  if (fl[F_DONE]) return;
-  double rr[1];
-  consume<NT, 1>(slot_rr, G, rr);
-  const int it = fl[F_IT];
-  double rmax = sc[S_RMAX + (it & 1)];
-  if (fl[F_PCUR]) {
-    double rt[1];
-    consume<NT, 1>(slot_true, Gt, rt);
-    const double ratio2 = rt[0] * sc[S_INV0];
-    if (ratio2 < sc[S_TOL2]) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
+  } else {
+    double rmax = sc[S_RMAX + (it & 1)];
+    if (fl[F_PCUR]) {
+      const double ratio2 = rt[0] * sc[S_INV0];
+      if (ratio2 < sc[S_TOL2]) {
         sc[S_CONV] = sqrt(ratio2);
         sc[S_RR] = rt[0];
         fl[F_ITOUT] = it - 1;
         fl[F_DONE] = 1;
+        return;
       }
+      if (stall_update(rt[0], sc[S_TBEST], fl[F_STALL])) {
+        sc[S_CONV] = sqrt(ratio2);
+        sc[S_RR] = rt[0];
+        fl[F_DONE] = 3;
+        return;
+      }
+      rmax = rt[0];
+    }
+    if (it + 1 >= fl[F_MAXIT]) {
+      sc[S_CONV] = sqrt(rr[0] * sc[S_INV0]);
+      fl[F_DONE] = 2;
       return;
     }
-    // block 0 alone keeps the stagnation state; the other blocks stop at the done flag of the next kernel
-    if (blockIdx.x == 0 && threadIdx.x == 0 && stall_update(rt[0], sc[S_TBEST], fl[F_STALL])) {
-      sc[S_CONV] = sqrt(ratio2);
-      sc[S_RR] = rt[0];
-      fl[F_DONE] = 3;
-    }
-    rmax = rt[0];
-  }
-  if (it + 1 >= fl[F_MAXIT]) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc[S_CONV] = sqrt(rr[0] * sc[S_INV0]);
-      sc[S_RR] = rr[0];
-      fl[F_DONE] = 2;
-    }
-    return;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
     const bool due = rr[0] * sc[S_INV0] < sc[S_TOL2] || rr[0] < sc[S_DELTA2] * rmax ||
                      it - (int)sc[S_LASTUPD] >= kForceUpdate;
-    sc[S_RR] = rr[0];
     sc[S_RMAX + ((it + 1) & 1)] = due ? rr[0] : fmax(rmax, rr[0]);
     fl[F_PNEXT] = due ? 1 : 0;
     if (due) {
@@ -314,12 +344,30 @@ __global__ void __launch_bounds__(NT) k_mx_judge_rz(long long m, const float* r,
       sc[S_LASTUPD] = (double)it;
     }
   }
-  double v[1] = {0.0};
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT)
-    v[0] += (double)r[i] * (double)z[i];
-  produce<NT, 1>(slot_rz, v);
+  sc[S_BETA] = rz[0] / sc[S_RZ + (it & 1)];
+  sc[S_RZ + ((it + 1) & 1)] = rz[0];
+  if (MX) fl[F_PCUR] = fl[F_PNEXT];
 }
 
+// p = beta p; p += z; mixed: the FP64 solution takes the partial solution when the next SpMV forms the true
+// residual
+template <typename T, bool MX>
+__global__ void __launch_bounds__(NT) k_pcg_pupdate(long long m, T* p, const T* z, const double* sc, const int* fl,
+                                                     float* xlo, double* x64) {
+  if (fl[F_DONE]) return;
+  const T bt = (T)sc[S_BETA];
+  const bool fold = MX && fl[F_PCUR];
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) {
+    const T t = p[i] * bt;
+    p[i] = t + z[i];
+    if (MX && fold) {
+      x64[i] += (double)xlo[i];
+      xlo[i] = 0.0f;
+    }
+  }
+}
+
+// ---- mixed precision: FP32 PCG with reliable updates ------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_mx_to32(long long m, const double* a, float* b) {
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < m; i += (long long)gridDim.x * NT) b[i] = (float)a[i];
 }
@@ -564,11 +612,61 @@ static int poll(Sys& s, hipStream_t st) {
   return DFM_OK;
 }
 
+// sweep partial slots: one per workgroup of each phase
+static int sweep_blocks(const Sweep& w) {
+  int nb = 0;
+  for (int p = 0; p < w.nphase(); p++) nb += (w.phase_gptr[p + 1] - w.phase_gptr[p] + kWPB - 1) / kWPB;
+  return nb;
+}
+
+template <typename T, typename TV>
+static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const TV* fdval, const TV* bval,
+                               double* part_rr, double* part_rz, hipStream_t st) {
+  int* fl = s.d_iflag.p;
+  const double* sc = s.d_scal.p;
+  T* r = (T*)v.r;
+  T* z = (T*)v.z;
+  const Sweep& f = s.fwd;
+  const SlabArgs fa{f.d_gmeta.p, f.d_srow.p, f.d_scnt.p, f.d_ecol.p};
+  int off = 0;
+  for (int p = 0; p < f.nphase(); p++) {
+    const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0, nb = (ng + kWPB - 1) / kWPB;
+    if (!ng) continue;
+    const bool narrow = sizeof(T) == 8 && sizeof(TV) == 8;
+#define DFM_FU(B)                                                                                                    \
+  if (s.len == B)                                                                                                    \
+    with_slab_caps(f.phase_wo[p], f.phase_wi[p], B == 3 && narrow, [&]<int MO, int MI>() {                           \
+      hipLaunchKernelGGL((k_fwd_update<B, MO, MI, TV, T>), dim3(nb), dim3(64 * kWPB), 0, st, g0, g0 + ng, fa, fval,  \
+                         fdval, r, (const T*)v.q, (const T*)v.p, (T*)v.x, z, sc, fl, part_rr + off);                 \
+    });
+    DFM_FU(1) DFM_FU(2) DFM_FU(3)
+#undef DFM_FU
+    off += nb;
+  }
+  const Sweep& b = s.bwd;
+  const SlabArgs ba{b.d_gmeta.p, b.d_srow.p, b.d_scnt.p, b.d_ecol.p};
+  off = 0;
+  for (int p = 0; p < b.nphase(); p++) {
+    const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0, nb = (ng + kWPB - 1) / kWPB;
+    if (!ng) continue;
+    const bool narrow = sizeof(T) == 8 && sizeof(TV) == 8;
+#define DFM_BZ(B)                                                                                                  \
+  if (s.len == B)                                                                                                  \
+    with_slab_caps(b.phase_wo[p], b.phase_wi[p], B == 3 && narrow, [&]<int MO, int MI>() {                         \
+      hipLaunchKernelGGL((k_bwd_rz<B, MO, MI, TV, T>), dim3(nb), dim3(64 * kWPB), 0, st, g0, g0 + ng, ba, bval,     \
+                         (const T*)r, z, fl, part_rz + off);                                                       \
+    });
+    DFM_BZ(1) DFM_BZ(2) DFM_BZ(3)
+#undef DFM_BZ
+    off += nb;
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
   const long long m = (long long)s.n * s.len;
   const dim3 vg = vgrid(m);
-  const int G = (int)vg.x;
   unsigned Gs = 0;
   int* fl = s.d_iflag.p;
   double* sc = s.d_scal.p;
@@ -576,36 +674,57 @@ static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
   T* z = (T*)v.z;
   T* p = (T*)v.p;
   T* q = (T*)v.q;
+  const int nbf = sweep_blocks(s.fwd), nbb = sweep_blocks(s.bwd);
+  double* part_rr = s.d_part_sw.p;
+  double* part_rz = s.d_part_sw.p + nbf;
+  hipError_t e;
+  if (mixed)
+    e = pcg_spmv<T, true>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, v.x64, v.b64, r, slot(s, P_C), st);
+  else
+    e = pcg_spmv<T, false>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, nullptr, nullptr, nullptr,
+                           nullptr, st);
+  if (e != hipSuccess) return DFM_ERR_HIP;
+  hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedNT), 0, st, (const double*)slot(s, P_A), (int)Gs, sc, (const int*)fl);
+  if (mixed)
+    e = fused_sweeps<float, float>(s, v, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, part_rr, part_rz, st);
+  else if (v.f32ilu)
+    e = fused_sweeps<double, float>(s, v, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, part_rr, part_rz, st);
+  else
+    e = fused_sweeps<double, double>(s, v, s.fwd.d_eval.p, s.fwd.d_dval.p, s.bwd.d_eval.p, part_rr, part_rz, st);
+  if (e != hipSuccess) return DFM_ERR_HIP;
   if (mixed) {
-    if (pcg_spmv<T, true>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, v.x64, v.b64, r, slot(s, P_C),
-                          st) != hipSuccess)
-      return DFM_ERR_HIP;
-    hipLaunchKernelGGL((k_pcg_update<float, float>), vg, dim3(NT), 0, st, m, (float*)r, (float*)v.x, (const float*)q,
-                       (const float*)p, slot(s, P_A), (int)Gs, slot(s, P_B), sc, fl);
-    if (precond_apply32(s, (const float*)r, (float*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
-    hipLaunchKernelGGL(k_mx_judge_rz, vg, dim3(NT), 0, st, m, (const float*)r, (const float*)z, slot(s, P_B), G,
-                       slot(s, P_C), (int)Gs, slot(s, P_D), sc, fl);
-    hipLaunchKernelGGL((k_pcg_pupdate<T, true>), vg, dim3(NT), 0, st, m, p, (const T*)z, slot(s, P_D), G, sc, fl,
-                       (float*)v.x, v.x64);
+    hipLaunchKernelGGL(k_pcg_judge_beta<true>, dim3(1), dim3(kRedNT), 0, st, (const double*)part_rr, nbf,
+                       (const double*)slot(s, P_C), (int)Gs, (const double*)part_rz, nbb, sc, fl);
+    hipLaunchKernelGGL((k_pcg_pupdate<T, true>), vg, dim3(NT), 0, st, m, p, (const T*)z, (const double*)sc,
+                       (const int*)fl, (float*)v.x, v.x64);
   } else {
-    if (pcg_spmv<T, false>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, nullptr, nullptr, nullptr,
-                           nullptr, st) != hipSuccess)
-      return DFM_ERR_HIP;
-    hipLaunchKernelGGL((k_pcg_update<double, double>), vg, dim3(NT), 0, st, m, (double*)r, (double*)v.x,
-                       (const double*)q, (const double*)p, slot(s, P_A), (int)Gs, slot(s, P_B), sc, fl);
-    if (precond_apply_d(s, v, (const double*)r, (double*)z, fl, st) != hipSuccess) return DFM_ERR_HIP;
-    hipLaunchKernelGGL(k_pcg_judge_rz, vg, dim3(NT), 0, st, m, (const double*)r, (const double*)z, slot(s, P_B), G,
-                       slot(s, P_D), sc, fl);
-    hipLaunchKernelGGL((k_pcg_pupdate<T, false>), vg, dim3(NT), 0, st, m, p, (const T*)z, slot(s, P_D), G, sc, fl,
-                       (float*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL(k_pcg_judge_beta<false>, dim3(1), dim3(kRedNT), 0, st, (const double*)part_rr, nbf,
+                       (const double*)nullptr, 0, (const double*)part_rz, nbb, sc, fl);
+    hipLaunchKernelGGL((k_pcg_pupdate<T, false>), vg, dim3(NT), 0, st, m, p, (const T*)z, (const double*)sc,
+                       (const int*)fl, (float*)nullptr, (double*)nullptr);
   }
-  return DFM_OK;
+  return hipGetLastError() == hipSuccess ? DFM_OK : DFM_ERR_HIP;
+}
+
+// Every device pointer an iteration's launches take: a captured graph is valid while these are unchanged
+// (re-ordering, re-factoring or growing a work buffer may move them).
+static std::vector<const void*> graph_key(const Sys& s, const PcgBufs& v) {
+  std::vector<const void*> k{v.crs, v.dia, v.r, v.z, v.p, v.q, v.x, v.x64, v.b64, s.d_crs.p, s.d_dia.p,
+                             s.d_rowptr.p, s.d_colind.p, s.d_partial.p, s.d_part_sw.p, s.d_scal.p, s.d_iflag.p,
+                             s.sell.d_sptr.p, s.sell.d_col.p, s.sell.d_val.p, s.sell.d_val32.p};
+  for (const Sweep* w : {&s.fwd, &s.bwd})
+    for (const void* p : {(const void*)w->d_gmeta.p, (const void*)w->d_srow.p, (const void*)w->d_scnt.p,
+                          (const void*)w->d_ecol.p, (const void*)w->d_eval.p, (const void*)w->d_dval.p,
+                          (const void*)w->d_eval32.p, (const void*)w->d_dval32.p})
+      k.push_back(p);
+  k.push_back((const void*)(size_t)v.f32ilu);
+  return k;
 }
 
 // A chunk of `k` iterations captured once as a hipGraph and replayed (the kernels take every scalar from
-// device memory, so a replay is exactly the same launches). Rebuilt when the work buffers change.
+// device memory, so a replay is exactly the same launches). Captured again when a pointer it holds changes.
 template <typename F>
-static int graph_chunk(Sys& s, int kind, const void* key, F enqueue_iteration, hipStream_t st) {
+static int graph_chunk(Sys& s, int kind, std::vector<const void*> key, F enqueue_iteration, hipStream_t st) {
   if (s.gexec[kind] && s.gkey[kind] == key) return DFM_OK;
   if (s.gexec[kind]) {
     (void)hipGraphExecDestroy(s.gexec[kind]);
@@ -621,7 +740,7 @@ static int graph_chunk(Sys& s, int kind, const void* key, F enqueue_iteration, h
   e = hipGraphInstantiate(&s.gexec[kind], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   DFM_HIP(e);
-  s.gkey[kind] = key;
+  s.gkey[kind] = std::move(key);
   return DFM_OK;
 }
 
@@ -706,13 +825,17 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
   int* fl = s.d_iflag.p;
   double* sc = s.d_scal.p;
   {
+    const size_t need = (size_t)(sweep_blocks(s.fwd) + sweep_blocks(s.bwd));
+    if (s.d_part_sw.n < need) DFM_HIP(s.d_part_sw.alloc(need));
+  }
+  {
     const int rc = pcg_start(s, v, mixed, tol, maxit, sc, fl, st);
     if (rc != DFM_OK) return rc;
   }
   const int gk = mixed ? 1 : (v.f32ilu ? 3 : 0);
   const bool use_graph = getenv("DFM_NO_GRAPH") == nullptr;
   if (use_graph) {
-    const int rc = graph_chunk(s, gk, v.r, [&]() { return pcg_iteration<T>(s, v, mixed, st); }, st);
+    const int rc = graph_chunk(s, gk, graph_key(s, v), [&]() { return pcg_iteration<T>(s, v, mixed, st); }, st);
     if (rc != DFM_OK) return rc;
   }
   unsigned launched = 0, chunk = 32, since = 0;

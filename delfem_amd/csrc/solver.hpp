@@ -14,6 +14,8 @@ enum {
   S_RMAX = 5,   // reliable updates: max ||r||^2 since the last update, two parity slots 5/6
   S_DELTA2 = 7, // reliable updates: trigger ratio^2
   S_RR = 8,     // last ||r||^2 seen by the convergence test
+  S_ALPHA = 9,  // alpha of the current iteration
+  S_BETA = 10,  // beta of the current iteration
   S_TBEST = 11, // mixed precision: smallest true ||r||^2 that counted as progress
   S_LASTUPD = 12, // mixed precision: iteration of the last reliable update
   S_TMP = 15
