@@ -643,20 +643,89 @@ void cr_ilu0_pattern(int n, const unsigned* rowptr, const unsigned* colind, cons
   }
 }
 
+/* Row i of the new pattern is a sorted singly linked list over column numbers (nxt[], -1 ends it, head hd);  */
+/* lv[] holds the entry levels. Rows before i are final in frowptr/fcolind/flev. Visiting the row's entries in */
+/* ascending order while inserting behind the visited one is the traversal of AddFracPtn (:1330-1361): the   */
+/* first insertion of a column fixes its level, max(lev_ik, lev_kj) + 1 (:1354-1356); entries at or above    */
+/* lev_fill neither pivot nor spread.                                                                           */
+long cr_iluk_pattern(int n, const unsigned* rowptr, const unsigned* colind, int lev_fill, long cap, unsigned* frowptr,
+                     unsigned* fcolind, unsigned* diaind, int* flev) {
+  int* nxt = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int* lv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int* tmplev = flev ? NULL : (int*)malloc(sizeof(int) * (size_t)(cap > 0 ? cap : 1));
+  int* L = flev ? flev : tmplev;
+  long pos = 0, need = 0;
+  int overflow = 0;
+  frowptr[0] = 0;
+  for (int i = 0; i < n; i++) {
+    int hd = -1, tail = -1;
+    for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+      const int c = (int)colind[q];
+      lv[c] = 0;
+      nxt[c] = -1;
+      if (tail < 0) hd = c; else nxt[tail] = c;
+      tail = c;
+    }
+    if (lev_fill != 0 && !overflow) {
+      for (int k = hd; k >= 0; k = nxt[k]) {
+        const int lik = lv[k];
+        if (lev_fill != -1 && lik + 1 > lev_fill) continue;
+        if (k >= i) break;
+        int at = k; /* insertion point: every new column of this pivot lies behind it, in ascending order */
+        for (unsigned kj = diaind[k]; kj < frowptr[k + 1]; kj++) {
+          const int lkj = L[kj];
+          if (lev_fill != -1 && lkj + 1 > lev_fill) continue;
+          const int j = (int)fcolind[kj];
+          if (j == i) continue;
+          while (nxt[at] >= 0 && nxt[at] < j) at = nxt[at];
+          if (nxt[at] == j) continue;
+          lv[j] = (lik > lkj ? lik : lkj) + 1;
+          nxt[j] = nxt[at];
+          nxt[at] = j;
+          at = j;
+        }
+      }
+    }
+    long cnt = 0;
+    for (int c = hd; c >= 0; c = nxt[c]) cnt++;
+    need += cnt;
+    if (overflow || pos + cnt > cap) {
+      overflow = 1;
+      continue;
+    }
+    const unsigned end = (unsigned)(pos + cnt);
+    diaind[i] = end;
+    for (int c = hd; c >= 0; c = nxt[c]) {
+      if ((unsigned)c > (unsigned)i && diaind[i] == end) diaind[i] = (unsigned)pos;
+      fcolind[pos] = (unsigned)c;
+      L[pos] = lv[c];
+      pos++;
+    }
+    frowptr[i + 1] = (unsigned)pos;
+  }
+  free(nxt);
+  free(lv);
+  free(tmplev);
+  return overflow ? -need : pos;
+}
+
+/* src/matvec/matdiafrac_blkcrs.h:90-117 (SetValue_Initialize) -> CMat_BlkCrs::SetValue: the factor pattern  */
+/* (possibly with fill) takes A's values where A has an entry, zero elsewhere; under an ordering new row i0   */
+/* takes old row order[i0].                                                                                    */
 void cr_ilu0_setvalue(int n, int len, const unsigned* rowptr, const unsigned* colind, const double* crs,
                       const double* dia, const int* order, const unsigned* frowptr, const unsigned* fcolind,
                       double* fcrs, double* fdia) {
   const int bs = len * len;
   memset(fcrs, 0, sizeof(double) * (size_t)frowptr[n] * bs);
-  if (!order) {
+  if (!order && frowptr[n] == rowptr[n]) {
     memcpy(fcrs, crs, sizeof(double) * (size_t)rowptr[n] * bs);
     memcpy(fdia, dia, sizeof(double) * (size_t)n * bs);
     return;
   }
   int* inv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-  for (int i = 0; i < n; i++) inv[order[i]] = i;
+  for (int i = 0; i < n; i++) inv[order ? order[i] : i] = i;
   for (int i0 = 0; i0 < n; i0++) {
-    const int i1 = order[i0];
+    const int i1 = order ? order[i0] : i0;
     for (unsigned q1 = rowptr[i1]; q1 < rowptr[i1 + 1]; q1++) {
       const unsigned j0 = (unsigned)inv[colind[q1]];
       for (unsigned q0 = frowptr[i0]; q0 < frowptr[i0 + 1]; q0++)

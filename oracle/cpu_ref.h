@@ -52,6 +52,11 @@ void cr_matvec(int n, int len, const unsigned* rowptr, const unsigned* colind, c
 void cr_ilu0_pattern(int n, const unsigned* rowptr, const unsigned* colind, const int* order,
                      unsigned* frowptr, unsigned* fcolind, unsigned* diaind);
 /* SetValue_Initialize: copy A into the factor pattern (permuted when order != NULL) */
+/* Level-of-fill pattern (ILU(k) symbolic) of an ordered pattern: src/matvec/matdiafrac_blkcrs.cpp:1254-1381
+   (AddFracPtn) with CListRLN (include/delfem/matvec/matdiafrac_blkcrs.h:185-301). lev_fill -1: no limit.
+   Returns the entry count, or -(count needed) when cap is too small. flev may be NULL. */
+long cr_iluk_pattern(int n, const unsigned* rowptr, const unsigned* colind, int lev_fill, long cap, unsigned* frowptr,
+                     unsigned* fcolind, unsigned* diaind, int* flev);
 void cr_ilu0_setvalue(int n, int len, const unsigned* rowptr, const unsigned* colind, const double* crs,
                       const double* dia, const int* order, const unsigned* frowptr, const unsigned* fcolind,
                       double* fcrs, double* fdia);

@@ -34,6 +34,9 @@ def cpu_ref():
         _cr = C.CDLL(CPU_REF_SO)
         _cr.cr_pattern_count.restype = C.c_long
         _cr.cr_set_bc.restype = C.c_double
+        _cr.cr_iluk_pattern.restype = C.c_long
+        _cr.cr_iluk_pattern.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_long, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]
     return _cr
 
 
@@ -101,15 +104,31 @@ class CpuSystem:
         return y
 
     def ilu0(self, order=None):
+        return self.ilu(order, 0)
+
+    def ilu(self, order=None, lev=0):
+        """ILU(lev) (lev -1: no fill limit) of the matrix under `order` (new -> old): the level-of-fill pattern
+        of the ordered pattern, A's values scattered into it, DoILUDecomp. Entry levels in f["flev"]."""
         n, bs = self.nnode, self.len * self.len
         ncrs = int(self.rowptr[-1])
-        f = {"frowptr": np.zeros(n + 1, np.uint32), "fcolind": np.zeros(max(ncrs, 1), np.uint32)[:ncrs],
-             "diaind": np.zeros(n, np.uint32), "fcrs": np.zeros(ncrs * bs), "fdia": np.zeros(n * bs)}
         order = None if order is None else _i32(order)
         p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
         lib = cpu_ref()
-        lib.cr_ilu0_pattern(n, p(self.rowptr), p(self.colind), p(order), p(f["frowptr"]), p(f["fcolind"]),
-                            p(f["diaind"]))
+        rp0 = np.zeros(n + 1, np.uint32)
+        ci0 = np.zeros(max(ncrs, 1), np.uint32)
+        di0 = np.zeros(n, np.uint32)
+        lib.cr_ilu0_pattern(n, p(self.rowptr), p(self.colind), p(order), p(rp0), p(ci0), p(di0))
+        cap = max(ncrs, 1)
+        while True:
+            f = {"frowptr": np.zeros(n + 1, np.uint32), "fcolind": np.zeros(cap, np.uint32),
+                 "diaind": np.zeros(n, np.uint32), "flev": np.zeros(cap, np.int32)}
+            got = lib.cr_iluk_pattern(n, p(rp0), p(ci0), lev, cap, p(f["frowptr"]), p(f["fcolind"]),
+                                      p(f["diaind"]), p(f["flev"]))
+            if got >= 0:
+                break
+            cap = max(2 * cap, -got)
+        f["fcolind"], f["flev"] = f["fcolind"][:got], f["flev"][:got]
+        f["fcrs"], f["fdia"] = np.zeros(max(got, 1) * bs)[: got * bs], np.zeros(n * bs)
         lib.cr_ilu0_setvalue(n, self.len, p(self.rowptr), p(self.colind), p(self.crs), p(self.dia), p(order),
                              p(f["frowptr"]), p(f["fcolind"]), p(f["fcrs"]), p(f["fdia"]))
         ok = lib.cr_ilu0_factor(n, self.len, p(f["frowptr"]), p(f["fcolind"]), p(f["diaind"]), p(f["fcrs"]),
@@ -171,6 +190,8 @@ def reference():
                      "ref_sys_set_vector", "ref_sys_matvec", "ref_ilu0_factor", "ref_ilu_apply", "ref_sys_solve",
                      "ref_sys_update_field"]:
             getattr(_ref, name).argtypes = None
+        _ref.ref_ilu_factor.restype = C.c_long
+        _ref.ref_ilu_factor.argtypes = [C.c_void_p, C.c_int, C.c_long] + [C.c_void_p] * 5
         _ref.ref_sys_destroy.argtypes = [C.c_void_p]
         _ref.ref_sys_ncrs.argtypes = [C.c_void_p]
     return _ref
@@ -265,6 +286,22 @@ class RefSystem:
         p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
         ok = reference().ref_ilu0_factor(self.h, p(order), p(rp), p(ci), p(crs), p(dia))
         return bool(ok), rp, ci[: self.ncrs], crs[: self.ncrs * bs], dia
+
+    def ilu(self, lev):
+        """The reference's ILU(lev) factor in the natural order: (ok, rowptr, colind, crs, dia)."""
+        bs = self.len * self.len
+        cap = max(self.ncrs, 1)
+        while True:
+            rp = np.zeros(self.nnode + 1, np.uint32)
+            ci = np.zeros(cap, np.uint32)
+            crs = np.zeros(cap * bs)
+            dia = np.zeros(self.nnode * bs)
+            ok = C.c_int(0)
+            p = lambda a: a.ctypes.data_as(C.c_void_p)
+            got = reference().ref_ilu_factor(self.h, lev, cap, p(rp), p(ci), p(crs), p(dia), C.byref(ok))
+            if got >= 0:
+                return bool(ok.value), rp, ci[:got], crs[: got * bs], dia
+            cap = -got
 
     def ilu_apply(self, r, lev=0, order=None):
         r = _f64(r)

@@ -281,6 +281,35 @@ int ref_ilu0_factor(void* h, const int* order, unsigned int* rowptr, unsigned in
   return ok ? 1 : 0;
 }
 
+// ILU(lev) numeric factor of the current matrix in the natural order: CMatDiaFrac_BlkCrs(lev, A)
+// (src/matvec/matdiafrac_blkcrs.cpp:227-287: MakePattern_Initialize, AddFracPtn, MakePatternFinalize), then
+// SetValue_Initialize and DoILUDecomp. Writes the factor pattern and values when they fit in cap entries and
+// returns the entry count, or -(entry count) when cap is too small; *ok = DoILUDecomp's result.
+long ref_ilu_factor(void* h, int lev, long cap, unsigned int* rowptr, unsigned int* colind, double* crs, double* dia,
+                    int* ok) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CMatDia_BlkCrs& m = Mat(s);
+  MatVec::CMatDiaFrac_BlkCrs f(lev, m);
+  f.SetValue_Initialize(m);
+  *ok = f.DoILUDecomp() ? 1 : 0;
+  const long ncrs = (long)f.NCrs();
+  if (ncrs > cap) return -ncrs;
+  const int bs = s->len * s->len;
+  size_t pos = 0;
+  rowptr[0] = 0;
+  for (int i = 0; i < s->nnode; i++) {
+    unsigned int np = 0;
+    const unsigned int* cols = f.GetPtrIndPSuP(i, np);
+    const double* v = f.GetPtrValPSuP(i, np);
+    for (unsigned int k = 0; k < np; k++) colind[pos + k] = cols[k];
+    if (np) std::memcpy(&crs[pos * bs], v, sizeof(double) * np * bs);
+    pos += np;
+    rowptr[i + 1] = (unsigned int)pos;
+    std::memcpy(&dia[(size_t)i * bs], f.GetPtrValDia(i), sizeof(double) * bs);
+  }
+  return ncrs;
+}
+
 // z = M^-1 r with the reference's CPreconditioner_ILU (ILU(lev) or ordered ILU(0)).
 int ref_ilu_apply(void* h, int lev, const int* order, const double* r, double* z) {
   RefSys* s = (RefSys*)h;

@@ -144,6 +144,88 @@ def test_ilu0_bit_exact(ndim, length, nxyz, ordered):
 
 
 @pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
+@pytest.mark.parametrize("lev", [1, 2, 3, -1])
+def test_iluk_bit_exact(ndim, length, nxyz, lev):
+    """ILU(lev) (level-of-fill pattern, A scattered into it, DoILUDecomp) against CMatDiaFrac_BlkCrs(lev, A),
+    and the apply against CPreconditioner_ILU with SetFillInLevel(lev)."""
+    ref, cpu = _assembled(ndim, length, nxyz)
+    ok, rp, ci, crs, dia = ref.ilu(lev)
+    f = cpu.ilu(None, lev)
+    assert ok == f["ok"]
+    assert f["frowptr"][-1] > cpu.rowptr[-1]  # there is fill
+    assert np.array_equal(rp, f["frowptr"]) and np.array_equal(ci, f["fcolind"])
+    assert np.array_equal(crs, f["fcrs"]) and np.array_equal(dia, f["fdia"])
+    r = np.random.default_rng(9).normal(size=cpu.nnode * length)
+    assert np.array_equal(ref.ilu_apply(r, lev), cpu.ilu_apply(f, r))
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
+def test_iluk_levels(ndim, length, nxyz):
+    """Unlimited fill reaches a fixed point (ILU(n) = ILU(-1)) that contains every ILU(k) pattern; ILU(1)'s
+    level-0 entries are exactly the matrix pattern and its fill has level 1; ILU(-1) is a complete LU: its apply
+    solves A z = r. (ILU(k) need not lie within ILU(k+1): the first insertion of an entry fixes its level, so a
+    higher limit can give an entry a level that no longer lets it pivot.)"""
+    _, cpu = _assembled(ndim, length, nxyz)
+    fs = [cpu.ilu(None, k) for k in range(4)] + [cpu.ilu(None, cpu.nnode), cpu.ilu(None, -1)]
+    full = fs[-1]
+    for a in fs[:-1]:
+        for i in range(cpu.nnode):
+            ra = set(a["fcolind"][a["frowptr"][i]:a["frowptr"][i + 1]].tolist())
+            rb = set(full["fcolind"][full["frowptr"][i]:full["frowptr"][i + 1]].tolist())
+            assert ra <= rb
+    assert np.array_equal(fs[-2]["frowptr"], full["frowptr"]) and np.array_equal(fs[-2]["fcolind"], full["fcolind"])
+    f1 = fs[1]
+    for i in range(cpu.nnode):
+        cols = f1["fcolind"][f1["frowptr"][i]:f1["frowptr"][i + 1]]
+        lev = f1["flev"][f1["frowptr"][i]:f1["frowptr"][i + 1]]
+        acols = cpu.colind[cpu.rowptr[i]:cpu.rowptr[i + 1]]
+        assert set(cols[lev == 0].tolist()) == set(acols.tolist())
+        assert np.all(lev <= 1)
+    lu = fs[-1]
+    r = np.random.default_rng(3).normal(size=cpu.nnode * length)
+    z = cpu.ilu_apply(lu, r)
+    assert np.linalg.norm(cpu.matvec(1.0, z, 0.0, np.zeros_like(r)) - r) <= 1e-9 * np.linalg.norm(r)
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES[:3])
+@pytest.mark.parametrize("lev", [1, 2])
+def test_iluk_ordered_is_relabelled_reference(ndim, length, nxyz, lev):
+    """ILU(lev) under an ordering equals the reference's natural-order ILU(lev) of the same mesh with its nodes
+    renumbered by that ordering (the reference applies orderings to ILU(0) only)."""
+    co, ln = mesh2d(*nxyz) if ndim == 2 else mesh3d(*nxyz)
+    n = co.size // ndim
+    order = np.random.default_rng(5).permutation(n).astype(np.int32)
+    inv = np.empty(n, np.int32)
+    inv[order] = np.arange(n, dtype=np.int32)
+    co2 = co.reshape(n, ndim)[order].ravel()
+    ln2 = inv[ln]
+    _, cpu = both(ndim, co, ln, length)
+    ref2, _ = both(ndim, co2, ln2, length)
+    kind = {1: P.POISSON, 2: P.SOLID2D, 3: P.SOLID3D}[length]
+    par = {1: [1.3, 1.0], 2: [LAM2, MU2, 1.0, 0.0, -0.01], 3: [LAM3, MU3, 1.0, 0.0, 0.0, -0.01]}[length]
+    cpu.assemble(kind, par)
+    cpu.set_bc(clamp_flags(co, ndim, length))
+    ref2.set_bc(clamp_flags(co2, ndim, length))
+    ref2.assemble(kind, np.array(par)[None, :])
+    f = cpu.ilu(order, lev)
+    ok, rp, ci, crs, dia = ref2.ilu(lev)
+    assert ok and f["ok"]
+    assert np.array_equal(rp, f["frowptr"]) and np.array_equal(ci, f["fcolind"])
+    assert np.array_equal(crs, f["fcrs"]) and np.array_equal(dia, f["fdia"])
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
+@pytest.mark.parametrize("lev", [1, 2])
+def test_iluk_pcg_bit_exact(ndim, length, nxyz, lev):
+    ref, cpu = _assembled(ndim, length, nxyz)
+    f = cpu.ilu(None, lev)
+    ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-10, 1000)
+    rok, rconv, rit = ref.solve("pcg", 1e-10, 1000, lev, None)
+    assert (ok, conv, it) == (bool(rok), rconv, rit)
+    assert np.array_equal(ref.vector(-2), x)
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
 @pytest.mark.parametrize("method", ["pcg", "pbicgstab"])
 @pytest.mark.parametrize("ordered", [False, True], ids=["natural", "ordered"])
 def test_krylov_bit_exact(ndim, length, nxyz, method, ordered):
