@@ -145,6 +145,43 @@ def run_c5_batch(args, N, W):
     print(json.dumps(out), flush=True)
 
 
+def run_c4_plain(N, W, ctx, n):
+    """The plain FP64 GPU path on requests 0..n-1 of the stream, one at a time as a single-system user runs it:
+    pattern, block-multicolour ordering, Dirichlet flags, assembly, ILU(0) and FP64 PCG to a 1e-8 relative
+    residual. Request generation (the mesh) is outside the timed calls."""
+    import numpy as np
+
+    if n <= 0:
+        return None
+    t_all = t_solve = 0.0
+    dofs = 0
+    iters = []
+    for i in range(n):
+        r = W.c4_request(i)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        s = N.System(ctx, r["ndim"], r["coords"], r["lnods"], r["len"])
+        order, _ = s.make_ordering(N.ORDER_BLOCKCOLOR)
+        s.set_ordering(order)
+        s.set_bc(r["bc"])
+        s.assemble(r["kind"], r["params"])
+        s.factor()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ok, conv, it = s.solve_pcg(1e-8, 20000, N.FP64)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        if not ok:
+            raise RuntimeError(f"plain FP64 request {i} did not converge ({conv}, {it})")
+        s.close()
+        t_all += t2 - t0
+        t_solve += t2 - t1
+        dofs += r["ndof"]
+        iters.append(it)
+    return {"requests": n, "dofs_per_s": dofs / t_all, "solve_dofs_per_s": dofs / t_solve, "wall_s": t_all,
+            "solve_s": t_solve, "total_dofs": float(dofs), "mean_pcg_iters": float(np.mean(iters))}
+
+
 def run_c4_serving(args, N, W):
     """Serving stream: requests (2D Poisson or 2D elasticity, 50/50, dofs log-uniform in [2k, 500k]) arriving as
     a seeded Poisson process at --rate requests/s; continuous batching under a 64 GB admission budget, mixed
@@ -156,7 +193,8 @@ def run_c4_serving(args, N, W):
     import numpy as np
 
     n = args.requests
-    arrivals = W.c4_arrivals(n, args.rate)
+    # rate 0: every request arrives at once (the server's capacity)
+    arrivals = W.c4_arrivals(n, args.rate) if args.rate > 0 else np.zeros(n)
     q = queue.Queue(maxsize=256)
 
     def gen():
@@ -203,8 +241,9 @@ def run_c4_serving(args, N, W):
             time.sleep(max(0.0, min(0.001, arrivals[nxt] - (time.perf_counter() - t0))))
     wall = last_done - t0 - arrivals[0]
     srv.close()
-    ctx.close()
     lat_s = sorted(lat)
+    plain = run_c4_plain(N, W, ctx, n if args.plain < 0 else min(args.plain, n))
+    ctx.close()
 
     def pct(v, qq):
         return float(np.percentile(v, qq))
@@ -225,14 +264,23 @@ def run_c4_serving(args, N, W):
         "p99_ms": pct(lat_s, 99) * 1e3,
         "p50_solve_ms": pct(solve, 50) * 1e3,
         "wall_s": wall,
-        "offered_dofs_per_s": total / (arrivals[-1] - arrivals[0]) if n > 1 else None,
+        "offered_dofs_per_s": total / (arrivals[-1] - arrivals[0]) if n > 1 and args.rate > 0 else None,
         "total_dofs": total,
         "mean_pcg_iters": float(np.mean(iters)),
         "fell_back_to_fp64": int(np.sum(fb)),
         "max_active": max_active, "max_queued": max_queue,
     }
+    if plain:
+        # the same first requests, served: their share of the served run is not separable, so the comparison is
+        # of whole-run rates; with every request solved both ways the rates cover the same work
+        out["plain_fp64"] = plain
+        out["served_vs_plain_fp64"] = (total / wall) / plain["dofs_per_s"]
     log(f"[bench] c4: {total / wall / 1e6:.2f} M DOF-solves/s over {wall:.1f}s, p50 {out['p50_ms']:.0f} ms, "
         f"p99 {out['p99_ms']:.0f} ms, max active {max_active}, max queued {max_queue}, fell back {int(np.sum(fb))}")
+    if plain:
+        log(f"[bench] c4 plain FP64, one request at a time ({plain['requests']} requests): "
+            f"{plain['dofs_per_s'] / 1e6:.2f} M DOF-solves/s end to end, {plain['solve_dofs_per_s'] / 1e6:.2f} M in "
+            f"the solves alone; served / plain = {out['served_vs_plain_fp64']:.2f}")
     print(json.dumps(out), flush=True)
 
 
@@ -251,6 +299,9 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=100.0, help="c4: arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=64, help="c4: systems solved at once")
+    ap.add_argument("--plain", type=int, default=-1,
+                    help="c4: requests (the first ones of the stream) also solved by the plain FP64 path one at a "
+                         "time; -1: all, 0: none")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every GPU box of this build has one; --gpus {args.gpus} is not "

@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
       fl[F_DONE] = 1;
       return;
     }
-    if (stall_update(rt, sc[S_TBEST], fl[F_STALL])) {
+    if (mx_slow(rt, sc[S_RRTRIG], sc[S_TPREV], sc[S_SLOW], sc[S_TBEST], fl[F_STALL])) {
       sc[S_CONV] = sqrt(ratio2);
       sc[S_RR] = rt;
       fl[F_DONE] = 3;
@@ -213,6 +213,7 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
   if (due) {
     fl[F_NREL] = fl[F_NREL] + 1;
     sc[S_LASTUPD] = (double)it;
+    sc[S_RRTRIG] = rr;
   }
 }
 
@@ -316,6 +317,7 @@ struct BatchSeg {
   int state = 0;  // 0 running, 1 converged, 2 stopped at max_iter
   double conv = 0;
   unsigned iters = 0;
+  unsigned before = 0;  // iterations of the mixed-precision loop before a switch to FP64 vectors
   int nrel = 0;
   double t_add = 0, t_done = 0;  // host clock (s)
 };
@@ -324,7 +326,7 @@ struct Batch {
   Ctx* ctx = nullptr;
   int prec = DFM_MIXED;  // mode of added systems
   int cap = 0;
-  DBuf<double> d_sc;  // 16 per slot
+  DBuf<double> d_sc;  // kNScal per slot
   DBuf<int> d_fl;     // 8 per slot
   int* h_fl = nullptr;
   double* h_sc = nullptr;
@@ -406,7 +408,7 @@ static int rebuild(Batch& bt) {
       d.x = sg.v.x;
       d.x64 = sg.v.x64;
       d.b64 = sg.v.b64;
-      d.sc = bt.d_sc.p + 16 * (size_t)sg.slot;
+      d.sc = bt.d_sc.p + kNScal * (size_t)sg.slot;
       d.fl = bt.d_fl.p + 8 * (size_t)sg.slot;
       d.ps0 = (int)sblk.size();
       for (int r = 0; r < s.n; r += rs) sblk.push_back(si);
@@ -508,31 +510,35 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
 
 // A mixed-precision segment whose true residual stalled restarts as an FP64 solve of the same system with the
 // FP32 factor (its residual handle still holds the right-hand side).
+// The FP32 matrix limits progress: the system continues from its FP64 solution with FP64 vectors and matrix and
+// the FP32 factor (as solve_pcg does); max_iter bounds both loops together.
 static int restart_fp64(Batch& bt, BatchSeg& sg) {
   Sys& s = *sg.s;
   sg.mode = DFM_FP32_ILU;
   sg.fell_back = true;
   sg.nrel = bt.h_fl[8 * sg.slot + F_NREL];
+  sg.before = (unsigned)bt.h_fl[8 * sg.slot + F_IT] + 1;
   int rc = sys_solve_setup(s, DFM_FP32_ILU);
   if (rc != DFM_OK) return rc;
   if (s.ctx != bt.ctx) DFM_HIP(hipStreamSynchronize(s.ctx->stream));
   s.prepared = -1;
   sg.v = pcg_bufs(s, false);
   sg.v.f32ilu = true;
-  rc = pcg_start(s, sg.v, false, sg.tol, sg.maxit, bt.d_sc.p + 16 * (size_t)sg.slot, bt.d_fl.p + 8 * (size_t)sg.slot,
-                 bt.ctx->stream);
+  const unsigned left = sg.maxit > sg.before ? sg.maxit - sg.before : 0;
+  rc = pcg_start_warm(s, sg.v, sg.tol, left, bt.d_sc.p + kNScal * (size_t)sg.slot, bt.d_fl.p + 8 * (size_t)sg.slot,
+                      bt.ctx->stream);
   bt.dirty = true;
   return rc;
 }
 
 static int finish_seg(Batch& bt, BatchSeg& sg) {
   const int* fl = bt.h_fl + 8 * sg.slot;
-  const double* sc = bt.h_sc + 16 * sg.slot;
+  const double* sc = bt.h_sc + kNScal * sg.slot;
   const int done = fl[F_DONE];
   if (done == 3) return restart_fp64(bt, sg);
   sg.state = done == 1 ? 1 : 2;
   sg.conv = done ? sc[S_CONV] : std::sqrt(sc[S_RR] * sc[S_INV0]);
-  sg.iters = done == 1 ? (unsigned)fl[F_ITOUT] : sg.maxit;
+  sg.iters = done == 1 ? (unsigned)fl[F_ITOUT] + sg.before : sg.maxit;
   if (!sg.fell_back) sg.nrel = fl[F_NREL];
   sg.t_done = now_s();
   const int rc = pcg_finish(*sg.s, sg.v, sg.mode == DFM_MIXED, done, bt.ctx->stream);
@@ -543,7 +549,7 @@ static int finish_seg(Batch& bt, BatchSeg& sg) {
 static int poll(Batch& bt) {
   hipStream_t st = bt.ctx->stream;
   DFM_HIP(hipMemcpyAsync(bt.h_fl, bt.d_fl.p, sizeof(int) * 8 * bt.cap, hipMemcpyDeviceToHost, st));
-  DFM_HIP(hipMemcpyAsync(bt.h_sc, bt.d_sc.p, sizeof(double) * 16 * bt.cap, hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipMemcpyAsync(bt.h_sc, bt.d_sc.p, sizeof(double) * kNScal * bt.cap, hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
   bool any = false;
   for (auto& sg : bt.segs)
@@ -596,9 +602,9 @@ int dfm_batch_create(dfm_ctx ctx, int precision, int max_systems, dfm_batch* out
   bt.ctx = &ctx->c;
   bt.prec = precision;
   bt.cap = max_systems;
-  if (bt.d_sc.alloc((size_t)16 * bt.cap) != hipSuccess || bt.d_fl.alloc((size_t)8 * bt.cap) != hipSuccess ||
+  if (bt.d_sc.alloc((size_t)kNScal * bt.cap) != hipSuccess || bt.d_fl.alloc((size_t)8 * bt.cap) != hipSuccess ||
       hipHostMalloc((void**)&bt.h_fl, sizeof(int) * 8 * bt.cap) != hipSuccess ||
-      hipHostMalloc((void**)&bt.h_sc, sizeof(double) * 16 * bt.cap) != hipSuccess) {
+      hipHostMalloc((void**)&bt.h_sc, sizeof(double) * kNScal * bt.cap) != hipSuccess) {
     delete h;
     set_error("dfm_batch_create: out of memory");
     return DFM_ERR_NOMEM;
@@ -657,7 +663,7 @@ int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
   hipStream_t st = bt.ctx->stream;
   rc = pcg_prepare(s, sg.v, prec == DFM_MIXED, st);
   if (rc == DFM_OK)
-    rc = pcg_start(s, sg.v, prec == DFM_MIXED, tol, max_iter, bt.d_sc.p + 16 * (size_t)sg.slot,
+    rc = pcg_start(s, sg.v, prec == DFM_MIXED, tol, max_iter, bt.d_sc.p + kNScal * (size_t)sg.slot,
                    bt.d_fl.p + 8 * (size_t)sg.slot, st);
   if (rc != DFM_OK) return rc;
   bt.free_slots.pop_back();

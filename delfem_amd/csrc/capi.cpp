@@ -199,10 +199,10 @@ static hipError_t sys_alloc_device(Sys& s) {
   // per-block partial sums: up to 2 values per block of the widest reduction grid (SpMV rows at 32 lanes)
   if ((e = s.d_partial.alloc(8 * 2048)) != hipSuccess) return e;
   if ((e = s.d_counter.alloc(8)) != hipSuccess) return e;
-  if ((e = s.d_scal.alloc(16)) != hipSuccess) return e;
+  if ((e = s.d_scal.alloc(kNScal)) != hipSuccess) return e;
   if ((e = s.d_iflag.alloc(8)) != hipSuccess) return e;
   if ((e = hipHostMalloc((void**)&s.h_pinned, 16 * sizeof(int))) != hipSuccess) return e;
-  if ((e = hipHostMalloc((void**)&s.h_pinned_d, 16 * sizeof(double))) != hipSuccess) return e;
+  if ((e = hipHostMalloc((void**)&s.h_pinned_d, kNScal * sizeof(double))) != hipSuccess) return e;
   if ((e = hipMemsetAsync(s.d_counter.p, 0, 8 * sizeof(unsigned), st)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(s.d_bc.p, 0, m * sizeof(int), st)) != hipSuccess) return e;
   if ((e = launch_fill(s.d_u.p, 0.0, m, st)) != hipSuccess) return e;
@@ -541,8 +541,8 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   s.factored = s.factored32 = false;
   s.prepared = -1;
   if (res_norm) {
-    DFM_HIP(launch_dot(s.d_res.p, s.d_res.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + 15, st));
-    DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
+    DFM_HIP(launch_dot(s.d_res.p, s.d_res.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP, st));
+    DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + S_TMP, sizeof(double), hipMemcpyDeviceToHost, st));
     DFM_HIP(hipStreamSynchronize(st));
     *res_norm = std::sqrt(s.h_pinned_d[0]);
   }
@@ -608,8 +608,8 @@ int dfm_sys_dot(dfm_sys sys, int iv1, int iv2, double* out) {
   double* b = vec_ptr(s, iv2);
   DFM_CHECK(a && b, DFM_ERR_ARG, "dfm_sys_dot: bad vector handle");
   hipStream_t st = s.ctx->stream;
-  DFM_HIP(launch_dot(a, b, mdof(s), s.d_partial.p, s.d_counter.p, s.d_scal.p + 15, st));
-  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
+  DFM_HIP(launch_dot(a, b, mdof(s), s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP, st));
+  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + S_TMP, sizeof(double), hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
   *out = s.h_pinned_d[0];
   return DFM_OK;
@@ -847,9 +847,9 @@ int dfm_sys_true_residual(dfm_sys sys, double* rel) {
   DFM_HIP(launch_copy(s.d_rhs.p, t, m, st));
   DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
                       s.d_upd.p, 1.0, t, st));
-  DFM_HIP(launch_dot(t, t, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + 14, st));
-  DFM_HIP(launch_dot(s.d_rhs.p, s.d_rhs.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + 15, st));
-  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + 14, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  DFM_HIP(launch_dot(t, t, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP, st));
+  DFM_HIP(launch_dot(s.d_rhs.p, s.d_rhs.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP + 1, st));
+  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + S_TMP, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
   *rel = s.h_pinned_d[1] > 0 ? std::sqrt(s.h_pinned_d[0] / s.h_pinned_d[1]) : 0.0;
   return DFM_OK;

@@ -88,6 +88,24 @@ __device__ inline bool stall_update(double rt, double& tbest, int& stall) {
   return stall >= kStallUpdates;
 }
 
+// FP32-matrix progress test at a reliable update (squared norms): rt, the FP64 true residual; rrec, the recursive
+// residual that scheduled the update (same iterate); tprev, the true residual at the previous update (||r0||^2
+// before the first; updated here). A cycle is slow when the FP32 matrix held the true residual well above the
+// recursive one (rt > kGap2 rrec), or when the true residual gained less than 4x over the cycle (the FP32
+// recurrence crawls: the cycles of a healthy run gain about 10x, delta = 0.1). kSlowCycles slow cycles in a row,
+// or kStallUpdates updates without halving the best true residual, end the FP32-matrix loop: a cycle must gain
+// about 5x for the FP32 matrix and vectors to pay off against FP64 vectors with the FP32 factor.
+constexpr double kGap2 = 4.0;
+constexpr double kProg2 = 1.0 / 16.0;
+constexpr int kSlowCycles = 2;
+__device__ inline bool mx_slow(double rt, double rrec, double& tprev, double& slow, double& tbest, int& stall) {
+  const bool st = stall_update(rt, tbest, stall);
+  const bool cycle_slow = rt > kGap2 * rrec || rt > kProg2 * tprev;
+  tprev = rt;
+  slow = cycle_slow ? slow + 1.0 : 0.0;
+  return st || slow >= (double)kSlowCycles;
+}
+
 inline unsigned vec_blocks(long long m, int nt) {
   long long b = (m + nt - 1) / nt;
   if (b < 1) b = 1;

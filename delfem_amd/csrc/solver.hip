@@ -20,7 +20,7 @@ constexpr int NT = 256;
 
 __global__ void k_solver_init(double* sc, int* fl, double tol2, int maxit, double delta2) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    for (int k = 0; k < 16; k++) sc[k] = 0.0;
+    for (int k = 0; k < kNScal; k++) sc[k] = 0.0;
     sc[S_TOL2] = tol2;
     sc[S_DELTA2] = delta2;
     fl[F_DONE] = 0;
@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(NT) k_pcg_start(const double* slot, int G, dou
     } else {
       sc[S_INV0] = 1.0 / t[0];
       sc[S_RMAX] = t[0];
+      sc[S_TPREV] = t[0];
     }
   }
 }
@@ -322,7 +323,7 @@ __global__ void __launch_bounds__(kRedNT) k_pcg_judge_beta(const double* slot_rr
         fl[F_DONE] = 1;
         return;
       }
-      if (stall_update(rt[0], sc[S_TBEST], fl[F_STALL])) {
+      if (mx_slow(rt[0], sc[S_RRTRIG], sc[S_TPREV], sc[S_SLOW], sc[S_TBEST], fl[F_STALL])) {
         sc[S_CONV] = sqrt(ratio2);
         sc[S_RR] = rt[0];
         fl[F_DONE] = 3;
@@ -342,6 +343,7 @@ __global__ void __launch_bounds__(kRedNT) k_pcg_judge_beta(const double* slot_rr
     if (due) {
       fl[F_NREL] = fl[F_NREL] + 1;
       sc[S_LASTUPD] = (double)it;
+      sc[S_RRTRIG] = rr[0];
     }
   }
   sc[S_BETA] = rz[0] / sc[S_RZ + (it & 1)];
@@ -607,7 +609,7 @@ static double* slot(Sys& s, int k) { return s.d_partial.p + (size_t)k * kMaxPart
 
 static int poll(Sys& s, hipStream_t st) {
   DFM_HIP(hipMemcpyAsync(s.h_pinned, s.d_iflag.p, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
-  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p, sizeof(double) * 16, hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p, sizeof(double) * kNScal, hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
@@ -783,6 +785,49 @@ static int pcg_start_T(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigne
   return DFM_OK;
 }
 
+// warm start: inv0 from ||b||^2 (slot_b), ||r||^2 (slot_r) of r = b - A x; converged at once when below tol
+__global__ void __launch_bounds__(NT) k_pcg_start_warm(const double* slot_b, const double* slot_r, int G, double* sc,
+                                                       int* fl) {
+  double bb[1], rr[1];
+  consume<NT, 1>(slot_b, G, bb);
+  consume<NT, 1>(slot_r, G, rr);
+  if (threadIdx.x == 0) {
+    sc[S_INV0] = 1.0 / bb[0];
+    sc[S_RR] = rr[0];
+    sc[S_RMAX] = rr[0];
+    if (rr[0] * sc[S_INV0] < sc[S_TOL2]) {
+      sc[S_CONV] = sqrt(rr[0] * sc[S_INV0]);
+      fl[F_ITOUT] = 0;
+      fl[F_DONE] = 1;
+    }
+  }
+}
+
+int pcg_start_warm(Sys& s, const PcgBufs& v, double tol, unsigned maxit, double* sc, int* fl, hipStream_t st) {
+  const long long m = (long long)s.n * s.len;
+  const dim3 vg = vgrid(m);
+  const int G = (int)vg.x;
+  double* r = (double*)v.r;
+  double* z = (double*)v.z;
+  double* p = (double*)v.p;
+  hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, st, sc, fl, tol * tol, (int)maxit, 0.01);
+  hipLaunchKernelGGL(k_vdot<double>, vg, dim3(NT), 0, st, (const double*)r, (const double*)r, m, slot(s, P_B),
+                     (const int*)nullptr);
+  DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
+                      (const double*)v.x, 1.0, r, st));
+  hipLaunchKernelGGL(k_vdot<double>, vg, dim3(NT), 0, st, (const double*)r, (const double*)r, m, slot(s, P_A),
+                     (const int*)nullptr);
+  hipLaunchKernelGGL(k_pcg_start_warm, dim3(1), dim3(NT), 0, st, (const double*)slot(s, P_B),
+                     (const double*)slot(s, P_A), G, sc, fl);
+  if (precond_apply_d(s, v, (const double*)r, z, fl, st) != hipSuccess) return DFM_ERR_HIP;
+  hipLaunchKernelGGL(k_vcopy<double>, vg, dim3(NT), 0, st, (const double*)z, p, m, (const int*)fl);
+  hipLaunchKernelGGL(k_vdot<double>, vg, dim3(NT), 0, st, (const double*)r, (const double*)z, m, slot(s, P_A),
+                     (const int*)fl);
+  hipLaunchKernelGGL(k_pcg_rho0, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, sc, fl);
+  DFM_HIP(hipGetLastError());
+  return DFM_OK;
+}
+
 int pcg_start(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double* sc, int* fl, hipStream_t st) {
   return mixed ? pcg_start_T<float>(s, v, true, tol, maxit, 1.0e-30, sc, fl, st)
                : pcg_start_T<double>(s, v, false, tol, maxit, 1.0e-30, sc, fl, st);
@@ -820,7 +865,7 @@ int pcg_finish(Sys& s, const PcgBufs& v, bool mixed, int done, hipStream_t st) {
 
 template <typename T>
 static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double& conv, unsigned& iter,
-                   bool& converged) {
+                   bool& converged, bool warm = false) {
   hipStream_t st = s.ctx->stream;
   int* fl = s.d_iflag.p;
   double* sc = s.d_scal.p;
@@ -829,7 +874,7 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
     if (s.d_part_sw.n < need) DFM_HIP(s.d_part_sw.alloc(need));
   }
   {
-    const int rc = pcg_start(s, v, mixed, tol, maxit, sc, fl, st);
+    const int rc = warm ? pcg_start_warm(s, v, tol, maxit, sc, fl, st) : pcg_start(s, v, mixed, tol, maxit, sc, fl, st);
     if (rc != DFM_OK) return rc;
   }
   const int gk = mixed ? 1 : (v.f32ilu ? 3 : 0);
@@ -898,14 +943,21 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
     rc = mixed ? pcg_run<float>(s, v, true, tol, maxit, conv, it, ok) : pcg_run<double>(s, v, false, tol, maxit, conv, it, ok);
   if (rc != DFM_OK) return rc;
   int nrel = s.h_pinned[F_NREL];
+  unsigned before = 0;  // iterations of a mixed-precision loop that was stopped
   if (mixed && s.h_pinned[F_DONE] == 3) {
-    // the FP32 matrix cannot reach the tolerance (cond(A) eps32 not small): solve again with FP64 vectors and
-    // matrix, keeping the FP32 factor (the preconditioner's precision does not bound the attainable accuracy)
+    // the FP32 matrix limits progress (cond(A) eps32 not small): continue from the FP64 solution with FP64
+    // vectors and matrix, keeping the FP32 factor (the preconditioner's precision does not bound the attainable
+    // accuracy); max_iter bounds both loops together
     s.last_fallback = 1;
+    before = (unsigned)s.h_pinned[F_IT] + 1;
     PcgBufs v64 = pcg_bufs(s, false);
     v64.f32ilu = true;
-    rc = pcg_run<double>(s, v64, false, tol, maxit, conv, it, ok);
-    if (rc != DFM_OK) return rc;
+    if (before < maxit) {
+      rc = pcg_run<double>(s, v64, false, tol, maxit - before, conv, it, ok, true);
+      if (rc != DFM_OK) return rc;
+    } else {
+      ok = false;
+    }
   }
   DFM_HIP(hipEventRecord(e1, st));
   DFM_HIP(hipEventSynchronize(e1));
@@ -914,8 +966,8 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *conv_ratio = conv;
-  if (ok) *iteration = it;
-  const unsigned its = ok ? it + 1 : maxit;
+  if (ok) *iteration = it + before;
+  const unsigned its = ok ? it + before + 1 : maxit;
   s.last_ms = ms;
   s.last_inner = its;
   s.last_ms_iter = its ? ms / its : 0.0;

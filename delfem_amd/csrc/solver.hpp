@@ -18,7 +18,11 @@ enum {
   S_BETA = 10,  // beta of the current iteration
   S_TBEST = 11, // mixed precision: smallest true ||r||^2 that counted as progress
   S_LASTUPD = 12, // mixed precision: iteration of the last reliable update
-  S_TMP = 15
+  S_RRTRIG = 13,  // mixed precision: recursive ||r||^2 that scheduled the pending reliable update
+  S_SLOW = 14,    // mixed precision: slow reliable-update cycles in a row
+  S_TPREV = 16,   // mixed precision: true ||r||^2 at the previous reliable update (||r0||^2 before the first)
+  S_TMP = 22,     // two slots of scratch for host-side dot products outside solves
+  kNScal = 24     // doubles per system
 };
 // int flags
 // F_DONE: 0 running, 1 converged, 2 max_iter, 3 mixed precision stalled (the FP32 matrix cannot reach the
@@ -42,12 +46,15 @@ struct PcgBufs {
   bool f32ilu = false;  // FP64 loop with the FP32 factor copy as preconditioner
 };
 
-// Start of a PCG solve on the system's own buffers with scalars/flags at sc/fl (16 doubles, 8 ints): x = 0,
+// Start of a PCG solve on the system's own buffers with scalars/flags at sc/fl (kNScal doubles, 8 ints): x = 0,
 // guard on ||r0||, z = M^-1 r, p = z, rho. pcg_prepare first (mixed: r = (float) rhs). pcg_finish after the
 // loop stopped with flag `done` (1 converged, 2 max_iter): final fold (mixed) and the residual handle.
+// pcg_start_warm: the FP64 loop continued from the FP64 solution a stopped mixed-precision loop left (done 3):
+// the residual handle still holds b; r = b - A x, and the tolerance stays relative to ||b||.
 PcgBufs pcg_bufs(Sys& s, bool mixed);
 int pcg_prepare(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st);
 int pcg_start(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, double* sc, int* fl, hipStream_t st);
+int pcg_start_warm(Sys& s, const PcgBufs& v, double tol, unsigned maxit, double* sc, int* fl, hipStream_t st);
 int pcg_finish(Sys& s, const PcgBufs& v, bool mixed, int done, hipStream_t st);
 
 int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration);

@@ -2,9 +2,11 @@
 
 - DFM_FP32_ILU (FP64 loop, FP32 copy of the ILU factor as preconditioner): converges to the FP64 tolerance with
   an iteration count close to the FP64 one; the true residual (oracle matrix) meets the tolerance.
-- DFM_MIXED on a problem whose condition number defeats an FP32 matrix (thin clamped elasticity strip, tolerance
-  1e-10): the stagnation of the FP64 true residual is detected and the solve is repeated with the FP32 factor
-  in an FP64 loop, which converges; the same in a batch.
+- DFM_MIXED on a problem where FP32 vectors slow the Krylov recurrence (thin clamped elasticity strip, tolerance
+  1e-10): slow reliable-update cycles are detected, and the solve continues from its FP64 solution in an FP64 loop
+  with the FP32 factor, which converges to the true residual an FP64 loop reaches; the switch costs at most a
+  few times an FP32_ILU solve from scratch (the mixed loop's progress is only judged at reliable updates); the
+  same in a batch.
 """
 import numpy as np
 import pytest
@@ -54,17 +56,25 @@ def test_mixed_stall_falls_back(ctx):
     w, s = _strip(ctx)
     ok, conv, it = s.solve_pcg(1e-10, 20000, N.MIXED)
     nrel, fell_back = s.last_solve_info()
-    assert ok and conv < 1e-10
-    assert fell_back
-    # the fallback is the FP64 loop with the FP32 factor: the iterations of an FP32_ILU solve from scratch
+    true_rel = s.true_residual()  # FP64 matrix, the solution left in the update handle
     W.assemble(s, w)
     s.factor()
     ok2, conv2, it2 = s.solve_pcg(1e-10, 20000, N.FP32_ILU)
-    assert ok2 and it2 == it
+    true_rel2 = s.true_residual()
+    print(f"mixed then FP64 vectors: {it} iterations ({nrel} reliable updates), true residual {true_rel:.3e}; "
+          f"FP32_ILU from scratch: {it2} iterations, true residual {true_rel2:.3e}")
+    assert ok and conv < 1e-10 and fell_back
+    # the recursive FP64 residual of this strip drifts from the true one by about 1e-9 relative, in any FP64 loop
+    assert true_rel <= max(1.01e-10, 2.0 * true_rel2)
+    assert ok2 and it <= 3 * it2
 
 
 def test_mixed_stall_falls_back_in_batch(ctx):
     w, s = _strip(ctx)
+    ok, conv1, it1 = s.solve_pcg(1e-10, 20000, N.MIXED)
+    x1 = s.get_vector(-2)
+    W.assemble(s, w)
+    s.factor()
     b = N.Batch(ctx, N.MIXED, 4)
     b.add(s, 1e-10, 20000)
     while b.step(64):
@@ -72,4 +82,8 @@ def test_mixed_stall_falls_back_in_batch(ctx):
     st, conv, it, _ = b.status(s)
     nrel, fell_back = b.info(s)
     assert st == 1 and conv < 1e-10 and fell_back
+    # the batch engine reduces in its own order (one workgroup per system), so the counts are close, not equal
+    assert abs(it - it1) <= 0.02 * it1 + 5, (it, it1)
+    x = s.get_vector(-2)
+    assert np.linalg.norm(x - x1) <= 1e-6 * np.linalg.norm(x1)
     b.close()
