@@ -285,15 +285,25 @@ class System:
     def prec_apply(self, iv):
         _check(lib().dfm_prec_apply(self.h, iv))
 
+    def set_fill_level(self, lev):
+        """ILU(lev) (CPreconditioner_ILU::SetFillInLevel): 0 ILU(0), -1 no limit; factor again afterwards."""
+        _check(lib().dfm_prec_set_fill_level(self.h, int(lev)))
+
+    def factor_size(self):
+        n = C.c_longlong()
+        _check(lib().dfm_prec_factor_size(self.h, C.byref(n)))
+        return n.value
+
     def get_factor(self):
         bs = self.len * self.len
-        f = {"frowptr": np.empty(self.nnode + 1, np.uint32), "fcolind": np.empty(max(self.ncrs, 1), np.uint32),
-             "diaind": np.empty(self.nnode, np.uint32), "fcrs": np.empty(max(self.ncrs, 1) * bs),
+        nf = self.factor_size()
+        f = {"frowptr": np.empty(self.nnode + 1, np.uint32), "fcolind": np.empty(max(nf, 1), np.uint32),
+             "diaind": np.empty(self.nnode, np.uint32), "fcrs": np.empty(max(nf, 1) * bs),
              "fdia": np.empty(self.nnode * bs)}
         _check(lib().dfm_prec_get_factor(self.h, _p(f["frowptr"]), _p(f["fcolind"]), _p(f["diaind"]), _p(f["fcrs"]),
                                          _p(f["fdia"])))
-        f["fcolind"] = f["fcolind"][: self.ncrs]
-        f["fcrs"] = f["fcrs"][: self.ncrs * bs]
+        f["fcolind"] = f["fcolind"][:nf]
+        f["fcrs"] = f["fcrs"][: nf * bs]
         return f
 
     # solvers

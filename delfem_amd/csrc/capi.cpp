@@ -35,6 +35,7 @@ long long device_bytes_in_use() { return g_dev_bytes.load(); }
 
 Sys::~Sys() {
   d_rowptr.release(); d_colind.release(); d_diaind.release(); d_lsplit.release(); d_usplit.release();
+  d_frowptr.release(); d_fcolind.release(); d_fdiaind.release(); d_a2f.release();
   d_order.release(); d_inv.release(); d_elsup_ptr.release(); d_elsup.release(); d_lnods.release(); d_bc.release();
   d_coords.release(); d_u.release(); d_v.release();
   d_crs.release(); d_dia.release(); d_crs32.release(); d_dia32.release();
@@ -61,19 +62,15 @@ static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
 }
 
 // device pattern, ordering maps and sweep schedules for the current order
-static int upload_structure(Sys& s) {
+// Factor pattern and sweep schedule (device copies); the slab values are filled by dfm_prec_factor.
+static int upload_sweeps(Sys& s) {
   hipStream_t st = s.ctx->stream;
-  DFM_HIP(upload(s.d_rowptr, s.rowptr, st));
-  DFM_HIP(upload(s.d_colind, s.colind, st));
-  DFM_HIP(upload(s.d_diaind, s.diaind, st));
   DFM_HIP(upload(s.d_lsplit, s.lsplit, st));
   DFM_HIP(upload(s.d_usplit, s.usplit, st));
-  DFM_HIP(upload(s.d_order, s.order, st));
-  DFM_HIP(upload(s.d_inv, s.inv, st));
-  s.sell.release_dev();
-  DFM_HIP(upload(s.sell.d_sptr, s.sell.sptr, st));
-  DFM_HIP(upload(s.sell.d_col, s.sell.col, st));
-  DFM_HIP(upload(s.sell.d_src, s.sell.src, st));
+  DFM_HIP(upload(s.d_frowptr, s.frowptr, st));
+  DFM_HIP(upload(s.d_fcolind, s.fcolind, st));
+  DFM_HIP(upload(s.d_fdiaind, s.fdiaind, st));
+  DFM_HIP(upload(s.d_a2f, s.a2f, st));
   for (Sweep* w : {&s.fwd, &s.bwd}) {
     w->release_dev();
     DFM_HIP(upload(w->d_grp, w->grp, st));
@@ -86,6 +83,21 @@ static int upload_structure(Sys& s) {
     DFM_HIP(upload(w->d_esrc, w->esrc, st));
   }
   return DFM_OK;
+}
+
+// Device copies of the matrix pattern, its SELL layout (values filled at assembly) and the sweeps.
+static int upload_structure(Sys& s) {
+  hipStream_t st = s.ctx->stream;
+  DFM_HIP(upload(s.d_rowptr, s.rowptr, st));
+  DFM_HIP(upload(s.d_colind, s.colind, st));
+  DFM_HIP(upload(s.d_diaind, s.diaind, st));
+  DFM_HIP(upload(s.d_order, s.order, st));
+  DFM_HIP(upload(s.d_inv, s.inv, st));
+  s.sell.release_dev();
+  DFM_HIP(upload(s.sell.d_sptr, s.sell.sptr, st));
+  DFM_HIP(upload(s.sell.d_col, s.sell.col, st));
+  DFM_HIP(upload(s.sell.d_src, s.sell.src, st));
+  return upload_sweeps(s);
 }
 
 static int apply_order(Sys& s, const std::vector<int>& order, const std::vector<int>& color,
@@ -267,6 +279,7 @@ int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
   s.order = a.order; s.inv = a.inv; s.natural = a.natural; s.ncolors = a.ncolors;
   s.color_of = a.color_of; s.tile_of = a.tile_of;
   s.rowptr = a.rowptr; s.colind = a.colind; s.diaind = a.diaind; s.lsplit = a.lsplit; s.usplit = a.usplit;
+  s.fill_level = a.fill_level; s.frowptr = a.frowptr; s.fcolind = a.fcolind; s.fdiaind = a.fdiaind; s.a2f = a.a2f;
   s.perm_crs = a.perm_crs;
   copy_sweep_host(s.fwd, a.fwd);
   copy_sweep_host(s.bwd, a.bwd);
@@ -695,15 +708,23 @@ int dfm_prec_factor(dfm_sys sys) {
   Sys& s = sys->s;
   DFM_CHECK(s.assembled, DFM_ERR_STATE, "dfm_prec_factor: matrix not assembled");
   hipStream_t st = s.ctx->stream;
-  if (!s.d_fcrs.p) DFM_HIP(s.d_fcrs.alloc((size_t)s.ncrs * s.bs));
+  const long long fncrs = (long long)s.fcolind.size();
+  if (s.d_fcrs.n != (size_t)std::max<long long>(fncrs * s.bs, 1))
+    DFM_HIP(s.d_fcrs.alloc((size_t)std::max<long long>(fncrs * s.bs, 1)));
   if (!s.d_fdia.p) DFM_HIP(s.d_fdia.alloc((size_t)s.n * s.bs));
-  // SetValue_Initialize: the factor pattern is the matrix pattern (ILU(0)), values copied
-  DFM_HIP(launch_copy(s.d_crs.p, s.d_fcrs.p, (long long)s.ncrs * s.bs, st));
+  // SetValue_Initialize: the factor pattern takes the matrix values (a copy when the patterns coincide, ILU(0);
+  // a scatter into the zeroed fill pattern otherwise)
+  if (s.a2f.empty()) {
+    DFM_HIP(launch_copy(s.d_crs.p, s.d_fcrs.p, (long long)s.ncrs * s.bs, st));
+  } else {
+    DFM_HIP(launch_fill(s.d_fcrs.p, 0.0, fncrs * s.bs, st));
+    DFM_HIP(launch_scatter_blocks(s.ncrs, s.bs, s.d_a2f.p, s.d_crs.p, s.d_fcrs.p, st));
+  }
   DFM_HIP(launch_copy(s.d_dia.p, s.d_fdia.p, (long long)s.n * s.bs, st));
   DFM_HIP(hipMemsetAsync(s.d_iflag.p + 4, 0, sizeof(int), st));
   SweepDev fw{s.fwd.nphase(), s.fwd.phase_gptr.data(), s.fwd.phase_nlev.data(), s.fwd.d_grp.p,
               s.fwd.rowlist.empty() ? nullptr : s.fwd.d_rowlist.p, s.fwd.d_lev.p};
-  DFM_HIP(launch_ilu_factor(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_diaind.p, s.d_fcrs.p, s.d_fdia.p, fw,
+  DFM_HIP(launch_ilu_factor(s.n, s.len, s.d_frowptr.p, s.d_fcolind.p, s.d_fdiaind.p, s.d_fcrs.p, s.d_fdia.p, fw,
                             s.d_iflag.p + 4, st));
   int nsing = 0;
   DFM_HIP(hipMemcpyAsync(s.h_pinned + 8, s.d_iflag.p + 4, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -742,16 +763,47 @@ int dfm_prec_apply(dfm_sys sys, int iv) {
   return DFM_OK;
 }
 
+int dfm_prec_set_fill_level(dfm_sys sys, int lev) {
+  NEED(sys, "dfm_prec_set_fill_level");
+  Sys& s = sys->s;
+  DFM_CHECK(lev >= -1, DFM_ERR_ARG, "dfm_prec_set_fill_level: level must be >= -1");
+  DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_prec_set_fill_level: the system is being solved in a batch");
+  if (lev == s.fill_level) return DFM_OK;
+  const int old = s.fill_level;
+  s.fill_level = lev;
+  try {
+    build_sweeps(s);
+  } catch (const std::exception& ex) {
+    s.fill_level = old;
+    build_sweeps(s);
+    set_error(std::string("dfm_prec_set_fill_level: ") + ex.what());
+    return DFM_ERR_ARG;
+  }
+  // the matrix (pattern, values, SELL copy) is unchanged; the factor and its slabs must be computed again
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  s.factored = s.factored32 = false;
+  s.prepared = -1;
+  return upload_sweeps(s);
+}
+
+int dfm_prec_factor_size(dfm_sys sys, long long* fncrs) {
+  NEED(sys, "dfm_prec_factor_size");
+  NEED(fncrs, "dfm_prec_factor_size");
+  *fncrs = (long long)sys->s.fcolind.size();
+  return DFM_OK;
+}
+
 int dfm_prec_get_factor(dfm_sys sys, unsigned* frowptr, unsigned* fcolind, unsigned* diaind, double* fcrs,
                         double* fdia) {
   NEED(sys, "dfm_prec_get_factor");
   Sys& s = sys->s;
   DFM_CHECK(s.factored, DFM_ERR_STATE, "dfm_prec_get_factor: preconditioner not factored");
   DFM_HIP(hipStreamSynchronize(s.ctx->stream));
-  if (frowptr) std::memcpy(frowptr, s.rowptr.data(), sizeof(unsigned) * (s.n + 1));
-  if (fcolind) std::memcpy(fcolind, s.colind.data(), sizeof(unsigned) * s.ncrs);
-  if (diaind) std::memcpy(diaind, s.diaind.data(), sizeof(unsigned) * s.n);
-  if (fcrs) DFM_HIP(hipMemcpy(fcrs, s.d_fcrs.p, sizeof(double) * s.ncrs * s.bs, hipMemcpyDeviceToHost));
+  const size_t fncrs = s.fcolind.size();
+  if (frowptr) std::memcpy(frowptr, s.frowptr.data(), sizeof(unsigned) * (s.n + 1));
+  if (fcolind) std::memcpy(fcolind, s.fcolind.data(), sizeof(unsigned) * fncrs);
+  if (diaind) std::memcpy(diaind, s.fdiaind.data(), sizeof(unsigned) * s.n);
+  if (fcrs) DFM_HIP(hipMemcpy(fcrs, s.d_fcrs.p, sizeof(double) * fncrs * s.bs, hipMemcpyDeviceToHost));
   if (fdia) DFM_HIP(hipMemcpy(fdia, s.d_fdia.p, sizeof(double) * s.n * s.bs, hipMemcpyDeviceToHost));
   return DFM_OK;
 }

@@ -135,19 +135,25 @@ struct Sys {
   std::vector<int> color_of;  // per device row (MC/BMC), empty for natural / external orderings
   std::vector<int> tile_of;   // per device row (BMC)
 
-  // device-numbering pattern (shared by the permuted matrix and the ILU(0) factor)
-  std::vector<unsigned> rowptr, colind, diaind, lsplit, usplit;
+  // device-numbering pattern of the permuted matrix
+  std::vector<unsigned> rowptr, colind, diaind;
   std::vector<long long> perm_crs;  // original crs index -> device crs index
+  // ILU factor pattern (device numbering): the matrix pattern for ILU(0), the level-of-fill pattern for ILU(k);
+  // lsplit/usplit split each factor row into entries outside / inside its sweep group
+  int fill_level = 0;  // 0: ILU(0); k > 0: ILU(k); -1: no limit
+  std::vector<unsigned> frowptr, fcolind, fdiaind, lsplit, usplit;
+  std::vector<unsigned> a2f;  // matrix entry -> factor entry (empty when the patterns coincide)
   Sweep fwd, bwd;
   Sell sell;
 
   // device arrays
   DBuf<unsigned> d_rowptr, d_colind, d_diaind, d_lsplit, d_usplit;
+  DBuf<unsigned> d_frowptr, d_fcolind, d_fdiaind, d_a2f;
   DBuf<int> d_order, d_inv, d_elsup_ptr, d_elsup, d_lnods, d_bc;
   DBuf<double> d_coords, d_u, d_v;
   DBuf<double> d_crs, d_dia;         // matrix, device numbering, FP64
   DBuf<float> d_crs32, d_dia32;      // FP32 copy (mixed precision)
-  DBuf<double> d_fcrs, d_fdia;       // ILU(0) factor FP64
+  DBuf<double> d_fcrs, d_fdia;       // ILU factor FP64 (factor pattern)
   DBuf<double> d_res, d_upd, d_rhs;  // handles -1, -2 and a copy of the assembled right-hand side
   std::vector<DBuf<double>*> d_tmp;  // handles 0..
   DBuf<double> d_work;               // solver work vectors
@@ -194,7 +200,8 @@ void batch_detach(struct Batch* b, Sys* s);
 void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vector<unsigned>& rowptr,
                    std::vector<unsigned>& colind, std::vector<int>& elsup_ptr, std::vector<int>& elsup);
 void build_device_pattern(Sys& s);  // from order/inv: rowptr, colind, diaind, perm_crs
-void build_schedule(Sys& s);        // fwd/bwd sweeps, lsplit/usplit, slabs
+void build_schedule(Sys& s);        // SELL copy of the pattern, then build_sweeps
+void build_sweeps(Sys& s);          // factor pattern, fwd/bwd sweeps, lsplit/usplit, slabs
 void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors);
 void make_blockcolor(const Sys& s, int tile, std::vector<int>& order, std::vector<int>& color_of_new,
                      std::vector<int>& tile_of_new, int& ncolors);

@@ -1,6 +1,7 @@
 // Host symbolic phase: CRS pattern from the mesh, ILU(0) orderings, the device-numbering pattern and the
 // phase/group/level schedules of the triangular sweeps and the numeric factorisation.
 #include <algorithm>
+#include <queue>
 #include <cmath>
 #include <deque>
 #include <numeric>
@@ -89,6 +90,79 @@ void build_device_pattern(Sys& s) {
 
 static const int kGroup = 64;  // rows per group: one wavefront, one row per lane
 
+// ILU factor pattern in the device numbering. ILU(0): the matrix pattern. ILU(k): the level-of-fill pattern of
+// CMatDiaFrac_BlkCrs::AddFracPtn (src/matvec/matdiafrac_blkcrs.cpp:1254-1381), row by row: the row's entries
+// with level < k pivot in ascending column order, each spreading the upper entries of its own (final) factor
+// row whose level is < k; an entry met for the first time gets level max(lev_ik, lev_kj) + 1 (:1354-1356) and
+// keeps it (a later, lower level does not replace it). Pivots are taken from a min-heap, so fill below the
+// diagonal pivots in its turn. Then the matrix entry -> factor entry map for the value scatter (SetValue).
+static void build_factor_pattern(Sys& s) {
+  const int n = s.n;
+  if (s.fill_level == 0) {
+    s.frowptr = s.rowptr;
+    s.fcolind = s.colind;
+    s.fdiaind = s.diaind;
+    s.a2f.clear();
+    return;
+  }
+  const int kmax = s.fill_level;  // -1: no limit
+  auto pivots = [&](int lev) { return kmax < 0 || lev < kmax; };
+  std::vector<unsigned> fp((size_t)n + 1, 0), fc, fd((size_t)n, 0);
+  std::vector<int> fl;  // level of every factor entry
+  fc.reserve(s.colind.size() * 2);
+  fl.reserve(s.colind.size() * 2);
+  std::vector<int> lev_of((size_t)n, -1);  // level of column j in the current row, -1: absent
+  std::vector<int> touched;
+  std::priority_queue<int, std::vector<int>, std::greater<int>> heap;
+  for (int i = 0; i < n; i++) {
+    touched.clear();
+    for (unsigned q = s.rowptr[i]; q < s.rowptr[i + 1]; q++) {
+      const int j = (int)s.colind[q];
+      lev_of[j] = 0;
+      touched.push_back(j);
+      if (j < i && pivots(0)) heap.push(j);
+    }
+    while (!heap.empty()) {
+      const int k = heap.top();
+      heap.pop();
+      const int lik = lev_of[k];
+      for (unsigned kj = fd[k]; kj < fp[k + 1]; kj++) {
+        const int lkj = fl[kj];
+        if (!pivots(lkj)) continue;
+        const int j = (int)fc[kj];
+        if (j == i || lev_of[j] >= 0) continue;
+        const int l = std::max(lik, lkj) + 1;
+        lev_of[j] = l;
+        touched.push_back(j);
+        if (j < i && pivots(l)) heap.push(j);
+      }
+    }
+    std::sort(touched.begin(), touched.end());
+    fd[i] = fp[i] + (unsigned)touched.size();
+    for (size_t t = 0; t < touched.size(); t++) {
+      const int j = touched[t];
+      if (j > i && fd[i] == fp[i] + touched.size()) fd[i] = fp[i] + (unsigned)t;
+      fc.push_back((unsigned)j);
+      fl.push_back(lev_of[j]);
+      lev_of[j] = -1;
+    }
+    if (fc.size() > 0xFFFFFFF0ull) throw std::runtime_error("ILU(k): factor pattern too large");
+    fp[i + 1] = (unsigned)fc.size();
+  }
+  s.frowptr.swap(fp);
+  s.fcolind.swap(fc);
+  s.fdiaind.swap(fd);
+  // both rows sorted: merge
+  s.a2f.assign(s.colind.size(), 0);
+  for (int i = 0; i < n; i++) {
+    unsigned f = s.frowptr[i];
+    for (unsigned q = s.rowptr[i]; q < s.rowptr[i + 1]; q++) {
+      while (s.fcolind[f] != s.colind[q]) f++;
+      s.a2f[q] = f;
+    }
+  }
+}
+
 // Level sets of the whole matrix (natural or arbitrary orderings): every level is a phase of independent
 // rows, handed out in chunks of 64 through a row list.
 static void level_sweeps(Sys& s) {
@@ -100,14 +174,14 @@ static void level_sweeps(Sys& s) {
     if (dir == 0) {
       for (int i = 0; i < n; i++) {
         int l = 0;
-        for (unsigned q = s.rowptr[i]; q < s.diaind[i]; q++) l = std::max(l, lev[s.colind[q]] + 1);
+        for (unsigned q = s.frowptr[i]; q < s.fdiaind[i]; q++) l = std::max(l, lev[s.fcolind[q]] + 1);
         lev[i] = l;
         nlev = std::max(nlev, l + 1);
       }
     } else {
       for (int i = n - 1; i >= 0; i--) {
         int l = 0;
-        for (unsigned q = s.diaind[i]; q < s.rowptr[i + 1]; q++) l = std::max(l, lev[s.colind[q]] + 1);
+        for (unsigned q = s.fdiaind[i]; q < s.frowptr[i + 1]; q++) l = std::max(l, lev[s.fcolind[q]] + 1);
         lev[i] = l;
         nlev = std::max(nlev, l + 1);
       }
@@ -134,84 +208,82 @@ static void level_sweeps(Sys& s) {
     }
     w.lev.assign((size_t)n, 0);
   }
-  s.lsplit = s.diaind;
-  s.usplit = s.diaind;
+  s.lsplit = s.fdiaind;
+  s.usplit = s.fdiaind;
 }
 
-// Contiguous groups (MC / BMC orderings): group g = device rows [gstart[g], gstart[g+1]) of colour
-// colour_of_group[g]; groups of one colour are independent, dependencies inside a group are resolved by
-// intra-group levels.
-static void group_sweeps(Sys& s, const std::vector<int>& gstart, const std::vector<int>& gcolor, int ncolors) {
+// Contiguous groups (MC / BMC orderings): group g = device rows [gstart[g], gstart[g+1]); dependencies inside a
+// group are resolved by intra-group levels, those on other groups by phases: a group's forward phase is one more
+// than the latest phase of the groups its rows' lower entries reach (backward: upper entries), so every group
+// runs after the groups it reads. For ILU(0) of a colour-major ordering this is at most the colour; fill (ILU(k))
+// may link groups of one colour, which then take later phases.
+static void group_sweeps(Sys& s, const std::vector<int>& gstart) {
   const int n = s.n;
   const int ngroup = (int)gstart.size() - 1;
+  const std::vector<unsigned>& rp = s.frowptr;
+  const std::vector<unsigned>& ci = s.fcolind;
+  const std::vector<unsigned>& di = s.fdiaind;
   std::vector<int> grp_of((size_t)n);
   for (int g = 0; g < ngroup; g++)
     for (int r = gstart[g]; r < gstart[g + 1]; r++) grp_of[r] = g;
   s.lsplit.assign((size_t)n, 0);
   s.usplit.assign((size_t)n, 0);
-  std::vector<int> flev((size_t)n, 0), blev((size_t)n, 0);
+  std::vector<int> flev((size_t)n, 0), blev((size_t)n, 0), fph((size_t)ngroup, 0), bph((size_t)ngroup, 0);
   for (int i = 0; i < n; i++) {
     const int g = grp_of[i], g0 = gstart[g];
-    unsigned ls = s.diaind[i];
-    for (unsigned q = s.rowptr[i]; q < s.diaind[i]; q++) {
-      const int j = (int)s.colind[q];
+    unsigned ls = di[i];
+    for (unsigned q = rp[i]; q < di[i]; q++) {
+      const int j = (int)ci[q];
       if (j >= g0) { ls = q; break; }
-      if (gcolor[grp_of[j]] >= gcolor[g]) throw std::runtime_error("ordering: lower dependency inside a colour");
+      fph[g] = std::max(fph[g], fph[grp_of[j]] + 1);
     }
     s.lsplit[i] = ls;
     int l = 0;
-    for (unsigned q = ls; q < s.diaind[i]; q++) l = std::max(l, flev[s.colind[q]] + 1);
+    for (unsigned q = ls; q < di[i]; q++) l = std::max(l, flev[ci[q]] + 1);
     flev[i] = l;
   }
   for (int i = n - 1; i >= 0; i--) {
     const int g = grp_of[i], g1 = gstart[g + 1];
-    unsigned us = s.rowptr[i + 1];
-    for (unsigned q = s.diaind[i]; q < s.rowptr[i + 1]; q++) {
-      const int j = (int)s.colind[q];
+    unsigned us = rp[i + 1];
+    for (unsigned q = di[i]; q < rp[i + 1]; q++) {
+      const int j = (int)ci[q];
       if (j >= g1) {
-        if (us == s.rowptr[i + 1]) us = q;
-        if (gcolor[grp_of[j]] <= gcolor[g]) throw std::runtime_error("ordering: upper dependency inside a colour");
+        if (us == rp[i + 1]) us = q;
+        bph[g] = std::max(bph[g], bph[grp_of[j]] + 1);
       }
     }
     s.usplit[i] = us;
     int l = 0;
-    for (unsigned q = s.diaind[i]; q < us; q++) l = std::max(l, blev[s.colind[q]] + 1);
+    for (unsigned q = di[i]; q < us; q++) l = std::max(l, blev[ci[q]] + 1);
     blev[i] = l;
   }
-  // groups are ordered by colour in the device numbering
-  std::vector<int> cstart((size_t)ncolors + 1, 0);
-  for (int g = 0; g < ngroup; g++) cstart[gcolor[g] + 1]++;
-  for (int c = 0; c < ncolors; c++) cstart[c + 1] += cstart[c];
-  for (int g = 1; g < ngroup; g++)
-    if (gcolor[g] < gcolor[g - 1]) throw std::runtime_error("ordering: groups not colour-major");
   for (int dir = 0; dir < 2; dir++) {
     Sweep& w = dir == 0 ? s.fwd : s.bwd;
     const std::vector<int>& lev = dir == 0 ? flev : blev;
+    const std::vector<int>& ph = dir == 0 ? fph : bph;
+    int nph = 0;
+    for (int g = 0; g < ngroup; g++) nph = std::max(nph, ph[g] + 1);
     w.rowlist.clear();
     w.lev.assign((size_t)n, 0);
-    for (int i = 0; i < n; i++) w.lev[i] = (uint8_t)lev[i];
-    // phase p runs colour p (forward) or colour ncolors-1-p (backward); group lists in execution order
+    for (int i = 0; i < n; i++) {
+      if (lev[i] > 255) throw std::runtime_error("schedule: more than 256 levels inside a group");
+      w.lev[i] = (uint8_t)lev[i];
+    }
+    // groups of each phase in device order; (start, end) pairs in execution order
+    std::vector<std::vector<int>> members((size_t)nph);
+    for (int g = 0; g < ngroup; g++) members[ph[g]].push_back(g);
     w.grp.clear();
     w.phase_gptr.assign(1, 0);
     w.phase_nlev.clear();
-    std::vector<int> gl;  // group starts in execution order, then the final end
-    for (int p = 0; p < ncolors; p++) {
-      const int c = dir == 0 ? p : ncolors - 1 - p;
+    for (int p = 0; p < nph; p++) {
       int nl = 1;
-      for (int g = cstart[c]; g < cstart[c + 1]; g++) {
-        int mx = 0;
-        for (int r = gstart[g]; r < gstart[g + 1]; r++) mx = std::max(mx, lev[r]);
-        nl = std::max(nl, mx + 1);
-        gl.push_back(g);
+      for (int g : members[p]) {
+        for (int r = gstart[g]; r < gstart[g + 1]; r++) nl = std::max(nl, lev[r] + 1);
+        w.grp.push_back(gstart[g]);
+        w.grp.push_back(gstart[g + 1]);
       }
-      w.phase_gptr.push_back((int)gl.size());
+      w.phase_gptr.push_back((int)w.grp.size() / 2);
       w.phase_nlev.push_back(nl);
-    }
-    // groups are stored as (start, end) pairs in execution order: grp[2k], grp[2k+1]
-    w.grp.resize(gl.size() * 2);
-    for (size_t k = 0; k < gl.size(); k++) {
-      w.grp[2 * k] = gstart[gl[k]];
-      w.grp[2 * k + 1] = gstart[gl[k] + 1];
     }
   }
 }
@@ -237,9 +309,9 @@ static void build_slabs(const Sys& s, Sweep& w, bool forward) {
     for (int l = 0; l < nrows; l++) {
       const int row = indirect ? w.rowlist[a + l] : a + l;
       if (forward) {
-        o0[l] = s.rowptr[row]; o1[l] = s.lsplit[row]; i0[l] = s.lsplit[row]; i1[l] = s.diaind[row];
+        o0[l] = s.frowptr[row]; o1[l] = s.lsplit[row]; i0[l] = s.lsplit[row]; i1[l] = s.fdiaind[row];
       } else {
-        i0[l] = s.diaind[row]; i1[l] = s.usplit[row]; o0[l] = s.usplit[row]; o1[l] = s.rowptr[row + 1];
+        i0[l] = s.fdiaind[row]; i1[l] = s.usplit[row]; o0[l] = s.usplit[row]; o1[l] = s.frowptr[row + 1];
       }
       const int no = (int)(o1[l] - o0[l]), ni = (int)(i1[l] - i0[l]);
       if (no > 255 || ni > 255) throw std::runtime_error("schedule: more than 255 entries per row and class");
@@ -261,12 +333,12 @@ static void build_slabs(const Sys& s, Sweep& w, bool forward) {
     for (int l = 0; l < nrows; l++) {
       for (unsigned q = o0[l]; q < o1[l]; q++) {
         const size_t k = base + (size_t)(q - o0[l]) * kGroup + l;
-        w.ecol[k] = s.colind[q];
+        w.ecol[k] = s.fcolind[q];
         w.esrc[k] = q;
       }
       for (unsigned q = i0[l]; q < i1[l]; q++) {
         const size_t k = base + (size_t)(wo + (q - i0[l])) * kGroup + l;
-        w.ecol[k] = s.colind[q] - (unsigned)a;  // contiguous group: lane of the source row
+        w.ecol[k] = s.fcolind[q] - (unsigned)a;  // contiguous group: lane of the source row
         w.esrc[k] = q;
       }
     }
@@ -307,6 +379,11 @@ static void build_sell(Sys& s) {
 
 void build_schedule(Sys& s) {
   build_sell(s);
+  build_sweeps(s);
+}
+
+void build_sweeps(Sys& s) {
+  build_factor_pattern(s);
   if (s.color_of.empty()) {
     level_sweeps(s);
     build_slabs(s, s.fwd, true);
@@ -314,7 +391,7 @@ void build_schedule(Sys& s) {
     return;
   }
   // contiguous groups: tiles (BMC) or 64-row chunks of each colour (MC)
-  std::vector<int> gstart(1, 0), gcolor;
+  std::vector<int> gstart(1, 0);
   const int n = s.n;
   int i = 0;
   while (i < n) {
@@ -324,10 +401,9 @@ void build_schedule(Sys& s) {
     while (j < n && s.color_of[j] == c && (t < 0 ? (j - i) < kGroup : s.tile_of[j] == t)) j++;
     if (j - i > kGroup) throw std::runtime_error("ordering: tile larger than 64 rows");
     gstart.push_back(j);
-    gcolor.push_back(c);
     i = j;
   }
-  group_sweeps(s, gstart, gcolor, s.ncolors);
+  group_sweeps(s, gstart);
   build_slabs(s, s.fwd, true);
   build_slabs(s, s.bwd, false);
 }

@@ -50,6 +50,21 @@ hipError_t launch_fill_f(float* x, float v, long long m, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_f, stream_grid(m), dim3(256), 0, st, x, v, m);
   return hipGetLastError();
 }
+// dst block map[q] = src block q (bs doubles per block): matrix values into the ILU(k) fill pattern
+__global__ void k_scatter_blocks(long long nblk, int bs, const unsigned* __restrict__ map, const double* __restrict__ src,
+                                 double* __restrict__ dst) {
+  const long long m = nblk * bs;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x) {
+    const long long q = i / bs;
+    dst[(long long)map[q] * bs + (i - q * bs)] = src[i];
+  }
+}
+hipError_t launch_scatter_blocks(long long nblk, int bs, const unsigned* map, const double* src, double* dst,
+                                 hipStream_t st) {
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_blocks, stream_grid(nblk * bs), dim3(256), 0, st, nblk, bs, map, src, dst);
+  return hipGetLastError();
+}
 hipError_t launch_copy(const double* x, double* y, long long m, hipStream_t st) {
   if (m <= 0 || x == y) return hipSuccess;
   hipLaunchKernelGGL(k_copy, stream_grid(m), dim3(256), 0, st, x, y, m);

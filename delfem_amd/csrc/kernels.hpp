@@ -39,6 +39,8 @@ hipError_t launch_update_field(int n, int len, const int* order, const double* u
 hipError_t launch_fill(double* x, double v, long long m, hipStream_t st);
 hipError_t launch_fill_f(float* x, float v, long long m, hipStream_t st);
 hipError_t launch_copy(const double* x, double* y, long long m, hipStream_t st);
+hipError_t launch_scatter_blocks(long long nblk, int bs, const unsigned* map, const double* src, double* dst,
+                                 hipStream_t st);
 hipError_t launch_scal(double* x, double a, long long m, hipStream_t st);
 hipError_t launch_axpy(double a, const double* x, double* y, long long m, hipStream_t st);
 hipError_t launch_to_f32(const double* x, float* y, long long m, hipStream_t st);

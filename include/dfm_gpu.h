@@ -133,13 +133,21 @@ int dfm_sys_matvec(dfm_sys sys, double alpha, int iv1, double beta, int iv2);
 int dfm_sys_get_vector(dfm_sys sys, int iv, double* out);
 int dfm_sys_set_vector(dfm_sys sys, int iv, const double* in);
 
-/* ---- ILU(0) preconditioner (CPreconditioner_ILU, src/ls/preconditioner.cpp:57-335) ------------------- */
+/* ---- ILU preconditioner (CPreconditioner_ILU, src/ls/preconditioner.cpp:57-335) ---------------------- */
+/* Level of fill (CPreconditioner_ILU::SetFillInLevel, include/delfem/ls/preconditioner.h:77, symbolic part
+ * SetLinearSystem, src/ls/preconditioner.cpp:57-211 -> CMatDiaFrac_BlkCrs::AddFracPtn,
+ * src/matvec/matdiafrac_blkcrs.cpp:1254-1381): 0 ILU(0) (the default), k > 0 ILU(k), -1 no limit. The
+ * level-of-fill pattern is built for the current ordering (the reference builds it in the natural order; under
+ * an ordering it is that of the renumbered matrix); the factor must be computed again. */
+int dfm_prec_set_fill_level(dfm_sys sys, int lev);
+/* Number of off-diagonal blocks of the factor pattern (the matrix's for ILU(0)). */
+int dfm_prec_factor_size(dfm_sys sys, long long* fncrs);
 /* Numeric factorisation of the assembled matrix (SetValue -> DoILUDecomp, matdiafrac_blkcrs.cpp:774-1107).
  * Returns DFM_ERR_SINGULAR when more than 10 pivots were singular (the reference returns false). */
 int dfm_prec_factor(dfm_sys sys);
 /* In-place z = M^-1 z on handle iv (SolvePrecond). */
 int dfm_prec_apply(dfm_sys sys, int iv);
-/* The factor in the device numbering: frowptr[n+1], fcolind[ncrs], diaind[n], fcrs, fdia (L unscaled,
+/* The factor in the device numbering: frowptr[n+1], fcolind[fncrs], diaind[n], fcrs, fdia (L unscaled,
  * D^-1, D^-1 U), for comparison with CMatDiaFrac_BlkCrs under the same ordering. */
 int dfm_prec_get_factor(dfm_sys sys, unsigned* frowptr, unsigned* fcolind, unsigned* diaind, double* fcrs,
                         double* fdia);
