@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdfm_gpu.
 
 POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D = range(6)
 ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKCOLOR = range(3)
-FP64, MIXED, FP32_ILU = 0, 1, 2
+FP64, MIXED, FP32_ILU, MIXED_AUTO = 0, 1, 2, 3
 ERR_SINGULAR = -5
 NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3}
 

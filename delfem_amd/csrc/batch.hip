@@ -324,7 +324,7 @@ struct BatchSeg {
 
 struct Batch {
   Ctx* ctx = nullptr;
-  int prec = DFM_MIXED;  // mode of added systems
+  int prec = DFM_MIXED;  // mode of added systems (DFM_MIXED_AUTO: by block size)
   int cap = 0;
   DBuf<double> d_sc;  // kNScal per slot
   DBuf<int> d_fl;     // 8 per slot
@@ -593,8 +593,8 @@ extern "C" {
 
 int dfm_batch_create(dfm_ctx ctx, int precision, int max_systems, dfm_batch* out) {
   DFM_CHECK(ctx && out, DFM_ERR_ARG, "dfm_batch_create: null argument");
-  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU, DFM_ERR_ARG,
-            "dfm_batch_create: bad precision");
+  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU ||
+                precision == DFM_MIXED_AUTO, DFM_ERR_ARG, "dfm_batch_create: bad precision");
   DFM_CHECK(max_systems > 0 && max_systems <= (1 << 20), DFM_ERR_ARG, "dfm_batch_create: bad max_systems");
   DFM_HIP(hipSetDevice(ctx->c.device));
   auto* h = new dfm_batch_s;
@@ -643,7 +643,7 @@ int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
   DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_batch_add: the system is already in a batch");
   DFM_CHECK(!bt.free_slots.empty(), DFM_ERR_STATE, "dfm_batch_add: batch full");
   DFM_CHECK(max_iter > 0, DFM_ERR_ARG, "dfm_batch_add: max_iter must be positive");
-  const int prec = bt.prec;
+  const int prec = resolve_precision(s, bt.prec);
   int rc = DFM_OK;
   if (s.prepared != prec) {
     rc = sys_solve_setup(s, prec);
@@ -655,7 +655,7 @@ int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
   BatchSeg sg;
   sg.s = &s;
   sg.slot = bt.free_slots.back();
-  sg.mode = bt.prec;
+  sg.mode = prec;
   sg.v = pcg_bufs(s, prec == DFM_MIXED);
   sg.v.f32ilu = prec == DFM_FP32_ILU;
   sg.tol = tol;

@@ -845,7 +845,13 @@ static int ensure_fp32(Sys& s) {
 }  // extern "C"
 
 namespace dfm {
+int resolve_precision(const Sys& s, int precision) {
+  if (precision == DFM_MIXED_AUTO) return s.len == 1 ? DFM_MIXED : DFM_FP32_ILU;
+  return precision;
+}
+
 int sys_solve_setup(Sys& s, int precision) {
+  precision = resolve_precision(s, precision);
   DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU, DFM_ERR_ARG,
             "solve: bad precision");
   DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "solve: assemble and factor first");
@@ -869,6 +875,7 @@ int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iter
   NEED(iteration, "dfm_solve_pcg");
   Sys& s = sys->s;
   DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_solve_pcg: the system is being solved in a batch");
+  precision = resolve_precision(s, precision);
   const int rc = sys_solve_setup(s, precision);
   if (rc != DFM_OK) return rc;
   s.prepared = -1;  // consumed by this solve

@@ -191,6 +191,8 @@ struct Sys {
   ~Sys();
 };
 
+// The precision a solve of this system runs in: DFM_MIXED_AUTO resolved by block size (capi.cpp).
+int resolve_precision(const Sys& s, int precision);
 // Buffers, FP32 copies and the right-hand side copy for a solve of the given precision (capi.cpp).
 int sys_solve_setup(Sys& s, int precision);
 // Take a system out of the batch it is solved in (batch.hip).

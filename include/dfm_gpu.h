@@ -50,10 +50,13 @@ enum {
 };
 
 /* precision of the device Krylov solvers: DFM_FP64 the reference algorithm in FP64; DFM_MIXED FP32 matrix,
- * factor and Krylov vectors with FP64 dots, solution and true residual (reliable updates), falling back to
- * DFM_FP32_ILU when its true residual stalls; DFM_FP32_ILU the FP64 loop with the FP32 copy of the factor as
- * preconditioner (attainable accuracy of FP64, half the factor bytes). */
-enum { DFM_FP64 = 0, DFM_MIXED = 1, DFM_FP32_ILU = 2 };
+ * factor and Krylov vectors with FP64 dots, solution and true residual (reliable updates), continuing as
+ * DFM_FP32_ILU from its FP64 solution when its reliable-update cycles make slow progress; DFM_FP32_ILU the FP64
+ * loop with the FP32 copy of the factor as preconditioner (attainable accuracy of FP64, half the factor bytes);
+ * DFM_MIXED_AUTO per system: DFM_MIXED for scalar systems (one dof per node), DFM_FP32_ILU for vector systems,
+ * where the reliable-update cycles of the FP32 matrix were measured to need about 1.4x the FP64 iterations on
+ * slender 2D elasticity. */
+enum { DFM_FP64 = 0, DFM_MIXED = 1, DFM_FP32_ILU = 2, DFM_MIXED_AUTO = 3 };
 
 const char* dfm_last_error(void);
 int dfm_device_count(int* count);

@@ -44,6 +44,29 @@ def test_fp32_ilu_mode(ctx, case):
     assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b)
 
 
+def test_mixed_auto_resolves_by_block_size(ctx):
+    """DFM_MIXED_AUTO: a scalar system solves as DFM_MIXED, a vector system as DFM_FP32_ILU, in a single solve
+    and in a batch (bit-identical results)."""
+    for case, want in [(KINDS[0], N.MIXED), (KINDS[3], N.FP32_ILU)]:
+        cpu, gs, order, _ = setup(N, ctx, case, N.ORDER_BLOCKCOLOR, small=False)
+        gs.factor()
+        ref = gs.solve_pcg(1e-8, 3000, want)
+        xr = gs.get_vector(-2)
+        gs.set_vector(-1, cpu.res)
+        got = gs.solve_pcg(1e-8, 3000, N.MIXED_AUTO)
+        assert got == ref and np.array_equal(gs.get_vector(-2), xr)
+        res = []
+        for prec in (want, N.MIXED_AUTO):
+            gs.set_vector(-1, cpu.res)
+            b = N.Batch(ctx, prec, 2)
+            b.add(gs, 1e-8, 3000)
+            while b.step(16):
+                pass
+            res.append((b.status(gs)[:3], gs.get_vector(-2)))
+            b.close()
+        assert res[0][0] == res[1][0] and np.array_equal(res[0][1], res[1][1])
+
+
 def _strip(ctx):
     w = W.make("c1")
     s = W.build_system(ctx, w, N.ORDER_BLOCKCOLOR)
