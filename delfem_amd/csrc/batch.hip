@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "dev_common.hpp"
@@ -32,12 +33,8 @@ constexpr int DV = NT * VPT;      // dofs per vector block
 
 struct SegDev {
   int n, m;
-  const unsigned* rp;
-  const unsigned* ci;
-  const void* crs;  // matrix of the Krylov SpMV (FP32 in mixed mode)
-  const void* dia;
-  const double* crs64;  // FP64 matrix (true residual, mixed mode)
-  const double* dia64;
+  const void* dia;      // diagonal blocks of the Krylov SpMV (FP32 in mixed mode)
+  const double* dia64;  // FP64 diagonal blocks (true residual, mixed mode)
   SlabArgs fa, ba;
   const void* fval;
   const void* fdval;
@@ -49,12 +46,14 @@ struct SegDev {
   int* fl;
   int ps0, ps1;  // SpMV blocks of the segment
   int pv0, pv1;  // vector blocks of the segment
+  int nslice;           // SELL-64 copy of the matrix: slices, slab rows per slice, columns, values
+  const int* sptr;
+  const unsigned* scol;
+  const void* sval;     // values of the Krylov SpMV (FP32 in mixed mode)
+  const double* sval64; // FP64 values (true residual, mixed mode)
 };
 
-template <int B>
-constexpr int tpr_of() {
-  return B == 3 ? 16 : 8;
-}
+constexpr int kBSellWaves = 16;  // SELL slices (one per wavefront) per SpMV workgroup
 
 // scalar slots of the batch loop beyond those of pcg_start (rho stays in S_RZ, the reliable-update maximum in
 // S_RMAX: only the per-segment reduction kernels write them, one workgroup per segment)
@@ -73,47 +72,55 @@ __device__ inline void produce_at(double* slot, int idx, double v) {
   if (threadIdx.x == 0) slot[idx] = t[0];
 }
 
-// q = A p, partials of (p, q); mixed, when a reliable update is due: r = b - A64 x64 with its partials
-template <int B, int TPR, typename T, bool REL>
-__global__ void __launch_bounds__(NT) k_b_spmv(const SegDev* __restrict__ segs, const int* __restrict__ bseg,
-                                               double* slot_pq, double* slot_true) {
+// q = A p, partials of (p, q); mixed, when a reliable update is due: r = b - A64 x64 with its partials. Over the
+// SELL-64 copy of each system: one slice (64 rows, one per lane) per wavefront, row sums in CRS order with the
+// diagonal block last (MatVec, src/matvec/matdia_blkcrs.cpp:921-1094). TM: matrix values, T: vectors.
+template <int B, typename TM, typename T, bool REL>
+__global__ void __launch_bounds__(64 * kBSellWaves) k_b_spmv(const SegDev* __restrict__ segs,
+                                                             const int* __restrict__ bseg, double* slot_pq,
+                                                             double* slot_true) {
+  constexpr int CH = SellChunk<B>::CH;
   const int b = blockIdx.x;
   const SegDev& d = segs[bseg[b]];
   const int* fl = d.fl;
   if (fl[F_DONE]) return;
   const bool rel = REL && fl[F_PCUR];
-  constexpr int RS = NT / TPR;
-  const int lane = threadIdx.x % TPR;
-  const int row = (b - d.ps0) * RS + (int)threadIdx.x / TPR;
+  const int wave = wave_in_block(), lane = (int)(threadIdx.x % 64);
+  const int sl = (b - d.ps0) * kBSellWaves + wave;
   const T* p = (const T*)d.p;
-  double v = 0.0, vt = 0.0;
-  if (row < d.n) {
+  double v[2] = {0.0, 0.0};
+  if (sl < d.nslice) {
+    const int r0 = d.sptr[sl], w = d.sptr[sl + 1] - r0;
+    const int row = sl * 64 + lane;
     T acc[B];
-    spmv_row<B, TPR, T, T, T>(d.rp[row], d.rp[row + 1], d.ci, (const T*)d.crs, p, lane, acc);
-    if (lane == 0) {
-      dia_row<B, T, T, T>((const T*)d.dia, p, row, acc);
+    sell_row<B, CH, TM, T, T>(d.scol, (const TM*)d.sval, r0, w, lane, p, acc);
+    if (row < d.n) {
+      dia_row<B, TM, T, T>((const TM*)d.dia, p, row, acc);
 #pragma unroll
       for (int k = 0; k < B; k++) {
         ((T*)d.q)[(size_t)row * B + k] = acc[k];
-        v += (double)p[(size_t)row * B + k] * (double)acc[k];
+        v[0] += (double)p[(size_t)row * B + k] * (double)acc[k];
       }
     }
     if (REL && rel) {
       double a64[B];
-      spmv_row<B, TPR, double, double, double>(d.rp[row], d.rp[row + 1], d.ci, d.crs64, d.x64, lane, a64);
-      if (lane == 0) {
+      sell_row<B, CH, double, double, double>(d.scol, d.sval64, r0, w, lane, d.x64, a64);
+      if (row < d.n) {
         dia_row<B, double, double, double>(d.dia64, d.x64, row, a64);
 #pragma unroll
         for (int k = 0; k < B; k++) {
           const double t = d.b64[(size_t)row * B + k] - a64[k];
           ((T*)d.r)[(size_t)row * B + k] = (T)t;
-          vt += t * t;
+          v[1] += t * t;
         }
       }
     }
   }
-  produce_at(slot_pq, b, v);
-  if (REL && rel) produce_at(slot_true, b, vt);
+  block_allsum<64 * kBSellWaves, 2>(v);
+  if (threadIdx.x == 0) {
+    slot_pq[b] = v[0];
+    if (REL && rel) slot_true[b] = v[1];
+  }
 }
 
 // per segment: iteration counter, alpha = rho / (p, q)   (solver_ls_iter.cpp:137-138)
@@ -376,8 +383,6 @@ static int rebuild(Batch& bt) {
     Batch::Group& g = bt.grp[gi];
     g.B = gi % 3 + 1;
     g.mode = gi / 3;
-    const int tpr = g.B == 3 ? tpr_of<3>() : tpr_of<1>();
-    const int rs = NT / tpr;
     std::vector<SegDev> h;
     std::vector<int> sblk, vblk;
     std::vector<std::vector<int2>> fph, bph;
@@ -389,11 +394,7 @@ static int rebuild(Batch& bt) {
       const int si = (int)h.size();
       d.n = s.n;
       d.m = s.n * s.len;
-      d.rp = s.d_rowptr.p;
-      d.ci = s.d_colind.p;
-      d.crs = sg.v.crs;
       d.dia = sg.v.dia;
-      d.crs64 = s.d_crs.p;
       d.dia64 = s.d_dia.p;
       d.fa = SlabArgs{s.fwd.d_gmeta.p, s.fwd.d_srow.p, s.fwd.d_scnt.p, s.fwd.d_ecol.p};
       d.ba = SlabArgs{s.bwd.d_gmeta.p, s.bwd.d_srow.p, s.bwd.d_scnt.p, s.bwd.d_ecol.p};
@@ -410,8 +411,13 @@ static int rebuild(Batch& bt) {
       d.b64 = sg.v.b64;
       d.sc = bt.d_sc.p + kNScal * (size_t)sg.slot;
       d.fl = bt.d_fl.p + 8 * (size_t)sg.slot;
+      d.nslice = s.sell.nslice();
+      d.sptr = s.sell.d_sptr.p;
+      d.scol = s.sell.d_col.p;
+      d.sval = sg.mode == DFM_MIXED ? (const void*)s.sell.d_val32.p : (const void*)s.sell.d_val.p;
+      d.sval64 = s.sell.d_val.p;
       d.ps0 = (int)sblk.size();
-      for (int r = 0; r < s.n; r += rs) sblk.push_back(si);
+      for (int sl = 0; sl < d.nslice; sl += kBSellWaves) sblk.push_back(si);
       d.ps1 = (int)sblk.size();
       d.pv0 = (int)vblk.size();
       for (long long i = 0; i < (long long)d.m; i += DV) vblk.push_back(si);
@@ -479,10 +485,12 @@ static int rebuild(Batch& bt) {
 // T: vectors; MX: reliable-update loop; TF: factor values
 template <int B, typename T, bool MX, typename TF>
 static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
-  constexpr int TPR = tpr_of<B>();
+  using TM = std::conditional_t<MX, float, double>;
   const SegDev* segs = g.d_segs.p;
   const dim3 nseg((unsigned)g.members.size());
-  hipLaunchKernelGGL((k_b_spmv<B, TPR, T, MX>), dim3(g.nsb), dim3(NT), 0, st, segs, g.d_sblk.p, g.d_pq.p, g.d_true.p);
+  if (g.nsb)
+    hipLaunchKernelGGL((k_b_spmv<B, TM, T, MX>), dim3(g.nsb), dim3(64 * kBSellWaves), 0, st, segs, g.d_sblk.p,
+                       g.d_pq.p, g.d_true.p);
   hipLaunchKernelGGL(k_b_alpha, nseg, dim3(NT), 0, st, segs, (const double*)g.d_pq.p);
   hipLaunchKernelGGL((k_b_update<T>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p, g.d_rr.p);
   hipLaunchKernelGGL((k_b_judge<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p, (const double*)g.d_true.p);
