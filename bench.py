@@ -8,10 +8,12 @@ assembly -> Dirichlet rows/columns -> ILU(0) numeric factorisation -> PCG to a 1
 `value` is the mixed-precision path (FP32 matrix/factor/SpMV/ILU inside FP64 iterative refinement on the
 true residual); the plain FP64 path is timed beside it on the same system.
 
-The roofline object is for the dominant kernel family of the mixed path's inner loop, the ILU(0) phase
-sweeps, and `roofline_spmv` for the SELL SpMV, both timed with HIP events on the library's stream; `traffic` is
-the HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/c2_hbm_traffic.md). The CPU baseline is the oracle's single-threaded C
-restatement of the reference, timed on a bounded sample of the same workload (see its `sample` field).
+The roofline object is for the dominant kernel family of the mixed path's inner loop, the ILU(0) sweep phases
+as the PCG iteration runs them (the r/x update folded into the forward phases, (r, z) into the backward ones),
+and `roofline_spmv` for the SELL SpMV, both timed with HIP events on the library's stream; `traffic` is the HBM
+bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/c2_hbm_traffic.md). The CPU baseline
+is the oracle's single-threaded C restatement of the reference, timed on a bounded sample of the same workload
+(see its `sample` field).
 """
 import argparse
 import json
@@ -367,15 +369,21 @@ def main():
     spmv64_ms, spmv64_b = s.bench_kernel(0, 50)
     ilu32_ms, ilu32_b = s.bench_kernel(3, 20)
     ilu64_ms, ilu64_b = s.bench_kernel(2, 20)
+    # the sweep phases as the PCG iteration runs them (r/x update folded into the forward phases, (r, z) into
+    # the backward ones): the dominant kernels of the timed loop
+    sw32_ms, sw32_b = s.bench_kernel(4, 20)
+    sw64_ms, sw64_b = s.bench_kernel(5, 20)
     achieved = spmv32_b / (spmv32_ms * 1e-3) / 1e9
     log(f"[bench] spmv fp32 {spmv32_ms * 1e3:.1f} us ({achieved:.0f} GB/s), spmv fp64 {spmv64_ms * 1e3:.1f} us "
         f"({spmv64_b / spmv64_ms / 1e6:.0f} GB/s), ilu fp32 {ilu32_ms * 1e3:.1f} us "
-        f"({ilu32_b / ilu32_ms / 1e6:.0f} GB/s), ilu fp64 {ilu64_ms * 1e3:.1f} us ({ilu64_b / ilu64_ms / 1e6:.0f} GB/s)")
+        f"({ilu32_b / ilu32_ms / 1e6:.0f} GB/s), ilu fp64 {ilu64_ms * 1e3:.1f} us ({ilu64_b / ilu64_ms / 1e6:.0f} GB/s), "
+        f"pcg sweep phases fp32 {sw32_ms * 1e3:.1f} us ({sw32_b / sw32_ms / 1e6:.0f} GB/s), "
+        f"fp64 {sw64_ms * 1e3:.1f} us ({sw64_b / sw64_ms / 1e6:.0f} GB/s)")
 
     nlaunch = nph[0] + nph[1]
     sweep_traffic = spmv_traffic = None
     if args.config == "c2" and args.ordering == "blockcolor":
-        sweep_traffic = measured_traffic(["dfm::k_sweep_fwd<1, float", "dfm::k_sweep_bwd<1, float"])
+        sweep_traffic = measured_traffic(["dfm::k_fwd_update<1, 4, 4, float", "dfm::k_bwd_rz<1, 4, 4, float"])
         spmv_traffic = measured_traffic(["dfm::k_pcg_spmv_sell<1, float, false>"])
     mixed, fp64 = res["mixed"], res["fp64"]
     K = args.steps
@@ -441,19 +449,20 @@ def main():
         "ms_per_pcg_iter_fp64": fp64["stats"]["ms_per_iter"],
         "solve_ms_mixed": mixed["stats"]["ms"],
         "solve_ms_fp64": fp64["stats"]["ms"],
-        # dominant kernel family of the mixed inner loop: the ILU(0) sweep phases (one launch per colour and
-        # direction); per launch = one apply's algorithmic bytes and HIP-event time over its launches
+        # dominant kernel family of the mixed inner loop: the sweep phases (one launch per phase and direction,
+        # forward phases with the r/x update, backward phases with (r, z)); per launch = one iteration's
+        # algorithmic bytes of the phases and HIP-event time, over its launches
         "roofline": {
-            "kernel": "k_sweep_fwd/k_sweep_bwd<B=%d, FP32> (ILU(0) phase sweeps, %d launches per apply)"
+            "kernel": "k_fwd_update/k_bwd_rz<B=%d, FP32> (ILU(0) phase sweeps of the PCG iteration, %d launches)"
                       % (w["len"], nlaunch),
             "bound": "hbm",
-            "achieved": ilu32_b / (ilu32_ms * 1e-3) / 1e9,
+            "achieved": sw32_b / (sw32_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": ilu32_b / (ilu32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "frac": sw32_b / (sw32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic": sweep_traffic,
-            "avg_launch_us": ilu32_ms * 1e3 / nlaunch,
-            "algorithmic_bytes": ilu32_b / nlaunch,
+            "avg_launch_us": sw32_ms * 1e3 / nlaunch,
+            "algorithmic_bytes": sw32_b / nlaunch,
         },
         "roofline_spmv": {
             "kernel": "k_pcg_spmv_sell<B=%d, FP32> (SELL-64 SpMV + dot)" % w["len"],
@@ -470,6 +479,7 @@ def main():
             "spmv_fp64": {"us": spmv64_ms * 1e3, "GB/s": spmv64_b / spmv64_ms / 1e6, "bytes": spmv64_b},
             "ilu_apply_fp32": {"us": ilu32_ms * 1e3, "GB/s": ilu32_b / ilu32_ms / 1e6, "bytes": ilu32_b},
             "ilu_apply_fp64": {"us": ilu64_ms * 1e3, "GB/s": ilu64_b / ilu64_ms / 1e6, "bytes": ilu64_b},
+            "pcg_sweep_phases_fp64": {"us": sw64_ms * 1e3, "GB/s": sw64_b / sw64_ms / 1e6, "bytes": sw64_b},
         },
     }
     s.close()

@@ -917,11 +917,11 @@ int dfm_sys_true_residual(dfm_sys sys, double* rel) {
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes) {
   NEED(sys, "dfm_sys_bench_kernel");
   Sys& s = sys->s;
-  DFM_CHECK(which >= 0 && which <= 3 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
+  DFM_CHECK(which >= 0 && which <= 5 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
   DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble and factor first");
   int rc = ensure_work(s, 2, 5);
   if (rc != DFM_OK) return rc;
-  if (which == 1 || which == 3) {
+  if (which == 1 || which == 3 || which == 4) {
     rc = ensure_fp32(s);
     if (rc != DFM_OK) return rc;
   }

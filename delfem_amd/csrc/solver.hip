@@ -1067,12 +1067,24 @@ int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes) {
   float* y32 = s.d_work32.p + m;
   hipLaunchKernelGGL(k_mx_to32, vgrid(m), dim3(NT), 0, st, m, s.d_rhs.p, x32);
   unsigned G = 0;
+  // 4, 5: the sweep phases of the PCG iteration (forward phases with the r/x update and (r, r), backward phases
+  // with (r, z)) in the mixed (FP32) and FP64 loops, on the solve's own buffers (alpha = 0 after the reset above:
+  // r and x keep their values)
+  const PcgBufs v32 = pcg_bufs(s, true), v64 = pcg_bufs(s, false);
+  const size_t need = (size_t)(sweep_blocks(s.fwd) + sweep_blocks(s.bwd));
+  if ((which == 4 || which == 5) && s.d_part_sw.n < need) DFM_HIP(s.d_part_sw.alloc(need));
+  double* prr = s.d_part_sw.p;
+  double* prz = s.d_part_sw.p + sweep_blocks(s.fwd);
   auto launch = [&]() -> hipError_t {
     switch (which) {
       case 0: return pcg_spmv<double, false>(s, s.d_crs.p, s.d_dia.p, s.d_rhs.p, s.d_work.p, slot(s, P_A), G, nullptr, nullptr, nullptr, nullptr, st);
       case 1: return pcg_spmv<float, false>(s, s.d_crs32.p, s.d_dia32.p, x32, y32, slot(s, P_A), G, nullptr, nullptr, nullptr, nullptr, st);
       case 2: return precond_apply64(s, s.d_rhs.p, s.d_work.p, fl, st);
-      default: return precond_apply32(s, x32, y32, fl, st);
+      case 3: return precond_apply32(s, x32, y32, fl, st);
+      case 4:
+        return fused_sweeps<float, float>(s, v32, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, prr, prz, st);
+      default:
+        return fused_sweeps<double, double>(s, v64, s.fwd.d_eval.p, s.fwd.d_dval.p, s.bwd.d_eval.p, prr, prz, st);
     }
   };
   for (int w = 0; w < 3; w++)
@@ -1090,11 +1102,17 @@ int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (avg_ms) *avg_ms = ms / reps;
-  const double sv = (which == 0 || which == 2) ? 8.0 : 4.0;
+  // algorithmic bytes: matrix or factor values and column indices once, plus the vectors each pass must move
+  // (SpMV: x read, y written; apply: b read, z written forward, z read and written backward; fused phases: r, q,
+  // p, x read, r, x, z written forward, z read and written and r read backward)
+  const double sv = (which == 0 || which == 2 || which == 5) ? 8.0 : 4.0;
   const double sx = sv;
   const double b2 = (double)s.bs;
-  const double mat = ((double)s.ncrs + s.n) * b2 * sv + 4.0 * s.ncrs + 4.0 * (s.n + 1);
-  if (bytes) *bytes = mat + ((which < 2) ? 2.0 : 4.0) * (double)m * sx;
+  const double fn = (double)s.fcolind.size();
+  const double mat = which < 2 ? ((double)s.ncrs + s.n) * b2 * sv + 4.0 * s.ncrs + 4.0 * (s.n + 1)
+                               : (fn + s.n) * b2 * sv + 4.0 * fn + 4.0 * (s.n + 1);
+  const double nvec = which < 2 ? 2.0 : which < 4 ? 4.0 : 10.0;
+  if (bytes) *bytes = mat + nvec * (double)m * sx;
   return DFM_OK;
 }
 

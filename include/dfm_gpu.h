@@ -182,8 +182,10 @@ int dfm_sys_last_solve_stats(dfm_sys sys, double* ms_total, double* ms_per_iter,
  * solve was repeated in FP64. */
 int dfm_sys_last_solve_info(dfm_sys sys, int* reliable_updates, int* fell_back);
 /* Average device time of one launch of a hot kernel over `reps` back-to-back launches, timed with HIP events
- * on the system's own stream: which = 0 SpMV FP64 (y = A x), 1 SpMV FP32, 2 ILU(0) apply FP64 (all phases of
- * the forward and backward sweeps), 3 ILU(0) apply FP32. Also reports the algorithmic bytes of one launch. */
+ * on the system's own stream: which = 0 SpMV FP64 (y = A x), 1 SpMV FP32, 2 ILU apply FP64 (all phases of
+ * the forward and backward sweeps), 3 ILU apply FP32, 4 / 5 the sweep phases of a PCG iteration as the mixed
+ * (FP32) / FP64 loop runs them (forward phases with the r and x update, backward phases with (r, z)). Also
+ * reports the algorithmic bytes of one launch (all phases). No counterpart in the reference. */
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes);
 /* ---- batches: many systems solved together (serving, instance batches) ------------------------------
  * A batch advances every system added to it with one launch per kernel for all running systems of one block
