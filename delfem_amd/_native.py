@@ -189,9 +189,10 @@ class System:
         _check(lib().dfm_sys_set_ordering(self.h, _p(_i32(order))))
 
     def schedule_info(self):
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
-        _check(lib().dfm_sys_schedule_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
-        return a.value, b.value, c.value
+        """(forward phases, backward phases, most levels in a group, sweep launches per PCG iteration)"""
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().dfm_sys_schedule_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     def set_bc(self, flags):
         _check(lib().dfm_sys_set_bc(self.h, _p(_i32(flags))))

@@ -81,6 +81,7 @@ static int upload_sweeps(Sys& s) {
     DFM_HIP(upload(w->d_scnt, w->scnt, st));
     DFM_HIP(upload(w->d_ecol, w->ecol, st));
     DFM_HIP(upload(w->d_esrc, w->esrc, st));
+    DFM_HIP(upload(w->d_fuse_bgrp, w->fuse_bgrp, st));
   }
   return DFM_OK;
 }
@@ -264,6 +265,7 @@ static void copy_sweep_host(Sweep& d, const Sweep& a) {
   d.phase_gptr = a.phase_gptr; d.phase_nlev = a.phase_nlev; d.grp = a.grp; d.rowlist = a.rowlist; d.lev = a.lev;
   d.phase_wo = a.phase_wo; d.phase_wi = a.phase_wi;
   d.gmeta = a.gmeta; d.srow = a.srow; d.scnt = a.scnt; d.ecol = a.ecol; d.esrc = a.esrc; d.nslab = a.nslab;
+  d.fuse_bgrp = a.fuse_bgrp;
 }
 
 int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
@@ -395,11 +397,13 @@ int dfm_sys_set_ordering(dfm_sys sys, const int* order) {
   return DFM_OK;
 }
 
-int dfm_sys_schedule_info(dfm_sys sys, int* nphase_fwd, int* nphase_bwd, int* max_group_levels) {
+int dfm_sys_schedule_info(dfm_sys sys, int* nphase_fwd, int* nphase_bwd, int* max_group_levels,
+                          int* pcg_sweep_launches) {
   NEED(sys, "dfm_sys_schedule_info");
   const Sys& s = sys->s;
   if (nphase_fwd) *nphase_fwd = s.fwd.nphase();
   if (nphase_bwd) *nphase_bwd = s.bwd.nphase();
+  if (pcg_sweep_launches) *pcg_sweep_launches = s.fwd.nphase() + s.bwd.nphase() - (s.fwd.fuse_bgrp.empty() ? 0 : 1);
   if (max_group_levels) {
     int mx = 0;
     for (int v : s.fwd.phase_nlev) mx = std::max(mx, v);

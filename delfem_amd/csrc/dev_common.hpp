@@ -432,17 +432,17 @@ __device__ inline void sweep_fwd_wave(const SlabArgs& sa, int g, const T* __rest
 }
 
 // Backward sweep of group g by the calling wavefront, in place on x: x_i -= U_in x_in (in-group, ascending) then
-// U_out x_out. On return t holds the lane's result (for lanes with a row). Returns false when the done flag is set.
-template <int B, int MO, int MI, typename T, typename A, typename VX>
+// U_out x_out. `init(ss, t)` fills t with the lane's starting value (x of its row, or values the caller holds).
+// On return t holds the lane's result (for lanes with a row). Returns false when the done flag is set.
+template <int B, int MO, int MI, typename T, typename A, typename VX, typename Init>
 __device__ inline bool sweep_bwd_core(const SlabArgs& sa, int g, const T* __restrict__ val, VX* x,
-                                      const int* __restrict__ done, A* xs, SlabSlot& ss, A (&t)[B]) {
+                                      const int* __restrict__ done, A* xs, SlabSlot& ss, A (&t)[B], Init&& init) {
   constexpr int BS = B * B;
   if (!slab_slot(sa, g, done, ss)) return false;
   T vo[MO][BS], vi[MI][BS];
   A xo[MO][B];
   unsigned jo[MO], li[MI];
-#pragma unroll
-  for (int r = 0; r < B; r++) t[r] = (A)x[(size_t)ss.row * B + r];
+  init(ss, t);
   slab_load_class<B, MO, T, A>(sa.ecol, val, ss.off, ss.wo, ss.lane, jo, vo);
   slab_load_class<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.lane, li, vi);
   slab_gather<B, MO, A, VX>(x, ss.wo, jo, xo);
@@ -469,7 +469,10 @@ __device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __rest
                                       const int* __restrict__ done, A* xs) {
   SlabSlot ss;
   A t[B];
-  sweep_bwd_core<B, MO, MI, T, A, VX>(sa, g, val, x, done, xs, ss, t);
+  sweep_bwd_core<B, MO, MI, T, A, VX>(sa, g, val, x, done, xs, ss, t, [&](const SlabSlot& s, A(&u)[B]) {
+#pragma unroll
+    for (int r = 0; r < B; r++) u[r] = (A)x[(size_t)s.row * B + r];
+  });
 }
 
 // One phase of one system: groups [gbase, gend), kWPB groups per workgroup.

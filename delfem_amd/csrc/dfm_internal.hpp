@@ -93,10 +93,14 @@ struct Sweep {
   DBuf<unsigned> d_scnt, d_ecol, d_esrc;
   DBuf<double> d_eval, d_dval;   // FP64 values (d_dval: forward sweep only, D^-1 per slot)
   DBuf<float> d_eval32, d_dval32;
+  // forward sweep only: when the last forward phase and the first backward phase hold the same tiles, the
+  // backward group of each group of the last forward phase (in its order); empty otherwise
+  std::vector<int> fuse_bgrp;
+  DBuf<int> d_fuse_bgrp;
   void release_dev() {
     d_grp.release(); d_rowlist.release(); d_lev.release();
     d_gmeta.release(); d_srow.release(); d_scnt.release(); d_ecol.release(); d_esrc.release();
-    d_eval.release(); d_dval.release(); d_eval32.release(); d_dval32.release();
+    d_eval.release(); d_dval.release(); d_eval32.release(); d_dval32.release(); d_fuse_bgrp.release();
   }
 };
 

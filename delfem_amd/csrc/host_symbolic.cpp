@@ -377,6 +377,34 @@ static void build_sell(Sys& s) {
   }
 }
 
+// The last forward phase and the first backward phase may hold the same tiles (ILU(0) of a colour-major
+// ordering: the last colour, which no later row needs before the backward sweep and which needs no later row):
+// then one wavefront can run a tile's forward sweep and, with the values still in registers, its backward sweep.
+static void fuse_last_first(Sys& s) {
+  Sweep& f = s.fwd;
+  const Sweep& b = s.bwd;
+  f.fuse_bgrp.clear();
+  if (f.nphase() == 0 || b.nphase() == 0) return;
+  const int f0 = f.phase_gptr[f.nphase() - 1], f1 = f.phase_gptr[f.nphase()];
+  const int b0 = b.phase_gptr[0], b1 = b.phase_gptr[1];
+  if (f1 - f0 != b1 - b0) return;
+  std::vector<std::pair<int, int>> byrow;  // (first row, backward group)
+  for (int g = b0; g < b1; g++) {
+    if (b.grp[2 * g + 1] - b.grp[2 * g] < 1) return;
+    byrow.push_back({b.grp[2 * g], g});
+  }
+  std::sort(byrow.begin(), byrow.end());
+  std::vector<int> map;
+  for (int g = f0; g < f1; g++) {
+    auto it = std::lower_bound(byrow.begin(), byrow.end(), std::make_pair(f.grp[2 * g], -1));
+    if (it == byrow.end() || it->first != f.grp[2 * g]) return;
+    const int bg = it->second;
+    if (b.grp[2 * bg + 1] != f.grp[2 * g + 1]) return;
+    map.push_back(bg);
+  }
+  f.fuse_bgrp.swap(map);
+}
+
 void build_schedule(Sys& s) {
   build_sell(s);
   build_sweeps(s);
@@ -388,6 +416,7 @@ void build_sweeps(Sys& s) {
     level_sweeps(s);
     build_slabs(s, s.fwd, true);
     build_slabs(s, s.bwd, false);
+    s.fwd.fuse_bgrp.clear();
     return;
   }
   // contiguous groups: tiles (BMC) or 64-row chunks of each colour (MC)
@@ -406,6 +435,7 @@ void build_sweeps(Sys& s) {
   group_sweeps(s, gstart);
   build_slabs(s, s.fwd, true);
   build_slabs(s, s.bwd, false);
+  fuse_last_first(s);
 }
 
 // Greedy node colouring in natural order; order = colour-major, natural inside a colour.

@@ -275,7 +275,10 @@ __global__ void __launch_bounds__(64 * kWPB)
   if (g < gend) {
     SlabSlot ss;
     T t[B];
-    sweep_bwd_core<B, MO, MI, TV, T, T>(sa, g, val, z, nullptr, xsb[w], ss, t);
+    sweep_bwd_core<B, MO, MI, TV, T, T>(sa, g, val, z, nullptr, xsb[w], ss, t, [&](const SlabSlot& s, T(&u)[B]) {
+#pragma unroll
+      for (int k = 0; k < B; k++) u[k] = z[(size_t)s.row * B + k];
+    });
     if (ss.active)
 #pragma unroll
       for (int k = 0; k < B; k++) rz += (double)r[(size_t)ss.row * B + k] * (double)t[k];
@@ -283,6 +286,64 @@ __global__ void __launch_bounds__(64 * kWPB)
   double v[1] = {rz};
   block_allsum<64 * kWPB, 1>(v);
   if (threadIdx.x == 0) slot_rz[blockIdx.x] = v[0];
+}
+
+// The last forward phase fused with the first backward phase when they hold the same tiles: per tile, the
+// forward sweep with the r/x update, then the backward sweep starting from the forward values the lanes hold (the
+// backward groups of this phase have no entries outside their tile); (r, r) and (r, z) partials of the rows.
+template <int B, int MO, int MI, typename TV, typename T>
+__global__ void __launch_bounds__(64 * kWPB)
+    k_fwd_bwd_tile(int gbase, int gend, SlabArgs fa, const TV* __restrict__ fval, const TV* __restrict__ fdval,
+                   SlabArgs ba, const TV* __restrict__ bval, const int* __restrict__ bgrp, T* r, const T* q,
+                   const T* p, T* x, T* z, const double* sc, const int* fl, double* slot_rr, double* slot_rz) {
+  __shared__ T xsb[kWPB][64 * B];
+  if (fl[F_DONE]) return;
+  const int w = wave_in_block();
+  const int k = (int)blockIdx.x * kWPB + w;
+  const int g = gbase + k;
+  double v[2] = {0.0, 0.0};
+  if (g < gend) {
+    const double alpha = sc[S_ALPHA];
+    const T ma = (T)(-alpha), pa = (T)alpha;
+    T rn[B], xn[B];
+    SlabSlot ss;
+    T t[B];
+    sweep_fwd_core<B, MO, MI, TV, T, T>(
+        fa, g, fval, fdval,
+        [&](const SlabSlot& s, T(&u)[B]) {
+#pragma unroll
+          for (int c = 0; c < B; c++) {
+            const size_t i = (size_t)s.row * B + c;
+            rn[c] = r[i] + ma * q[i];
+            xn[c] = x[i] + pa * p[i];
+            u[c] = rn[c];
+          }
+        },
+        z, nullptr, xsb[w], ss, t);
+    if (ss.active)
+#pragma unroll
+      for (int c = 0; c < B; c++) {
+        const size_t i = (size_t)ss.row * B + c;
+        r[i] = rn[c];
+        x[i] = xn[c];
+        v[0] += (double)rn[c] * (double)rn[c];
+      }
+    wave_lds_sync();
+    SlabSlot sb;
+    T u[B];
+    sweep_bwd_core<B, MO, MI, TV, T, T>(ba, bgrp[k], bval, z, nullptr, xsb[w], sb, u, [&](const SlabSlot&, T(&a)[B]) {
+#pragma unroll
+      for (int c = 0; c < B; c++) a[c] = t[c];
+    });
+    if (sb.active)
+#pragma unroll
+      for (int c = 0; c < B; c++) v[1] += (double)rn[c] * (double)u[c];
+  }
+  block_allsum<64 * kWPB, 2>(v);
+  if (threadIdx.x == 0) {
+    slot_rr[blockIdx.x] = v[0];
+    slot_rz[blockIdx.x] = v[1];
+  }
 }
 
 // Judgement and beta in one workgroup. FP64 (solver_ls_iter.cpp:146-166): on the recursive residual. Mixed: on
@@ -630,11 +691,28 @@ static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const T
   T* z = (T*)v.z;
   const Sweep& f = s.fwd;
   const SlabArgs fa{f.d_gmeta.p, f.d_srow.p, f.d_scnt.p, f.d_ecol.p};
+  const Sweep& b = s.bwd;
+  const SlabArgs ba{b.d_gmeta.p, b.d_srow.p, b.d_scnt.p, b.d_ecol.p};
+  const bool fuse = !f.fuse_bgrp.empty();
   int off = 0;
   for (int p = 0; p < f.nphase(); p++) {
     const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0, nb = (ng + kWPB - 1) / kWPB;
     if (!ng) continue;
     const bool narrow = sizeof(T) == 8 && sizeof(TV) == 8;
+    if (fuse && p == f.nphase() - 1) {
+      const int wo = std::max(f.phase_wo[p], b.phase_wo[0]), wi = std::max(f.phase_wi[p], b.phase_wi[0]);
+#define DFM_FB(B)                                                                                                  \
+  if (s.len == B)                                                                                                  \
+    with_slab_caps(wo, wi, B == 3 && narrow, [&]<int MO, int MI>() {                                               \
+      hipLaunchKernelGGL((k_fwd_bwd_tile<B, MO, MI, TV, T>), dim3(nb), dim3(64 * kWPB), 0, st, g0, g0 + ng, fa, fval, \
+                         fdval, ba, bval, (const int*)f.d_fuse_bgrp.p, r, (const T*)v.q, (const T*)v.p, (T*)v.x, z,  \
+                         sc, fl, part_rr + off, part_rz);                                                          \
+    });
+      DFM_FB(1) DFM_FB(2) DFM_FB(3)
+#undef DFM_FB
+      off += nb;
+      continue;
+    }
 #define DFM_FU(B)                                                                                                    \
   if (s.len == B)                                                                                                    \
     with_slab_caps(f.phase_wo[p], f.phase_wi[p], B == 3 && narrow, [&]<int MO, int MI>() {                           \
@@ -645,12 +723,14 @@ static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const T
 #undef DFM_FU
     off += nb;
   }
-  const Sweep& b = s.bwd;
-  const SlabArgs ba{b.d_gmeta.p, b.d_srow.p, b.d_scnt.p, b.d_ecol.p};
   off = 0;
   for (int p = 0; p < b.nphase(); p++) {
     const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0, nb = (ng + kWPB - 1) / kWPB;
     if (!ng) continue;
+    if (fuse && p == 0) {  // done with the last forward phase
+      off += nb;
+      continue;
+    }
     const bool narrow = sizeof(T) == 8 && sizeof(TV) == 8;
 #define DFM_BZ(B)                                                                                                  \
   if (s.len == B)                                                                                                  \
@@ -720,6 +800,7 @@ static std::vector<const void*> graph_key(const Sys& s, const PcgBufs& v) {
                           (const void*)w->d_eval32.p, (const void*)w->d_dval32.p})
       k.push_back(p);
   k.push_back((const void*)(size_t)v.f32ilu);
+  k.push_back((const void*)s.fwd.d_fuse_bgrp.p);
   return k;
 }
 
@@ -1111,7 +1192,14 @@ int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes) {
   const double fn = (double)s.fcolind.size();
   const double mat = which < 2 ? ((double)s.ncrs + s.n) * b2 * sv + 4.0 * s.ncrs + 4.0 * (s.n + 1)
                                : (fn + s.n) * b2 * sv + 4.0 * fn + 4.0 * (s.n + 1);
-  const double nvec = which < 2 ? 2.0 : which < 4 ? 4.0 : 10.0;
+  double nvec = which < 2 ? 2.0 : which < 4 ? 4.0 : 10.0;
+  if (which >= 4 && !s.fwd.fuse_bgrp.empty()) {
+    // rows of the fused last/first phase: z is written once and not read back
+    const Sweep& f = s.fwd;
+    long long rows = 0;
+    for (int g = f.phase_gptr[f.nphase() - 1]; g < f.phase_gptr[f.nphase()]; g++) rows += f.grp[2 * g + 1] - f.grp[2 * g];
+    nvec -= 3.0 * (double)rows / (double)s.n;
+  }
   if (bytes) *bytes = mat + nvec * (double)m * sx;
   return DFM_OK;
 }

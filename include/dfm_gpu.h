@@ -97,8 +97,12 @@ int dfm_sys_get_pattern(dfm_sys sys, unsigned* rowptr, unsigned* colind);
 int dfm_sys_set_ordering(dfm_sys sys, const int* order);
 /* Build an ordering of the given kind (tile = nodes per tile edge for BLOCKCOLOR, 0 = default). */
 int dfm_sys_make_ordering(dfm_sys sys, int kind, int tile, int* order_out, int* ncolors_out);
-/* Number of sequential phases of the forward triangular sweep under the current ordering. */
-int dfm_sys_schedule_info(dfm_sys sys, int* nphase_fwd, int* nphase_bwd, int* max_group_levels);
+/* Sequential phases of the forward and backward triangular sweeps under the current ordering and fill level,
+ * the most intra-group levels of a phase, and the sweep launches of one PCG iteration (one fewer than the phases
+ * when the last forward and the first backward phase hold the same tiles and run as one kernel). No counterpart
+ * in the reference. */
+int dfm_sys_schedule_info(dfm_sys sys, int* nphase_fwd, int* nphase_bwd, int* max_group_levels,
+                          int* pcg_sweep_launches);
 
 /* Dirichlet flags, n*len ints (CLinearSystem_Field::SetFixedBoundaryCondition_Field -> CBCFlag::SetBC,
  * src/femls/linearsystem_field.cpp:618-710). */
