@@ -384,7 +384,10 @@ def main():
     nlaunch = nph[3]  # sweep launches per PCG iteration (the last forward and first backward phase may be one)
     sweep_traffic = spmv_traffic = None
     if args.config == "c2" and args.ordering == "blockcolor":
-        sweep_traffic = measured_traffic(["dfm::k_fwd_update<1, 4, 4, float", "dfm::k_bwd_rz<1, 4, 4, float"])
+        fused = nph[3] < nph[0] + nph[1]
+        kinds = (["dfm::k_fwd_update<1, 4, 4, float"] * (nph[0] - fused) + ["dfm::k_bwd_rz<1, 4, 4, float"] *
+                 (nph[1] - fused) + ["dfm::k_fwd_bwd_tile<1, 4, 4, float"] * fused)
+        sweep_traffic = measured_traffic(kinds)  # per launch, over the iteration's launches
         spmv_traffic = measured_traffic(["dfm::k_pcg_spmv_sell<1, float, false>"])
     mixed, fp64 = res["mixed"], res["fp64"]
     K = args.steps
@@ -454,8 +457,8 @@ def main():
         # forward phases with the r/x update, backward phases with (r, z)); per launch = one iteration's
         # algorithmic bytes of the phases and HIP-event time, over its launches
         "roofline": {
-            "kernel": "k_fwd_update/k_bwd_rz<B=%d, FP32> (ILU(0) phase sweeps of the PCG iteration, %d launches)"
-                      % (w["len"], nlaunch),
+            "kernel": "k_fwd_update/k_fwd_bwd_tile/k_bwd_rz<B=%d, FP32> (ILU(0) phase sweeps of the PCG "
+                      "iteration, %d launches)" % (w["len"], nlaunch),
             "bound": "hbm",
             "achieved": sw32_b / (sw32_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
