@@ -196,15 +196,19 @@ def run_c4_serving(args, N, W):
     import numpy as np
 
     n = args.requests
-    # rate 0: every request arrives at once (the server's capacity)
+    # rate 0: every request arrives at once (the server's capacity); the requests (meshes) are then generated
+    # before the clock starts, as a client would hold them; with a rate they are generated ahead of their arrival
     arrivals = W.c4_arrivals(n, args.rate) if args.rate > 0 else np.zeros(n)
-    q = queue.Queue(maxsize=256)
+    q = queue.Queue(maxsize=n if args.rate <= 0 else 256)
 
     def gen():
         for i in range(n):
             q.put((i, W.c4_request(i)))
 
-    threading.Thread(target=gen, daemon=True).start()
+    if args.rate <= 0:
+        gen()
+    else:
+        threading.Thread(target=gen, daemon=True).start()
     ctx = N.Context(0)
     nworkers = max(1, min(16, (os.cpu_count() or 8) - 2))
     # mixed precision per request: FP32 matrix/factor/vectors for the scalar (Poisson) requests, the FP32 factor

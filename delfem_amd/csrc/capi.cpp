@@ -6,6 +6,10 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <vector>
+#include <unordered_map>
+#include <mutex>
+#include <map>
 #include <stdexcept>
 
 #include "dfm_internal.hpp"
@@ -17,23 +21,122 @@ namespace dfm {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
-static std::atomic<long long> g_dev_bytes{0};
+static std::atomic<long long> g_dev_bytes{0};  // bytes handed out (cached blocks not counted)
 
-hipError_t DevAlloc::alloc(void** p, size_t bytes) {
-  hipError_t e = hipMalloc(p, bytes);
-  if (e == hipSuccess) g_dev_bytes += (long long)bytes;
-  return e;
+namespace {
+constexpr int kMaxDevices = 64;
+constexpr size_t kCacheCap = 16ull << 30;  // cached device bytes per device
+constexpr size_t kHostCacheCap = 64ull << 20;
+thread_local int g_cache_guard = 0;
+
+size_t size_class(size_t b) {
+  if (b <= 4096) return (b + 255) & ~size_t(255);
+  int k = 63 - __builtin_clzll((unsigned long long)b);
+  const size_t step = size_t(1) << (k - 3);
+  return (b + step - 1) & ~(step - 1);
 }
-void DevAlloc::release(void* p) {
-  size_t sz = 0;
-  // the size is recovered from the allocation itself for the accounting
-  if (hipMemPtrGetInfo(p, &sz) == hipSuccess) g_dev_bytes -= (long long)sz;
-  (void)hipFree(p);
+
+struct Pool {
+  std::mutex mu;
+  std::multimap<size_t, void*> free_blocks[kMaxDevices + 1];  // [kMaxDevices]: pinned host memory
+  std::unordered_map<void*, std::pair<int, size_t>> owner;    // block -> (device or kMaxDevices, class size)
+  size_t cached[kMaxDevices + 1] = {};
+};
+Pool& pool() {
+  static Pool* p = new Pool;  // never destroyed: releases may come from static destructors
+  return *p;
 }
+
+hipError_t pool_alloc(void** p, size_t bytes, bool host) {
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  int dev = kMaxDevices;
+  if (!host) {
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+  }
+  Pool& P = pool();
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto& fb = P.free_blocks[dev];
+    auto it = fb.find(cls);
+    if (it != fb.end()) {
+      *p = it->second;
+      fb.erase(it);
+      P.cached[dev] -= cls;
+      if (!host) g_dev_bytes += (long long)cls;
+      return hipSuccess;
+    }
+  }
+  hipError_t e = host ? hipHostMalloc(p, cls) : hipMalloc(p, cls);
+  if (e != hipSuccess && !host) {
+    // out of memory with blocks cached: return them and retry once
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> lk(P.mu);
+      for (auto& kv : P.free_blocks[dev]) drop.push_back(kv.second);
+      P.free_blocks[dev].clear();
+      P.cached[dev] = 0;
+      for (void* q : drop) P.owner.erase(q);
+    }
+    for (void* q : drop) (void)hipFree(q);
+    (void)hipGetLastError();
+    e = hipMalloc(p, cls);
+  }
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(P.mu);
+  P.owner[*p] = {dev, cls};
+  if (!host) g_dev_bytes += (long long)cls;
+  return hipSuccess;
+}
+
+void pool_release(void* p, bool host) {
+  Pool& P = pool();
+  std::vector<void*> drop;
+  int dev = 0;
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.owner.find(p);
+    if (it == P.owner.end()) return;
+    dev = it->second.first;
+    const size_t cls = it->second.second;
+    if (!host) g_dev_bytes -= (long long)cls;
+    if (g_cache_guard == 0) {
+      P.owner.erase(it);
+      drop.push_back(p);
+    } else {
+      P.free_blocks[dev].insert({cls, p});
+      P.cached[dev] += cls;
+      const size_t cap = host ? kHostCacheCap : kCacheCap;
+      auto& fb = P.free_blocks[dev];
+      while (P.cached[dev] > cap && !fb.empty()) {
+        auto last = std::prev(fb.end());
+        P.cached[dev] -= last->first;
+        P.owner.erase(last->second);
+        drop.push_back(last->second);
+        fb.erase(last);
+      }
+    }
+  }
+  for (void* q : drop) {
+    if (host) (void)hipHostFree(q);
+    else (void)hipFree(q);
+  }
+}
+}  // namespace
+
+hipError_t DevAlloc::alloc(void** p, size_t bytes) { return pool_alloc(p, bytes, false); }
+void DevAlloc::release(void* p) { pool_release(p, false); }
+hipError_t DevAlloc::alloc_host(void** p, size_t bytes) { return pool_alloc(p, bytes, true); }
+void DevAlloc::release_host(void* p) { pool_release(p, true); }
+CacheRelease::CacheRelease() { g_cache_guard++; }
+CacheRelease::~CacheRelease() { g_cache_guard--; }
 
 long long device_bytes_in_use() { return g_dev_bytes.load(); }
 
 Sys::~Sys() {
+  // no queued work on the system's stream may touch the blocks once they are cached for reuse (dfm_sys_destroy
+  // has synchronised it already; failure paths of the create functions may not have)
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  CacheRelease guard;
   d_rowptr.release(); d_colind.release(); d_diaind.release(); d_lsplit.release(); d_usplit.release();
   d_frowptr.release(); d_fcolind.release(); d_fdiaind.release(); d_a2f.release();
   d_order.release(); d_inv.release(); d_elsup_ptr.release(); d_elsup.release(); d_lnods.release(); d_bc.release();
@@ -48,8 +151,8 @@ Sys::~Sys() {
   d_par.release(); d_lptr.release(); d_loads.release();
   for (auto& g : gexec)
     if (g) (void)hipGraphExecDestroy(g);
-  if (h_pinned) (void)hipHostFree(h_pinned);
-  if (h_pinned_d) (void)hipHostFree(h_pinned_d);
+  if (h_pinned) DevAlloc::release_host(h_pinned);
+  if (h_pinned_d) DevAlloc::release_host(h_pinned_d);
 }
 
 template <typename T>
@@ -214,8 +317,8 @@ static hipError_t sys_alloc_device(Sys& s) {
   if ((e = s.d_counter.alloc(8)) != hipSuccess) return e;
   if ((e = s.d_scal.alloc(kNScal)) != hipSuccess) return e;
   if ((e = s.d_iflag.alloc(8)) != hipSuccess) return e;
-  if ((e = hipHostMalloc((void**)&s.h_pinned, 16 * sizeof(int))) != hipSuccess) return e;
-  if ((e = hipHostMalloc((void**)&s.h_pinned_d, kNScal * sizeof(double))) != hipSuccess) return e;
+  if ((e = DevAlloc::alloc_host((void**)&s.h_pinned, 16 * sizeof(int))) != hipSuccess) return e;
+  if ((e = DevAlloc::alloc_host((void**)&s.h_pinned_d, kNScal * sizeof(double))) != hipSuccess) return e;
   if ((e = hipMemsetAsync(s.d_counter.p, 0, 8 * sizeof(unsigned), st)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(s.d_bc.p, 0, m * sizeof(int), st)) != hipSuccess) return e;
   if ((e = launch_fill(s.d_u.p, 0.0, m, st)) != hipSuccess) return e;

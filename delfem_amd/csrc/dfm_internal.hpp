@@ -32,9 +32,20 @@ void set_error(const std::string& msg);
   } while (0)
 
 // Device allocation with byte accounting (the serving admission budget reads it).
+// Device allocations go through a per-device cache: blocks are sized in classes (<= 12.5 % above the request)
+// and a block released while a CacheRelease guard is alive (a system being destroyed after its stream was
+// synchronised, so no queued kernel can still touch it) is kept for reuse instead of returned with hipFree, which
+// stalls the whole device. Releases outside a guard (a buffer regrown while work may be queued) free at once.
+// The cache holds at most a fixed number of bytes per device; beyond that the largest blocks are freed.
 struct DevAlloc {
   static hipError_t alloc(void** p, size_t bytes);
   static void release(void* p);
+  static hipError_t alloc_host(void** p, size_t bytes);  // pinned host memory, same caching rules
+  static void release_host(void* p);
+};
+struct CacheRelease {
+  CacheRelease();
+  ~CacheRelease();
 };
 
 template <typename T>
