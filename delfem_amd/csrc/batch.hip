@@ -46,6 +46,8 @@ struct SegDev {
   int* fl;
   int ps0, ps1;  // SpMV blocks of the segment
   int pv0, pv1;  // vector blocks of the segment
+  int pr0, pr1;  // sweep partials of the segment: forward (r, r) [pr0, pr1), backward (r, z) [pz0, pz1)
+  int pz0, pz1;
   int nslice;           // SELL-64 copy of the matrix: slices, slab rows per slice, columns, values
   const int* sptr;
   const unsigned* scol;
@@ -64,12 +66,6 @@ __device__ inline double consume_range(const double* __restrict__ slot, int b0, 
   for (int b = b0 + (int)threadIdx.x; b < b1; b += NT) t[0] += slot[b];
   block_allsum<NT, 1>(t);
   return t[0];
-}
-
-__device__ inline void produce_at(double* slot, int idx, double v) {
-  double t[1] = {v};
-  block_allsum<NT, 1>(t);
-  if (threadIdx.x == 0) slot[idx] = t[0];
 }
 
 // q = A p, partials of (p, q); mixed, when a reliable update is due: r = b - A64 x64 with its partials. Over the
@@ -134,45 +130,19 @@ __global__ void __launch_bounds__(NT) k_b_alpha(const SegDev* __restrict__ segs,
   }
 }
 
-// r -= alpha q; x += alpha p; partials of (r, r)   (:139-144)
-template <typename T>
-__global__ void __launch_bounds__(NT) k_b_update(const SegDev* __restrict__ segs, const int* __restrict__ bseg,
-                                                 double* slot_rr) {
-  const int b = blockIdx.x;
-  const SegDev& d = segs[bseg[b]];
-  if (d.fl[F_DONE]) return;
-  const double alpha = d.sc[S_ALPHA];
-  const T ma = (T)(-alpha), pa = (T)alpha;
-  T* r = (T*)d.r;
-  T* x = (T*)d.x;
-  const T* q = (const T*)d.q;
-  const T* p = (const T*)d.p;
-  double v = 0.0;
-  const long long i0 = (long long)(b - d.pv0) * DV + threadIdx.x;
-#pragma unroll
-  for (int j = 0; j < VPT; j++) {
-    const long long i = i0 + (long long)j * NT;
-    if (i < d.m) {
-      const T ri = r[i] + ma * q[i];
-      r[i] = ri;
-      x[i] = x[i] + pa * p[i];
-      v += (double)ri * (double)ri;
-    }
-  }
-  produce_at(slot_rr, b, v);
-}
-
-// per segment: convergence judgement. FP64: on the recursive residual (:146-155). Mixed: on the true residual
-// when one was formed this iteration (the solution is then the previous iteration's), else the recursive
-// residual only schedules the next reliable update (it dropped by delta, or looks converged).
+// per segment: convergence judgement, then beta. FP64: on the recursive residual (:146-155). Mixed: on the true
+// residual when one was formed this iteration (the solution is then the previous iteration's), else the
+// recursive residual only schedules the next reliable update (it dropped by delta, or looks converged). Then
+// beta = rho_new / rho (:158-159), and the scheduled reliable update becomes current.
 template <bool MX>
-__global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs, const double* slot_rr,
-                                                const double* slot_true) {
+__global__ void __launch_bounds__(NT) k_b_judge_beta(const SegDev* __restrict__ segs, const double* slot_rr,
+                                                     const double* slot_rz, const double* slot_true) {
   const SegDev& d = segs[blockIdx.x];
   int* fl = d.fl;
   double* sc = d.sc;
   if (fl[F_DONE]) return;
-  const double rr = consume_range(slot_rr, d.pv0, d.pv1);
+  const double rr = consume_range(slot_rr, d.pr0, d.pr1);
+  const double rz = consume_range(slot_rz, d.pz0, d.pz1);
   const double rt = (MX && fl[F_PCUR]) ? consume_range(slot_true, d.ps0, d.ps1) : 0.0;
   if (threadIdx.x) return;
   const int it = fl[F_IT];
@@ -187,7 +157,10 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
       } else {
         fl[F_DONE] = 2;
       }
+      return;
     }
+    sc[S_BETA] = rz / sc[S_RZ];
+    sc[S_RZ] = rz;
     return;
   }
   double rmax = sc[S_RMAX];
@@ -222,64 +195,85 @@ __global__ void __launch_bounds__(NT) k_b_judge(const SegDev* __restrict__ segs,
     sc[S_LASTUPD] = (double)it;
     sc[S_RRTRIG] = rr;
   }
+  sc[S_BETA] = rz / sc[S_RZ];
+  sc[S_RZ] = rz;
+  fl[F_PCUR] = fl[F_PNEXT];
 }
 
-// TF: factor values (FP64 or the FP32 copy); T: vectors and arithmetic; MO/MI: register-held entries per class
+// TF: factor values (FP64 or the FP32 copy); T: vectors and arithmetic; MO/MI: register-held entries per class.
+// One wavefront per (segment, tile) entry. Forward: the tile rows' r -= alpha q, x += alpha p (registers, stored
+// at the end), the forward sweep from the new r, and the wavefront's (r, r) partial at the entry's slot (each
+// segment's slots are contiguous, so its reduction does not depend on the other segments).
 template <int B, int MO, int MI, typename TF, typename T>
 __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
-                                                     int count) {
+                                                     const int* __restrict__ eslot, int count, double* slot_rr) {
   __shared__ T xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int e = (int)blockIdx.x * kWPB + w;
   if (e >= count) return;
   const int2 en = ent[e];
   const SegDev& d = segs[en.x];
-  sweep_fwd_wave<B, MO, MI, TF, T, T, T>(d.fa, en.y, (const TF*)d.fval, (const TF*)d.fdval, (const T*)d.r, (T*)d.z, d.fl,
-                                 xsb[w]);
+  if (d.fl[F_DONE]) return;
+  const double alpha = d.sc[S_ALPHA];
+  const T ma = (T)(-alpha), pa = (T)alpha;
+  T* r = (T*)d.r;
+  T* x = (T*)d.x;
+  const T* q = (const T*)d.q;
+  const T* p = (const T*)d.p;
+  T rn[B], xn[B];
+  SlabSlot ss;
+  T t[B];
+  sweep_fwd_core<B, MO, MI, TF, T, T>(
+      d.fa, en.y, (const TF*)d.fval, (const TF*)d.fdval,
+      [&](const SlabSlot& sl, T(&u)[B]) {
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+          const size_t i = (size_t)sl.row * B + c;
+          rn[c] = r[i] + ma * q[i];
+          xn[c] = x[i] + pa * p[i];
+          u[c] = rn[c];
+        }
+      },
+      (T*)d.z, nullptr, xsb[w], ss, t);
+  double v = 0.0;
+  if (ss.active)
+#pragma unroll
+    for (int c = 0; c < B; c++) {
+      const size_t i = (size_t)ss.row * B + c;
+      r[i] = rn[c];
+      x[i] = xn[c];
+      v += (double)rn[c] * (double)rn[c];
+    }
+  v = wave_sum(v);
+  if (threadIdx.x % 64 == 0) slot_rr[eslot[e]] = v;
 }
 
+// Backward sweep of one (segment, tile) entry and the wavefront's (r, z) partial at the entry's slot.
 template <int B, int MO, int MI, typename TF, typename T>
 __global__ void __launch_bounds__(64 * kWPB) k_b_bwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
-                                                     int count) {
+                                                     const int* __restrict__ eslot, int count, double* slot_rz) {
   __shared__ T xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int e = (int)blockIdx.x * kWPB + w;
   if (e >= count) return;
   const int2 en = ent[e];
   const SegDev& d = segs[en.x];
-  sweep_bwd_wave<B, MO, MI, TF, T, T>(d.ba, en.y, (const TF*)d.bval, (T*)d.z, d.fl, xsb[w]);
-}
-
-// partials of (r, z)
-template <typename T>
-__global__ void __launch_bounds__(NT) k_b_rz(const SegDev* __restrict__ segs, const int* __restrict__ bseg,
-                                             double* slot_rz) {
-  const int b = blockIdx.x;
-  const SegDev& d = segs[bseg[b]];
   if (d.fl[F_DONE]) return;
+  T* z = (T*)d.z;
   const T* r = (const T*)d.r;
-  const T* z = (const T*)d.z;
-  double v = 0.0;
-  const long long i0 = (long long)(b - d.pv0) * DV + threadIdx.x;
+  SlabSlot ss;
+  T t[B];
+  sweep_bwd_core<B, MO, MI, TF, T, T>(d.ba, en.y, (const TF*)d.bval, z, nullptr, xsb[w], ss, t,
+                                      [&](const SlabSlot& sl, T(&u)[B]) {
 #pragma unroll
-  for (int j = 0; j < VPT; j++) {
-    const long long i = i0 + (long long)j * NT;
-    if (i < d.m) v += (double)r[i] * (double)z[i];
-  }
-  produce_at(slot_rz, b, v);
-}
-
-// per segment: beta = rho_new / rho (:158-159); the scheduled reliable update becomes current
-template <bool MX>
-__global__ void __launch_bounds__(NT) k_b_beta(const SegDev* __restrict__ segs, const double* slot_rz) {
-  const SegDev& d = segs[blockIdx.x];
-  if (d.fl[F_DONE]) return;
-  const double rz = consume_range(slot_rz, d.pv0, d.pv1);
-  if (threadIdx.x == 0) {
-    d.sc[S_BETA] = rz / d.sc[S_RZ];
-    d.sc[S_RZ] = rz;
-    if (MX) d.fl[F_PCUR] = d.fl[F_PNEXT];
-  }
+                                        for (int c = 0; c < B; c++) u[c] = z[(size_t)sl.row * B + c];
+                                      });
+  double v = 0.0;
+  if (ss.active)
+#pragma unroll
+    for (int c = 0; c < B; c++) v += (double)r[(size_t)ss.row * B + c] * (double)t[c];
+  v = wave_sum(v);
+  if (threadIdx.x % 64 == 0) slot_rz[eslot[e]] = v;
 }
 
 // p = beta p; p += z (:160-166); mixed: the FP64 solution takes the partial solution when an update is due
@@ -346,6 +340,7 @@ struct Batch {
     DBuf<SegDev> d_segs;
     DBuf<int> d_sblk, d_vblk;
     DBuf<int2> d_fent, d_bent;
+    DBuf<int> d_fslot, d_bslot;  // partial slot of each entry (segment-contiguous)
     std::vector<int> fptr, bptr;  // per phase offsets into the entry tables
     std::vector<int> fwo, fwi, bwo, bwi;  // per phase widest classes over the segments
     int nsb = 0, nvb = 0;
@@ -355,6 +350,7 @@ struct Batch {
   ~Batch() {
     for (auto& g : grp) {
       g.d_segs.release(); g.d_sblk.release(); g.d_vblk.release(); g.d_fent.release(); g.d_bent.release();
+      g.d_fslot.release(); g.d_bslot.release();
       g.d_pq.release(); g.d_true.release(); g.d_rr.release(); g.d_rz.release();
     }
     d_sc.release();
@@ -386,6 +382,8 @@ static int rebuild(Batch& bt) {
     std::vector<SegDev> h;
     std::vector<int> sblk, vblk;
     std::vector<std::vector<int2>> fph, bph;
+    std::vector<std::vector<int>> fsl, bsl;  // per phase: partial slot of each entry
+    int nfslot = 0, nbslot = 0;
     std::vector<int> fwo, fwi, bwo, bwi;
     for (int k : g.members) {
       BatchSeg& sg = bt.segs[k];
@@ -422,12 +420,17 @@ static int rebuild(Batch& bt) {
       d.pv0 = (int)vblk.size();
       for (long long i = 0; i < (long long)d.m; i += DV) vblk.push_back(si);
       d.pv1 = (int)vblk.size();
+      d.pr0 = nfslot;
+      d.pz0 = nbslot;
       for (int dir = 0; dir < 2; dir++) {
         const Sweep& w = dir == 0 ? s.fwd : s.bwd;
         auto& ph = dir == 0 ? fph : bph;
+        auto& psl = dir == 0 ? fsl : bsl;
+        int& nslot = dir == 0 ? nfslot : nbslot;
         auto& po = dir == 0 ? fwo : bwo;
         auto& pi = dir == 0 ? fwi : bwi;
         if ((int)ph.size() < w.nphase()) ph.resize(w.nphase());
+        if ((int)psl.size() < w.nphase()) psl.resize(w.nphase());
         if ((int)po.size() < w.nphase()) {
           po.resize(w.nphase(), 0);
           pi.resize(w.nphase(), 0);
@@ -437,8 +440,13 @@ static int rebuild(Batch& bt) {
           pi[p] = std::max(pi[p], w.phase_wi[p]);
         }
         for (int p = 0; p < w.nphase(); p++)
-          for (int q = w.phase_gptr[p]; q < w.phase_gptr[p + 1]; q++) ph[p].push_back(make_int2(si, q));
+          for (int q = w.phase_gptr[p]; q < w.phase_gptr[p + 1]; q++) {
+            ph[p].push_back(make_int2(si, q));
+            psl[p].push_back(nslot++);
+          }
       }
+      d.pr1 = nfslot;
+      d.pz1 = nbslot;
       h.push_back(d);
     }
     g.nsb = (int)sblk.size();
@@ -449,14 +457,17 @@ static int rebuild(Batch& bt) {
     g.bwi = bwi;
     if (h.empty()) continue;
     std::vector<int2> fe, be;
+    std::vector<int> fs, bs;
     g.fptr.assign(1, 0);
     g.bptr.assign(1, 0);
-    for (auto& v : fph) {
-      fe.insert(fe.end(), v.begin(), v.end());
+    for (size_t p = 0; p < fph.size(); p++) {
+      fe.insert(fe.end(), fph[p].begin(), fph[p].end());
+      fs.insert(fs.end(), fsl[p].begin(), fsl[p].end());
       g.fptr.push_back((int)fe.size());
     }
-    for (auto& v : bph) {
-      be.insert(be.end(), v.begin(), v.end());
+    for (size_t p = 0; p < bph.size(); p++) {
+      be.insert(be.end(), bph[p].begin(), bph[p].end());
+      bs.insert(bs.end(), bsl[p].begin(), bsl[p].end());
       g.bptr.push_back((int)be.size());
     }
     hipError_t e = hipSuccess;
@@ -465,16 +476,20 @@ static int rebuild(Batch& bt) {
     grow(g.d_vblk, vblk.size(), e);
     grow(g.d_fent, fe.size(), e);
     grow(g.d_bent, be.size(), e);
+    grow(g.d_fslot, fs.size(), e);
+    grow(g.d_bslot, bs.size(), e);
     grow(g.d_pq, sblk.size(), e);
     grow(g.d_true, sblk.size(), e);
-    grow(g.d_rr, vblk.size(), e);
-    grow(g.d_rz, vblk.size(), e);
+    grow(g.d_rr, (size_t)nfslot, e);
+    grow(g.d_rz, (size_t)nbslot, e);
     DFM_HIP(e);
     DFM_HIP(hipMemcpyAsync(g.d_segs.p, h.data(), h.size() * sizeof(SegDev), hipMemcpyHostToDevice, st));
     DFM_HIP(hipMemcpyAsync(g.d_sblk.p, sblk.data(), sblk.size() * sizeof(int), hipMemcpyHostToDevice, st));
     DFM_HIP(hipMemcpyAsync(g.d_vblk.p, vblk.data(), vblk.size() * sizeof(int), hipMemcpyHostToDevice, st));
     if (!fe.empty()) DFM_HIP(hipMemcpyAsync(g.d_fent.p, fe.data(), fe.size() * sizeof(int2), hipMemcpyHostToDevice, st));
     if (!be.empty()) DFM_HIP(hipMemcpyAsync(g.d_bent.p, be.data(), be.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    if (!fs.empty()) DFM_HIP(hipMemcpyAsync(g.d_fslot.p, fs.data(), fs.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (!bs.empty()) DFM_HIP(hipMemcpyAsync(g.d_bslot.p, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice, st));
     // the tables are host temporaries: the copies must finish before they go out of scope
     DFM_HIP(hipStreamSynchronize(st));
   }
@@ -492,14 +507,12 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
     hipLaunchKernelGGL((k_b_spmv<B, TM, T, MX>), dim3(g.nsb), dim3(64 * kBSellWaves), 0, st, segs, g.d_sblk.p,
                        g.d_pq.p, g.d_true.p);
   hipLaunchKernelGGL(k_b_alpha, nseg, dim3(NT), 0, st, segs, (const double*)g.d_pq.p);
-  hipLaunchKernelGGL((k_b_update<T>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p, g.d_rr.p);
-  hipLaunchKernelGGL((k_b_judge<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p, (const double*)g.d_true.p);
   for (size_t p = 0; p + 1 < g.fptr.size(); p++) {
     const int c = g.fptr[p + 1] - g.fptr[p];
     if (c)
       with_slab_caps(g.fwo[p], g.fwi[p], B == 3 && sizeof(T) == 8 && sizeof(TF) == 8, [&]<int MO, int MI>() {
         hipLaunchKernelGGL((k_b_fwd<B, MO, MI, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
-                           (const int2*)g.d_fent.p + g.fptr[p], c);
+                           (const int2*)g.d_fent.p + g.fptr[p], (const int*)g.d_fslot.p + g.fptr[p], c, g.d_rr.p);
       });
   }
   for (size_t p = 0; p + 1 < g.bptr.size(); p++) {
@@ -507,11 +520,11 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
     if (c)
       with_slab_caps(g.bwo[p], g.bwi[p], B == 3 && sizeof(T) == 8 && sizeof(TF) == 8, [&]<int MO, int MI>() {
         hipLaunchKernelGGL((k_b_bwd<B, MO, MI, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
-                           (const int2*)g.d_bent.p + g.bptr[p], c);
+                           (const int2*)g.d_bent.p + g.bptr[p], (const int*)g.d_bslot.p + g.bptr[p], c, g.d_rz.p);
       });
   }
-  hipLaunchKernelGGL((k_b_rz<T>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p, g.d_rz.p);
-  hipLaunchKernelGGL((k_b_beta<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rz.p);
+  hipLaunchKernelGGL((k_b_judge_beta<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p,
+                     (const double*)g.d_rz.p, (const double*)g.d_true.p);
   hipLaunchKernelGGL((k_b_pupdate<T, MX>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p);
   return hipGetLastError();
 }
