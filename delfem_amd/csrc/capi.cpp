@@ -199,6 +199,7 @@ static int upload_structure(Sys& s) {
   DFM_HIP(upload(s.d_inv, s.inv, st));
   s.sell.release_dev();
   DFM_HIP(upload(s.sell.d_sptr, s.sell.sptr, st));
+  DFM_HIP(upload(s.sell.d_visit, s.sell.visit, st));
   DFM_HIP(upload(s.sell.d_col, s.sell.col, st));
   DFM_HIP(upload(s.sell.d_src, s.sell.src, st));
   return upload_sweeps(s);
@@ -389,6 +390,7 @@ int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
   copy_sweep_host(s.fwd, a.fwd);
   copy_sweep_host(s.bwd, a.bwd);
   s.sell.sptr = a.sell.sptr; s.sell.col = a.sell.col; s.sell.src = a.sell.src; s.sell.nslab = a.sell.nslab;
+  s.sell.visit = a.sell.visit;
   auto fail = [&](hipError_t e) {
     set_error(std::string("dfm_sys_clone: ") + hipGetErrorString(e));
     delete h;

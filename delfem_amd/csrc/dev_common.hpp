@@ -26,6 +26,7 @@ __device__ inline T wave_sum(T v) {
 // visible; no block ever waits on another, and no device-scope fence (an L2 write-back on a multi-XCD part)
 // is issued per block.
 constexpr int kMaxPartial = 2048;  // producer grids are capped at this many blocks
+static_assert(kMaxPartial % 8 == 0, "strided grids keep workgroup positions on their XCD");
 
 // Sum of K values over the block; the totals are returned in every thread. NT = block size.
 template <int NT, int K>

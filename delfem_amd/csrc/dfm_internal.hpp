@@ -115,20 +115,28 @@ struct Sweep {
   }
 };
 
+// Launch shapes the host schedule is laid out for: sweep kernels run kSweepWavesPerWG groups per workgroup, the
+// SELL SpMV kSellSlicesPerWG slices per workgroup.
+constexpr int kSweepWavesPerWG = 4;
+constexpr int kSellSlicesPerWG = 16;
+
 // SELL-64 copy of the off-diagonal blocks for the SpMV of the Krylov loops: slice s = device rows
 // [64 s, 64 s + 64), one row per lane; the slice's entries are slab rows [sptr[s], sptr[s+1]) in CRS order per
 // lane (padding: column 0, no source), value e of entry (slab row r, lane l) at (r*bs + e)*64 + l.
+// visit[i]: the slice the SpMV handles at position i (workgroup i / kSellSlicesPerWG), -1 for none; the slices
+// are placed so that each XCD works on one spatial strip of the mesh.
 struct Sell {
   std::vector<int> sptr;          // [nslice+1]
   std::vector<unsigned> col, src; // [64*nslab]; src = crs block index or ~0u
+  std::vector<int> visit;         // [npos], npos a multiple of 8 * kSellSlicesPerWG
   long long nslab = 0;
-  DBuf<int> d_sptr;
+  DBuf<int> d_sptr, d_visit;
   DBuf<unsigned> d_col, d_src;
   DBuf<double> d_val;
   DBuf<float> d_val32;
   int nslice() const { return sptr.empty() ? 0 : (int)sptr.size() - 1; }
   void release_dev() {
-    d_sptr.release(); d_col.release(); d_src.release(); d_val.release(); d_val32.release();
+    d_sptr.release(); d_visit.release(); d_col.release(); d_src.release(); d_val.release(); d_val32.release();
   }
 };
 

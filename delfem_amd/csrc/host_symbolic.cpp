@@ -1,6 +1,7 @@
 // Host symbolic phase: CRS pattern from the mesh, ILU(0) orderings, the device-numbering pattern and the
 // phase/group/level schedules of the triangular sweeps and the numeric factorisation.
 #include <algorithm>
+#include <cstdlib>
 #include <queue>
 #include <cmath>
 #include <deque>
@@ -89,6 +90,70 @@ void build_device_pattern(Sys& s) {
 }
 
 static const int kGroup = 64;  // rows per group: one wavefront, one row per lane
+
+// Spatial key of each device row: its node's coordinate along the longest side of the mesh's bounding box
+// (empty when the system has no coordinates).
+static std::vector<double> row_key(const Sys& s) {
+  std::vector<double> key;
+  if (s.ndim < 1 || s.coords.size() != (size_t)s.n * s.ndim || s.order.size() != (size_t)s.n) return key;
+  if (const char* e = std::getenv("DFM_XCD_PLACEMENT"); e && e[0] == '0') return key;
+  int axis = 0;
+  double best = -1.0;
+  for (int d = 0; d < s.ndim; d++) {
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < s.n; i++) {
+      lo = std::min(lo, s.coords[(size_t)i * s.ndim + d]);
+      hi = std::max(hi, s.coords[(size_t)i * s.ndim + d]);
+    }
+    if (hi - lo >= best) { best = hi - lo; axis = d; }
+  }
+  key.resize((size_t)s.n);
+  for (int r = 0; r < s.n; r++) key[r] = s.coords[(size_t)s.order[r] * s.ndim + axis];
+  return key;
+}
+
+// The dispatcher hands the workgroups of a launch to the eight XCDs in turn (workgroup b to XCD b % 8), and every
+// XCD has its own L2. xcd_positions orders m independent work items (item k has spatial key key[k]) over npos >= m
+// positions, per_wg consecutive positions per workgroup, so that each XCD receives one spatial strip: the items
+// sorted by key are cut into eight runs, run x filling XCD x's positions in ascending item order. Neighbouring
+// items then read the lines of their shared neighbours through one L2 instead of eight. With npos == m every
+// position is filled (runs sized to the XCDs' shares); otherwise the runs are balanced and the spare positions
+// hold -1.
+static constexpr int kXcd = 8;
+
+static std::vector<int> xcd_positions(const std::vector<double>& key, int npos, int per_wg) {
+  const int m = (int)key.size();
+  std::vector<int> sorted((size_t)m);
+  std::iota(sorted.begin(), sorted.end(), 0);
+  std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<long long> cap(kXcd, 0), run(kXcd, 0);
+  for (int i = 0; i < npos; i++) cap[(i / per_wg) % kXcd]++;
+  for (int x = 0; x < kXcd; x++) run[x] = npos == m ? cap[x] : (long long)m * (x + 1) / kXcd - (long long)m * x / kXcd;
+  std::vector<long long> next(kXcd, 0), end(kXcd, 0);
+  long long acc = 0;
+  for (int x = 0; x < kXcd; x++) {
+    if (run[x] > cap[x]) throw std::runtime_error("schedule: XCD run larger than its positions");
+    next[x] = acc;
+    acc += run[x];
+    end[x] = acc;
+    // inside a run the items keep their own order (for slices and tiles: device order, so each XCD streams its
+    // strip's values through ascending addresses)
+    std::sort(sorted.begin() + next[x], sorted.begin() + end[x]);
+  }
+  std::vector<int> pos((size_t)npos, -1);
+  for (int i = 0; i < npos; i++) {
+    const int x = (i / per_wg) % kXcd;
+    if (next[x] < end[x]) pos[i] = sorted[(size_t)next[x]++];
+  }
+  return pos;
+}
+
+// Mean key of the rows [a, b) (a group or a slice).
+static double range_key(const std::vector<double>& key, int a, int b) {
+  double t = 0.0;
+  for (int r = a; r < b; r++) t += key[r];
+  return b > a ? t / (b - a) : 0.0;
+}
 
 // ILU factor pattern in the device numbering. ILU(0): the matrix pattern. ILU(k): the level-of-fill pattern of
 // CMatDiaFrac_BlkCrs::AddFracPtn (src/matvec/matdiafrac_blkcrs.cpp:1254-1381), row by row: the row's entries
@@ -219,6 +284,7 @@ static void level_sweeps(Sys& s) {
 // may link groups of one colour, which then take later phases.
 static void group_sweeps(Sys& s, const std::vector<int>& gstart) {
   const int n = s.n;
+  const std::vector<double> key = row_key(s);
   const int ngroup = (int)gstart.size() - 1;
   const std::vector<unsigned>& rp = s.frowptr;
   const std::vector<unsigned>& ci = s.fcolind;
@@ -269,9 +335,19 @@ static void group_sweeps(Sys& s, const std::vector<int>& gstart) {
       if (lev[i] > 255) throw std::runtime_error("schedule: more than 256 levels inside a group");
       w.lev[i] = (uint8_t)lev[i];
     }
-    // groups of each phase in device order; (start, end) pairs in execution order
+    // groups of each phase (independent of one another) in device order, then placed over the XCDs by
+    // position; (start, end) pairs in execution order
     std::vector<std::vector<int>> members((size_t)nph);
     for (int g = 0; g < ngroup; g++) members[ph[g]].push_back(g);
+    if (!key.empty())
+      for (auto& mem : members) {
+        std::vector<double> gk(mem.size());
+        for (size_t k = 0; k < mem.size(); k++) gk[k] = range_key(key, gstart[mem[k]], gstart[mem[k] + 1]);
+        const std::vector<int> pos = xcd_positions(gk, (int)mem.size(), kSweepWavesPerWG);
+        std::vector<int> placed(mem.size());
+        for (size_t k = 0; k < mem.size(); k++) placed[k] = mem[pos[k]];
+        mem.swap(placed);
+      }
     w.grp.clear();
     w.phase_gptr.assign(1, 0);
     w.phase_nlev.clear();
@@ -365,6 +441,18 @@ static void build_sell(Sys& s) {
     c.sptr[sl + 1] = c.sptr[sl] + (int)w;
   }
   c.nslab = c.sptr[ns];
+  // visiting order of the slices over the XCDs: positions of kSellSlicesPerWG slices per workgroup, the
+  // workgroup count rounded up to a multiple of the XCD count
+  const int nwg = ((ns + kSellSlicesPerWG - 1) / kSellSlicesPerWG + kXcd - 1) / kXcd * kXcd;
+  const std::vector<double> key = row_key(s);
+  if (key.empty() || ns == 0) {
+    c.visit.resize((size_t)nwg * kSellSlicesPerWG);
+    for (size_t i = 0; i < c.visit.size(); i++) c.visit[i] = i < (size_t)ns ? (int)i : -1;
+  } else {
+    std::vector<double> sk((size_t)ns);
+    for (int sl = 0; sl < ns; sl++) sk[sl] = range_key(key, sl * kGroup, std::min(n, (sl + 1) * kGroup));
+    c.visit = xcd_positions(sk, nwg * kSellSlicesPerWG, kSellSlicesPerWG);
+  }
   c.col.assign((size_t)c.nslab * kGroup, 0u);
   c.src.assign((size_t)c.nslab * kGroup, ~0u);
   for (int r = 0; r < n; r++) {

@@ -133,10 +133,12 @@ __global__ void __launch_bounds__(NT) k_pcg_spmv(int n, const unsigned* __restri
 // The same over the SELL-64 copy of the matrix: one slice (64 rows, one per lane) per wavefront, kSellWaves
 // wavefronts per block; the row sums follow CRS order with the diagonal block last, as MatVec.
 constexpr int kSellWaves = 16;
+static_assert(kSellWaves == kSellSlicesPerWG && kWPB == kSweepWavesPerWG, "launch shapes of the host schedule");
 
 template <int B, typename T, bool REL>
 __global__ void __launch_bounds__(64 * kSellWaves)
-    k_pcg_spmv_sell(int n, int nslice, const int* __restrict__ sptr, const unsigned* __restrict__ col,
+    k_pcg_spmv_sell(int n, int npos, const int* __restrict__ visit, const int* __restrict__ sptr,
+                    const unsigned* __restrict__ col,
                     const T* __restrict__ val, const T* __restrict__ dia, const T* __restrict__ p, T* __restrict__ q,
                     double* slot, const double* __restrict__ val64, const double* __restrict__ dia64,
                     const double* __restrict__ x64, const double* __restrict__ b64, T* __restrict__ r, double* slot_true,
@@ -147,7 +149,9 @@ __global__ void __launch_bounds__(64 * kSellWaves)
   const bool rel = REL && fl[F_PCUR];
   const int wave = wave_in_block(), lane = threadIdx.x % 64;
   double v[1] = {0.0}, vt[1] = {0.0};
-  for (int sl = (int)blockIdx.x * kSellWaves + wave; sl < nslice; sl += (int)gridDim.x * kSellWaves) {
+  for (int pos = (int)blockIdx.x * kSellWaves + wave; pos < npos; pos += (int)gridDim.x * kSellWaves) {
+    const int sl = visit[pos];
+    if (sl < 0) continue;
     const int r0 = sptr[sl], w = sptr[sl + 1] - r0;
     const int row = sl * 64 + lane;
     T acc[B];
@@ -620,14 +624,15 @@ template <typename T, bool REL>
 static hipError_t pcg_spmv(const Sys& s, const T* crs, const T* dia, const T* p, T* q, double* slot, unsigned& G,
                            const double* x64, const double* b64, T* r, double* slot_true, hipStream_t st) {
   if (s.sell.nslice() > 0) {
-    const int ns = s.sell.nslice();
-    G = (unsigned)std::min<long long>(kMaxPartial, (ns + kSellWaves - 1) / kSellWaves);
+    // kMaxPartial is a multiple of 8, so a strided grid keeps every position on the XCD it was placed for
+    const int npos = (int)s.sell.visit.size();
+    G = (unsigned)std::min<long long>(kMaxPartial, npos / kSellWaves);
     const T* sval = sizeof(T) == 4 ? (const T*)s.sell.d_val32.p : (const T*)s.sell.d_val.p;
 #define DFM_SS(B)                                                                                              \
   if (s.len == B) {                                                                                            \
-    hipLaunchKernelGGL((k_pcg_spmv_sell<B, T, REL>), dim3(G), dim3(64 * kSellWaves), 0, st, s.n, ns,           \
-                       s.sell.d_sptr.p, s.sell.d_col.p, sval, dia, p, q, slot, s.sell.d_val.p, s.d_dia.p, x64, b64, \
-                       r, slot_true, s.d_iflag.p);                                                             \
+    hipLaunchKernelGGL((k_pcg_spmv_sell<B, T, REL>), dim3(G), dim3(64 * kSellWaves), 0, st, s.n, npos,         \
+                       s.sell.d_visit.p, s.sell.d_sptr.p, s.sell.d_col.p, sval, dia, p, q, slot, s.sell.d_val.p,  \
+                       s.d_dia.p, x64, b64, r, slot_true, s.d_iflag.p);                                         \
     return hipGetLastError();                                                                                  \
   }
     DFM_SS(1) DFM_SS(2) DFM_SS(3)
@@ -793,7 +798,7 @@ static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
 static std::vector<const void*> graph_key(const Sys& s, const PcgBufs& v) {
   std::vector<const void*> k{v.crs, v.dia, v.r, v.z, v.p, v.q, v.x, v.x64, v.b64, s.d_crs.p, s.d_dia.p,
                              s.d_rowptr.p, s.d_colind.p, s.d_partial.p, s.d_part_sw.p, s.d_scal.p, s.d_iflag.p,
-                             s.sell.d_sptr.p, s.sell.d_col.p, s.sell.d_val.p, s.sell.d_val32.p};
+                             s.sell.d_sptr.p, s.sell.d_visit.p, s.sell.d_col.p, s.sell.d_val.p, s.sell.d_val32.p};
   for (const Sweep* w : {&s.fwd, &s.bwd})
     for (const void* p : {(const void*)w->d_gmeta.p, (const void*)w->d_srow.p, (const void*)w->d_scnt.p,
                           (const void*)w->d_ecol.p, (const void*)w->d_eval.p, (const void*)w->d_dval.p,
