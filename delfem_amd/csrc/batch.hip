@@ -70,12 +70,16 @@ __device__ inline double consume_range(const double* __restrict__ slot, int b0, 
 
 // q = A p, partials of (p, q); mixed, when a reliable update is due: r = b - A64 x64 with its partials. Over the
 // SELL-64 copy of each system: one slice (64 rows, one per lane) per wavefront, row sums in CRS order with the
-// diagonal block last (MatVec, src/matvec/matdia_blkcrs.cpp:921-1094). TM: matrix values, T: vectors.
+// diagonal block last (MatVec, src/matvec/matdia_blkcrs.cpp:921-1094). TM: matrix values, T: vectors. The mixed
+// loop of scalar systems (ZQ) folds the direction update in as the single-system SpMV does: p = beta p + z,
+// q = beta q + A z, and its true residual at a reliable update is b - A (x_hi + x_lo) (the forward phases fold x_lo
+// afterwards).
 template <int B, typename TM, typename T, bool REL>
 __global__ void __launch_bounds__(64 * kBSellWaves) k_b_spmv(const SegDev* __restrict__ segs,
                                                              const int* __restrict__ bseg, double* slot_pq,
                                                              double* slot_true) {
   constexpr int CH = SellChunk<B>::CH;
+  constexpr bool ZQ = REL && B == 1;
   const int b = blockIdx.x;
   const SegDev& d = segs[bseg[b]];
   const int* fl = d.fl;
@@ -83,26 +87,40 @@ __global__ void __launch_bounds__(64 * kBSellWaves) k_b_spmv(const SegDev* __res
   const bool rel = REL && fl[F_PCUR];
   const int wave = wave_in_block(), lane = (int)(threadIdx.x % 64);
   const int sl = (b - d.ps0) * kBSellWaves + wave;
-  const T* p = (const T*)d.p;
+  T* p = (T*)d.p;
+  T* q = (T*)d.q;
+  const T* z = (const T*)d.z;
+  const T bt = ZQ ? (T)d.sc[S_BETA] : T(0);
   double v[2] = {0.0, 0.0};
   if (sl < d.nslice) {
     const int r0 = d.sptr[sl], w = d.sptr[sl + 1] - r0;
     const int row = sl * 64 + lane;
     T acc[B];
-    sell_row<B, CH, TM, T, T>(d.scol, (const TM*)d.sval, r0, w, lane, p, acc);
+    sell_row<B, CH, TM, T, T>(d.scol, (const TM*)d.sval, r0, w, lane, ZQ ? z : (const T*)p, acc);
     if (row < d.n) {
-      dia_row<B, TM, T, T>((const TM*)d.dia, p, row, acc);
+      dia_row<B, TM, T, T>((const TM*)d.dia, ZQ ? z : (const T*)p, row, acc);
 #pragma unroll
       for (int k = 0; k < B; k++) {
-        ((T*)d.q)[(size_t)row * B + k] = acc[k];
-        v[0] += (double)p[(size_t)row * B + k] * (double)acc[k];
+        const size_t i = (size_t)row * B + k;
+        if (ZQ) {
+          const T pn = p[i] * bt + z[i];
+          const T qn = q[i] * bt + acc[k];
+          p[i] = pn;
+          q[i] = qn;
+          v[0] += (double)pn * (double)qn;
+        } else {
+          q[i] = acc[k];
+          v[0] += (double)p[i] * (double)acc[k];
+        }
       }
     }
     if (REL && rel) {
+      // ZQ: x_lo not folded yet; otherwise folded by the p update (x_lo = 0)
+      const SumVec xs{d.x64, (const float*)d.x};
       double a64[B];
-      sell_row<B, CH, double, double, double>(d.scol, d.sval64, r0, w, lane, d.x64, a64);
+      sell_row<B, CH, double, double, double>(d.scol, d.sval64, r0, w, lane, xs, a64);
       if (row < d.n) {
-        dia_row<B, double, double, double>(d.dia64, d.x64, row, a64);
+        dia_row<B, double, double, double>(d.dia64, xs, row, a64);
 #pragma unroll
         for (int k = 0; k < B; k++) {
           const double t = d.b64[(size_t)row * B + k] - a64[k];
@@ -216,6 +234,8 @@ __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ 
   if (d.fl[F_DONE]) return;
   const double alpha = d.sc[S_ALPHA];
   const T ma = (T)(-alpha), pa = (T)alpha;
+  // scalar mixed segments: x is x_lo; at a reliable update it is folded into x_hi before alpha p is added
+  const bool fold = B == 1 && sizeof(T) == 4 && d.fl[F_PCUR];
   T* r = (T*)d.r;
   T* x = (T*)d.x;
   const T* q = (const T*)d.q;
@@ -230,7 +250,12 @@ __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ 
         for (int c = 0; c < B; c++) {
           const size_t i = (size_t)sl.row * B + c;
           rn[c] = r[i] + ma * q[i];
-          xn[c] = x[i] + pa * p[i];
+          T xo = x[i];
+          if (fold) {
+            d.x64[i] += (double)xo;
+            xo = T(0);
+          }
+          xn[c] = xo + pa * p[i];
           u[c] = rn[c];
         }
       },
@@ -277,6 +302,7 @@ __global__ void __launch_bounds__(64 * kWPB) k_b_bwd(const SegDev* __restrict__ 
 }
 
 // p = beta p; p += z (:160-166); mixed: the FP64 solution takes the partial solution when an update is due
+// (scalar mixed systems form p in the SpMV and fold in the forward phases instead)
 template <typename T, bool MX>
 __global__ void __launch_bounds__(NT) k_b_pupdate(const SegDev* __restrict__ segs, const int* __restrict__ bseg) {
   const int b = blockIdx.x;
@@ -525,7 +551,8 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
   }
   hipLaunchKernelGGL((k_b_judge_beta<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p,
                      (const double*)g.d_rz.p, (const double*)g.d_true.p);
-  hipLaunchKernelGGL((k_b_pupdate<T, MX>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p);
+  // the scalar mixed SpMV forms p itself
+  if (!(MX && B == 1)) hipLaunchKernelGGL((k_b_pupdate<T, MX>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p);
   return hipGetLastError();
 }
 

@@ -142,8 +142,9 @@ __device__ inline void spmv_row(unsigned q0, unsigned q1, const unsigned* __rest
     for (int r = 0; r < B; r++) acc[r] += __shfl_xor(acc[r], o, TPR);
 }
 
-template <int B, typename T, typename A, typename VX>
-__device__ inline void dia_row(const T* __restrict__ dia, const VX* __restrict__ x, int i, A (&acc)[B]) {
+// x: a vector pointer, or any value indexable like one (SumVec)
+template <int B, typename T, typename A, typename VX, typename XP = const VX*>
+__device__ inline void dia_row(const T* __restrict__ dia, XP x, int i, A (&acc)[B]) {
   const T* d = dia + (size_t)i * (B * B);
   A xi[B];
 #pragma unroll
@@ -161,9 +162,9 @@ __device__ inline void dia_row(const T* __restrict__ dia, const VX* __restrict__
 // entries' columns and values are loaded together, then their x gathered, then accumulated. Padding entries
 // (value 0, column 0) add exact zeros. The caller adds the diagonal block last, as MatVec does
 // (src/matvec/matdia_blkcrs.cpp:1008-1025).
-template <int B, int CH, typename T, typename A, typename VX>
+template <int B, int CH, typename T, typename A, typename VX, typename XP = const VX*>
 __device__ inline void sell_row(const unsigned* __restrict__ col, const T* __restrict__ val, int r0, int w, int lane,
-                                const VX* __restrict__ x, A (&acc)[B]) {
+                                XP x, A (&acc)[B]) {
 #pragma unroll
   for (int r = 0; r < B; r++) acc[r] = A(0);
   for (int k0 = 0; k0 < w; k0 += CH) {
@@ -194,6 +195,13 @@ __device__ inline void sell_row(const unsigned* __restrict__ col, const T* __res
         }
   }
 }
+
+// x_hi + x_lo (FP64 solution plus FP32 partial solution) read like one FP64 vector by the SpMV helpers
+struct SumVec {
+  const double* __restrict__ hi;
+  const float* __restrict__ lo;
+  __device__ double operator[](size_t i) const { return hi[i] + (double)lo[i]; }
+};
 
 template <int B>
 struct SellChunk {

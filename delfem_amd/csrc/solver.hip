@@ -182,11 +182,70 @@ __global__ void __launch_bounds__(64 * kSellWaves)
   if (REL && rel) produce<64 * kSellWaves, 1>(slot_true, vt);
 }
 
+// The mixed loop's SpMV with the direction update folded in: p = beta p + z (solver_ls_iter.cpp:160-166) and
+// q = A p formed as beta q + A z, which is A (beta p + z) by linearity (the rounding differs from forming A p), so
+// the SpMV gathers z alone and no separate pass over p is launched. Partials of (p, q). When a reliable update is
+// due: r = b - A (x_hi + x_lo) in FP64 (the forward phases then fold x_lo into x_hi). q and p are read and written
+// only for the lane's own rows; z, x_hi and x_lo are not written here.
+template <int B, typename T, bool REL>
+__global__ void __launch_bounds__(64 * kSellWaves)
+    k_pcg_spmv_zq(int n, int npos, const int* __restrict__ visit, const int* __restrict__ sptr,
+                  const unsigned* __restrict__ col, const T* __restrict__ val, const T* __restrict__ dia,
+                  const T* __restrict__ z, T* __restrict__ p, T* __restrict__ q, const double* __restrict__ sc,
+                  double* slot, const double* __restrict__ val64, const double* __restrict__ dia64,
+                  const double* __restrict__ x64, const float* __restrict__ xlo, const double* __restrict__ b64,
+                  T* __restrict__ r, double* slot_true, int* fl) {
+  constexpr int CH = SellChunk<B>::CH;
+  if (fl[F_DONE]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) fl[F_IT] = fl[F_IT] + 1;
+  const bool rel = REL && fl[F_PCUR];
+  const T bt = (T)sc[S_BETA];
+  const int wave = wave_in_block(), lane = threadIdx.x % 64;
+  double v[1] = {0.0}, vt[1] = {0.0};
+  for (int pos = (int)blockIdx.x * kSellWaves + wave; pos < npos; pos += (int)gridDim.x * kSellWaves) {
+    const int sl = visit[pos];
+    if (sl < 0) continue;
+    const int r0 = sptr[sl], w = sptr[sl + 1] - r0;
+    const int row = sl * 64 + lane;
+    T acc[B];
+    sell_row<B, CH, T, T, T>(col, val, r0, w, lane, z, acc);
+    if (row < n) {
+      dia_row<B, T, T, T>(dia, z, row, acc);
+#pragma unroll
+      for (int k = 0; k < B; k++) {
+        const size_t i = (size_t)row * B + k;
+        const T pn = p[i] * bt + z[i];
+        const T qn = q[i] * bt + acc[k];
+        p[i] = pn;
+        q[i] = qn;
+        v[0] += (double)pn * (double)qn;
+      }
+    }
+    if (REL && rel) {
+      const SumVec xs{x64, xlo};
+      double a64[B];
+      sell_row<B, CH, double, double, double>(col, val64, r0, w, lane, xs, a64);
+      if (row < n) {
+        dia_row<B, double, double, double>(dia64, xs, row, a64);
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          const double t = b64[(size_t)row * B + k] - a64[k];
+          r[(size_t)row * B + k] = (T)t;
+          vt[0] += t * t;
+        }
+      }
+    }
+  }
+  produce<64 * kSellWaves, 1>(slot, v);
+  if (REL && rel) produce<64 * kSellWaves, 1>(slot_true, vt);
+}
+
 // ---- PCG iteration with the vector work folded into the sweeps ---------------------------------------------
 // The update r -= alpha q, x += alpha p touches each row once, so every forward phase does it for its own rows
 // (solver_ls_iter.cpp:139-144) and yields the partials of (r, r); every backward phase yields the partials
-// of (r, z) for the rows it finalises. One-workgroup kernels form alpha, and the judgement with beta, from the
-// partials. Sweep partials: one per workgroup, phase after phase.
+// of (r, z) for the rows it finalises. The first forward phase forms alpha from the SpMV's partials; a
+// one-workgroup kernel forms the judgement with beta from the sweep partials. Sweep partials: one per workgroup,
+// phase after phase.
 
 // One-workgroup reductions of per-workgroup partials: every thread sums a fixed strided subset of each list with
 // four independent accumulators (loads in flight together), then the workgroup sums; the order is fixed, so the
@@ -211,14 +270,36 @@ __device__ inline void sum_partials(const double* const (&slot)[K], const int (&
   block_allsum<kRedNT, K>(tot);
 }
 
-// alpha = rho / (p, q)   (solver_ls_iter.cpp:137-138)
-__global__ void __launch_bounds__(kRedNT) k_pcg_alpha(const double* slot_pq, int G, double* sc, const int* fl) {
-  if (fl[F_DONE]) return;
-  const double* const sl[1] = {slot_pq};
-  const int g[1] = {G};
-  double pq[1];
-  sum_partials<1>(sl, g, pq);
-  if (threadIdx.x == 0) sc[S_ALPHA] = sc[S_RZ + (fl[F_IT] & 1)] / pq[0];
+// alpha = rho / (p, q) (solver_ls_iter.cpp:137-138) inside the first forward phase: every workgroup sums the
+// SpMV's (p, q) partials in the same fixed order (so all hold the same alpha), workgroup 0 stores it for the later
+// phases. Saves the launch of a separate reduction kernel between the SpMV and the sweeps.
+__device__ inline double phase_alpha(const double* __restrict__ slot_pq, int G, double* sc, const int* fl) {
+  __shared__ double red[kWPB];
+  double s0 = 0.0, s1 = 0.0;
+  int b = (int)threadIdx.x;
+  for (; b + 64 * kWPB < G; b += 2 * 64 * kWPB) {
+    s0 += slot_pq[b];
+    s1 += slot_pq[b + 64 * kWPB];
+  }
+  if (b < G) s0 += slot_pq[b];
+  const double v = wave_sum(s0 + s1);
+  if (threadIdx.x % 64 == 0) red[wave_in_block()] = v;
+  __syncthreads();
+  double pq = red[0];
+#pragma unroll
+  for (int k = 1; k < kWPB; k++) pq += red[k];
+  const double alpha = sc[S_RZ + (fl[F_IT] & 1)] / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHA] = alpha;
+  return alpha;
+}
+
+// x_lo of a row before alpha p is added; when a reliable update is due (the SpMV has just formed the true residual
+// from x_hi + x_lo) x_lo is folded into the FP64 solution x_hi first and restarts from 0.
+template <typename T>
+__device__ inline T fold_xlo(double* xhi, bool fold, const T* x, size_t i) {
+  if (!fold) return x[i];
+  xhi[i] += (double)x[i];
+  return T(0);
 }
 
 // forward phase with the update of its rows: r_i -= alpha q_i, x_i += alpha p_i, then z = L-sweep of r; the new
@@ -228,14 +309,16 @@ __global__ void __launch_bounds__(kRedNT) k_pcg_alpha(const double* slot_pq, int
 template <int B, int MO, int MI, typename TV, typename T>
 __global__ void __launch_bounds__(64 * kWPB)
     k_fwd_update(int gbase, int gend, SlabArgs sa, const TV* __restrict__ val, const TV* __restrict__ dval, T* r,
-                 const T* q, const T* p, T* x, T* z, const double* sc, const int* fl, double* slot_rr) {
+                 const T* q, const T* p, T* x, T* z, double* sc, const int* fl, double* slot_rr,
+                 const double* __restrict__ slot_pq, int Gpq, double* xhi) {
   __shared__ T xsb[kWPB][64 * B];
   if (fl[F_DONE]) return;
   const int w = wave_in_block();
   const int g = gbase + (int)blockIdx.x * kWPB + w;
   double rr = 0.0;
+  const double alpha = slot_pq ? phase_alpha(slot_pq, Gpq, sc, fl) : sc[S_ALPHA];
+  const bool fold = xhi && fl[F_PCUR];
   if (g < gend) {
-    const double alpha = sc[S_ALPHA];
     const T ma = (T)(-alpha), pa = (T)alpha;
     T rn[B], xn[B];
     SlabSlot ss;
@@ -247,7 +330,7 @@ __global__ void __launch_bounds__(64 * kWPB)
           for (int k = 0; k < B; k++) {
             const size_t i = (size_t)s.row * B + k;
             rn[k] = r[i] + ma * q[i];
-            xn[k] = x[i] + pa * p[i];
+            xn[k] = fold_xlo(xhi, fold, x, i) + pa * p[i];
             u[k] = rn[k];
           }
         },
@@ -299,15 +382,17 @@ template <int B, int MO, int MI, typename TV, typename T>
 __global__ void __launch_bounds__(64 * kWPB)
     k_fwd_bwd_tile(int gbase, int gend, SlabArgs fa, const TV* __restrict__ fval, const TV* __restrict__ fdval,
                    SlabArgs ba, const TV* __restrict__ bval, const int* __restrict__ bgrp, T* r, const T* q,
-                   const T* p, T* x, T* z, const double* sc, const int* fl, double* slot_rr, double* slot_rz) {
+                   const T* p, T* x, T* z, double* sc, const int* fl, double* slot_rr, double* slot_rz,
+                   const double* __restrict__ slot_pq, int Gpq, double* xhi) {
   __shared__ T xsb[kWPB][64 * B];
   if (fl[F_DONE]) return;
   const int w = wave_in_block();
   const int k = (int)blockIdx.x * kWPB + w;
   const int g = gbase + k;
   double v[2] = {0.0, 0.0};
+  const double alpha = slot_pq ? phase_alpha(slot_pq, Gpq, sc, fl) : sc[S_ALPHA];
+  const bool fold = xhi && fl[F_PCUR];
   if (g < gend) {
-    const double alpha = sc[S_ALPHA];
     const T ma = (T)(-alpha), pa = (T)alpha;
     T rn[B], xn[B];
     SlabSlot ss;
@@ -319,7 +404,7 @@ __global__ void __launch_bounds__(64 * kWPB)
           for (int c = 0; c < B; c++) {
             const size_t i = (size_t)s.row * B + c;
             rn[c] = r[i] + ma * q[i];
-            xn[c] = x[i] + pa * p[i];
+            xn[c] = fold_xlo(xhi, fold, x, i) + pa * p[i];
             u[c] = rn[c];
           }
         },
@@ -689,9 +774,10 @@ static int sweep_blocks(const Sweep& w) {
 
 template <typename T, typename TV>
 static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const TV* fdval, const TV* bval,
-                               double* part_rr, double* part_rz, hipStream_t st) {
+                               double* part_rr, double* part_rz, const double* slot_pq, int Gpq, double* xhi,
+                               hipStream_t st) {
   int* fl = s.d_iflag.p;
-  const double* sc = s.d_scal.p;
+  double* sc = s.d_scal.p;
   T* r = (T*)v.r;
   T* z = (T*)v.z;
   const Sweep& f = s.fwd;
@@ -704,6 +790,9 @@ static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const T
     const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0, nb = (ng + kWPB - 1) / kWPB;
     if (!ng) continue;
     const bool narrow = sizeof(T) == 8 && sizeof(TV) == 8;
+    // the first launched forward phase forms alpha from the SpMV partials
+    const double* apq = slot_pq;
+    slot_pq = nullptr;
     if (fuse && p == f.nphase() - 1) {
       const int wo = std::max(f.phase_wo[p], b.phase_wo[0]), wi = std::max(f.phase_wi[p], b.phase_wi[0]);
 #define DFM_FB(B)                                                                                                  \
@@ -711,7 +800,7 @@ static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const T
     with_slab_caps(wo, wi, B == 3 && narrow, [&]<int MO, int MI>() {                                               \
       hipLaunchKernelGGL((k_fwd_bwd_tile<B, MO, MI, TV, T>), dim3(nb), dim3(64 * kWPB), 0, st, g0, g0 + ng, fa, fval, \
                          fdval, ba, bval, (const int*)f.d_fuse_bgrp.p, r, (const T*)v.q, (const T*)v.p, (T*)v.x, z,  \
-                         sc, fl, part_rr + off, part_rz);                                                          \
+                         sc, fl, part_rr + off, part_rz, apq, Gpq, xhi);                                           \
     });
       DFM_FB(1) DFM_FB(2) DFM_FB(3)
 #undef DFM_FB
@@ -722,7 +811,7 @@ static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const T
   if (s.len == B)                                                                                                    \
     with_slab_caps(f.phase_wo[p], f.phase_wi[p], B == 3 && narrow, [&]<int MO, int MI>() {                           \
       hipLaunchKernelGGL((k_fwd_update<B, MO, MI, TV, T>), dim3(nb), dim3(64 * kWPB), 0, st, g0, g0 + ng, fa, fval,  \
-                         fdval, r, (const T*)v.q, (const T*)v.p, (T*)v.x, z, sc, fl, part_rr + off);                 \
+                         fdval, r, (const T*)v.q, (const T*)v.p, (T*)v.x, z, sc, fl, part_rr + off, apq, Gpq, xhi);  \
     });
     DFM_FU(1) DFM_FU(2) DFM_FU(3)
 #undef DFM_FU
@@ -750,6 +839,23 @@ static hipError_t fused_sweeps(Sys& s, const PcgBufs& v, const TV* fval, const T
   return hipGetLastError();
 }
 
+static hipError_t pcg_spmv_zq(Sys& s, const PcgBufs& v, unsigned& G, hipStream_t st) {
+  const int npos = (int)s.sell.visit.size();
+  G = (unsigned)std::min<long long>(kMaxPartial, npos / kSellWaves);
+#define DFM_ZQ(B)                                                                                                 \
+  if (s.len == B) {                                                                                               \
+    hipLaunchKernelGGL((k_pcg_spmv_zq<B, float, true>), dim3(G), dim3(64 * kSellWaves), 0, st, s.n, npos,         \
+                       s.sell.d_visit.p, s.sell.d_sptr.p, s.sell.d_col.p, s.sell.d_val32.p, (const float*)v.dia,   \
+                       (const float*)v.z, (float*)v.p, (float*)v.q, (const double*)s.d_scal.p, slot(s, P_A),       \
+                       s.sell.d_val.p, s.d_dia.p, v.x64, (const float*)v.x, v.b64, (float*)v.r, slot(s, P_C),      \
+                       s.d_iflag.p);                                                                              \
+    return hipGetLastError();                                                                                     \
+  }
+  DFM_ZQ(1) DFM_ZQ(2) DFM_ZQ(3)
+#undef DFM_ZQ
+  return hipErrorInvalidValue;
+}
+
 template <typename T>
 static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
   const long long m = (long long)s.n * s.len;
@@ -765,25 +871,34 @@ static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
   double* part_rr = s.d_part_sw.p;
   double* part_rz = s.d_part_sw.p + nbf;
   hipError_t e;
-  if (mixed)
+  // mixed, scalar systems over the SELL copy: the direction update rides in the SpMV and x_lo is folded in the
+  // forward phases (for 2x2 and 3x3 blocks the strided own-row traffic costs the SpMV more than the separate pass)
+  const bool zq = mixed && s.sell.nslice() > 0 && s.len == 1;
+  if (zq)
+    e = pcg_spmv_zq(s, v, Gs, st);
+  else if (mixed)
     e = pcg_spmv<T, true>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, v.x64, v.b64, r, slot(s, P_C), st);
   else
     e = pcg_spmv<T, false>(s, (const T*)v.crs, (const T*)v.dia, p, q, slot(s, P_A), Gs, nullptr, nullptr, nullptr,
                            nullptr, st);
   if (e != hipSuccess) return DFM_ERR_HIP;
-  hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedNT), 0, st, (const double*)slot(s, P_A), (int)Gs, sc, (const int*)fl);
+  const double* pq = slot(s, P_A);
   if (mixed)
-    e = fused_sweeps<float, float>(s, v, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, part_rr, part_rz, st);
+    e = fused_sweeps<float, float>(s, v, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, part_rr, part_rz, pq,
+                                   (int)Gs, zq ? v.x64 : nullptr, st);
   else if (v.f32ilu)
-    e = fused_sweeps<double, float>(s, v, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, part_rr, part_rz, st);
+    e = fused_sweeps<double, float>(s, v, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, part_rr, part_rz, pq,
+                                    (int)Gs, nullptr, st);
   else
-    e = fused_sweeps<double, double>(s, v, s.fwd.d_eval.p, s.fwd.d_dval.p, s.bwd.d_eval.p, part_rr, part_rz, st);
+    e = fused_sweeps<double, double>(s, v, s.fwd.d_eval.p, s.fwd.d_dval.p, s.bwd.d_eval.p, part_rr, part_rz, pq,
+                                     (int)Gs, nullptr, st);
   if (e != hipSuccess) return DFM_ERR_HIP;
   if (mixed) {
     hipLaunchKernelGGL(k_pcg_judge_beta<true>, dim3(1), dim3(kRedNT), 0, st, (const double*)part_rr, nbf,
                        (const double*)slot(s, P_C), (int)Gs, (const double*)part_rz, nbb, sc, fl);
-    hipLaunchKernelGGL((k_pcg_pupdate<T, true>), vg, dim3(NT), 0, st, m, p, (const T*)z, (const double*)sc,
-                       (const int*)fl, (float*)v.x, v.x64);
+    if (!zq)
+      hipLaunchKernelGGL((k_pcg_pupdate<T, true>), vg, dim3(NT), 0, st, m, p, (const T*)z, (const double*)sc,
+                         (const int*)fl, (float*)v.x, v.x64);
   } else {
     hipLaunchKernelGGL(k_pcg_judge_beta<false>, dim3(1), dim3(kRedNT), 0, st, (const double*)part_rr, nbf,
                        (const double*)nullptr, 0, (const double*)part_rz, nbb, sc, fl);
@@ -865,6 +980,8 @@ static int pcg_start_T(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigne
                        : precond_apply_d(s, v, (const double*)r, (double*)z, fl, st);
   if (e != hipSuccess) return DFM_ERR_HIP;
   hipLaunchKernelGGL(k_vcopy<T>, vg, dim3(NT), 0, st, (const T*)z, p, m, fl);
+  // the mixed loop forms q as beta q + A z (beta = 0 in the first iteration): q starts finite
+  if (mixed) hipLaunchKernelGGL(k_vfill<T>, vg, dim3(NT), 0, st, (T*)v.q, (T)0, m, (const int*)nullptr);
   hipLaunchKernelGGL(k_vdot<T>, vg, dim3(NT), 0, st, (const T*)r, (const T*)z, m, slot(s, P_A), fl);
   hipLaunchKernelGGL(k_pcg_rho0, dim3(1), dim3(NT), 0, st, slot(s, P_A), G, sc, fl);
   DFM_HIP(hipGetLastError());
@@ -1153,14 +1270,22 @@ int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes) {
   float* y32 = s.d_work32.p + m;
   hipLaunchKernelGGL(k_mx_to32, vgrid(m), dim3(NT), 0, st, m, s.d_rhs.p, x32);
   unsigned G = 0;
-  // 4, 5: the sweep phases of the PCG iteration (forward phases with the r/x update and (r, r), backward phases
-  // with (r, z)) in the mixed (FP32) and FP64 loops, on the solve's own buffers (alpha = 0 after the reset above:
-  // r and x keep their values)
+  // 4, 5: the sweep phases of the PCG iteration (forward phases with alpha and the r/x update and (r, r), backward
+  // phases with (r, z)) in the mixed (FP32) and FP64 loops, on the solve's own buffers (alpha = 0 after the reset
+  // above: r and x keep their values)
   const PcgBufs v32 = pcg_bufs(s, true), v64 = pcg_bufs(s, false);
   const size_t need = (size_t)(sweep_blocks(s.fwd) + sweep_blocks(s.bwd));
   if ((which == 4 || which == 5) && s.d_part_sw.n < need) DFM_HIP(s.d_part_sw.alloc(need));
   double* prr = s.d_part_sw.p;
   double* prz = s.d_part_sw.p + sweep_blocks(s.fwd);
+  // the first forward phase forms alpha from SpMV partials: one SpMV provides them ((x, Ax) > 0, so alpha = 0)
+  unsigned Gpq = 0;
+  if (which == 4 && pcg_spmv<float, false>(s, s.d_crs32.p, s.d_dia32.p, x32, y32, slot(s, P_A), Gpq, nullptr, nullptr,
+                                           nullptr, nullptr, st) != hipSuccess)
+    return DFM_ERR_HIP;
+  if (which == 5 && pcg_spmv<double, false>(s, s.d_crs.p, s.d_dia.p, s.d_rhs.p, s.d_work.p, slot(s, P_A), Gpq, nullptr,
+                                            nullptr, nullptr, nullptr, st) != hipSuccess)
+    return DFM_ERR_HIP;
   auto launch = [&]() -> hipError_t {
     switch (which) {
       case 0: return pcg_spmv<double, false>(s, s.d_crs.p, s.d_dia.p, s.d_rhs.p, s.d_work.p, slot(s, P_A), G, nullptr, nullptr, nullptr, nullptr, st);
@@ -1168,9 +1293,11 @@ int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes) {
       case 2: return precond_apply64(s, s.d_rhs.p, s.d_work.p, fl, st);
       case 3: return precond_apply32(s, x32, y32, fl, st);
       case 4:
-        return fused_sweeps<float, float>(s, v32, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, prr, prz, st);
+        return fused_sweeps<float, float>(s, v32, s.fwd.d_eval32.p, s.fwd.d_dval32.p, s.bwd.d_eval32.p, prr, prz,
+                                          slot(s, P_A), (int)Gpq, nullptr, st);
       default:
-        return fused_sweeps<double, double>(s, v64, s.fwd.d_eval.p, s.fwd.d_dval.p, s.bwd.d_eval.p, prr, prz, st);
+        return fused_sweeps<double, double>(s, v64, s.fwd.d_eval.p, s.fwd.d_dval.p, s.bwd.d_eval.p, prr, prz,
+                                            slot(s, P_A), (int)Gpq, nullptr, st);
     }
   };
   for (int w = 0; w < 3; w++)
