@@ -82,8 +82,9 @@ __global__ void __launch_bounds__(64 * kBSellWaves) k_b_spmv(const SegDev* __res
   constexpr bool ZQ = REL && B == 1;
   const int b = blockIdx.x;
   const SegDev& d = segs[bseg[b]];
-  const int* fl = d.fl;
+  int* fl = d.fl;
   if (fl[F_DONE]) return;
+  if (b == d.ps0 && threadIdx.x == 0) fl[F_IT] = fl[F_IT] + 1;  // the segment's iteration counter
   const bool rel = REL && fl[F_PCUR];
   const int wave = wave_in_block(), lane = (int)(threadIdx.x % 64);
   const int sl = (b - d.ps0) * kBSellWaves + wave;
@@ -134,17 +135,6 @@ __global__ void __launch_bounds__(64 * kBSellWaves) k_b_spmv(const SegDev* __res
   if (threadIdx.x == 0) {
     slot_pq[b] = v[0];
     if (REL && rel) slot_true[b] = v[1];
-  }
-}
-
-// per segment: iteration counter, alpha = rho / (p, q)   (solver_ls_iter.cpp:137-138)
-__global__ void __launch_bounds__(NT) k_b_alpha(const SegDev* __restrict__ segs, const double* slot_pq) {
-  const SegDev& d = segs[blockIdx.x];
-  if (d.fl[F_DONE]) return;
-  const double pq = consume_range(slot_pq, d.ps0, d.ps1);
-  if (threadIdx.x == 0) {
-    d.fl[F_IT] = d.fl[F_IT] + 1;
-    d.sc[S_ALPHA] = d.sc[S_RZ] / pq;
   }
 }
 
@@ -222,9 +212,13 @@ __global__ void __launch_bounds__(NT) k_b_judge_beta(const SegDev* __restrict__ 
 // One wavefront per (segment, tile) entry. Forward: the tile rows' r -= alpha q, x += alpha p (registers, stored
 // at the end), the forward sweep from the new r, and the wavefront's (r, r) partial at the entry's slot (each
 // segment's slots are contiguous, so its reduction does not depend on the other segments).
+// In the first forward phase (slot_pq given) every wavefront forms its segment's alpha = rho / (p, q)
+// (solver_ls_iter.cpp:137-138) from the segment's SpMV partials, in a fixed order, and the wavefront of the
+// segment's tile 0 stores it for the later phases.
 template <int B, int MO, int MI, typename TF, typename T>
 __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
-                                                     const int* __restrict__ eslot, int count, double* slot_rr) {
+                                                     const int* __restrict__ eslot, int count, double* slot_rr,
+                                                     const double* __restrict__ slot_pq) {
   __shared__ T xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int e = (int)blockIdx.x * kWPB + w;
@@ -232,7 +226,15 @@ __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ 
   const int2 en = ent[e];
   const SegDev& d = segs[en.x];
   if (d.fl[F_DONE]) return;
-  const double alpha = d.sc[S_ALPHA];
+  double alpha;
+  if (slot_pq) {
+    double pq = 0.0;
+    for (int b = d.ps0 + (int)(threadIdx.x % 64); b < d.ps1; b += 64) pq += slot_pq[b];
+    alpha = d.sc[S_RZ] / wave_sum(pq);
+    if (en.y == 0 && threadIdx.x % 64 == 0) d.sc[S_ALPHA] = alpha;
+  } else {
+    alpha = d.sc[S_ALPHA];
+  }
   const T ma = (T)(-alpha), pa = (T)alpha;
   // scalar mixed segments: x is x_lo; at a reliable update it is folded into x_hi before alpha p is added
   const bool fold = B == 1 && sizeof(T) == 4 && d.fl[F_PCUR];
@@ -532,13 +534,13 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
   if (g.nsb)
     hipLaunchKernelGGL((k_b_spmv<B, TM, T, MX>), dim3(g.nsb), dim3(64 * kBSellWaves), 0, st, segs, g.d_sblk.p,
                        g.d_pq.p, g.d_true.p);
-  hipLaunchKernelGGL(k_b_alpha, nseg, dim3(NT), 0, st, segs, (const double*)g.d_pq.p);
   for (size_t p = 0; p + 1 < g.fptr.size(); p++) {
     const int c = g.fptr[p + 1] - g.fptr[p];
     if (c)
       with_slab_caps(g.fwo[p], g.fwi[p], B == 3 && sizeof(T) == 8 && sizeof(TF) == 8, [&]<int MO, int MI>() {
         hipLaunchKernelGGL((k_b_fwd<B, MO, MI, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
-                           (const int2*)g.d_fent.p + g.fptr[p], (const int*)g.d_fslot.p + g.fptr[p], c, g.d_rr.p);
+                           (const int2*)g.d_fent.p + g.fptr[p], (const int*)g.d_fslot.p + g.fptr[p], c, g.d_rr.p,
+                           p == 0 ? (const double*)g.d_pq.p : nullptr);
       });
   }
   for (size_t p = 0; p + 1 < g.bptr.size(); p++) {
