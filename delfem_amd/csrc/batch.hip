@@ -216,9 +216,9 @@ __global__ void __launch_bounds__(NT) k_b_judge_beta(const SegDev* __restrict__ 
 // (solver_ls_iter.cpp:137-138) from the segment's SpMV partials, in a fixed order, and the wavefront of the
 // segment's tile 0 stores it for the later phases.
 template <int B, int MO, int MI, typename TF, typename T>
-__global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
-                                                     const int* __restrict__ eslot, int count, double* slot_rr,
-                                                     const double* __restrict__ slot_pq) {
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<B, MO, MI, T>::kWaves)))
+    k_b_fwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent, const int* __restrict__ eslot, int count,
+        double* slot_rr, const double* __restrict__ slot_pq) {
   __shared__ T xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int e = (int)blockIdx.x * kWPB + w;
@@ -277,8 +277,9 @@ __global__ void __launch_bounds__(64 * kWPB) k_b_fwd(const SegDev* __restrict__ 
 
 // Backward sweep of one (segment, tile) entry and the wavefront's (r, z) partial at the entry's slot.
 template <int B, int MO, int MI, typename TF, typename T>
-__global__ void __launch_bounds__(64 * kWPB) k_b_bwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent,
-                                                     const int* __restrict__ eslot, int count, double* slot_rz) {
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<B, MO, MI, T>::kWaves)))
+    k_b_bwd(const SegDev* __restrict__ segs, const int2* __restrict__ ent, const int* __restrict__ eslot, int count,
+        double* slot_rz) {
   __shared__ T xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int e = (int)blockIdx.x * kWPB + w;

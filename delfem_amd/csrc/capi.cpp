@@ -368,7 +368,7 @@ int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, 
 static void copy_sweep_host(Sweep& d, const Sweep& a) {
   d.phase_gptr = a.phase_gptr; d.phase_nlev = a.phase_nlev; d.grp = a.grp; d.rowlist = a.rowlist; d.lev = a.lev;
   d.phase_wo = a.phase_wo; d.phase_wi = a.phase_wi;
-  d.gmeta = a.gmeta; d.srow = a.srow; d.scnt = a.scnt; d.ecol = a.ecol; d.esrc = a.esrc; d.nslab = a.nslab;
+  d.gmeta = a.gmeta; d.srow = a.srow; d.scnt = a.scnt; d.ecol = a.ecol; d.esrc = a.esrc; d.nent = a.nent;
   d.fuse_bgrp = a.fuse_bgrp;
 }
 
@@ -844,9 +844,10 @@ int dfm_prec_factor(dfm_sys sys) {
   // slab copies of the factor for the sweep kernels
   for (int dir = 0; dir < 2; dir++) {
     Sweep& w = dir == 0 ? s.fwd : s.bwd;
-    const long long ne = w.nslab * 64;
+    const long long ne = w.nent;
     if (w.d_eval.n < (size_t)(ne * s.bs)) DFM_HIP(w.d_eval.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
-    DFM_HIP(launch_slab_fill(s.len, ne, w.d_esrc.p, s.d_fcrs.p, w.d_eval.p, st));
+    DFM_HIP(launch_sweep_fill(s.len, (int)(w.gmeta.size() / 4), w.d_gmeta.p, w.d_scnt.p, w.d_esrc.p, s.d_fcrs.p,
+                              w.d_eval.p, st));
     if (dir == 0) {
       const long long ns = (long long)w.srow.size();
       if (w.d_dval.n < (size_t)(ns * s.bs)) DFM_HIP(w.d_dval.alloc((size_t)ns * s.bs));
@@ -938,9 +939,10 @@ static int ensure_fp32(Sys& s) {
   }
   for (int dir = 0; dir < 2; dir++) {
     Sweep& w = dir == 0 ? s.fwd : s.bwd;
-    const long long ne = w.nslab * 64;
+    const long long ne = w.nent;
     if (w.d_eval32.n < (size_t)(ne * s.bs)) DFM_HIP(w.d_eval32.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
-    DFM_HIP(launch_slab_fill(s.len, ne, w.d_esrc.p, s.d_fcrs.p, w.d_eval32.p, st));
+    DFM_HIP(launch_sweep_fill(s.len, (int)(w.gmeta.size() / 4), w.d_gmeta.p, w.d_scnt.p, w.d_esrc.p, s.d_fcrs.p,
+                              w.d_eval32.p, st));
     if (dir == 0) {
       const long long ns = (long long)w.srow.size();
       if (w.d_dval32.n < (size_t)(ns * s.bs)) DFM_HIP(w.d_dval32.alloc((size_t)ns * s.bs));

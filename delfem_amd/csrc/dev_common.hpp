@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace dfm {
 
@@ -263,21 +264,39 @@ struct SlabArgs {
 };
 
 // Per-slot state common to both sweeps. The group descriptor is wave-uniform (scalar loads); every vector load
-// below depends only on the slot, so all of them are in flight together. Padding entries of a slab row hold
-// column 0 and value 0 and empty lanes map to row 0: loads are unconditional under the wave-uniform widths,
-// only the arithmetic is predicated by the lane's own counts.
+// below depends only on the slot, so all of them are in flight together. bo / bi: first entry of the group's
+// outside and in-group classes. Empty lanes map to row 0 and have no entries.
 struct SlabSlot {
-  int g, lane, off, wo, wi, nlev, row, no, ni, lv;
+  int g, lane, bo, bi, wo, wi, nlev, row, no, ni, lv;
   bool active;
 };
+
+// Exclusive prefix sum over the wavefront of 8-bit lane values, from one ballot per bit (all lanes of the
+// wavefront call it).
+__device__ inline int wave_scan_u8(unsigned v) {
+  int s = 0;
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    const unsigned long long m = __ballot((v >> b) & 1u);
+    s += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
+  }
+  return s;
+}
+
+// Lanes below the calling one in mask m.
+__device__ inline int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
 
 // g: the group (wave-uniform); returns false when the system's done flag is set.
 __device__ inline bool slab_slot(const SlabArgs& sa, int g, const int* done, SlabSlot& ss) {
   ss.g = g;
   const int* m = sa.gmeta + 4 * (size_t)g;
-  const int m0 = m[0], m1 = m[1], row0 = m[2], nrows = m[3];
+  const int m0 = m[0], m1 = m[1], row0 = m[2], m3 = m[3];
   if (done && *done) return false;
-  ss.off = m0;
+  const int nrows = m3 & 255;
+  ss.bo = m0;
+  ss.bi = m0 + (m3 >> 8);
   ss.wo = m1 & 255;
   ss.wi = (m1 >> 8) & 255;
   ss.nlev = m1 >> 16;
@@ -294,46 +313,122 @@ __device__ inline bool slab_slot(const SlabArgs& sa, int g, const int* done, Sla
 
 __device__ inline int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64)); }
 
-template <int B, typename T, typename A>
-__device__ inline void slab_load_blk(const T* __restrict__ val, size_t srow_, int lane, A (&a)[B * B]) {
-#pragma unroll
-  for (int e = 0; e < B * B; e++) a[e] = (A)val[(srow_ * (B * B) + e) * 64 + lane];
-}
+// Entries per lane and class stored in jagged rows. A class of a group's entries (see Sweep) starts with its
+// jagged rows: row k < kJag holds the k-th entry of every lane that has one, in lane order, with the row's b*b
+// values element-major (value e of the row's p-th entry at (row start)*bs + e*(row length) + p), so a wavefront
+// reads a row as one contiguous segment per value; a lane finds its place in the row from a ballot of the lanes'
+// counts. Entries k >= kJag follow, lane after lane, each entry's values together: a lane finds them from its
+// first one alone, also inside branches that only part of the wavefront takes.
+constexpr int kJag = 4;
 
-// Loads of the register-held entries of one class: MX slab rows starting at r0, wave-uniform count w.
-template <int B, int MX, typename T, typename A>
-__device__ inline void slab_load_class(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int w,
-                                       int lane, unsigned (&col)[MX], T (&v)[MX][B * B]) {
+// Where the lane's class entries k >= kJag lie.
+struct ClassTail {
+  int n;    // the lane's count
+  int csr;  // the lane's first entry with k >= kJag
+};
+
+// How a lane treats register-held entries it does not have (measured choice): with 1x1 blocks it loads the first
+// entry of the row (or of the class) anyway, so that every load is unconditional and in flight at once; with
+// larger blocks, where such loads would fetch b*b values each, it loads nothing and holds column 0 and zeros.
+// Either way what it holds is valid to use, and the arithmetic, predicated by its count, discards it.
+template <int B>
+struct SlabLoads {
+  static constexpr bool kAll = B == 1;
+  using Idx = std::conditional_t<B == 1, unsigned, size_t>;  // entry index arithmetic (32-bit keeps 1x1 lean)
+};
+
+// Whole wavefront: loads the register-held entries k < MX of one class (base: its first entry; w: the
+// wave-uniform width; n: the lane's count).
+template <int B, int MX, typename T>
+__device__ inline void slab_load_class(const unsigned* __restrict__ ecol, const T* __restrict__ val, int base, int w,
+                                       int n, unsigned (&col)[MX], T (&v)[MX][B * B], ClassTail& ct) {
+  static_assert(MX >= kJag, "the jagged rows are register-held");
+  constexpr int BS = B * B;
+  constexpr bool all = SlabLoads<B>::kAll;
+  if (!all) {
 #pragma unroll
-  for (int k = 0; k < MX; k++)
+    for (int k = 0; k < MX; k++)
+      if (k < w) {
+        col[k] = 0u;
+#pragma unroll
+        for (int q = 0; q < BS; q++) v[k][q] = T(0);
+      }
+  }
+  int r = base;
+#pragma unroll
+  for (int k = 0; k < kJag; k++)
     if (k < w) {
-      col[k] = ecol[(size_t)(r0 + k) * 64 + lane];
-      slab_load_blk<B, T, T>(val, (size_t)(r0 + k), lane, v[k]);
+      const unsigned long long m = __ballot(n > k);
+      const unsigned len = (unsigned)__popcll(m), pos = k < n ? (unsigned)lanes_below(m) : 0u;
+      if (all) {
+        // a wave-uniform row base plus a 32-bit lane offset
+        const unsigned* crow = ecol + r;
+        const T* vrow = val + (size_t)r * BS;
+        col[k] = crow[pos];
+#pragma unroll
+        for (int q = 0; q < BS; q++) v[k][q] = vrow[(unsigned)q * len + pos];
+      } else if (k < n) {
+        col[k] = ecol[(size_t)r + pos];
+#pragma unroll
+        for (int q = 0; q < BS; q++) v[k][q] = val[(size_t)r * BS + (size_t)q * len + pos];
+      }
+      r += (int)len;
+    }
+  ct.n = n;
+  ct.csr = r + (w > kJag ? wave_scan_u8(n > kJag ? (unsigned)(n - kJag) : 0u) : 0);
+  using Idx = typename SlabLoads<B>::Idx;
+#pragma unroll
+  for (int k = kJag; k < MX; k++)
+    if (k < w && (all || k < n)) {
+      const Idx e = k < n ? (Idx)ct.csr + (Idx)(k - kJag) : (Idx)base;
+      col[k] = ecol[e];
+#pragma unroll
+      for (int q = 0; q < BS; q++) v[k][q] = val[e * (Idx)BS + (Idx)q];
     }
 }
 
-template <int B, int MX, typename A, typename VX>
-__device__ inline void slab_gather(const VX* x, int w, const unsigned (&col)[MX], A (&xo)[MX][B]) {
+// Calls f(column, values) for the lane's class entries k = MX .. n-1 in ascending k (safe in divergent code).
+template <int B, int MX, typename T, typename A, typename F>
+__device__ inline void class_tail(const unsigned* __restrict__ ecol, const T* __restrict__ val, const ClassTail& ct,
+                                  F&& f) {
+  constexpr int BS = B * B;
+  using Idx = typename SlabLoads<B>::Idx;
+  for (int k = MX; k < ct.n; k++) {
+    const Idx e = (Idx)ct.csr + (Idx)(k - kJag);
+    A a[BS];
 #pragma unroll
-  for (int k = 0; k < MX; k++)
-    if (k < w)
-#pragma unroll
-      for (int c = 0; c < B; c++) xo[k][c] = (A)x[(size_t)col[k] * B + c];
+    for (int q = 0; q < BS; q++) a[q] = (A)val[e * (Idx)BS + (Idx)q];
+    f(ecol[e], a);
+  }
 }
 
-// t -= sum over entries [MX, n) of the class starting at slab row r0, values x from global memory (final)
+// x of the register-held outside entries (as slab_load_class: all of them with 1x1 blocks, else those the lane
+// has, zeros for the others)
+template <int B, int MX, typename A, typename VX>
+__device__ inline void slab_gather(const VX* x, int w, int n, const unsigned (&col)[MX], A (&xo)[MX][B]) {
+  constexpr bool all = SlabLoads<B>::kAll;
+#pragma unroll
+  for (int k = 0; k < MX; k++)
+    if (k < w) {
+      if (!all)
+#pragma unroll
+        for (int c = 0; c < B; c++) xo[k][c] = A(0);
+      if (all || k < n)
+#pragma unroll
+        for (int c = 0; c < B; c++) xo[k][c] = (A)x[(size_t)col[k] * B + c];
+    }
+}
+
+// t -= sum over the lane's outside entries beyond the register-held ones, x from global memory (final)
 template <int B, int MX, typename T, typename A, typename VX>
-__device__ inline void slab_tail_global(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int n,
-                                        int lane, const VX* x, A (&t)[B]) {
-  for (int k = MX; k < n; k++) {
-    const size_t sr = (size_t)(r0 + k);
-    const unsigned j = ecol[sr * 64 + lane];
-    A a[B * B], v[B];
-    slab_load_blk<B, T, A>(val, sr, lane, a);
+__device__ inline void slab_tail_global(const unsigned* __restrict__ ecol, const T* __restrict__ val,
+                                        const ClassTail& ct, const VX* x, A (&t)[B]) {
+  class_tail<B, MX, T, A>(ecol, val, ct, [&](unsigned j, const A (&a)[B * B]) {
+    A v[B];
 #pragma unroll
     for (int c = 0; c < B; c++) v[c] = (A)x[(size_t)j * B + c];
     blk_sub_reg<B, A>(t, a, v);
-  }
+  });
 }
 
 // t -= v y where the lane's own count says the entry exists (k < n); a select, not a branch, so the
@@ -351,11 +446,11 @@ __device__ inline void blk_sub_if(bool on, A (&t)[B], const TV (&a)[B * B], cons
 }
 
 // t -= sum over the in-group entries of the lane (values of this wavefront's rows from LDS), ascending. All
-// reads of the wave-uniform width w are issued together (padding columns point at lane 0).
+// reads of the register-held entries are issued together (every column held is a valid lane).
 template <int B, int MX, typename T, typename A>
-__device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T* __restrict__ val, int r0, int w, int n,
-                                     int lane, const unsigned (&col)[MX], const T (&v)[MX][B * B], const A* xs,
-                                     A (&t)[B]) {
+__device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T* __restrict__ val, int w, int n,
+                                     const unsigned (&col)[MX], const T (&v)[MX][B * B], const ClassTail& ct,
+                                     const A* xs, A (&t)[B]) {
   A y[MX][B];
 #pragma unroll
   for (int k = 0; k < MX; k++)
@@ -365,15 +460,12 @@ __device__ inline void slab_in_group(const unsigned* __restrict__ ecol, const T*
 #pragma unroll
   for (int k = 0; k < MX; k++)
     if (k < w) blk_sub_if<B, A, T>(k < n, t, v[k], y[k]);
-  for (int k = MX; k < n; k++) {
-    const size_t sr = (size_t)(r0 + k);
-    const unsigned j = ecol[sr * 64 + lane];
-    A a[B * B], z[B];
-    slab_load_blk<B, T, A>(val, sr, lane, a);
+  class_tail<B, MX, T, A>(ecol, val, ct, [&](unsigned j, const A (&a)[B * B]) {
+    A z[B];
 #pragma unroll
     for (int c = 0; c < B; c++) z[c] = xs[j * B + c];
     blk_sub_reg<B, A>(t, a, z);
-  }
+  });
 }
 
 // Forward sweep of group g by the calling wavefront: x = D^-1 (b - L x) for the group's rows. `rhs(ss, t)`
@@ -393,16 +485,17 @@ __device__ inline bool sweep_fwd_core(const SlabArgs& sa, int g, const T* __rest
   rhs(ss, t);
 #pragma unroll
   for (int e = 0; e < BS; e++) D[e] = (A)dval[((size_t)ss.g * BS + e) * 64 + ss.lane];
-  slab_load_class<B, MO, T, A>(sa.ecol, val, ss.off, ss.wo, ss.lane, jo, vo);
-  slab_load_class<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.lane, li, vi);
-  slab_gather<B, MO, A, VO>(x, ss.wo, jo, xo);
+  ClassTail to, ti;
+  slab_load_class<B, MO, T>(sa.ecol, val, ss.bo, ss.wo, ss.no, jo, vo, to);
+  slab_load_class<B, MI, T>(sa.ecol, val, ss.bi, ss.wi, ss.ni, li, vi, ti);
+  slab_gather<B, MO, A, VO>(x, ss.wo, ss.no, jo, xo);
 #pragma unroll
   for (int k = 0; k < MO; k++)
     if (k < ss.wo) blk_sub_if<B, A, T>(k < ss.no, t, vo[k], xo[k]);
-  slab_tail_global<B, MO, T, A, VO>(sa.ecol, val, ss.off, ss.no, ss.lane, x, t);
+  slab_tail_global<B, MO, T, A, VO>(sa.ecol, val, to, x, t);
   for (int l = 0; l < ss.nlev; l++) {
     if (ss.lv == l) {
-      slab_in_group<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.ni, ss.lane, li, vi, xs, t);
+      slab_in_group<B, MI, T, A>(sa.ecol, val, ss.wi, ss.ni, li, vi, ti, xs, t);
       A y[B];
 #pragma unroll
       for (int r = 0; r < B; r++) {
@@ -452,16 +545,17 @@ __device__ inline bool sweep_bwd_core(const SlabArgs& sa, int g, const T* __rest
   A xo[MO][B];
   unsigned jo[MO], li[MI];
   init(ss, t);
-  slab_load_class<B, MO, T, A>(sa.ecol, val, ss.off, ss.wo, ss.lane, jo, vo);
-  slab_load_class<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.lane, li, vi);
-  slab_gather<B, MO, A, VX>(x, ss.wo, jo, xo);
+  ClassTail to, ti;
+  slab_load_class<B, MO, T>(sa.ecol, val, ss.bo, ss.wo, ss.no, jo, vo, to);
+  slab_load_class<B, MI, T>(sa.ecol, val, ss.bi, ss.wi, ss.ni, li, vi, ti);
+  slab_gather<B, MO, A, VX>(x, ss.wo, ss.no, jo, xo);
   for (int l = 0; l < ss.nlev; l++) {
     if (ss.lv == l) {
-      slab_in_group<B, MI, T, A>(sa.ecol, val, ss.off + ss.wo, ss.wi, ss.ni, ss.lane, li, vi, xs, t);
+      slab_in_group<B, MI, T, A>(sa.ecol, val, ss.wi, ss.ni, li, vi, ti, xs, t);
 #pragma unroll
       for (int k = 0; k < MO; k++)
         if (k < ss.wo) blk_sub_if<B, A, T>(k < ss.no, t, vo[k], xo[k]);
-      slab_tail_global<B, MO, T, A, VX>(sa.ecol, val, ss.off, ss.no, ss.lane, x, t);
+      slab_tail_global<B, MO, T, A, VX>(sa.ecol, val, to, x, t);
 #pragma unroll
       for (int r = 0; r < B; r++) xs[ss.lane * B + r] = t[r];
     }
@@ -484,11 +578,19 @@ __device__ inline void sweep_bwd_wave(const SlabArgs& sa, int g, const T* __rest
   });
 }
 
+// Wavefronts per SIMD the sweep kernels are compiled for (the register budget follows; T: the arithmetic type):
+// scalar systems are latency-bound and gain from every wavefront in flight (measured: the FP64 PCG phases of a
+// 2D Poisson system at 5 instead of 6 waves took 10 % longer); block systems keep the compiler's choice.
+template <int B, int MO, int MI, typename T>
+struct SweepOccupancy {
+  static constexpr int kWaves = B != 1 ? 1 : sizeof(T) == 4 ? 8 : MO + MI <= 8 ? 6 : 5;
+};
+
 // One phase of one system: groups [gbase, gend), kWPB groups per workgroup.
 template <int B, int MO, int MI, typename T, typename A, typename VI, typename VO>
-__global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
-                                                      const T* __restrict__ dval, const VI* b, VO* x,
-                                                      const int* __restrict__ done) {
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<B, MO, MI, A>::kWaves)))
+    k_sweep_fwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val, const T* __restrict__ dval, const VI* b,
+        VO* x, const int* __restrict__ done) {
   __shared__ A xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int g = gbase + (int)blockIdx.x * kWPB + w;
@@ -497,8 +599,8 @@ __global__ void __launch_bounds__(64 * kWPB) k_sweep_fwd(int gbase, int gend, Sl
 }
 
 template <int B, int MO, int MI, typename T, typename A, typename VX>
-__global__ void __launch_bounds__(64 * kWPB) k_sweep_bwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val,
-                                                      VX* x, const int* __restrict__ done) {
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<B, MO, MI, A>::kWaves)))
+    k_sweep_bwd(int gbase, int gend, SlabArgs sa, const T* __restrict__ val, VX* x, const int* __restrict__ done) {
   __shared__ A xsb[kWPB][64 * B];
   const int w = wave_in_block();
   const int g = gbase + (int)blockIdx.x * kWPB + w;

@@ -87,22 +87,25 @@ struct Sweep {
   DBuf<uint8_t> d_lev;
   int nphase() const { return (int)phase_nlev.size(); }
 
-  // Slab layout of the factor for the sweep kernels (one 64-lane slot per row, lane = position in its group).
-  // Group k (execution order) owns slab rows [off, off + wo + wi): first wo rows of entries referencing rows of
-  // earlier phases ("outside", in ascending column order per lane), then wi rows of in-group entries (ascending;
-  // the column is stored as the source lane). Entry (slab row r, lane l) sits at index r*64 + l; its b*b values
-  // are stored element-major: value e at (r*bs + e)*64 + l, so every load instruction of a wavefront is one
-  // contiguous 64-element segment.
-  std::vector<int> gmeta;        // [4*ngroup] off, wo | wi << 8 | nlev << 16, first row (contiguous) or -1, nrows
+  // Packed layout of the factor for the sweep kernels (one 64-lane slot per row, lane = position in its group).
+  // Group k (execution order) owns entries [off, off + its count): first the class of entries referencing rows of
+  // earlier phases ("outside"), then the in-group class (the column stored as the source lane); each lane's
+  // entries of a class in ascending column order. A class is stored as jagged rows k < 4 (row k: the k-th entry of
+  // every lane that has one, in lane order; the row's b*b values element-major: value e of its p-th entry at
+  // (row start)*bs + e*(row length) + p) followed by the entries k >= 4 lane after lane (values of an entry
+  // together, row-major). Nothing is padded.
+  std::vector<int> gmeta;        // [4*ngroup] off, wo | wi << 8 | nlev << 16, first row (contiguous) or -1,
+                                 // nrows | (outside entries of the group) << 8
   std::vector<int> srow;         // [64*ngroup] device row of each slot, -1 for an empty lane
   std::vector<unsigned> scnt;    // [64*ngroup] nout | nin << 8 | level << 16
-  std::vector<unsigned> ecol;    // [64*nslab] column (outside) or source lane (in-group)
-  std::vector<unsigned> esrc;    // [64*nslab] factor block index of the entry, ~0u for padding
-  long long nslab = 0;
+  std::vector<unsigned> ecol;    // [nent] column (outside) or source lane (in-group)
+  std::vector<unsigned> esrc;    // [nent] factor block index of the entry
+  long long nent = 0;
   std::vector<int> phase_wo, phase_wi;  // [nphase] widest classes of the phase's groups
   DBuf<int> d_gmeta, d_srow;
   DBuf<unsigned> d_scnt, d_ecol, d_esrc;
-  DBuf<double> d_eval, d_dval;   // FP64 values (d_dval: forward sweep only, D^-1 per slot)
+  DBuf<double> d_eval, d_dval;   // FP64 values (d_dval: forward sweep only, D^-1 per slot, value e of slot
+                                 // (g, l) at (g*bs + e)*64 + l)
   DBuf<float> d_eval32, d_dval32;
   // forward sweep only: when the last forward phase and the first backward phase hold the same tiles, the
   // backward group of each group of the last forward phase (in its order); empty otherwise

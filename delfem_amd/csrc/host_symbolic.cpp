@@ -364,9 +364,13 @@ static void group_sweeps(Sys& s, const std::vector<int>& gstart) {
   }
 }
 
-// Slab layout of one sweep (see Sweep): per group, the entries of each row split into "outside" (rows of
+// Packed layout of one sweep (see Sweep): per group, the entries of each row split into "outside" (rows of
 // earlier phases, final before the phase starts) and "in-group" (resolved level by level inside the wavefront),
-// each class in ascending column order as the reference's substitution loops visit them.
+// each class in ascending column order as the reference's substitution loops visit them. A class is stored as
+// jagged rows k < 4 (row k: the k-th entry of every lane that has one, in lane order), then the entries k >= 4
+// lane after lane; no padding.
+static constexpr int kJagRows = 4;  // the device's kJag
+
 static void build_slabs(const Sys& s, Sweep& w, bool forward) {
   const int ngroup = (int)w.grp.size() / 2;
   const bool indirect = !w.rowlist.empty();
@@ -382,6 +386,7 @@ static void build_slabs(const Sys& s, Sweep& w, bool forward) {
     int wo = 0, wi = 0, nlev = 1;
     // per lane: [o0, o1) outside entries, [i0, i1) in-group entries
     unsigned o0[kGroup], o1[kGroup], i0[kGroup], i1[kGroup];
+    long long nent = 0, nout = 0;
     for (int l = 0; l < nrows; l++) {
       const int row = indirect ? w.rowlist[a + l] : a + l;
       if (forward) {
@@ -394,33 +399,34 @@ static void build_slabs(const Sys& s, Sweep& w, bool forward) {
       if (ni && indirect) throw std::runtime_error("schedule: in-group entries in an indirect group");
       wo = std::max(wo, no);
       wi = std::max(wi, ni);
+      nent += no + ni;
+      nout += no;
       nlev = std::max(nlev, (int)w.lev[row] + 1);
       w.srow[(size_t)g * kGroup + l] = row;
       w.scnt[(size_t)g * kGroup + l] = (unsigned)no | ((unsigned)ni << 8) | ((unsigned)w.lev[row] << 16);
     }
-    if (off > INT32_MAX - 512) throw std::runtime_error("schedule: slab too large");
+    if (off + nent > INT32_MAX) throw std::runtime_error("schedule: factor too large for 32-bit entry offsets");
     w.gmeta[4 * g + 0] = (int)off;
     w.gmeta[4 * g + 1] = wo | (wi << 8) | (nlev << 16);
     w.gmeta[4 * g + 2] = indirect ? -1 : a;
-    w.gmeta[4 * g + 3] = nrows;
-    const size_t base = w.ecol.size();
-    w.ecol.resize(base + (size_t)(wo + wi) * kGroup, 0u);
-    w.esrc.resize(base + (size_t)(wo + wi) * kGroup, ~0u);
-    for (int l = 0; l < nrows; l++) {
-      for (unsigned q = o0[l]; q < o1[l]; q++) {
-        const size_t k = base + (size_t)(q - o0[l]) * kGroup + l;
-        w.ecol[k] = s.fcolind[q];
-        w.esrc[k] = q;
-      }
-      for (unsigned q = i0[l]; q < i1[l]; q++) {
-        const size_t k = base + (size_t)(wo + (q - i0[l])) * kGroup + l;
-        w.ecol[k] = s.fcolind[q] - (unsigned)a;  // contiguous group: lane of the source row
-        w.esrc[k] = q;
-      }
-    }
-    off += wo + wi;
+    w.gmeta[4 * g + 3] = nrows | (int)(nout << 8);
+    // one class: jagged rows, then the rest lane by lane
+    auto put = [&](const unsigned* c0, const unsigned* c1, int width, bool ingroup) {
+      auto push = [&](unsigned q) {
+        w.ecol.push_back(ingroup ? s.fcolind[q] - (unsigned)a : s.fcolind[q]);  // in-group: lane of the source row
+        w.esrc.push_back(q);
+      };
+      for (int k = 0; k < std::min(width, kJagRows); k++)
+        for (int l = 0; l < nrows; l++)
+          if (c0[l] + k < c1[l]) push(c0[l] + k);
+      for (int l = 0; l < nrows; l++)
+        for (unsigned q = c0[l] + kJagRows; q < c1[l]; q++) push(q);
+    };
+    put(o0, o1, wo, false);
+    put(i0, i1, wi, true);
+    off += nent;
   }
-  w.nslab = off;
+  w.nent = off;
   w.phase_wo.assign(w.phase_nlev.size(), 0);
   w.phase_wi.assign(w.phase_nlev.size(), 0);
   for (size_t p = 0; p < w.phase_wo.size(); p++)

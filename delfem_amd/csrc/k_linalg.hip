@@ -302,6 +302,64 @@ static hipError_t slab_fill_T(int len, long long nent, const unsigned* src, cons
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
+// Values of a sweep's packed entries from the FP64 factor: one wavefront per group, each lane placing its
+// entries of both classes where the sweep kernels read them (jagged rows element-major, the rest per entry).
+template <int B, typename T>
+__global__ void __launch_bounds__(256) k_sweep_fill(int ngroup, const int* __restrict__ gmeta,
+                                                    const unsigned* __restrict__ scnt, const unsigned* __restrict__ src,
+                                                    const double* __restrict__ f, T* __restrict__ val) {
+  constexpr int BS = B * B;
+  const int g = (int)blockIdx.x * 4 + wave_in_block();
+  if (g >= ngroup) return;
+  const int* m = gmeta + 4 * (size_t)g;
+  const int base = m[0], wo = m[1] & 255, wi = (m[1] >> 8) & 255, nout = m[3] >> 8;
+  const unsigned c = scnt[(size_t)g * 64 + threadIdx.x % 64];
+  const int cnt[2] = {(int)(c & 255u), (int)((c >> 8) & 255u)};
+  const int wid[2] = {wo, wi};
+  for (int cls = 0; cls < 2; cls++) {
+    const int n = cnt[cls], w = wid[cls];
+    int r = cls == 0 ? base : base + nout;
+    for (int k = 0; k < kJag && k < w; k++) {
+      const unsigned long long msk = __ballot(n > k);
+      const int len = __popcll(msk), pos = lanes_below(msk);
+      if (k < n) {
+        const size_t q = src[(size_t)(r + pos)];
+#pragma unroll
+        for (int e = 0; e < BS; e++) val[(size_t)r * BS + (size_t)e * len + pos] = (T)f[q * BS + e];
+      }
+      r += len;
+    }
+    if (w > kJag) {
+      const int s0 = r + wave_scan_u8(n > kJag ? (unsigned)(n - kJag) : 0u);
+      for (int k = kJag; k < n; k++) {
+        const size_t i = (size_t)s0 + (k - kJag), q = src[i];
+#pragma unroll
+        for (int e = 0; e < BS; e++) val[i * BS + e] = (T)f[q * BS + e];
+      }
+    }
+  }
+}
+
+template <typename T>
+static hipError_t sweep_fill_T(int len, int ngroup, const int* gmeta, const unsigned* scnt, const unsigned* src,
+                               const double* f, T* val, hipStream_t st) {
+  if (ngroup <= 0) return hipSuccess;
+  const dim3 g((unsigned)((ngroup + 3) / 4));
+  if (len == 1) hipLaunchKernelGGL((k_sweep_fill<1, T>), g, dim3(256), 0, st, ngroup, gmeta, scnt, src, f, val);
+  else if (len == 2) hipLaunchKernelGGL((k_sweep_fill<2, T>), g, dim3(256), 0, st, ngroup, gmeta, scnt, src, f, val);
+  else if (len == 3) hipLaunchKernelGGL((k_sweep_fill<3, T>), g, dim3(256), 0, st, ngroup, gmeta, scnt, src, f, val);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t launch_sweep_fill(int len, int ngroup, const int* gmeta, const unsigned* scnt, const unsigned* src,
+                             const double* f, double* val, hipStream_t st) {
+  return sweep_fill_T<double>(len, ngroup, gmeta, scnt, src, f, val, st);
+}
+hipError_t launch_sweep_fill(int len, int ngroup, const int* gmeta, const unsigned* scnt, const unsigned* src,
+                             const double* f, float* val, hipStream_t st) {
+  return sweep_fill_T<float>(len, ngroup, gmeta, scnt, src, f, val, st);
+}
+
 hipError_t launch_slab_fill(int len, long long nent, const unsigned* src, const double* f, double* val, hipStream_t st) {
   return slab_fill_T<double>(len, nent, src, f, val, st);
 }

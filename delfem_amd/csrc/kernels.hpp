@@ -66,7 +66,13 @@ struct SweepDev {
 hipError_t launch_ilu_factor(int n, int len, const unsigned* rowptr, const unsigned* colind,
                              const unsigned* diaind, double* fcrs, double* fdia, const SweepDev& fwd,
                              int* nsing, hipStream_t st);
-// Slab values of a sweep from the FP64 factor: val[(r*bs + e)*64 + l] = f[src[r*64 + l]*bs + e], 0 where
+// Values of a sweep's packed entries (jagged rows, then per-entry) from the FP64 factor through src (the factor
+// block of each entry). T = double or float.
+hipError_t launch_sweep_fill(int len, int ngroup, const int* gmeta, const unsigned* scnt, const unsigned* src,
+                             const double* f, double* val, hipStream_t st);
+hipError_t launch_sweep_fill(int len, int ngroup, const int* gmeta, const unsigned* scnt, const unsigned* src,
+                             const double* f, float* val, hipStream_t st);
+// 64-lane slab values (SELL copy, per-slot diagonal): val[(r*bs + e)*64 + l] = f[src[r*64 + l]*bs + e], 0 where
 // src = ~0u. nent = 64 * slab rows. T = double or float.
 hipError_t launch_slab_fill(int len, long long nent, const unsigned* src, const double* f, double* val, hipStream_t st);
 hipError_t launch_slab_fill(int len, long long nent, const unsigned* src, const double* f, float* val, hipStream_t st);

@@ -307,7 +307,7 @@ __device__ inline T fold_xlo(double* xhi, bool fold, const T* x, size_t i) {
 // q and z may be one buffer: a lane reads q of its rows before the sweep writes z there, and the sweep reads z
 // only of rows finished in earlier phases or in LDS.
 template <int B, int MO, int MI, typename TV, typename T>
-__global__ void __launch_bounds__(64 * kWPB)
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<B, MO, MI, T>::kWaves)))
     k_fwd_update(int gbase, int gend, SlabArgs sa, const TV* __restrict__ val, const TV* __restrict__ dval, T* r,
                  const T* q, const T* p, T* x, T* z, double* sc, const int* fl, double* slot_rr,
                  const double* __restrict__ slot_pq, int Gpq, double* xhi) {
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(64 * kWPB)
 
 // backward phase, then the partials of (r, z) of its rows
 template <int B, int MO, int MI, typename TV, typename T>
-__global__ void __launch_bounds__(64 * kWPB)
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<B, MO, MI, T>::kWaves)))
     k_bwd_rz(int gbase, int gend, SlabArgs sa, const TV* __restrict__ val, const T* __restrict__ r, T* z,
              const int* fl, double* slot_rz) {
   __shared__ T xsb[kWPB][64 * B];
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(64 * kWPB)
 // forward sweep with the r/x update, then the backward sweep starting from the forward values the lanes hold (the
 // backward groups of this phase have no entries outside their tile); (r, r) and (r, z) partials of the rows.
 template <int B, int MO, int MI, typename TV, typename T>
-__global__ void __launch_bounds__(64 * kWPB)
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<B, MO, MI, T>::kWaves)))
     k_fwd_bwd_tile(int gbase, int gend, SlabArgs fa, const TV* __restrict__ fval, const TV* __restrict__ fdval,
                    SlabArgs ba, const TV* __restrict__ bval, const int* __restrict__ bgrp, T* r, const T* q,
                    const T* p, T* x, T* z, double* sc, const int* fl, double* slot_rr, double* slot_rz,
