@@ -12,7 +12,7 @@ import numpy as np
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdfm_gpu.so")
 
 POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D = range(6)
-ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKCOLOR = range(3)
+ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKCOLOR, ORDER_RCM, ORDER_RCM2, ORDER_AMD = range(6)
 FP64, MIXED, FP32_ILU, MIXED_AUTO = 0, 1, 2, 3
 ERR_SINGULAR = -5
 NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3}
@@ -102,6 +102,16 @@ def rng_uniform(seed, stream, n):
     out = np.empty(n)
     _check(lib().dfm_rng_uniform(C.c_ulonglong(seed), C.c_ulonglong(stream), C.c_size_t(n), _p(out)))
     return out
+
+
+def pattern_ordering(kind, rowptr, colind):
+    """COrdering_Blk's ordering (ORDER_RCM / ORDER_RCM2 / ORDER_AMD) of a block pattern, new -> old (host only)."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.uint32)
+    ci = np.ascontiguousarray(colind, dtype=np.uint32)
+    n = rp.size - 1
+    out = np.zeros(max(n, 1), np.int32)
+    _check(lib().dfm_pattern_ordering(kind, n, _p(rp), _p(ci), _p(out)))
+    return out[:n]
 
 
 def rng_normal(seed, stream, n):

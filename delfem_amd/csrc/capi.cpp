@@ -457,12 +457,29 @@ int dfm_sys_make_ordering(dfm_sys sys, int kind, int tile, int* order_out, int* 
     make_multicolor(s, p.order, p.color, p.ncolors);
   } else if (kind == DFM_ORDER_BLOCKCOLOR) {
     make_blockcolor(s, tile, p.order, p.color, p.tile, p.ncolors);
+  } else if (kind == DFM_ORDER_RCM || kind == DFM_ORDER_RCM2 || kind == DFM_ORDER_AMD) {
+    DFM_CHECK(make_reference_ordering(kind, s.n, s.rowptr0.data(), s.colind0.data(), p.order), DFM_ERR_ARG,
+              "dfm_sys_make_ordering: the block graph is in several pieces (the reference asserts)");
   } else {
     DFM_CHECK(false, DFM_ERR_ARG, "dfm_sys_make_ordering: unknown kind");
   }
   if (order_out) std::memcpy(order_out, p.order.data(), sizeof(int) * s.n);
   if (ncolors_out) *ncolors_out = p.ncolors;
   g_pending = std::move(p);
+  return DFM_OK;
+}
+
+int dfm_pattern_ordering(int kind, int n, const unsigned* rowptr, const unsigned* colind, int* order) {
+  DFM_CHECK(n >= 0 && (n == 0 || (rowptr && colind && order)), DFM_ERR_ARG, "dfm_pattern_ordering: bad arguments");
+  DFM_CHECK(kind == DFM_ORDER_NATURAL || kind == DFM_ORDER_RCM || kind == DFM_ORDER_RCM2 || kind == DFM_ORDER_AMD,
+            DFM_ERR_ARG, "dfm_pattern_ordering: kind must be NATURAL, RCM, RCM2 or AMD");
+  for (int i = 0; i < n; i++)
+    for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++)
+      DFM_CHECK(colind[q] < (unsigned)n, DFM_ERR_ARG, "dfm_pattern_ordering: column out of range");
+  std::vector<int> ord;
+  DFM_CHECK(make_reference_ordering(kind, n, rowptr, colind, ord), DFM_ERR_ARG,
+            "dfm_pattern_ordering: the block graph is in several pieces (the reference asserts)");
+  if (n) std::memcpy(order, ord.data(), sizeof(int) * n);
   return DFM_OK;
 }
 

@@ -231,6 +231,9 @@ void build_device_pattern(Sys& s);  // from order/inv: rowptr, colind, diaind, p
 void build_schedule(Sys& s);        // SELL copy of the pattern, then build_sweeps
 void build_sweeps(Sys& s);          // factor pattern, fwd/bwd sweeps, lsplit/usplit, slabs
 void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors);
+// COrdering_Blk's orderings of a block pattern (DFM_ORDER_RCM / _RCM2 / _AMD), new -> old as the reference stores
+// them; false where the reference stops on an assertion (host_symbolic.cpp)
+bool make_reference_ordering(int kind, int n, const unsigned* rowptr, const unsigned* colind, std::vector<int>& order);
 void make_blockcolor(const Sys& s, int tile, std::vector<int>& order, std::vector<int>& color_of_new,
                      std::vector<int>& tile_of_new, int& ncolors);
 

@@ -532,6 +532,100 @@ void build_sweeps(Sys& s) {
   fuse_last_first(s);
 }
 
+// COrdering_Blk's orderings (src/matvec/ordering_blk.cpp), on the block pattern without the diagonal.
+// RCM (MakeOrdering_RCM, :62-134): breadth-first levels from block 0; every level is taken in ascending degree
+// (std::multimap keyed by degree keeps equal degrees in the order they were reached), each block numbered as it is
+// taken and its unvisited neighbours (ascending) appended to the next level; the numbers are then reversed, and the
+// array slot of block b holds b's new number, which the reference then reads as NewToOld.
+// RCM2 (MakeOrdering_RCM2, :169-262): level 0 the blocks without lower neighbours; a candidate of the next level
+// joins it only when none of its lower neighbours is unnumbered or in the current level (else it leaves the level
+// and may be reached again); new -> old in the order blocks are taken.
+// AMD (MakeOrdering_AMD, :320-485): repeatedly the uneliminated block of least degree, lowest index first; the
+// degree updates there loop over element lists that its commented-out AroundEnode call would fill, so the degrees
+// stay the initial ones.
+bool make_reference_ordering(int kind, int n, const unsigned* rowptr, const unsigned* colind, std::vector<int>& order) {
+  order.assign((size_t)n, -1);
+  auto deg = [&](int i) { return rowptr[i + 1] - rowptr[i]; };
+  if (kind == DFM_ORDER_NATURAL) {
+    std::iota(order.begin(), order.end(), 0);
+    return true;
+  }
+  if (kind == DFM_ORDER_RCM) {
+    if (n == 0) return true;
+    std::vector<int> level((size_t)n, -1), next, cur;
+    level[0] = 0;
+    next.push_back(0);
+    int taken = 0, lev = 0;
+    while (!next.empty()) {
+      cur.assign(next.begin(), next.end());
+      std::stable_sort(cur.begin(), cur.end(), [&](int a, int b) { return deg(a) < deg(b); });
+      next.clear();
+      for (int b : cur) {
+        order[b] = taken++;
+        for (unsigned q = rowptr[b]; q < rowptr[b + 1]; q++) {
+          const int j = (int)colind[q];
+          if (level[j] != -1) continue;
+          level[j] = lev + 1;
+          next.push_back(j);
+        }
+      }
+      lev++;
+    }
+    if (taken != n) return false;  // "isolated island": the reference asserts
+    for (int b = 0; b < n; b++) order[b] = n - 1 - order[b];
+    return true;
+  }
+  if (kind == DFM_ORDER_RCM2) {
+    std::vector<int> level((size_t)n, -1), cand, next;
+    int taken = 0, lev = 0;
+    for (int b = 0; b < n; b++) {
+      bool lower = false;
+      for (unsigned q = rowptr[b]; q < rowptr[b + 1]; q++)
+        if ((int)colind[q] < b) { lower = true; break; }
+      if (lower) continue;
+      level[b] = lev;
+      order[taken++] = b;
+      for (unsigned q = rowptr[b]; q < rowptr[b + 1]; q++) {
+        const int j = (int)colind[q];
+        if (level[j] == lev || level[j] == lev + 1) continue;
+        cand.push_back(j);
+        level[j] = lev + 1;
+      }
+    }
+    lev++;
+    while (!cand.empty()) {
+      next.clear();
+      for (int b : cand) {
+        bool blocked = false;
+        for (unsigned q = rowptr[b]; q < rowptr[b + 1]; q++) {
+          const int j = (int)colind[q];
+          if (j < b && (level[j] == lev || level[j] == -1)) { blocked = true; break; }
+        }
+        if (blocked) { level[b] = -1; continue; }
+        next.push_back(b);
+      }
+      cand.clear();
+      for (int b : next) {
+        order[taken++] = b;
+        for (unsigned q = rowptr[b]; q < rowptr[b + 1]; q++) {
+          const int j = (int)colind[q];
+          if (level[j] != -1 || j < b) continue;
+          level[j] = lev + 1;
+          cand.push_back(j);
+        }
+      }
+      lev++;
+    }
+    return taken == n;
+  }
+  if (kind == DFM_ORDER_AMD) {
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return deg(a) < deg(b); });
+    return true;
+  }
+  return false;
+}
+
 // Greedy node colouring in natural order; order = colour-major, natural inside a colour.
 void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors) {
   const int n = s.n;

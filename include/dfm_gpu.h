@@ -42,11 +42,18 @@ enum {
   DFM_DIFFUSION2D = 5  /* (rho, alpha, source), glob=(dt, gamma)  src/femeqn/eqn_diffusion.cpp:240-345 */
 };
 
-/* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering) */
+/* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering); the last three are COrdering_Blk's, exactly
+ * as the reference computes them (src/matvec/ordering_blk.cpp), for CPreconditioner_ILU::SetOrdering */
 enum {
   DFM_ORDER_NATURAL = 0,     /* identity: the reference's default CPreconditioner_ILU                    */
   DFM_ORDER_MULTICOLOR = 1,  /* greedy node colouring, colour-major                                      */
-  DFM_ORDER_BLOCKCOLOR = 2   /* block multicolour: geometric tiles of <= 64 nodes, tiles coloured        */
+  DFM_ORDER_BLOCKCOLOR = 2,  /* block multicolour: geometric tiles of <= 64 nodes, tiles coloured        */
+  DFM_ORDER_RCM = 3,         /* MakeOrdering_RCM (ordering_blk.cpp:62-134): breadth-first levels from block 0,
+                                each level by ascending degree, reversed; stored, like the reference, as the
+                                old -> new positions in the new -> old slot (NewToOld returns them)        */
+  DFM_ORDER_RCM2 = 4,        /* MakeOrdering_RCM2 (:169-262): level sets of the lower graph               */
+  DFM_ORDER_AMD = 5          /* MakeOrdering_AMD (:320-485) as compiled: its element absorption calls are
+                                commented out, so it takes blocks by ascending degree, lowest index first */
 };
 
 /* precision of the device Krylov solvers: DFM_FP64 the reference algorithm in FP64; DFM_MIXED FP32 matrix,
@@ -97,6 +104,11 @@ int dfm_sys_get_pattern(dfm_sys sys, unsigned* rowptr, unsigned* colind);
 int dfm_sys_set_ordering(dfm_sys sys, const int* order);
 /* Build an ordering of the given kind (tile = nodes per tile edge for BLOCKCOLOR, 0 = default). */
 int dfm_sys_make_ordering(dfm_sys sys, int kind, int tile, int* order_out, int* ncolors_out);
+/* One of COrdering_Blk's orderings (DFM_ORDER_RCM, _RCM2, _AMD; NATURAL gives the identity) of a block pattern
+ * (rowptr[n+1], colind: the off-diagonal blocks of each row, ascending, as CMatDia_BlkCrs::GetPtrIndPSuP gives
+ * them) into order[n] (new -> old, NewToOld). Host only: needs no device. DFM_ERR_ARG where the reference would
+ * stop on an assertion (RCM: a graph in several pieces; RCM2: blocks its level rule leaves unnumbered). */
+int dfm_pattern_ordering(int kind, int n, const unsigned* rowptr, const unsigned* colind, int* order);
 /* Sequential phases of the forward and backward triangular sweeps under the current ordering and fill level,
  * the most intra-group levels of a phase, and the sweep launches of one PCG iteration (one fewer than the phases
  * when the last forward and the first backward phase hold the same tiles and run as one kernel). No counterpart

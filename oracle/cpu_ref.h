@@ -55,6 +55,10 @@ void cr_ilu0_pattern(int n, const unsigned* rowptr, const unsigned* colind, cons
 /* Level-of-fill pattern (ILU(k) symbolic) of an ordered pattern: src/matvec/matdiafrac_blkcrs.cpp:1254-1381
    (AddFracPtn) with CListRLN (include/delfem/matvec/matdiafrac_blkcrs.h:185-301). lev_fill -1: no limit.
    Returns the entry count, or -(count needed) when cap is too small. flev may be NULL. */
+/* COrdering_Blk's orderings of a block pattern (kind 3 MakeOrdering_RCM, 4 MakeOrdering_RCM2, 5 MakeOrdering_AMD as
+ * compiled), written into order[n] as the reference's m_pOrder (read by NewToOld). Returns 0 where the reference
+ * asserts (graph in several pieces), 1 otherwise. */
+int cr_ordering(int kind, int n, const unsigned* rowptr, const unsigned* colind, int* order);
 long cr_iluk_pattern(int n, const unsigned* rowptr, const unsigned* colind, int lev_fill, long cap, unsigned* frowptr,
                      unsigned* fcolind, unsigned* diaind, int* flev);
 void cr_ilu0_setvalue(int n, int len, const unsigned* rowptr, const unsigned* colind, const double* crs,

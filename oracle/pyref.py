@@ -37,7 +37,23 @@ def cpu_ref():
         _cr.cr_iluk_pattern.restype = C.c_long
         _cr.cr_iluk_pattern.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_long, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p]
+        _cr.cr_ordering.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     return _cr
+
+
+ORDER_RCM, ORDER_RCM2, ORDER_AMD = 3, 4, 5
+
+
+def ordering(kind, rowptr, colind):
+    """COrdering_Blk's ordering (restated in cpu_ref.c) of a block pattern: new -> old as NewToOld reads it, or None
+    where the reference asserts."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.uint32)
+    ci = np.ascontiguousarray(colind, dtype=np.uint32)
+    n = rp.size - 1
+    out = np.zeros(max(n, 1), np.int32)
+    ok = cpu_ref().cr_ordering(kind, n, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                               out.ctypes.data_as(C.c_void_p))
+    return out[:n] if ok else None
 
 
 class CpuSystem:
@@ -188,7 +204,7 @@ def reference():
         for name in ["ref_sys_destroy", "ref_sys_ncrs", "ref_sys_pattern", "ref_sys_set_field", "ref_sys_get_field",
                      "ref_sys_set_bc", "ref_sys_assemble", "ref_sys_get_matrix", "ref_sys_get_vector",
                      "ref_sys_set_vector", "ref_sys_matvec", "ref_ilu0_factor", "ref_ilu_apply", "ref_sys_solve",
-                     "ref_sys_update_field"]:
+                     "ref_sys_update_field", "ref_ordering"]:
             getattr(_ref, name).argtypes = None
         _ref.ref_ilu_factor.restype = C.c_long
         _ref.ref_ilu_factor.argtypes = [C.c_void_p, C.c_int, C.c_long] + [C.c_void_p] * 5
@@ -226,6 +242,12 @@ class RefSystem:
         ci = np.zeros(max(self.ncrs, 1), np.uint32)
         reference().ref_sys_pattern(self.h, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p))
         return rp, ci[: self.ncrs]
+
+    def ordering(self, kind):
+        """COrdering_Blk::MakeOrdering_RCM / _RCM2 / _AMD (kind 3 / 4 / 5) of the matrix, as NewToOld."""
+        out = np.zeros(self.nnode, np.int32)
+        reference().ref_ordering(self.h, kind, out.ctypes.data_as(C.c_void_p))
+        return out
 
     def set_field(self, u=None, v=None):
         u = None if u is None else _f64(u)

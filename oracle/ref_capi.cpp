@@ -349,6 +349,18 @@ int ref_sys_solve(void* h, int method, int lev, const int* order, double* conv_r
 }
 
 // u_field += update (CLinearSystem_Field::UpdateValueOfField, VALUE)
+// COrdering_Blk's orderings of the system's matrix (kind 3 MakeOrdering_RCM, 4 MakeOrdering_RCM2, 5
+// MakeOrdering_AMD), written as NewToOld(i) for every i.
+void ref_ordering(void* h, int kind, int* order) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CMatDia_BlkCrs& m = Mat(s);
+  MatVec::COrdering_Blk o;
+  if (kind == 3) o.MakeOrdering_RCM(m);
+  else if (kind == 4) o.MakeOrdering_RCM2(m);
+  else o.MakeOrdering_AMD(m);
+  for (unsigned int i = 0; i < m.NBlkMatCol(); i++) order[i] = o.NewToOld(i);
+}
+
 void ref_sys_update_field(void* h) {
   RefSys* s = (RefSys*)h;
   s->ls.UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);

@@ -84,3 +84,52 @@ def test_workload_shapes(name):
     assert w["bc"].size == n * w["len"]
     if w["faces"] is not None:
         assert w["trac"].size == w["faces"].size
+
+
+def _pattern(ndim):
+    co, ln = M.rect_tri(14, 6, 1.0, 0.3, 21) if ndim == 2 else M.box_tet(4, 3, 5, 1.0, 0.7, 0.9, 22)
+    from oracle import pyref as P
+
+    cpu = P.CpuSystem(ndim, co, ln, 1)
+    return cpu.rowptr, cpu.colind
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("kind", [3, 4, 5], ids=["rcm", "rcm2", "amd"])
+def test_pattern_ordering_matches_oracle(ndim, kind):
+    # host-only entry point: COrdering_Blk's orderings, against the restatement (itself pinned to the reference in
+    # test_oracle_vs_reference.py); no device is touched
+    from delfem_amd import _native as N
+    from oracle import pyref as P
+
+    rp, ci = _pattern(ndim)
+    g = N.pattern_ordering(kind, rp, ci)
+    assert np.array_equal(g, P.ordering(kind, rp, ci))
+    assert np.array_equal(np.sort(g), np.arange(rp.size - 1))
+    deg = np.diff(rp.astype(np.int64))
+    if kind == 5:
+        # taken by ascending degree, lowest index first among equals
+        assert np.all(np.diff(deg[g]) >= 0)
+        for d in np.unique(deg):
+            assert np.all(np.diff(g[deg[g] == d]) > 0)
+    if kind == 3:
+        # the numbers written into the slots are the reversed breadth-first positions: block 0 is taken first, so
+        # its slot holds the last number
+        assert g[0] == rp.size - 2
+
+
+def test_pattern_ordering_disconnected_graph():
+    # two separate edges 0-1 and 2-3: RCM's search from block 0 never reaches 2 and 3, where the reference asserts
+    # ("isolated island", ordering_blk.cpp:81-85); the library returns an error. RCM2 seeds every block without a
+    # lower neighbour (0 and 2) and the degree ordering has no such restriction.
+    from delfem_amd import _native as N
+    from oracle import pyref as P
+
+    rp = np.array([0, 1, 2, 3, 4], np.uint32)
+    ci = np.array([1, 0, 3, 2], np.uint32)
+    with pytest.raises(N.DfmError):
+        N.pattern_ordering(3, rp, ci)
+    assert P.ordering(3, rp, ci) is None
+    assert np.array_equal(N.pattern_ordering(4, rp, ci), [0, 2, 1, 3])
+    assert np.array_equal(N.pattern_ordering(4, rp, ci), P.ordering(4, rp, ci))
+    assert np.array_equal(N.pattern_ordering(5, rp, ci), np.arange(4))

@@ -28,7 +28,8 @@ def ctx():
     c.close()
 
 
-ORDERS = [("natural", 0), ("multicolor", 1), ("blockcolor", 2)]
+# the last three: COrdering_Blk's orderings as the reference computes them (level-set sweeps)
+ORDERS = [("natural", 0), ("multicolor", 1), ("blockcolor", 2), ("rcm", 3), ("rcm2", 4), ("amd", 5)]
 
 
 @pytest.mark.parametrize("case", KINDS, ids=[k[0] for k in KINDS])
@@ -154,3 +155,11 @@ def test_singular_factor_reports(ctx):
     with pytest.raises(N.DfmError) as e:
         gs.factor()
     assert e.value.code == N.ERR_SINGULAR
+
+
+@pytest.mark.parametrize("kind", [3, 4, 5], ids=["rcm", "rcm2", "amd"])
+def test_reference_orderings_of_a_system(ctx, kind):
+    # dfm_sys_make_ordering of the reference's kinds equals the restatement on the system's pattern
+    cpu, gs, order, _ = setup(N, ctx, KINDS[0], N.ORDER_NATURAL)
+    o, _ = gs.make_ordering(kind)
+    assert np.array_equal(o, P.ordering(kind, cpu.rowptr, cpu.colind))

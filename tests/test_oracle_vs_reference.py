@@ -267,3 +267,17 @@ def test_unpreconditioned_max_iter_quirks(method):
     rok, rconv, rit = ref.solve(method, 1e-14, 3)
     assert ok and rok == 1 and it == rit == 3 and conv == rconv == 1e-14
     assert np.array_equal(ref.vector(-2), x)
+
+
+@pytest.mark.parametrize("kind", [P.ORDER_RCM, P.ORDER_RCM2, P.ORDER_AMD], ids=["rcm", "rcm2", "amd"])
+@pytest.mark.parametrize("ndim,length", [(2, 1), (2, 2), (3, 3)])
+def test_orderings_match_reference(kind, ndim, length):
+    # COrdering_Blk::MakeOrdering_RCM / _RCM2 / _AMD (src/matvec/ordering_blk.cpp) on the system's matrix, read
+    # through NewToOld, against the restatement on the same pattern
+    co, ln = mesh2d(13, 9) if ndim == 2 else mesh3d(5, 4, 3)
+    ref = P.RefSystem(ndim, co, ln, length)
+    rp, ci = ref.pattern()
+    r = ref.ordering(kind)
+    o = P.ordering(kind, rp, ci)
+    assert o is not None and np.array_equal(o, r)
+    assert np.array_equal(np.sort(r), np.arange(r.size))
