@@ -114,6 +114,17 @@ def pattern_ordering(kind, rowptr, colind):
     return out[:n]
 
 
+def read_gid(path):
+    """GiD .msh of P1 triangles / tetrahedra (dfm_mesh_read_gid): (ndim, coords[nnode*ndim], lnods[nelem*npoel])."""
+    nd, nn, ne, npe = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    p = path.encode()
+    _check(lib().dfm_mesh_read_gid(p, C.byref(nd), C.byref(nn), C.byref(ne), C.byref(npe), None, None))
+    co = np.zeros(nn.value * nd.value)
+    ln = np.zeros(ne.value * npe.value, np.int32)
+    _check(lib().dfm_mesh_read_gid(p, C.byref(nd), C.byref(nn), C.byref(ne), C.byref(npe), _p(co), _p(ln)))
+    return nd.value, co, ln
+
+
 def rng_normal(seed, stream, n):
     out = np.empty(n)
     _check(lib().dfm_rng_normal(C.c_ulonglong(seed), C.c_ulonglong(stream), C.c_size_t(n), _p(out)))

@@ -3,6 +3,9 @@
 // RNG: SplitMix64 finaliser mix(); u64(seed, stream, idx) = mix(mix(seed ^ stream*G) + (idx+1)*G).
 // Drawn on the host only and uploaded, so that every consumer sees bit-identical inputs.
 #include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
 
 #include "dfm_internal.hpp"
 
@@ -117,5 +120,102 @@ extern "C" int dfm_mesh_box_tet(int nx, int ny, int nz, double lx, double ly, do
           for (int k = 0; k < 4; k++) o[k] = path[k];
         }
       }
+  return DFM_OK;
+}
+
+// GiD .msh reader (CMesher2D::ReadFromFile_GiDMsh, src/msh/mesher2d.cpp:2463-2620; Msh::CMesh3D::ReadFromFile_GiDMsh,
+// src/msh/mesh3d.cpp:601-760): a header "MESH dimension D ElemType T Nnode K", "Coordinates", numbered node lines
+// up to "end coordinates", a blank line, "Elements", numbered element lines (1-based nodes) up to "end elements".
+// Triangles: with D == 3 the points must lie in a plane (the smallest per-axis variance at most 1e-4 of their sum,
+// :2507-2515) and keep the two axes that follow the dropped one cyclically (:2516-2519).
+extern "C" int dfm_mesh_read_gid(const char* path, int* ndim, int* nnode, int* nelem, int* npoel, double* coords,
+                                 int* lnods) {
+  using dfm::set_error;
+  if (!path || !ndim || !nnode || !nelem || !npoel) {
+    set_error("dfm_mesh_read_gid: null argument");
+    return DFM_ERR_ARG;
+  }
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) {
+    set_error(std::string("dfm_mesh_read_gid: cannot open ") + path);
+    return DFM_ERR_ARG;
+  }
+  auto fail = [&](const char* why) {
+    std::fclose(fp);
+    set_error(std::string("dfm_mesh_read_gid: ") + why);
+    return DFM_ERR_ARG;
+  };
+  char line[512], w1[64], w2[64], w3[64], type[64], w5[64];
+  int dim = 0, nnoel = 0;
+  if (!std::fgets(line, sizeof line, fp) ||
+      std::sscanf(line, "%63s %63s %d %63s %63s %63s %d", w1, w2, &dim, w3, type, w5, &nnoel) != 7)
+    return fail("bad header");
+  const bool tri = std::strcmp(type, "Triangle") == 0, tet = std::strcmp(type, "Tetrahedra") == 0;
+  if (!(tri && nnoel == 3) && !(tet && nnoel == 4)) return fail("only P1 triangles and tetrahedra are supported");
+  if (tet && dim != 3) return fail("tetrahedra need dimension 3");
+  if (dim != 2 && dim != 3) return fail("bad dimension");
+  if (!std::fgets(line, sizeof line, fp)) return fail("missing Coordinates");
+  std::vector<double> xyz;
+  for (int count = 0;; count++) {
+    if (!std::fgets(line, sizeof line, fp)) return fail("missing end coordinates");
+    if (std::strncmp(line, "end coordinates", 15) == 0) break;
+    int id = 0;
+    double c[3] = {0.0, 0.0, 0.0};
+    const int got = std::sscanf(line, "%d%lf%lf%lf", &id, &c[0], &c[1], &c[2]);
+    if (got < 1 + dim || id != count + 1) return fail("bad coordinate line");
+    xyz.insert(xyz.end(), c, c + 3);
+  }
+  const int nn = (int)(xyz.size() / 3);
+  if (nn == 0) return fail("no nodes");
+  // triangles: the plane's two axes
+  int a0 = 0, a1 = 1;
+  if (tri && dim == 3) {
+    double m[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    for (int i = 0; i < nn; i++)
+      for (int d = 0; d < 3; d++) {
+        m[d] += xyz[(size_t)i * 3 + d];
+        v[d] += xyz[(size_t)i * 3 + d] * xyz[(size_t)i * 3 + d];
+      }
+    for (int d = 0; d < 3; d++) {
+      m[d] /= nn;
+      v[d] = v[d] / nn - m[d] * m[d];
+    }
+    int dmin = 0;
+    if (v[1] < v[dmin]) dmin = 1;
+    if (v[2] < v[dmin]) dmin = 2;
+    if (v[dmin] > (v[0] + v[1] + v[2]) * 0.0001) return fail("triangles not in a plane");
+    a0 = (dmin + 1) % 3;
+    a1 = (dmin + 2) % 3;
+  }
+  if (!std::fgets(line, sizeof line, fp) || !std::fgets(line, sizeof line, fp)) return fail("missing Elements");
+  std::vector<int> con;
+  for (int count = 0;; count++) {
+    if (!std::fgets(line, sizeof line, fp)) return fail("missing end elements");
+    if (std::strncmp(line, "end elements", 12) == 0) break;
+    int id = 0, v[4] = {0, 0, 0, 0};
+    const int got = std::sscanf(line, "%d%d%d%d%d", &id, &v[0], &v[1], &v[2], &v[3]);
+    if (got < 1 + nnoel || id != count + 1) return fail("bad element line");
+    for (int k = 0; k < nnoel; k++) {
+      if (v[k] < 1 || v[k] > nn) return fail("element node out of range");
+      con.push_back(v[k] - 1);
+    }
+  }
+  std::fclose(fp);
+  const int outdim = tri ? 2 : 3;
+  *ndim = outdim;
+  *nnode = nn;
+  *nelem = (int)(con.size() / nnoel);
+  *npoel = nnoel;
+  if (coords) {
+    for (int i = 0; i < nn; i++) {
+      if (tri) {
+        coords[(size_t)i * 2] = xyz[(size_t)i * 3 + a0];
+        coords[(size_t)i * 2 + 1] = xyz[(size_t)i * 3 + a1];
+      } else {
+        for (int d = 0; d < 3; d++) coords[(size_t)i * 3 + d] = xyz[(size_t)i * 3 + d];
+      }
+    }
+  }
+  if (lnods) std::memcpy(lnods, con.data(), sizeof(int) * con.size());
   return DFM_OK;
 }

@@ -81,6 +81,14 @@ int dfm_mesh_rect_tri(int nx, int ny, double lx, double ly, unsigned long long s
 /* 3D box, nx*ny*nz hexes split into 6 Kuhn tets each. coords[(nx+1)(ny+1)(nz+1)*3], tet[nx*ny*nz*6*4]. */
 int dfm_mesh_box_tet(int nx, int ny, int nz, double lx, double ly, double lz, unsigned long long seed,
                      double jitter, double* coords, int* tet);
+/* GiD mesh file (.msh) of P1 triangles or tetrahedra, as the reference's readers take it
+ * (CMesher2D::ReadFromFile_GiDMsh, src/msh/mesher2d.cpp:2463-2620, for triangles: coplanar points of a
+ * "dimension 3" file are projected onto the plane of their two widest axes; Msh::CMesh3D::ReadFromFile_GiDMsh,
+ * src/msh/mesh3d.cpp:601-760, for tetrahedra), nodes and elements in file order, 0-based. Call with coords and
+ * lnods NULL for the sizes (ndim, nnode, nelem, npoel = ndim + 1), then with arrays of those sizes.
+ * DFM_ERR_ARG for other element types (quadrilaterals, hexahedra), a malformed file, or triangles that are not
+ * coplanar; host only. */
+int dfm_mesh_read_gid(const char* path, int* ndim, int* nnode, int* nelem, int* npoel, double* coords, int* lnods);
 /* Counter-based SplitMix64 draws (uniform [0,1) / standard normal) for seeded materials and loads. */
 int dfm_rng_uniform(unsigned long long seed, unsigned long long stream, size_t n, double* out);
 int dfm_rng_normal(unsigned long long seed, unsigned long long stream, size_t n, double* out);

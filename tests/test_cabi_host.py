@@ -133,3 +133,61 @@ def test_pattern_ordering_disconnected_graph():
     assert np.array_equal(N.pattern_ordering(4, rp, ci), [0, 2, 1, 3])
     assert np.array_equal(N.pattern_ordering(4, rp, ci), P.ordering(4, rp, ci))
     assert np.array_equal(N.pattern_ordering(5, rp, ci), np.arange(4))
+
+
+GID = os.path.join(ROOT, "tests", "golden", "gid")
+
+
+def _parse_gid(path):
+    # independent twin of the reader: header, numbered coordinates, numbered elements (1-based)
+    lines = open(path).read().splitlines()
+    head = lines[0].split()
+    dim, etype, nnoel = int(head[2]), head[4], int(head[6])
+    i = lines.index("Coordinates") + 1
+    xyz = []
+    while not lines[i].startswith("end coordinates"):
+        f = lines[i].split()
+        assert int(f[0]) == len(xyz) + 1
+        xyz.append([float(v) for v in f[1:1 + dim]] + [0.0] * (3 - dim))
+        i += 1
+    i = lines.index("Elements") + 1
+    con = []
+    while not lines[i].startswith("end elements"):
+        f = lines[i].split()
+        assert int(f[0]) == len(con) + 1
+        con.append([int(v) - 1 for v in f[1:1 + nnoel]])
+        i += 1
+    return etype, np.array(xyz), np.array(con, np.int32)
+
+
+@pytest.mark.parametrize("name", ["rect_tri", "rect_tri2", "hexa_tri", "cylinder_tet"])
+def test_read_gid_matches_parser(name):
+    from delfem_amd import _native as N
+
+    ndim, co, ln = N.read_gid(os.path.join(GID, name + ".msh"))
+    etype, xyz, con = _parse_gid(os.path.join(GID, name + ".msh"))
+    assert np.array_equal(ln, con.ravel())
+    if etype == "Triangle":
+        # coplanar "dimension 3" points: the constant axis is dropped, the next two kept cyclically
+        v = xyz.var(axis=0)
+        d = int(np.argmin(v))
+        assert ndim == 2 and v[d] <= 1e-4 * v.sum()
+        assert np.array_equal(co.reshape(-1, 2), xyz[:, [(d + 1) % 3, (d + 2) % 3]])
+        p = co.reshape(-1, 2)[con]
+        area = 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                      - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+        assert np.all(np.abs(area) > 0)
+    else:
+        assert ndim == 3 and np.array_equal(co.reshape(-1, 3), xyz)
+        p = co.reshape(-1, 3)[con]
+        vol = np.einsum("ij,ij->i", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0])) / 6
+        assert np.all(np.abs(vol) > 0)
+
+
+def test_read_gid_refusals():
+    from delfem_amd import _native as N
+
+    with pytest.raises(N.DfmError):
+        N.read_gid(os.path.join(GID, "rect_quad.msh"))  # quadrilaterals: not P1
+    with pytest.raises(N.DfmError):
+        N.read_gid(os.path.join(GID, "missing.msh"))
