@@ -355,6 +355,13 @@ class System:
         """LsSol::Solve_BiCGSTAB (no preconditioner), host-driven."""
         return self._solve_fn(lib().dfm_solve_bicgstab, tol, max_iter)
 
+    def solve_mat(self, method, tol, max_iter):
+        """MatVec::Sol::Solve_CG / _PCG / _BiCGSTAB / _PBiCGSTAB (method "cg", "pcg", "bicgstab", "pbicgstab"), the
+        matrix-level twins, host-driven over the device handle operations."""
+        fn = {"cg": lib().dfm_solve_mat_cg, "pcg": lib().dfm_solve_mat_pcg, "bicgstab": lib().dfm_solve_mat_bicgstab,
+              "pbicgstab": lib().dfm_solve_mat_pbicgstab}[method]
+        return self._solve_fn(fn, tol, max_iter)
+
     def solve_pcg_host(self, tol, max_iter):
         """LsSol::Solve_PCG driven from the host over the seam (C++ layer)."""
         return self._solve_fn(lib().dfm_solve_pcg_host, tol, max_iter)

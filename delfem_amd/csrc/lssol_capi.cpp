@@ -1,6 +1,6 @@
 // C-ABI entry points of the host-driven Krylov loops (include/dfm_lssol.hpp): the reference's unpreconditioned
 // Solve_CG / Solve_BiCGSTAB, and Solve_PCG / Solve_PBiCGSTAB driven from the host over the seam (one device
-// call per vector operation) beside their device-resident forms.
+// call per vector operation) beside their device-resident forms, and the matrix-level MatVec::Sol twins.
 #include <exception>
 #include <string>
 
@@ -56,6 +56,28 @@ int dfm_solve_pcg_host(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
 int dfm_solve_pbicgstab_host(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
   return run(sys, conv_ratio, iteration, true, "dfm_solve_pbicgstab_host", [&](dfm::lssol::DeviceLinearSystem& ls) {
     return dfm::lssol::Solve_PBiCGSTAB(*conv_ratio, *iteration, ls);
+  });
+}
+
+int dfm_solve_mat_cg(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
+  return run(sys, conv_ratio, iteration, false, "dfm_solve_mat_cg",
+             [&](dfm::lssol::DeviceLinearSystem& ls) { return dfm::matsol::Solve_CG(*conv_ratio, *iteration, ls); });
+}
+
+int dfm_solve_mat_pcg(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
+  return run(sys, conv_ratio, iteration, true, "dfm_solve_mat_pcg",
+             [&](dfm::lssol::DeviceLinearSystem& ls) { return dfm::matsol::Solve_PCG(*conv_ratio, *iteration, ls); });
+}
+
+int dfm_solve_mat_bicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
+  return run(sys, conv_ratio, iteration, false, "dfm_solve_mat_bicgstab", [&](dfm::lssol::DeviceLinearSystem& ls) {
+    return dfm::matsol::Solve_BiCGSTAB(*conv_ratio, *iteration, ls);
+  });
+}
+
+int dfm_solve_mat_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
+  return run(sys, conv_ratio, iteration, true, "dfm_solve_mat_pbicgstab", [&](dfm::lssol::DeviceLinearSystem& ls) {
+    return dfm::matsol::Solve_PBiCGSTAB(*conv_ratio, *iteration, ls);
   });
 }
 

@@ -196,6 +196,16 @@ int dfm_solve_cg(dfm_sys sys, double* conv_ratio, unsigned* iteration);
 int dfm_solve_bicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
 int dfm_solve_pcg_host(dfm_sys sys, double* conv_ratio, unsigned* iteration);
 int dfm_solve_pbicgstab_host(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+/* The matrix-level twins MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB
+ * (src/matvec/solver_mat_iter.cpp:37-113, :115-184, :191-299, :305-423) on the system's matrix, the residual
+ * (in: right-hand side, out: final residual) and the update as solution, the preconditioned ones with the
+ * system's ILU factor; their own contracts: zero right-hand side below 1e-30 gives conv_ratio 0; Solve_CG
+ * tests the residual of the previous step (one step late) and counts from 1; Solve_PCG returns 0 with
+ * conv_ratio and iteration untouched when not converged; the others return 1. */
+int dfm_solve_mat_cg(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+int dfm_solve_mat_pcg(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+int dfm_solve_mat_bicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+int dfm_solve_mat_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
 /* ||b - A x|| / ||b|| in FP64 for b = the residual as assembled and x = the update (after a solve). */
 int dfm_sys_true_residual(dfm_sys sys, double* rel);
 

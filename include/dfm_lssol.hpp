@@ -8,6 +8,7 @@
 // operation is one call into the device library (host-driven loop); dfm_solve_pcg / dfm_solve_pbicgstab are
 // the device-resident forms of the preconditioned loops.
 //
+// dfm::matsol holds the matrix-level twins (MatVec::Sol::Solve_*, src/matvec/solver_mat_iter.cpp) over the same seam.
 // DeviceLinearSystem adapts a dfm_sys (and its ILU(0) preconditioner) to the seam.
 #pragma once
 
@@ -216,6 +217,88 @@ inline bool Solve_PBiCGSTAB(double& conv_ratio, unsigned int& num_iter, ILinearS
   }
   return true;
 }
+
+}  // namespace lssol
+
+// The matrix-level twins MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB
+// (src/matvec/solver_mat_iter.cpp:37-423), which take the matrix, the residual vector (in: right-hand side, out:
+// final residual) and the solution vector directly; here the matrix is the seam's, the residual is handle -1 and the
+// solution handle -2. Their console output is not reproduced.
+namespace matsol {
+
+// MatVec::Sol::Solve_CG (:37-113): loop from 1; the convergence test reads the residual norm of the iteration
+// before the update just made (so it stops one step late), and the solution takes alpha p only after the test.
+// Not converged: iteration = max, conv_ratio keeps its input; returns true either way.
+inline bool Solve_CG(double& conv_ratio, unsigned int& iteration, lssol::ILinearSystem_Sol& ls) {
+  const double tol = conv_ratio;
+  const unsigned int mx_iter = iteration;
+  if (ls.GetTmpVectorArySize() < 2) ls.ReSizeTmpVecSolver(2);
+  const int iu = -2, ir = -1, ip = 0, iap = 1;
+  ls.SCAL(0.0, iu);
+  double sq = ls.DOT(ir, ir);
+  if (sq < 1.0e-30) {
+    conv_ratio = 0.0;
+    iteration = 0;
+    return true;
+  }
+  const double sq_inv = 1.0 / sq;
+  ls.COPY(ir, ip);
+  iteration = mx_iter;
+  for (unsigned int it = 1; it < mx_iter; it++) {
+    ls.MATVEC(1.0, ip, 0.0, iap);
+    const double alpha = sq / ls.DOT(ip, iap);
+    ls.AXPY(-alpha, iap, ir);
+    const double sq1 = ls.DOT(ir, ir);
+    if (sq * sq_inv < tol * tol) {
+      conv_ratio = std::sqrt(sq * sq_inv);
+      iteration = it;
+      return true;
+    }
+    ls.AXPY(alpha, ip, iu);
+    const double beta = sq1 / sq;
+    sq = sq1;
+    ls.SCAL(beta, ip);
+    ls.AXPY(1.0, ir, ip);
+  }
+  return true;
+}
+
+// MatVec::Sol::Solve_PCG (:115-184): the steps of LsSol::Solve_PCG (its (r, z), recomputed at the top of each
+// iteration, is bit for bit the one LsSol carries); not converged it returns false with conv_ratio and iteration
+// as they came in.
+inline bool Solve_PCG(double& conv_ratio, unsigned int& iteration, lssol::ILinearSystemPreconditioner_Sol& ls) {
+  const double tol = conv_ratio;
+  const unsigned int mx = iteration;
+  const bool ok = lssol::Solve_PCG(conv_ratio, iteration, ls);
+  if (!ok) {
+    conv_ratio = tol;
+    iteration = mx;
+  }
+  return ok;
+}
+
+// MatVec::Sol::Solve_BiCGSTAB / Solve_PBiCGSTAB (:191-299, :305-423): the steps of the LsSol loops but with the
+// guard ||r0||^2 < 1e-30, which returns conv_ratio 0 (the solution zeroed first).
+inline bool zero_rhs_guard(double& conv_ratio, unsigned int& iteration, lssol::ILinearSystem_Sol& ls) {
+  if (ls.DOT(-1, -1) >= 1.0e-30) return false;
+  ls.SCAL(0.0, -2);
+  conv_ratio = 0.0;
+  iteration = 0;
+  return true;
+}
+inline bool Solve_BiCGSTAB(double& conv_ratio, unsigned int& iteration, lssol::ILinearSystem_Sol& ls) {
+  if (zero_rhs_guard(conv_ratio, iteration, ls)) return true;
+  return lssol::Solve_BiCGSTAB(conv_ratio, iteration, ls);
+}
+inline bool Solve_PBiCGSTAB(double& conv_ratio, unsigned int& iteration, lssol::ILinearSystemPreconditioner_Sol& ls) {
+  if (zero_rhs_guard(conv_ratio, iteration, ls)) return true;
+  lssol::Solve_PBiCGSTAB(conv_ratio, iteration, ls);
+  return true;
+}
+
+}  // namespace matsol
+
+namespace lssol {
 
 // The seam over one device system: every operation is one C-ABI call (errors become exceptions, as the
 // reference's contract violations assert). The system is not owned.

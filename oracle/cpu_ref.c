@@ -1347,3 +1347,87 @@ int cr_ordering(int kind, int n, const unsigned* rowptr, const unsigned* colind,
   if (kind == 5) return cr_order_amd(n, rowptr, colind, order);
   return 0;
 }
+
+/* ---- MatVec::Sol twins of the Krylov loops (src/matvec/solver_mat_iter.cpp) ---------------------------- */
+
+/* MatVec::Sol::Solve_CG (:37-113): x = 0; guard ||r0||^2 < 1e-30 gives conv_ratio 0, iteration 0 (:59-64); loop
+ * from 1 (:74): alpha = (r,r)/(p,Ap), r -= alpha Ap, then the test on the residual norm of the iteration before
+ * the update (sq_norm_res, not the one just formed: :86-93), x += alpha p only after the test, beta from the new
+ * norm. Not converged: iteration = max, conv_ratio keeps its input; returns 1 either way. */
+int cr_solve_mat_cg(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const unsigned mx_iter = *iter;
+  const double tol = *conv_ratio;
+  const long m = (long)s->n * s->len;
+  v_scal(m, 0.0, x);
+  double sq = v_dot(m, r, r);
+  if (sq < 1.0e-30) {
+    *conv_ratio = 0.0;
+    *iter = 0;
+    return 1;
+  }
+  const double sq_inv = 1.0 / sq;
+  double* p = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* ap = (double*)calloc((size_t)m + 1, sizeof(double));
+  memcpy(p, r, sizeof(double) * m);
+  *iter = mx_iter;
+  for (unsigned it = 1; it < mx_iter; it++) {
+    double alpha;
+    {
+      matvec(s, 1.0, p, 0.0, ap);
+      const double pap = v_dot(m, p, ap);
+      alpha = sq / pap;
+    }
+    v_axpy(m, -alpha, ap, r);
+    const double sq1 = v_dot(m, r, r);
+    if (sq * sq_inv < tol * tol) {
+      *conv_ratio = sqrt(sq * sq_inv);
+      *iter = it;
+      break;
+    }
+    v_axpy(m, alpha, p, x);
+    const double beta = sq1 / sq;
+    sq = sq1;
+    v_scal(m, beta, p);
+    v_axpy(m, 1.0, r, p);
+  }
+  free(p);
+  free(ap);
+  return 1;
+}
+
+/* MatVec::Sol::Solve_PCG (:115-184): the steps of Solve_PCG (same guard, 0-based iteration, same arithmetic: its
+ * (r, z) recomputed at the top of each iteration equals the carried one bit for bit); not converged: returns 0
+ * and leaves conv_ratio and iteration as they came in. */
+int cr_solve_mat_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const double tol = *conv_ratio;
+  const unsigned mx = *iter;
+  const int ok = cr_solve_pcg(s, r, x, conv_ratio, iter);
+  if (!ok) {
+    *conv_ratio = tol;
+    *iter = mx;
+  }
+  return ok;
+}
+
+/* MatVec::Sol::Solve_BiCGSTAB (:191-299) and Solve_PBiCGSTAB (:305-423): the steps of the LsSol loops except the
+ * guard, ||r0||^2 < 1e-30 giving conv_ratio 0 (:219-224, :348-353); their (r, r0*) recomputed at the top of each
+ * iteration equals the carried one bit for bit. */
+static int mat_guard(const cr_system* s, const double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const long m = (long)s->n * s->len;
+  if (v_dot(m, r, r) < 1.0e-30) {
+    v_scal(m, 0.0, x);
+    *conv_ratio = 0.0;
+    *iter = 0;
+    return 1;
+  }
+  return 0;
+}
+int cr_solve_mat_bicgstab(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  if (mat_guard(s, r, x, conv_ratio, iter)) return 1;
+  return cr_solve_bicgstab(s, r, x, conv_ratio, iter);
+}
+int cr_solve_mat_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  if (mat_guard(s, r, x, conv_ratio, iter)) return 1;
+  cr_solve_pbicgstab(s, r, x, conv_ratio, iter);
+  return 1;
+}

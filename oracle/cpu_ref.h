@@ -88,6 +88,11 @@ int cr_solve_pcg(const cr_system* s, double* res, double* upd, double* conv_rati
 int cr_solve_pbicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
 /* unpreconditioned Solve_CG / Solve_BiCGSTAB (the factor fields of s are not used) */
 int cr_solve_cg(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+/* The matrix-level twins MatVec::Sol::Solve_CG / _PCG / _BiCGSTAB / _PBiCGSTAB (src/matvec/solver_mat_iter.cpp). */
+int cr_solve_mat_cg(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+int cr_solve_mat_pcg(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+int cr_solve_mat_bicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
+int cr_solve_mat_pbicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
 int cr_solve_bicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
 
 #ifdef __cplusplus

@@ -181,7 +181,9 @@ class CpuSystem:
         conv = C.c_double(tol)
         it = C.c_uint(max_iter)
         fn = {"pcg": cpu_ref().cr_solve_pcg, "pbicgstab": cpu_ref().cr_solve_pbicgstab, "cg": cpu_ref().cr_solve_cg,
-              "bicgstab": cpu_ref().cr_solve_bicgstab}[method]
+              "bicgstab": cpu_ref().cr_solve_bicgstab, "mat_cg": cpu_ref().cr_solve_mat_cg,
+              "mat_pcg": cpu_ref().cr_solve_mat_pcg, "mat_bicgstab": cpu_ref().cr_solve_mat_bicgstab,
+              "mat_pbicgstab": cpu_ref().cr_solve_mat_pbicgstab}[method]
         ok = fn(C.byref(s), r.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p), C.byref(conv),
                 C.byref(it))
         return bool(ok), conv.value, it.value, x, r
@@ -204,7 +206,7 @@ def reference():
         for name in ["ref_sys_destroy", "ref_sys_ncrs", "ref_sys_pattern", "ref_sys_set_field", "ref_sys_get_field",
                      "ref_sys_set_bc", "ref_sys_assemble", "ref_sys_get_matrix", "ref_sys_get_vector",
                      "ref_sys_set_vector", "ref_sys_matvec", "ref_ilu0_factor", "ref_ilu_apply", "ref_sys_solve",
-                     "ref_sys_update_field", "ref_ordering"]:
+                     "ref_sys_update_field", "ref_ordering", "ref_mat_solve"]:
             getattr(_ref, name).argtypes = None
         _ref.ref_ilu_factor.restype = C.c_long
         _ref.ref_ilu_factor.argtypes = [C.c_void_p, C.c_int, C.c_long] + [C.c_void_p] * 5
@@ -333,6 +335,14 @@ class RefSystem:
         ok = reference().ref_ilu_apply(self.h, C.c_int(lev), p(order), p(r), p(z))
         assert ok
         return z
+
+    def solve_mat(self, method, tol=1e-10, max_iter=1000):
+        """MatVec::Sol::Solve_CG / _PCG / _BiCGSTAB / _PBiCGSTAB (ILU(0), natural order) on the system."""
+        conv = C.c_double(tol)
+        it = C.c_uint(max_iter)
+        code = {"cg": 0, "pcg": 1, "bicgstab": 2, "pbicgstab": 3}[method]
+        ok = reference().ref_mat_solve(self.h, C.c_int(code), C.byref(conv), C.byref(it))
+        return ok, conv.value, it.value
 
     def solve(self, method="pcg", tol=1e-10, max_iter=1000, lev=0, order=None):
         conv = C.c_double(tol)

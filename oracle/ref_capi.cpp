@@ -24,6 +24,7 @@
 #include "delfem/matvec/matdia_blkcrs.h"
 #include "delfem/matvec/matdiafrac_blkcrs.h"
 #include "delfem/matvec/ordering_blk.h"
+#include "delfem/matvec/solver_mat_iter.h"
 #include "delfem/matvec/vector_blk.h"
 #include "delfem/matvec/bcflag_blk.h"
 #include "delfem/femeqn/eqn_poisson.h"
@@ -348,7 +349,6 @@ int ref_sys_solve(void* h, int method, int lev, const int* order, double* conv_r
   return ok ? 1 : 0;
 }
 
-// u_field += update (CLinearSystem_Field::UpdateValueOfField, VALUE)
 // COrdering_Blk's orderings of the system's matrix (kind 3 MakeOrdering_RCM, 4 MakeOrdering_RCM2, 5
 // MakeOrdering_AMD), written as NewToOld(i) for every i.
 void ref_ordering(void* h, int kind, int* order) {
@@ -361,6 +361,24 @@ void ref_ordering(void* h, int kind, int* order) {
   for (unsigned int i = 0; i < m.NBlkMatCol(); i++) order[i] = o.NewToOld(i);
 }
 
+// The matrix-level twins MatVec::Sol::Solve_CG / _PCG / _BiCGSTAB / _PBiCGSTAB (method 0..3; src/matvec/
+// solver_mat_iter.cpp) on the system's matrix with its residual (in: right-hand side, out: final residual) and
+// update vectors; the preconditioned ones with CMatDiaFrac_BlkCrs ILU(0) in the natural order as the CPrecond_Blk.
+int ref_mat_solve(void* h, int method, double* conv_ratio, unsigned int* iter) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CMatDia_BlkCrs& a = Mat(s);
+  MatVec::CVector_Blk& r = s->ls.m_ls.GetVector(-1, 0);
+  MatVec::CVector_Blk& u = s->ls.m_ls.GetVector(-2, 0);
+  if (method == 0) return MatVec::Sol::Solve_CG(*conv_ratio, *iter, a, r, u) ? 1 : 0;
+  if (method == 2) return MatVec::Sol::Solve_BiCGSTAB(*conv_ratio, *iter, a, r, u) ? 1 : 0;
+  MatVec::CMatDiaFrac_BlkCrs frac(0, a);
+  frac.SetValue_Initialize(a);
+  if (!frac.DoILUDecomp()) return -1;
+  if (method == 1) return MatVec::Sol::Solve_PCG(*conv_ratio, *iter, a, r, u, frac, frac) ? 1 : 0;
+  return MatVec::Sol::Solve_PBiCGSTAB(*conv_ratio, *iter, a, r, u, frac, frac) ? 1 : 0;
+}
+
+// u_field += update (CLinearSystem_Field::UpdateValueOfField, VALUE)
 void ref_sys_update_field(void* h) {
   RefSys* s = (RefSys*)h;
   s->ls.UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);

@@ -75,3 +75,34 @@ def test_unpreconditioned_max_iter_contract(ctx, method):
     ok, conv, it = (gs.solve_cg if method == "cg" else gs.solve_bicgstab)(1e-14, 4)
     rok, rconv, rit, _, _ = cpu.solve(None, method, 1e-14, 4)
     assert ok and rok and it == rit == 4 and conv == rconv == 1e-14
+
+
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+@pytest.mark.parametrize("method", ["cg", "pcg", "bicgstab", "pbicgstab"])
+def test_matrix_level_twins_vs_oracle(ctx, case, method):
+    """MatVec::Sol::Solve_* through dfm_solve_mat_*: same contract and iterates as the oracle's twins (iteration
+    count within 2: parallel dot products), the final residual left in handle -1."""
+    cpu, gs, order, _ = setup(N, ctx, case, N.ORDER_NATURAL, small=False)
+    tol = 1e-8
+    f = None
+    if method in ("pcg", "pbicgstab"):
+        gs.factor()
+        f = cpu.ilu0()
+    rok, rconv, rit, rx, rr = cpu.solve(f, "mat_" + method, tol, 5000)
+    ok, conv, it = gs.solve_mat(method, tol, 5000)
+    # CG-type loops within 2 iterations; BiCGSTAB's count follows rounding (parallel dot products): within 10 %
+    slack = 2 if method in ("cg", "pcg") else max(2, int(0.1 * rit))
+    assert ok == rok and conv < tol and abs(it - rit) <= slack, (ok, conv, it, rok, rconv, rit)
+    x = gs.get_vector(-2)
+    b = cpu.res
+    rtrue = b - cpu.matvec(1.0, x, 0.0, np.zeros_like(b))
+    assert np.linalg.norm(rtrue) <= 10 * tol * np.linalg.norm(b)
+    assert np.linalg.norm(x - rx) <= 1e-4 * np.linalg.norm(rx)
+
+
+def test_matrix_level_pcg_nonconvergence(ctx):
+    """MatVec::Sol::Solve_PCG not converged: returns 0 with conv_ratio and iteration as given."""
+    cpu, gs, order, _ = setup(N, ctx, KINDS[3], N.ORDER_NATURAL, small=False)
+    gs.factor()
+    ok, conv, it = gs.solve_mat("pcg", 1e-14, 3)
+    assert not ok and conv == 1e-14 and it == 3

@@ -281,3 +281,47 @@ def test_orderings_match_reference(kind, ndim, length):
     o = P.ordering(kind, rp, ci)
     assert o is not None and np.array_equal(o, r)
     assert np.array_equal(np.sort(r), np.arange(r.size))
+
+
+def _ref_mat(ndim, length, nxyz, method, tol, maxit):
+    # the reference's matrix-level loops read uninitialised temporaries (see tests/ref_mat_runner.py): run them in a
+    # fresh process
+    import json
+    import os
+    import subprocess
+    import sys
+
+    runner = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_mat_runner.py")
+    out = subprocess.run([sys.executable, runner, str(ndim), str(length), *map(str, nxyz), method, repr(tol),
+                          str(maxit)], capture_output=True, text=True, check=True).stdout
+    d = json.loads(out.strip().splitlines()[-1])
+    return d["ok"], d["conv"], d["it"], np.array(d["x"]), np.array(d["r"])
+
+
+@pytest.mark.parametrize("ndim,length,nxyz", ILU_CASES)
+@pytest.mark.parametrize("method", ["cg", "pcg", "bicgstab", "pbicgstab"])
+def test_matrix_level_krylov_bit_exact(ndim, length, nxyz, method):
+    """MatVec::Sol::Solve_* (src/matvec/solver_mat_iter.cpp) on the system's matrix: iterates, iteration count and
+    conv_ratio, ILU(0) in the natural order for the preconditioned ones."""
+    ref, cpu = _assembled(ndim, length, nxyz)
+    f = cpu.ilu0() if method in ("pcg", "pbicgstab") else None
+    ok, conv, it, x, r = cpu.solve(f, "mat_" + method, 1e-10, 2000)
+    rok, rconv, rit, rx, rr = _ref_mat(ndim, length, nxyz, method, 1e-10, 2000)
+    assert (ok, conv, it) == (bool(rok), rconv, rit)
+    assert np.array_equal(rx, x)
+    assert np.array_equal(rr, r)
+
+
+@pytest.mark.parametrize("method", ["cg", "pcg", "bicgstab", "pbicgstab"])
+def test_matrix_level_quirks(method):
+    """Not converged within 3 iterations: CG / BiCGSTAB / PBiCGSTAB return true with iteration = max; PCG returns
+    false with conv_ratio and iteration untouched. A zero right-hand side gives conv_ratio 0, iteration 0."""
+    ref, cpu = _assembled(2, 2, (9, 4))
+    f = cpu.ilu0() if method in ("pcg", "pbicgstab") else None
+    ok, conv, it, x, _ = cpu.solve(f, "mat_" + method, 1e-14, 3)
+    rok, rconv, rit, rx, _ = _ref_mat(2, 2, (9, 4), method, 1e-14, 3)
+    assert (ok, conv, it) == (bool(rok), rconv, rit)
+    assert ok == (method != "pcg") and it == 3 and conv == 1e-14
+    assert np.array_equal(rx, x)
+    ok, conv, it, x, _ = cpu.solve(f, "mat_" + method, 1e-10, 100, res=np.zeros(cpu.nnode * cpu.len))
+    assert ok and conv == 0.0 and it == 0 and not x.any()
