@@ -271,6 +271,34 @@ class System:
         _check(lib().dfm_sys_get_matrix(self.h, _p(crs), _p(dia)))
         return crs[: self.ncrs * bs], dia
 
+    # saved stiffness (CLinearSystem_Save)
+    def assemble_save(self, kind, params):
+        """InitializeMarge + *_SaveStiffMat + FinalizeMarge of CLinearSystem_Save: matrix, force, boundary
+        coupling (include/delfem/femls/linearsystem_fieldsave.h:59-106). Set the Dirichlet flags first."""
+        params = _f64(params)
+        npar, pstride = (params.size, 0) if params.ndim == 1 else (params.shape[1], params.shape[1])
+        _check(lib().dfm_sys_assemble_save(self.h, kind, _p(params), npar, pstride))
+
+    def make_residual(self):
+        """CLinearSystem_Save::MakeResidual: residual = f - K_b u_field, update = u_field; returns ||residual||."""
+        rn = C.c_double(0.0)
+        _check(lib().dfm_sys_make_residual(self.h, C.byref(rn)))
+        return rn.value
+
+    def update_field_save(self):
+        """CLinearSystem_Save::UpdateValueOfField (VALUE): u_field = update, Dirichlet values kept."""
+        _check(lib().dfm_sys_update_field_save(self.h))
+
+    def get_force(self):
+        out = np.empty(self.nnode * self.len)
+        _check(lib().dfm_sys_get_force(self.h, _p(out)))
+        return out
+
+    def boundary_terms(self):
+        n = C.c_longlong(0)
+        _check(lib().dfm_sys_boundary_terms(self.h, C.byref(n)))
+        return n.value
+
     # handle ops (ILinearSystemPreconditioner_Sol)
     def resize_tmp(self, n):
         _check(lib().dfm_sys_resize_tmp(self.h, n))

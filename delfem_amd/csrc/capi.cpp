@@ -149,6 +149,7 @@ Sys::~Sys() {
   for (Sweep* w : {&fwd, &bwd}) w->release_dev();
   sell.release_dev();
   d_par.release(); d_lptr.release(); d_loads.release();
+  d_force.release(); d_bptr.release(); d_bcol.release(); d_bsrc.release(); d_bval.release();
   for (auto& g : gexec)
     if (g) (void)hipGraphExecDestroy(g);
   if (h_pinned) DevAlloc::release_host(h_pinned);
@@ -205,6 +206,16 @@ static int upload_structure(Sys& s) {
   return upload_sweeps(s);
 }
 
+// Dirichlet flags into the device numbering (again after a new ordering)
+static int upload_bc(Sys& s) {
+  if (s.bc0.empty()) return DFM_OK;
+  std::vector<int> bc((size_t)s.n * s.len);
+  for (int i0 = 0; i0 < s.n; i0++)
+    for (int d = 0; d < s.len; d++) bc[(size_t)i0 * s.len + d] = s.bc0[(size_t)s.order[i0] * s.len + d];
+  DFM_HIP(hipMemcpy(s.d_bc.p, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice));
+  return DFM_OK;
+}
+
 static int apply_order(Sys& s, const std::vector<int>& order, const std::vector<int>& color,
                        const std::vector<int>& tile, int ncolors) {
   s.order = order;
@@ -224,7 +235,10 @@ static int apply_order(Sys& s, const std::vector<int>& order, const std::vector<
     return DFM_ERR_ARG;
   }
   s.assembled = s.factored = s.factored32 = false;
-  return upload_structure(s);
+  s.saved = false;
+  const int rc = upload_structure(s);
+  if (rc != DFM_OK) return rc;
+  return upload_bc(s);
 }
 
 static double* vec_ptr(Sys& s, int iv) {
@@ -387,6 +401,7 @@ int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
   s.rowptr = a.rowptr; s.colind = a.colind; s.diaind = a.diaind; s.lsplit = a.lsplit; s.usplit = a.usplit;
   s.fill_level = a.fill_level; s.frowptr = a.frowptr; s.fcolind = a.fcolind; s.fdiaind = a.fdiaind; s.a2f = a.a2f;
   s.perm_crs = a.perm_crs;
+  s.bc0 = a.bc0;
   copy_sweep_host(s.fwd, a.fwd);
   copy_sweep_host(s.bwd, a.bwd);
   s.sell.sptr = a.sell.sptr; s.sell.col = a.sell.col; s.sell.src = a.sell.src; s.sell.nslab = a.sell.nslab;
@@ -539,11 +554,10 @@ int dfm_sys_set_bc(dfm_sys sys, const int* flags) {
   NEED(sys, "dfm_sys_set_bc");
   NEED(flags, "dfm_sys_set_bc");
   Sys& s = sys->s;
-  std::vector<int> bc((size_t)mdof(s));
-  for (int i0 = 0; i0 < s.n; i0++)
-    for (int d = 0; d < s.len; d++) bc[(size_t)i0 * s.len + d] = flags[(size_t)s.order[i0] * s.len + d] ? 1 : 0;
-  DFM_HIP(hipMemcpy(s.d_bc.p, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice));
-  return DFM_OK;
+  s.bc0.assign((size_t)mdof(s), 0);
+  for (long long k = 0; k < mdof(s); k++) s.bc0[(size_t)k] = flags[k] ? 1 : 0;
+  s.saved = false;  // the boundary coupling depends on the flags
+  return upload_bc(s);
 }
 
 int dfm_sys_set_field(dfm_sys sys, const double* u, const double* v) {
@@ -635,9 +649,7 @@ int dfm_sys_set_load_scale(dfm_sys sys, double scale) {
   return DFM_OK;
 }
 
-int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double* res_norm) {
-  NEED(sys, "dfm_sys_assemble_resident");
-  Sys& s = sys->s;
+static int check_kind(const Sys& s, int kind) {
   static const int need_npar[6] = {2, 5, 6, 5, 6, 3};
   static const int need_len[6] = {1, 2, 3, 2, 3, 1};
   DFM_CHECK(kind >= 0 && kind < 6, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
@@ -648,6 +660,25 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
             "dfm_sys_assemble: element kind needs a 2D mesh");
   DFM_CHECK((kind != DFM_SOLID3D && kind != DFM_STVK3D) || s.ndim == 3, DFM_ERR_ARG,
             "dfm_sys_assemble: element kind needs a 3D mesh");
+  return DFM_OK;
+}
+
+// SELL copy of the matrix for the Krylov SpMV; the system is assembled and needs a new factor
+static int finish_matrix(Sys& s, hipStream_t st) {
+  const long long ne = s.sell.nslab * 64;
+  if (s.sell.d_val.n < (size_t)(ne * s.bs)) DFM_HIP(s.sell.d_val.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
+  DFM_HIP(launch_slab_fill(s.len, ne, s.sell.d_src.p, s.d_crs.p, s.sell.d_val.p, st));
+  s.assembled = true;
+  s.factored = s.factored32 = false;
+  s.prepared = -1;
+  return DFM_OK;
+}
+
+int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double* res_norm) {
+  NEED(sys, "dfm_sys_assemble_resident");
+  Sys& s = sys->s;
+  int rc = check_kind(s, kind);
+  if (rc != DFM_OK) return rc;
   hipStream_t st = s.ctx->stream;
   const long long m = mdof(s);
   // InitializeMarge (src/ls/linearsystem.cpp:74-88): matrix, residual and update set to zero
@@ -670,15 +701,9 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   // FinalizeMarge (:92-128): Dirichlet rows/columns, residual zeroed at fixed dofs, ||r|| returned
   DFM_HIP(launch_set_bc(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_bc.p, st));
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
-  // SELL copy of the matrix for the Krylov SpMV
-  {
-    const long long ne = s.sell.nslab * 64;
-    if (s.sell.d_val.n < (size_t)(ne * s.bs)) DFM_HIP(s.sell.d_val.alloc((size_t)std::max<long long>(ne, 1) * s.bs));
-    DFM_HIP(launch_slab_fill(s.len, ne, s.sell.d_src.p, s.d_crs.p, s.sell.d_val.p, st));
-  }
-  s.assembled = true;
-  s.factored = s.factored32 = false;
-  s.prepared = -1;
+  rc = finish_matrix(s, st);
+  if (rc != DFM_OK) return rc;
+  s.saved = false;
   if (res_norm) {
     DFM_HIP(launch_dot(s.d_res.p, s.d_res.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP, st));
     DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + S_TMP, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -715,6 +740,140 @@ int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia) {
   if (dia)
     for (int i0 = 0; i0 < s.n; i0++)
       std::memcpy(&dia[(size_t)s.order[i0] * s.bs], &d[(size_t)i0 * s.bs], sizeof(double) * s.bs);
+  return DFM_OK;
+}
+
+// K_b as scalar terms per device dof: for a free dof, the fixed columns of its diagonal block, then those of
+// each block of its row in pattern order. This is the boundary matrix of the reference after its Dirichlet
+// treatment (CMat_BlkCrs::SetPatternBoundary, src/matvec/mat_blkcrs.cpp:1177-1265, keeps the diagonal block
+// first and the blocks with a fixed column; SetBoundaryConditionInverse_Colum, :876-900, zeroes the free
+// columns; SetBoundaryCondition_Row, :910-953, the fixed rows), less the entries that are zero by that treatment.
+static int build_boundary_terms(Sys& s) {
+  const int B = s.len, bs = s.bs;
+  const long long m = mdof(s), ndia = (long long)s.n * bs;
+  std::vector<int> fl = s.bc0;
+  if (fl.empty()) fl.assign((size_t)m, 0);
+  std::vector<unsigned> bptr((size_t)m + 1, 0);
+  std::vector<unsigned> bcol;
+  std::vector<long long> bsrc;
+  for (int i0 = 0; i0 < s.n; i0++) {
+    const int i1 = s.order[i0];
+    for (int d = 0; d < B; d++) {
+      const long long k = (long long)i0 * B + d;
+      if (!fl[(size_t)i1 * B + d]) {
+        for (int jd = 0; jd < B; jd++)
+          if (fl[(size_t)i1 * B + jd]) {
+            bcol.push_back((unsigned)((long long)i1 * B + jd));
+            bsrc.push_back((long long)i0 * bs + d * B + jd);
+          }
+        for (unsigned q = s.rowptr0[i1]; q < s.rowptr0[i1 + 1]; q++) {
+          const unsigned j1 = s.colind0[q];
+          for (int jd = 0; jd < B; jd++)
+            if (fl[(size_t)j1 * B + jd]) {
+              bcol.push_back((unsigned)((long long)j1 * B + jd));
+              bsrc.push_back(ndia + s.perm_crs[q] * bs + d * B + jd);
+            }
+        }
+      }
+      DFM_CHECK(bcol.size() < 0xffffffffull, DFM_ERR_ARG, "dfm_sys_assemble_save: too many boundary terms");
+      bptr[(size_t)k + 1] = (unsigned)bcol.size();
+    }
+  }
+  hipStream_t st = s.ctx->stream;
+  s.nbterm = (long long)bcol.size();
+  DFM_HIP(upload(s.d_bptr, bptr, st));
+  DFM_HIP(upload(s.d_bcol, bcol, st));
+  DFM_HIP(upload(s.d_bsrc, bsrc, st));
+  if (s.d_bval.n < (size_t)std::max<long long>(s.nbterm, 1)) DFM_HIP(s.d_bval.alloc((size_t)std::max<long long>(s.nbterm, 1)));
+  return DFM_OK;
+}
+
+int dfm_sys_assemble_save(dfm_sys sys, int kind, const double* params, int npar, int pstride) {
+  NEED(sys, "dfm_sys_assemble_save");
+  NEED(params, "dfm_sys_assemble_save");
+  Sys& s = sys->s;
+  DFM_CHECK(kind == DFM_POISSON || kind == DFM_SOLID2D || kind == DFM_SOLID3D, DFM_ERR_ARG,
+            "dfm_sys_assemble_save: the reference saves the stiffness of Poisson, LinearSolid2D and LinearSolid3D only");
+  DFM_CHECK(kind != DFM_POISSON || s.ndim == 2, DFM_ERR_ARG,
+            "dfm_sys_assemble_save: the reference's saved-stiffness Poisson loop takes triangles only");
+  int rc = dfm_sys_set_params(sys, params, npar, pstride);
+  if (rc != DFM_OK) return rc;
+  rc = check_kind(s, kind);
+  if (rc != DFM_OK) return rc;
+  hipStream_t st = s.ctx->stream;
+  const long long m = mdof(s);
+  if (s.d_force.n != (size_t)m) DFM_HIP(s.d_force.alloc((size_t)m));
+  // InitializeMarge of CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp:169-187): matrix, residual,
+  // update, force and boundary matrix set to zero
+  DFM_HIP(launch_fill(s.d_crs.p, 0.0, (long long)s.ncrs * s.bs, st));
+  DFM_HIP(launch_fill(s.d_dia.p, 0.0, (long long)s.n * s.bs, st));
+  DFM_HIP(launch_fill(s.d_res.p, 0.0, m, st));
+  DFM_HIP(launch_fill(s.d_upd.p, 0.0, m, st));
+  DFM_HIP(launch_fill(s.d_force.p, 0.0, m, st));
+  AsmArgs a{};
+  a.kind = kind; a.ndim = s.ndim; a.npoel = s.npoel; a.len = s.len; a.n = s.n;
+  a.coords = s.d_coords.p; a.lnods = s.d_lnods.p; a.elsup_ptr = s.d_elsup_ptr.p; a.elsup = s.d_elsup.p;
+  a.order = s.d_order.p; a.inv = s.d_inv.p;
+  a.params = s.d_par.p; a.npar = s.npar; a.pstride = s.pstride;
+  a.rowptr = s.d_rowptr.p; a.colind = s.d_colind.p; a.crs = s.d_crs.p; a.dia = s.d_dia.p; a.res = s.d_force.p;
+  a.save = true;
+  DFM_HIP(launch_assemble(a, st));
+  rc = build_boundary_terms(s);
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(launch_gather_terms(s.nbterm, s.d_bsrc.p, (long long)s.n * s.bs, s.d_dia.p, s.d_crs.p, s.d_bval.p, st));
+  // FinalizeMarge (:189-204): the matrix's Dirichlet rows and columns (the residual is still zero)
+  DFM_HIP(launch_set_bc(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_bc.p, st));
+  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
+  rc = finish_matrix(s, st);
+  if (rc != DFM_OK) return rc;
+  s.saved = true;
+  DFM_HIP(hipStreamSynchronize(st));
+  return DFM_OK;
+}
+
+int dfm_sys_make_residual(dfm_sys sys, double* res_norm) {
+  NEED(sys, "dfm_sys_make_residual");
+  Sys& s = sys->s;
+  DFM_CHECK(s.saved, DFM_ERR_STATE, "dfm_sys_make_residual: assemble with dfm_sys_assemble_save first");
+  hipStream_t st = s.ctx->stream;
+  const long long m = mdof(s);
+  DFM_HIP(launch_make_residual(s.n, s.len, s.d_order.p, s.d_bptr.p, s.d_bcol.p, s.d_bval.p, s.d_force.p, s.d_u.p,
+                               s.d_res.p, s.d_upd.p, st));
+  s.prepared = -1;
+  if (res_norm) {
+    DFM_HIP(launch_dot(s.d_res.p, s.d_res.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP, st));
+    DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + S_TMP, sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  DFM_HIP(hipStreamSynchronize(st));
+  if (res_norm) *res_norm = std::sqrt(s.h_pinned_d[0]);
+  return DFM_OK;
+}
+
+int dfm_sys_update_field_save(dfm_sys sys) {
+  NEED(sys, "dfm_sys_update_field_save");
+  Sys& s = sys->s;
+  DFM_HIP(launch_update_field_save(s.n, s.len, s.d_order.p, s.d_bc.p, s.d_upd.p, s.d_u.p, s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_get_force(dfm_sys sys, double* f) {
+  NEED(sys, "dfm_sys_get_force");
+  NEED(f, "dfm_sys_get_force");
+  Sys& s = sys->s;
+  DFM_CHECK(s.saved, DFM_ERR_STATE, "dfm_sys_get_force: assemble with dfm_sys_assemble_save first");
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  std::vector<double> h((size_t)mdof(s));
+  DFM_HIP(hipMemcpy(h.data(), s.d_force.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i0 = 0; i0 < s.n; i0++)
+    std::memcpy(&f[(size_t)s.order[i0] * s.len], &h[(size_t)i0 * s.len], sizeof(double) * s.len);
+  return DFM_OK;
+}
+
+int dfm_sys_boundary_terms(dfm_sys sys, long long* nterm) {
+  NEED(sys, "dfm_sys_boundary_terms");
+  NEED(nterm, "dfm_sys_boundary_terms");
+  *nterm = sys->s.saved ? sys->s.nbterm : 0;
   return DFM_OK;
 }
 
@@ -1045,7 +1204,34 @@ int dfm_sys_true_residual(dfm_sys sys, double* rel) {
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes) {
   NEED(sys, "dfm_sys_bench_kernel");
   Sys& s = sys->s;
-  DFM_CHECK(which >= 0 && which <= 5 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
+  DFM_CHECK(which >= 0 && which <= 6 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
+  if (which == 6) {
+    // MakeResidual of the saved stiffness: force, residual, update and the field once per dof, the order map,
+    // the term ranges, and per term its value, its column and the field value it gathers
+    DFM_CHECK(s.saved, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble with dfm_sys_assemble_save first");
+    hipStream_t st = s.ctx->stream;
+    auto launch = [&]() {
+      return launch_make_residual(s.n, s.len, s.d_order.p, s.d_bptr.p, s.d_bcol.p, s.d_bval.p, s.d_force.p, s.d_u.p,
+                                  s.d_res.p, s.d_upd.p, st);
+    };
+    for (int w = 0; w < 3; w++) DFM_HIP(launch());
+    hipEvent_t e0, e1;
+    DFM_HIP(hipEventCreate(&e0));
+    DFM_HIP(hipEventCreate(&e1));
+    DFM_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) DFM_HIP(launch());
+    DFM_HIP(hipEventRecord(e1, st));
+    DFM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (avg_ms) *avg_ms = ms / reps;
+    const double m = (double)mdof(s);
+    if (bytes) *bytes = m * (4 * 8.0 + 4.0) + 4.0 + 4.0 * s.n + (double)s.nbterm * (8.0 + 4.0 + 8.0);
+    s.prepared = -1;
+    return DFM_OK;
+  }
   DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble and factor first");
   int rc = ensure_work(s, 2, 5);
   if (rc != DFM_OK) return rc;

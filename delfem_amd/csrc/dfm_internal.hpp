@@ -192,6 +192,19 @@ struct Sys {
   int* h_pinned = nullptr;           // host-pinned mirror of the done flag + iteration
   double* h_pinned_d = nullptr;
 
+  // Dirichlet flags in the original numbering (d_bc holds them in the device numbering)
+  std::vector<int> bc0;
+
+  // saved stiffness (CLinearSystem_Save): the external force and the boundary coupling K_b as scalar terms per
+  // device dof, in the order the reference's boundary matrix visits them
+  DBuf<double> d_force;     // f, device numbering
+  DBuf<unsigned> d_bptr;    // [n*len + 1] term ranges per device dof
+  DBuf<unsigned> d_bcol;    // u index (original numbering) of each term
+  DBuf<long long> d_bsrc;   // matrix value of each term: diagonal block entry (< n*bs) or n*bs + crs entry
+  DBuf<double> d_bval;      // K_b values, gathered before the Dirichlet zeroing
+  long long nbterm = 0;
+  bool saved = false;
+
   // resident assembly inputs
   DBuf<double> d_par;
   int npar = 0, pstride = 0;

@@ -337,8 +337,56 @@ __device__ inline void row_stvk3d(int a, const double co[4][3], const double u[4
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// Element rows of the saved-stiffness loops: the element matrix as those loops write it and the external
+// force only (no internal force, the field values are not read).
 
-template <int KIND, int NDIM, int B>
+// AddLinearSystem_Poisson2D_P1 on CLinearSystem_Save, src/femeqn/eqn_poisson.cpp:494-569. p = (alpha, source)
+__device__ inline void row_poisson2d_save(int a, const double co[4][3], const double* p, double* em, double* er) {
+  const double alpha = p[0], source = p[1];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double d[3][2];
+  tri_dldx(d, co);
+  for (int j = 0; j < 3; j++) em[j] = alpha * area * (d[a][0] * d[j][0] + d[a][1] * d[j][1]);
+  er[0] = source * area * 0.333333333333333333;
+}
+
+// AddLinSys_LinearSolid2D_Static_P1_SaveStiffMat, src/femeqn/eqn_linear_solid2d.cpp:861-960 (its element matrix
+// is written in another order than the static loop's) and AddLinSys_LinearSolid3D_Static_P1 on
+// CLinearSystem_Save, src/femeqn/eqn_linear_solid3d.cpp:425-521. p = (lambda, myu, rho, g...)
+template <int NDIM>
+__device__ inline void row_solid_save(int a, const double co[4][3], const double* p, double* em, double* er) {
+  constexpr int NNO = NDIM + 1;
+  constexpr int BS = NDIM * NDIM;
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  double d[4][3];
+  double w;
+  if (NDIM == 2) {
+    w = tri_area(co[0], co[1], co[2]);
+    double d2[3][2];
+    tri_dldx(d2, co);
+    for (int k = 0; k < 3; k++) {
+      d[k][0] = d2[k][0];
+      d[k][1] = d2[k][1];
+    }
+  } else {
+    w = tet_volume(co);
+    tet_dldx(d, co);
+  }
+  for (int j = 0; j < NNO; j++) {
+    double* e = &em[j * BS];
+    double t = 0.0;
+    for (int r = 0; r < NDIM; r++) {
+      for (int s = 0; s < NDIM; s++) e[r * NDIM + s] = w * (lambda * d[a][r] * d[j][s] + myu * d[j][r] * d[a][s]);
+      t += d[a][r] * d[j][r];
+    }
+    for (int r = 0; r < NDIM; r++) e[r * NDIM + r] += w * myu * t;
+  }
+  for (int r = 0; r < NDIM; r++) er[r] = (NDIM == 2) ? w * rho * p[3 + r] * 0.3333333333333333333 : w * rho * p[3 + r] * 0.25;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+
+template <int KIND, int NDIM, int B, bool SAVE>
 __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (i0 >= A.n) return;
@@ -366,12 +414,17 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
     }
     const double* p = &A.params[(long long)e * A.pstride];
     double em[NNO * BS], er[B];
-    if (KIND == DFM_POISSON) row_poisson<NDIM>(a, co, val, p, em, er);
-    if (KIND == DFM_DIFFUSION2D) row_diffusion2d(a, co, val, vval, p, A.glob0, A.glob1, em, er);
-    if (KIND == DFM_SOLID2D) row_solid2d(a, co, u, p, em, er);
-    if (KIND == DFM_SOLID3D) row_solid3d(a, co, u, p, em, er);
-    if (KIND == DFM_STVK2D) row_stvk2d(a, co, u, p, em, er);
-    if (KIND == DFM_STVK3D) row_stvk3d(a, co, u, p, em, er);
+    if (SAVE) {
+      if (KIND == DFM_POISSON) row_poisson2d_save(a, co, p, em, er);
+      if (KIND == DFM_SOLID2D || KIND == DFM_SOLID3D) row_solid_save<NDIM>(a, co, p, em, er);
+    } else {
+      if (KIND == DFM_POISSON) row_poisson<NDIM>(a, co, val, p, em, er);
+      if (KIND == DFM_DIFFUSION2D) row_diffusion2d(a, co, val, vval, p, A.glob0, A.glob1, em, er);
+      if (KIND == DFM_SOLID2D) row_solid2d(a, co, u, p, em, er);
+      if (KIND == DFM_SOLID3D) row_solid3d(a, co, u, p, em, er);
+      if (KIND == DFM_STVK2D) row_stvk2d(a, co, u, p, em, er);
+      if (KIND == DFM_STVK3D) row_stvk3d(a, co, u, p, em, er);
+    }
     for (int c = 0; c < NNO; c++) {
       const double* blk = &em[c * BS];
       if (c == a) {
@@ -401,8 +454,17 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
   const int nt = 128;
   const int nb = (a.n + nt - 1) / nt;
   if (a.n == 0) return hipSuccess;
+#define DFM_ASM_SAVE(K, D, B) \
+  if (a.kind == K && a.ndim == D && a.len == B) { hipLaunchKernelGGL((k_assemble<K, D, B, true>), dim3(nb), dim3(nt), 0, st, a); return hipGetLastError(); }
+  if (a.save) {
+    DFM_ASM_SAVE(DFM_POISSON, 2, 1)
+    DFM_ASM_SAVE(DFM_SOLID2D, 2, 2)
+    DFM_ASM_SAVE(DFM_SOLID3D, 3, 3)
+    return hipErrorInvalidValue;
+  }
+#undef DFM_ASM_SAVE
 #define DFM_ASM(K, D, B) \
-  if (a.kind == K && a.ndim == D && a.len == B) { hipLaunchKernelGGL((k_assemble<K, D, B>), dim3(nb), dim3(nt), 0, st, a); return hipGetLastError(); }
+  if (a.kind == K && a.ndim == D && a.len == B) { hipLaunchKernelGGL((k_assemble<K, D, B, false>), dim3(nb), dim3(nt), 0, st, a); return hipGetLastError(); }
   DFM_ASM(DFM_POISSON, 2, 1)
   DFM_ASM(DFM_POISSON, 3, 1)
   DFM_ASM(DFM_DIFFUSION2D, 2, 1)
@@ -471,6 +533,79 @@ hipError_t launch_update_field(int n, int len, const int* order, const double* u
   if (m == 0) return hipSuccess;
   const int nt = 256;
   hipLaunchKernelGGL(k_update_field, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, upd, u);
+  return hipGetLastError();
+}
+
+// K_b values, read from the assembled matrix before the Dirichlet zeroing. The boundary matrix of the reference
+// receives the same element blocks in the same element order as the matrix (CMat_BlkCrs::Mearge,
+// src/matvec/mat_blkcrs.cpp:754-823), so each of its entries is the matrix entry of the same position.
+__global__ void k_gather_terms(long long nterm, const long long* src, long long ndia, const double* dia,
+                               const double* crs, double* val) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nterm) return;
+  const long long k = src[t];
+  val[t] = (k < ndia) ? dia[k] : crs[k - ndia];
+}
+
+hipError_t launch_gather_terms(long long nterm, const long long* src, long long ndia, const double* dia,
+                               const double* crs, double* val, hipStream_t st) {
+  if (nterm == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_gather_terms, dim3((unsigned)((nterm + nt - 1) / nt)), dim3(nt), 0, st, nterm, src, ndia, dia,
+                     crs, val);
+  return hipGetLastError();
+}
+
+// CLinearSystem_Save::MakeResidual (src/femls/linearsystem_fieldsave.cpp:206-247): update = field values,
+// residual = force, then CMat_BlkCrs::MatVec(-1, update, 1, residual) over the boundary matrix
+// (src/matvec/mat_blkcrs.cpp:955-1170): residual[i] += (-1 * K_b[i][j]) * u[j] entry by entry, which is
+// residual[i] - K_b[i][j] * u[j] exactly. The boundary matrix keeps only fixed columns in free rows; its other
+// entries are zero and leave the sum unchanged, so each dof runs over its kept terms, in the reference's order
+// (diagonal block first, then the row's blocks in pattern order). One thread per scalar dof.
+__global__ void k_make_residual(long long m, int len, const int* order, const unsigned* bptr, const unsigned* bcol,
+                                const double* bval, const double* force, const double* u, double* res, double* upd) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long long i0 = k / len;
+  const int d = (int)(k - i0 * len);
+  const long long i1 = order ? order[i0] : i0;
+  double r = force[k];
+  for (unsigned t = bptr[k]; t < bptr[k + 1]; t++) r = r - bval[t] * u[bcol[t]];
+  res[k] = r;
+  upd[k] = u[i1 * len + d];
+}
+
+hipError_t launch_make_residual(int n, int len, const int* order, const unsigned* bptr, const unsigned* bcol,
+                                const double* bval, const double* force, const double* u, double* res, double* upd,
+                                hipStream_t st) {
+  const long long m = (long long)n * len;
+  if (m == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_make_residual, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, bptr, bcol,
+                     bval, force, u, res, upd);
+  return hipGetLastError();
+}
+
+// CLinearSystem_Save::UpdateValueOfField, VALUE (src/femls/linearsystem_fieldsave.cpp:249-296): the update takes
+// the field values at fixed dofs, then the field takes the update (CNodeAry::SetValueToNodeSegment).
+__global__ void k_update_field_save(long long m, int len, const int* order, const int* bc, double* upd, double* u) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long long i0 = k / len;
+  const int d = (int)(k - i0 * len);
+  const long long i1 = order ? order[i0] : i0;
+  double* f = &u[i1 * len + d];
+  if (bc[k]) upd[k] = *f;
+  *f = upd[k];
+}
+
+hipError_t launch_update_field_save(int n, int len, const int* order, const int* bc, double* upd, double* u,
+                                    hipStream_t st) {
+  const long long m = (long long)n * len;
+  if (m == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_update_field_save, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, bc,
+                     upd, u);
   return hipGetLastError();
 }
 

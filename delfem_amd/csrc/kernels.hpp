@@ -29,11 +29,21 @@ struct AsmArgs {
   double* crs;
   double* dia;
   double* res;
+  bool save;  // the *_SaveStiffMat element loops: external force into res, no internal force
 };
 hipError_t launch_assemble(const AsmArgs& a, hipStream_t st);
 hipError_t launch_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
                          double* res, const int* bc, hipStream_t st);
 hipError_t launch_update_field(int n, int len, const int* order, const double* upd, double* u, hipStream_t st);
+// saved stiffness: K_b values from the matrix before the Dirichlet zeroing (src < ndia: diagonal blocks, else
+// off-diagonal blocks at src - ndia); MakeResidual; UpdateValueOfField of CLinearSystem_Save
+hipError_t launch_gather_terms(long long nterm, const long long* src, long long ndia, const double* dia,
+                               const double* crs, double* val, hipStream_t st);
+hipError_t launch_make_residual(int n, int len, const int* order, const unsigned* bptr, const unsigned* bcol,
+                                const double* bval, const double* force, const double* u, double* res, double* upd,
+                                hipStream_t st);
+hipError_t launch_update_field_save(int n, int len, const int* order, const int* bc, double* upd, double* u,
+                                    hipStream_t st);
 
 // ---- linear algebra (k_linalg.hip) ----
 hipError_t launch_fill(double* x, double v, long long m, hipStream_t st);

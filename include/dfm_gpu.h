@@ -149,6 +149,28 @@ int dfm_sys_set_load_scale(dfm_sys sys, double scale);
 /* Matrix values in the original numbering: crs[ncrs*len*len], dia[n*len*len]. */
 int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia);
 
+/* ---- Saved stiffness (CLinearSystem_Save, include/delfem/femls/linearsystem_fieldsave.h:59-106) -------- */
+/* InitializeMarge + the *_SaveStiffMat element loop + FinalizeMarge of CLinearSystem_Save
+ * (src/femls/linearsystem_fieldsave.cpp:169-204): K as dfm_sys_assemble builds it (Dirichlet rows and columns
+ * applied), the external force f kept apart (GetForce, :158-167), and the boundary coupling K_b, the entries
+ * of K in free rows and fixed columns (GetMatrix_Boundary, :144-156, pattern :334-362), kept from before the
+ * Dirichlet zeroing. The residual handle is left zero. Kinds, as the reference has them:
+ * DFM_POISSON on triangles (AddLinSys_Poisson on CLinearSystem_Save, src/femeqn/eqn_poisson.cpp:494-569;
+ * the reference stops on tetrahedra), DFM_SOLID2D (AddLinSys_LinearSolid2D_Static_SaveStiffMat,
+ * src/femeqn/eqn_linear_solid2d.cpp:861-960), DFM_SOLID3D (AddLinSys_LinearSolid3D_Static_SaveStiffMat,
+ * src/femeqn/eqn_linear_solid3d.cpp:102-126, :425-521). Set the Dirichlet flags first. */
+int dfm_sys_assemble_save(dfm_sys sys, int kind, const double* params, int npar, int pstride);
+/* CLinearSystem_Save::MakeResidual (src/femls/linearsystem_fieldsave.cpp:206-247): the update handle takes the
+ * field values, the residual handle r = f - K_b * u (the same terms in the same order as the reference), and
+ * ||r|| is returned in *res_norm (may be NULL). Re-solving after new Dirichlet values needs only this call. */
+int dfm_sys_make_residual(dfm_sys sys, double* res_norm);
+/* CLinearSystem_Save::UpdateValueOfField (VALUE, :249-332): the update takes the field values at fixed dofs,
+ * then the field is set to the update (u = update, not u += update). */
+int dfm_sys_update_field_save(dfm_sys sys);
+/* The saved external force f (original numbering, n*len) and the number of scalar terms of K_b. */
+int dfm_sys_get_force(dfm_sys sys, double* f);
+int dfm_sys_boundary_terms(dfm_sys sys, long long* nterm);
+
 /* ---- ILinearSystemPreconditioner_Sol handle ops (include/delfem/ls/linearsystem_interface_solver.h:63-82) */
 int dfm_sys_resize_tmp(dfm_sys sys, int ntmp);
 int dfm_sys_ntmp(dfm_sys sys, int* ntmp);
@@ -218,8 +240,9 @@ int dfm_sys_last_solve_info(dfm_sys sys, int* reliable_updates, int* fell_back);
 /* Average device time of one launch of a hot kernel over `reps` back-to-back launches, timed with HIP events
  * on the system's own stream: which = 0 SpMV FP64 (y = A x), 1 SpMV FP32, 2 ILU apply FP64 (all phases of
  * the forward and backward sweeps), 3 ILU apply FP32, 4 / 5 the sweep phases of a PCG iteration as the mixed
- * (FP32) / FP64 loop runs them (forward phases with the r and x update, backward phases with (r, z)). Also
- * reports the algorithmic bytes of one launch (all phases). No counterpart in the reference. */
+ * (FP32) / FP64 loop runs them (forward phases with the r and x update, backward phases with (r, z)), 6 the
+ * MakeResidual kernel of a saved system (it rewrites the residual and update handles as dfm_sys_make_residual
+ * does). Also reports the algorithmic bytes of one launch (all phases). No counterpart in the reference. */
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes);
 /* ---- batches: many systems solved together (serving, instance batches) ------------------------------
  * A batch advances every system added to it with one launch per kernel for all running systems of one block

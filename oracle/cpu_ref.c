@@ -1431,3 +1431,190 @@ int cr_solve_mat_pbicgstab(const cr_system* s, double* r, double* x, double* con
   cr_solve_pbicgstab(s, r, x, conv_ratio, iter);
   return 1;
 }
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Saved stiffness: CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp) with its boundary matrix,  */
+/* a CMat_BlkCrs (src/matvec/mat_blkcrs.cpp), restated step by step.                                     */
+
+long cr_boundary_pattern(int n, int len, const unsigned* rowptr, const unsigned* colind, const int* flags,
+                         unsigned* browptr, unsigned* bcolind) {
+  /* SetPatternBoundary (:1177-1265): rows with a free dof; in them the diagonal block first when it has a
+   * fixed dof, then the blocks of the row (pattern order) whose node has a fixed dof */
+  long nb = 0;
+  if (browptr) browptr[0] = 0;
+  for (long i = 0; i < n; i++) {
+    int has_free = 0, has_fixed = 0;
+    for (int d = 0; d < len; d++) {
+      if (flags[i * len + d] == 0) has_free = 1;
+      else has_fixed = 1;
+    }
+    if (has_free) {
+      if (has_fixed) {
+        if (bcolind) bcolind[nb] = (unsigned)i;
+        nb++;
+      }
+      for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        const unsigned j = colind[q];
+        int fj = 0;
+        for (int d = 0; d < len; d++)
+          if (flags[(long)j * len + d] != 0) fj = 1;
+        if (!fj) continue;
+        if (bcolind) bcolind[nb] = j;
+        nb++;
+      }
+    }
+    if (browptr) browptr[i + 1] = (unsigned)nb;
+  }
+  return nb;
+}
+
+/* AddLinearSystem_Poisson2D_P1 on CLinearSystem_Save, src/femeqn/eqn_poisson.cpp:494-569 */
+static void elem_poisson2d_save(const double co[][3], const double* p, double* emat, double* eforce) {
+  const double alpha = p[0], source = p[1];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double dldx[3][2];
+  tri_dldx(dldx, co[0], co[1], co[2]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      emat[i * 3 + j] = alpha * area * (dldx[i][0] * dldx[j][0] + dldx[i][1] * dldx[j][1]);
+  for (int i = 0; i < 3; i++) eforce[i] = source * area * 0.333333333333333333;
+}
+
+/* AddLinSys_LinearSolid2D_Static_P1_SaveStiffMat, src/femeqn/eqn_linear_solid2d.cpp:861-960, and
+ * AddLinSys_LinearSolid3D_Static_P1 on CLinearSystem_Save, src/femeqn/eqn_linear_solid3d.cpp:425-521:
+ * emat[ino][jno][idim][jdim], zero-initialised and accumulated as written there */
+static void elem_solid_save(int ndim, const double co[][3], const double* p, double* emat, double* eforce) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const int nno = ndim + 1, bs = ndim * ndim;
+  double dldx[4][3];
+  double w;
+  if (ndim == 2) {
+    double d2[3][2];
+    w = tri_area(co[0], co[1], co[2]);
+    tri_dldx(d2, co[0], co[1], co[2]);
+    for (int k = 0; k < 3; k++) { dldx[k][0] = d2[k][0]; dldx[k][1] = d2[k][1]; }
+  } else {
+    w = tet_volume(co[0], co[1], co[2], co[3]);
+    tet_dldx(dldx, co[0], co[1], co[2], co[3]);
+  }
+  for (int i = 0; i < nno * nno * bs; i++) emat[i] = 0.0;
+  for (int i = 0; i < nno * ndim; i++) eforce[i] = 0.0;
+  for (int ino = 0; ino < nno; ino++) {
+    for (int jno = 0; jno < nno; jno++) {
+      double* e = &emat[(ino * nno + jno) * bs];
+      double dtmp1 = 0.0;
+      for (int idim = 0; idim < ndim; idim++) {
+        for (int jdim = 0; jdim < ndim; jdim++)
+          e[idim * ndim + jdim] += w * (lambda * dldx[ino][idim] * dldx[jno][jdim] + myu * dldx[jno][idim] * dldx[ino][jdim]);
+        dtmp1 += dldx[ino][idim] * dldx[jno][idim];
+      }
+      for (int idim = 0; idim < ndim; idim++) e[idim * ndim + idim] += w * myu * dtmp1;
+    }
+  }
+  for (int ino = 0; ino < nno; ino++)
+    for (int d = 0; d < ndim; d++)
+      eforce[ino * ndim + d] = (ndim == 2) ? w * rho * p[3 + d] * 0.3333333333333333333 : w * rho * p[3 + d] * 0.25;
+}
+
+/* CMat_BlkCrs::Mearge (src/matvec/mat_blkcrs.cpp:754-823): blocks absent from the pattern are skipped */
+static void merge_crs(int nno, const int* no, int bs, const double* emat, const unsigned* browptr,
+                      const unsigned* bcolind, double* bval) {
+  for (int a = 0; a < nno; a++) {
+    const int i = no[a];
+    for (int c = 0; c < nno; c++) {
+      const unsigned j = (unsigned)no[c];
+      long pos = -1;
+      for (unsigned q = browptr[i]; q < browptr[i + 1]; q++)
+        if (bcolind[q] == j) { pos = q; break; }
+      if (pos < 0) continue;
+      const double* pin = &emat[(a * nno + c) * bs];
+      double* pout = &bval[pos * bs];
+      for (int k = 0; k < bs; k++) pout[k] += pin[k];
+    }
+  }
+}
+
+int cr_assemble_save(int kind, int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                     const double* params, int pstride, int len, const unsigned* rowptr, const unsigned* colind,
+                     double* crs, double* dia, const unsigned* browptr, const unsigned* bcolind, double* bval,
+                     double* force) {
+  (void)nnode;
+  const int nno = (ndim == 2) ? 3 : 4;
+  const int bs = len * len;
+  double emat[144], eforce[12], co[4][3];
+  if (kind == CR_POISSON && ndim != 2) return -1; /* TET11: "Not Implimented" there (eqn_poisson.cpp:704-707) */
+  if (kind != CR_POISSON && kind != CR_SOLID2D && kind != CR_SOLID3D) return -1;
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * nno];
+    const double* p = &params[e * pstride];
+    for (int a = 0; a < nno; a++)
+      for (int d = 0; d < 3; d++) co[a][d] = (d < ndim) ? coords[(long)no[a] * ndim + d] : 0.0;
+    if (kind == CR_POISSON) elem_poisson2d_save(co, p, emat, eforce);
+    else elem_solid_save(ndim, co, p, emat, eforce);
+    merge(nno, no, bs, emat, rowptr, colind, crs, dia);
+    merge_crs(nno, no, bs, emat, browptr, bcolind, bval);
+    for (int a = 0; a < nno; a++)
+      for (int d = 0; d < len; d++) force[(long)no[a] * len + d] += eforce[a * len + d];
+  }
+  return 0;
+}
+
+void cr_boundary_bc(int n, int len, const unsigned* browptr, const unsigned* bcolind, double* bval, const int* flags) {
+  const int bs = len * len;
+  /* SetBoundaryConditionInverse_Colum (:876-900): columns of free dofs to zero */
+  for (unsigned q = 0; q < browptr[n]; q++) {
+    const unsigned j = bcolind[q];
+    for (int jd = 0; jd < len; jd++) {
+      if (flags[(long)j * len + jd] != 0) continue;
+      for (int id = 0; id < len; id++) bval[(long)q * bs + id * len + jd] = 0.0;
+    }
+  }
+  /* SetBoundaryCondition_Row (:910-953): rows of fixed dofs to zero */
+  for (long i = 0; i < n; i++)
+    for (int id = 0; id < len; id++) {
+      if (flags[i * len + id] == 0) continue;
+      for (unsigned q = browptr[i]; q < browptr[i + 1]; q++)
+        for (int jd = 0; jd < len; jd++) bval[(long)q * bs + id * len + jd] = 0.0;
+    }
+}
+
+double cr_make_residual_save(int n, int len, const unsigned* browptr, const unsigned* bcolind, const double* bval,
+                             const double* force, const double* u, double* res, double* upd) {
+  const double alpha = -1.0, beta = 1.0;
+  const long m = (long)n * len;
+  const int bs = len * len;
+  /* update = field values; residual = force (MakeResidual, src/femls/linearsystem_fieldsave.cpp:206-247) */
+  for (long k = 0; k < m; k++) upd[k] = u[k];
+  for (long k = 0; k < m; k++) res[k] = force[k];
+  /* CMat_BlkCrs::MatVec(alpha, update, beta, residual) (:955-1170): 1x1 blocks (:1007-1020), otherwise the
+   * general branch (:1150-1168) */
+  if (len == 1) {
+    for (long i = 0; i < n; i++) {
+      double* lval = &res[i];
+      *lval *= beta;
+      for (unsigned q = browptr[i]; q < browptr[i + 1]; q++) *lval += alpha * bval[q] * upd[bcolind[q]];
+    }
+  } else {
+    for (long i = 0; i < n; i++) {
+      double* iy = &res[i * len];
+      for (int id = 0; id < len; id++) iy[id] *= beta;
+      for (unsigned q = browptr[i]; q < browptr[i + 1]; q++) {
+        const double* jx = &upd[(long)bcolind[q] * len];
+        for (int id = 0; id < len; id++)
+          for (int jd = 0; jd < len; jd++) iy[id] += alpha * bval[(long)q * bs + id * len + jd] * jx[jd];
+      }
+    }
+  }
+  /* sqrt(DOT(-1,-1)) */
+  double sq = 0.0;
+  for (long k = 0; k < m; k++) sq += res[k] * res[k];
+  return sqrt(sq);
+}
+
+void cr_update_field_save(int n, int len, const int* flags, double* upd, double* u) {
+  /* UpdateValueOfField of CLinearSystem_Save, VALUE (src/femls/linearsystem_fieldsave.cpp:249-296) */
+  const long m = (long)n * len;
+  for (long k = 0; k < m; k++)
+    if (flags[k] != 0) upd[k] = u[k];
+  for (long k = 0; k < m; k++) u[k] = upd[k];
+}

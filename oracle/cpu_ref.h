@@ -95,6 +95,22 @@ int cr_solve_mat_bicgstab(const cr_system* s, double* res, double* upd, double* 
 int cr_solve_mat_pbicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
 int cr_solve_bicgstab(const cr_system* s, double* res, double* upd, double* conv_ratio, unsigned* iter);
 
+/* Saved stiffness (CLinearSystem_Save, src/femls/linearsystem_fieldsave.cpp): the boundary matrix pattern
+ * (CMat_BlkCrs::SetPatternBoundary; returns its block count, fills browptr/bcolind when non-NULL), the
+ * *_SaveStiffMat element loops (matrix, boundary matrix, external force; kinds CR_POISSON on triangles, CR_SOLID2D,
+ * CR_SOLID3D), the boundary matrix's Dirichlet treatment of FinalizeMarge, MakeResidual (returns ||res||) and
+ * UpdateValueOfField. */
+long cr_boundary_pattern(int n, int len, const unsigned* rowptr, const unsigned* colind, const int* flags,
+                         unsigned* browptr, unsigned* bcolind);
+int cr_assemble_save(int kind, int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                     const double* params, int pstride, int len, const unsigned* rowptr, const unsigned* colind,
+                     double* crs, double* dia, const unsigned* browptr, const unsigned* bcolind, double* bval,
+                     double* force);
+void cr_boundary_bc(int n, int len, const unsigned* browptr, const unsigned* bcolind, double* bval, const int* flags);
+double cr_make_residual_save(int n, int len, const unsigned* browptr, const unsigned* bcolind, const double* bval,
+                             const double* force, const double* u, double* res, double* upd);
+void cr_update_field_save(int n, int len, const int* flags, double* upd, double* u);
+
 #ifdef __cplusplus
 }
 #endif

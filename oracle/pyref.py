@@ -38,6 +38,8 @@ def cpu_ref():
         _cr.cr_iluk_pattern.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_long, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p]
         _cr.cr_ordering.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _cr.cr_boundary_pattern.restype = C.c_long
+        _cr.cr_make_residual_save.restype = C.c_double
     return _cr
 
 
@@ -98,6 +100,53 @@ class CpuSystem:
                                    p(params), npar, pstride, p(glob), p(u), p(v), self.len, p(self.rowptr),
                                    p(self.colind), p(self.crs), p(self.dia), p(self.res))
         assert rc == 0
+
+    def assemble_save(self, kind, params, flags):
+        """CLinearSystem_Save (restated): the *_SaveStiffMat element loop into the matrix, the boundary matrix
+        (self.brp / self.bci / self.bval) and the force (self.force); FinalizeMarge's Dirichlet treatment of both.
+        The residual stays zero."""
+        params = _f64(params)
+        pstride = 0 if params.ndim == 1 else params.shape[1]
+        flags = _i32(flags)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        lib = cpu_ref()
+        n, bs = self.nnode, self.len * self.len
+        nb = lib.cr_boundary_pattern(n, self.len, p(self.rowptr), p(self.colind), p(flags), None, None)
+        self.brp = np.zeros(n + 1, np.uint32)
+        bci = np.zeros(max(nb, 1), np.uint32)
+        lib.cr_boundary_pattern(n, self.len, p(self.rowptr), p(self.colind), p(flags), p(self.brp), p(bci))
+        self.bci = bci[:nb]
+        self.bval = np.zeros(max(nb, 1) * bs)
+        self.force = np.zeros(n * self.len)
+        self.zero()
+        rc = lib.cr_assemble_save(kind, self.ndim, n, p(self.coords), self.nelem, p(self.lnods), p(params), pstride,
+                                  self.len, p(self.rowptr), p(self.colind), p(self.crs), p(self.dia), p(self.brp),
+                                  p(bci), p(self.bval), p(self.force))
+        assert rc == 0, rc
+        self.bval = self.bval[: nb * bs]
+        self.set_bc(flags)
+        lib.cr_boundary_bc(n, self.len, p(self.brp), p(bci), p(self.bval), p(flags))
+        self.flags = flags
+
+    def make_residual(self, u):
+        """MakeResidual (restated): returns (||r||, residual, update); the residual also becomes self.res."""
+        u = _f64(u)
+        res = np.zeros_like(u)
+        upd = np.zeros_like(u)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        bci = self.bci if self.bci.size else np.zeros(1, np.uint32)
+        bval = self.bval if self.bval.size else np.zeros(1)
+        norm = cpu_ref().cr_make_residual_save(self.nnode, self.len, p(self.brp), p(bci), p(bval), p(self.force),
+                                               p(u), p(res), p(upd))
+        self.res = res.copy()
+        return norm, res, upd
+
+    def update_field_save(self, upd, u):
+        """UpdateValueOfField of CLinearSystem_Save (restated): returns (update, field)."""
+        upd, u = _f64(upd).copy(), _f64(u).copy()
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        cpu_ref().cr_update_field_save(self.nnode, self.len, p(self.flags), p(upd), p(u))
+        return upd, u
 
     def traction(self, faces, trac):
         faces, trac = _i32(faces), _f64(trac)
@@ -203,6 +252,9 @@ def reference():
             raise RuntimeError(f"{REF_SO} is not built (make -C oracle ref, where the reference sources exist)")
         _ref = C.CDLL(REF_SO)
         _ref.ref_sys_create.restype = C.c_void_p
+        _ref.ref_sys_create2.restype = C.c_void_p
+        _ref.ref_sys_make_residual.restype = C.c_double
+        _ref.ref_sys_make_residual.argtypes = [C.c_void_p]
         for name in ["ref_sys_destroy", "ref_sys_ncrs", "ref_sys_pattern", "ref_sys_set_field", "ref_sys_get_field",
                      "ref_sys_set_bc", "ref_sys_assemble", "ref_sys_get_matrix", "ref_sys_get_vector",
                      "ref_sys_set_vector", "ref_sys_matvec", "ref_ilu0_factor", "ref_ilu_apply", "ref_sys_solve",
@@ -218,7 +270,7 @@ def reference():
 class RefSystem:
     """The reference DelFEM driven through its own API (oracle/ref_capi.cpp)."""
 
-    def __init__(self, ndim, coords, lnods, len_, groups=None):
+    def __init__(self, ndim, coords, lnods, len_, groups=None, save=False):
         self.ndim, self.len = ndim, len_
         self.coords = _f64(coords)
         self.lnods = _i32(lnods)
@@ -230,8 +282,8 @@ class RefSystem:
         self.ngroup = ngroup
         lib = reference()
         p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
-        self.h = C.c_void_p(lib.ref_sys_create(ndim, self.nnode, p(self.coords), self.nelem, p(self.lnods), ngroup,
-                                               p(self.groups), len_))
+        self.h = C.c_void_p(lib.ref_sys_create2(ndim, self.nnode, p(self.coords), self.nelem, p(self.lnods), ngroup,
+                                                p(self.groups), len_, 1 if save else 0))
         self.ncrs = lib.ref_sys_ncrs(self.h)
 
     def __del__(self):
@@ -355,3 +407,23 @@ class RefSystem:
 
     def update_field(self):
         reference().ref_sys_update_field(self.h)
+
+    def assemble_save(self, kind, group_params):
+        """CLinearSystem_Save: InitializeMarge, the *_SaveStiffMat loop, FinalizeMarge (system made with save)."""
+        gp = _f64(group_params)
+        if gp.ndim == 1:
+            gp = gp[None, :]
+        assert gp.shape[0] == self.ngroup
+        rc = reference().ref_sys_assemble_save(self.h, kind, gp.ctypes.data_as(C.c_void_p), gp.shape[1])
+        assert rc == 0, rc
+
+    def make_residual(self):
+        return reference().ref_sys_make_residual(self.h)
+
+    def force(self):
+        out = np.zeros(self.nnode * self.len)
+        reference().ref_sys_get_force(self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def update_field_save(self):
+        reference().ref_sys_update_field_save(self.h)

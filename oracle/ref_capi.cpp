@@ -18,6 +18,7 @@
 #include "delfem/field.h"
 #include "delfem/mesh_interface.h"
 #include "delfem/femls/linearsystem_field.h"
+#include "delfem/femls/linearsystem_fieldsave.h"
 #include "delfem/ls/linearsystem.h"
 #include "delfem/ls/preconditioner.h"
 #include "delfem/ls/solver_ls_iter.h"
@@ -64,7 +65,9 @@ class CGroupMesh : public Msh::IMesh {
 
 struct RefSys {
   Fem::Field::CFieldWorld world;
-  Fem::Ls::CLinearSystem_Field ls;
+  Fem::Ls::CLinearSystem_Field* ls = 0;  // a CLinearSystem_Save for the saved-stiffness path
+  bool save = false;
+  ~RefSys() { delete ls; }
   CGroupMesh mesh;
   unsigned int id_base, id_field;
   std::vector<unsigned int> id_ea;       // per group
@@ -72,16 +75,20 @@ struct RefSys {
   int ndim, nnode, len;
 };
 
-MatVec::CMatDia_BlkCrs& Mat(RefSys* s) { return s->ls.m_ls.GetMatrix(0); }
+MatVec::CMatDia_BlkCrs& Mat(RefSys* s) { return s->ls->m_ls.GetMatrix(0); }
 
 }  // namespace
 
 extern "C" {
 
 // kind of field: len = 1 (scalar) or ndim (vector). Element groups are 0..ngroup-1 per element.
-void* ref_sys_create(int ndim, int nnode, const double* coords, int nelem, const int* lnods,
-                     int ngroup, const int* elem_group, int len) {
+// save != 0: the linear system is a CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp)
+void* ref_sys_create2(int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                      int ngroup, const int* elem_group, int len, int save) {
   RefSys* s = new RefSys;
+  s->save = save != 0;
+  if (s->save) s->ls = new Fem::Ls::CLinearSystem_Save;
+  else s->ls = new Fem::Ls::CLinearSystem_Field;
   s->ndim = ndim; s->nnode = nnode; s->len = len;
   const int npoel = (ndim == 2) ? 3 : 4;
   s->mesh.ndim = ndim;
@@ -102,13 +109,18 @@ void* ref_sys_create(int ndim, int nnode, const double* coords, int nelem, const
   const Fem::Field::CField& field = s->world.GetField(s->id_field);
   s->id_ea = field.GetAryIdEA();
   assert((int)s->id_ea.size() == ngroup);
-  s->ls.AddPattern_Field(s->id_field, s->world);
+  s->ls->AddPattern_Field(s->id_field, s->world);
   if (ndim == 3 && len == 3 && ngroup > 1) {
     for (int g = 0; g < ngroup; g++) {
       s->id_partial.push_back(s->world.GetPartialField(s->id_field, s->id_ea[g]));
     }
   }
   return s;
+}
+
+void* ref_sys_create(int ndim, int nnode, const double* coords, int nelem, const int* lnods,
+                     int ngroup, const int* elem_group, int len) {
+  return ref_sys_create2(ndim, nnode, coords, nelem, lnods, ngroup, elem_group, len, 0);
 }
 
 void ref_sys_destroy(void* h) { delete (RefSys*)h; }
@@ -155,8 +167,8 @@ void ref_sys_get_field(void* h, double* u) {
 
 void ref_sys_set_bc(void* h, const int* flags) {
   RefSys* s = (RefSys*)h;
-  s->ls.m_ls.ClearFixedBoundaryCondition();
-  MatVec::CBCFlag& bc = s->ls.m_ls.GetBCFlag(0);
+  s->ls->m_ls.ClearFixedBoundaryCondition();
+  MatVec::CBCFlag& bc = s->ls->m_ls.GetBCFlag(0);
   for (int i = 0; i < s->nnode; i++)
     for (int d = 0; d < s->len; d++)
       if (flags[(size_t)i * s->len + d]) bc.SetBC(i, d);
@@ -169,41 +181,41 @@ void ref_sys_set_bc(void* h, const int* flags) {
 int ref_sys_assemble(void* h, int kind, const double* params, int npar, const double* glob,
                      double* res_norm) {
   RefSys* s = (RefSys*)h;
-  s->ls.InitializeMarge();
+  s->ls->InitializeMarge();
   const int ngroup = (int)s->id_ea.size();
   for (int g = 0; g < ngroup; g++) {
     const double* p = &params[(size_t)g * npar];
     const unsigned int id_ea = s->id_ea[g];
     bool ok = true;
     switch (kind) {
-      case 0: ok = Fem::Eqn::AddLinSys_Poisson(s->ls, p[0], p[1], s->world, s->id_field, id_ea); break;
+      case 0: ok = Fem::Eqn::AddLinSys_Poisson(*s->ls, p[0], p[1], s->world, s->id_field, id_ea); break;
       case 1:
-        ok = Fem::Eqn::AddLinSys_LinearSolid2D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], s->world,
+        ok = Fem::Eqn::AddLinSys_LinearSolid2D_Static(*s->ls, p[0], p[1], p[2], p[3], p[4], s->world,
                                                       s->id_field, id_ea);
         break;
       case 2: {
         const unsigned int idf = s->id_partial.empty() ? s->id_field : s->id_partial[g];
-        ok = Fem::Eqn::AddLinSys_LinearSolid3D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], p[5], s->world, idf);
+        ok = Fem::Eqn::AddLinSys_LinearSolid3D_Static(*s->ls, p[0], p[1], p[2], p[3], p[4], p[5], s->world, idf);
         if (s->id_partial.empty()) g = ngroup;  // one call covers every element array
         break;
       }
       case 3:
-        ok = Fem::Eqn::AddLinSys_StVenant2D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], s->world,
+        ok = Fem::Eqn::AddLinSys_StVenant2D_Static(*s->ls, p[0], p[1], p[2], p[3], p[4], s->world,
                                                    s->id_field, id_ea);
         break;
       case 4:
-        ok = Fem::Eqn::AddLinSys_StVenant3D_Static(s->ls, p[0], p[1], p[2], p[3], p[4], p[5], s->world,
+        ok = Fem::Eqn::AddLinSys_StVenant3D_Static(*s->ls, p[0], p[1], p[2], p[3], p[4], p[5], s->world,
                                                    s->id_field, id_ea);
         break;
       case 5:
-        ok = Fem::Eqn::AddLinSys_Diffusion(glob[0], glob[1], s->ls, p[0], p[1], p[2], s->world,
+        ok = Fem::Eqn::AddLinSys_Diffusion(glob[0], glob[1], *s->ls, p[0], p[1], p[2], s->world,
                                            s->id_field, id_ea);
         break;
       default: return -1;
     }
     if (!ok) return -2;
   }
-  *res_norm = s->ls.FinalizeMarge();
+  *res_norm = s->ls->FinalizeMarge();
   return 0;
 }
 
@@ -225,14 +237,14 @@ void ref_sys_get_matrix(void* h, double* crs, double* dia) {
 // iv: -1 residual, -2 update
 void ref_sys_get_vector(void* h, int iv, double* out) {
   RefSys* s = (RefSys*)h;
-  const MatVec::CVector_Blk& v = s->ls.m_ls.GetVector(iv, 0);
+  const MatVec::CVector_Blk& v = s->ls->m_ls.GetVector(iv, 0);
   for (int i = 0; i < s->nnode; i++)
     for (int d = 0; d < s->len; d++) out[(size_t)i * s->len + d] = v.GetValue(i, d);
 }
 
 void ref_sys_set_vector(void* h, int iv, const double* in) {
   RefSys* s = (RefSys*)h;
-  MatVec::CVector_Blk& v = s->ls.m_ls.GetVector(iv, 0);
+  MatVec::CVector_Blk& v = s->ls->m_ls.GetVector(iv, 0);
   for (int i = 0; i < s->nnode; i++)
     for (int d = 0; d < s->len; d++) v.SetValue(i, d, in[(size_t)i * s->len + d]);
 }
@@ -242,7 +254,7 @@ void ref_sys_matvec(void* h, double alpha, const double* x, double beta, double*
   RefSys* s = (RefSys*)h;
   ref_sys_set_vector(h, -1, x);
   ref_sys_set_vector(h, -2, y);
-  s->ls.m_ls.MATVEC(alpha, -1, beta, -2);
+  s->ls->m_ls.MATVEC(alpha, -1, beta, -2);
   ref_sys_get_vector(h, -2, y);
 }
 
@@ -317,13 +329,13 @@ int ref_ilu_apply(void* h, int lev, const int* order, const double* r, double* z
   LsSol::CPreconditioner_ILU prec;
   prec.SetFillInLevel(lev);
   if (order) prec.SetOrdering(std::vector<int>(order, order + s->nnode));
-  prec.SetLinearSystem(s->ls.m_ls);
-  if (!prec.SetValue(s->ls.m_ls)) return 0;
-  if (s->ls.m_ls.GetTmpVectorArySize() < 1) s->ls.m_ls.ReSizeTmpVecSolver(1);
-  MatVec::CVector_Blk& v = s->ls.m_ls.GetVector(0, 0);
+  prec.SetLinearSystem(s->ls->m_ls);
+  if (!prec.SetValue(s->ls->m_ls)) return 0;
+  if (s->ls->m_ls.GetTmpVectorArySize() < 1) s->ls->m_ls.ReSizeTmpVecSolver(1);
+  MatVec::CVector_Blk& v = s->ls->m_ls.GetVector(0, 0);
   for (int i = 0; i < s->nnode; i++)
     for (int d = 0; d < s->len; d++) v.SetValue(i, d, r[(size_t)i * s->len + d]);
-  prec.SolvePrecond(s->ls.m_ls, 0);
+  prec.SolvePrecond(s->ls->m_ls, 0);
   for (int i = 0; i < s->nnode; i++)
     for (int d = 0; d < s->len; d++) z[(size_t)i * s->len + d] = v.GetValue(i, d);
   return 1;
@@ -335,14 +347,14 @@ int ref_ilu_apply(void* h, int lev, const int* order, const double* r, double* z
 // method: 0 Solve_PCG, 1 Solve_PBiCGSTAB (ILU(lev) under `order`), 2 Solve_CG, 3 Solve_BiCGSTAB (no preconditioner)
 int ref_sys_solve(void* h, int method, int lev, const int* order, double* conv_ratio, unsigned int* iter) {
   RefSys* s = (RefSys*)h;
-  if (method == 2) return LsSol::Solve_CG(*conv_ratio, *iter, s->ls.m_ls) ? 1 : 0;
-  if (method == 3) return LsSol::Solve_BiCGSTAB(*conv_ratio, *iter, s->ls.m_ls) ? 1 : 0;
+  if (method == 2) return LsSol::Solve_CG(*conv_ratio, *iter, s->ls->m_ls) ? 1 : 0;
+  if (method == 3) return LsSol::Solve_BiCGSTAB(*conv_ratio, *iter, s->ls->m_ls) ? 1 : 0;
   LsSol::CPreconditioner_ILU prec;
   prec.SetFillInLevel(lev);
   if (order) prec.SetOrdering(std::vector<int>(order, order + s->nnode));
-  prec.SetLinearSystem(s->ls.m_ls);
-  if (!prec.SetValue(s->ls.m_ls)) return -1;
-  LsSol::CLinearSystemPreconditioner lsp(s->ls.m_ls, prec);
+  prec.SetLinearSystem(s->ls->m_ls);
+  if (!prec.SetValue(s->ls->m_ls)) return -1;
+  LsSol::CLinearSystemPreconditioner lsp(s->ls->m_ls, prec);
   bool ok = false;
   if (method == 0) ok = LsSol::Solve_PCG(*conv_ratio, *iter, lsp);
   else ok = LsSol::Solve_PBiCGSTAB(*conv_ratio, *iter, lsp);
@@ -367,8 +379,8 @@ void ref_ordering(void* h, int kind, int* order) {
 int ref_mat_solve(void* h, int method, double* conv_ratio, unsigned int* iter) {
   RefSys* s = (RefSys*)h;
   const MatVec::CMatDia_BlkCrs& a = Mat(s);
-  MatVec::CVector_Blk& r = s->ls.m_ls.GetVector(-1, 0);
-  MatVec::CVector_Blk& u = s->ls.m_ls.GetVector(-2, 0);
+  MatVec::CVector_Blk& r = s->ls->m_ls.GetVector(-1, 0);
+  MatVec::CVector_Blk& u = s->ls->m_ls.GetVector(-2, 0);
   if (method == 0) return MatVec::Sol::Solve_CG(*conv_ratio, *iter, a, r, u) ? 1 : 0;
   if (method == 2) return MatVec::Sol::Solve_BiCGSTAB(*conv_ratio, *iter, a, r, u) ? 1 : 0;
   MatVec::CMatDiaFrac_BlkCrs frac(0, a);
@@ -381,7 +393,61 @@ int ref_mat_solve(void* h, int method, double* conv_ratio, unsigned int* iter) {
 // u_field += update (CLinearSystem_Field::UpdateValueOfField, VALUE)
 void ref_sys_update_field(void* h) {
   RefSys* s = (RefSys*)h;
-  s->ls.UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);
+  s->ls->UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);
+}
+
+// Saved stiffness on a system made with save = 1: InitializeMarge + the *_SaveStiffMat element loop +
+// FinalizeMarge of CLinearSystem_Save. kind 0 Poisson (triangles), 1 LinearSolid2D, 2 LinearSolid3D.
+int ref_sys_assemble_save(void* h, int kind, const double* params, int npar) {
+  RefSys* s = (RefSys*)h;
+  if (!s->save) return -3;
+  Fem::Ls::CLinearSystem_Save& ls = *(Fem::Ls::CLinearSystem_Save*)s->ls;
+  ls.InitializeMarge();
+  const int ngroup = (int)s->id_ea.size();
+  for (int g = 0; g < ngroup; g++) {
+    const double* p = &params[(size_t)g * npar];
+    const unsigned int id_ea = s->id_ea[g];
+    bool ok = true;
+    switch (kind) {
+      case 0: ok = Fem::Eqn::AddLinSys_Poisson(ls, p[0], p[1], s->world, s->id_field, id_ea); break;
+      case 1:
+        ok = Fem::Eqn::AddLinSys_LinearSolid2D_Static_SaveStiffMat(ls, p[0], p[1], p[2], p[3], p[4], s->world,
+                                                                   s->id_field, id_ea);
+        break;
+      case 2: {
+        const unsigned int idf = s->id_partial.empty() ? s->id_field : s->id_partial[g];
+        ok = Fem::Eqn::AddLinSys_LinearSolid3D_Static_SaveStiffMat(ls, p[0], p[1], p[2], p[3], p[4], p[5], s->world,
+                                                                   idf);
+        if (s->id_partial.empty()) g = ngroup;
+        break;
+      }
+      default: return -1;
+    }
+    if (!ok) return -2;
+  }
+  ls.FinalizeMarge();
+  return 0;
+}
+
+// CLinearSystem_Save::MakeResidual; returns its norm
+double ref_sys_make_residual(void* h) {
+  RefSys* s = (RefSys*)h;
+  return ((Fem::Ls::CLinearSystem_Save*)s->ls)->MakeResidual(s->world);
+}
+
+// CLinearSystem_Save::GetForce of the field's corner segment (n*len)
+void ref_sys_get_force(void* h, double* f) {
+  RefSys* s = (RefSys*)h;
+  const MatVec::CVector_Blk& v =
+      ((Fem::Ls::CLinearSystem_Save*)s->ls)->GetForce(s->id_field, Fem::Field::CORNER, s->world);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++) f[(size_t)i * s->len + d] = v.GetValue(i, d);
+}
+
+// CLinearSystem_Save::UpdateValueOfField (VALUE)
+void ref_sys_update_field_save(void* h) {
+  RefSys* s = (RefSys*)h;
+  ((Fem::Ls::CLinearSystem_Save*)s->ls)->UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);
 }
 
 }  // extern "C"

@@ -223,3 +223,77 @@ def test_traction_loads_closed_form():
     a = 0.5 / nx / nx
     top_corner = (nx * (nx + 1) + 0) * (nx + 1) + 0 + 0  # node (0,0,nx)
     assert np.allclose(cpu.res.reshape(-1, 3)[top_corner], np.array([0.1, -0.2, 0.3]) * 2 * a / 3)
+
+
+# ---- saved stiffness: what the mathematics fixes about MakeResidual and the saved solve --------------------
+def _save_setup(ndim, length, kind, par, seed=3):
+    co, ln = (M.rect_tri(7, 4, 1.0, 0.4, seed) if ndim == 2 else M.box_tet(3, 2, 2, 1.0, 0.7, 0.6, seed))
+    x = co.reshape(-1, ndim)[:, 0]
+    f = np.zeros((x.size, length), np.int32)
+    f[x == x.min()] = 1
+    f[x == x.max(), 0] = 1
+    flags = f.ravel()
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-0.05, 0.05, flags.size)
+    u[flags == 1] = rng.uniform(0.01, 0.1, int(flags.sum()))
+    full = P.CpuSystem(ndim, co, ln, length)  # the full matrix, no Dirichlet treatment
+    full.assemble(kind, par)
+    cpu = P.CpuSystem(ndim, co, ln, length)
+    cpu.assemble_save(kind, par, flags)
+    return co, ln, flags, u, full, cpu
+
+
+SAVE_MATH = [(2, 1, P.POISSON, [1.3, 0.8]), (2, 2, P.SOLID2D, [0.9, 0.4, 1.0, 0.1, -0.2]),
+             (3, 3, P.SOLID3D, [0.6, 0.4, 1.0, 0.0, 0.2, -0.1])]
+
+
+@pytest.mark.parametrize("ndim,length,kind,par", SAVE_MATH, ids=["poisson2d", "solid2d", "solid3d"])
+def test_saved_residual_is_coupling_to_prescribed_values(ndim, length, kind, par):
+    """MakeResidual: r_free = f_free - K[free, fixed] u_fixed and r_fixed = f_fixed, with K the full stiffness
+    (a transposed coupling, a dropped diagonal block or a free column left in would break it)."""
+    _, _, flags, u, full, cpu = _save_setup(ndim, length, kind, par)
+    K = dense(full)
+    fr, fx = flags == 0, flags == 1
+    _, res, upd = cpu.make_residual(u)
+    want = cpu.force.copy()
+    want[fr] -= K[np.ix_(fr, fx)] @ u[fx]
+    assert np.allclose(res, want, rtol=1e-13, atol=1e-15 * np.abs(cpu.force).max())
+    assert np.array_equal(res[fx], cpu.force[fx])
+    assert np.array_equal(upd, u)
+
+
+@pytest.mark.parametrize("ndim,length,kind,par", SAVE_MATH, ids=["poisson2d", "solid2d", "solid3d"])
+def test_saved_force_is_the_static_load(ndim, length, kind, par):
+    """The saved force is the external load: the static loop's residual at u = 0 (same expression there)."""
+    _, _, _, _, full, cpu = _save_setup(ndim, length, kind, par)
+    assert np.allclose(cpu.force, full.res, rtol=1e-15, atol=0.0)
+    # the saved matrix is the static one (2D elasticity writes its element matrix in another order: rounding)
+    K_save, K = dense(cpu), dense(full)
+    fr = np.flatnonzero(cpu.flags == 0)
+    assert np.allclose(K_save[np.ix_(fr, fr)], K[np.ix_(fr, fr)], rtol=1e-14, atol=1e-14 * np.abs(K).max())
+
+
+@pytest.mark.parametrize("ndim,length,kind,par", SAVE_MATH, ids=["poisson2d", "solid2d", "solid3d"])
+def test_saved_solve_equals_incremental_solve(ndim, length, kind, par):
+    """Solving K_ff u_f = f_f - K_fb u_b and setting u = update gives the field of the incremental formulation
+    (r = f - K u, K du = r, u += du) and the direct dense solve."""
+    co, ln, flags, u, full, cpu = _save_setup(ndim, length, kind, par)
+    cpu.make_residual(u)
+    f = cpu.ilu(None, 0)
+    ok, _, _, x, _ = cpu.solve(f, "pcg", 1e-13, 2000)
+    assert ok
+    _, u_save = cpu.update_field_save(x, u)
+    inc = P.CpuSystem(ndim, co, ln, length)
+    inc.assemble(kind, par, u)
+    inc.set_bc(flags)
+    ok, _, _, du, _ = inc.solve(inc.ilu0(), "pcg", 1e-13, 2000)
+    assert ok
+    u_inc = u + du
+    K = dense(full)
+    fr, fx = flags == 0, flags == 1
+    u_dir = u.copy()
+    u_dir[fr] = np.linalg.solve(K[np.ix_(fr, fr)], cpu.force[fr] - K[np.ix_(fr, fx)] @ u[fx])
+    scale = np.abs(u_dir).max()
+    assert np.allclose(u_save, u_dir, rtol=0, atol=1e-9 * scale)
+    assert np.allclose(u_inc, u_dir, rtol=0, atol=1e-9 * scale)
+    assert np.array_equal(u_save[fx], u[fx])

@@ -325,3 +325,91 @@ def test_matrix_level_quirks(method):
     assert np.array_equal(rx, x)
     ok, conv, it, x, _ = cpu.solve(f, "mat_" + method, 1e-10, 100, res=np.zeros(cpu.nnode * cpu.len))
     assert ok and conv == 0.0 and it == 0 and not x.any()
+
+
+# ---- saved stiffness (CLinearSystem_Save) ---------------------------------------------------------------
+SAVE_CASES = [
+    ("poisson2d", 2, 1, P.POISSON, lambda r: [r.uniform(0.5, 2.0), 1.0]),
+    ("solid2d", 2, 2, P.SOLID2D, lambda r: [LAM2 * r.uniform(0.5, 2), MU2, 1.0, 0.0, -0.01]),
+    ("solid3d", 3, 3, P.SOLID3D, lambda r: [LAM3, MU3 * r.uniform(0.5, 2), 1.0, 0.01, 0.0, -0.02]),
+]
+
+
+def save_flags(coords, ndim, length):
+    """x = min clamped; at x = max only the first dof of each node fixed (blocks with free and fixed dofs)."""
+    x = coords.reshape(-1, ndim)[:, 0]
+    f = np.zeros((x.size, length), np.int32)
+    f[x == x.min()] = 1
+    f[x == x.max(), 0] = 1
+    return f.ravel()
+
+
+def save_field(coords, ndim, length, flags, seed):
+    """Field values: prescribed values at fixed dofs, arbitrary values elsewhere (MakeResidual reads both)."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-0.05, 0.05, coords.size // ndim * length)
+    u[flags == 1] = rng.uniform(0.01, 0.1, int(flags.sum()))
+    return u
+
+
+def _saved_pair(case, per_element, seed=5):
+    name, ndim, length, kind, pfun = case
+    if per_element:
+        co, ln = mesh2d(4, 2) if ndim == 2 else mesh3d(2, 2, 1)
+    else:
+        co, ln = mesh2d(9, 5) if ndim == 2 else mesh3d(3, 3, 2)
+    nelem = ln.size // (ndim + 1)
+    rng = np.random.default_rng(seed)
+    if per_element:
+        params = np.array([pfun(rng) for _ in range(nelem)])
+        groups = np.arange(nelem, dtype=np.int32)
+    else:
+        params = np.array(pfun(rng))
+        groups = None
+    ref = P.RefSystem(ndim, co, ln, length, groups, save=True)
+    cpu = P.CpuSystem(ndim, co, ln, length)
+    flags = save_flags(co, ndim, length)
+    u = save_field(co, ndim, length, flags, seed + 1)
+    ref.set_field(u)
+    ref.set_bc(flags)
+    ref.assemble_save(kind, params if per_element else params[None, :])
+    cpu.assemble_save(kind, params, flags)
+    return ref, cpu, u, kind
+
+
+@pytest.mark.parametrize("case", SAVE_CASES, ids=[c[0] for c in SAVE_CASES])
+@pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
+def test_saved_stiffness_bit_exact(case, per_element):
+    """Matrix, force, residual of MakeResidual and the update it writes, element for element."""
+    ref, cpu, u, _ = _saved_pair(case, per_element)
+    crs, dia = ref.matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    assert np.array_equal(ref.force(), cpu.force)
+    assert not ref.vector(-1).any() and not cpu.res.any()
+    rn = ref.make_residual()
+    norm, res, upd = cpu.make_residual(u)
+    assert np.array_equal(ref.vector(-1), res), np.abs(ref.vector(-1) - res).max()
+    assert np.array_equal(ref.vector(-2), upd)
+    assert rn == norm
+    assert np.any(res != cpu.force)  # the boundary terms contribute
+
+
+@pytest.mark.parametrize("case", SAVE_CASES, ids=[c[0] for c in SAVE_CASES])
+def test_saved_stiffness_solve_and_update_bit_exact(case):
+    """MakeResidual -> Solve_PCG (ILU(1) for the solids, as CEqn_Solid3D_Linear / CEqnSystem_Solid2D use it;
+    ILU(0) for the scalar system) -> UpdateValueOfField of CLinearSystem_Save."""
+    ref, cpu, u, kind = _saved_pair(case, False)
+    lev = 0 if kind == P.POISSON else 1
+    ref.make_residual()
+    cpu.make_residual(u)
+    f = cpu.ilu(None, lev)
+    ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-10, 1000)
+    rok, rconv, rit = ref.solve("pcg", 1e-10, 1000, lev, None)
+    assert (ok, conv, it) == (bool(rok), rconv, rit)
+    assert np.array_equal(ref.vector(-2), x)
+    ref.update_field_save()
+    upd2, u2 = cpu.update_field_save(x, u)
+    assert np.array_equal(ref.vector(-2), upd2)
+    assert np.array_equal(ref.get_field(), u2)
+    flags = cpu.flags
+    assert np.array_equal(u2[flags == 1], u[flags == 1])  # Dirichlet values kept
