@@ -410,6 +410,94 @@ def run_saved(args, N, W, order_kind, w, ctx):
     print(json.dumps(out), flush=True)
 
 
+def run_newmark(args, N, W, order_kind, w, ctx):
+    """Newmark-beta time steps of linear elastodynamics (CEqn_Solid3D_Linear / CEqnSystem_Solid2D, not
+    stationary): per step the Newmark assembly (M + beta dt^2 K, predictor residual), Dirichlet treatment, ILU
+    factorisation, PCG to the workload's tolerance, UpdateValueOfField_NewmarkBeta. The workload's geometry,
+    materials (density 1) and Dirichlet flags, body force along -z (-y in 2D), starting at rest."""
+    import numpy as np
+
+    s = W.build_system(ctx, w, order_kind)
+    ndof = s.ndof
+    kind = N.SOLID3D_NEWMARK if w["ndim"] == 3 else N.SOLID2D_NEWMARK
+    par = np.array(w["params"], dtype=np.float64)
+    if par.ndim == 1:
+        par = par[None, :].repeat(w["lnods"].size // (w["ndim"] + 1), axis=0)
+    par = par.copy()
+    par[:, 2] = 1.0
+    par[:, 3:] = 0.0
+    par[:, -1] = -1e-2
+    dt, gamma, beta = 0.05, 0.5, 0.25
+    glob = (dt, gamma, beta, 1.0)
+    s.set_field(None, None)
+    s.set_acceleration(None)
+    s.set_params(par)
+
+    def step(precision):
+        ctx.synchronize()
+        a = time.perf_counter()
+        s.assemble_resident(kind, glob=glob, norm=False)
+        s.factor()
+        ok, conv, it = s.solve_pcg(w["tol"], args.max_iter, precision)
+        true_rel = s.true_residual()
+        s.update_field_newmark(gamma, beta, dt, True)
+        ctx.synchronize()
+        return time.perf_counter() - a, ok, it, true_rel
+
+    res = {}
+    for prec, name in [(N.MIXED_AUTO, "auto"), (N.FP64, "fp64")]:
+        s.set_field(None, None)
+        s.set_acceleration(None)
+        for _ in range(args.warmup):
+            step(prec)
+        lat, its, rels, ok_all = [], [], [], True
+        for _ in range(args.steps):
+            dt_, ok, it, rel = step(prec)
+            lat.append(dt_)
+            its.append(it)
+            rels.append(rel)
+            ok_all &= ok
+        res[name] = dict(lat=lat, its=its, ok=ok_all, rel=max(rels))
+        log(f"[bench] newmark {name}: {statistics.median(lat) * 1e3:.1f} ms/step, iters {its}, "
+            f"max true rel residual {max(rels):.2e}")
+    sw_ms, sw_b = s.bench_kernel(5, 20)
+    achieved = sw_b / (sw_ms * 1e-3) / 1e9
+    head = res["auto"]
+    t = sum(head["lat"])
+    out = {
+        "metric": "Newmark-beta time steps: DOF-steps/s (assembly + factor + PCG to the tolerance + field update)",
+        "value": ndof * args.steps / t,
+        "unit": "DOF-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w['name']} geometry, elastodynamics, density 1, dt {dt}, beta {beta}, "
+                               f"gamma {gamma}", "dofs": ndof, "ordering": args.ordering,
+                   "tol_true_rel_residual": w["tol"], "precision": "DFM_MIXED_AUTO"},
+        "fp64_value": ndof * args.steps / sum(res["fp64"]["lat"]),
+        "p50_ms": statistics.median(head["lat"]) * 1e3,
+        "fp64_p50_ms": statistics.median(res["fp64"]["lat"]) * 1e3,
+        "iters": head["its"],
+        "iters_fp64": res["fp64"]["its"],
+        "max_true_rel_residual": head["rel"],
+        "converged": bool(head["ok"] and res["fp64"]["ok"]),
+        "roofline": {
+            "kernel": "k_fwd_update/k_fwd_bwd_tile/k_bwd_rz<B=%d, FP64> (ILU sweep phases of the PCG iteration)" % w["len"],
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": sw_b,
+        },
+    }
+    s.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,6 +506,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--ordering", default="blockcolor", choices=["natural", "multicolor", "blockcolor"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--newmark", action="store_true",
+                    help="c1/c3: Newmark-beta time steps of linear elastodynamics on the workload's geometry")
     ap.add_argument("--saved", action="store_true",
                     help="c1/c2/c3: saved-stiffness re-solves with new Dirichlet values per step")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -450,6 +540,11 @@ def main():
                   "blockcolor": N.ORDER_BLOCKCOLOR}[args.ordering]
     if args.saved:
         return run_saved(args, N, W, order_kind, w, ctx)
+    if args.newmark:
+        if w["len"] == 1:
+            print("--newmark needs a vector workload (c1 or c3)", file=sys.stderr)
+            sys.exit(2)
+        return run_newmark(args, N, W, order_kind, w, ctx)
     s = W.build_system(ctx, w, order_kind)
     s.set_params(w["params"])
     s.set_loads(w["faces"], w["trac"])

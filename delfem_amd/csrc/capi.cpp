@@ -140,7 +140,7 @@ Sys::~Sys() {
   d_rowptr.release(); d_colind.release(); d_diaind.release(); d_lsplit.release(); d_usplit.release();
   d_frowptr.release(); d_fcolind.release(); d_fdiaind.release(); d_a2f.release();
   d_order.release(); d_inv.release(); d_elsup_ptr.release(); d_elsup.release(); d_lnods.release(); d_bc.release();
-  d_coords.release(); d_u.release(); d_v.release();
+  d_coords.release(); d_u.release(); d_v.release(); d_a.release();
   d_crs.release(); d_dia.release(); d_crs32.release(); d_dia32.release();
   d_fcrs.release(); d_fdia.release();
   d_res.release(); d_upd.release(); d_rhs.release();
@@ -589,6 +589,51 @@ int dfm_sys_update_field(dfm_sys sys) {
   return DFM_OK;
 }
 
+// the ACCELERATION field, zero until set
+static int ensure_acc(Sys& s) {
+  if (s.d_a.p) return DFM_OK;
+  DFM_HIP(s.d_a.alloc((size_t)mdof(s)));
+  DFM_HIP(launch_fill(s.d_a.p, 0.0, mdof(s), s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_set_acceleration(dfm_sys sys, const double* a) {
+  NEED(sys, "dfm_sys_set_acceleration");
+  Sys& s = sys->s;
+  int rc = ensure_acc(s);
+  if (rc != DFM_OK) return rc;
+  // on the library's stream, after the zero fill of a first use (the stream does not wait for the null stream)
+  if (a) DFM_HIP(hipMemcpyAsync(s.d_a.p, a, sizeof(double) * mdof(s), hipMemcpyHostToDevice, s.ctx->stream));
+  else DFM_HIP(launch_fill(s.d_a.p, 0.0, mdof(s), s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_get_fields(dfm_sys sys, double* u, double* v, double* a) {
+  NEED(sys, "dfm_sys_get_fields");
+  Sys& s = sys->s;
+  const size_t bytes = sizeof(double) * mdof(s);
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  if (u) DFM_HIP(hipMemcpy(u, s.d_u.p, bytes, hipMemcpyDeviceToHost));
+  if (v) DFM_HIP(hipMemcpy(v, s.d_v.p, bytes, hipMemcpyDeviceToHost));
+  if (a) {
+    if (s.d_a.p) DFM_HIP(hipMemcpy(a, s.d_a.p, bytes, hipMemcpyDeviceToHost));
+    else std::memset(a, 0, bytes);
+  }
+  return DFM_OK;
+}
+
+int dfm_sys_update_field_newmark(dfm_sys sys, double gamma, double beta, double dt, int is_initial) {
+  NEED(sys, "dfm_sys_update_field_newmark");
+  Sys& s = sys->s;
+  int rc = ensure_acc(s);
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(launch_update_field_newmark(s.n, s.len, s.d_order.p, s.d_upd.p, gamma, beta, dt, is_initial ? 1 : 0,
+                                      s.d_u.p, s.d_v.p, s.d_a.p, s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
 int dfm_sys_set_params(dfm_sys sys, const double* params, int npar, int pstride) {
   NEED(sys, "dfm_sys_set_params");
   NEED(params, "dfm_sys_set_params");
@@ -650,16 +695,15 @@ int dfm_sys_set_load_scale(dfm_sys sys, double scale) {
 }
 
 static int check_kind(const Sys& s, int kind) {
-  static const int need_npar[6] = {2, 5, 6, 5, 6, 3};
-  static const int need_len[6] = {1, 2, 3, 2, 3, 1};
-  DFM_CHECK(kind >= 0 && kind < 6, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
+  static const int need_npar[8] = {2, 5, 6, 5, 6, 3, 5, 6};
+  static const int need_len[8] = {1, 2, 3, 2, 3, 1, 2, 3};
+  DFM_CHECK(kind >= 0 && kind < 8, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
   DFM_CHECK(s.d_par.p != nullptr, DFM_ERR_STATE, "dfm_sys_assemble: no element parameters set");
   DFM_CHECK(s.npar >= need_npar[kind], DFM_ERR_ARG, "dfm_sys_assemble: too few parameters for the element kind");
   DFM_CHECK(need_len[kind] == s.len, DFM_ERR_ARG, "dfm_sys_assemble: element kind does not match dofs per node");
-  DFM_CHECK((kind == DFM_SOLID3D || kind == DFM_STVK3D || kind == DFM_POISSON) || s.ndim == 2, DFM_ERR_ARG,
-            "dfm_sys_assemble: element kind needs a 2D mesh");
-  DFM_CHECK((kind != DFM_SOLID3D && kind != DFM_STVK3D) || s.ndim == 3, DFM_ERR_ARG,
-            "dfm_sys_assemble: element kind needs a 3D mesh");
+  const bool k3 = kind == DFM_SOLID3D || kind == DFM_STVK3D || kind == DFM_SOLID3D_NEWMARK;
+  DFM_CHECK(k3 || kind == DFM_POISSON || s.ndim == 2, DFM_ERR_ARG, "dfm_sys_assemble: element kind needs a 2D mesh");
+  DFM_CHECK(!k3 || s.ndim == 3, DFM_ERR_ARG, "dfm_sys_assemble: element kind needs a 3D mesh");
   return DFM_OK;
 }
 
@@ -679,6 +723,12 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   Sys& s = sys->s;
   int rc = check_kind(s, kind);
   if (rc != DFM_OK) return rc;
+  const bool newmark = kind == DFM_SOLID2D_NEWMARK || kind == DFM_SOLID3D_NEWMARK;
+  if (newmark) {
+    DFM_CHECK(glob != nullptr, DFM_ERR_ARG, "dfm_sys_assemble: Newmark kinds need glob = (dt, gamma, beta, is_initial)");
+    rc = ensure_acc(s);
+    if (rc != DFM_OK) return rc;
+  }
   hipStream_t st = s.ctx->stream;
   const long long m = mdof(s);
   // InitializeMarge (src/ls/linearsystem.cpp:74-88): matrix, residual and update set to zero
@@ -692,7 +742,9 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   a.order = s.d_order.p; a.inv = s.d_inv.p;
   a.params = s.d_par.p; a.npar = s.npar; a.pstride = s.pstride;
   a.glob0 = glob ? glob[0] : 0.0; a.glob1 = glob ? glob[1] : 0.0;
-  a.u = s.d_u.p; a.v = s.d_v.p;
+  a.glob2 = newmark ? glob[2] : 0.0;
+  a.glob3 = kind == DFM_SOLID2D_NEWMARK ? glob[3] : 0.0;
+  a.u = s.d_u.p; a.v = s.d_v.p; a.acc = s.d_a.p;
   a.load_ptr = s.has_loads ? s.d_lptr.p : nullptr;
   a.loads = s.has_loads ? s.d_loads.p : nullptr;
   a.load_scale = s.load_scale;

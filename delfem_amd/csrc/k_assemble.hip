@@ -384,6 +384,68 @@ __device__ inline void row_solid_save(int a, const double co[4][3], const double
   for (int r = 0; r < NDIM; r++) er[r] = (NDIM == 2) ? w * rho * p[3 + r] * 0.3333333333333333333 : w * rho * p[3 + r] * 0.25;
 }
 
+// Newmark-beta rows (AddLinSys_LinearSolid2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_linear_solid2d.cpp:1182-1337;
+// AddLinSys_LinearSolid3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_linear_solid3d.cpp:525-690): stiffness K as
+// the saved loops write it, consistent mass M (rho w / 12 in 2D, rho w / 20 in 3D, doubled on the diagonal
+// blocks), element matrix M + beta dt^2 K, residual f - K u - M a [- dt K v - dt^2/2 K a: 2D when is_initial,
+// 3D always]. p = (lambda, myu, rho, g...); dt, beta, initial from the launch.
+template <int NDIM>
+__device__ inline void row_solid_newmark(int a, const double co[4][3], const double u[4][3], const double vv[4][3],
+                                         const double aa[4][3], const double* p, double dt, double beta, bool initial,
+                                         double* em, double* er) {
+  constexpr int NNO = NDIM + 1;
+  constexpr int BS = NDIM * NDIM;
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  double d[4][3];
+  double w;
+  if (NDIM == 2) {
+    w = tri_area(co[0], co[1], co[2]);
+    double d2[3][2];
+    tri_dldx(d2, co);
+    for (int k = 0; k < 3; k++) {
+      d[k][0] = d2[k][0];
+      d[k][1] = d2[k][1];
+    }
+  } else {
+    w = tet_volume(co);
+    tet_dldx(d, co);
+  }
+  double K[NNO][BS], M[NNO][BS];
+  for (int j = 0; j < NNO; j++) {
+    double t = 0.0;
+    for (int r = 0; r < NDIM; r++) {
+      // 0.0 + x as the zero-initialised element matrix accumulates it (a -0.0 term becomes +0.0)
+      for (int c = 0; c < NDIM; c++) K[j][r * NDIM + c] = 0.0 + w * (lambda * d[a][r] * d[j][c] + myu * d[j][r] * d[a][c]);
+      t += d[a][r] * d[j][r];
+    }
+    for (int r = 0; r < NDIM; r++) K[j][r * NDIM + r] += w * myu * t;
+  }
+  const double mm = (NDIM == 2) ? w * rho * 0.0833333333333333333333333 : w * rho * 0.05;
+  for (int j = 0; j < NNO; j++)
+    for (int k = 0; k < BS; k++) M[j][k] = (k % (NDIM + 1) == 0) ? mm : 0.0;
+  for (int r = 0; r < NDIM; r++) M[a][r * NDIM + r] += mm;
+  const double bdt2 = beta * dt * dt;
+  for (int j = 0; j < NNO; j++)
+    for (int k = 0; k < BS; k++) em[j * BS + k] = M[j][k] + bdt2 * K[j][k];
+  for (int r = 0; r < NDIM; r++)
+    er[r] = (NDIM == 2) ? w * rho * p[3 + r] * 0.33333333333333333333333333 : w * rho * p[3 + r] * 0.25;
+  auto row_dot = [&](const double* blk, int r, const double x[3]) {
+    double t = blk[r * NDIM] * x[0];
+    for (int c = 1; c < NDIM; c++) t = t + blk[r * NDIM + c] * x[c];
+    return t;
+  };
+  for (int j = 0; j < NNO; j++)
+    for (int r = 0; r < NDIM; r++) er[r] -= row_dot(K[j], r, u[j]);
+  for (int j = 0; j < NNO; j++)
+    for (int r = 0; r < NDIM; r++) er[r] -= row_dot(M[j], r, aa[j]);
+  if (NDIM == 3 || initial) {
+    for (int j = 0; j < NNO; j++)
+      for (int r = 0; r < NDIM; r++) er[r] -= dt * row_dot(K[j], r, vv[j]);
+    for (int j = 0; j < NNO; j++)
+      for (int r = 0; r < NDIM; r++) er[r] -= 0.5 * dt * dt * row_dot(K[j], r, aa[j]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------------
 
 template <int KIND, int NDIM, int B, bool SAVE>
@@ -405,12 +467,19 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
     int a = 0;
     for (int c = 0; c < NNO; c++)
       if (no[c] == i1) a = c;
+    constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK);
     double co[4][3], u[4][3], val[4], vval[4];
+    double vv[4][3], aa[4][3];
     for (int c = 0; c < NNO; c++) {
       for (int d = 0; d < 3; d++) co[c][d] = (d < NDIM) ? A.coords[(long long)no[c] * NDIM + d] : 0.0;
       for (int d = 0; d < 3; d++) u[c][d] = (A.u && d < A.len) ? A.u[(long long)no[c] * A.len + d] : 0.0;
       val[c] = A.u ? A.u[(long long)no[c] * A.len] : 0.0;
       vval[c] = A.v ? A.v[(long long)no[c] * A.len] : 0.0;
+      if (NM)
+        for (int d = 0; d < 3; d++) {
+          vv[c][d] = (d < B) ? A.v[(long long)no[c] * B + d] : 0.0;
+          aa[c][d] = (d < B) ? A.acc[(long long)no[c] * B + d] : 0.0;
+        }
     }
     const double* p = &A.params[(long long)e * A.pstride];
     double em[NNO * BS], er[B];
@@ -424,6 +493,7 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
       if (KIND == DFM_SOLID3D) row_solid3d(a, co, u, p, em, er);
       if (KIND == DFM_STVK2D) row_stvk2d(a, co, u, p, em, er);
       if (KIND == DFM_STVK3D) row_stvk3d(a, co, u, p, em, er);
+      if (NM) row_solid_newmark<NDIM>(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
     }
     for (int c = 0; c < NNO; c++) {
       const double* blk = &em[c * BS];
@@ -472,6 +542,8 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
   DFM_ASM(DFM_SOLID3D, 3, 3)
   DFM_ASM(DFM_STVK2D, 2, 2)
   DFM_ASM(DFM_STVK3D, 3, 3)
+  DFM_ASM(DFM_SOLID2D_NEWMARK, 2, 2)
+  DFM_ASM(DFM_SOLID3D_NEWMARK, 3, 3)
 #undef DFM_ASM
   return hipErrorInvalidValue;
 }
@@ -533,6 +605,41 @@ hipError_t launch_update_field(int n, int len, const int* order, const double* u
   if (m == 0) return hipSuccess;
   const int nt = 256;
   hipLaunchKernelGGL(k_update_field, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, upd, u);
+  return hipGetLastError();
+}
+
+// CLinearSystem_Field::UpdateValueOfField_NewmarkBeta (src/femls/linearsystem_field.cpp:966-1059): the reference
+// makes whole-array passes (CNodeAry::AddValueToNodeSegment, src/femfield/node_ary.cpp:197-251, x += alpha * y),
+// each dof reading only its own values, so one thread per dof runs the passes in their order.
+__global__ void k_update_field_newmark(long long m, int len, const int* order, const double* upd, double gamma,
+                                       double beta, double dt, int initial, double* u, double* v, double* a) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long long i0 = k / len;
+  const int d = (int)(k - i0 * len);
+  const long long i1 = (order ? order[i0] : i0) * len + d;
+  const double du = upd[k];
+  double uu = u[i1], vv = v[i1], aa = a[i1];
+  if (initial) {
+    uu += dt * vv;
+    uu += 0.5 * dt * dt * aa;
+  }
+  uu += beta * dt * dt * du;
+  if (initial) vv += dt * aa;
+  vv += gamma * dt * du;
+  aa += 1.0 * du;
+  u[i1] = uu;
+  v[i1] = vv;
+  a[i1] = aa;
+}
+
+hipError_t launch_update_field_newmark(int n, int len, const int* order, const double* upd, double gamma, double beta,
+                                       double dt, int initial, double* u, double* v, double* a, hipStream_t st) {
+  const long long m = (long long)n * len;
+  if (m == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_update_field_newmark, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, upd,
+                     gamma, beta, dt, initial, u, v, a);
   return hipGetLastError();
 }
 

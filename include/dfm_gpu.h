@@ -39,7 +39,14 @@ enum {
   DFM_SOLID3D = 2,     /* (lambda, myu, rho, gx, gy, gz)   src/femeqn/eqn_linear_solid3d.cpp:241-334   */
   DFM_STVK2D = 3,      /* (lambda, myu, rho, gx, gy)       src/femeqn/eqn_st_venant.cpp:47-272         */
   DFM_STVK3D = 4,      /* (lambda, myu, rho, gx, gy, gz)   src/femeqn/eqn_st_venant.cpp:724-939        */
-  DFM_DIFFUSION2D = 5  /* (rho, alpha, source), glob=(dt, gamma)  src/femeqn/eqn_diffusion.cpp:240-345 */
+  DFM_DIFFUSION2D = 5, /* (rho, alpha, source), glob=(dt, gamma)  src/femeqn/eqn_diffusion.cpp:240-345 */
+  /* Newmark-beta dynamic linear elasticity (unknown: the acceleration increment; matrix M + beta dt^2 K):
+   * AddLinSys_LinearSolid2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_linear_solid2d.cpp:1182-1337,
+   * (lambda, myu, rho, gx, gy), glob = (dt, gamma, beta, is_initial) */
+  DFM_SOLID2D_NEWMARK = 6,
+  /* AddLinSys_LinearSolid3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_linear_solid3d.cpp:525-690 (its predictor
+   * terms always on), (lambda, myu, rho, gx, gy, gz), glob = (dt, gamma, beta) */
+  DFM_SOLID3D_NEWMARK = 7
 };
 
 /* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering); the last three are COrdering_Blk's, exactly
@@ -132,6 +139,13 @@ int dfm_sys_set_field(dfm_sys sys, const double* u, const double* v);
 int dfm_sys_get_field(dfm_sys sys, double* u);
 /* u_field += update (CLinearSystem_Field::UpdateValueOfField, src/femls/linearsystem_field.cpp:716-794) */
 int dfm_sys_update_field(dfm_sys sys);
+/* ACCELERATION field values (n*len; NULL = zero), and all three fields out (any pointer may be NULL). */
+int dfm_sys_set_acceleration(dfm_sys sys, const double* a);
+int dfm_sys_get_fields(dfm_sys sys, double* u, double* v, double* a);
+/* CLinearSystem_Field::UpdateValueOfField_NewmarkBeta (src/femls/linearsystem_field.cpp:966-1059), the update
+ * being the acceleration increment: [is_initial: u += dt v, u += dt^2/2 a], u += beta dt^2 du,
+ * [is_initial: v += dt a], v += gamma dt du, a += du; in that order, as the reference's node-array passes. */
+int dfm_sys_update_field_newmark(dfm_sys sys, double gamma, double beta, double dt, int is_initial);
 
 /* InitializeMarge + Fem::Eqn::AddLinSys_<kind> + [surface loads] + FinalizeMarge in one call
  * (src/ls/linearsystem.cpp:74-128). params: npar doubles per element with stride pstride (0 = shared).

@@ -1618,3 +1618,134 @@ void cr_update_field_save(int n, int len, const int* flags, double* upd, double*
     if (flags[k] != 0) upd[k] = u[k];
   for (long k = 0; k < m; k++) u[k] = upd[k];
 }
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Newmark-beta dynamic linear elasticity: AddLinSys_LinearSolid2D_NonStatic_NewmarkBeta_P1               */
+/* (src/femeqn/eqn_linear_solid2d.cpp:1182-1337) and AddLinSys_LinearSolid3D_NonStatic_NewmarkBeta_P1      */
+/* (src/femeqn/eqn_linear_solid3d.cpp:525-690), restated loop by loop. glob = (dt, gamma, beta, initial).  */
+
+static void elem_solid_newmark(int ndim, const double co[][3], const double disp[][3], const double velo[][3],
+                               const double acc[][3], const double* p, const double* glob, double* emat,
+                               double* eres) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double dt = glob[0], beta = glob[2];
+  const int is_initial = glob[3] != 0.0;
+  const int nno = ndim + 1, bs = ndim * ndim;
+  double eK[4 * 4 * 9], eM[4 * 4 * 9], eqf[12];
+  double dldx[4][3];
+  double w;
+  if (ndim == 2) {
+    double d2[3][2];
+    w = tri_area(co[0], co[1], co[2]);
+    tri_dldx(d2, co[0], co[1], co[2]);
+    for (int k = 0; k < 3; k++) { dldx[k][0] = d2[k][0]; dldx[k][1] = d2[k][1]; }
+  } else {
+    w = tet_volume(co[0], co[1], co[2], co[3]);
+    tet_dldx(dldx, co[0], co[1], co[2], co[3]);
+  }
+  for (int i = 0; i < nno * nno * bs; i++) { emat[i] = 0.0; eK[i] = 0.0; eM[i] = 0.0; }
+  for (int i = 0; i < nno * ndim; i++) eqf[i] = 0.0;
+  for (int ino = 0; ino < nno; ino++)
+    for (int jno = 0; jno < nno; jno++) {
+      double* k = &eK[(ino * nno + jno) * bs];
+      double dtmp1 = 0.0;
+      for (int idim = 0; idim < ndim; idim++) {
+        for (int jdim = 0; jdim < ndim; jdim++)
+          k[idim * ndim + jdim] += w * (lambda * dldx[ino][idim] * dldx[jno][jdim] + myu * dldx[jno][idim] * dldx[ino][jdim]);
+        dtmp1 += dldx[ino][idim] * dldx[jno][idim];
+      }
+      for (int idim = 0; idim < ndim; idim++) k[idim * ndim + idim] += w * myu * dtmp1;
+    }
+  {
+    const double dtmp1 = (ndim == 2) ? w * rho * 0.0833333333333333333333333 : w * rho * 0.05;
+    for (int ino = 0; ino < nno; ino++) {
+      for (int jno = 0; jno < nno; jno++)
+        for (int d = 0; d < ndim; d++) eM[(ino * nno + jno) * bs + d * ndim + d] += dtmp1;
+      for (int d = 0; d < ndim; d++) eM[(ino * nno + ino) * bs + d * ndim + d] += dtmp1;
+    }
+  }
+  for (int ino = 0; ino < nno; ino++)
+    for (int d = 0; d < ndim; d++)
+      eqf[ino * ndim + d] = (ndim == 2) ? w * rho * p[3 + d] * 0.33333333333333333333333333 : w * rho * p[3 + d] * 0.25;
+  {
+    const double dtmp1 = beta * dt * dt;
+    for (int i = 0; i < nno * nno * bs; i++) emat[i] = eM[i] + dtmp1 * eK[i];
+  }
+  for (int ino = 0; ino < nno; ino++) {
+    double* r = &eres[ino * ndim];
+    for (int d = 0; d < ndim; d++) r[d] = eqf[ino * ndim + d];
+    for (int jno = 0; jno < nno; jno++) {
+      const double* k = &eK[(ino * nno + jno) * bs];
+      for (int d = 0; d < ndim; d++) {
+        double t = k[d * ndim] * disp[jno][0];
+        for (int e = 1; e < ndim; e++) t = t + k[d * ndim + e] * disp[jno][e];
+        r[d] -= t;
+      }
+    }
+    for (int jno = 0; jno < nno; jno++) {
+      const double* m = &eM[(ino * nno + jno) * bs];
+      for (int d = 0; d < ndim; d++) {
+        double t = m[d * ndim] * acc[jno][0];
+        for (int e = 1; e < ndim; e++) t = t + m[d * ndim + e] * acc[jno][e];
+        r[d] -= t;
+      }
+    }
+    if (ndim == 3 || is_initial) {  /* the 3D loop has no is_initial switch */
+      for (int jno = 0; jno < nno; jno++) {
+        const double* k = &eK[(ino * nno + jno) * bs];
+        for (int d = 0; d < ndim; d++) {
+          double t = k[d * ndim] * velo[jno][0];
+          for (int e = 1; e < ndim; e++) t = t + k[d * ndim + e] * velo[jno][e];
+          r[d] -= dt * t;
+        }
+      }
+      for (int jno = 0; jno < nno; jno++) {
+        const double* k = &eK[(ino * nno + jno) * bs];
+        for (int d = 0; d < ndim; d++) {
+          double t = k[d * ndim] * acc[jno][0];
+          for (int e = 1; e < ndim; e++) t = t + k[d * ndim + e] * acc[jno][e];
+          r[d] -= 0.5 * dt * dt * t;
+        }
+      }
+    }
+  }
+}
+
+int cr_assemble_newmark(int ndim, int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                        int pstride, const double* glob, const double* u, const double* v, const double* a,
+                        const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res) {
+  (void)nnode;
+  const int nno = ndim + 1, len = ndim, bs = ndim * ndim;
+  double emat[144], eres[12], co[4][3], dsp[4][3], vel[4][3], acc[4][3];
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * nno];
+    for (int k = 0; k < nno; k++)
+      for (int d = 0; d < 3; d++) {
+        co[k][d] = (d < ndim) ? coords[(long)no[k] * ndim + d] : 0.0;
+        dsp[k][d] = (d < len) ? u[(long)no[k] * len + d] : 0.0;
+        vel[k][d] = (d < len) ? v[(long)no[k] * len + d] : 0.0;
+        acc[k][d] = (d < len) ? a[(long)no[k] * len + d] : 0.0;
+      }
+    elem_solid_newmark(ndim, co, dsp, vel, acc, &params[e * pstride], glob, emat, eres);
+    merge(nno, no, bs, emat, rowptr, colind, crs, dia);
+    for (int k = 0; k < nno; k++)
+      for (int d = 0; d < len; d++) res[(long)no[k] * len + d] += eres[k * len + d];
+  }
+  return 0;
+}
+
+void cr_update_field_newmark(long m, double gamma, double beta, double dt, int is_initial, const double* upd, double* u,
+                             double* v, double* a) {
+  /* UpdateValueOfField_NewmarkBeta (src/femls/linearsystem_field.cpp:966-1059): whole-array passes of
+   * CNodeAry::AddValueToNodeSegment (src/femfield/node_ary.cpp:197-251) in this order */
+  long k;
+  if (is_initial) {
+    const double c1 = dt, c2 = 0.5 * dt * dt;
+    for (k = 0; k < m; k++) u[k] += c1 * v[k];
+    for (k = 0; k < m; k++) u[k] += c2 * a[k];
+  }
+  { const double c = beta * dt * dt; for (k = 0; k < m; k++) u[k] += c * upd[k]; }
+  if (is_initial) { for (k = 0; k < m; k++) v[k] += dt * a[k]; }
+  { const double c = gamma * dt; for (k = 0; k < m; k++) v[k] += c * upd[k]; }
+  for (k = 0; k < m; k++) a[k] += 1.0 * upd[k];
+}

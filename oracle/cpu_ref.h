@@ -111,6 +111,15 @@ double cr_make_residual_save(int n, int len, const unsigned* browptr, const unsi
                              const double* force, const double* u, double* res, double* upd);
 void cr_update_field_save(int n, int len, const int* flags, double* upd, double* u);
 
+/* Newmark-beta dynamic linear elasticity (2D and 3D P1): element loops into matrix and residual (u, v, a the
+ * VALUE, VELOCITY and ACCELERATION fields; glob = (dt, gamma, beta, is_initial)), and UpdateValueOfField_NewmarkBeta
+ * over m values. */
+int cr_assemble_newmark(int ndim, int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                        int pstride, const double* glob, const double* u, const double* v, const double* a,
+                        const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res);
+void cr_update_field_newmark(long m, double gamma, double beta, double dt, int is_initial, const double* upd, double* u,
+                             double* v, double* a);
+
 #ifdef __cplusplus
 }
 #endif

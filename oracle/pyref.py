@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CPU_REF_SO = os.path.join(HERE, "libcpu_ref.so")
 REF_SO = os.path.join(HERE, "_ref", "libdfm_ref.so")
 
-POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D = range(6)
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK = range(8)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -148,6 +148,27 @@ class CpuSystem:
         cpu_ref().cr_update_field_save(self.nnode, self.len, p(self.flags), p(upd), p(u))
         return upd, u
 
+    def assemble_newmark(self, params, u, v, a, glob):
+        """Newmark-beta element loop (restated), 2D or 3D by the mesh: matrix and residual accumulate into
+        self.crs / self.dia / self.res (zero them first). glob = (dt, gamma, beta, is_initial)."""
+        params = _f64(params)
+        pstride = 0 if params.ndim == 1 else params.shape[1]
+        u, v, a, glob = _f64(u), _f64(v), _f64(a), _f64(glob)
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        rc = cpu_ref().cr_assemble_newmark(self.ndim, self.nnode, p(self.coords), self.nelem, p(self.lnods),
+                                           p(params), pstride, p(glob), p(u), p(v), p(a), p(self.rowptr),
+                                           p(self.colind), p(self.crs), p(self.dia), p(self.res))
+        assert rc == 0
+
+    @staticmethod
+    def update_field_newmark(gamma, beta, dt, is_initial, upd, u, v, a):
+        """UpdateValueOfField_NewmarkBeta (restated): returns new (u, v, a)."""
+        u, v, a, upd = _f64(u).copy(), _f64(v).copy(), _f64(a).copy(), _f64(upd)
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        cpu_ref().cr_update_field_newmark(C.c_long(u.size), C.c_double(gamma), C.c_double(beta), C.c_double(dt),
+                                          1 if is_initial else 0, p(upd), p(u), p(v), p(a))
+        return u, v, a
+
     def traction(self, faces, trac):
         faces, trac = _i32(faces), _f64(trac)
         cpu_ref().cr_traction_loads(faces.size // 3, faces.ctypes.data_as(C.c_void_p),
@@ -270,7 +291,7 @@ def reference():
 class RefSystem:
     """The reference DelFEM driven through its own API (oracle/ref_capi.cpp)."""
 
-    def __init__(self, ndim, coords, lnods, len_, groups=None, save=False):
+    def __init__(self, ndim, coords, lnods, len_, groups=None, save=False, dynamic=False):
         self.ndim, self.len = ndim, len_
         self.coords = _f64(coords)
         self.lnods = _i32(lnods)
@@ -283,7 +304,7 @@ class RefSystem:
         lib = reference()
         p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
         self.h = C.c_void_p(lib.ref_sys_create2(ndim, self.nnode, p(self.coords), self.nelem, p(self.lnods), ngroup,
-                                                p(self.groups), len_, 1 if save else 0))
+                                                p(self.groups), len_, (1 if save else 0) | (2 if dynamic else 0)))
         self.ncrs = lib.ref_sys_ncrs(self.h)
 
     def __del__(self):
@@ -407,6 +428,19 @@ class RefSystem:
 
     def update_field(self):
         reference().ref_sys_update_field(self.h)
+
+    def set_acc(self, a=None):
+        a = None if a is None else _f64(a)
+        reference().ref_sys_set_acc(self.h, None if a is None else a.ctypes.data_as(C.c_void_p))
+
+    def get_fields(self):
+        out = [np.zeros(self.nnode * self.len) for _ in range(3)]
+        reference().ref_sys_get_fields(self.h, *[o.ctypes.data_as(C.c_void_p) for o in out])
+        return out
+
+    def update_field_newmark(self, gamma, beta, dt, is_initial):
+        reference().ref_sys_update_field_newmark(self.h, C.c_double(gamma), C.c_double(beta), C.c_double(dt),
+                                                 1 if is_initial else 0)
 
     def assemble_save(self, kind, group_params):
         """CLinearSystem_Save: InitializeMarge, the *_SaveStiffMat loop, FinalizeMarge (system made with save)."""

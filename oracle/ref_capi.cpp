@@ -82,11 +82,13 @@ MatVec::CMatDia_BlkCrs& Mat(RefSys* s) { return s->ls->m_ls.GetMatrix(0); }
 extern "C" {
 
 // kind of field: len = 1 (scalar) or ndim (vector). Element groups are 0..ngroup-1 per element.
-// save != 0: the linear system is a CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp)
+// mode bit 0: the linear system is a CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp); bit 1: the field
+// also has ACCELERATION values (Newmark). Without bit 1 the field is the one the other drivers use (VALUE |
+// VELOCITY): the reference's matrix-level loops read uninitialised heap memory, whose contents depend on it.
 void* ref_sys_create2(int ndim, int nnode, const double* coords, int nelem, const int* lnods,
-                      int ngroup, const int* elem_group, int len, int save) {
+                      int ngroup, const int* elem_group, int len, int mode) {
   RefSys* s = new RefSys;
-  s->save = save != 0;
+  s->save = (mode & 1) != 0;
   if (s->save) s->ls = new Fem::Ls::CLinearSystem_Save;
   else s->ls = new Fem::Ls::CLinearSystem_Field;
   s->ndim = ndim; s->nnode = nnode; s->len = len;
@@ -104,7 +106,8 @@ void* ref_sys_create2(int ndim, int nnode, const double* coords, int nelem, cons
   if (len == 2) ft = Fem::Field::VECTOR2;
   if (len == 3) ft = Fem::Field::VECTOR3;
   s->id_field = s->world.MakeField_FieldElemDim(s->id_base, ndim, ft,
-                                                Fem::Field::VALUE | Fem::Field::VELOCITY,
+                                                Fem::Field::VALUE | Fem::Field::VELOCITY |
+                                                    ((mode & 2) ? Fem::Field::ACCELERATION : 0),
                                                 Fem::Field::CORNER);
   const Fem::Field::CField& field = s->world.GetField(s->id_field);
   s->id_ea = field.GetAryIdEA();
@@ -155,6 +158,35 @@ void ref_sys_set_field(void* h, const double* u, const double* v) {
       ns_v.SetValue(i, d, v ? v[(size_t)i * s->len + d] : 0.0);
     }
   }
+}
+
+// ACCELERATION field values (n*len)
+void ref_sys_set_acc(void* h, const double* a) {
+  RefSys* s = (RefSys*)h;
+  Fem::Field::CField& field = s->world.GetField(s->id_field);
+  Fem::Field::CNodeAry::CNodeSeg& ns_a =
+      field.GetNodeSeg(Fem::Field::CORNER, true, s->world, Fem::Field::ACCELERATION);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++) ns_a.SetValue(i, d, a ? a[(size_t)i * s->len + d] : 0.0);
+}
+
+void ref_sys_get_fields(void* h, double* u, double* v, double* a) {
+  RefSys* s = (RefSys*)h;
+  Fem::Field::CField& field = s->world.GetField(s->id_field);
+  const Fem::Field::FIELD_DERIVATION_TYPE fdt[3] = {Fem::Field::VALUE, Fem::Field::VELOCITY,
+                                                     Fem::Field::ACCELERATION};
+  double* out[3] = {u, v, a};
+  for (int k = 0; k < 3; k++) {
+    if (!out[k]) continue;
+    const Fem::Field::CNodeAry::CNodeSeg& ns = field.GetNodeSeg(Fem::Field::CORNER, true, s->world, fdt[k]);
+    for (int i = 0; i < s->nnode; i++) ns.GetValue(i, &out[k][(size_t)i * s->len]);
+  }
+}
+
+// CLinearSystem_Field::UpdateValueOfField_NewmarkBeta
+void ref_sys_update_field_newmark(void* h, double gamma, double beta, double dt, int is_initial) {
+  RefSys* s = (RefSys*)h;
+  s->ls->UpdateValueOfField_NewmarkBeta(gamma, beta, dt, s->id_field, s->world, is_initial != 0);
 }
 
 void ref_sys_get_field(void* h, double* u) {
@@ -211,6 +243,18 @@ int ref_sys_assemble(void* h, int kind, const double* params, int npar, const do
         ok = Fem::Eqn::AddLinSys_Diffusion(glob[0], glob[1], *s->ls, p[0], p[1], p[2], s->world,
                                            s->id_field, id_ea);
         break;
+      case 6:  // glob = (dt, gamma, beta, is_initial)
+        ok = Fem::Eqn::AddLinSys_LinearSolid2D_NonStatic_NewmarkBeta(glob[0], glob[1], glob[2], *s->ls, p[0], p[1],
+                                                                     p[2], p[3], p[4], s->world, s->id_field,
+                                                                     glob[3] != 0.0, id_ea);
+        break;
+      case 7: {
+        const unsigned int idf = s->id_partial.empty() ? s->id_field : s->id_partial[g];
+        ok = Fem::Eqn::AddLinSys_LinearSolid3D_NonStatic_NewmarkBeta(glob[0], glob[1], glob[2], *s->ls, p[0], p[1],
+                                                                     p[2], p[3], p[4], p[5], s->world, idf);
+        if (s->id_partial.empty()) g = ngroup;
+        break;
+      }
       default: return -1;
     }
     if (!ok) return -2;

@@ -297,3 +297,91 @@ def test_saved_solve_equals_incremental_solve(ndim, length, kind, par):
     assert np.allclose(u_save, u_dir, rtol=0, atol=1e-9 * scale)
     assert np.allclose(u_inc, u_dir, rtol=0, atol=1e-9 * scale)
     assert np.array_equal(u_save[fx], u[fx])
+
+
+# ---- Newmark-beta: what the mathematics fixes -------------------------------------------------------------------
+def _nm_system(ndim, seed=4):
+    co, ln = (M.rect_tri(6, 3, 1.0, 0.3, seed) if ndim == 2 else M.box_tet(3, 2, 2, 1.0, 0.6, 0.5, seed))
+    return co, ln, co.size // ndim
+
+
+def _nm_dense(co, ln, ndim, par, glob, u=None, v=None, a=None, flags=None):
+    n = co.size // ndim
+    z = np.zeros(n * ndim)
+    cpu = P.CpuSystem(ndim, co, ln, ndim)
+    cpu.assemble_newmark(par, z if u is None else u, z if v is None else v, z if a is None else a, glob)
+    if flags is not None:
+        cpu.set_bc(flags)
+    return cpu, dense(cpu)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_newmark_mass_and_stiffness_parts(ndim):
+    """beta = 0: the matrix is the consistent mass (rho w / 12 (2D) or / 20 (3D) per node pair, doubled on the
+    diagonal; total mass rho * |domain| per direction); rho = 0, beta dt^2 = 1: it is the stiffness, entry for
+    entry the saved loops' K; u = v = a = 0: the residual is the body force."""
+    co, ln, n = _nm_system(ndim)
+    g = [0.0, -0.3] if ndim == 2 else [0.1, 0.0, -0.3]
+    rho = 1.7
+    cpu, Mm = _nm_dense(co, ln, ndim, [0.8, 0.5, rho] + g, (0.1, 0.5, 0.0, 1.0))
+    vol = sum(abs(np.linalg.det(np.column_stack([co.reshape(-1, ndim)[e[k]] - co.reshape(-1, ndim)[e[0]]
+                                                    for k in range(1, ndim + 1)]))) / (2 if ndim == 2 else 6)
+              for e in ln.reshape(-1, ndim + 1))
+    for d in range(ndim):
+        assert np.isclose(Mm[d::ndim, d::ndim].sum(), rho * vol, rtol=1e-12)
+    assert np.allclose(Mm, Mm.T, atol=1e-15)
+    e0 = ln.reshape(-1, ndim + 1)[0]
+    one = P.CpuSystem(ndim, co, e0, ndim)
+    one.assemble_newmark([0.8, 0.5, rho] + g, np.zeros(n * ndim), np.zeros(n * ndim), np.zeros(n * ndim),
+                         (0.1, 0.5, 0.0, 1.0))
+    M1 = dense(one)[np.ix_(np.repeat(e0, ndim) * ndim + np.tile(np.arange(ndim), ndim + 1),
+                           np.repeat(e0, ndim) * ndim + np.tile(np.arange(ndim), ndim + 1))]
+    w = abs(np.linalg.det(np.column_stack([co.reshape(-1, ndim)[e0[k]] - co.reshape(-1, ndim)[e0[0]]
+                                           for k in range(1, ndim + 1)]))) / (2 if ndim == 2 else 6)
+    want = np.kron(rho * w / (12 if ndim == 2 else 20) * (np.ones((ndim + 1, ndim + 1)) + np.eye(ndim + 1)),
+                   np.eye(ndim))
+    assert np.allclose(M1, want, rtol=1e-12)
+    body = np.zeros(n * ndim)
+    for e in ln.reshape(-1, ndim + 1):
+        we = abs(np.linalg.det(np.column_stack([co.reshape(-1, ndim)[e[k]] - co.reshape(-1, ndim)[e[0]]
+                                                for k in range(1, ndim + 1)]))) / (2 if ndim == 2 else 6)
+        for k in e:
+            body[k * ndim:(k + 1) * ndim] += we * rho * np.array(g) / (ndim + 1)
+    assert np.allclose(cpu.res, body, rtol=1e-12, atol=1e-15)
+    # stiffness part equals the saved loops' K bit for bit
+    kk, K = _nm_dense(co, ln, ndim, [0.8, 0.5, 0.0] + g, (1.0, 0.5, 1.0, 1.0))
+    sv = P.CpuSystem(ndim, co, ln, ndim)
+    sv.assemble_save(P.SOLID2D if ndim == 2 else P.SOLID3D, [0.8, 0.5, 0.0] + g, np.zeros(n * ndim, np.int32))
+    assert np.array_equal(kk.crs, sv.crs) and np.array_equal(kk.dia, sv.dia)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_newmark_average_acceleration_conserves_energy(ndim):
+    """beta = 1/4, gamma = 1/2 on an undamped linear system: E = v'Mv/2 + u'Ku/2 is constant from step to step
+    (consistent initial acceleration), and the update of UpdateValueOfField_NewmarkBeta with the solved increment
+    satisfies M a + K u = 0 after every step."""
+    co, ln, n = _nm_system(ndim, 6)
+    m = n * ndim
+    x = co.reshape(-1, ndim)
+    flags = np.zeros((n, ndim), np.int32)
+    flags[x[:, 0] == x[:, 0].min()] = 1
+    flags = flags.ravel()
+    par = [0.9, 0.6, 1.3] + [0.0] * ndim
+    dt, gamma, beta = 0.02, 0.5, 0.25
+    _, K = _nm_dense(co, ln, ndim, par[:2] + [0.0] + par[3:], (1.0, gamma, 1.0, 1.0))
+    _, Mm = _nm_dense(co, ln, ndim, par, (dt, gamma, 0.0, 1.0))
+    fr = flags == 0
+    u = np.zeros(m)
+    u[fr] = 0.01 * np.sin(3 * x.repeat(ndim, axis=0)[fr, 0])
+    v = np.zeros(m)
+    a = np.zeros(m)
+    a[fr] = np.linalg.solve(Mm[np.ix_(fr, fr)], -(K[np.ix_(fr, fr)] @ u[fr]))
+    energy = lambda u, v: 0.5 * v @ Mm @ v + 0.5 * u @ K @ u
+    e0 = energy(u, v)
+    for _ in range(20):
+        cpu, A = _nm_dense(co, ln, ndim, par, (dt, gamma, beta, 1.0), u, v, a, flags)
+        du = np.linalg.solve(A, cpu.res)
+        u, v, a = P.CpuSystem.update_field_newmark(gamma, beta, dt, True, du, u, v, a)
+        assert np.allclose((Mm @ a + K @ u)[fr], 0.0, atol=1e-10 * np.abs(K @ u).max())
+        assert abs(energy(u, v) - e0) <= 1e-10 * e0
+    assert np.abs(v).max() > 0

@@ -413,3 +413,91 @@ def test_saved_stiffness_solve_and_update_bit_exact(case):
     assert np.array_equal(ref.get_field(), u2)
     flags = cpu.flags
     assert np.array_equal(u2[flags == 1], u[flags == 1])  # Dirichlet values kept
+
+
+# ---- Newmark-beta dynamic linear elasticity -----------------------------------------------------------------
+NM_CASES = [
+    ("solid2d", 2, P.SOLID2D_NEWMARK, lambda r: [LAM2 * r.uniform(0.5, 2), MU2, r.uniform(0.5, 2), 0.0, -0.01]),
+    ("solid3d", 3, P.SOLID3D_NEWMARK, lambda r: [LAM3, MU3 * r.uniform(0.5, 2), r.uniform(0.5, 2), 0.01, 0.0, -0.02]),
+]
+
+
+@pytest.mark.parametrize("case", NM_CASES, ids=[c[0] for c in NM_CASES])
+@pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
+@pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
+def test_newmark_assembly_bit_exact(case, per_element, initial):
+    name, ndim, kind, pfun = case
+    if per_element:
+        co, ln = mesh2d(4, 2) if ndim == 2 else mesh3d(2, 2, 1)
+    else:
+        co, ln = mesh2d(9, 5) if ndim == 2 else mesh3d(3, 3, 2)
+    n, nelem = co.size // ndim, ln.size // (ndim + 1)
+    rng = np.random.default_rng(9)
+    if per_element:
+        params = np.array([pfun(rng) for _ in range(nelem)])
+        groups = np.arange(nelem, dtype=np.int32)
+    else:
+        params = np.array(pfun(rng))
+        groups = None
+    ref = P.RefSystem(ndim, co, ln, ndim, groups, dynamic=True)
+    cpu = P.CpuSystem(ndim, co, ln, ndim)
+    u, v, a = (_rand_u(n, ndim, s, 0.05) for s in (1, 2, 3))
+    glob = (0.03, 0.6, 0.3025, 1.0 if initial else 0.0)
+    ref.set_field(u, v)
+    ref.set_acc(a)
+    flags = clamp_flags(co, ndim, ndim)
+    ref.set_bc(flags)
+    rn = ref.assemble(kind, params if per_element else params[None, :], glob)
+    cpu.assemble_newmark(params, u, v, a, glob)
+    rn2 = cpu.set_bc(flags)
+    crs, dia = ref.matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    assert np.array_equal(ref.vector(-1), cpu.res), np.abs(ref.vector(-1) - cpu.res).max()
+    assert rn == rn2
+
+
+@pytest.mark.parametrize("case", NM_CASES, ids=[c[0] for c in NM_CASES])
+def test_newmark_time_steps_bit_exact(case):
+    """Three steps of CEqn_Solid3D_Linear / CEqnSystem_Solid2D's dynamic flow: assemble (is_initial), ILU(1) PCG,
+    UpdateValueOfField_NewmarkBeta(is_initial) — fields after every step, element for element."""
+    name, ndim, kind, pfun = case
+    co, ln = mesh2d(9, 4) if ndim == 2 else mesh3d(3, 2, 2)
+    n = co.size // ndim
+    params = np.array(pfun(np.random.default_rng(2)))
+    ref = P.RefSystem(ndim, co, ln, ndim, dynamic=True)
+    cpu = P.CpuSystem(ndim, co, ln, ndim)
+    flags = clamp_flags(co, ndim, ndim)
+    ref.set_bc(flags)
+    dt, gamma, beta = 0.05, 0.6, 0.3025
+    u, v, a = np.zeros(n * ndim), np.zeros(n * ndim), np.zeros(n * ndim)
+    ref.set_field(u, v)
+    ref.set_acc(a)
+    for step in range(3):
+        ref.assemble(kind, params[None, :], (dt, gamma, beta, 1.0))
+        cpu.zero()
+        cpu.assemble_newmark(params, u, v, a, (dt, gamma, beta, 1.0))
+        cpu.set_bc(flags)
+        f = cpu.ilu(None, 1)
+        ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-10, 1000)
+        rok, rconv, rit = ref.solve("pcg", 1e-10, 1000, 1, None)
+        assert (ok, conv, it) == (bool(rok), rconv, rit)
+        assert np.array_equal(ref.vector(-2), x)
+        ref.update_field_newmark(gamma, beta, dt, True)
+        u, v, a = cpu.update_field_newmark(gamma, beta, dt, True, x, u, v, a)
+        ru, rv, ra = ref.get_fields()
+        assert np.array_equal(ru, u) and np.array_equal(rv, v) and np.array_equal(ra, a)
+    assert np.abs(u).max() > 0
+
+
+def test_newmark_update_not_initial_bit_exact():
+    co, ln = mesh2d(5, 3)
+    n = co.size // 2
+    ref = P.RefSystem(2, co, ln, 2, dynamic=True)
+    u, v, a, x = (_rand_u(n, 2, s, 0.1) for s in (4, 5, 6, 7))
+    ref.set_field(u, v)
+    ref.set_acc(a)
+    ref.set_vector(-2, x)
+    ref.update_field_newmark(0.55, 0.27, 0.02, False)
+    got = P.CpuSystem.update_field_newmark(0.55, 0.27, 0.02, False, x, u, v, a)
+    for r, c in zip(ref.get_fields(), got):
+        assert np.array_equal(r, c)
