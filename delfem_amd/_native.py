@@ -11,11 +11,11 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdfm_gpu.so")
 
-POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK = range(8)
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER = range(9)
 ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKCOLOR, ORDER_RCM, ORDER_RCM2, ORDER_AMD = range(6)
 FP64, MIXED, FP32_ILU, MIXED_AUTO = 0, 1, 2, 3
 ERR_SINGULAR = -5
-NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3, SOLID2D_NEWMARK: 5, SOLID3D_NEWMARK: 6}
+NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3, SOLID2D_NEWMARK: 5, SOLID3D_NEWMARK: 6, SOLID2D_BEULER: 5}
 
 _lib = None
 _live = weakref.WeakSet()  # systems and contexts still open, closed in order at interpreter exit
@@ -239,6 +239,13 @@ class System:
         """CLinearSystem_Field::UpdateValueOfField_NewmarkBeta with the update as the acceleration increment."""
         _check(lib().dfm_sys_update_field_newmark(self.h, C.c_double(gamma), C.c_double(beta), C.c_double(dt),
                                                   1 if is_initial else 0))
+
+    def set_velocity_prev(self, v0=None):
+        _check(lib().dfm_sys_set_velocity_prev(self.h, _p(_f64(v0))))
+
+    def update_field_beuler(self, dt, is_initial=True):
+        """CLinearSystem_Field::UpdateValueOfField_BackwardEular with the update as the velocity increment."""
+        _check(lib().dfm_sys_update_field_beuler(self.h, C.c_double(dt), 1 if is_initial else 0))
 
     def update_field(self):
         _check(lib().dfm_sys_update_field(self.h))

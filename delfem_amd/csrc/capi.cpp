@@ -140,7 +140,7 @@ Sys::~Sys() {
   d_rowptr.release(); d_colind.release(); d_diaind.release(); d_lsplit.release(); d_usplit.release();
   d_frowptr.release(); d_fcolind.release(); d_fdiaind.release(); d_a2f.release();
   d_order.release(); d_inv.release(); d_elsup_ptr.release(); d_elsup.release(); d_lnods.release(); d_bc.release();
-  d_coords.release(); d_u.release(); d_v.release(); d_a.release();
+  d_coords.release(); d_u.release(); d_v.release(); d_a.release(); d_v0.release();
   d_crs.release(); d_dia.release(); d_crs32.release(); d_dia32.release();
   d_fcrs.release(); d_fdia.release();
   d_res.release(); d_upd.release(); d_rhs.release();
@@ -597,6 +597,34 @@ static int ensure_acc(Sys& s) {
   return DFM_OK;
 }
 
+// the previous velocity of the backward-Euler loop, zero until set
+static int ensure_v0(Sys& s) {
+  if (s.d_v0.p) return DFM_OK;
+  DFM_HIP(s.d_v0.alloc((size_t)mdof(s)));
+  DFM_HIP(launch_fill(s.d_v0.p, 0.0, mdof(s), s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_set_velocity_prev(dfm_sys sys, const double* v0) {
+  NEED(sys, "dfm_sys_set_velocity_prev");
+  Sys& s = sys->s;
+  int rc = ensure_v0(s);
+  if (rc != DFM_OK) return rc;
+  if (v0) DFM_HIP(hipMemcpyAsync(s.d_v0.p, v0, sizeof(double) * mdof(s), hipMemcpyHostToDevice, s.ctx->stream));
+  else DFM_HIP(launch_fill(s.d_v0.p, 0.0, mdof(s), s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
+int dfm_sys_update_field_beuler(dfm_sys sys, double dt, int is_initial) {
+  NEED(sys, "dfm_sys_update_field_beuler");
+  Sys& s = sys->s;
+  DFM_HIP(launch_update_field_beuler(s.n, s.len, s.d_order.p, s.d_upd.p, dt, is_initial ? 1 : 0, s.d_u.p, s.d_v.p,
+                                     s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
 int dfm_sys_set_acceleration(dfm_sys sys, const double* a) {
   NEED(sys, "dfm_sys_set_acceleration");
   Sys& s = sys->s;
@@ -695,9 +723,9 @@ int dfm_sys_set_load_scale(dfm_sys sys, double scale) {
 }
 
 static int check_kind(const Sys& s, int kind) {
-  static const int need_npar[8] = {2, 5, 6, 5, 6, 3, 5, 6};
-  static const int need_len[8] = {1, 2, 3, 2, 3, 1, 2, 3};
-  DFM_CHECK(kind >= 0 && kind < 8, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
+  static const int need_npar[9] = {2, 5, 6, 5, 6, 3, 5, 6, 5};
+  static const int need_len[9] = {1, 2, 3, 2, 3, 1, 2, 3, 2};
+  DFM_CHECK(kind >= 0 && kind < 9, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
   DFM_CHECK(s.d_par.p != nullptr, DFM_ERR_STATE, "dfm_sys_assemble: no element parameters set");
   DFM_CHECK(s.npar >= need_npar[kind], DFM_ERR_ARG, "dfm_sys_assemble: too few parameters for the element kind");
   DFM_CHECK(need_len[kind] == s.len, DFM_ERR_ARG, "dfm_sys_assemble: element kind does not match dofs per node");
@@ -729,6 +757,11 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
     rc = ensure_acc(s);
     if (rc != DFM_OK) return rc;
   }
+  if (kind == DFM_SOLID2D_BEULER) {
+    DFM_CHECK(glob != nullptr, DFM_ERR_ARG, "dfm_sys_assemble: the backward-Euler kind needs glob = (dt, is_initial)");
+    rc = ensure_v0(s);
+    if (rc != DFM_OK) return rc;
+  }
   hipStream_t st = s.ctx->stream;
   const long long m = mdof(s);
   // InitializeMarge (src/ls/linearsystem.cpp:74-88): matrix, residual and update set to zero
@@ -744,7 +777,7 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   a.glob0 = glob ? glob[0] : 0.0; a.glob1 = glob ? glob[1] : 0.0;
   a.glob2 = newmark ? glob[2] : 0.0;
   a.glob3 = kind == DFM_SOLID2D_NEWMARK ? glob[3] : 0.0;
-  a.u = s.d_u.p; a.v = s.d_v.p; a.acc = s.d_a.p;
+  a.u = s.d_u.p; a.v = s.d_v.p; a.acc = kind == DFM_SOLID2D_BEULER ? s.d_v0.p : s.d_a.p;
   a.load_ptr = s.has_loads ? s.d_lptr.p : nullptr;
   a.loads = s.has_loads ? s.d_loads.p : nullptr;
   a.load_scale = s.load_scale;

@@ -177,7 +177,8 @@ struct Sys {
   DBuf<unsigned> d_frowptr, d_fcolind, d_fdiaind, d_a2f;
   DBuf<int> d_order, d_inv, d_elsup_ptr, d_elsup, d_lnods, d_bc;
   DBuf<double> d_coords, d_u, d_v;
-  DBuf<double> d_a;  // ACCELERATION field, allocated on first use (Newmark)
+  DBuf<double> d_a;   // ACCELERATION field, allocated on first use (Newmark)
+  DBuf<double> d_v0;  // previous velocity of the backward-Euler loop, allocated on first use
   DBuf<double> d_crs, d_dia;         // matrix, device numbering, FP64
   DBuf<float> d_crs32, d_dia32;      // FP32 copy (mixed precision)
   DBuf<double> d_fcrs, d_fdia;       // ILU factor FP64 (factor pattern)

@@ -446,6 +446,44 @@ __device__ inline void row_solid_newmark(int a, const double co[4][3], const dou
   }
 }
 
+// Backward-Euler row (AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_linear_solid2d.cpp:
+// 1338-1496; unknown: the velocity increment): the Newmark loop's K and M, element matrix M + dt^2 K, residual
+// dt f - dt K u, then - dt^2 K v when is_initial, else - M (v - v_prev). aa holds v_prev here.
+__device__ inline void row_solid2d_beuler(int a, const double co[4][3], const double u[4][3], const double vv[4][3],
+                                          const double v0[4][3], const double* p, double dt, bool initial,
+                                          double* em, double* er) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  const double w = tri_area(co[0], co[1], co[2]);
+  double d[3][2];
+  tri_dldx(d, co);
+  double K[3][4], M[3][4];
+  for (int j = 0; j < 3; j++) {
+    double t = 0.0;
+    for (int r = 0; r < 2; r++) {
+      for (int c = 0; c < 2; c++) K[j][r * 2 + c] = 0.0 + w * (lambda * d[a][r] * d[j][c] + myu * d[j][r] * d[a][c]);
+      t += d[a][r] * d[j][r];
+    }
+    for (int r = 0; r < 2; r++) K[j][r * 2 + r] += w * myu * t;
+  }
+  const double mm = w * rho * 0.0833333333333333333333333;
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 4; k++) M[j][k] = (k % 3 == 0) ? mm : 0.0;
+  for (int r = 0; r < 2; r++) M[a][r * 2 + r] += mm;
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 4; k++) em[j * 4 + k] = M[j][k] + dt * dt * K[j][k];
+  for (int r = 0; r < 2; r++) er[r] = w * rho * p[3 + r] * 0.33333333333333333333333333 * dt;
+  for (int j = 0; j < 3; j++)
+    for (int r = 0; r < 2; r++) er[r] -= dt * (K[j][r * 2] * u[j][0] + K[j][r * 2 + 1] * u[j][1]);
+  if (initial) {
+    for (int j = 0; j < 3; j++)
+      for (int r = 0; r < 2; r++) er[r] -= dt * dt * (K[j][r * 2] * vv[j][0] + K[j][r * 2 + 1] * vv[j][1]);
+  } else {
+    for (int j = 0; j < 3; j++)
+      for (int r = 0; r < 2; r++)
+        er[r] -= M[j][r * 2] * (vv[j][0] - v0[j][0]) + M[j][r * 2 + 1] * (vv[j][1] - v0[j][1]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------------
 
 template <int KIND, int NDIM, int B, bool SAVE>
@@ -467,7 +505,7 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
     int a = 0;
     for (int c = 0; c < NNO; c++)
       if (no[c] == i1) a = c;
-    constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK);
+    constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER);
     double co[4][3], u[4][3], val[4], vval[4];
     double vv[4][3], aa[4][3];
     for (int c = 0; c < NNO; c++) {
@@ -493,7 +531,9 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
       if (KIND == DFM_SOLID3D) row_solid3d(a, co, u, p, em, er);
       if (KIND == DFM_STVK2D) row_stvk2d(a, co, u, p, em, er);
       if (KIND == DFM_STVK3D) row_stvk3d(a, co, u, p, em, er);
-      if (NM) row_solid_newmark<NDIM>(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+      if (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK)
+        row_solid_newmark<NDIM>(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+      if (KIND == DFM_SOLID2D_BEULER) row_solid2d_beuler(a, co, u, vv, aa, p, A.glob0, A.glob1 != 0.0, em, er);
     }
     for (int c = 0; c < NNO; c++) {
       const double* blk = &em[c * BS];
@@ -544,6 +584,7 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
   DFM_ASM(DFM_STVK3D, 3, 3)
   DFM_ASM(DFM_SOLID2D_NEWMARK, 2, 2)
   DFM_ASM(DFM_SOLID3D_NEWMARK, 3, 3)
+  DFM_ASM(DFM_SOLID2D_BEULER, 2, 2)
 #undef DFM_ASM
   return hipErrorInvalidValue;
 }
@@ -640,6 +681,31 @@ hipError_t launch_update_field_newmark(int n, int len, const int* order, const d
   const int nt = 256;
   hipLaunchKernelGGL(k_update_field_newmark, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, upd,
                      gamma, beta, dt, initial, u, v, a);
+  return hipGetLastError();
+}
+
+// CLinearSystem_Field::UpdateValueOfField_BackwardEular (src/femls/linearsystem_field.cpp:1061-1101), the update
+// being the velocity increment: v += du, then u += dt v (is_initial) or u += dt du.
+__global__ void k_update_field_beuler(long long m, int len, const int* order, const double* upd, double dt,
+                                      int initial, double* u, double* v) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long long i0 = k / len;
+  const int d = (int)(k - i0 * len);
+  const long long i1 = (order ? order[i0] : i0) * len + d;
+  const double du = upd[k];
+  const double vv = v[i1] + 1.0 * du;
+  v[i1] = vv;
+  u[i1] = initial ? u[i1] + dt * vv : u[i1] + dt * du;
+}
+
+hipError_t launch_update_field_beuler(int n, int len, const int* order, const double* upd, double dt, int initial,
+                                      double* u, double* v, hipStream_t st) {
+  const long long m = (long long)n * len;
+  if (m == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_update_field_beuler, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, upd,
+                     dt, initial, u, v);
   return hipGetLastError();
 }
 

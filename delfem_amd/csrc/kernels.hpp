@@ -21,7 +21,7 @@ struct AsmArgs {
   double glob0, glob1, glob2, glob3;  // kind-specific scalars (diffusion: dt, gamma; Newmark: dt, gamma, beta, initial)
   const double* u;  // field values (original numbering), may be null
   const double* v;
-  const double* acc;  // ACCELERATION field values (original numbering), Newmark kinds only
+  const double* acc;  // ACCELERATION field (Newmark) or the previous velocity (backward Euler), original numbering
   const int* load_ptr;   // node -> surface-load entries (original numbering), may be null
   const double* loads;   // [nentries*len]
   double load_scale;     // multiplies the surface loads (load stepping); 1 = as given
@@ -36,6 +36,8 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st);
 hipError_t launch_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
                          double* res, const int* bc, hipStream_t st);
 hipError_t launch_update_field(int n, int len, const int* order, const double* upd, double* u, hipStream_t st);
+hipError_t launch_update_field_beuler(int n, int len, const int* order, const double* upd, double dt, int initial,
+                                      double* u, double* v, hipStream_t st);
 hipError_t launch_update_field_newmark(int n, int len, const int* order, const double* upd, double gamma, double beta,
                                        double dt, int initial, double* u, double* v, double* a, hipStream_t st);
 // saved stiffness: K_b values from the matrix before the Dirichlet zeroing (src < ndia: diagonal blocks, else

@@ -46,7 +46,11 @@ enum {
   DFM_SOLID2D_NEWMARK = 6,
   /* AddLinSys_LinearSolid3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_linear_solid3d.cpp:525-690 (its predictor
    * terms always on), (lambda, myu, rho, gx, gy, gz), glob = (dt, gamma, beta) */
-  DFM_SOLID3D_NEWMARK = 7
+  DFM_SOLID3D_NEWMARK = 7,
+  /* backward-Euler dynamic 2D elasticity (unknown: the velocity increment; matrix M + dt^2 K):
+   * AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_linear_solid2d.cpp:1338-1497,
+   * (lambda, myu, rho, gx, gy), glob = (dt, is_initial); the previous velocity from dfm_sys_set_velocity_prev */
+  DFM_SOLID2D_BEULER = 8
 };
 
 /* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering); the last three are COrdering_Blk's, exactly
@@ -146,6 +150,11 @@ int dfm_sys_get_fields(dfm_sys sys, double* u, double* v, double* a);
  * being the acceleration increment: [is_initial: u += dt v, u += dt^2/2 a], u += beta dt^2 du,
  * [is_initial: v += dt a], v += gamma dt du, a += du; in that order, as the reference's node-array passes. */
 int dfm_sys_update_field_newmark(dfm_sys sys, double gamma, double beta, double dt, int is_initial);
+/* The velo_pre vector of AddLinSys_LinearSolid2D_NonStatic_BackwardEular (n*len; NULL = zero), and
+ * CLinearSystem_Field::UpdateValueOfField_BackwardEular (src/femls/linearsystem_field.cpp:1061-1101), the update
+ * being the velocity increment: v += du, then u += dt v (is_initial) or u += dt du. */
+int dfm_sys_set_velocity_prev(dfm_sys sys, const double* v0);
+int dfm_sys_update_field_beuler(dfm_sys sys, double dt, int is_initial);
 
 /* InitializeMarge + Fem::Eqn::AddLinSys_<kind> + [surface loads] + FinalizeMarge in one call
  * (src/ls/linearsystem.cpp:74-128). params: npar doubles per element with stride pstride (0 = shared).

@@ -1749,3 +1749,95 @@ void cr_update_field_newmark(long m, double gamma, double beta, double dt, int i
   { const double c = gamma * dt; for (k = 0; k < m; k++) v[k] += c * upd[k]; }
   for (k = 0; k < m; k++) a[k] += 1.0 * upd[k];
 }
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Backward-Euler dynamic 2D elasticity: AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1               */
+/* (src/femeqn/eqn_linear_solid2d.cpp:1338-1497) and UpdateValueOfField_BackwardEular                     */
+/* (src/femls/linearsystem_field.cpp:1061-1101), restated loop by loop.                                   */
+
+int cr_assemble_beuler(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                       int pstride, double dt, int is_initial, const double* u, const double* v,
+                       const double* velo_pre, const unsigned* rowptr, const unsigned* colind, double* crs,
+                       double* dia, double* res) {
+  (void)nnode;
+  double eK[3][3][2][2], eM[3][3][2][2], emat[3][3][2][2], eqf[3][2], eres[3][2];
+  double co[4][3], disp[3][2], velo[3][2], velo0[3][2];
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * 3];
+    const double* p = &params[e * pstride];
+    const double lambda = p[0], myu = p[1], rho = p[2], g_x = p[3], g_y = p[4];
+    for (int k = 0; k < 3; k++) {
+      co[k][0] = coords[(long)no[k] * 2];
+      co[k][1] = coords[(long)no[k] * 2 + 1];
+      co[k][2] = 0.0;
+      for (int d = 0; d < 2; d++) {
+        disp[k][d] = u[(long)no[k] * 2 + d];
+        velo[k][d] = v[(long)no[k] * 2 + d];
+        velo0[k][d] = velo_pre[(long)no[k] * 2 + d];
+      }
+    }
+    for (int i = 0; i < 36; i++) { (&emat[0][0][0][0])[i] = 0.0; (&eK[0][0][0][0])[i] = 0.0; (&eM[0][0][0][0])[i] = 0.0; }
+    for (int i = 0; i < 6; i++) (&eqf[0][0])[i] = 0.0;
+    const double area = tri_area(co[0], co[1], co[2]);
+    double dldx[3][2];
+    tri_dldx(dldx, co[0], co[1], co[2]);
+    for (int ino = 0; ino < 3; ino++)
+      for (int jno = 0; jno < 3; jno++) {
+        double dtmp1 = 0.0;
+        for (int idim = 0; idim < 2; idim++) {
+          for (int jdim = 0; jdim < 2; jdim++)
+            eK[ino][jno][idim][jdim] += area * (lambda * dldx[ino][idim] * dldx[jno][jdim] + myu * dldx[jno][idim] * dldx[ino][jdim]);
+          dtmp1 += dldx[ino][idim] * dldx[jno][idim];
+        }
+        for (int idim = 0; idim < 2; idim++) eK[ino][jno][idim][idim] += area * myu * dtmp1;
+      }
+    {
+      const double dtmp1 = area * rho * 0.0833333333333333333333333;
+      for (int ino = 0; ino < 3; ino++) {
+        for (int jno = 0; jno < 3; jno++) {
+          eM[ino][jno][0][0] += dtmp1;
+          eM[ino][jno][1][1] += dtmp1;
+        }
+        eM[ino][ino][0][0] += dtmp1;
+        eM[ino][ino][1][1] += dtmp1;
+      }
+    }
+    for (int ino = 0; ino < 3; ino++) {
+      eqf[ino][0] = area * rho * g_x * 0.33333333333333333333333333;
+      eqf[ino][1] = area * rho * g_y * 0.33333333333333333333333333;
+    }
+    for (int i = 0; i < 36; i++) (&emat[0][0][0][0])[i] = (&eM[0][0][0][0])[i] + dt * dt * (&eK[0][0][0][0])[i];
+    for (int ino = 0; ino < 3; ino++) {
+      eres[ino][0] = eqf[ino][0] * dt;
+      eres[ino][1] = eqf[ino][1] * dt;
+      for (int jno = 0; jno < 3; jno++) {
+        eres[ino][0] -= dt * (eK[ino][jno][0][0] * disp[jno][0] + eK[ino][jno][0][1] * disp[jno][1]);
+        eres[ino][1] -= dt * (eK[ino][jno][1][0] * disp[jno][0] + eK[ino][jno][1][1] * disp[jno][1]);
+      }
+    }
+    if (is_initial) {
+      for (int ino = 0; ino < 3; ino++)
+        for (int jno = 0; jno < 3; jno++) {
+          eres[ino][0] -= dt * dt * (eK[ino][jno][0][0] * velo[jno][0] + eK[ino][jno][0][1] * velo[jno][1]);
+          eres[ino][1] -= dt * dt * (eK[ino][jno][1][0] * velo[jno][0] + eK[ino][jno][1][1] * velo[jno][1]);
+        }
+    } else {
+      for (int ino = 0; ino < 3; ino++)
+        for (int jno = 0; jno < 3; jno++) {
+          eres[ino][0] -= eM[ino][jno][0][0] * (velo[jno][0] - velo0[jno][0]) + eM[ino][jno][0][1] * (velo[jno][1] - velo0[jno][1]);
+          eres[ino][1] -= eM[ino][jno][1][0] * (velo[jno][0] - velo0[jno][0]) + eM[ino][jno][1][1] * (velo[jno][1] - velo0[jno][1]);
+        }
+    }
+    merge(3, no, 4, &emat[0][0][0][0], rowptr, colind, crs, dia);
+    for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 2; d++) res[(long)no[k] * 2 + d] += eres[k][d];
+  }
+  return 0;
+}
+
+void cr_update_field_beuler(long m, double dt, int is_initial, const double* upd, double* u, double* v) {
+  long k;
+  for (k = 0; k < m; k++) v[k] += 1.0 * upd[k];
+  if (is_initial) { for (k = 0; k < m; k++) u[k] += dt * v[k]; }
+  else { for (k = 0; k < m; k++) u[k] += dt * upd[k]; }
+}

@@ -120,6 +120,14 @@ int cr_assemble_newmark(int ndim, int nnode, const double* coords, int nelem, co
 void cr_update_field_newmark(long m, double gamma, double beta, double dt, int is_initial, const double* upd, double* u,
                              double* v, double* a);
 
+/* Backward-Euler dynamic 2D elasticity: element loop (velo_pre the previous velocity) and
+ * UpdateValueOfField_BackwardEular over m values. */
+int cr_assemble_beuler(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                       int pstride, double dt, int is_initial, const double* u, const double* v,
+                       const double* velo_pre, const unsigned* rowptr, const unsigned* colind, double* crs,
+                       double* dia, double* res);
+void cr_update_field_beuler(long m, double dt, int is_initial, const double* upd, double* u, double* v);
+
 #ifdef __cplusplus
 }
 #endif

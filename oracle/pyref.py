@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CPU_REF_SO = os.path.join(HERE, "libcpu_ref.so")
 REF_SO = os.path.join(HERE, "_ref", "libdfm_ref.so")
 
-POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK = range(8)
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER = range(9)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -159,6 +159,25 @@ class CpuSystem:
                                            p(params), pstride, p(glob), p(u), p(v), p(a), p(self.rowptr),
                                            p(self.colind), p(self.crs), p(self.dia), p(self.res))
         assert rc == 0
+
+    def assemble_beuler(self, params, u, v, v0, dt, is_initial):
+        """Backward-Euler 2D element loop (restated) into self.crs / self.dia / self.res (zero them first)."""
+        params = _f64(params)
+        pstride = 0 if params.ndim == 1 else params.shape[1]
+        u, v, v0 = _f64(u), _f64(v), _f64(v0)
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        rc = cpu_ref().cr_assemble_beuler(self.nnode, p(self.coords), self.nelem, p(self.lnods), p(params), pstride,
+                                          C.c_double(dt), 1 if is_initial else 0, p(u), p(v), p(v0), p(self.rowptr),
+                                          p(self.colind), p(self.crs), p(self.dia), p(self.res))
+        assert rc == 0
+
+    @staticmethod
+    def update_field_beuler(dt, is_initial, upd, u, v):
+        """UpdateValueOfField_BackwardEular (restated): returns new (u, v)."""
+        u, v, upd = _f64(u).copy(), _f64(v).copy(), _f64(upd)
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        cpu_ref().cr_update_field_beuler(C.c_long(u.size), C.c_double(dt), 1 if is_initial else 0, p(upd), p(u), p(v))
+        return u, v
 
     @staticmethod
     def update_field_newmark(gamma, beta, dt, is_initial, upd, u, v, a):
@@ -433,10 +452,19 @@ class RefSystem:
         a = None if a is None else _f64(a)
         reference().ref_sys_set_acc(self.h, None if a is None else a.ctypes.data_as(C.c_void_p))
 
-    def get_fields(self):
-        out = [np.zeros(self.nnode * self.len) for _ in range(3)]
-        reference().ref_sys_get_fields(self.h, *[o.ctypes.data_as(C.c_void_p) for o in out])
-        return out
+    def get_fields(self, acc=True):
+        """(u, v, a) of the field; a is None (not read) when acc is False (a field made without ACCELERATION)."""
+        out = [np.zeros(self.nnode * self.len) for _ in range(3 if acc else 2)]
+        ptrs = [o.ctypes.data_as(C.c_void_p) for o in out] + ([] if acc else [None])
+        reference().ref_sys_get_fields(self.h, *ptrs)
+        return out if acc else out + [None]
+
+    def set_velo_pre(self, v0=None):
+        v0 = None if v0 is None else _f64(v0)
+        reference().ref_sys_set_velo_pre(self.h, None if v0 is None else v0.ctypes.data_as(C.c_void_p))
+
+    def update_field_beuler(self, dt, is_initial):
+        reference().ref_sys_update_field_beuler(self.h, C.c_double(dt), 1 if is_initial else 0)
 
     def update_field_newmark(self, gamma, beta, dt, is_initial):
         reference().ref_sys_update_field_newmark(self.h, C.c_double(gamma), C.c_double(beta), C.c_double(dt),

@@ -67,7 +67,8 @@ struct RefSys {
   Fem::Field::CFieldWorld world;
   Fem::Ls::CLinearSystem_Field* ls = 0;  // a CLinearSystem_Save for the saved-stiffness path
   bool save = false;
-  ~RefSys() { delete ls; }
+  MatVec::CVector_Blk* velo_pre = 0;  // the backward-Euler loop's previous velocity
+  ~RefSys() { delete ls; delete velo_pre; }
   CGroupMesh mesh;
   unsigned int id_base, id_field;
   std::vector<unsigned int> id_ea;       // per group
@@ -183,6 +184,19 @@ void ref_sys_get_fields(void* h, double* u, double* v, double* a) {
   }
 }
 
+void ref_sys_set_velo_pre(void* h, const double* v0) {
+  RefSys* s = (RefSys*)h;
+  if (!s->velo_pre) s->velo_pre = new MatVec::CVector_Blk(s->nnode, s->len);
+  for (int i = 0; i < s->nnode; i++)
+    for (int d = 0; d < s->len; d++) s->velo_pre->SetValue(i, d, v0 ? v0[(size_t)i * s->len + d] : 0.0);
+}
+
+// CLinearSystem_Field::UpdateValueOfField_BackwardEular
+void ref_sys_update_field_beuler(void* h, double dt, int is_initial) {
+  RefSys* s = (RefSys*)h;
+  s->ls->UpdateValueOfField_BackwardEular(dt, s->id_field, s->world, is_initial != 0);
+}
+
 // CLinearSystem_Field::UpdateValueOfField_NewmarkBeta
 void ref_sys_update_field_newmark(void* h, double gamma, double beta, double dt, int is_initial) {
   RefSys* s = (RefSys*)h;
@@ -247,6 +261,11 @@ int ref_sys_assemble(void* h, int kind, const double* params, int npar, const do
         ok = Fem::Eqn::AddLinSys_LinearSolid2D_NonStatic_NewmarkBeta(glob[0], glob[1], glob[2], *s->ls, p[0], p[1],
                                                                      p[2], p[3], p[4], s->world, s->id_field,
                                                                      glob[3] != 0.0, id_ea);
+        break;
+      case 8:  // glob = (dt, is_initial); the previous velocity from ref_sys_set_velo_pre
+        ok = Fem::Eqn::AddLinSys_LinearSolid2D_NonStatic_BackwardEular(glob[0], *s->ls, p[0], p[1], p[2], p[3], p[4],
+                                                                       s->world, s->id_field, *s->velo_pre,
+                                                                       glob[1] != 0.0, id_ea);
         break;
       case 7: {
         const unsigned int idf = s->id_partial.empty() ? s->id_field : s->id_partial[g];

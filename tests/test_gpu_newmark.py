@@ -179,3 +179,36 @@ def test_newmark_errors(ctx):
         gs.assemble(6, [LAM2, MU2, 1.0, 0.0, 0.0])  # no (dt, gamma, beta, is_initial)
     with pytest.raises(N.DfmError):
         gs.assemble(7, [LAM3, MU3, 1.0, 0.0, 0.0, 0.0], glob=(0.1, 0.5, 0.25, 1.0))  # 3D kind on a 2D mesh
+
+
+# ---- backward-Euler dynamic 2D elasticity ------------------------------------------------------------------
+@pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
+@pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
+def test_beuler_assembly_and_update_bit_exact(ctx, oname, ordering, initial):
+    print(f"backward euler {oname} initial={initial}", flush=True)
+    co, ln = mesh(2, True)
+    n = co.size // 2
+    params = NM[0][3](np.random.default_rng(17), ln.size // 3)
+    rng = np.random.default_rng(31)
+    u, v, v0 = (rng.uniform(-0.05, 0.05, 2 * n) for _ in range(3))
+    fl = flags(co, 2, 2)
+    cpu = P.CpuSystem(2, co, ln, 2)
+    cpu.assemble_beuler(params, u, v, v0, 0.04, initial)
+    cpu.set_bc(fl)
+    gs = N.System(ctx, 2, co, ln, 2)
+    if ordering != 0:
+        order, _ = gs.make_ordering(ordering)
+        gs.set_ordering(order)
+    gs.set_bc(fl)
+    gs.set_field(u, v)
+    gs.set_velocity_prev(v0)
+    gs.assemble(8, params, glob=(0.04, 1.0 if initial else 0.0))
+    crs, dia = gs.get_matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    assert np.array_equal(gs.get_vector(-1), cpu.res), np.abs(gs.get_vector(-1) - cpu.res).max()
+    x = rng.normal(size=2 * n)
+    gs.set_vector(-2, x)
+    gs.update_field_beuler(0.04, initial)
+    wu, wv = P.CpuSystem.update_field_beuler(0.04, initial, x, u, v)
+    gu, gv, _ = gs.get_fields()
+    assert np.array_equal(gu, wu) and np.array_equal(gv, wv)

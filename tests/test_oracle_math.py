@@ -385,3 +385,28 @@ def test_newmark_average_acceleration_conserves_energy(ndim):
         assert np.allclose((Mm @ a + K @ u)[fr], 0.0, atol=1e-10 * np.abs(K @ u).max())
         assert abs(energy(u, v) - e0) <= 1e-10 * e0
     assert np.abs(v).max() > 0
+
+
+def test_beuler_step_is_implicit_euler():
+    """The backward-Euler loop's system M dv = dt f - dt K u - dt^2 K v (is_initial) with u += dt v_new is one
+    implicit Euler step of M v' = f - K u, u' = v: M (v1 - v) = dt (f - K u1) with u1 = u + dt v1."""
+    co, ln = M.rect_tri(5, 3, 1.0, 0.3, 9)
+    n = co.size // 2
+    par = [0.7, 0.4, 1.3, 0.0, -0.2]
+    rng = np.random.default_rng(1)
+    u, v = rng.uniform(-0.01, 0.01, 2 * n), rng.uniform(-0.01, 0.01, 2 * n)
+    dt = 0.05
+    cpu = P.CpuSystem(2, co, ln, 2)
+    cpu.assemble_beuler(par, u, v, np.zeros(2 * n), dt, True)
+    A = dense(cpu)
+    du = np.linalg.solve(A, cpu.res)
+    u1, v1 = P.CpuSystem.update_field_beuler(dt, True, du, u, v)
+    cm = P.CpuSystem(2, co, ln, 2)
+    cm.assemble_newmark(par, np.zeros(2 * n), np.zeros(2 * n), np.zeros(2 * n), (dt, 0.5, 0.0, 1.0))
+    Mm, f = dense(cm), cm.res
+    ck = P.CpuSystem(2, co, ln, 2)
+    ck.assemble_newmark(par[:2] + [0.0, 0.0, 0.0], np.zeros(2 * n), np.zeros(2 * n), np.zeros(2 * n), (1.0, 0.5, 1.0, 1.0))
+    K = dense(ck)
+    assert np.allclose(u1, u + dt * v1, rtol=0, atol=1e-15)
+    lhs, rhs = Mm @ (v1 - v), dt * (f - K @ u1)
+    assert np.allclose(lhs, rhs, rtol=0, atol=1e-12 * np.abs(rhs).max())

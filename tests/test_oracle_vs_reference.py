@@ -501,3 +501,43 @@ def test_newmark_update_not_initial_bit_exact():
     got = P.CpuSystem.update_field_newmark(0.55, 0.27, 0.02, False, x, u, v, a)
     for r, c in zip(ref.get_fields(), got):
         assert np.array_equal(r, c)
+
+
+# ---- backward-Euler dynamic 2D elasticity -------------------------------------------------------------------
+@pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
+@pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
+def test_beuler_assembly_bit_exact(per_element, initial):
+    co, ln = mesh2d(4, 2) if per_element else mesh2d(9, 5)
+    n, nelem = co.size // 2, ln.size // 3
+    rng = np.random.default_rng(12)
+    pf = lambda r: [LAM2 * r.uniform(0.5, 2), MU2, r.uniform(0.5, 2), 0.01, -0.02]
+    params = np.array([pf(rng) for _ in range(nelem)]) if per_element else np.array(pf(rng))
+    groups = np.arange(nelem, dtype=np.int32) if per_element else None
+    ref = P.RefSystem(2, co, ln, 2, groups)
+    cpu = P.CpuSystem(2, co, ln, 2)
+    u, v, v0 = (_rand_u(n, 2, s, 0.05) for s in (1, 2, 3))
+    ref.set_field(u, v)
+    ref.set_velo_pre(v0)
+    flags = clamp_flags(co, 2, 2)
+    ref.set_bc(flags)
+    rn = ref.assemble(P.SOLID2D_BEULER, params if per_element else params[None, :], (0.04, 1.0 if initial else 0.0))
+    cpu.assemble_beuler(params, u, v, v0, 0.04, initial)
+    rn2 = cpu.set_bc(flags)
+    crs, dia = ref.matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    assert np.array_equal(ref.vector(-1), cpu.res)
+    assert rn == rn2
+
+
+@pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
+def test_beuler_update_bit_exact(initial):
+    co, ln = mesh2d(5, 3)
+    n = co.size // 2
+    ref = P.RefSystem(2, co, ln, 2)
+    u, v, x = (_rand_u(n, 2, s, 0.1) for s in (4, 5, 7))
+    ref.set_field(u, v)
+    ref.set_vector(-2, x)
+    ref.update_field_beuler(0.03, initial)
+    gu, gv = P.CpuSystem.update_field_beuler(0.03, initial, x, u, v)
+    ru, rv, _ = ref.get_fields(acc=False)
+    assert np.array_equal(ru, gu) and np.array_equal(rv, gv)
