@@ -447,7 +447,7 @@ __device__ inline void row_solid_newmark(int a, const double co[4][3], const dou
 }
 
 // Backward-Euler row (AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_linear_solid2d.cpp:
-// 1338-1496; unknown: the velocity increment): the Newmark loop's K and M, element matrix M + dt^2 K, residual
+// 1338-1497; unknown: the velocity increment): the Newmark loop's K and M, element matrix M + dt^2 K, residual
 // dt f - dt K u, then - dt^2 K v when is_initial, else - M (v - v_prev). aa holds v_prev here.
 __device__ inline void row_solid2d_beuler(int a, const double co[4][3], const double u[4][3], const double vv[4][3],
                                           const double v0[4][3], const double* p, double dt, bool initial,
