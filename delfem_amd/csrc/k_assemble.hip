@@ -446,6 +446,37 @@ __device__ inline void row_solid_newmark(int a, const double co[4][3], const dou
   }
 }
 
+// StVK Newmark-beta row (AddLinSys_StVenant3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:1109-1242):
+// tangent K and internal force from the static element (SetElemMatFin_StVenant3D; its row with density 0 gives
+// -f_in exactly), mass rho vol / 20, element matrix M + beta dt^2 K, residual vol rho g / 4 - f_in - M a term by
+// term, and when is_initial - (dt K v + dt^2/2 K a) term by term.
+__device__ inline void row_stvk3d_newmark(int a, const double co[4][3], const double u[4][3], const double vv[4][3],
+                                          const double aa[4][3], const double* p, double dt, double beta, bool initial,
+                                          double* em, double* er) {
+  const double p0[6] = {p[0], p[1], 0.0, p[3], p[4], p[5]};
+  double K[4 * 9], fin[3];
+  row_stvk3d(a, co, u, p0, K, fin);  // fin = -f_in
+  const double rho = p[2];
+  const double vol = tet_volume(co);
+  const double mm = rho * vol * 0.05;
+  double M[4][9];
+  for (int j = 0; j < 4; j++)
+    for (int k = 0; k < 9; k++) M[j][k] = (k % 4 == 0) ? mm : 0.0;
+  for (int r = 0; r < 3; r++) M[a][r * 3 + r] += mm;
+  for (int j = 0; j < 4; j++)
+    for (int k = 0; k < 9; k++) em[j * 9 + k] = M[j][k] + beta * dt * dt * K[j * 9 + k];
+  for (int r = 0; r < 3; r++) {
+    double x = vol * rho * p[3 + r] * 0.25 + fin[r];
+    for (int j = 0; j < 4; j++)
+      for (int c = 0; c < 3; c++) x -= M[j][r * 3 + c] * aa[j][c];
+    if (initial)
+      for (int j = 0; j < 4; j++)
+        for (int c = 0; c < 3; c++)
+          x -= dt * K[j * 9 + r * 3 + c] * vv[j][c] + 0.5 * dt * dt * K[j * 9 + r * 3 + c] * aa[j][c];
+    er[r] = x;
+  }
+}
+
 // Backward-Euler row (AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_linear_solid2d.cpp:
 // 1338-1497; unknown: the velocity increment): the Newmark loop's K and M, element matrix M + dt^2 K, residual
 // dt f - dt K u, then - dt^2 K v when is_initial, else - M (v - v_prev). aa holds v_prev here.
@@ -505,7 +536,8 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
     int a = 0;
     for (int c = 0; c < NNO; c++)
       if (no[c] == i1) a = c;
-    constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER);
+    constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER ||
+                         KIND == DFM_STVK3D_NEWMARK);
     double co[4][3], u[4][3], val[4], vval[4];
     double vv[4][3], aa[4][3];
     for (int c = 0; c < NNO; c++) {
@@ -533,6 +565,8 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
       if (KIND == DFM_STVK3D) row_stvk3d(a, co, u, p, em, er);
       if (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK)
         row_solid_newmark<NDIM>(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+      if (KIND == DFM_STVK3D_NEWMARK)
+        row_stvk3d_newmark(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
       if (KIND == DFM_SOLID2D_BEULER) row_solid2d_beuler(a, co, u, vv, aa, p, A.glob0, A.glob1 != 0.0, em, er);
     }
     for (int c = 0; c < NNO; c++) {
@@ -585,6 +619,7 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
   DFM_ASM(DFM_SOLID2D_NEWMARK, 2, 2)
   DFM_ASM(DFM_SOLID3D_NEWMARK, 3, 3)
   DFM_ASM(DFM_SOLID2D_BEULER, 2, 2)
+  DFM_ASM(DFM_STVK3D_NEWMARK, 3, 3)
 #undef DFM_ASM
   return hipErrorInvalidValue;
 }

@@ -50,7 +50,11 @@ enum {
   /* backward-Euler dynamic 2D elasticity (unknown: the velocity increment; matrix M + dt^2 K):
    * AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_linear_solid2d.cpp:1338-1497,
    * (lambda, myu, rho, gx, gy), glob = (dt, is_initial); the previous velocity from dfm_sys_set_velocity_prev */
-  DFM_SOLID2D_BEULER = 8
+  DFM_SOLID2D_BEULER = 8,
+  /* Newmark-beta dynamic St.Venant-Kirchhoff 3D (the Newton step of CEqn_Solid3D_Linear, geometrically nonlinear,
+   * not stationary): AddLinSys_StVenant3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:1109-1242,
+   * (lambda, myu, rho, gx, gy, gz), glob = (dt, gamma, beta, is_initial) */
+  DFM_STVK3D_NEWMARK = 9
 };
 
 /* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering); the last three are COrdering_Blk's, exactly

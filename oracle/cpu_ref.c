@@ -1841,3 +1841,60 @@ void cr_update_field_beuler(long m, double dt, int is_initial, const double* upd
   if (is_initial) { for (k = 0; k < m; k++) u[k] += dt * v[k]; }
   else { for (k = 0; k < m; k++) u[k] += dt * upd[k]; }
 }
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* AddLinSys_StVenant3D_NonStatic_NewmarkBeta_P1 (src/femeqn/eqn_st_venant.cpp:1109-1242): the tangent and */
+/* internal force of SetElemMatFin_StVenant3D (the static element restated above, with density 0 its     */
+/* residual is -f_in exactly), the mass, and the residual terms in the loop's order.                     */
+int cr_assemble_stvk3d_newmark(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                               int pstride, const double* glob, const double* u, const double* v, const double* a,
+                               const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res) {
+  (void)nnode;
+  const double dt = glob[0], beta = glob[2];
+  const int is_initial = glob[3] != 0.0;
+  double eK[4][4][3][3], eM[4][4][3][3], emat[4][4][3][3], fin[4][3], eres[4][3];
+  double co[4][3], disp[4][3], velo[4][3], acc[4][3];
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * 4];
+    const double* p = &params[e * pstride];
+    const double rho = p[2];
+    const double g[3] = {p[3], p[4], p[5]};
+    const double p0[6] = {p[0], p[1], 0.0, p[3], p[4], p[5]};
+    for (int k = 0; k < 4; k++)
+      for (int d = 0; d < 3; d++) {
+        co[k][d] = coords[(long)no[k] * 3 + d];
+        disp[k][d] = u[(long)no[k] * 3 + d];
+        velo[k][d] = v[(long)no[k] * 3 + d];
+        acc[k][d] = a[(long)no[k] * 3 + d];
+      }
+    elem_stvk3d(co, disp, p0, &eK[0][0][0][0], &fin[0][0]);
+    const double vol = tet_volume(co[0], co[1], co[2], co[3]);
+    for (int i = 0; i < 144; i++) (&eM[0][0][0][0])[i] = 0.0;
+    {
+      const double tmp1 = rho * vol * 0.05;
+      for (int ino = 0; ino < 4; ino++)
+        for (int jno = 0; jno < 4; jno++)
+          for (int d = 0; d < 3; d++) eM[ino][jno][d][d] += tmp1;
+      for (int ino = 0; ino < 4; ino++)
+        for (int d = 0; d < 3; d++) eM[ino][ino][d][d] += tmp1;
+    }
+    for (int i = 0; i < 144; i++) (&emat[0][0][0][0])[i] = (&eM[0][0][0][0])[i] + beta * dt * dt * (&eK[0][0][0][0])[i];
+    for (int ino = 0; ino < 4; ino++)
+      for (int idim = 0; idim < 3; idim++) {
+        eres[ino][idim] = vol * rho * g[idim] * 0.25 + fin[ino][idim];
+        for (int jno = 0; jno < 4; jno++)
+          for (int jdim = 0; jdim < 3; jdim++) eres[ino][idim] -= eM[ino][jno][idim][jdim] * acc[jno][jdim];
+      }
+    if (is_initial)
+      for (int ino = 0; ino < 4; ino++)
+        for (int idim = 0; idim < 3; idim++)
+          for (int jno = 0; jno < 4; jno++)
+            for (int jdim = 0; jdim < 3; jdim++)
+              eres[ino][idim] -= dt * eK[ino][jno][idim][jdim] * velo[jno][jdim] +
+                                 0.5 * dt * dt * eK[ino][jno][idim][jdim] * acc[jno][jdim];
+    merge(4, no, 9, &emat[0][0][0][0], rowptr, colind, crs, dia);
+    for (int k = 0; k < 4; k++)
+      for (int d = 0; d < 3; d++) res[(long)no[k] * 3 + d] += eres[k][d];
+  }
+  return 0;
+}

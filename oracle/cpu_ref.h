@@ -128,6 +128,11 @@ int cr_assemble_beuler(int nnode, const double* coords, int nelem, const int* ln
                        double* dia, double* res);
 void cr_update_field_beuler(long m, double dt, int is_initial, const double* upd, double* u, double* v);
 
+/* Newmark-beta dynamic StVK 3D element loop; glob = (dt, gamma, beta, is_initial). */
+int cr_assemble_stvk3d_newmark(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                               int pstride, const double* glob, const double* u, const double* v, const double* a,
+                               const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -410,3 +410,22 @@ def test_beuler_step_is_implicit_euler():
     assert np.allclose(u1, u + dt * v1, rtol=0, atol=1e-15)
     lhs, rhs = Mm @ (v1 - v), dt * (f - K @ u1)
     assert np.allclose(lhs, rhs, rtol=0, atol=1e-12 * np.abs(rhs).max())
+
+
+def test_stvk3d_newmark_small_strain_limit():
+    """At vanishing displacement the dynamic StVK loop is the linear Newmark loop: same matrix, same residual (to
+    rounding; the tangent and the linear stiffness are written differently)."""
+    co, ln = M.box_tet(2, 2, 2, 1.0, 0.8, 0.6, 5)
+    n = co.size // 3
+    par = [0.6, 0.4, 1.2, 0.0, 0.1, -0.2]
+    rng = np.random.default_rng(2)
+    z = np.zeros(3 * n)
+    v, a = rng.uniform(-0.1, 0.1, 3 * n), rng.uniform(-0.1, 0.1, 3 * n)
+    glob = (0.05, 0.5, 0.25, 1.0)
+    nl = P.CpuSystem(3, co, ln, 3)
+    nl.assemble_stvk3d_newmark(par, z, v, a, glob)
+    li = P.CpuSystem(3, co, ln, 3)
+    li.assemble_newmark(par, z, v, a, glob)
+    A, B = dense(nl), dense(li)
+    assert np.allclose(A, B, rtol=0, atol=1e-13 * np.abs(B).max())
+    assert np.allclose(nl.res, li.res, rtol=0, atol=1e-13 * np.abs(li.res).max())

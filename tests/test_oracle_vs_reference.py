@@ -541,3 +541,30 @@ def test_beuler_update_bit_exact(initial):
     gu, gv = P.CpuSystem.update_field_beuler(0.03, initial, x, u, v)
     ru, rv, _ = ref.get_fields(acc=False)
     assert np.array_equal(ru, gu) and np.array_equal(rv, gv)
+
+
+@pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
+@pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
+def test_stvk3d_newmark_assembly_bit_exact(per_element, initial):
+    """AddLinSys_StVenant3D_NonStatic_NewmarkBeta: matrix and residual element for element."""
+    co, ln = mesh3d(2, 2, 1) if per_element else mesh3d(3, 3, 2)
+    n, nelem = co.size // 3, ln.size // 4
+    rng = np.random.default_rng(13)
+    pf = lambda r: [LAM3 * r.uniform(0.5, 2), MU3, r.uniform(0.5, 2), 0.0, 0.01, -0.05]
+    params = np.array([pf(rng) for _ in range(nelem)]) if per_element else np.array(pf(rng))
+    groups = np.arange(nelem, dtype=np.int32) if per_element else None
+    ref = P.RefSystem(3, co, ln, 3, groups, dynamic=True)
+    cpu = P.CpuSystem(3, co, ln, 3)
+    u, v, a = (_rand_u(n, 3, s, 0.05) for s in (1, 2, 3))
+    glob = (0.03, 0.6, 0.3025, 1.0 if initial else 0.0)
+    ref.set_field(u, v)
+    ref.set_acc(a)
+    flags = clamp_flags(co, 3, 3)
+    ref.set_bc(flags)
+    rn = ref.assemble(P.STVK3D_NEWMARK, params if per_element else params[None, :], glob)
+    cpu.assemble_stvk3d_newmark(params, u, v, a, glob)
+    rn2 = cpu.set_bc(flags)
+    crs, dia = ref.matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    assert np.array_equal(ref.vector(-1), cpu.res), np.abs(ref.vector(-1) - cpu.res).max()
+    assert rn == rn2
