@@ -477,6 +477,35 @@ __device__ inline void row_stvk3d_newmark(int a, const double co[4][3], const do
   }
 }
 
+// StVK Newmark-beta row in 2D (AddLinSys_StVenant2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:
+// 314-462): as in 3D, with mass rho area / 12 and the external force area rho g / 3 added to zero.
+__device__ inline void row_stvk2d_newmark(int a, const double co[4][3], const double u[4][3], const double vv[4][3],
+                                          const double aa[4][3], const double* p, double dt, double beta, bool initial,
+                                          double* em, double* er) {
+  const double p0[5] = {p[0], p[1], 0.0, p[3], p[4]};
+  double K[3 * 4], fin[2];
+  row_stvk2d(a, co, u, p0, K, fin);  // fin = -f_in
+  const double rho = p[2];
+  const double area = tri_area(co[0], co[1], co[2]);
+  const double mm = rho * area / 12.0;
+  double M[3][4];
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 4; k++) M[j][k] = (k % 3 == 0) ? mm : 0.0;
+  for (int r = 0; r < 2; r++) M[a][r * 2 + r] += mm;
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 4; k++) em[j * 4 + k] = M[j][k] + beta * dt * dt * K[j * 4 + k];
+  for (int r = 0; r < 2; r++) {
+    double x = (0.0 + area * rho * p[3 + r] / 3.0) + fin[r];
+    for (int j = 0; j < 3; j++)
+      for (int c = 0; c < 2; c++) x -= M[j][r * 2 + c] * aa[j][c];
+    if (initial)
+      for (int j = 0; j < 3; j++)
+        for (int c = 0; c < 2; c++)
+          x -= dt * K[j * 4 + r * 2 + c] * vv[j][c] + 0.5 * dt * dt * K[j * 4 + r * 2 + c] * aa[j][c];
+    er[r] = x;
+  }
+}
+
 // Backward-Euler row (AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_linear_solid2d.cpp:
 // 1338-1497; unknown: the velocity increment): the Newmark loop's K and M, element matrix M + dt^2 K, residual
 // dt f - dt K u, then - dt^2 K v when is_initial, else - M (v - v_prev). aa holds v_prev here.
@@ -537,7 +566,7 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
     for (int c = 0; c < NNO; c++)
       if (no[c] == i1) a = c;
     constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER ||
-                         KIND == DFM_STVK3D_NEWMARK);
+                         KIND == DFM_STVK3D_NEWMARK || KIND == DFM_STVK2D_NEWMARK);
     double co[4][3], u[4][3], val[4], vval[4];
     double vv[4][3], aa[4][3];
     for (int c = 0; c < NNO; c++) {
@@ -567,6 +596,8 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
         row_solid_newmark<NDIM>(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
       if (KIND == DFM_STVK3D_NEWMARK)
         row_stvk3d_newmark(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+      if (KIND == DFM_STVK2D_NEWMARK)
+        row_stvk2d_newmark(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
       if (KIND == DFM_SOLID2D_BEULER) row_solid2d_beuler(a, co, u, vv, aa, p, A.glob0, A.glob1 != 0.0, em, er);
     }
     for (int c = 0; c < NNO; c++) {
@@ -620,6 +651,7 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
   DFM_ASM(DFM_SOLID3D_NEWMARK, 3, 3)
   DFM_ASM(DFM_SOLID2D_BEULER, 2, 2)
   DFM_ASM(DFM_STVK3D_NEWMARK, 3, 3)
+  DFM_ASM(DFM_STVK2D_NEWMARK, 2, 2)
 #undef DFM_ASM
   return hipErrorInvalidValue;
 }

@@ -54,7 +54,10 @@ enum {
   /* Newmark-beta dynamic St.Venant-Kirchhoff 3D (the Newton step of CEqn_Solid3D_Linear, geometrically nonlinear,
    * not stationary): AddLinSys_StVenant3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:1109-1242,
    * (lambda, myu, rho, gx, gy, gz), glob = (dt, gamma, beta, is_initial) */
-  DFM_STVK3D_NEWMARK = 9
+  DFM_STVK3D_NEWMARK = 9,
+  /* its 2D counterpart: AddLinSys_StVenant2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:314-460,
+   * (lambda, myu, rho, gx, gy), glob = (dt, gamma, beta, is_initial) */
+  DFM_STVK2D_NEWMARK = 10
 };
 
 /* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering); the last three are COrdering_Blk's, exactly

@@ -1898,3 +1898,64 @@ int cr_assemble_stvk3d_newmark(int nnode, const double* coords, int nelem, const
   }
   return 0;
 }
+
+/* AddLinSys_StVenant2D_NonStatic_NewmarkBeta_P1 (src/femeqn/eqn_st_venant.cpp:314-460), as the 3D loop above with
+ * mass rho area / 12 and external force area rho g / 3 accumulated from zero. */
+int cr_assemble_stvk2d_newmark(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                               int pstride, const double* glob, const double* u, const double* v, const double* a,
+                               const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res) {
+  (void)nnode;
+  const double dt = glob[0], beta = glob[2];
+  const int is_initial = glob[3] != 0.0;
+  double eK[3][3][2][2], eM[3][3][2][2], emat[3][3][2][2], fin[3][2], fex[3][2], eres[3][2];
+  double co[4][3], disp[4][3], velo[3][2], acc[3][2];
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * 3];
+    const double* p = &params[e * pstride];
+    const double rho = p[2];
+    const double g[2] = {p[3], p[4]};
+    const double p0[5] = {p[0], p[1], 0.0, p[3], p[4]};
+    for (int k = 0; k < 3; k++) {
+      co[k][2] = 0.0;
+      disp[k][2] = 0.0;
+      for (int d = 0; d < 2; d++) {
+        co[k][d] = coords[(long)no[k] * 2 + d];
+        disp[k][d] = u[(long)no[k] * 2 + d];
+        velo[k][d] = v[(long)no[k] * 2 + d];
+        acc[k][d] = a[(long)no[k] * 2 + d];
+      }
+    }
+    elem_stvk2d(co, disp, p0, &eK[0][0][0][0], &fin[0][0]);
+    const double area = tri_area(co[0], co[1], co[2]);
+    for (int i = 0; i < 36; i++) (&eM[0][0][0][0])[i] = 0.0;
+    for (int i = 0; i < 6; i++) (&fex[0][0])[i] = 0.0;
+    {
+      const double tmp1 = rho * area / 12.0;
+      for (int ino = 0; ino < 3; ino++) {
+        for (int jno = 0; jno < 3; jno++)
+          for (int d = 0; d < 2; d++) eM[ino][jno][d][d] += tmp1;
+        for (int d = 0; d < 2; d++) eM[ino][ino][d][d] += tmp1;
+      }
+    }
+    for (int ino = 0; ino < 3; ino++)
+      for (int d = 0; d < 2; d++) fex[ino][d] += area * rho * g[d] / (double)3;
+    for (int i = 0; i < 36; i++) (&emat[0][0][0][0])[i] = (&eM[0][0][0][0])[i] + beta * dt * dt * (&eK[0][0][0][0])[i];
+    for (int ino = 0; ino < 3; ino++)
+      for (int idim = 0; idim < 2; idim++) {
+        eres[ino][idim] = fex[ino][idim] + fin[ino][idim];
+        for (int jno = 0; jno < 3; jno++)
+          for (int jdim = 0; jdim < 2; jdim++) eres[ino][idim] -= eM[ino][jno][idim][jdim] * acc[jno][jdim];
+      }
+    if (is_initial)
+      for (int ino = 0; ino < 3; ino++)
+        for (int idim = 0; idim < 2; idim++)
+          for (int jno = 0; jno < 3; jno++)
+            for (int jdim = 0; jdim < 2; jdim++)
+              eres[ino][idim] -= dt * eK[ino][jno][idim][jdim] * velo[jno][jdim] +
+                                 0.5 * dt * dt * eK[ino][jno][idim][jdim] * acc[jno][jdim];
+    merge(3, no, 4, &emat[0][0][0][0], rowptr, colind, crs, dia);
+    for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 2; d++) res[(long)no[k] * 2 + d] += eres[k][d];
+  }
+  return 0;
+}

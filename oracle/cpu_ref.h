@@ -132,6 +132,9 @@ void cr_update_field_beuler(long m, double dt, int is_initial, const double* upd
 int cr_assemble_stvk3d_newmark(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
                                int pstride, const double* glob, const double* u, const double* v, const double* a,
                                const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res);
+int cr_assemble_stvk2d_newmark(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                               int pstride, const double* glob, const double* u, const double* v, const double* a,
+                               const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res);
 
 #ifdef __cplusplus
 }

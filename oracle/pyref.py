@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CPU_REF_SO = os.path.join(HERE, "libcpu_ref.so")
 REF_SO = os.path.join(HERE, "_ref", "libdfm_ref.so")
 
-POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER, STVK3D_NEWMARK = range(10)
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER, STVK3D_NEWMARK, STVK2D_NEWMARK = range(11)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -161,12 +161,13 @@ class CpuSystem:
         assert rc == 0
 
     def assemble_stvk3d_newmark(self, params, u, v, a, glob):
-        """Newmark-beta StVK 3D element loop (restated) into self.crs / self.dia / self.res (zero them first)."""
+        """Newmark-beta StVK element loop (restated; 2D or 3D by the mesh) into self.crs / self.dia / self.res (zero them first)."""
         params = _f64(params)
         pstride = 0 if params.ndim == 1 else params.shape[1]
         u, v, a, glob = _f64(u), _f64(v), _f64(a), _f64(glob)
         p = lambda x: x.ctypes.data_as(C.c_void_p)
-        rc = cpu_ref().cr_assemble_stvk3d_newmark(self.nnode, p(self.coords), self.nelem, p(self.lnods), p(params),
+        fn = cpu_ref().cr_assemble_stvk3d_newmark if self.ndim == 3 else cpu_ref().cr_assemble_stvk2d_newmark
+        rc = fn(self.nnode, p(self.coords), self.nelem, p(self.lnods), p(params),
                                                   pstride, p(glob), p(u), p(v), p(a), p(self.rowptr), p(self.colind),
                                                   p(self.crs), p(self.dia), p(self.res))
         assert rc == 0

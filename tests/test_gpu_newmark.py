@@ -214,31 +214,32 @@ def test_beuler_assembly_and_update_bit_exact(ctx, oname, ordering, initial):
     assert np.array_equal(gu, wu) and np.array_equal(gv, wv)
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
 @pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
-def test_stvk3d_newmark_assembly_bit_exact(ctx, oname, ordering, initial):
-    """Dynamic StVK 3D (the Newton step of the geometrically nonlinear, non-stationary solid)."""
-    print(f"stvk3d newmark {oname} initial={initial}", flush=True)
-    co, ln = mesh(3, True)
-    n = co.size // 3
-    nelem = ln.size // 4
+def test_stvk3d_newmark_assembly_bit_exact(ctx, ndim, oname, ordering, initial):
+    """Dynamic StVK (the Newton step of the geometrically nonlinear, non-stationary solids), 2D and 3D."""
+    print(f"stvk newmark {ndim}d {oname} initial={initial}", flush=True)
+    co, ln = mesh(ndim, True)
+    n = co.size // ndim
+    nelem = ln.size // (ndim + 1)
     r = np.random.default_rng(19)
     params = np.stack([LAM3 * r.uniform(0.5, 2, nelem), np.full(nelem, MU3), r.uniform(0.5, 2, nelem),
-                       np.zeros(nelem), np.full(nelem, 0.01), np.full(nelem, -0.05)], 1)
-    u, v, a = (r.uniform(-0.05, 0.05, 3 * n) for _ in range(3))
-    fl = flags(co, 3, 3)
+                       np.zeros(nelem), np.full(nelem, 0.01), np.full(nelem, -0.05)][: 3 + ndim], 1)
+    u, v, a = (r.uniform(-0.05, 0.05, ndim * n) for _ in range(3))
+    fl = flags(co, ndim, ndim)
     glob = (0.03, 0.6, 0.3025, 1.0 if initial else 0.0)
-    cpu = P.CpuSystem(3, co, ln, 3)
+    cpu = P.CpuSystem(ndim, co, ln, ndim)
     cpu.assemble_stvk3d_newmark(params, u, v, a, glob)
     cpu.set_bc(fl)
-    gs = N.System(ctx, 3, co, ln, 3)
+    gs = N.System(ctx, ndim, co, ln, ndim)
     if ordering != 0:
         order, _ = gs.make_ordering(ordering)
         gs.set_ordering(order)
     gs.set_bc(fl)
     gs.set_field(u, v)
     gs.set_acceleration(a)
-    gs.assemble(9, params, glob=glob)
+    gs.assemble(9 if ndim == 3 else 10, params, glob=glob)
     crs, dia = gs.get_matrix()
     assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
     assert np.array_equal(gs.get_vector(-1), cpu.res), np.abs(gs.get_vector(-1) - cpu.res).max()

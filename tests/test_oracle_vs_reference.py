@@ -543,25 +543,30 @@ def test_beuler_update_bit_exact(initial):
     assert np.array_equal(ru, gu) and np.array_equal(rv, gv)
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
 @pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
-def test_stvk3d_newmark_assembly_bit_exact(per_element, initial):
-    """AddLinSys_StVenant3D_NonStatic_NewmarkBeta: matrix and residual element for element."""
-    co, ln = mesh3d(2, 2, 1) if per_element else mesh3d(3, 3, 2)
-    n, nelem = co.size // 3, ln.size // 4
+def test_stvk3d_newmark_assembly_bit_exact(ndim, per_element, initial):
+    """AddLinSys_StVenant2D/3D_NonStatic_NewmarkBeta: matrix and residual element for element."""
+    if ndim == 3:
+        co, ln = mesh3d(2, 2, 1) if per_element else mesh3d(3, 3, 2)
+    else:
+        co, ln = mesh2d(4, 2) if per_element else mesh2d(9, 5)
+    n, nelem = co.size // ndim, ln.size // (ndim + 1)
     rng = np.random.default_rng(13)
-    pf = lambda r: [LAM3 * r.uniform(0.5, 2), MU3, r.uniform(0.5, 2), 0.0, 0.01, -0.05]
+    pf = lambda r: [LAM3 * r.uniform(0.5, 2), MU3, r.uniform(0.5, 2), 0.0, 0.01, -0.05][: 3 + ndim]
     params = np.array([pf(rng) for _ in range(nelem)]) if per_element else np.array(pf(rng))
     groups = np.arange(nelem, dtype=np.int32) if per_element else None
-    ref = P.RefSystem(3, co, ln, 3, groups, dynamic=True)
-    cpu = P.CpuSystem(3, co, ln, 3)
-    u, v, a = (_rand_u(n, 3, s, 0.05) for s in (1, 2, 3))
+    ref = P.RefSystem(ndim, co, ln, ndim, groups, dynamic=True)
+    cpu = P.CpuSystem(ndim, co, ln, ndim)
+    u, v, a = (_rand_u(n, ndim, s, 0.05) for s in (1, 2, 3))
     glob = (0.03, 0.6, 0.3025, 1.0 if initial else 0.0)
     ref.set_field(u, v)
     ref.set_acc(a)
-    flags = clamp_flags(co, 3, 3)
+    flags = clamp_flags(co, ndim, ndim)
     ref.set_bc(flags)
-    rn = ref.assemble(P.STVK3D_NEWMARK, params if per_element else params[None, :], glob)
+    kind = P.STVK3D_NEWMARK if ndim == 3 else P.STVK2D_NEWMARK
+    rn = ref.assemble(kind, params if per_element else params[None, :], glob)
     cpu.assemble_stvk3d_newmark(params, u, v, a, glob)
     rn2 = cpu.set_bc(flags)
     crs, dia = ref.matrix()
