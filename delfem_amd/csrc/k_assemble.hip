@@ -478,7 +478,7 @@ __device__ inline void row_stvk3d_newmark(int a, const double co[4][3], const do
 }
 
 // StVK Newmark-beta row in 2D (AddLinSys_StVenant2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:
-// 314-462): as in 3D, with mass rho area / 12 and the external force area rho g / 3 added to zero.
+// 314-460): as in 3D, with mass rho area / 12 and the external force area rho g / 3 added to zero.
 __device__ inline void row_stvk2d_newmark(int a, const double co[4][3], const double u[4][3], const double vv[4][3],
                                           const double aa[4][3], const double* p, double dt, double beta, bool initial,
                                           double* em, double* er) {

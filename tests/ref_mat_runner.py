@@ -2,7 +2,8 @@
 the result as JSON: those loops create their temporaries with `new double[]` and pass them to
 CMatDia_BlkCrs::MatVec with beta = 0, which multiplies the uninitialised values by 0 (src/matvec/
 solver_mat_iter.cpp:219-231 with src/matvec/matdia_blkcrs.cpp:940), so a heap that held a NaN makes them
-diverge. A fresh process hands out zeroed pages, which makes the reference's result reproducible.
+diverge. The caller runs it in a fresh process with MALLOC_PERTURB_=255, under which glibc hands out every
+allocation filled with zero bytes, which makes the reference's result reproducible.
 Usage: python ref_mat_runner.py ndim length nx ny [nz] method tol max_iter"""
 import json
 import os

@@ -292,8 +292,11 @@ def _ref_mat(ndim, length, nxyz, method, tol, maxit):
     import sys
 
     runner = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_mat_runner.py")
+    # glibc's MALLOC_PERTURB_=255 fills every allocation with zero bytes (perturb byte ^ 0xff), so the
+    # uninitialised temporaries read as 0.0 whatever the heap held before
+    env = dict(os.environ, MALLOC_PERTURB_="255")
     out = subprocess.run([sys.executable, runner, str(ndim), str(length), *map(str, nxyz), method, repr(tol),
-                          str(maxit)], capture_output=True, text=True, check=True).stdout
+                          str(maxit)], capture_output=True, text=True, check=True, env=env).stdout
     d = json.loads(out.strip().splitlines()[-1])
     return d["ok"], d["conv"], d["it"], np.array(d["x"]), np.array(d["r"])
 
