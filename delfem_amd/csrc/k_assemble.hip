@@ -410,40 +410,47 @@ __device__ inline void row_solid_newmark(int a, const double co[4][3], const dou
     w = tet_volume(co);
     tet_dldx(d, co);
   }
-  double K[NNO][BS], M[NNO][BS];
+  // K is formed in em (the element matrix replaces it in place once the residual has used it); M is diagonal per
+  // block and is written out as a function, zeros included, so that every product of the loops is kept
+  double* K = em;
   for (int j = 0; j < NNO; j++) {
     double t = 0.0;
     for (int r = 0; r < NDIM; r++) {
       // 0.0 + x as the zero-initialised element matrix accumulates it (a -0.0 term becomes +0.0)
-      for (int c = 0; c < NDIM; c++) K[j][r * NDIM + c] = 0.0 + w * (lambda * d[a][r] * d[j][c] + myu * d[j][r] * d[a][c]);
+      for (int c = 0; c < NDIM; c++)
+        K[j * BS + r * NDIM + c] = 0.0 + w * (lambda * d[a][r] * d[j][c] + myu * d[j][r] * d[a][c]);
       t += d[a][r] * d[j][r];
     }
-    for (int r = 0; r < NDIM; r++) K[j][r * NDIM + r] += w * myu * t;
+    for (int r = 0; r < NDIM; r++) K[j * BS + r * NDIM + r] += w * myu * t;
   }
   const double mm = (NDIM == 2) ? w * rho * 0.0833333333333333333333333 : w * rho * 0.05;
-  for (int j = 0; j < NNO; j++)
-    for (int k = 0; k < BS; k++) M[j][k] = (k % (NDIM + 1) == 0) ? mm : 0.0;
-  for (int r = 0; r < NDIM; r++) M[a][r * NDIM + r] += mm;
-  const double bdt2 = beta * dt * dt;
-  for (int j = 0; j < NNO; j++)
-    for (int k = 0; k < BS; k++) em[j * BS + k] = M[j][k] + bdt2 * K[j][k];
-  for (int r = 0; r < NDIM; r++)
-    er[r] = (NDIM == 2) ? w * rho * p[3 + r] * 0.33333333333333333333333333 : w * rho * p[3 + r] * 0.25;
-  auto row_dot = [&](const double* blk, int r, const double x[3]) {
-    double t = blk[r * NDIM] * x[0];
-    for (int c = 1; c < NDIM; c++) t = t + blk[r * NDIM + c] * x[c];
+  const double mm2 = mm + mm;
+  auto Mv = [&](int j, int r, int c) { return c != r ? 0.0 : (j == a ? mm2 : mm); };
+  auto kdot = [&](int j, int r, const double x[3]) {
+    double t = K[j * BS + r * NDIM] * x[0];
+    for (int c = 1; c < NDIM; c++) t = t + K[j * BS + r * NDIM + c] * x[c];
     return t;
   };
+  auto mdot = [&](int j, int r, const double x[3]) {
+    double t = Mv(j, r, 0) * x[0];
+    for (int c = 1; c < NDIM; c++) t = t + Mv(j, r, c) * x[c];
+    return t;
+  };
+  for (int r = 0; r < NDIM; r++)
+    er[r] = (NDIM == 2) ? w * rho * p[3 + r] * 0.33333333333333333333333333 : w * rho * p[3 + r] * 0.25;
   for (int j = 0; j < NNO; j++)
-    for (int r = 0; r < NDIM; r++) er[r] -= row_dot(K[j], r, u[j]);
+    for (int r = 0; r < NDIM; r++) er[r] -= kdot(j, r, u[j]);
   for (int j = 0; j < NNO; j++)
-    for (int r = 0; r < NDIM; r++) er[r] -= row_dot(M[j], r, aa[j]);
+    for (int r = 0; r < NDIM; r++) er[r] -= mdot(j, r, aa[j]);
   if (NDIM == 3 || initial) {
     for (int j = 0; j < NNO; j++)
-      for (int r = 0; r < NDIM; r++) er[r] -= dt * row_dot(K[j], r, vv[j]);
+      for (int r = 0; r < NDIM; r++) er[r] -= dt * kdot(j, r, vv[j]);
     for (int j = 0; j < NNO; j++)
-      for (int r = 0; r < NDIM; r++) er[r] -= 0.5 * dt * dt * row_dot(K[j], r, aa[j]);
+      for (int r = 0; r < NDIM; r++) er[r] -= 0.5 * dt * dt * kdot(j, r, aa[j]);
   }
+  const double bdt2 = beta * dt * dt;
+  for (int j = 0; j < NNO; j++)
+    for (int k = 0; k < BS; k++) em[j * BS + k] = Mv(j, k / NDIM, k % NDIM) + bdt2 * K[j * BS + k];
 }
 
 // StVK Newmark-beta row (AddLinSys_StVenant3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:1109-1242):
@@ -454,27 +461,26 @@ __device__ inline void row_stvk3d_newmark(int a, const double co[4][3], const do
                                           const double aa[4][3], const double* p, double dt, double beta, bool initial,
                                           double* em, double* er) {
   const double p0[6] = {p[0], p[1], 0.0, p[3], p[4], p[5]};
-  double K[4 * 9], fin[3];
+  double fin[3];
+  double* K = em;  // the tangent, replaced in place by M + beta dt^2 K once the residual has used it
   row_stvk3d(a, co, u, p0, K, fin);  // fin = -f_in
   const double rho = p[2];
   const double vol = tet_volume(co);
   const double mm = rho * vol * 0.05;
-  double M[4][9];
-  for (int j = 0; j < 4; j++)
-    for (int k = 0; k < 9; k++) M[j][k] = (k % 4 == 0) ? mm : 0.0;
-  for (int r = 0; r < 3; r++) M[a][r * 3 + r] += mm;
-  for (int j = 0; j < 4; j++)
-    for (int k = 0; k < 9; k++) em[j * 9 + k] = M[j][k] + beta * dt * dt * K[j * 9 + k];
+  const double mm2 = mm + mm;
+  auto Mv = [&](int j, int r, int c) { return c != r ? 0.0 : (j == a ? mm2 : mm); };
   for (int r = 0; r < 3; r++) {
     double x = vol * rho * p[3 + r] * 0.25 + fin[r];
     for (int j = 0; j < 4; j++)
-      for (int c = 0; c < 3; c++) x -= M[j][r * 3 + c] * aa[j][c];
+      for (int c = 0; c < 3; c++) x -= Mv(j, r, c) * aa[j][c];
     if (initial)
       for (int j = 0; j < 4; j++)
         for (int c = 0; c < 3; c++)
           x -= dt * K[j * 9 + r * 3 + c] * vv[j][c] + 0.5 * dt * dt * K[j * 9 + r * 3 + c] * aa[j][c];
     er[r] = x;
   }
+  for (int j = 0; j < 4; j++)
+    for (int k = 0; k < 9; k++) em[j * 9 + k] = Mv(j, k / 3, k % 3) + beta * dt * dt * K[j * 9 + k];
 }
 
 // StVK Newmark-beta row in 2D (AddLinSys_StVenant2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:
