@@ -11,11 +11,11 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdfm_gpu.so")
 
-POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER, STVK3D_NEWMARK, STVK2D_NEWMARK = range(11)
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER, STVK3D_NEWMARK, STVK2D_NEWMARK, STVK2D_BEULER = range(12)
 ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKCOLOR, ORDER_RCM, ORDER_RCM2, ORDER_AMD = range(6)
 FP64, MIXED, FP32_ILU, MIXED_AUTO = 0, 1, 2, 3
 ERR_SINGULAR = -5
-NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3, SOLID2D_NEWMARK: 5, SOLID3D_NEWMARK: 6, SOLID2D_BEULER: 5, STVK3D_NEWMARK: 6, STVK2D_NEWMARK: 5}
+NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3, SOLID2D_NEWMARK: 5, SOLID3D_NEWMARK: 6, SOLID2D_BEULER: 5, STVK3D_NEWMARK: 6, STVK2D_NEWMARK: 5, STVK2D_BEULER: 5}
 
 _lib = None
 _live = weakref.WeakSet()  # systems and contexts still open, closed in order at interpreter exit

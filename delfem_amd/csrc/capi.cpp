@@ -723,9 +723,9 @@ int dfm_sys_set_load_scale(dfm_sys sys, double scale) {
 }
 
 static int check_kind(const Sys& s, int kind) {
-  static const int need_npar[11] = {2, 5, 6, 5, 6, 3, 5, 6, 5, 6, 5};
-  static const int need_len[11] = {1, 2, 3, 2, 3, 1, 2, 3, 2, 3, 2};
-  DFM_CHECK(kind >= 0 && kind < 11, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
+  static const int need_npar[12] = {2, 5, 6, 5, 6, 3, 5, 6, 5, 6, 5, 5};
+  static const int need_len[12] = {1, 2, 3, 2, 3, 1, 2, 3, 2, 3, 2, 2};
+  DFM_CHECK(kind >= 0 && kind < 12, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
   DFM_CHECK(s.d_par.p != nullptr, DFM_ERR_STATE, "dfm_sys_assemble: no element parameters set");
   DFM_CHECK(s.npar >= need_npar[kind], DFM_ERR_ARG, "dfm_sys_assemble: too few parameters for the element kind");
   DFM_CHECK(need_len[kind] == s.len, DFM_ERR_ARG, "dfm_sys_assemble: element kind does not match dofs per node");
@@ -758,7 +758,7 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
     rc = ensure_acc(s);
     if (rc != DFM_OK) return rc;
   }
-  if (kind == DFM_SOLID2D_BEULER) {
+  if (kind == DFM_SOLID2D_BEULER || kind == DFM_STVK2D_BEULER) {
     DFM_CHECK(glob != nullptr, DFM_ERR_ARG, "dfm_sys_assemble: the backward-Euler kind needs glob = (dt, is_initial)");
     rc = ensure_v0(s);
     if (rc != DFM_OK) return rc;
@@ -778,7 +778,7 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   a.glob0 = glob ? glob[0] : 0.0; a.glob1 = glob ? glob[1] : 0.0;
   a.glob2 = newmark ? glob[2] : 0.0;
   a.glob3 = (kind == DFM_SOLID2D_NEWMARK || kind == DFM_STVK3D_NEWMARK || kind == DFM_STVK2D_NEWMARK) ? glob[3] : 0.0;
-  a.u = s.d_u.p; a.v = s.d_v.p; a.acc = kind == DFM_SOLID2D_BEULER ? s.d_v0.p : s.d_a.p;
+  a.u = s.d_u.p; a.v = s.d_v.p; a.acc = (kind == DFM_SOLID2D_BEULER || kind == DFM_STVK2D_BEULER) ? s.d_v0.p : s.d_a.p;
   a.load_ptr = s.has_loads ? s.d_lptr.p : nullptr;
   a.loads = s.has_loads ? s.d_loads.p : nullptr;
   a.load_scale = s.load_scale;

@@ -512,6 +512,36 @@ __device__ inline void row_stvk2d_newmark(int a, const double co[4][3], const do
   }
 }
 
+// StVK backward-Euler row (AddLinSys_StVenant2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_st_venant.cpp:509-664):
+// the static StVK tangent and internal force, mass rho area / 12, element matrix M + dt^2 K, residual
+// (f_ex - f_in) dt, then - K v dt dt term by term (is_initial) or - M (v - v_prev) term by term.
+__device__ inline void row_stvk2d_beuler(int a, const double co[4][3], const double u[4][3], const double vv[4][3],
+                                         const double v0[4][3], const double* p, double dt, bool initial, double* em,
+                                         double* er) {
+  const double p0[5] = {p[0], p[1], 0.0, p[3], p[4]};
+  double fin[2];
+  double* K = em;
+  row_stvk2d(a, co, u, p0, K, fin);  // fin = -f_in
+  const double rho = p[2];
+  const double area = tri_area(co[0], co[1], co[2]);
+  const double mm = rho * area / 12.0;
+  const double mm2 = mm + mm;
+  auto Mv = [&](int j, int r, int c) { return c != r ? 0.0 : (j == a ? mm2 : mm); };
+  for (int r = 0; r < 2; r++) {
+    double x = ((0.0 + area * rho * p[3 + r] * 0.33333333333333333333) + fin[r]) * dt;
+    if (initial) {
+      for (int j = 0; j < 3; j++)
+        for (int c = 0; c < 2; c++) x -= K[j * 4 + r * 2 + c] * vv[j][c] * dt * dt;
+    } else {
+      for (int j = 0; j < 3; j++)
+        for (int c = 0; c < 2; c++) x -= Mv(j, r, c) * (vv[j][c] - v0[j][c]);
+    }
+    er[r] = x;
+  }
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 4; k++) em[j * 4 + k] = Mv(j, k / 2, k % 2) + dt * dt * K[j * 4 + k];
+}
+
 // Backward-Euler row (AddLinSys_LinearSolid2D_NonStatic_BackwardEular_P1, src/femeqn/eqn_linear_solid2d.cpp:
 // 1338-1497; unknown: the velocity increment): the Newmark loop's K and M, element matrix M + dt^2 K, residual
 // dt f - dt K u, then - dt^2 K v when is_initial, else - M (v - v_prev). aa holds v_prev here.
@@ -572,7 +602,7 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
     for (int c = 0; c < NNO; c++)
       if (no[c] == i1) a = c;
     constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER ||
-                         KIND == DFM_STVK3D_NEWMARK || KIND == DFM_STVK2D_NEWMARK);
+                         KIND == DFM_STVK3D_NEWMARK || KIND == DFM_STVK2D_NEWMARK || KIND == DFM_STVK2D_BEULER);
     double co[4][3], u[4][3], val[4], vval[4];
     double vv[4][3], aa[4][3];
     for (int c = 0; c < NNO; c++) {
@@ -604,6 +634,7 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
         row_stvk3d_newmark(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
       if (KIND == DFM_STVK2D_NEWMARK)
         row_stvk2d_newmark(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+      if (KIND == DFM_STVK2D_BEULER) row_stvk2d_beuler(a, co, u, vv, aa, p, A.glob0, A.glob1 != 0.0, em, er);
       if (KIND == DFM_SOLID2D_BEULER) row_solid2d_beuler(a, co, u, vv, aa, p, A.glob0, A.glob1 != 0.0, em, er);
     }
     for (int c = 0; c < NNO; c++) {
@@ -658,6 +689,7 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
   DFM_ASM(DFM_SOLID2D_BEULER, 2, 2)
   DFM_ASM(DFM_STVK3D_NEWMARK, 3, 3)
   DFM_ASM(DFM_STVK2D_NEWMARK, 2, 2)
+  DFM_ASM(DFM_STVK2D_BEULER, 2, 2)
 #undef DFM_ASM
   return hipErrorInvalidValue;
 }

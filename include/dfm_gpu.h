@@ -57,7 +57,11 @@ enum {
   DFM_STVK3D_NEWMARK = 9,
   /* its 2D counterpart: AddLinSys_StVenant2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_st_venant.cpp:314-460,
    * (lambda, myu, rho, gx, gy), glob = (dt, gamma, beta, is_initial) */
-  DFM_STVK2D_NEWMARK = 10
+  DFM_STVK2D_NEWMARK = 10,
+  /* backward-Euler dynamic St.Venant-Kirchhoff 2D: AddLinSys_StVenant2D_NonStatic_BackwardEular_P1,
+   * src/femeqn/eqn_st_venant.cpp:509-664, (lambda, myu, rho, gx, gy), glob = (dt, is_initial); the previous velocity
+   * from dfm_sys_set_velocity_prev, the update with dfm_sys_update_field_beuler */
+  DFM_STVK2D_BEULER = 11
 };
 
 /* orderings for the ILU(0) preconditioner (dfm_sys_make_ordering); the last three are COrdering_Blk's, exactly

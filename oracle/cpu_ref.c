@@ -1959,3 +1959,60 @@ int cr_assemble_stvk2d_newmark(int nnode, const double* coords, int nelem, const
   }
   return 0;
 }
+
+/* AddLinSys_StVenant2D_NonStatic_BackwardEular_P1 (src/femeqn/eqn_st_venant.cpp:509-664): the static StVK tangent and
+ * internal force (density 0: -f_in), mass rho area / 12, M + dt^2 K, residual (f_ex - f_in) dt, then the
+ * is_initial or previous-velocity terms one product at a time. */
+int cr_assemble_stvk2d_beuler(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                              int pstride, double dt, int is_initial, const double* u, const double* v,
+                              const double* velo_pre, const unsigned* rowptr, const unsigned* colind, double* crs,
+                              double* dia, double* res) {
+  (void)nnode;
+  double eK[3][3][2][2], eM[3][3][2][2], emat[3][3][2][2], fin[3][2], fex[3][2], eres[3][2];
+  double co[4][3], disp[4][3], velo[3][2], velo0[3][2];
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * 3];
+    const double* p = &params[e * pstride];
+    const double rho = p[2];
+    const double g[2] = {p[3], p[4]};
+    const double p0[5] = {p[0], p[1], 0.0, p[3], p[4]};
+    for (int k = 0; k < 3; k++) {
+      co[k][2] = 0.0;
+      disp[k][2] = 0.0;
+      for (int d = 0; d < 2; d++) {
+        co[k][d] = coords[(long)no[k] * 2 + d];
+        disp[k][d] = u[(long)no[k] * 2 + d];
+        velo[k][d] = v[(long)no[k] * 2 + d];
+        velo0[k][d] = velo_pre[(long)no[k] * 2 + d];
+      }
+    }
+    elem_stvk2d(co, disp, p0, &eK[0][0][0][0], &fin[0][0]);
+    const double area = tri_area(co[0], co[1], co[2]);
+    for (int i = 0; i < 36; i++) (&eM[0][0][0][0])[i] = 0.0;
+    for (int i = 0; i < 6; i++) (&fex[0][0])[i] = 0.0;
+    {
+      const double tmp1 = rho * area / 12.0;
+      for (int ino = 0; ino < 3; ino++) {
+        for (int jno = 0; jno < 3; jno++)
+          for (int d = 0; d < 2; d++) eM[ino][jno][d][d] += tmp1;
+        for (int d = 0; d < 2; d++) eM[ino][ino][d][d] += tmp1;
+      }
+    }
+    for (int ino = 0; ino < 3; ino++)
+      for (int d = 0; d < 2; d++) fex[ino][d] += area * rho * g[d] * 0.33333333333333333333;
+    for (int i = 0; i < 36; i++) (&emat[0][0][0][0])[i] = (&eM[0][0][0][0])[i] + dt * dt * (&eK[0][0][0][0])[i];
+    for (int ino = 0; ino < 3; ino++)
+      for (int idim = 0; idim < 2; idim++) eres[ino][idim] = (fex[ino][idim] + fin[ino][idim]) * dt;
+    for (int ino = 0; ino < 3; ino++)
+      for (int idim = 0; idim < 2; idim++)
+        for (int jno = 0; jno < 3; jno++)
+          for (int jdim = 0; jdim < 2; jdim++) {
+            if (is_initial) eres[ino][idim] -= eK[ino][jno][idim][jdim] * velo[jno][jdim] * dt * dt;
+            else eres[ino][idim] -= eM[ino][jno][idim][jdim] * (velo[jno][jdim] - velo0[jno][jdim]);
+          }
+    merge(3, no, 4, &emat[0][0][0][0], rowptr, colind, crs, dia);
+    for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 2; d++) res[(long)no[k] * 2 + d] += eres[k][d];
+  }
+  return 0;
+}

@@ -135,6 +135,10 @@ int cr_assemble_stvk3d_newmark(int nnode, const double* coords, int nelem, const
 int cr_assemble_stvk2d_newmark(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
                                int pstride, const double* glob, const double* u, const double* v, const double* a,
                                const unsigned* rowptr, const unsigned* colind, double* crs, double* dia, double* res);
+int cr_assemble_stvk2d_beuler(int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                              int pstride, double dt, int is_initial, const double* u, const double* v,
+                              const double* velo_pre, const unsigned* rowptr, const unsigned* colind, double* crs,
+                              double* dia, double* res);
 
 #ifdef __cplusplus
 }

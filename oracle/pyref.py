@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CPU_REF_SO = os.path.join(HERE, "libcpu_ref.so")
 REF_SO = os.path.join(HERE, "_ref", "libdfm_ref.so")
 
-POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER, STVK3D_NEWMARK, STVK2D_NEWMARK = range(11)
+POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER, STVK3D_NEWMARK, STVK2D_NEWMARK, STVK2D_BEULER = range(12)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -172,13 +172,15 @@ class CpuSystem:
                                                   p(self.crs), p(self.dia), p(self.res))
         assert rc == 0
 
-    def assemble_beuler(self, params, u, v, v0, dt, is_initial):
-        """Backward-Euler 2D element loop (restated) into self.crs / self.dia / self.res (zero them first)."""
+    def assemble_beuler(self, params, u, v, v0, dt, is_initial, stvk=False):
+        """Backward-Euler 2D element loop (restated; linear elasticity, or St.Venant-Kirchhoff with stvk) into
+        self.crs / self.dia / self.res (zero them first)."""
         params = _f64(params)
         pstride = 0 if params.ndim == 1 else params.shape[1]
         u, v, v0 = _f64(u), _f64(v), _f64(v0)
         p = lambda x: x.ctypes.data_as(C.c_void_p)
-        rc = cpu_ref().cr_assemble_beuler(self.nnode, p(self.coords), self.nelem, p(self.lnods), p(params), pstride,
+        fn = cpu_ref().cr_assemble_stvk2d_beuler if stvk else cpu_ref().cr_assemble_beuler
+        rc = fn(self.nnode, p(self.coords), self.nelem, p(self.lnods), p(params), pstride,
                                           C.c_double(dt), 1 if is_initial else 0, p(u), p(v), p(v0), p(self.rowptr),
                                           p(self.colind), p(self.crs), p(self.dia), p(self.res))
         assert rc == 0

@@ -272,6 +272,11 @@ int ref_sys_assemble(void* h, int kind, const double* params, int npar, const do
                                                                   p[3], p[4], s->world, s->id_field, glob[3] != 0.0,
                                                                   id_ea);
         break;
+      case 11:  // glob = (dt, is_initial); the previous velocity from ref_sys_set_velo_pre
+        ok = Fem::Eqn::AddLinSys_StVenant2D_NonStatic_BackwardEular(glob[0], *s->ls, p[0], p[1], p[2], p[3], p[4],
+                                                                    s->world, s->id_field, *s->velo_pre,
+                                                                    glob[1] != 0.0, id_ea);
+        break;
       case 8:  // glob = (dt, is_initial); the previous velocity from ref_sys_set_velo_pre
         ok = Fem::Eqn::AddLinSys_LinearSolid2D_NonStatic_BackwardEular(glob[0], *s->ls, p[0], p[1], p[2], p[3], p[4],
                                                                        s->world, s->id_field, *s->velo_pre,

@@ -182,10 +182,11 @@ def test_newmark_errors(ctx):
 
 
 # ---- backward-Euler dynamic 2D elasticity ------------------------------------------------------------------
+@pytest.mark.parametrize("stvk", [False, True], ids=["linear", "stvk"])
 @pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
 @pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
-def test_beuler_assembly_and_update_bit_exact(ctx, oname, ordering, initial):
-    print(f"backward euler {oname} initial={initial}", flush=True)
+def test_beuler_assembly_and_update_bit_exact(ctx, stvk, oname, ordering, initial):
+    print(f"backward euler stvk={stvk} {oname} initial={initial}", flush=True)
     co, ln = mesh(2, True)
     n = co.size // 2
     params = NM[0][3](np.random.default_rng(17), ln.size // 3)
@@ -193,7 +194,7 @@ def test_beuler_assembly_and_update_bit_exact(ctx, oname, ordering, initial):
     u, v, v0 = (rng.uniform(-0.05, 0.05, 2 * n) for _ in range(3))
     fl = flags(co, 2, 2)
     cpu = P.CpuSystem(2, co, ln, 2)
-    cpu.assemble_beuler(params, u, v, v0, 0.04, initial)
+    cpu.assemble_beuler(params, u, v, v0, 0.04, initial, stvk)
     cpu.set_bc(fl)
     gs = N.System(ctx, 2, co, ln, 2)
     if ordering != 0:
@@ -202,7 +203,7 @@ def test_beuler_assembly_and_update_bit_exact(ctx, oname, ordering, initial):
     gs.set_bc(fl)
     gs.set_field(u, v)
     gs.set_velocity_prev(v0)
-    gs.assemble(8, params, glob=(0.04, 1.0 if initial else 0.0))
+    gs.assemble(11 if stvk else 8, params, glob=(0.04, 1.0 if initial else 0.0))
     crs, dia = gs.get_matrix()
     assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
     assert np.array_equal(gs.get_vector(-1), cpu.res), np.abs(gs.get_vector(-1) - cpu.res).max()

@@ -507,9 +507,10 @@ def test_newmark_update_not_initial_bit_exact():
 
 
 # ---- backward-Euler dynamic 2D elasticity -------------------------------------------------------------------
+@pytest.mark.parametrize("stvk", [False, True], ids=["linear", "stvk"])
 @pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
 @pytest.mark.parametrize("initial", [True, False], ids=["initial", "not_initial"])
-def test_beuler_assembly_bit_exact(per_element, initial):
+def test_beuler_assembly_bit_exact(stvk, per_element, initial):
     co, ln = mesh2d(4, 2) if per_element else mesh2d(9, 5)
     n, nelem = co.size // 2, ln.size // 3
     rng = np.random.default_rng(12)
@@ -523,8 +524,9 @@ def test_beuler_assembly_bit_exact(per_element, initial):
     ref.set_velo_pre(v0)
     flags = clamp_flags(co, 2, 2)
     ref.set_bc(flags)
-    rn = ref.assemble(P.SOLID2D_BEULER, params if per_element else params[None, :], (0.04, 1.0 if initial else 0.0))
-    cpu.assemble_beuler(params, u, v, v0, 0.04, initial)
+    kind = P.STVK2D_BEULER if stvk else P.SOLID2D_BEULER
+    rn = ref.assemble(kind, params if per_element else params[None, :], (0.04, 1.0 if initial else 0.0))
+    cpu.assemble_beuler(params, u, v, v0, 0.04, initial, stvk)
     rn2 = cpu.set_bc(flags)
     crs, dia = ref.matrix()
     assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
