@@ -265,7 +265,12 @@ int dfm_server_submit(dfm_server s, const dfm_request* q, long long* ticket) {
   const size_t np = q->pstride ? (size_t)q->pstride * q->nelem : (size_t)q->npar;
   r->params.assign(q->params, q->params + np);
   r->bc.assign(q->bc, q->bc + (size_t)q->nnode * q->len);
-  if (q->glob) r->glob.assign(q->glob, q->glob + 2);
+  if (q->glob) {
+    // the Newmark kinds read (dt, gamma, beta, is_initial), the others at most two values
+    const bool nm = q->kind == DFM_SOLID2D_NEWMARK || q->kind == DFM_SOLID3D_NEWMARK || q->kind == DFM_STVK3D_NEWMARK ||
+                    q->kind == DFM_STVK2D_NEWMARK;
+    r->glob.assign(q->glob, q->glob + (nm ? 4 : 2));
+  }
   if (q->nface > 0) {
     DFM_CHECK(q->faces && q->trac, DFM_ERR_ARG, "dfm_server_submit: faces without tractions");
     r->faces.assign(q->faces, q->faces + (size_t)q->nface * 3);

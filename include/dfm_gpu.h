@@ -316,7 +316,7 @@ typedef struct {
   const int* lnods;             /* [nelem*(ndim+1)]                                                    */
   const double* params;         /* npar per element (pstride) or shared (pstride = 0)                  */
   int npar, pstride;
-  const double* glob;           /* 2 values or NULL                                                    */
+  const double* glob;           /* 2 values (4 for the Newmark kinds) or NULL                          */
   const int* bc;                /* [nnode*len] Dirichlet flags                                         */
   int nface;                    /* surface loads (3D), may be 0                                        */
   const int* faces;
