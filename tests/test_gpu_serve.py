@@ -104,3 +104,33 @@ def test_admission_limits(ctx):
         if all(srv.poll(t)["state"] >= N.REQ_CONVERGED for t in tickets):
             break
     srv.close()
+
+
+def test_served_newmark_request(ctx):
+    """A Newmark request (four kind scalars) is served like the same system solved alone in FP64: the served
+    update solves the same system to the tolerance."""
+    import meshgen as Mg
+
+    co, ln = Mg.box_tet(4, 3, 3, 1.0, 0.8, 0.7, 41)
+    n = co.size // 3
+    x = co.reshape(-1, 3)
+    bc = np.zeros((n, 3), np.int32)
+    bc[x[:, 0] == x[:, 0].min()] = 1
+    bc = bc.ravel()
+    par = np.array([0.5, 0.4, 1.0, 0.0, 0.0, -0.05])
+    glob = np.array([0.05, 0.5, 0.25, 1.0])
+    print("served newmark request", flush=True)
+    srv = N.Server(ctx, precision=N.FP64, budget_bytes=1 << 30, max_active=4, nworkers=1)
+    t = srv.submit(3, 3, N.SOLID3D_NEWMARK, co, ln, par, bc, tol=1e-10, max_iter=5000, glob=glob)
+    _drain(srv, [t])
+    assert srv.poll(t)["state"] == N.REQ_CONVERGED
+    got = srv.result(t, 3 * n)
+    srv.release(t)
+    srv.close()
+    cpu = P.CpuSystem(3, co, ln, 3)
+    z = np.zeros(3 * n)
+    cpu.assemble_newmark(par, z, z, z, glob)
+    cpu.set_bc(bc)
+    ok, _, _, want, _ = cpu.solve(cpu.ilu0(), "pcg", 1e-12, 5000)
+    assert ok
+    assert np.linalg.norm(got - want) <= 1e-7 * np.linalg.norm(want)
