@@ -148,6 +148,7 @@ Sys::~Sys() {
   d_work.release(); d_work32.release(); d_partial.release(); d_part_sw.release(); d_counter.release(); d_scal.release(); d_iflag.release();
   for (Sweep* w : {&fwd, &bwd}) w->release_dev();
   sell.release_dev();
+  fplan.release_dev();
   d_par.release(); d_lptr.release(); d_loads.release();
   d_force.release(); d_bptr.release(); d_bcol.release(); d_bsrc.release(); d_bval.release();
   for (auto& g : gexec)
@@ -169,6 +170,9 @@ static hipError_t upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
 // Factor pattern and sweep schedule (device copies); the slab values are filled by dfm_prec_factor.
 static int upload_sweeps(Sys& s) {
   hipStream_t st = s.ctx->stream;
+  // the launch shapes of a captured Krylov chunk (sweep block counts, phase ranges, SELL slices, register caps)
+  // follow this schedule: a new one invalidates the graphs even where the allocator hands back the same addresses
+  s.struct_gen++;
   DFM_HIP(upload(s.d_lsplit, s.lsplit, st));
   DFM_HIP(upload(s.d_usplit, s.usplit, st));
   DFM_HIP(upload(s.d_frowptr, s.frowptr, st));
@@ -186,6 +190,16 @@ static int upload_sweeps(Sys& s) {
     DFM_HIP(upload(w->d_ecol, w->ecol, st));
     DFM_HIP(upload(w->d_esrc, w->esrc, st));
     DFM_HIP(upload(w->d_fuse_bgrp, w->fuse_bgrp, st));
+  }
+  FactorPlan& fp = s.fplan;
+  fp.release_dev();
+  if (fp.ok) {
+    DFM_HIP(upload(fp.d_gpass, fp.gpass, st));
+    DFM_HIP(upload(fp.d_pmeta, fp.pmeta, st));
+    DFM_HIP(upload(fp.d_etab, fp.etab, st));
+    DFM_HIP(upload(fp.d_utab, fp.utab, st));
+    DFM_HIP(upload(fp.d_ukj, fp.ukj, st));
+    DFM_HIP(upload(fp.d_ustep, fp.ustep, st));
   }
   return DFM_OK;
 }
@@ -287,7 +301,7 @@ int dfm_ctx_create(int device, dfm_ctx* out) {
 }  // extern "C"
 
 void ctx_release(dfm_ctx ctx) {
-  if (--ctx->c.refs > 0) return;
+  if (ctx->c.refs.fetch_sub(1) != 1) return;
   (void)hipStreamSynchronize(ctx->c.stream);
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
@@ -319,8 +333,9 @@ static hipError_t sys_alloc_device(Sys& s) {
   if ((e = upload(s.d_lnods, s.lnods, st)) != hipSuccess) return e;
   if ((e = upload(s.d_elsup_ptr, s.elsup_ptr, st)) != hipSuccess) return e;
   if ((e = upload(s.d_elsup, s.elsup, st)) != hipSuccess) return e;
-  if ((e = s.d_crs.alloc((size_t)s.ncrs * s.bs)) != hipSuccess) return e;
-  if ((e = s.d_dia.alloc((size_t)nnode * s.bs)) != hipSuccess) return e;
+  // two doubles of padding: the factor kernel reads 3x3 blocks as five aligned 16-byte loads, one past the end
+  if ((e = s.d_crs.alloc((size_t)s.ncrs * s.bs + 2)) != hipSuccess) return e;
+  if ((e = s.d_dia.alloc((size_t)nnode * s.bs + 2)) != hipSuccess) return e;
   if ((e = s.d_res.alloc(m)) != hipSuccess) return e;
   if ((e = s.d_upd.alloc(m)) != hipSuccess) return e;
   if ((e = s.d_rhs.alloc(m)) != hipSuccess) return e;
@@ -406,6 +421,8 @@ int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
   copy_sweep_host(s.bwd, a.bwd);
   s.sell.sptr = a.sell.sptr; s.sell.col = a.sell.col; s.sell.src = a.sell.src; s.sell.nslab = a.sell.nslab;
   s.sell.visit = a.sell.visit;
+  s.fplan.ok = a.fplan.ok; s.fplan.gpass = a.fplan.gpass; s.fplan.pmeta = a.fplan.pmeta; s.fplan.etab = a.fplan.etab;
+  s.fplan.utab = a.fplan.utab; s.fplan.ukj = a.fplan.ukj; s.fplan.ustep = a.fplan.ustep;
   auto fail = [&](hipError_t e) {
     set_error(std::string("dfm_sys_clone: ") + hipGetErrorString(e));
     delete h;
@@ -564,11 +581,13 @@ int dfm_sys_set_field(dfm_sys sys, const double* u, const double* v) {
   NEED(sys, "dfm_sys_set_field");
   Sys& s = sys->s;
   const size_t bytes = sizeof(double) * mdof(s);
-  if (u) DFM_HIP(hipMemcpy(s.d_u.p, u, bytes, hipMemcpyHostToDevice));
-  else DFM_HIP(launch_fill(s.d_u.p, 0.0, mdof(s), s.ctx->stream));
-  if (v) DFM_HIP(hipMemcpy(s.d_v.p, v, bytes, hipMemcpyHostToDevice));
-  else DFM_HIP(launch_fill(s.d_v.p, 0.0, mdof(s), s.ctx->stream));
-  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  hipStream_t st = s.ctx->stream;
+  // on the library's stream (non-blocking: a null-stream copy would not wait for an assembly still reading u, v)
+  if (u) DFM_HIP(hipMemcpyAsync(s.d_u.p, u, bytes, hipMemcpyHostToDevice, st));
+  else DFM_HIP(launch_fill(s.d_u.p, 0.0, mdof(s), st));
+  if (v) DFM_HIP(hipMemcpyAsync(s.d_v.p, v, bytes, hipMemcpyHostToDevice, st));
+  else DFM_HIP(launch_fill(s.d_v.p, 0.0, mdof(s), st));
+  DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
 
@@ -576,8 +595,9 @@ int dfm_sys_get_field(dfm_sys sys, double* u) {
   NEED(sys, "dfm_sys_get_field");
   NEED(u, "dfm_sys_get_field");
   Sys& s = sys->s;
-  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
-  DFM_HIP(hipMemcpy(u, s.d_u.p, sizeof(double) * mdof(s), hipMemcpyDeviceToHost));
+  hipStream_t st = s.ctx->stream;
+  DFM_HIP(hipMemcpyAsync(u, s.d_u.p, sizeof(double) * mdof(s), hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
 
@@ -641,13 +661,14 @@ int dfm_sys_get_fields(dfm_sys sys, double* u, double* v, double* a) {
   NEED(sys, "dfm_sys_get_fields");
   Sys& s = sys->s;
   const size_t bytes = sizeof(double) * mdof(s);
-  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
-  if (u) DFM_HIP(hipMemcpy(u, s.d_u.p, bytes, hipMemcpyDeviceToHost));
-  if (v) DFM_HIP(hipMemcpy(v, s.d_v.p, bytes, hipMemcpyDeviceToHost));
+  hipStream_t st = s.ctx->stream;
+  if (u) DFM_HIP(hipMemcpyAsync(u, s.d_u.p, bytes, hipMemcpyDeviceToHost, st));
+  if (v) DFM_HIP(hipMemcpyAsync(v, s.d_v.p, bytes, hipMemcpyDeviceToHost, st));
   if (a) {
-    if (s.d_a.p) DFM_HIP(hipMemcpy(a, s.d_a.p, bytes, hipMemcpyDeviceToHost));
+    if (s.d_a.p) DFM_HIP(hipMemcpyAsync(a, s.d_a.p, bytes, hipMemcpyDeviceToHost, st));
     else std::memset(a, 0, bytes);
   }
+  DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
 
@@ -775,9 +796,10 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   a.coords = s.d_coords.p; a.lnods = s.d_lnods.p; a.elsup_ptr = s.d_elsup_ptr.p; a.elsup = s.d_elsup.p;
   a.order = s.d_order.p; a.inv = s.d_inv.p;
   a.params = s.d_par.p; a.npar = s.npar; a.pstride = s.pstride;
-  a.glob0 = glob ? glob[0] : 0.0; a.glob1 = glob ? glob[1] : 0.0;
-  a.glob2 = newmark ? glob[2] : 0.0;
-  a.glob3 = (kind == DFM_SOLID2D_NEWMARK || kind == DFM_STVK3D_NEWMARK || kind == DFM_STVK2D_NEWMARK) ? glob[3] : 0.0;
+  // only the kind's own scalars are read (kind_glob_count): a caller's array may hold no more
+  const int ng = glob ? kind_glob_count(kind) : 0;
+  a.glob0 = ng > 0 ? glob[0] : 0.0; a.glob1 = ng > 1 ? glob[1] : 0.0;
+  a.glob2 = ng > 2 ? glob[2] : 0.0; a.glob3 = ng > 3 ? glob[3] : 0.0;
   a.u = s.d_u.p; a.v = s.d_v.p; a.acc = (kind == DFM_SOLID2D_BEULER || kind == DFM_STVK2D_BEULER) ? s.d_v0.p : s.d_a.p;
   a.load_ptr = s.has_loads ? s.d_lptr.p : nullptr;
   a.loads = s.has_loads ? s.d_loads.p : nullptr;
@@ -1074,29 +1096,47 @@ int dfm_sys_set_vector(dfm_sys sys, int iv, const double* in) {
   return DFM_OK;
 }
 
+// SetValue_Initialize + DoILUDecomp on the device, enqueued on st: the factor pattern takes the matrix values
+// (ILU(0): the patterns coincide and the lane-parallel kernel reads the matrix itself; otherwise a scatter into
+// the zeroed fill pattern, factored in place), then the phases of the numeric factorisation; singular pivots
+// are counted in d_iflag[4].
+static hipError_t factor_numeric(Sys& s, hipStream_t st) {
+  hipError_t e;
+  const long long fncrs = (long long)s.fcolind.size();
+  const bool lanes = s.fplan.ok && std::getenv("DFM_ILU_ROWKERNEL") == nullptr;
+  const double* a_crs = s.d_fcrs.p;
+  const double* a_dia = s.d_fdia.p;
+  if (s.a2f.empty()) {
+    if (lanes) a_crs = s.d_crs.p, a_dia = s.d_dia.p;
+    else if ((e = launch_copy(s.d_crs.p, s.d_fcrs.p, (long long)s.ncrs * s.bs, st)) != hipSuccess) return e;
+  } else {
+    if ((e = launch_fill(s.d_fcrs.p, 0.0, fncrs * s.bs, st)) != hipSuccess) return e;
+    if ((e = launch_scatter_blocks(s.ncrs, s.bs, s.d_a2f.p, s.d_crs.p, s.d_fcrs.p, st)) != hipSuccess) return e;
+  }
+  if (a_dia == s.d_fdia.p && (e = launch_copy(s.d_dia.p, s.d_fdia.p, (long long)s.n * s.bs, st)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(s.d_iflag.p + 4, 0, sizeof(int), st)) != hipSuccess) return e;
+  if (lanes) {
+    const FactorPlan& fp = s.fplan;
+    FactorPlanDev pd{fp.d_gpass.p, fp.d_pmeta.p, fp.d_etab.p, fp.d_utab.p, fp.d_ukj.p, fp.d_ustep.p};
+    return launch_ilu_rows(s.len, pd, s.fwd.phase_gptr.data(), s.fwd.nphase(), a_crs, a_dia, s.d_fcrs.p, s.d_fdia.p,
+                           s.d_iflag.p + 4, st);
+  }
+  SweepDev fw{s.fwd.nphase(), s.fwd.phase_gptr.data(), s.fwd.phase_nlev.data(), s.fwd.d_grp.p,
+              s.fwd.rowlist.empty() ? nullptr : s.fwd.d_rowlist.p, s.fwd.d_lev.p};
+  return launch_ilu_factor(s.n, s.len, s.d_frowptr.p, s.d_fcolind.p, s.d_fdiaind.p, s.d_fcrs.p, s.d_fdia.p, fw,
+                           s.d_iflag.p + 4, st);
+}
+
 int dfm_prec_factor(dfm_sys sys) {
   NEED(sys, "dfm_prec_factor");
   Sys& s = sys->s;
   DFM_CHECK(s.assembled, DFM_ERR_STATE, "dfm_prec_factor: matrix not assembled");
   hipStream_t st = s.ctx->stream;
   const long long fncrs = (long long)s.fcolind.size();
-  if (s.d_fcrs.n != (size_t)std::max<long long>(fncrs * s.bs, 1))
-    DFM_HIP(s.d_fcrs.alloc((size_t)std::max<long long>(fncrs * s.bs, 1)));
-  if (!s.d_fdia.p) DFM_HIP(s.d_fdia.alloc((size_t)s.n * s.bs));
-  // SetValue_Initialize: the factor pattern takes the matrix values (a copy when the patterns coincide, ILU(0);
-  // a scatter into the zeroed fill pattern otherwise)
-  if (s.a2f.empty()) {
-    DFM_HIP(launch_copy(s.d_crs.p, s.d_fcrs.p, (long long)s.ncrs * s.bs, st));
-  } else {
-    DFM_HIP(launch_fill(s.d_fcrs.p, 0.0, fncrs * s.bs, st));
-    DFM_HIP(launch_scatter_blocks(s.ncrs, s.bs, s.d_a2f.p, s.d_crs.p, s.d_fcrs.p, st));
-  }
-  DFM_HIP(launch_copy(s.d_dia.p, s.d_fdia.p, (long long)s.n * s.bs, st));
-  DFM_HIP(hipMemsetAsync(s.d_iflag.p + 4, 0, sizeof(int), st));
-  SweepDev fw{s.fwd.nphase(), s.fwd.phase_gptr.data(), s.fwd.phase_nlev.data(), s.fwd.d_grp.p,
-              s.fwd.rowlist.empty() ? nullptr : s.fwd.d_rowlist.p, s.fwd.d_lev.p};
-  DFM_HIP(launch_ilu_factor(s.n, s.len, s.d_frowptr.p, s.d_fcolind.p, s.d_fdiaind.p, s.d_fcrs.p, s.d_fdia.p, fw,
-                            s.d_iflag.p + 4, st));
+  // padded by two doubles, as the matrix
+  if (s.d_fcrs.n != (size_t)(fncrs * s.bs + 2)) DFM_HIP(s.d_fcrs.alloc((size_t)(fncrs * s.bs + 2)));
+  if (!s.d_fdia.p) DFM_HIP(s.d_fdia.alloc((size_t)s.n * s.bs + 2));
+  DFM_HIP(factor_numeric(s, st));
   int nsing = 0;
   DFM_HIP(hipMemcpyAsync(s.h_pinned + 8, s.d_iflag.p + 4, sizeof(int), hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
@@ -1287,35 +1327,58 @@ int dfm_sys_true_residual(dfm_sys sys, double* rel) {
   return DFM_OK;
 }
 
+}  // extern "C"
+
+// Average device time of `reps` back-to-back enqueues of launch() on st (after three untimed ones), HIP events.
+template <typename F>
+static int time_launches(hipStream_t st, int reps, F launch, double* avg_ms) {
+  for (int w = 0; w < 3; w++) DFM_HIP(launch());
+  hipEvent_t e0, e1;
+  DFM_HIP(hipEventCreate(&e0));
+  DFM_HIP(hipEventCreate(&e1));
+  DFM_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; r++) DFM_HIP(launch());
+  DFM_HIP(hipEventRecord(e1, st));
+  DFM_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (avg_ms) *avg_ms = ms / reps;
+  return DFM_OK;
+}
+
+extern "C" {
+
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes) {
   NEED(sys, "dfm_sys_bench_kernel");
   Sys& s = sys->s;
-  DFM_CHECK(which >= 0 && which <= 6 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
+  DFM_CHECK(which >= 0 && which <= 7 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
+  hipStream_t st = s.ctx->stream;
   if (which == 6) {
     // MakeResidual of the saved stiffness: force, residual, update and the field once per dof, the order map,
     // the term ranges, and per term its value, its column and the field value it gathers
     DFM_CHECK(s.saved, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble with dfm_sys_assemble_save first");
-    hipStream_t st = s.ctx->stream;
     auto launch = [&]() {
       return launch_make_residual(s.n, s.len, s.d_order.p, s.d_bptr.p, s.d_bcol.p, s.d_bval.p, s.d_force.p, s.d_u.p,
                                   s.d_res.p, s.d_upd.p, st);
     };
-    for (int w = 0; w < 3; w++) DFM_HIP(launch());
-    hipEvent_t e0, e1;
-    DFM_HIP(hipEventCreate(&e0));
-    DFM_HIP(hipEventCreate(&e1));
-    DFM_HIP(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; r++) DFM_HIP(launch());
-    DFM_HIP(hipEventRecord(e1, st));
-    DFM_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (avg_ms) *avg_ms = ms / reps;
+    const int rc = time_launches(st, reps, launch, avg_ms);
+    if (rc != DFM_OK) return rc;
     const double m = (double)mdof(s);
     if (bytes) *bytes = m * (4 * 8.0 + 4.0) + 4.0 + 4.0 * s.n + (double)s.nbterm * (8.0 + 4.0 + 8.0);
     s.prepared = -1;
+    return DFM_OK;
+  }
+  if (which == 7) {
+    // the numeric ILU factorisation (initial values and factor kernels, not the slab copies): it rewrites the
+    // factor with the same values. Algorithmic bytes: the matrix read and the factor written once (FP64 blocks),
+    // the factor's column indices and row pointers
+    DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble and factor first");
+    const int rc = time_launches(st, reps, [&]() { return factor_numeric(s, st); }, avg_ms);
+    if (rc != DFM_OK) return rc;
+    const double fn = (double)s.fcolind.size();
+    if (bytes) *bytes = 2.0 * (fn + s.n) * s.bs * 8.0 + 4.0 * fn + 4.0 * (s.n + 1);
     return DFM_OK;
   }
   DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble and factor first");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -69,7 +70,9 @@ struct DBuf {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  int refs = 1;            // the handle itself + one per live system
+  // the handle itself + one per live system; systems are created and destroyed from several threads (the
+  // server's workers and its client), so the count is atomic
+  std::atomic<int> refs{1};
   bool destroyed = false;  // dfm_ctx_destroy called; freed when the last system goes
 };
 
@@ -143,6 +146,30 @@ struct Sell {
   }
 };
 
+// Lane plan of the numeric ILU factorisation (k_ilu_rows). Every factor row is held by consecutive lanes of one
+// wavefront, one lane per block in the row's order: its lower entries, the diagonal, its upper entries. The
+// forward schedule's groups keep their phases and their intra-group levels; the rows of one level of a group are
+// packed into passes of at most 64 lanes. Each lane has the list of its updates A_ij -= A_ik A_kj as (factor
+// entry kj, step = position of k among the row's lower entries), in ascending k, the order the reference's IKJ
+// loop applies them (src/matvec/matdiafrac_blkcrs.cpp:774-1107); at step q the lane of the row's lower entry q
+// hands its final A_ik to the row's lanes. Lane data and lists are stored in pass order.
+struct FactorPlan {
+  bool ok = false;                // every row fits one wavefront and the lists fit 32-bit offsets
+  std::vector<int> gpass;         // [ngroup+1] pass range of each forward group (execution order)
+  std::vector<unsigned> pmeta;    // [8*npass] row-start lane mask (lo, hi), diagonal lane mask (lo, hi),
+                                  // lanes | steps << 8 | (first pass of a later level) << 16, 3 unused
+  std::vector<unsigned> etab;     // [64*npass] per lane: its off-diagonal factor entry, or its row (diagonal)
+  std::vector<unsigned> utab;     // [64*npass + 1] per lane: start of its updates (the next lane's start ends them)
+  std::vector<unsigned> ukj;      // the factor entries kj of the updates
+  std::vector<uint8_t> ustep;     // the step of each update
+  DBuf<int> d_gpass;
+  DBuf<unsigned> d_pmeta, d_etab, d_utab, d_ukj;
+  DBuf<uint8_t> d_ustep;
+  void release_dev() {
+    d_gpass.release(); d_pmeta.release(); d_etab.release(); d_utab.release(); d_ukj.release(); d_ustep.release();
+  }
+};
+
 struct Sys {
   Ctx* ctx = nullptr;
   int ndim = 0, n = 0, len = 0, bs = 0, npoel = 0, nelem = 0;
@@ -171,6 +198,7 @@ struct Sys {
   std::vector<unsigned> a2f;  // matrix entry -> factor entry (empty when the patterns coincide)
   Sweep fwd, bwd;
   Sell sell;
+  FactorPlan fplan;
 
   // device arrays
   DBuf<unsigned> d_rowptr, d_colind, d_diaind, d_lsplit, d_usplit;
@@ -220,6 +248,7 @@ struct Sys {
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<const void*> gkey[4];  // every device pointer the graph was captured with
   int gchunk = 16;
+  unsigned long long struct_gen = 0;  // bumped whenever the pattern, ordering, fill level or schedule is rebuilt
 
   bool assembled = false, factored = false, factored32 = false;
   double last_ms = 0.0, last_ms_iter = 0.0;
@@ -231,6 +260,19 @@ struct Sys {
 
   ~Sys();
 };
+
+// Number of kind-specific scalars (glob) an element kind reads: diffusion (dt, gamma), the Newmark kinds
+// (dt, gamma, beta, is_initial; the 3D linear one has no is_initial), backward Euler (dt, is_initial), none for
+// the static kinds. Assembly and the server's request copy both go by it.
+inline int kind_glob_count(int kind) {
+  switch (kind) {
+    case DFM_DIFFUSION2D: return 2;
+    case DFM_SOLID2D_NEWMARK: case DFM_STVK3D_NEWMARK: case DFM_STVK2D_NEWMARK: return 4;
+    case DFM_SOLID3D_NEWMARK: return 3;
+    case DFM_SOLID2D_BEULER: case DFM_STVK2D_BEULER: return 2;
+    default: return 0;
+  }
+}
 
 // The precision a solve of this system runs in: DFM_MIXED_AUTO resolved by block size (capi.cpp).
 int resolve_precision(const Sys& s, int precision);
@@ -244,7 +286,7 @@ void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vecto
                    std::vector<unsigned>& colind, std::vector<int>& elsup_ptr, std::vector<int>& elsup);
 void build_device_pattern(Sys& s);  // from order/inv: rowptr, colind, diaind, perm_crs
 void build_schedule(Sys& s);        // SELL copy of the pattern, then build_sweeps
-void build_sweeps(Sys& s);          // factor pattern, fwd/bwd sweeps, lsplit/usplit, slabs
+void build_sweeps(Sys& s);          // factor pattern, fwd/bwd sweeps, lsplit/usplit, slabs, factor plan
 void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors);
 // COrdering_Blk's orderings of a block pattern (DFM_ORDER_RCM / _RCM2 / _AMD), new -> old as the reference stores
 // them; false where the reference stops on an assertion (host_symbolic.cpp)

@@ -504,6 +504,91 @@ void build_schedule(Sys& s) {
   build_sweeps(s);
 }
 
+// The lane plan of the numeric factorisation over the forward schedule (FactorPlan). Off (the per-row kernel
+// runs instead) when a row is wider than a wavefront or the update lists outgrow 32-bit offsets.
+static void build_factor_plan(Sys& s) {
+  FactorPlan& fp = s.fplan;
+  const int n = s.n;
+  const std::vector<unsigned>& rp = s.frowptr;
+  const std::vector<unsigned>& ci = s.fcolind;
+  const std::vector<unsigned>& di = s.fdiaind;
+  fp.ok = false;
+  fp.gpass.clear(); fp.pmeta.clear(); fp.etab.clear(); fp.utab.clear(); fp.ukj.clear(); fp.ustep.clear();
+  for (int i = 0; i < n; i++)
+    if (rp[i + 1] - rp[i] + 1 > 64) return;
+  fp.ukj.reserve(rp[n] * 2);
+  fp.ustep.reserve(rp[n] * 2);
+  std::vector<int> pos((size_t)n, -1);
+  std::vector<std::vector<unsigned>> lkj(64);
+  std::vector<std::vector<uint8_t>> lst(64);
+  // a row's lanes: lower entries, the diagonal, upper entries; their update lists in pass order
+  auto emit_row = [&](int i) {
+    const unsigned r0 = rp[i], r1 = rp[i + 1], d = di[i];
+    const int nl = (int)(d - r0), w = (int)(r1 - r0) + 1;
+    for (int t = 0; t < w; t++) lkj[t].clear(), lst[t].clear();
+    for (unsigned q = r0; q < r1; q++) pos[ci[q]] = (int)(q - r0) + (q >= d ? 1 : 0);
+    pos[i] = nl;
+    for (unsigned ik = r0; ik < d; ik++) {
+      const unsigned k = ci[ik];
+      for (unsigned kj = di[k]; kj < rp[k + 1]; kj++) {
+        const int t = pos[ci[kj]];
+        if (t >= 0) lkj[t].push_back(kj), lst[t].push_back((uint8_t)(ik - r0));
+      }
+    }
+    for (unsigned q = r0; q < r1; q++) pos[ci[q]] = -1;
+    pos[i] = -1;
+    for (int t = 0; t < w; t++) {
+      fp.etab.push_back(t == nl ? (unsigned)i : r0 + (unsigned)(t < nl ? t : t - 1));
+      fp.utab.push_back((unsigned)fp.ukj.size());
+      fp.ukj.insert(fp.ukj.end(), lkj[t].begin(), lkj[t].end());
+      fp.ustep.insert(fp.ustep.end(), lst[t].begin(), lst[t].end());
+    }
+  };
+  const Sweep& w = s.fwd;
+  const int ngroup = (int)(w.grp.size() / 2);
+  fp.gpass.assign(1, 0);
+  std::vector<int> rows;
+  for (int g = 0; g < ngroup; g++) {
+    rows.clear();
+    for (int r = w.grp[2 * g]; r < w.grp[2 * g + 1]; r++) rows.push_back(w.rowlist.empty() ? r : w.rowlist[r]);
+    std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return w.lev[a] < w.lev[b]; });
+    size_t k = 0;
+    int prev_lev = -1;
+    while (k < rows.size()) {
+      const int lev = w.lev[rows[k]];
+      unsigned long long smask = 0, dmask = 0;
+      int lanes = 0, steps = 0;
+      while (k < rows.size() && w.lev[rows[k]] == lev) {
+        const int i = rows[k];
+        const int wi = (int)(rp[i + 1] - rp[i]) + 1, nl = (int)(di[i] - rp[i]);
+        if (lanes + wi > 64) break;
+        smask |= 1ull << lanes;
+        dmask |= 1ull << (lanes + nl);
+        lanes += wi;
+        steps = std::max(steps, nl);
+        emit_row(i);
+        k++;
+      }
+      for (int l = lanes; l < 64; l++) {  // idle lanes: no entry, an empty update range
+        fp.etab.push_back(0);
+        fp.utab.push_back((unsigned)fp.ukj.size());
+      }
+      if (fp.ukj.size() > 0xFFFFFFF0ull) {
+        fp.gpass.clear(); fp.pmeta.clear(); fp.etab.clear(); fp.utab.clear(); fp.ukj.clear(); fp.ustep.clear();
+        return;
+      }
+      const unsigned info = (unsigned)lanes | (unsigned)steps << 8 | (prev_lev >= 0 && lev != prev_lev ? 1u << 16 : 0u);
+      const unsigned meta[8] = {(unsigned)(smask & 0xFFFFFFFFull), (unsigned)(smask >> 32),
+                                (unsigned)(dmask & 0xFFFFFFFFull), (unsigned)(dmask >> 32), info, 0u, 0u, 0u};
+      fp.pmeta.insert(fp.pmeta.end(), meta, meta + 8);
+      prev_lev = lev;
+    }
+    fp.gpass.push_back((int)(fp.pmeta.size() / 8));
+  }
+  fp.utab.push_back((unsigned)fp.ukj.size());
+  fp.ok = true;
+}
+
 void build_sweeps(Sys& s) {
   build_factor_pattern(s);
   if (s.color_of.empty()) {
@@ -511,6 +596,7 @@ void build_sweeps(Sys& s) {
     build_slabs(s, s.fwd, true);
     build_slabs(s, s.bwd, false);
     s.fwd.fuse_bgrp.clear();
+    build_factor_plan(s);
     return;
   }
   // contiguous groups: tiles (BMC) or 64-row chunks of each colour (MC)
@@ -530,6 +616,7 @@ void build_sweeps(Sys& s) {
   build_slabs(s, s.fwd, true);
   build_slabs(s, s.bwd, false);
   fuse_last_first(s);
+  build_factor_plan(s);
 }
 
 // COrdering_Blk's orderings (src/matvec/ordering_blk.cpp), on the block pattern without the diagonal.

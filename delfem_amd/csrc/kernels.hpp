@@ -81,6 +81,18 @@ struct SweepDev {
 hipError_t launch_ilu_factor(int n, int len, const unsigned* rowptr, const unsigned* colind,
                              const unsigned* diaind, double* fcrs, double* fdia, const SweepDev& fwd,
                              int* nsing, hipStream_t st);
+// The lane-parallel factorisation over the forward phases (FactorPlan; device arrays).
+struct FactorPlanDev {
+  const int* gpass;
+  const unsigned* pmeta;
+  const unsigned* etab;
+  const unsigned* utab;
+  const unsigned* ukj;
+  const uint8_t* ustep;
+};
+// a_crs/a_dia and fcrs/fdia carry two doubles of padding (3x3 blocks are read with aligned 16-byte loads).
+hipError_t launch_ilu_rows(int len, const FactorPlanDev& fp, const int* phase_gptr, int nphase, const double* a_crs,
+                           const double* a_dia, double* fcrs, double* fdia, int* nsing, hipStream_t st);
 // Values of a sweep's packed entries (jagged rows, then per-entry) from the FP64 factor through src (the factor
 // block of each entry). T = double or float.
 hipError_t launch_sweep_fill(int len, int ngroup, const int* gmeta, const unsigned* scnt, const unsigned* src,

@@ -158,11 +158,25 @@ struct Server {
       joining.assign(ready_q.begin(), ready_q.end());
       ready_q.clear();
     }
+    // a request that cannot join (e.g. no device memory for its solve buffers) fails alone; the others join
+    int failed = 0;
     for (Request* r : joining) {
       const int rc = dfm_batch_add(batch, r->sys, r->tol, r->max_iter);
-      if (rc != DFM_OK) return rc;
+      if (rc != DFM_OK) {
+        std::lock_guard<std::mutex> lk(mu);
+        r->state = Q_FAILED;
+        r->err = dfm_last_error();
+        r->t_done = now_s();
+        active--;
+        failed++;
+        continue;
+      }
       r->state = Q_SOLVING;
       solving.push_back(r);
+    }
+    if (failed) {
+      std::lock_guard<std::mutex> lk(mu);
+      admit_locked();
     }
     if (solving.empty()) return DFM_OK;
     int nrun = 0;
@@ -265,12 +279,8 @@ int dfm_server_submit(dfm_server s, const dfm_request* q, long long* ticket) {
   const size_t np = q->pstride ? (size_t)q->pstride * q->nelem : (size_t)q->npar;
   r->params.assign(q->params, q->params + np);
   r->bc.assign(q->bc, q->bc + (size_t)q->nnode * q->len);
-  if (q->glob) {
-    // the Newmark kinds read (dt, gamma, beta, is_initial), the others at most two values
-    const bool nm = q->kind == DFM_SOLID2D_NEWMARK || q->kind == DFM_SOLID3D_NEWMARK || q->kind == DFM_STVK3D_NEWMARK ||
-                    q->kind == DFM_STVK2D_NEWMARK;
-    r->glob.assign(q->glob, q->glob + (nm ? 4 : 2));
-  }
+  // exactly the kind's own scalars, as assembly reads them
+  if (q->glob) r->glob.assign(q->glob, q->glob + kind_glob_count(q->kind));
   if (q->nface > 0) {
     DFM_CHECK(q->faces && q->trac, DFM_ERR_ARG, "dfm_server_submit: faces without tractions");
     r->faces.assign(q->faces, q->faces + (size_t)q->nface * 3);

@@ -918,8 +918,9 @@ static int pcg_iteration(Sys& s, const PcgBufs& v, bool mixed, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? DFM_OK : DFM_ERR_HIP;
 }
 
-// Every device pointer an iteration's launches take: a captured graph is valid while these are unchanged
-// (re-ordering, re-factoring or growing a work buffer may move them).
+// Every device pointer an iteration's launches take, and the generation of the schedule whose launch shapes
+// the capture baked in: a captured graph is valid while these are unchanged (re-ordering, a new fill level,
+// re-factoring or growing a work buffer may move them).
 static std::vector<const void*> graph_key(const Sys& s, const PcgBufs& v) {
   std::vector<const void*> k{v.crs, v.dia, v.r, v.z, v.p, v.q, v.x, v.x64, v.b64, s.d_crs.p, s.d_dia.p,
                              s.d_rowptr.p, s.d_colind.p, s.d_partial.p, s.d_part_sw.p, s.d_scal.p, s.d_iflag.p,
@@ -931,6 +932,7 @@ static std::vector<const void*> graph_key(const Sys& s, const PcgBufs& v) {
       k.push_back(p);
   k.push_back((const void*)(size_t)v.f32ilu);
   k.push_back((const void*)s.fwd.d_fuse_bgrp.p);
+  k.push_back((const void*)(size_t)s.struct_gen);
   return k;
 }
 

@@ -276,7 +276,9 @@ int dfm_sys_last_solve_info(dfm_sys sys, int* reliable_updates, int* fell_back);
  * the forward and backward sweeps), 3 ILU apply FP32, 4 / 5 the sweep phases of a PCG iteration as the mixed
  * (FP32) / FP64 loop runs them (forward phases with the r and x update, backward phases with (r, z)), 6 the
  * MakeResidual kernel of a saved system (it rewrites the residual and update handles as dfm_sys_make_residual
- * does). Also reports the algorithmic bytes of one launch (all phases). No counterpart in the reference. */
+ * does), 7 the numeric ILU factorisation of the assembled matrix (its initial values and the factor kernels of
+ * every phase; it rewrites the same factor). Also reports the algorithmic bytes of one launch (all phases). No
+ * counterpart in the reference. */
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes);
 /* ---- batches: many systems solved together (serving, instance batches) ------------------------------
  * A batch advances every system added to it with one launch per kernel for all running systems of one block
@@ -316,7 +318,7 @@ typedef struct {
   const int* lnods;             /* [nelem*(ndim+1)]                                                    */
   const double* params;         /* npar per element (pstride) or shared (pstride = 0)                  */
   int npar, pstride;
-  const double* glob;           /* 2 values (4 for the Newmark kinds) or NULL                          */
+  const double* glob;           /* the kind's scalars (see the kinds above) or NULL; exactly those are read */
   const int* bc;                /* [nnode*len] Dirichlet flags                                         */
   int nface;                    /* surface loads (3D), may be 0                                        */
   const int* faces;
