@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
 """DOF-solves/s at 1e-8 relative residual, mixed precision vs FP64, with p50/p99 solve latency, on one GPU.
 
-Workload (default): 2D Poisson/diffusion, P1 triangles on the unit square 1024 x 1024 (1,050,625 nodes),
-seeded +-10 % node jitter, per-element conductivity lognormal(0,1), f = 1, u = 0 on the boundary; synthetic
-inputs generated on the host and resident in HBM before timing. One step = one solve of the hot path:
-assembly -> Dirichlet rows/columns -> ILU(0) numeric factorisation -> PCG to a 1e-8 true relative residual.
-`value` is the mixed-precision path (FP32 matrix/factor/SpMV/ILU inside FP64 iterative refinement on the
-true residual); the plain FP64 path is timed beside it on the same system.
+Default workload (the headline line): 3D linear elasticity, P1 tetrahedra from a 96^3 hex grid split 6 tets/hex
+(912,673 nodes, 2,738,019 dofs, 3x3 block-CRS), +-10 % node jitter, per-element E lognormal(0, 0.5), nu = 0.3,
+bottom face clamped, seeded traction on the top face; synthetic inputs generated on the host and resident in HBM
+before timing. One step = one solve of the hot path: assembly -> Dirichlet rows/columns -> ILU(0) numeric
+factorisation -> PCG to a 1e-8 relative residual. `value` is the mixed-precision path (FP32 matrix/factor/SpMV/ILU
+inside FP64 iterative refinement, convergence declared on the FP64 true residual); the plain FP64 path is timed
+beside it on the same system, and the error of the mixed solution against the FP64 one is reported.
 
-The roofline object is for the dominant kernel family of the mixed path's inner loop, the ILU(0) sweep phases
-as the PCG iteration runs them (the r/x update folded into the forward phases, (r, z) into the backward ones),
-and `roofline_spmv` for the SELL SpMV, both timed with HIP events on the library's stream; `traffic` is the HBM
-bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/c2_hbm_traffic.md). The CPU baseline
-is the oracle's single-threaded C restatement of the reference, timed on a bounded sample of the same workload
-(see its `sample` field).
+`served` (same line): the serving stream of 2,000 requests (2D Poisson or 2D elasticity, 50/50, dofs log-uniform in
+[2k, 500k]) arriving as a seeded Poisson process above capacity, continuous batching under a 64 GB budget, served
+in mixed precision and in FP64, and the plain FP64 GPU path one request at a time on the first requests; per-request
+error of the mixed result against the FP64 one; p50/p99 latency.
+
+The roofline object is for the dominant kernel family of the mixed path's inner loop, the ILU(0) sweep phases as
+the PCG iteration runs them (the r/x update folded into the forward phases, (r, z) into the backward ones), timed
+with HIP events on the library's stream; `traffic` is the HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE
+(profiles/c3_hbm_traffic.md). The CPU baseline is the reference DelFEM itself (oracle/_ref, compiled from its own
+sources with its own flags) on a bounded sample of the same mesh, or the oracle's C restatement where that build
+is absent (see its `kind` and `sample`).
 """
 import argparse
 import json
@@ -33,18 +39,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel_prefixes):
-    """HBM bytes per dispatch of the named kernels from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    of the C2 workload (profiles/c2_hbm_traffic.md, last column in MB), averaged over the kernels; None if the
-    summary or a kernel is missing."""
-    path = os.path.join(ROOT, "profiles", "c2_hbm_traffic.md")
+def measured_traffic(profile, kernel_prefixes):
+    """HBM bytes per dispatch of the named kernels from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/<profile>, last column in MB), averaged over the kernels; None if the summary or a kernel is
+    missing."""
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
     vals = {}
     for line in open(path):
         if line.startswith("| `"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            vals[cells[0].strip("`")] = float(cells[-1]) * 1e6
+            try:
+                vals[cells[0].strip("`")] = float(cells[-1]) * 1e6
+            except ValueError:
+                continue
     got = []
     for k in kernel_prefixes:
         m = [v for name, v in vals.items() if name.startswith(k)]
@@ -54,12 +63,83 @@ def measured_traffic(kernel_prefixes):
     return sum(got) / len(got)
 
 
-def cpu_baseline(w, gpu_iters, budget_s):
-    """The oracle (plain-C port of the reference arithmetic, one thread) on the full mesh: setup
-    (pattern, assembly, Dirichlet, ILU(0) in the reference's natural order) and a bounded number of PCG
-    iterations; the solve time is extrapolated with the iteration count of the GPU FP64 solve."""
+def _ref_params(w):
+    """One uniform material for the reference's per-call parameters (its element arrays take one set each):
+    E = 1, nu = 0.3 (plane stress in 2D), a body force giving a non-zero right-hand side; the work per element and
+    per iteration does not depend on the values."""
+    from delfem_amd import workloads as W
+
+    if w["kind"] == 2:  # 3D elasticity
+        lam, mu = W.lame3d(1.0, 0.3)
+        return [lam, mu, 1.0, 0.0, 0.0, -0.01]
+    if w["kind"] == 1:  # 2D elasticity
+        lam, mu = W.plane_stress(1.0, 0.3)
+        return [lam, mu, 1.0, 0.0, -0.01]
+    return [1.0, 1.0]  # Poisson: alpha, source
+
+
+class _StdoutToStderr:
+    """Sends what native code writes to file descriptor 1 to descriptor 2 (the reference library prints progress
+    lines), so that standard output carries only the result line."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
+def _cpu_reference(w, gpu_iters, budget_s):
+    """The reference DelFEM (oracle/_ref: its own sources and compiler flags, driven through its public API) on
+    the full mesh: CLinearSystem_Field pattern, AddLinSys_* + FinalizeMarge, then CPreconditioner_ILU (natural
+    order) + Solve_PCG for a bounded number of iterations, twice, the per-iteration time from the difference; the
+    solve is extrapolated to the GPU FP64 iteration count."""
     import numpy as np
 
+    from oracle import pyref as P
+
+    par = np.array(_ref_params(w))
+    with _StdoutToStderr():
+        t0 = time.perf_counter()
+        ref = P.RefSystem(w["ndim"], w["coords"], w["lnods"], w["len"])
+        t1 = time.perf_counter()
+        ref.set_bc(w["bc"])
+        ref.assemble(w["kind"], par)
+        t2 = time.perf_counter()
+        k1 = 2
+        a = time.perf_counter()
+        ref.solve("pcg", 1e-30, k1)
+        t_k1 = time.perf_counter() - a
+        k2 = k1 + int(max(4, min(60, budget_s / max(t_k1 / k1, 1e-6))))
+        a = time.perf_counter()
+        ref.solve("pcg", 1e-30, k2)
+        t_k2 = time.perf_counter() - a
+        t_iter = (t_k2 - t_k1) / (k2 - k1)
+        t_factor = max(0.0, t_k1 - k1 * t_iter)
+        total = (t1 - t0) + (t2 - t1) + t_factor + gpu_iters * t_iter
+        ndof = ref.nnode * ref.len
+        del ref
+    return {
+        "value": ndof / total, "unit": "DOF-solves/s", "cores": 1, "kind": "reference",
+        "sample": (f"{w['name']} full mesh ({ndof} dofs), reference DelFEM (oracle/_ref, g++ -O2 as its Makefile), one "
+                   f"thread, one uniform material (its element arrays take one parameter set each): pattern "
+                   f"{t1 - t0:.2f}s, AddLinSys+FinalizeMarge {t2 - t1:.2f}s, ILU(0) natural {t_factor:.2f}s, Solve_PCG "
+                   f"{t_iter * 1e3:.1f} ms/iteration from runs of {k1} and {k2} iterations; solve extrapolated to "
+                   f"{gpu_iters} iterations (GPU FP64 count) -> {total:.1f}s per solve"),
+        "ms_per_pcg_iter": t_iter * 1e3, "solve_s_extrapolated": total,
+    }
+
+
+def _cpu_port(w, gpu_iters, budget_s):
+    """The oracle (plain-C restatement of the reference arithmetic, one thread) on the full mesh: setup (pattern,
+    assembly, Dirichlet, ILU(0) in the reference's natural order) and a bounded number of PCG iterations; the solve
+    time is extrapolated with the iteration count of the GPU FP64 solve."""
     from oracle import pyref as P
 
     t0 = time.perf_counter()
@@ -81,16 +161,28 @@ def cpu_baseline(w, gpu_iters, budget_s):
     total = (t1 - t0) + (t2 - t1) + t_pcg0 + gpu_iters * t_iter
     ndof = cpu.nnode * cpu.len
     return {
-        "value": ndof / total,
-        "unit": "DOF-solves/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"{w['name']} full mesh ({ndof} dofs): pattern+assembly+BC {t1 - t0:.2f}s, ILU(0) natural "
-                   f"{t2 - t1:.2f}s, {k2} PCG iterations at {t_iter * 1e3:.1f} ms/iter; solve extrapolated to "
-                   f"{gpu_iters} iterations (GPU FP64 count) -> {total:.1f}s per solve"),
-        "ms_per_pcg_iter": t_iter * 1e3,
-        "solve_s_extrapolated": total,
+        "value": ndof / total, "unit": "DOF-solves/s", "cores": 1, "kind": "port",
+        "sample": (f"{w['name']} full mesh ({ndof} dofs), the oracle's C restatement: pattern+assembly+BC "
+                   f"{t1 - t0:.2f}s, ILU(0) natural {t2 - t1:.2f}s, {k2} PCG iterations at {t_iter * 1e3:.1f} ms/iter; "
+                   f"solve extrapolated to {gpu_iters} iterations (GPU FP64 count) -> {total:.1f}s per solve"),
+        "ms_per_pcg_iter": t_iter * 1e3, "solve_s_extrapolated": total,
     }
+
+
+def cpu_baseline(w, gpu_iters, budget_s):
+    """The reference build where it is present (oracle/_ref), else the oracle's restatement."""
+    from oracle import pyref as P
+
+    note = None
+    if P.ref_available():
+        try:
+            return _cpu_reference(w, gpu_iters, budget_s)
+        except Exception as e:  # e.g. the reference library does not load on this host
+            note = f"reference build unusable ({e}); "
+    out = _cpu_port(w, gpu_iters, budget_s)
+    if note:
+        out["sample"] = note + out["sample"]
+    return out
 
 
 def run_c5_batch(args, N, W):
@@ -105,6 +197,8 @@ def run_c5_batch(args, N, W):
     ctx = N.Context(0)
     t0 = time.perf_counter()
     base = W.build_system(ctx, w, N.ORDER_BLOCKCOLOR)
+    if args.lev:
+        base.set_fill_level(args.lev)  # clones take the fill level and its schedule
     systems = [base] + [base.clone() for _ in range(args.instances - 1)]
     scales = 0.001 + (0.02 - 0.001) * N.rng_uniform(W.SEEDS["c5"], 7, args.instances)
     log(f"[bench] c5 x{args.instances} ndof={base.ndof} setup {time.perf_counter() - t0:.1f}s")
@@ -138,7 +232,8 @@ def run_c5_batch(args, N, W):
         "config": {"workload": f"c5: 3D StVK cantilever 64x16x16 hexes x 6 tets, {ndof} dofs, "
                                f"{args.instances} instances, {args.load_steps} load steps",
                    "instances": args.instances, "load_steps": args.load_steps, "newton_tol": 1e-6,
-                   "pcg_tol": 1e-6, "precision": "DFM_MIXED_AUTO (FP32 ILU(0) factor, FP64 loop)"},
+                   "pcg_tol": 1e-6, "ilu_fill_level": args.lev,
+                   "precision": "DFM_MIXED_AUTO (FP32 ILU factor, FP64 loop)"},
         "fp64_value": work / res["fp64"]["t"],
         "mixed": res["mixed"], "fp64": res["fp64"],
     }
@@ -148,19 +243,28 @@ def run_c5_batch(args, N, W):
     print(json.dumps(out), flush=True)
 
 
-def run_c4_plain(N, W, ctx, n):
+def c4_requests(W, n):
+    """Requests 0..n-1 of the serving stream, generated on the host before any clock starts (a client holds
+    them), several at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        return list(ex.map(W.c4_request, range(n)))
+
+
+def run_c4_plain(N, ctx, reqs, n):
     """The plain FP64 GPU path on requests 0..n-1 of the stream, one at a time as a single-system user runs it:
-    pattern, block-multicolour ordering, Dirichlet flags, assembly, ILU(0) and FP64 PCG to a 1e-8 relative
-    residual. Request generation (the mesh) is outside the timed calls."""
+    pattern, block-multicolour ordering, Dirichlet flags, assembly, ILU(0) and FP64 PCG (the reference's recursive
+    residual test) to 1e-8; DOF-solves/s counts the requests whose true residual met 1e-8."""
     import numpy as np
 
     if n <= 0:
         return None
     t_all = t_solve = 0.0
     dofs = 0
-    iters = []
+    iters, trues = [], []
     for i in range(n):
-        r = W.c4_request(i)
+        r = reqs[i]
         ctx.synchronize()
         t0 = time.perf_counter()
         s = N.System(ctx, r["ndim"], r["coords"], r["lnods"], r["len"])
@@ -174,122 +278,156 @@ def run_c4_plain(N, W, ctx, n):
         ok, conv, it = s.solve_pcg(1e-8, 20000, N.FP64)
         ctx.synchronize()
         t2 = time.perf_counter()
-        if not ok:
-            raise RuntimeError(f"plain FP64 request {i} did not converge ({conv}, {it})")
+        rel = s.true_residual()  # outside the timed calls
         s.close()
         t_all += t2 - t0
         t_solve += t2 - t1
-        dofs += r["ndof"]
+        trues.append(rel)
+        if ok and rel <= 1e-8 * (1.0 + 1e-9):
+            dofs += r["ndof"]
         iters.append(it)
     return {"requests": n, "dofs_per_s": dofs / t_all, "solve_dofs_per_s": dofs / t_solve, "wall_s": t_all,
-            "solve_s": t_solve, "total_dofs": float(dofs), "mean_pcg_iters": float(np.mean(iters))}
+            "solve_s": t_solve, "solved_dofs": float(dofs), "met_tolerance": int(np.sum(np.array(trues) <= 1e-8 * (1 + 1e-9))),
+            "max_true_rel_residual": float(np.max(trues)), "mean_pcg_iters": float(np.mean(iters))}
 
 
-def run_c4_serving(args, N, W):
-    """Serving stream: requests (2D Poisson or 2D elasticity, 50/50, dofs log-uniform in [2k, 500k]) arriving as
-    a seeded Poisson process at --rate requests/s; continuous batching under a 64 GB admission budget, mixed
-    precision to a 1e-8 true relative residual. Reports aggregate DOF-solves/s (sum of solved dofs over the time
-    from the first arrival to the last completion) and p50/p99 latency (arrival to completion)."""
-    import queue
-    import threading
-
+def serve_stream(N, ctx, reqs, arrivals, slots, precision, ref=None, keep=False):
+    """The requests submitted at their arrival times (s from the start) to a continuous-batching server of the
+    given precision (64 GB admission budget); every result is fetched when it finishes. Aggregate DOF-solves/s
+    over the time from the first arrival to the last completion, latency from arrival to completion. ref: the
+    FP64 solutions of the same requests, for the relative error of each result; keep: return the solutions."""
     import numpy as np
 
-    n = args.requests
-    # rate 0: every request arrives at once (the server's capacity); the requests (meshes) are then generated
-    # before the clock starts, as a client would hold them; with a rate they are generated ahead of their arrival
-    arrivals = W.c4_arrivals(n, args.rate) if args.rate > 0 else np.zeros(n)
-    q = queue.Queue(maxsize=n if args.rate <= 0 else 256)
-
-    def gen():
-        for i in range(n):
-            q.put((i, W.c4_request(i)))
-
-    if args.rate <= 0:
-        gen()
-    else:
-        threading.Thread(target=gen, daemon=True).start()
-    ctx = N.Context(0)
+    n = len(reqs)
     nworkers = max(1, min(16, (os.cpu_count() or 8) - 2))
-    # mixed precision per request: FP32 matrix/factor/vectors for the scalar (Poisson) requests, the FP32 factor
-    # in the FP64 loop for the elasticity ones (their reliable-update cycles need ~1.4x the FP64 iterations)
-    srv = N.Server(ctx, N.MIXED_AUTO, 64 << 30, max_active=args.slots, nworkers=nworkers)
-    log(f"[bench] c4: {n} requests at {args.rate}/s, {args.slots} slots, {nworkers} setup workers")
-    lat, solve, iters, dofs, fb, kinds = [], [], [], [], [], []
+    srv = N.Server(ctx, precision, 64 << 30, max_active=slots, nworkers=nworkers)
+    tol = 1e-8
+    lat, iters, dofs, fb, trues, met, errs = [], [], [], [], [], [], []
+    sols = {} if keep or ref is not None else None
     inflight = {}
     nxt = 0
     t0 = time.perf_counter()
     last_done = t0
     max_active = max_queue = 0
+    tc = {"submit": 0.0, "progress": 0.0, "poll": 0.0, "fetch": 0.0}
     while nxt < n or inflight:
         now = time.perf_counter() - t0
+        c0 = time.perf_counter()
         while nxt < n and arrivals[nxt] <= now:
-            i, r = q.get()
-            ts = time.perf_counter() - t0
-            t = srv.submit(r["ndim"], r["len"], r["kind"], r["coords"], r["lnods"], r["params"], r["bc"], tol=1e-8)
-            inflight[t] = (arrivals[i], ts, r["ndof"], r["kind"])
+            r = reqs[nxt]
+            t = srv.submit(r["ndim"], r["len"], r["kind"], r["coords"], r["lnods"], r["params"], r["bc"], tol=tol)
+            inflight[t] = (nxt, time.perf_counter() - t0)
             nxt += 1
+        c1 = time.perf_counter()
         active, queued = srv.progress(16, 1)
+        c2 = time.perf_counter()
+        tc["submit"] += c1 - c0
+        tc["progress"] += c2 - c1
         max_active, max_queue = max(max_active, active), max(max_queue, queued)
         for t in list(inflight):
+            c3 = time.perf_counter()
             info = srv.poll(t)
+            tc["poll"] += time.perf_counter() - c3
             if info["state"] < N.REQ_CONVERGED:
                 continue
-            arr, ts, nd, kind = inflight.pop(t)
-            if info["state"] != N.REQ_CONVERGED:
-                raise RuntimeError(f"request failed: {info}")
-            lat.append(ts - arr + info["latency_s"])
-            solve.append(info["solve_s"])
-            iters.append(info["iters"])
-            dofs.append(nd)
-            fb.append(info["fell_back"])
-            kinds.append(kind)
+            i, ts = inflight.pop(t)
+            if info["state"] not in (N.REQ_CONVERGED, N.REQ_MAXITER):
+                raise RuntimeError(f"request {i} failed: {info}")
+            c3 = time.perf_counter()
+            x = srv.result(t, reqs[i]["ndof"])
+            rel = srv.true_residual(t)
             srv.release(t)
+            tc["fetch"] += time.perf_counter() - c3
             last_done = time.perf_counter()
+            lat.append(ts - arrivals[i] + info["latency_s"])
+            iters.append(info["iters"])
+            dofs.append(reqs[i]["ndof"])
+            fb.append(info["fell_back"])
+            trues.append(rel)
+            met.append(rel <= tol * (1.0 + 1e-9))
+            if keep or ref is not None:
+                sols[i] = x
         if nxt < n and not inflight:
             time.sleep(max(0.0, min(0.001, arrivals[nxt] - (time.perf_counter() - t0))))
     wall = last_done - t0 - arrivals[0]
+    stats = srv.stats()
     srv.close()
-    lat_s = sorted(lat)
-    plain = run_c4_plain(N, W, ctx, n if args.plain < 0 else min(args.plain, n))
-    ctx.close()
-
-    def pct(v, qq):
-        return float(np.percentile(v, qq))
-
+    if ref is not None:  # after the clock: the relative error of every result against its FP64 solution
+        for i, x in sols.items():
+            errs.append(float(np.linalg.norm(x - ref[i]) / max(np.linalg.norm(ref[i]), 1e-300)))
+        if not keep:
+            sols = None
     total = float(np.sum(dofs))
+    solved = float(np.sum(np.array(dofs)[np.array(met)]))
+    # DOF-solves/s counts the requests whose true residual met the tolerance; the others' time still counts
+    out = {"dofs_per_s": solved / wall, "dofs_per_s_all_results": total / wall, "wall_s": wall, "total_dofs": total,
+           "met_tolerance": int(np.sum(met)), "results": len(met),
+           "max_true_rel_residual": float(np.max(trues)), "median_true_rel_residual": float(np.median(trues)),
+           "p50_ms": float(np.percentile(lat, 50)) * 1e3, "p99_ms": float(np.percentile(lat, 99)) * 1e3,
+           "mean_pcg_iters": float(np.mean(iters)),
+           "fell_back_to_fp64": int(np.sum(fb)), "max_active": max_active, "max_queued": max_queue,
+           "setup_workers": nworkers, "server_host_time": stats, "client_host_time": tc}
+    if errs:
+        out["error_vs_fp64"] = {"max": float(np.max(errs)), "p99": float(np.percentile(errs, 99)),
+                                "median": float(np.median(errs)), "results": len(errs)}
+    return out, sols
+
+
+def served_object(args, N, W, ctx):
+    """The serving stream, served in FP64 and in mixed precision (DFM_MIXED_AUTO per request), and the plain FP64
+    path one request at a time on the first --plain requests."""
+    import numpy as np
+
+    n = args.requests
+    t0 = time.perf_counter()
+    reqs = c4_requests(W, n)
+    arrivals = W.c4_arrivals(n, args.rate) if args.rate > 0 else np.zeros(n)
+    total = float(sum(r["ndof"] for r in reqs))
+    log(f"[bench] c4: {n} requests ({total / 1e6:.1f} M dofs) generated in {time.perf_counter() - t0:.1f}s, "
+        f"arrivals {args.rate}/s, {args.slots} slots")
+    fp64, sols = serve_stream(N, ctx, reqs, arrivals, args.slots, N.FP64, keep=True)
+    log(f"[bench] c4 served fp64: {fp64['dofs_per_s'] / 1e6:.2f} M DOF-solves/s over {fp64['wall_s']:.1f}s, "
+        f"p50 {fp64['p50_ms']:.0f} ms, p99 {fp64['p99_ms']:.0f} ms, {fp64['met_tolerance']}/{n} met 1e-8 (max true "
+        f"residual {fp64['max_true_rel_residual']:.2e})")
+    mixed, _ = serve_stream(N, ctx, reqs, arrivals, args.slots, N.MIXED_AUTO, ref=sols)
+    del sols
+    log(f"[bench] c4 served mixed: {mixed['dofs_per_s'] / 1e6:.2f} M DOF-solves/s over {mixed['wall_s']:.1f}s, "
+        f"p50 {mixed['p50_ms']:.0f} ms, p99 {mixed['p99_ms']:.0f} ms, {mixed['met_tolerance']}/{n} met 1e-8 (max true "
+        f"residual {mixed['max_true_rel_residual']:.2e}), error vs fp64 max {mixed['error_vs_fp64']['max']:.2e}")
+    plain = run_c4_plain(N, ctx, reqs, min(args.plain, n) if args.plain >= 0 else n)
     out = {
-        "metric": "DOF-solves/s at 1e-8 rel. residual (serving, mixed precision per request); p50/p99 request latency",
-        "value": total / wall,
-        "unit": "DOF-solves/s",
-        "n_gpus": 1, "steps": n, "warmup": 0,
-        "ms_per_step": wall / n * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"c4: {n} requests, Poisson arrivals {args.rate}/s, 2D Poisson or 2D elasticity "
-                               f"50/50, dofs log-uniform [2k, 500k], 64 GB budget, {args.slots} slots",
-                   "requests": n, "rate_per_s": args.rate, "slots": args.slots, "setup_workers": nworkers},
-        "p50_ms": pct(lat_s, 50) * 1e3,
-        "p99_ms": pct(lat_s, 99) * 1e3,
-        "p50_solve_ms": pct(solve, 50) * 1e3,
-        "wall_s": wall,
+        "workload": (f"c4: {n} requests, seeded Poisson arrivals at {args.rate}/s, 2D Poisson (unit square, lognormal(0,1) "
+                     f"conductivity) or 2D plane-stress elasticity (5:1 strip), 50/50, dofs log-uniform in [2k, 500k], "
+                     f"64 GB admission budget, {args.slots} slots, 1e-8 relative residual"),
+        "requests": n, "rate_per_s": args.rate, "total_dofs": total,
         "offered_dofs_per_s": total / (arrivals[-1] - arrivals[0]) if n > 1 and args.rate > 0 else None,
-        "total_dofs": total,
-        "mean_pcg_iters": float(np.mean(iters)),
-        "fell_back_to_fp64": int(np.sum(fb)),
-        "max_active": max_active, "max_queued": max_queue,
+        "value": mixed["dofs_per_s"], "unit": "DOF-solves/s", "p50_ms": mixed["p50_ms"], "p99_ms": mixed["p99_ms"],
+        "mixed": mixed, "fp64": fp64,
+        "mixed_over_served_fp64": mixed["dofs_per_s"] / fp64["dofs_per_s"],
+        "precision": "DFM_MIXED_AUTO (Poisson: FP32 matrix/factor/vectors with FP64 reliable updates; elasticity: "
+                     "the FP32 factor in the FP64 loop); conv_ratio is the true relative residual",
     }
     if plain:
-        # the same first requests, served: their share of the served run is not separable, so the comparison is
-        # of whole-run rates; with every request solved both ways the rates cover the same work
         out["plain_fp64"] = plain
-        out["served_vs_plain_fp64"] = (total / wall) / plain["dofs_per_s"]
-    log(f"[bench] c4: {total / wall / 1e6:.2f} M DOF-solves/s over {wall:.1f}s, p50 {out['p50_ms']:.0f} ms, "
-        f"p99 {out['p99_ms']:.0f} ms, max active {max_active}, max queued {max_queue}, fell back {int(np.sum(fb))}")
-    if plain:
+        out["mixed_over_plain_fp64"] = mixed["dofs_per_s"] / plain["dofs_per_s"]
         log(f"[bench] c4 plain FP64, one request at a time ({plain['requests']} requests): "
-            f"{plain['dofs_per_s'] / 1e6:.2f} M DOF-solves/s end to end, {plain['solve_dofs_per_s'] / 1e6:.2f} M in "
-            f"the solves alone; served / plain = {out['served_vs_plain_fp64']:.2f}")
+            f"{plain['dofs_per_s'] / 1e6:.2f} M DOF-solves/s end to end; served mixed / plain = "
+            f"{out['mixed_over_plain_fp64']:.2f}, served mixed / served fp64 = {out['mixed_over_served_fp64']:.2f}")
+    return out
+
+
+def run_c4_serving(args, N, W):
+    """The serving stream alone: the served object as the line."""
+    ctx = N.Context(0)
+    sv = served_object(args, N, W, ctx)
+    ctx.close()
+    out = {
+        "metric": "DOF-solves/s at 1e-8 rel. residual (serving, mixed precision per request); p50/p99 request latency",
+        "value": sv["value"], "unit": "DOF-solves/s", "n_gpus": 1, "steps": args.requests, "warmup": 0,
+        "ms_per_step": sv["mixed"]["wall_s"] / args.requests * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic", "config": {"workload": sv["workload"]},
+        "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"], "served": sv,
+    }
     print(json.dumps(out), flush=True)
 
 
@@ -498,60 +636,9 @@ def run_newmark(args, N, W, order_kind, w, ctx):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--ordering", default="blockcolor", choices=["natural", "multicolor", "blockcolor"])
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--newmark", action="store_true",
-                    help="c1/c3: Newmark-beta time steps of linear elastodynamics on the workload's geometry")
-    ap.add_argument("--saved", action="store_true",
-                    help="c1/c2/c3: saved-stiffness re-solves with new Dirichlet values per step")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--max-iter", type=int, default=20000)
-    ap.add_argument("--instances", type=int, default=256, help="c5: instances in the batch")
-    ap.add_argument("--load-steps", type=int, default=10, help="c5: load steps")
-    ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
-    ap.add_argument("--rate", type=float, default=100.0, help="c4: arrival rate (requests/s)")
-    ap.add_argument("--slots", type=int, default=64, help="c4: systems solved at once")
-    ap.add_argument("--plain", type=int, default=-1,
-                    help="c4: requests (the first ones of the stream) also solved by the plain FP64 path one at a "
-                         "time; -1: all, 0: none")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU: every GPU box of this build has one; --gpus {args.gpus} is not "
-              f"supported (requests and batches shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
-    from delfem_amd import _native as N
-    from delfem_amd import workloads as W
-
-    if args.config == "c5":
-        return run_c5_batch(args, N, W)
-    if args.config == "c4":
-        return run_c4_serving(args, N, W)
-    t0 = time.perf_counter()
-    w = W.make(args.config)
-    ctx = N.Context(0)
-    order_kind = {"natural": N.ORDER_NATURAL, "multicolor": N.ORDER_MULTICOLOR,
-                  "blockcolor": N.ORDER_BLOCKCOLOR}[args.ordering]
-    if args.saved:
-        return run_saved(args, N, W, order_kind, w, ctx)
-    if args.newmark:
-        if w["len"] == 1:
-            print("--newmark needs a vector workload (c1 or c3)", file=sys.stderr)
-            sys.exit(2)
-        return run_newmark(args, N, W, order_kind, w, ctx)
-    s = W.build_system(ctx, w, order_kind)
-    s.set_params(w["params"])
-    s.set_loads(w["faces"], w["trac"])
-    nph = s.schedule_info()
-    ndof = s.ndof
-    log(f"[bench] {w['name']} ndof={ndof} ncrs={s.ncrs} ordering={args.ordering} phases={nph[:3]} "
-        f"setup {time.perf_counter() - t0:.1f}s")
+def solve_legs(args, N, s, w, ctx):
+    """Timed steps (assembly + Dirichlet + ILU factor + PCG) in mixed precision and in FP64; the solution of each
+    leg's last step is kept for the error of the mixed result against the FP64 one."""
     tol = w["tol"]
 
     def step(precision):
@@ -561,8 +648,7 @@ def main():
         s.factor()
         ok, conv, it = s.solve_pcg(tol, args.max_iter, precision)
         ctx.synchronize()
-        b = time.perf_counter()
-        return b - a, ok, conv, it
+        return time.perf_counter() - a, ok, conv, it
 
     res = {}
     for prec, name in [(N.MIXED, "mixed"), (N.FP64, "fp64")]:
@@ -580,55 +666,73 @@ def main():
         st = s.last_solve_stats()
         nrel, fell_back = s.last_solve_info()
         res[name] = dict(lat=lat, its=its, ok=ok_all, conv=conv, true_rel=true_rel, stats=st, nrel=nrel,
-                         fell_back=fell_back)
+                         fell_back=fell_back, x=s.get_vector(-2))
         log(f"[bench] {name}: {statistics.median(lat) * 1e3:.1f} ms/solve, iters {its[-1]}, conv {conv:.2e}, "
             f"true rel residual {true_rel:.2e}, solve {st['ms']:.1f} ms, {st['ms_per_iter'] * 1e3:.1f} us/iter"
             + (f", reliable updates {nrel}" if name == "mixed" else "") + (", fell back to FP64" if fell_back else ""))
+    return res
 
-    # kernels of the mixed path's inner loop, timed with HIP events on the library's stream
+
+def pct(v, q):
+    v = sorted(v)
+    if len(v) == 1:
+        return v[0]
+    i = q * (len(v) - 1)
+    lo = int(i)
+    hi = min(lo + 1, len(v) - 1)
+    return v[lo] + (v[hi] - v[lo]) * (i - lo)
+
+
+WORKLOAD_NAMES = {
+    "c1": "2D plane-stress elasticity P1, rectangle 1.0x0.2, 128x32 quads x 2 tris, E=1, nu=0.3, left edge clamped",
+    "c2": "2D Poisson P1, unit square 1024x1024 quads x 2 tris, lognormal(0,1) conductivity, f=1, u=0 on the boundary",
+    "c3": "3D linear elasticity P1 tets, unit cube 96^3 hexes x 6, E lognormal(0,0.5), nu=0.3, bottom clamped, "
+          "seeded top traction",
+}
+
+
+def run_single(args, N, W, w, ctx, order_kind):
+    """One system of a configuration: timed mixed and FP64 steps, kernel timings, roofline, CPU baseline; with
+    the serving stream beside it unless --no-served."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    s = W.build_system(ctx, w, order_kind)
+    s.set_params(w["params"])
+    s.set_loads(w["faces"], w["trac"])
+    nph = s.schedule_info()
+    ndof = s.ndof
+    log(f"[bench] {w['name']} ndof={ndof} ncrs={s.ncrs} ordering={args.ordering} phases={nph[:3]} "
+        f"setup {time.perf_counter() - t0:.1f}s")
+    res = solve_legs(args, N, s, w, ctx)
+    mixed, fp64 = res["mixed"], res["fp64"]
+    err = float(np.linalg.norm(mixed["x"] - fp64["x"]) / np.linalg.norm(fp64["x"]))
+
+    # kernels of the inner loops and the factorisation, timed with HIP events on the library's stream
     spmv32_ms, spmv32_b = s.bench_kernel(1, 50)
     spmv64_ms, spmv64_b = s.bench_kernel(0, 50)
     ilu32_ms, ilu32_b = s.bench_kernel(3, 20)
     ilu64_ms, ilu64_b = s.bench_kernel(2, 20)
-    # the sweep phases as the PCG iteration runs them (r/x update folded into the forward phases, (r, z) into
-    # the backward ones): the dominant kernels of the timed loop
     sw32_ms, sw32_b = s.bench_kernel(4, 20)
     sw64_ms, sw64_b = s.bench_kernel(5, 20)
-    achieved = spmv32_b / (spmv32_ms * 1e-3) / 1e9
-    log(f"[bench] spmv fp32 {spmv32_ms * 1e3:.1f} us ({achieved:.0f} GB/s), spmv fp64 {spmv64_ms * 1e3:.1f} us "
-        f"({spmv64_b / spmv64_ms / 1e6:.0f} GB/s), ilu fp32 {ilu32_ms * 1e3:.1f} us "
-        f"({ilu32_b / ilu32_ms / 1e6:.0f} GB/s), ilu fp64 {ilu64_ms * 1e3:.1f} us ({ilu64_b / ilu64_ms / 1e6:.0f} GB/s), "
-        f"pcg sweep phases fp32 {sw32_ms * 1e3:.1f} us ({sw32_b / sw32_ms / 1e6:.0f} GB/s), "
-        f"fp64 {sw64_ms * 1e3:.1f} us ({sw64_b / sw64_ms / 1e6:.0f} GB/s)")
-
+    fac_ms, fac_b = s.bench_kernel(7, 5)
+    log(f"[bench] spmv fp32 {spmv32_ms * 1e3:.1f} us ({spmv32_b / spmv32_ms / 1e6:.0f} GB/s), spmv fp64 "
+        f"{spmv64_ms * 1e3:.1f} us ({spmv64_b / spmv64_ms / 1e6:.0f} GB/s), pcg sweep phases fp32 {sw32_ms * 1e3:.1f} us "
+        f"({sw32_b / sw32_ms / 1e6:.0f} GB/s), fp64 {sw64_ms * 1e3:.1f} us ({sw64_b / sw64_ms / 1e6:.0f} GB/s), "
+        f"ILU factor {fac_ms:.3f} ms ({fac_b / fac_ms / 1e6:.0f} GB/s); mixed vs fp64 solution error {err:.2e}")
     nlaunch = nph[3]  # sweep launches per PCG iteration (the last forward and first backward phase may be one)
-    sweep_traffic = spmv_traffic = None
-    if args.config == "c2" and args.ordering == "blockcolor":
-        fused = nph[3] < nph[0] + nph[1]
-        kinds = (["dfm::k_fwd_update<1, 4, 4, float"] * (nph[0] - fused) + ["dfm::k_bwd_rz<1, 4, 4, float"] *
-                 (nph[1] - fused) + ["dfm::k_fwd_bwd_tile<1, 4, 4, float"] * fused)
-        sweep_traffic = measured_traffic(kinds)  # per launch, over the iteration's launches
-        spmv_traffic = measured_traffic(["dfm::k_pcg_spmv_sell<1, float, false>"])
-    mixed, fp64 = res["mixed"], res["fp64"]
+    fused = nph[3] < nph[0] + nph[1]
+    B = w["len"]
+    kinds = ([f"dfm::k_fwd_update<{B}, "] * (nph[0] - fused) + [f"dfm::k_bwd_rz<{B}, "] * (nph[1] - fused) +
+             [f"dfm::k_fwd_bwd_tile<{B}, "] * fused)
+    prof = f"{w['name']}_hbm_traffic.md"
+    sweep_traffic = measured_traffic(prof, kinds) if args.ordering == "blockcolor" else None
+    spmv_traffic = measured_traffic(prof, [f"dfm::k_pcg_spmv_sell<{B}, float"]) if args.ordering == "blockcolor" else None
     K = args.steps
-    t_mixed = sum(mixed["lat"])
-    t_fp64 = sum(fp64["lat"])
-    value = ndof * K / t_mixed
-    value64 = ndof * K / t_fp64
-    lat_sorted = sorted(mixed["lat"])
-    # the plain-FP64 configuration reports its FP64 path
-    head = fp64 if args.config == "c1" else mixed
-
-    def pct(v, q):
-        if len(v) == 1:
-            return v[0]
-        i = q * (len(v) - 1)
-        lo = int(i)
-        hi = min(lo + 1, len(v) - 1)
-        return v[lo] + (v[hi] - v[lo]) * (i - lo)
-
-    if head is fp64:
-        value, t_mixed, lat_sorted = value64, t_fp64, sorted(fp64["lat"])
+    head = fp64 if args.config == "c1" else mixed  # the plain-FP64 configuration reports its FP64 path
+    t_head = sum(head["lat"])
+    value = ndof * K / t_head
+    value64 = ndof * K / sum(fp64["lat"])
     out = {
         "metric": METRIC,
         "value": value,
@@ -636,39 +740,38 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": t_mixed / K * 1e3,
+        "ms_per_step": t_head / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if head is fp64 else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"{w['name']}: " + {"c1": "2D plane-stress elasticity P1, 128x32 quads",
-                                             "c2": "2D Poisson P1, unit square 1024x1024, lognormal(0,1) conductivity",
-                                             "c3": "3D elasticity P1 tets, 96^3 hexes x 6",
-                                             "c5": "3D StVK cantilever, 64x16x16 hexes x 6 (first Newton step)"}[
-                             w["name"]]),
+            "workload": f"{w['name']}: {WORKLOAD_NAMES[w['name']]}",
             "dofs": ndof,
             "nnz_blocks_offdiag": s.ncrs,
-            "block": w["len"],
-            "tol_true_rel_residual": tol,
+            "block": B,
+            "tol_rel_residual": w["tol"],
             "ordering": args.ordering,
             "ilu_phases_fwd_bwd": [nph[0], nph[1]],
             "step": "assemble + Dirichlet + ILU(0) factor + PCG solve",
         },
+        "value_path": "fp64" if head is fp64 else "mixed",
         "fp64_value": value64,
-        "mixed_over_fp64": value / value64,
-        "p50_ms": pct(lat_sorted, 0.5) * 1e3,
-        "p99_ms": pct(lat_sorted, 0.99) * 1e3,
-        "fp64_p50_ms": pct(sorted(fp64["lat"]), 0.5) * 1e3,
+        "mixed_value": ndof * K / sum(mixed["lat"]),
+        "mixed_over_fp64": (ndof * K / sum(mixed["lat"])) / value64,
+        "p50_ms": pct(head["lat"], 0.5) * 1e3,
+        "p99_ms": pct(head["lat"], 0.99) * 1e3,
+        "fp64_p50_ms": pct(fp64["lat"], 0.5) * 1e3,
+        "fp64_p99_ms": pct(fp64["lat"], 0.99) * 1e3,
         "iters_mixed_inner": mixed["its"][-1],
         "iters_fp64": fp64["its"][-1],
         "true_rel_residual_mixed": mixed["true_rel"],
         "true_rel_residual_fp64": fp64["true_rel"],
+        "mixed_error_vs_fp64": err,
         "converged": bool(mixed["ok"] and fp64["ok"]),
         "mixed_reliable_updates": mixed["nrel"],
         "mixed_fell_back_to_fp64": mixed["fell_back"],
-        "value_path": "fp64" if head is fp64 else "mixed",
         "ms_per_pcg_iter_mixed": mixed["stats"]["ms_per_iter"],
         "ms_per_pcg_iter_fp64": fp64["stats"]["ms_per_iter"],
         "solve_ms_mixed": mixed["stats"]["ms"],
@@ -678,7 +781,7 @@ def main():
         # algorithmic bytes of the phases and HIP-event time, over its launches
         "roofline": {
             "kernel": "k_fwd_update/k_fwd_bwd_tile/k_bwd_rz<B=%d, FP32> (ILU(0) phase sweeps of the PCG "
-                      "iteration, %d launches)" % (w["len"], nlaunch),
+                      "iteration, %d launches)" % (B, nlaunch),
             "bound": "hbm",
             "achieved": sw32_b / (sw32_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
@@ -689,12 +792,12 @@ def main():
             "algorithmic_bytes": sw32_b / nlaunch,
         },
         "roofline_spmv": {
-            "kernel": "k_pcg_spmv_sell<B=%d, FP32> (SELL-64 SpMV + dot)" % w["len"],
+            "kernel": "k_pcg_spmv_sell<B=%d, FP32> (SELL-64 SpMV + dot)" % B,
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": spmv32_b / (spmv32_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": spmv32_b / (spmv32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic": spmv_traffic,
             "avg_launch_us": spmv32_ms * 1e3,
             "algorithmic_bytes": spmv32_b,
@@ -704,18 +807,74 @@ def main():
             "ilu_apply_fp32": {"us": ilu32_ms * 1e3, "GB/s": ilu32_b / ilu32_ms / 1e6, "bytes": ilu32_b},
             "ilu_apply_fp64": {"us": ilu64_ms * 1e3, "GB/s": ilu64_b / ilu64_ms / 1e6, "bytes": ilu64_b},
             "pcg_sweep_phases_fp64": {"us": sw64_ms * 1e3, "GB/s": sw64_b / sw64_ms / 1e6, "bytes": sw64_b},
+            "ilu_factor_fp64": {"us": fac_ms * 1e3, "GB/s": fac_b / fac_ms / 1e6, "bytes": fac_b},
         },
     }
     s.close()
+    if args.config == "c3" and not args.no_served:
+        out["served"] = served_object(args, N, W, ctx)
     ctx.close()
     if not args.no_cpu:
-        log("[bench] cpu baseline (oracle port, 1 thread) ...")
+        log("[bench] cpu baseline ...")
         try:
             out["cpu_baseline"] = cpu_baseline(w, fp64["its"][-1] + 1, args.cpu_budget)
             out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+            log(f"[bench] cpu baseline: {out['cpu_baseline']['sample']}")
         except Exception as e:  # the baseline is reported, never the measured product
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--ordering", default="blockcolor", choices=["natural", "multicolor", "blockcolor"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="c3: leave out the serving stream")
+    ap.add_argument("--newmark", action="store_true",
+                    help="c1/c3: Newmark-beta time steps of linear elastodynamics on the workload's geometry")
+    ap.add_argument("--saved", action="store_true",
+                    help="c1/c2/c3: saved-stiffness re-solves with new Dirichlet values per step")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--instances", type=int, default=256, help="c5: instances in the batch")
+    ap.add_argument("--load-steps", type=int, default=10, help="c5: load steps")
+    ap.add_argument("--lev", type=int, default=0, help="c5: ILU fill level of the inner solves")
+    ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
+    ap.add_argument("--rate", type=float, default=100.0, help="c4: arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=64, help="c4: systems solved at once")
+    ap.add_argument("--plain", type=int, default=200,
+                    help="c4: requests (the first ones of the stream) also solved by the plain FP64 path one at a "
+                         "time; -1: all, 0: none")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU (--gpus {args.gpus} is not supported): requests and instance batches "
+              f"shard across GPUs with no traffic between them, one large system by domain decomposition, which "
+              f"this program does not run", file=sys.stderr)
+        sys.exit(2)
+
+    from delfem_amd import _native as N
+    from delfem_amd import workloads as W
+
+    if args.config == "c5":
+        return run_c5_batch(args, N, W)
+    if args.config == "c4":
+        return run_c4_serving(args, N, W)
+    w = W.make(args.config)
+    ctx = N.Context(0)
+    order_kind = {"natural": N.ORDER_NATURAL, "multicolor": N.ORDER_MULTICOLOR,
+                  "blockcolor": N.ORDER_BLOCKCOLOR}[args.ordering]
+    if args.saved:
+        return run_saved(args, N, W, order_kind, w, ctx)
+    if args.newmark:
+        if w["len"] == 1:
+            print("--newmark needs a vector workload (c1 or c3)", file=sys.stderr)
+            sys.exit(2)
+        return run_newmark(args, N, W, order_kind, w, ctx)
+    return run_single(args, N, W, w, ctx, order_kind)
 
 
 if __name__ == "__main__":

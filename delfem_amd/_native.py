@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdfm_gpu.
 
 POISSON, SOLID2D, SOLID3D, STVK2D, STVK3D, DIFFUSION2D, SOLID2D_NEWMARK, SOLID3D_NEWMARK, SOLID2D_BEULER, STVK3D_NEWMARK, STVK2D_NEWMARK, STVK2D_BEULER = range(12)
 ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKCOLOR, ORDER_RCM, ORDER_RCM2, ORDER_AMD = range(6)
-FP64, MIXED, FP32_ILU, MIXED_AUTO = 0, 1, 2, 3
+FP64, MIXED, FP32_ILU, MIXED_AUTO, FP64_TRUE = 0, 1, 2, 3, 4
 ERR_SINGULAR = -5
 NPAR = {POISSON: 2, SOLID2D: 5, SOLID3D: 6, STVK2D: 5, STVK3D: 6, DIFFUSION2D: 3, SOLID2D_NEWMARK: 5, SOLID3D_NEWMARK: 6, SOLID2D_BEULER: 5, STVK3D_NEWMARK: 6, STVK2D_NEWMARK: 5, STVK2D_BEULER: 5}
 
@@ -544,6 +544,18 @@ class Server:
                                      C.byref(it), C.byref(conv), C.byref(fb)))
         return dict(state=st.value, latency_s=lat.value, solve_s=sol.value, iters=it.value, conv=conv.value,
                     fell_back=bool(fb.value))
+
+    def stats(self):
+        out = (C.c_double * 12)()
+        _check(lib().dfm_server_stats(self.h, out, 12))
+        keys = ["steps", "s_steps", "s_rebuild", "rebuilds", "s_poll", "restarts", "s_idle", "s_setup", "setups",
+                "s_check", "s_join", "s_progress"]
+        return dict(zip(keys, list(out)))
+
+    def true_residual(self, ticket):
+        rel = C.c_double()
+        _check(lib().dfm_server_true_residual(self.h, C.c_longlong(ticket), C.byref(rel)))
+        return rel.value
 
     def result(self, ticket, ndof):
         u = np.empty(ndof)

@@ -340,6 +340,7 @@ struct BatchSeg {
   Sys* s = nullptr;
   int slot = -1;
   int mode = DFM_MIXED;  // DFM_FP64, DFM_MIXED (a stalled one restarts as DFM_FP32_ILU), DFM_FP32_ILU
+  bool verify = false;   // the true residual decides at exit (DFM_FP32_ILU, DFM_FP64_TRUE, a stalled mixed one)
   bool fell_back = false;
   PcgBufs v{};
   double tol = 0;
@@ -347,12 +348,16 @@ struct BatchSeg {
   int state = 0;  // 0 running, 1 converged, 2 stopped at max_iter
   double conv = 0;
   unsigned iters = 0;
-  unsigned before = 0;  // iterations of the mixed-precision loop before a switch to FP64 vectors
+  unsigned before = 0;  // iterations before the running loop (a mixed loop before its switch, earlier restarts)
+  int restarts = 0;     // restarts of the FP64 loop after its true residual was checked
+  double prev_rel = 0;  // the true residual that led to the last restart
   int nrel = 0;
   double t_add = 0, t_done = 0;  // host clock (s)
 };
 
 struct Batch {
+  double t_rebuild = 0, t_poll = 0;  // host seconds spent in rebuild() and poll()
+  long long n_rebuild = 0, n_restart = 0;
   Ctx* ctx = nullptr;
   int prec = DFM_MIXED;  // mode of added systems (DFM_MIXED_AUTO: by block size)
   int cap = 0;
@@ -399,7 +404,16 @@ static void grow(DBuf<T>& b, size_t n, hipError_t& e) {
   if (b.n < n) e = b.alloc(std::max<size_t>(n, 1));
 }
 
+static int rebuild_tables(Batch& bt);
 static int rebuild(Batch& bt) {
+  const double t0 = now_s();
+  const int rc = rebuild_tables(bt);
+  bt.t_rebuild += now_s() - t0;
+  bt.n_rebuild++;
+  return rc;
+}
+
+static int rebuild_tables(Batch& bt) {
   hipStream_t st = bt.ctx->stream;
   for (auto& g : bt.grp) g.members.clear();
   for (int k = 0; k < (int)bt.segs.size(); k++)
@@ -566,6 +580,7 @@ static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
 static int restart_fp64(Batch& bt, BatchSeg& sg) {
   Sys& s = *sg.s;
   sg.mode = DFM_FP32_ILU;
+  sg.verify = true;
   sg.fell_back = true;
   sg.nrel = bt.h_fl[8 * sg.slot + F_NREL];
   sg.before = (unsigned)bt.h_fl[8 * sg.slot + F_IT] + 1;
@@ -590,6 +605,38 @@ static int finish_seg(Batch& bt, BatchSeg& sg) {
   sg.state = done == 1 ? 1 : 2;
   sg.conv = done ? sc[S_CONV] : std::sqrt(sc[S_RR] * sc[S_INV0]);
   sg.iters = done == 1 ? (unsigned)fl[F_ITOUT] + sg.before : sg.maxit;
+  if (done != 1 && sg.restarts > 0) {
+    // a restarted loop ran out of iterations: report the true residual of its solution
+    const int rc = sys_true_residual(*sg.s, bt.ctx->stream, &sg.conv);
+    if (rc != DFM_OK) return rc;
+  }
+  if (done == 1 && sg.verify) {
+    // the FP64 loop with the FP32 factor stopped on its recursive residual: the true residual decides, and the
+    // loop restarts from its solution (r = b - A x) while it is above the tolerance, as solve_pcg does
+    Sys& s = *sg.s;
+    double rel = 0.0;
+    int rc = sys_true_residual(s, bt.ctx->stream, &rel);
+    if (rc != DFM_OK) return rc;
+    sg.conv = rel;
+    const unsigned used = sg.iters + 1;
+    if (rel > sg.tol) {
+      if (used >= sg.maxit || sg.restarts == kMaxRestarts || (sg.restarts > 0 && rel > kRestartGain * sg.prev_rel)) {
+        sg.state = 2;
+        sg.iters = sg.maxit;
+      } else {
+        sg.restarts++;
+        bt.n_restart++;
+        sg.prev_rel = rel;
+        sg.before = used;
+        sg.state = 0;
+        DFM_HIP(launch_copy(s.d_rhs.p, s.d_res.p, (long long)s.n * s.len, bt.ctx->stream));
+        rc = pcg_start_warm(s, sg.v, 0.5 * sg.tol, sg.maxit - used, bt.d_sc.p + kNScal * (size_t)sg.slot,
+                            bt.d_fl.p + 8 * (size_t)sg.slot, bt.ctx->stream);
+        bt.dirty = true;
+        return rc;
+      }
+    }
+  }
   if (!sg.fell_back) sg.nrel = fl[F_NREL];
   sg.t_done = now_s();
   const int rc = pcg_finish(*sg.s, sg.v, sg.mode == DFM_MIXED, done, bt.ctx->stream);
@@ -597,7 +644,15 @@ static int finish_seg(Batch& bt, BatchSeg& sg) {
   return rc;
 }
 
+static int poll_flags(Batch& bt);
 static int poll(Batch& bt) {
+  const double t0 = now_s();
+  const int rc = poll_flags(bt);
+  bt.t_poll += now_s() - t0;
+  return rc;
+}
+
+static int poll_flags(Batch& bt) {
   hipStream_t st = bt.ctx->stream;
   DFM_HIP(hipMemcpyAsync(bt.h_fl, bt.d_fl.p, sizeof(int) * 8 * bt.cap, hipMemcpyDeviceToHost, st));
   DFM_HIP(hipMemcpyAsync(bt.h_sc, bt.d_sc.p, sizeof(double) * kNScal * bt.cap, hipMemcpyDeviceToHost, st));
@@ -620,6 +675,13 @@ static int find_seg(Batch& bt, Sys* s) {
   return -1;
 }
 
+void batch_stats(const Batch* bt, double* out) {
+  out[0] = bt->t_rebuild;
+  out[1] = (double)bt->n_rebuild;
+  out[2] = bt->t_poll;
+  out[3] = (double)bt->n_restart;
+}
+
 void batch_detach(Batch* bt, Sys* s) {
   if (!bt) return;
   const int k = find_seg(*bt, s);
@@ -640,12 +702,15 @@ struct dfm_batch_s {
   dfm_ctx_s* owner = nullptr;
 };
 
+Batch* dfm::batch_of(dfm_batch b) { return &b->b; }
+
 extern "C" {
 
 int dfm_batch_create(dfm_ctx ctx, int precision, int max_systems, dfm_batch* out) {
   DFM_CHECK(ctx && out, DFM_ERR_ARG, "dfm_batch_create: null argument");
   DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU ||
-                precision == DFM_MIXED_AUTO, DFM_ERR_ARG, "dfm_batch_create: bad precision");
+                precision == DFM_MIXED_AUTO || precision == DFM_FP64_TRUE, DFM_ERR_ARG,
+            "dfm_batch_create: bad precision");
   DFM_CHECK(max_systems > 0 && max_systems <= (1 << 20), DFM_ERR_ARG, "dfm_batch_create: bad max_systems");
   DFM_HIP(hipSetDevice(ctx->c.device));
   auto* h = new dfm_batch_s;
@@ -706,7 +771,8 @@ int dfm_batch_add(dfm_batch b, dfm_sys sys, double tol, unsigned max_iter) {
   BatchSeg sg;
   sg.s = &s;
   sg.slot = bt.free_slots.back();
-  sg.mode = prec;
+  sg.mode = prec == DFM_FP64_TRUE ? DFM_FP64 : prec;  // the FP64 kernels; the check at exit is the difference
+  sg.verify = prec == DFM_FP32_ILU || prec == DFM_FP64_TRUE;
   sg.v = pcg_bufs(s, prec == DFM_MIXED);
   sg.v.f32ilu = prec == DFM_FP32_ILU;
   sg.tol = tol;

@@ -1067,18 +1067,26 @@ int dfm_sys_matvec(dfm_sys sys, double alpha, int iv1, double beta, int iv2) {
   return DFM_OK;
 }
 
-int dfm_sys_get_vector(dfm_sys sys, int iv, double* out) {
-  NEED(sys, "dfm_sys_get_vector");
-  NEED(out, "dfm_sys_get_vector");
-  Sys& s = sys->s;
+}  // extern "C"
+
+// A vector handle into the caller's numbering, read on stream st (the stream whose work last wrote it).
+int dfm::sys_download(Sys& s, int iv, double* out, hipStream_t st) {
   double* a = vec_ptr(s, iv);
   DFM_CHECK(a, DFM_ERR_ARG, "dfm_sys_get_vector: bad vector handle");
-  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
   std::vector<double> v((size_t)mdof(s));
-  DFM_HIP(hipMemcpy(v.data(), a, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+  DFM_HIP(hipMemcpyAsync(v.data(), a, v.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  DFM_HIP(hipStreamSynchronize(st));
   for (int i0 = 0; i0 < s.n; i0++)
     for (int d = 0; d < s.len; d++) out[(size_t)s.order[i0] * s.len + d] = v[(size_t)i0 * s.len + d];
   return DFM_OK;
+}
+
+extern "C" {
+
+int dfm_sys_get_vector(dfm_sys sys, int iv, double* out) {
+  NEED(sys, "dfm_sys_get_vector");
+  NEED(out, "dfm_sys_get_vector");
+  return sys_download(sys->s, iv, out, sys->s.ctx->stream);
 }
 
 int dfm_sys_set_vector(dfm_sys sys, int iv, const double* in) {
@@ -1259,18 +1267,18 @@ static int ensure_fp32(Sys& s) {
 
 namespace dfm {
 int resolve_precision(const Sys& s, int precision) {
-  if (precision == DFM_MIXED_AUTO) return s.len == 1 ? DFM_MIXED : DFM_FP32_ILU;
+  if (precision == DFM_MIXED_AUTO) return s.len == 1 ? DFM_MIXED : DFM_FP64_TRUE;
   return precision;
 }
 
 int sys_solve_setup(Sys& s, int precision) {
   precision = resolve_precision(s, precision);
-  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU, DFM_ERR_ARG,
-            "solve: bad precision");
+  DFM_CHECK(precision == DFM_FP64 || precision == DFM_MIXED || precision == DFM_FP32_ILU || precision == DFM_FP64_TRUE,
+            DFM_ERR_ARG, "solve: bad precision");
   DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "solve: assemble and factor first");
   int rc = ensure_work(s, 2, precision == DFM_MIXED ? 5 : 0);
   if (rc != DFM_OK) return rc;
-  if (precision != DFM_FP64) {
+  if (precision == DFM_MIXED || precision == DFM_FP32_ILU) {
     rc = ensure_fp32(s);
     if (rc != DFM_OK) return rc;
   }
@@ -1307,12 +1315,12 @@ int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
   return solve_pbicgstab(s, conv_ratio, iteration);
 }
 
-int dfm_sys_true_residual(dfm_sys sys, double* rel) {
-  NEED(sys, "dfm_sys_true_residual");
-  NEED(rel, "dfm_sys_true_residual");
-  Sys& s = sys->s;
+}  // extern "C"
+
+// ||b - A x|| / ||b|| in FP64 for b = the assembled right-hand side and x = the update handle, enqueued on st
+// and waited for; the system's own scalar slots and its first work vector are the scratch.
+int dfm::sys_true_residual(Sys& s, hipStream_t st, double* rel) {
   const long long m = mdof(s);
-  hipStream_t st = s.ctx->stream;
   const int rc = ensure_work(s, 1, 0);
   if (rc != DFM_OK) return rc;
   double* t = s.d_work.p;
@@ -1325,6 +1333,14 @@ int dfm_sys_true_residual(dfm_sys sys, double* rel) {
   DFM_HIP(hipStreamSynchronize(st));
   *rel = s.h_pinned_d[1] > 0 ? std::sqrt(s.h_pinned_d[0] / s.h_pinned_d[1]) : 0.0;
   return DFM_OK;
+}
+
+extern "C" {
+
+int dfm_sys_true_residual(dfm_sys sys, double* rel) {
+  NEED(sys, "dfm_sys_true_residual");
+  NEED(rel, "dfm_sys_true_residual");
+  return sys_true_residual(sys->s, sys->s.ctx->stream, rel);
 }
 
 }  // extern "C"

@@ -278,8 +278,15 @@ inline int kind_glob_count(int kind) {
 int resolve_precision(const Sys& s, int precision);
 // Buffers, FP32 copies and the right-hand side copy for a solve of the given precision (capi.cpp).
 int sys_solve_setup(Sys& s, int precision);
+// ||b - A x|| / ||b|| in FP64 (b the assembled right-hand side, x the update handle), synchronous on st (capi.cpp).
+int sys_true_residual(Sys& s, hipStream_t st, double* rel);
+// A vector handle (caller's numbering) read on st, the stream whose work last wrote it (capi.cpp).
+int sys_download(Sys& s, int iv, double* out, hipStream_t st);
 // Take a system out of the batch it is solved in (batch.hip).
 void batch_detach(struct Batch* b, Sys* s);
+// Host time of a batch: seconds in table rebuilds, rebuilds, seconds in polls, true-residual restarts (batch.hip).
+void batch_stats(const struct Batch* b, double* out);
+struct Batch* batch_of(dfm_batch b);
 
 // host symbolic routines (host_symbolic.cpp)
 void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vector<unsigned>& rowptr,

@@ -36,9 +36,11 @@ struct Request {
   long long est_bytes = 0;
   int state = Q_QUEUED;
   dfm_sys sys = nullptr;
+  int worker = -1;  // the worker that set it up: its stream is the system's own
   double t_submit = 0, t_start = 0, t_done = 0;
   unsigned iters = 0;
   double conv = 0;
+  double true_rel = -1;  // ||b - A x|| / ||b|| (FP64 matrix) of the finished solve
   int fell_back = 0;
   std::string err;
 };
@@ -71,26 +73,44 @@ struct Server {
   bool stop = false;
   long long next_id = 1;
   std::deque<Request*> pending, setup_q, ready_q;
+  // released systems, destroyed by the worker that created them (destruction waits for the system's own stream,
+  // which is that worker's; the client thread would otherwise wait for whatever setup the worker has queued)
+  std::vector<std::vector<dfm_sys>> grave;
   std::map<long long, Request*> all;
   std::vector<Request*> solving;
   long long bytes_admitted = 0;
   int active = 0;  // admitted and not finished
+  // host-time counters (seconds): batch steps, waits with nothing to iterate, per-result true residuals; the
+  // workers' setup time (under mu)
+  long long n_step = 0, n_setup = 0;
+  double t_step = 0, t_idle = 0, t_check = 0, t_setup = 0, t_join = 0, t_progress = 0;
 
   void worker(int w) {
     dfm_ctx c = wctx[w];
     (void)hipSetDevice(c->c.device);
+    std::vector<dfm_sys> dead;
     for (;;) {
       Request* r = nullptr;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stop || !setup_q.empty(); });
+        cv_work.wait(lk, [&] { return stop || !setup_q.empty() || !grave[w].empty(); });
         if (stop) return;
-        r = setup_q.front();
-        setup_q.pop_front();
+        dead.swap(grave[w]);
+        if (!setup_q.empty()) {
+          r = setup_q.front();
+          setup_q.pop_front();
+          r->worker = w;
+        }
       }
+      for (dfm_sys d : dead) dfm_sys_destroy(d);
+      dead.clear();
+      if (!r) continue;
+      const double t0 = now_s();
       const int rc = setup(c, *r);
       {
         std::lock_guard<std::mutex> lk(mu);
+        t_setup += now_s() - t0;
+        n_setup++;
         if (rc != DFM_OK) {
           r->state = Q_FAILED;
           r->err = dfm_last_error();
@@ -153,13 +173,17 @@ struct Server {
     {
       std::unique_lock<std::mutex> lk(mu);
       admit_locked();
-      if (solving.empty() && ready_q.empty() && active > 0 && wait_ms > 0)
+      if (solving.empty() && ready_q.empty() && active > 0 && wait_ms > 0) {
+        const double t0 = now_s();
         cv_ready.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !ready_q.empty(); });
+        t_idle += now_s() - t0;
+      }
       joining.assign(ready_q.begin(), ready_q.end());
       ready_q.clear();
     }
     // a request that cannot join (e.g. no device memory for its solve buffers) fails alone; the others join
     int failed = 0;
+    const double tj = now_s();
     for (Request* r : joining) {
       const int rc = dfm_batch_add(batch, r->sys, r->tol, r->max_iter);
       if (rc != DFM_OK) {
@@ -174,13 +198,17 @@ struct Server {
       r->state = Q_SOLVING;
       solving.push_back(r);
     }
+    t_join += now_s() - tj;
     if (failed) {
       std::lock_guard<std::mutex> lk(mu);
       admit_locked();
     }
     if (solving.empty()) return DFM_OK;
     int nrun = 0;
+    const double t0 = now_s();
     int rc = dfm_batch_step(batch, iterations, &nrun);
+    t_step += now_s() - t0;
+    n_step++;
     if (rc != DFM_OK) return rc;
     std::vector<Request*> still;
     int finished = 0;
@@ -193,9 +221,14 @@ struct Server {
         continue;
       }
       r->t_done = now_s();
-      r->state = st == 1 ? Q_CONVERGED : Q_MAXITER;
       rc = dfm_batch_info(batch, r->sys, nullptr, &r->fell_back);
       if (rc != DFM_OK) return rc;
+      // the true residual of every result (the FP64 loop's conv_ratio is its recursive one)
+      const double tc = now_s();
+      rc = sys_true_residual(r->sys->s, ctx->c.stream, &r->true_rel);
+      t_check += now_s() - tc;
+      if (rc != DFM_OK) return rc;
+      r->state = st == 1 ? Q_CONVERGED : Q_MAXITER;
       rc = dfm_batch_remove(batch, r->sys);
       if (rc != DFM_OK) return rc;
       finished++;
@@ -216,6 +249,8 @@ struct Server {
     }
     cv_work.notify_all();
     for (auto& t : workers) t.join();
+    for (auto& g : grave)
+      for (dfm_sys d : g) dfm_sys_destroy(d);
     if (batch) dfm_batch_destroy(batch);
     for (auto& kv : all) {
       if (kv.second->sys) dfm_sys_destroy(kv.second->sys);
@@ -256,6 +291,7 @@ int dfm_server_create(dfm_ctx ctx, int precision, long long budget_bytes, int ma
     delete h;
     return rc;
   }
+  sv.grave.resize((size_t)nworkers);
   for (int w = 0; w < nworkers; w++) sv.workers.emplace_back([&sv, w] { sv.worker(w); });
   *out = h;
   return DFM_OK;
@@ -300,7 +336,9 @@ int dfm_server_submit(dfm_server s, const dfm_request* q, long long* ticket) {
 int dfm_server_progress(dfm_server s, int iterations, int wait_ms, int* n_active, int* n_queued) {
   DFM_CHECK(s && iterations > 0, DFM_ERR_ARG, "dfm_server_progress: bad argument");
   Server& sv = s->s;
+  const double t0 = now_s();
   const int rc = sv.progress(iterations, wait_ms);
+  sv.t_progress += now_s() - t0;
   std::lock_guard<std::mutex> lk(sv.mu);
   if (n_active) *n_active = sv.active;
   if (n_queued) *n_queued = (int)sv.pending.size();
@@ -325,6 +363,30 @@ int dfm_server_poll(dfm_server s, long long ticket, int* state, double* latency_
   return DFM_OK;
 }
 
+int dfm_server_stats(dfm_server s, double* out, int n) {
+  DFM_CHECK(s && out && n > 0, DFM_ERR_ARG, "dfm_server_stats: bad argument");
+  Server& sv = s->s;
+  double b[4] = {0, 0, 0, 0};
+  batch_stats(batch_of(sv.batch), b);
+  std::lock_guard<std::mutex> lk(sv.mu);
+  const double v[12] = {(double)sv.n_step, sv.t_step, b[0], b[1], b[2], b[3], sv.t_idle, sv.t_setup,
+                        (double)sv.n_setup, sv.t_check, sv.t_join, sv.t_progress};
+  for (int k = 0; k < n && k < 12; k++) out[k] = v[k];
+  return DFM_OK;
+}
+
+int dfm_server_true_residual(dfm_server s, long long ticket, double* rel) {
+  DFM_CHECK(s && rel, DFM_ERR_ARG, "dfm_server_true_residual: null argument");
+  Server& sv = s->s;
+  std::lock_guard<std::mutex> lk(sv.mu);
+  auto it = sv.all.find(ticket);
+  DFM_CHECK(it != sv.all.end(), DFM_ERR_ARG, "dfm_server_true_residual: unknown ticket");
+  DFM_CHECK(it->second->state == Q_CONVERGED || it->second->state == Q_MAXITER, DFM_ERR_STATE,
+            "dfm_server_true_residual: not finished");
+  *rel = it->second->true_rel;
+  return DFM_OK;
+}
+
 int dfm_server_result(dfm_server s, long long ticket, double* u) {
   DFM_CHECK(s && u, DFM_ERR_ARG, "dfm_server_result: null argument");
   Server& sv = s->s;
@@ -336,7 +398,9 @@ int dfm_server_result(dfm_server s, long long ticket, double* u) {
     r = it->second;
   }
   DFM_CHECK(r->state == Q_CONVERGED || r->state == Q_MAXITER, DFM_ERR_STATE, "dfm_server_result: not finished");
-  return dfm_sys_get_vector(r->sys, -2, u);
+  // read on the server's stream, where the solve ran (the system's own stream belongs to the worker that set it
+  // up and may hold another request's setup)
+  return sys_download(r->sys->s, -2, u, sv.ctx->c.stream);
 }
 
 int dfm_server_release(dfm_server s, long long ticket) {
@@ -351,11 +415,11 @@ int dfm_server_release(dfm_server s, long long ticket) {
     DFM_CHECK(r->state >= Q_CONVERGED, DFM_ERR_STATE, "dfm_server_release: request not finished");
     sv.all.erase(it);
     sv.bytes_admitted -= r->est_bytes;
+    if (r->sys) sv.grave[(size_t)r->worker].push_back(r->sys);
+    sv.admit_locked();
   }
-  if (r->sys) dfm_sys_destroy(r->sys);
+  sv.cv_work.notify_all();
   delete r;
-  std::lock_guard<std::mutex> lk(sv.mu);
-  sv.admit_locked();
   return DFM_OK;
 }
 

@@ -1174,6 +1174,40 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
       ok = false;
     }
   }
+  // The FP64 loop (DFM_FP64_TRUE; with the FP32 factor DFM_FP32_ILU, or a mixed solve that fell back to it) stops on
+  // its recursive residual: its true residual is checked and the loop restarted from its solution (r = b - A x, the
+  // recursive target halved, still relative to ||b||) until the true residual meets the tolerance, so that
+  // conv_ratio is the true relative residual in every mode but DFM_FP64 (the reference's own test). A restart
+  // that does not lower the true residual by a fifth ends the solve unconverged: the residual of b - A x in
+  // FP64 has reached what that arithmetic resolves for this system.
+  if (ok && (precision == DFM_FP32_ILU || precision == DFM_FP64_TRUE || s.last_fallback)) {
+    PcgBufs v64 = pcg_bufs(s, false);
+    v64.f32ilu = precision != DFM_FP64_TRUE;
+    const long long m = (long long)s.n * s.len;
+    double prev = 0.0;
+    for (int restart = 0;; restart++) {
+      double rel = 0.0;
+      rc = sys_true_residual(s, st, &rel);
+      if (rc != DFM_OK) return rc;
+      conv = rel;
+      if (rel <= tol) break;
+      before += it + 1;
+      if (before >= maxit || restart == kMaxRestarts || (restart > 0 && rel > kRestartGain * prev)) {
+        ok = false;
+        break;
+      }
+      prev = rel;
+      DFM_HIP(launch_copy(s.d_rhs.p, s.d_res.p, m, st));  // the warm start forms b - A x from the right-hand side
+      double c2 = 0.0;
+      rc = pcg_run<double>(s, v64, false, 0.5 * tol, maxit - before, c2, it, ok, true);
+      if (rc != DFM_OK) return rc;
+      if (!ok) {
+        rc = sys_true_residual(s, st, &conv);
+        if (rc != DFM_OK) return rc;
+        break;
+      }
+    }
+  }
   DFM_HIP(hipEventRecord(e1, st));
   DFM_HIP(hipEventSynchronize(e1));
   float ms = 0;

@@ -57,6 +57,11 @@ int pcg_start(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned maxit, 
 int pcg_start_warm(Sys& s, const PcgBufs& v, double tol, unsigned maxit, double* sc, int* fl, hipStream_t st);
 int pcg_finish(Sys& s, const PcgBufs& v, bool mixed, int done, hipStream_t st);
 
+// Restarts of an FP64 loop with the FP32 factor after its true residual was checked: at most kMaxRestarts, each
+// with half the tolerance as recursive target, and only while every restart lowers the true residual below
+// kRestartGain times the previous one.
+constexpr int kMaxRestarts = 8;
+constexpr double kRestartGain = 0.8;
 int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration);
 int solve_pbicgstab(Sys& s, double* conv_ratio, unsigned* iteration);
 // z = M^-1 src with the FP64 factor (src may equal z); kernels skip when *done != 0 (done may be null)

@@ -82,10 +82,13 @@ enum {
  * factor and Krylov vectors with FP64 dots, solution and true residual (reliable updates), continuing as
  * DFM_FP32_ILU from its FP64 solution when its reliable-update cycles make slow progress; DFM_FP32_ILU the FP64
  * loop with the FP32 copy of the factor as preconditioner (attainable accuracy of FP64, half the factor bytes);
- * DFM_MIXED_AUTO per system: DFM_MIXED for scalar systems (one dof per node), DFM_FP32_ILU for vector systems,
- * where the reliable-update cycles of the FP32 matrix were measured to need about 1.4x the FP64 iterations on
- * slender 2D elasticity. */
-enum { DFM_FP64 = 0, DFM_MIXED = 1, DFM_FP32_ILU = 2, DFM_MIXED_AUTO = 3 };
+ * DFM_FP64_TRUE the reference loop in FP64 whose true residual is checked when it stops, restarting from its
+ * solution until the true residual meets the tolerance; DFM_MIXED_AUTO per system: DFM_MIXED for scalar systems
+ * (one dof per node), DFM_FP64_TRUE for vector systems. The choice for vector systems is measured on the serving
+ * stream's slender 2D elasticity: the FP32 matrix's reliable-update cycles need about 1.4x the FP64 iterations,
+ * and the FP32 factor adds 10-30 % iterations while the batched sweeps of such systems, latency- rather than
+ * bandwidth-bound, run no faster with it. */
+enum { DFM_FP64 = 0, DFM_MIXED = 1, DFM_FP32_ILU = 2, DFM_MIXED_AUTO = 3, DFM_FP64_TRUE = 4 };
 
 const char* dfm_last_error(void);
 int dfm_device_count(int* count);
@@ -239,9 +242,14 @@ int dfm_prec_get_factor(dfm_sys sys, unsigned* frowptr, unsigned* fcolind, unsig
  * Same contract as the reference: conv_ratio in = tolerance, out = achieved ratio; iteration in = max,
  * out = the 0-based index at convergence for PCG (1-based loop index for PBiCGSTAB). Returns 1 when
  * converged, 0 when not, negative on error. Scalars stay on the device; the host polls a done flag.
- * DFM_FP64 runs the reference algorithm in FP64 on the recursive residual. DFM_MIXED keeps FP32 copies of
- * the matrix and the ILU factor and runs FP32 SpMV/ILU inside FP64 iterative refinement on the true
- * residual r = b - A x (FP64 matrix, FP64 dots and update); its conv_ratio is the true relative residual. */
+ * DFM_FP64 runs the reference algorithm in FP64 on the recursive residual (its conv_ratio is that of the
+ * reference; dfm_sys_true_residual gives the true one). DFM_MIXED keeps FP32 copies of the matrix and the ILU
+ * factor and runs FP32 SpMV/ILU inside FP64 iterative refinement on the true residual r = b - A x (FP64 matrix,
+ * FP64 dots and update). DFM_FP32_ILU, DFM_FP64_TRUE and a mixed solve that continued in DFM_FP32_ILU check the
+ * true residual when their recursive one converged and restart from their solution (recursive target half the
+ * tolerance) while it is above the tolerance and each restart lowers it by a fifth; where FP64 cannot resolve the
+ * tolerance for the system the solve ends unconverged. In every mode but DFM_FP64 the returned conv_ratio is the
+ * true relative residual ||b - A x|| / ||b|| (FP64 matrix). */
 int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration);
 int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
 /* Host-driven loops over the handle operations (the C++ layer include/dfm_lssol.hpp), FP64, same contract as
@@ -340,6 +348,14 @@ int dfm_server_progress(dfm_server s, int iterations, int wait_ms, int* n_active
  * stalled and was repeated in FP64 (dfm_sys_last_solve_info). */
 int dfm_server_poll(dfm_server s, long long ticket, int* state, double* latency_s, double* solve_s,
                     unsigned* iteration, double* conv_ratio, int* fell_back);
+/* Host-time counters of the server, out[0..n): batch steps, seconds in them, seconds in batch table rebuilds,
+ * rebuilds, seconds in flag polls (each waits for the device), restarts after a true-residual check, seconds
+ * waiting with nothing to iterate, seconds of request setup summed over the workers, requests set up, seconds of
+ * per-result true residuals, seconds starting the solves of joining requests, seconds in dfm_server_progress. */
+int dfm_server_stats(dfm_server s, double* out, int n);
+/* ||b - A x|| / ||b|| with the FP64 matrix of a finished request, computed when it finished (for DFM_FP64 requests
+ * the conv_ratio of dfm_server_poll is the reference loop's recursive one). */
+int dfm_server_true_residual(dfm_server s, long long ticket, double* rel);
 /* The solution (update handle, caller's node numbering) of a finished request. */
 int dfm_server_result(dfm_server s, long long ticket, double* u);
 int dfm_server_release(dfm_server s, long long ticket);
