@@ -40,9 +40,8 @@ def log(msg):
 
 
 def measured_traffic(profile, kernel_prefixes):
-    """HBM bytes per dispatch of the named kernels from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (profiles/<profile>, last column in MB), averaged over the kernels; None if the summary or a kernel is
-    missing."""
+    """HBM bytes per dispatch of the named rows of a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/<profile>, last column in MB), averaged over the rows; None if the summary or a row is missing."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
@@ -56,10 +55,9 @@ def measured_traffic(profile, kernel_prefixes):
                 continue
     got = []
     for k in kernel_prefixes:
-        m = [v for name, v in vals.items() if name.startswith(k)]
-        if not m:
+        if k not in vals:
             return None
-        got.append(m[0])
+        got.append(vals[k])
     return sum(got) / len(got)
 
 
@@ -721,13 +719,13 @@ def run_single(args, N, W, w, ctx, order_kind):
         f"({sw32_b / sw32_ms / 1e6:.0f} GB/s), fp64 {sw64_ms * 1e3:.1f} us ({sw64_b / sw64_ms / 1e6:.0f} GB/s), "
         f"ILU factor {fac_ms:.3f} ms ({fac_b / fac_ms / 1e6:.0f} GB/s); mixed vs fp64 solution error {err:.2e}")
     nlaunch = nph[3]  # sweep launches per PCG iteration (the last forward and first backward phase may be one)
-    fused = nph[3] < nph[0] + nph[1]
     B = w["len"]
-    kinds = ([f"dfm::k_fwd_update<{B}, "] * (nph[0] - fused) + [f"dfm::k_bwd_rz<{B}, "] * (nph[1] - fused) +
-             [f"dfm::k_fwd_bwd_tile<{B}, "] * fused)
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated against a copy kernel) from the
+    # committed summary of this workload
     prof = f"{w['name']}_hbm_traffic.md"
-    sweep_traffic = measured_traffic(prof, kinds) if args.ordering == "blockcolor" else None
-    spmv_traffic = measured_traffic(prof, [f"dfm::k_pcg_spmv_sell<{B}, float"]) if args.ordering == "blockcolor" else None
+    blockcolor = args.ordering == "blockcolor"
+    sweep_traffic = measured_traffic(prof, ["sweep_phases_fp32_per_launch"]) if blockcolor else None
+    spmv_traffic = measured_traffic(prof, [f"dfm::k_pcg_spmv_sell<{B}, float, false>"]) if blockcolor else None
     K = args.steps
     head = fp64 if args.config == "c1" else mixed  # the plain-FP64 configuration reports its FP64 path
     t_head = sum(head["lat"])
