@@ -1318,17 +1318,10 @@ int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
 }  // extern "C"
 
 // ||b - A x|| / ||b|| in FP64 for b = the assembled right-hand side and x = the update handle, enqueued on st
-// and waited for; the system's own scalar slots and its first work vector are the scratch.
+// and waited for; one pass over the matrix, the system's partial-sum buffer and scalar slots as scratch.
 int dfm::sys_true_residual(Sys& s, hipStream_t st, double* rel) {
-  const long long m = mdof(s);
-  const int rc = ensure_work(s, 1, 0);
-  if (rc != DFM_OK) return rc;
-  double* t = s.d_work.p;
-  DFM_HIP(launch_copy(s.d_rhs.p, t, m, st));
-  DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
-                      s.d_upd.p, 1.0, t, st));
-  DFM_HIP(launch_dot(t, t, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP, st));
-  DFM_HIP(launch_dot(s.d_rhs.p, s.d_rhs.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP + 1, st));
+  DFM_HIP(launch_true_residual(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p,
+                               s.d_upd.p, s.d_rhs.p, s.d_partial.p, s.d_scal.p + S_TMP, st));
   DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + S_TMP, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
   *rel = s.h_pinned_d[1] > 0 ? std::sqrt(s.h_pinned_d[0] / s.h_pinned_d[1]) : 0.0;

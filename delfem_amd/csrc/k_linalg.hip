@@ -153,6 +153,65 @@ hipError_t launch_spmv(int n, int len, int tpr, const unsigned* rowptr, const un
   return hipErrorInvalidValue;
 }
 
+// ||b - A x||^2 and ||b||^2 in one pass over the matrix (no residual vector): a fixed grid of at most kMaxPartial
+// blocks walks the block rows (TPR lanes per row as k_spmv), each block writes its two partial sums
+// (slot[blockIdx.x], slot[kMaxPartial + blockIdx.x]); one block then sums them in block order into out[0..1].
+template <int B, int TPR>
+__global__ void __launch_bounds__(256) k_true_res_part(int n, const unsigned* __restrict__ rp,
+                                                       const unsigned* __restrict__ ci, const double* __restrict__ crs,
+                                                       const double* __restrict__ dia, const double* __restrict__ x,
+                                                       const double* __restrict__ b, double* part) {
+  constexpr int RPB = 256 / TPR;  // rows per block and sweep
+  const int lane = threadIdx.x % TPR;
+  double v[2] = {0.0, 0.0};
+  for (int r0 = (int)blockIdx.x * RPB; r0 < n; r0 += (int)gridDim.x * RPB) {
+    const int row = r0 + (int)threadIdx.x / TPR;
+    if (row < n) {
+      double acc[B];
+      spmv_row<B, TPR, double, double, double>(rp[row], rp[row + 1], ci, crs, x, lane, acc);
+      if (lane == 0) {
+        dia_row<B, double, double, double>(dia, x, row, acc);
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          const double bi = b[(size_t)row * B + k];
+          const double t = bi - acc[k];
+          v[0] += t * t;
+          v[1] += bi * bi;
+        }
+      }
+    }
+  }
+  produce<256, 2>(part, v);
+}
+
+__global__ void __launch_bounds__(256) k_true_res_finish(const double* part, int G, double* out) {
+  double t[2];
+  consume<256, 2>(part, G, t);
+  if (threadIdx.x == 0) {
+    out[0] = t[0];
+    out[1] = t[1];
+  }
+}
+
+hipError_t launch_true_residual(int n, int len, int tpr, const unsigned* rowptr, const unsigned* colind,
+                                const double* crs, const double* dia, const double* x, const double* b, double* part,
+                                double* out, hipStream_t st) {
+  if (n <= 0) return hipErrorInvalidValue;
+  const long long rows_per_block = 256 / tpr;
+  const unsigned G = (unsigned)std::min<long long>(kMaxPartial, (n + rows_per_block - 1) / rows_per_block);
+#define DFM_TRES(B, T)                                                                                          \
+  if (len == B && tpr == T) {                                                                                   \
+    hipLaunchKernelGGL((k_true_res_part<B, T>), dim3(G), dim3(256), 0, st, n, rowptr, colind, crs, dia, x, b, part); \
+    hipLaunchKernelGGL(k_true_res_finish, dim3(1), dim3(256), 0, st, (const double*)part, (int)G, out);         \
+    return hipGetLastError();                                                                                  \
+  }
+  DFM_TRES(1, 4) DFM_TRES(1, 8) DFM_TRES(1, 16) DFM_TRES(1, 32)
+  DFM_TRES(2, 4) DFM_TRES(2, 8) DFM_TRES(2, 16) DFM_TRES(2, 32)
+  DFM_TRES(3, 4) DFM_TRES(3, 8) DFM_TRES(3, 16) DFM_TRES(3, 32)
+#undef DFM_TRES
+  return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // ILU(0) numeric factorisation, row-wise IKJ exactly as DoILUDecomp: for row i and each k < i of its lower
 // part in ascending order, A_ij -= A_ik * U_kj for every j > k of row k that is in row i's pattern (or is

@@ -69,6 +69,12 @@ int spmv_tpr(long long ncrs, int n);
 hipError_t launch_spmv(int n, int len, int tpr, const unsigned* rowptr, const unsigned* colind, const double* crs,
                        const double* dia, double alpha, const double* x, double beta, double* y, hipStream_t st);
 
+// ||b - A x||^2 -> out[0], ||b||^2 -> out[1] (device numbering, FP64) without a residual vector: partial sums of a
+// fixed grid of at most kMaxPartial blocks in part[0 .. 2*kMaxPartial), then one block sums them in block order.
+hipError_t launch_true_residual(int n, int len, int tpr, const unsigned* rowptr, const unsigned* colind,
+                                const double* crs, const double* dia, const double* x, const double* b, double* part,
+                                double* out, hipStream_t st);
+
 // ILU(0)
 struct SweepDev {
   int nphase;

@@ -273,6 +273,14 @@ int dfm_solve_mat_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration
 /* ||b - A x|| / ||b|| in FP64 for b = the residual as assembled and x = the update (after a solve). */
 int dfm_sys_true_residual(dfm_sys sys, double* rel);
 
+/* dfm_sys_true_residual without device allocation. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid it). Its
+ * scratch, the partial sums of at most 2048 workgroups and the two totals, is a fixed array of 32,784 bytes on
+ * each device the library runs on, held until the process ends; the result is bit-identical to
+ * dfm_sys_true_residual's (same kernels). Systems of more than 2^31 - 1 dofs are refused (DFM_ERR_ARG). Calls that
+ * overlap on one device share that array and race: serialise them. */
+int dfm_sys_true_residual_noalloc(dfm_sys sys, double* rel);
+
 /* ---- timing of the last device solve (HIP events on the system's stream) ----------------------------- */
 int dfm_sys_last_solve_stats(dfm_sys sys, double* ms_total, double* ms_per_iter, unsigned* inner_iters);
 /* Mixed precision: reliable updates of the last solve, and whether its FP64 true residual stalled (six
