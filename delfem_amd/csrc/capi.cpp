@@ -140,6 +140,8 @@ Sys::~Sys() {
   d_rowptr.release(); d_colind.release(); d_diaind.release(); d_lsplit.release(); d_usplit.release();
   d_frowptr.release(); d_fcolind.release(); d_fdiaind.release(); d_a2f.release();
   d_order.release(); d_inv.release(); d_elsup_ptr.release(); d_elsup.release(); d_lnods.release(); d_bc.release();
+  d_eptr.release(); d_eidx.release(); d_eloc.release(); d_rowptr0.release(); d_colind0.release(); d_permq.release();
+  d_elnod.release();
   d_coords.release(); d_u.release(); d_v.release(); d_a.release(); d_v0.release();
   d_crs.release(); d_dia.release(); d_crs32.release(); d_dia32.release();
   d_fcrs.release(); d_fdia.release();
@@ -153,6 +155,7 @@ Sys::~Sys() {
   d_force.release(); d_bptr.release(); d_bcol.release(); d_bsrc.release(); d_bval.release();
   for (auto& g : gexec)
     if (g) (void)hipGraphExecDestroy(g);
+  delete last_asm;
   if (h_pinned) DevAlloc::release_host(h_pinned);
   if (h_pinned_d) DevAlloc::release_host(h_pinned_d);
 }
@@ -212,6 +215,12 @@ static int upload_structure(Sys& s) {
   DFM_HIP(upload(s.d_diaind, s.diaind, st));
   DFM_HIP(upload(s.d_order, s.order, st));
   DFM_HIP(upload(s.d_inv, s.inv, st));
+  DFM_HIP(upload(s.d_eptr, s.eptr, st));
+  DFM_HIP(upload(s.d_eidx, s.eidx, st));
+  if (s.perm_crs.size() < 0xFFFFFFFFull) {
+    std::vector<unsigned> pq(s.perm_crs.begin(), s.perm_crs.end());
+    DFM_HIP(upload(s.d_permq, pq, st));  // (upload waits for the copy)
+  }
   s.sell.release_dev();
   DFM_HIP(upload(s.sell.d_sptr, s.sell.sptr, st));
   DFM_HIP(upload(s.sell.d_visit, s.sell.visit, st));
@@ -303,6 +312,10 @@ int dfm_ctx_create(int device, dfm_ctx* out) {
 void ctx_release(dfm_ctx ctx) {
   if (ctx->c.refs.fetch_sub(1) != 1) return;
   (void)hipStreamSynchronize(ctx->c.stream);
+  {
+    CacheRelease guard;  // the stream is idle: the buffer can go back to the cache
+    ctx->c.ebuf.release();
+  }
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }
@@ -333,6 +346,9 @@ static hipError_t sys_alloc_device(Sys& s) {
   if ((e = upload(s.d_lnods, s.lnods, st)) != hipSuccess) return e;
   if ((e = upload(s.d_elsup_ptr, s.elsup_ptr, st)) != hipSuccess) return e;
   if ((e = upload(s.d_elsup, s.elsup, st)) != hipSuccess) return e;
+  if ((e = upload(s.d_eloc, s.eloc, st)) != hipSuccess) return e;
+  if ((e = upload(s.d_rowptr0, s.rowptr0, st)) != hipSuccess) return e;
+  if ((e = upload(s.d_colind0, s.colind0, st)) != hipSuccess) return e;
   // two doubles of padding: the factor kernel reads 3x3 blocks as five aligned 16-byte loads, one past the end
   if ((e = s.d_crs.alloc((size_t)s.ncrs * s.bs + 2)) != hipSuccess) return e;
   if ((e = s.d_dia.alloc((size_t)nnode * s.bs + 2)) != hipSuccess) return e;
@@ -414,8 +430,10 @@ int dfm_sys_clone(dfm_sys src, dfm_sys* out) {
   s.order = a.order; s.inv = a.inv; s.natural = a.natural; s.ncolors = a.ncolors;
   s.color_of = a.color_of; s.tile_of = a.tile_of;
   s.rowptr = a.rowptr; s.colind = a.colind; s.diaind = a.diaind; s.lsplit = a.lsplit; s.usplit = a.usplit;
+  s.max_rowlen = a.max_rowlen;
   s.fill_level = a.fill_level; s.frowptr = a.frowptr; s.fcolind = a.fcolind; s.fdiaind = a.fdiaind; s.a2f = a.a2f;
   s.perm_crs = a.perm_crs;
+  s.eptr = a.eptr; s.eidx = a.eidx; s.eloc = a.eloc;
   s.bc0 = a.bc0;
   copy_sweep_host(s.fwd, a.fwd);
   copy_sweep_host(s.bwd, a.bwd);
@@ -767,6 +785,59 @@ static int finish_matrix(Sys& s, hipStream_t st) {
   return DFM_OK;
 }
 
+}  // extern "C"
+
+// The element store of an assembly of s (its lists and its context's buffer, grown as needed); false where the
+// lists are not built or the buffer cannot be had (the per-row gather runs then). Linear elasticity with rows of at
+// most 63 off-diagonal blocks takes the row-group kernel, which needs neither.
+static bool elem_store(Sys& s, int kind, bool save, ElemStore& es) {
+  es = ElemStore{};
+  if (!save && (kind == DFM_SOLID2D || kind == DFM_SOLID3D) && s.nelem > 0 && s.max_rowlen < 64 && s.d_permq.p &&
+      s.d_rowptr0.p && s.d_colind0.p && s.npoel <= 4) {
+    if (!s.d_elnod.p) {
+      std::vector<int> en(s.elsup.size() * 4, -1);
+      for (size_t k = 0; k < s.elsup.size(); k++)
+        for (int c = 0; c < s.npoel; c++) en[k * 4 + c] = s.lnods[(size_t)s.elsup[k] * s.npoel + c];
+      if (upload(s.d_elnod, en, s.ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        s.d_elnod.release();
+        return false;
+      }
+    }
+    int lpr = 8;
+    while ((unsigned)lpr < s.max_rowlen + 1) lpr *= 2;
+    es.nelem = s.nelem;
+    es.ncrs = s.ncrs;
+    es.lpr = lpr;
+    return true;
+  }
+  if (s.eptr.empty() || !s.d_eptr.p || !s.d_eloc.p) return false;
+  const size_t need = (size_t)s.nelem * s.npoel * ((size_t)s.npoel * s.bs + s.len);
+  DBuf<double>& b = s.ctx->ebuf;
+  if (b.n < need) {
+    // the context's stream is the only user of the buffer: wait for it before the old one goes back to the cache
+    if (hipStreamSynchronize(s.ctx->stream) != hipSuccess) return false;
+    {
+      CacheRelease guard;
+      b.release();
+    }
+    if (b.alloc(need) != hipSuccess) {
+      (void)hipGetLastError();
+      b.release();
+      return false;
+    }
+  }
+  es.nelem = s.nelem;
+  es.ncrs = s.ncrs;
+  es.eptr = s.d_eptr.p;
+  es.eidx = s.d_eidx.p;
+  es.eloc = s.d_eloc.p;
+  es.ebuf = b.p;
+  return true;
+}
+
+extern "C" {
+
 int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double* res_norm) {
   NEED(sys, "dfm_sys_assemble_resident");
   Sys& s = sys->s;
@@ -786,10 +857,15 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   }
   hipStream_t st = s.ctx->stream;
   const long long m = mdof(s);
-  // InitializeMarge (src/ls/linearsystem.cpp:74-88): matrix, residual and update set to zero
-  DFM_HIP(launch_fill(s.d_crs.p, 0.0, (long long)s.ncrs * s.bs, st));
-  DFM_HIP(launch_fill(s.d_dia.p, 0.0, (long long)s.n * s.bs, st));
-  DFM_HIP(launch_fill(s.d_res.p, 0.0, m, st));
+  ElemStore es{};
+  const bool store = elem_store(s, kind, false, es);
+  // InitializeMarge (src/ls/linearsystem.cpp:74-88): matrix, residual and update set to zero (the element store
+  // writes every entry of the matrix and the residual itself)
+  if (!store) {
+    DFM_HIP(launch_fill(s.d_crs.p, 0.0, (long long)s.ncrs * s.bs, st));
+    DFM_HIP(launch_fill(s.d_dia.p, 0.0, (long long)s.n * s.bs, st));
+    DFM_HIP(launch_fill(s.d_res.p, 0.0, m, st));
+  }
   DFM_HIP(launch_fill(s.d_upd.p, 0.0, m, st));
   AsmArgs a{};
   a.kind = kind; a.ndim = s.ndim; a.npoel = s.npoel; a.len = s.len; a.n = s.n;
@@ -805,7 +881,10 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   a.loads = s.has_loads ? s.d_loads.p : nullptr;
   a.load_scale = s.load_scale;
   a.rowptr = s.d_rowptr.p; a.colind = s.d_colind.p; a.crs = s.d_crs.p; a.dia = s.d_dia.p; a.res = s.d_res.p;
-  DFM_HIP(launch_assemble(a, st));
+  a.rowptr0 = s.d_rowptr0.p; a.colind0 = s.d_colind0.p; a.permq = s.d_permq.p; a.elnod = s.d_elnod.p;
+  DFM_HIP(launch_assemble(a, store ? &es : nullptr, st));
+  if (!s.last_asm) s.last_asm = new AsmArgs;
+  *s.last_asm = a;
   // FinalizeMarge (:92-128): Dirichlet rows/columns, residual zeroed at fixed dofs, ||r|| returned
   DFM_HIP(launch_set_bc(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_bc.p, st));
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
@@ -925,7 +1004,9 @@ int dfm_sys_assemble_save(dfm_sys sys, int kind, const double* params, int npar,
   a.params = s.d_par.p; a.npar = s.npar; a.pstride = s.pstride;
   a.rowptr = s.d_rowptr.p; a.colind = s.d_colind.p; a.crs = s.d_crs.p; a.dia = s.d_dia.p; a.res = s.d_force.p;
   a.save = true;
-  DFM_HIP(launch_assemble(a, st));
+  ElemStore es{};
+  const bool store = elem_store(s, kind, true, es);
+  DFM_HIP(launch_assemble(a, store ? &es : nullptr, st));
   rc = build_boundary_terms(s);
   if (rc != DFM_OK) return rc;
   DFM_HIP(launch_gather_terms(s.nbterm, s.d_bsrc.p, (long long)s.n * s.bs, s.d_dia.p, s.d_crs.p, s.d_bval.p, st));
@@ -1362,7 +1443,7 @@ extern "C" {
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes) {
   NEED(sys, "dfm_sys_bench_kernel");
   Sys& s = sys->s;
-  DFM_CHECK(which >= 0 && which <= 7 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
+  DFM_CHECK(which >= 0 && which <= 8 && reps > 0, DFM_ERR_ARG, "dfm_sys_bench_kernel: bad arguments");
   hipStream_t st = s.ctx->stream;
   if (which == 6) {
     // MakeResidual of the saved stiffness: force, residual, update and the field once per dof, the order map,
@@ -1377,6 +1458,35 @@ int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, doubl
     const double m = (double)mdof(s);
     if (bytes) *bytes = m * (4 * 8.0 + 4.0) + 4.0 + 4.0 * s.n + (double)s.nbterm * (8.0 + 4.0 + 8.0);
     s.prepared = -1;
+    return DFM_OK;
+  }
+  if (which == 8) {
+    // the assembly kernels of the last dfm_sys_assemble[_resident] (element store and gathers, or the per-row
+    // gather into the zeroed matrix), rewriting the same matrix and residual; algorithmic bytes: the matrix,
+    // diagonal and residual written once; coordinates, connectivity, per-element parameters and the field read
+    DFM_CHECK(s.assembled && s.last_asm && !s.saved, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble first");
+    AsmArgs a = *s.last_asm;
+    ElemStore es{};
+    const bool store = elem_store(s, a.kind, false, es);
+    const long long m = mdof(s);
+    auto launch = [&]() -> hipError_t {
+      hipError_t e;
+      if (!store) {
+        if ((e = launch_fill(s.d_crs.p, 0.0, (long long)s.ncrs * s.bs, st)) != hipSuccess) return e;
+        if ((e = launch_fill(s.d_dia.p, 0.0, (long long)s.n * s.bs, st)) != hipSuccess) return e;
+        if ((e = launch_fill(s.d_res.p, 0.0, m, st)) != hipSuccess) return e;
+      }
+      return launch_assemble(a, store ? &es : nullptr, st);
+    };
+    const int rc = time_launches(st, reps, launch, avg_ms);
+    if (rc != DFM_OK) return rc;
+    // the matrix must be treated again (Dirichlet rows and columns, copies): a new assembly is needed to solve
+    s.assembled = s.factored = s.factored32 = false;
+    s.prepared = -1;
+    if (bytes)
+      *bytes = ((double)s.ncrs + s.n) * s.bs * 8.0 + (double)m * 8.0 + (double)s.n * s.ndim * 8.0 +
+               (double)s.nelem * s.npoel * 4.0 + (double)s.nelem * (a.pstride ? a.pstride : 0) * 8.0 +
+               (a.u ? (double)m * 8.0 : 0.0);
     return DFM_OK;
   }
   if (which == 7) {

@@ -74,6 +74,8 @@ struct Ctx {
   // server's workers and its client), so the count is atomic
   std::atomic<int> refs{1};
   bool destroyed = false;  // dfm_ctx_destroy called; freed when the last system goes
+  // element-store buffer of the assemblies run on this context's stream (they run one after another there)
+  DBuf<double> ebuf;
 };
 
 // Host-side schedule of one triangular sweep: phases run one after another (kernel boundaries); a phase is
@@ -190,7 +192,12 @@ struct Sys {
 
   // device-numbering pattern of the permuted matrix
   std::vector<unsigned> rowptr, colind, diaind;
+  unsigned max_rowlen = 0;  // the most off-diagonal blocks of a row
   std::vector<long long> perm_crs;  // original crs index -> device crs index
+  // element-store lists of the assembly (ElemStore, kernels.hpp): per device entry the (e*NNO + a)*NNO + c of the
+  // elements holding both of its nodes in ascending e; per node-element incidence (elsup order) e*NNO + a; empty
+  // where the indices do not fit 32 bits (the per-row gather runs instead)
+  std::vector<unsigned> eptr, eidx, eloc;
   // ILU factor pattern (device numbering): the matrix pattern for ILU(0), the level-of-fill pattern for ILU(k);
   // lsplit/usplit split each factor row into entries outside / inside its sweep group
   int fill_level = 0;  // 0: ILU(0); k > 0: ILU(k); -1: no limit
@@ -204,6 +211,10 @@ struct Sys {
   DBuf<unsigned> d_rowptr, d_colind, d_diaind, d_lsplit, d_usplit;
   DBuf<unsigned> d_frowptr, d_fcolind, d_fdiaind, d_a2f;
   DBuf<int> d_order, d_inv, d_elsup_ptr, d_elsup, d_lnods, d_bc;
+  DBuf<unsigned> d_eptr, d_eidx, d_eloc;
+  // the original pattern and, per original entry, its device entry (linear elasticity's row-group assembly)
+  DBuf<unsigned> d_rowptr0, d_colind0, d_permq;
+  DBuf<int> d_elnod;  // per node-element incidence the element's nodes (4 ints), built at the first such assembly
   DBuf<double> d_coords, d_u, d_v;
   DBuf<double> d_a;   // ACCELERATION field, allocated on first use (Newmark)
   DBuf<double> d_v0;  // previous velocity of the backward-Euler loop, allocated on first use
@@ -250,6 +261,8 @@ struct Sys {
   int gchunk = 16;
   unsigned long long struct_gen = 0;  // bumped whenever the pattern, ordering, fill level or schedule is rebuilt
 
+  // the kernel arguments of the last assembly (timed again by dfm_sys_bench_kernel)
+  struct AsmArgs* last_asm = nullptr;
   bool assembled = false, factored = false, factored32 = false;
   double last_ms = 0.0, last_ms_iter = 0.0;
   unsigned last_inner = 0;
@@ -291,7 +304,8 @@ struct Batch* batch_of(dfm_batch b);
 // host symbolic routines (host_symbolic.cpp)
 void build_pattern(int nnode, int nelem, int npoel, const int* lnods, std::vector<unsigned>& rowptr,
                    std::vector<unsigned>& colind, std::vector<int>& elsup_ptr, std::vector<int>& elsup);
-void build_device_pattern(Sys& s);  // from order/inv: rowptr, colind, diaind, perm_crs
+void build_device_pattern(Sys& s);  // from order/inv: rowptr, colind, diaind, perm_crs, element-store lists
+void build_elem_lists(Sys& s);      // the element-store lists for the current device numbering
 void build_schedule(Sys& s);        // SELL copy of the pattern, then build_sweeps
 void build_sweeps(Sys& s);          // factor pattern, fwd/bwd sweeps, lsplit/usplit, slabs, factor plan
 void make_multicolor(const Sys& s, std::vector<int>& order, std::vector<int>& color_of_new, int& ncolors);

@@ -82,10 +82,60 @@ void build_device_pattern(Sys& s) {
     }
   }
   s.diaind.assign((size_t)n, 0);
+  s.max_rowlen = 0;
   for (int i = 0; i < n; i++) {
+    s.max_rowlen = std::max(s.max_rowlen, s.rowptr[i + 1] - s.rowptr[i]);
     s.diaind[i] = s.rowptr[i + 1];
     for (unsigned q = s.rowptr[i]; q < s.rowptr[i + 1]; q++)
       if (s.colind[q] > (unsigned)i) { s.diaind[i] = q; break; }
+  }
+  build_elem_lists(s);
+}
+
+// Element-store lists (Sys::eptr/eidx/eloc). A device row visits its incident elements in ascending index, so each
+// entry's list comes out ascending.
+void build_elem_lists(Sys& s) {
+  const int n = s.n, NNO = s.npoel;
+  s.eptr.clear();
+  s.eidx.clear();
+  if (s.lnods.empty() || (unsigned long long)s.nelem * NNO * NNO >= 0xFFFFFFFFull) return;
+  if (s.eloc.size() != s.elsup.size()) {
+    s.eloc.assign(s.elsup.size(), 0);
+    for (int i1 = 0; i1 < n; i1++)
+      for (int k = s.elsup_ptr[i1]; k < s.elsup_ptr[i1 + 1]; k++) {
+        const int e = s.elsup[k];
+        int a = 0;
+        for (int c = 0; c < NNO; c++)
+          if (s.lnods[(size_t)e * NNO + c] == i1) a = c;
+        s.eloc[k] = (unsigned)e * NNO + (unsigned)a;
+      }
+  }
+  const size_t ncrs = s.colind.size();
+  s.eptr.assign(ncrs + 1, 0);
+  auto entry = [&](int i0, unsigned j0) {
+    const unsigned* b = s.colind.data() + s.rowptr[i0];
+    const unsigned* e = s.colind.data() + s.rowptr[i0 + 1];
+    return (size_t)(std::lower_bound(b, e, j0) - s.colind.data());
+  };
+  for (int pass = 0; pass < 2; pass++) {
+    std::vector<unsigned> fill;
+    if (pass == 1) {
+      for (size_t q = 0; q < ncrs; q++) s.eptr[q + 1] += s.eptr[q];
+      s.eidx.assign(s.eptr[ncrs], 0);
+      fill.assign(s.eptr.begin(), s.eptr.end() - 1);
+    }
+    for (int i0 = 0; i0 < n; i0++) {
+      const int i1 = s.order[i0];
+      for (int k = s.elsup_ptr[i1]; k < s.elsup_ptr[i1 + 1]; k++) {
+        const unsigned ea = s.eloc[k], e = ea / NNO, a = ea % NNO;
+        for (int c = 0; c < NNO; c++) {
+          if ((unsigned)c == a) continue;
+          const size_t q = entry(i0, (unsigned)s.inv[s.lnods[(size_t)e * NNO + c]]);
+          if (pass == 0) s.eptr[q + 1]++;
+          else s.eidx[fill[q]++] = ea * NNO + (unsigned)c;
+        }
+      }
+    }
   }
 }
 

@@ -1,9 +1,14 @@
-// Element assembly on the device: one thread per matrix row (device numbering) gathers the contributions
-// of the elements around its node in ascending element index, which is the order in which the reference's
-// element loop merges them (CMatDia_BlkCrs::Mearge, src/matvec/matdia_blkcrs.cpp:165-258, called per element
-// from AddLinSys_*). Every entry is therefore the same sum, in the same order, as in the reference: the
-// result is bit-identical, deterministic and needs no atomics. Each element row is recomputed by each of
-// its nodes (3 or 4 times); the element kernels are cheap next to the memory traffic of the sweep.
+// Element assembly on the device. Every matrix entry is the sum of its elements' contributions in ascending element
+// index, the order in which the reference's element loop merges them (CMatDia_BlkCrs::Mearge,
+// src/matvec/matdia_blkcrs.cpp:165-258, called per element from AddLinSys_*): the result is bit-identical,
+// deterministic and needs no atomics. Three paths: (1) linear elasticity: one kernel, a group of lanes per node
+// computes the node's block row of each incident element into LDS and each lane sums one entry of the row
+// (k_asm_solid_rowgroups); (2) the other kinds, and elasticity rows too long for (1): the element store, one
+// thread per element computes each of its block rows once and writes them (elasticity: only the element geometry);
+// one thread per off-diagonal entry then sums its elements' blocks through a list built with the pattern, one thread
+// per row its diagonal block and residual; in (1) and (2) every entry is written once, the matrix needs no zeroing.
+// (3) where the store is not built: one thread per row recomputes the rows of its incident elements and adds them
+// into the zeroed matrix.
 //
 // The element arithmetic follows the reference expression by expression; the library is compiled with
 // -ffp-contract=off so that no product is fused into an FMA.
@@ -115,22 +120,24 @@ __device__ inline void row_diffusion2d(int a, const double co[4][3], const doubl
 }
 
 // src/femeqn/eqn_linear_solid2d.cpp:505-598. p = (lambda, myu, rho, gx, gy)
-__device__ inline void row_solid2d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
-                                   double* er) {
+// Block (a, j) of the element matrix from the element's geometry (area, the gradients da = dldx[a], dj = dldx[j])
+// and Lame parameters.
+__device__ inline void solid2d_block(double area, const double* da, const double* dj, double lambda, double myu,
+                                     double* e) {
+  e[0] = area * (lambda + myu) * da[0] * dj[0];
+  e[1] = area * (lambda * da[0] * dj[1] + myu * dj[0] * da[1]);
+  e[2] = area * (lambda * da[1] * dj[0] + myu * dj[1] * da[0]);
+  e[3] = area * (lambda + myu) * da[1] * dj[1];
+  const double t = (da[1] * dj[1] + da[0] * dj[0]) * area * myu;
+  e[0] += t;
+  e[3] += t;
+}
+
+// Block row a and the residual of node a from the element's geometry.
+__device__ inline void solid2d_rows(int a, double area, const double d[3][2], const double u[4][3], const double* p,
+                                    double* em, double* er) {
   const double lambda = p[0], myu = p[1], rho = p[2], g_x = p[3], g_y = p[4];
-  const double area = tri_area(co[0], co[1], co[2]);
-  double d[3][2];
-  tri_dldx(d, co);
-  for (int j = 0; j < 3; j++) {
-    double* e = &em[j * 4];
-    e[0] = area * (lambda + myu) * d[a][0] * d[j][0];
-    e[1] = area * (lambda * d[a][0] * d[j][1] + myu * d[j][0] * d[a][1]);
-    e[2] = area * (lambda * d[a][1] * d[j][0] + myu * d[j][1] * d[a][0]);
-    e[3] = area * (lambda + myu) * d[a][1] * d[j][1];
-    const double t = (d[a][1] * d[j][1] + d[a][0] * d[j][0]) * area * myu;
-    e[0] += t;
-    e[3] += t;
-  }
+  for (int j = 0; j < 3; j++) solid2d_block(area, d[a], d[j], lambda, myu, &em[j * 4]);
   double r0 = area * rho * g_x * 0.33333333333333333;
   double r1 = area * rho * g_y * 0.33333333333333333;
   for (int j = 0; j < 3; j++) {
@@ -142,22 +149,51 @@ __device__ inline void row_solid2d(int a, const double co[4][3], const double u[
   er[1] = r1;
 }
 
-// src/femeqn/eqn_linear_solid3d.cpp:241-334. p = (lambda, myu, rho, gx, gy, gz)
-__device__ inline void row_solid3d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
-                                   double* er) {
-  const double lambda = p[0], myu = p[1], rho = p[2];
-  const double vol = tet_volume(co);
-  double d[4][3];
-  tet_dldx(d, co);
-  for (int j = 0; j < 4; j++) {
-    double* e = &em[j * 9];
-    for (int r = 0; r < 3; r++)
-      for (int s = 0; s < 3; s++) e[r * 3 + s] = vol * (lambda * d[a][r] * d[j][s] + myu * d[j][r] * d[a][s]);
-    const double t = d[a][0] * d[j][0] + d[a][1] * d[j][1] + d[a][2] * d[j][2];
-    e[0] += vol * myu * t;
-    e[4] += vol * myu * t;
-    e[8] += vol * myu * t;
+// solid2d_rows handing each block to put(j, block) as it is made: the same values, each residual value taking the
+// same operations in the same order.
+template <class Put>
+__device__ inline void solid2d_rows_put(int a, double area, const double d[3][2], const double u[4][3],
+                                        const double* p, double* er, Put&& put) {
+  const double lambda = p[0], myu = p[1], rho = p[2], g_x = p[3], g_y = p[4];
+  double r0 = area * rho * g_x * 0.33333333333333333;
+  double r1 = area * rho * g_y * 0.33333333333333333;
+  for (int j = 0; j < 3; j++) {
+    double e[4];
+    solid2d_block(area, d[a], d[j], lambda, myu, e);
+    put(j, e);
+    r0 -= e[0] * u[j][0] + e[1] * u[j][1];
+    r1 -= e[2] * u[j][0] + e[3] * u[j][1];
   }
+  er[0] = r0;
+  er[1] = r1;
+}
+
+__device__ inline void row_solid2d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
+                                   double* er) {
+  const double area = tri_area(co[0], co[1], co[2]);
+  double d[3][2];
+  tri_dldx(d, co);
+  solid2d_rows(a, area, d, u, p, em, er);
+}
+
+// src/femeqn/eqn_linear_solid3d.cpp:241-334. p = (lambda, myu, rho, gx, gy, gz)
+// Block (a, j) of the element matrix from the element's geometry (volume, the gradients da = dldx[a],
+// dj = dldx[j]) and Lame parameters.
+__device__ inline void solid3d_block(double vol, const double* da, const double* dj, double lambda, double myu,
+                                     double* e) {
+  for (int r = 0; r < 3; r++)
+    for (int s = 0; s < 3; s++) e[r * 3 + s] = vol * (lambda * da[r] * dj[s] + myu * dj[r] * da[s]);
+  const double t = da[0] * dj[0] + da[1] * dj[1] + da[2] * dj[2];
+  e[0] += vol * myu * t;
+  e[4] += vol * myu * t;
+  e[8] += vol * myu * t;
+}
+
+// Block row a and the residual of node a from the element's geometry.
+__device__ inline void solid3d_rows(int a, double vol, const double d[4][3], const double u[4][3], const double* p,
+                                    double* em, double* er) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  for (int j = 0; j < 4; j++) solid3d_block(vol, d[a], d[j], lambda, myu, &em[j * 9]);
   for (int r = 0; r < 3; r++) {
     double x = vol * rho * p[3 + r] * 0.25;
     for (int j = 0; j < 4; j++) {
@@ -166,6 +202,31 @@ __device__ inline void row_solid3d(int a, const double co[4][3], const double u[
     }
     er[r] = x;
   }
+}
+
+// solid3d_rows handing each block to put(j, block) as it is made: the same values, each residual value taking the
+// same operations in the same order.
+template <class Put>
+__device__ inline void solid3d_rows_put(int a, double vol, const double d[4][3], const double u[4][3],
+                                        const double* p, double* er, Put&& put) {
+  const double lambda = p[0], myu = p[1], rho = p[2];
+  double x[3];
+  for (int r = 0; r < 3; r++) x[r] = vol * rho * p[3 + r] * 0.25;
+  for (int j = 0; j < 4; j++) {
+    double e[9];
+    solid3d_block(vol, d[a], d[j], lambda, myu, e);
+    put(j, e);
+    for (int r = 0; r < 3; r++) x[r] -= e[r * 3 + 0] * u[j][0] + e[r * 3 + 1] * u[j][1] + e[r * 3 + 2] * u[j][2];
+  }
+  for (int r = 0; r < 3; r++) er[r] = x[r];
+}
+
+__device__ inline void row_solid3d(int a, const double co[4][3], const double u[4][3], const double* p, double* em,
+                                   double* er) {
+  const double vol = tet_volume(co);
+  double d[4][3];
+  tet_dldx(d, co);
+  solid3d_rows(a, vol, d, u, p, em, er);
 }
 
 // src/femeqn/eqn_st_venant.cpp:47-156 (AddElemMatFin_StVenant2D) and :157-272. p = (lambda, myu, rho, gx, gy)
@@ -582,6 +643,63 @@ __device__ inline void row_solid2d_beuler(int a, const double co[4][3], const do
 
 // ---------------------------------------------------------------------------------------------------------
 
+// The inputs of one element (nodes, coordinates, field values) and its block row a (em: NNO blocks, er: B values)
+// by the kind's element routine.
+template <int KIND, int NDIM, int B>
+struct ElemIn {
+  static constexpr int NNO = NDIM + 1;
+  int no[NNO];
+  double co[4][3], u[4][3], val[4], vval[4], vv[4][3], aa[4][3];
+  const double* p;
+};
+
+template <int KIND, int NDIM, int B>
+__device__ inline void load_elem(const AsmArgs& A, int e, ElemIn<KIND, NDIM, B>& E) {
+  constexpr int NNO = NDIM + 1;
+  constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER ||
+                       KIND == DFM_STVK3D_NEWMARK || KIND == DFM_STVK2D_NEWMARK || KIND == DFM_STVK2D_BEULER);
+  for (int c = 0; c < NNO; c++) E.no[c] = A.lnods[(long long)e * NNO + c];
+  for (int c = 0; c < NNO; c++) {
+    const long long nc = E.no[c];
+    for (int d = 0; d < 3; d++) E.co[c][d] = (d < NDIM) ? A.coords[nc * NDIM + d] : 0.0;
+    for (int d = 0; d < 3; d++) E.u[c][d] = (A.u && d < A.len) ? A.u[nc * A.len + d] : 0.0;
+    E.val[c] = A.u ? A.u[nc * A.len] : 0.0;
+    E.vval[c] = A.v ? A.v[nc * A.len] : 0.0;
+    if (NM)
+      for (int d = 0; d < 3; d++) {
+        E.vv[c][d] = (d < B) ? A.v[nc * B + d] : 0.0;
+        E.aa[c][d] = (d < B) ? A.acc[nc * B + d] : 0.0;
+      }
+  }
+  E.p = &A.params[(long long)e * A.pstride];
+}
+
+template <int KIND, int NDIM, int B, bool SAVE>
+__device__ inline void elem_row(const AsmArgs& A, const ElemIn<KIND, NDIM, B>& E, int a, double* em, double* er) {
+  const double* p = E.p;
+  if (SAVE) {
+    if (KIND == DFM_POISSON) row_poisson2d_save(a, E.co, p, em, er);
+    if (KIND == DFM_SOLID2D || KIND == DFM_SOLID3D) row_solid_save<NDIM>(a, E.co, p, em, er);
+  } else {
+    if (KIND == DFM_POISSON) row_poisson<NDIM>(a, E.co, E.val, p, em, er);
+    if (KIND == DFM_DIFFUSION2D) row_diffusion2d(a, E.co, E.val, E.vval, p, A.glob0, A.glob1, em, er);
+    if (KIND == DFM_SOLID2D) row_solid2d(a, E.co, E.u, p, em, er);
+    if (KIND == DFM_SOLID3D) row_solid3d(a, E.co, E.u, p, em, er);
+    if (KIND == DFM_STVK2D) row_stvk2d(a, E.co, E.u, p, em, er);
+    if (KIND == DFM_STVK3D) row_stvk3d(a, E.co, E.u, p, em, er);
+    if (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK)
+      row_solid_newmark<NDIM>(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+    if (KIND == DFM_STVK3D_NEWMARK)
+      row_stvk3d_newmark(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+    if (KIND == DFM_STVK2D_NEWMARK)
+      row_stvk2d_newmark(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
+    if (KIND == DFM_STVK2D_BEULER) row_stvk2d_beuler(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob1 != 0.0, em, er);
+    if (KIND == DFM_SOLID2D_BEULER) row_solid2d_beuler(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob1 != 0.0, em, er);
+  }
+}
+
+// Per-row gather (the fallback where the element store is not built): one thread per device row recomputes the
+// rows of its incident elements and adds them, in ascending element order, into the zeroed matrix and residual.
 template <int KIND, int NDIM, int B, bool SAVE>
 __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -596,53 +714,19 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
   for (int d = 0; d < B; d++) racc[d] = res[d];
   for (int k = A.elsup_ptr[i1]; k < A.elsup_ptr[i1 + 1]; k++) {
     const int e = A.elsup[k];
-    int no[NNO];
-    for (int c = 0; c < NNO; c++) no[c] = A.lnods[(long long)e * NNO + c];
+    ElemIn<KIND, NDIM, B> E;
+    load_elem<KIND, NDIM, B>(A, e, E);
     int a = 0;
     for (int c = 0; c < NNO; c++)
-      if (no[c] == i1) a = c;
-    constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER ||
-                         KIND == DFM_STVK3D_NEWMARK || KIND == DFM_STVK2D_NEWMARK || KIND == DFM_STVK2D_BEULER);
-    double co[4][3], u[4][3], val[4], vval[4];
-    double vv[4][3], aa[4][3];
-    for (int c = 0; c < NNO; c++) {
-      for (int d = 0; d < 3; d++) co[c][d] = (d < NDIM) ? A.coords[(long long)no[c] * NDIM + d] : 0.0;
-      for (int d = 0; d < 3; d++) u[c][d] = (A.u && d < A.len) ? A.u[(long long)no[c] * A.len + d] : 0.0;
-      val[c] = A.u ? A.u[(long long)no[c] * A.len] : 0.0;
-      vval[c] = A.v ? A.v[(long long)no[c] * A.len] : 0.0;
-      if (NM)
-        for (int d = 0; d < 3; d++) {
-          vv[c][d] = (d < B) ? A.v[(long long)no[c] * B + d] : 0.0;
-          aa[c][d] = (d < B) ? A.acc[(long long)no[c] * B + d] : 0.0;
-        }
-    }
-    const double* p = &A.params[(long long)e * A.pstride];
+      if (E.no[c] == i1) a = c;
     double em[NNO * BS], er[B];
-    if (SAVE) {
-      if (KIND == DFM_POISSON) row_poisson2d_save(a, co, p, em, er);
-      if (KIND == DFM_SOLID2D || KIND == DFM_SOLID3D) row_solid_save<NDIM>(a, co, p, em, er);
-    } else {
-      if (KIND == DFM_POISSON) row_poisson<NDIM>(a, co, val, p, em, er);
-      if (KIND == DFM_DIFFUSION2D) row_diffusion2d(a, co, val, vval, p, A.glob0, A.glob1, em, er);
-      if (KIND == DFM_SOLID2D) row_solid2d(a, co, u, p, em, er);
-      if (KIND == DFM_SOLID3D) row_solid3d(a, co, u, p, em, er);
-      if (KIND == DFM_STVK2D) row_stvk2d(a, co, u, p, em, er);
-      if (KIND == DFM_STVK3D) row_stvk3d(a, co, u, p, em, er);
-      if (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK)
-        row_solid_newmark<NDIM>(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
-      if (KIND == DFM_STVK3D_NEWMARK)
-        row_stvk3d_newmark(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
-      if (KIND == DFM_STVK2D_NEWMARK)
-        row_stvk2d_newmark(a, co, u, vv, aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
-      if (KIND == DFM_STVK2D_BEULER) row_stvk2d_beuler(a, co, u, vv, aa, p, A.glob0, A.glob1 != 0.0, em, er);
-      if (KIND == DFM_SOLID2D_BEULER) row_solid2d_beuler(a, co, u, vv, aa, p, A.glob0, A.glob1 != 0.0, em, er);
-    }
+    elem_row<KIND, NDIM, B, SAVE>(A, E, a, em, er);
     for (int c = 0; c < NNO; c++) {
       const double* blk = &em[c * BS];
       if (c == a) {
         for (int t = 0; t < BS; t++) dia[t] += blk[t];
       } else {
-        const unsigned j0 = (unsigned)(A.inv ? A.inv[no[c]] : no[c]);
+        const unsigned j0 = (unsigned)(A.inv ? A.inv[E.no[c]] : E.no[c]);
         for (unsigned q = q0; q < q1; q++) {
           if (A.colind[q] == j0) {
             double* o = &A.crs[(long long)q * BS];
@@ -662,12 +746,448 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
   for (int d = 0; d < B; d++) res[d] = racc[d];
 }
 
-hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
-  const int nt = 128;
-  const int nb = (a.n + nt - 1) / nt;
+// Element store: one thread per element computes every block row of it once (the element routine is called per
+// row a, as the row-gather kernel calls it); row a of element e (NNO blocks, then its B residual values) goes to
+// ebuf[(a*nelem + e)*STR ...]. A workgroup of 64 elements stages each row a in LDS and writes the 64 rows, which
+// are contiguous, with coalesced stores.
+template <int KIND, int NDIM, int B, bool SAVE>
+__global__ void __launch_bounds__(64) k_elem_store(AsmArgs A, int nelem, double* __restrict__ ebuf) {
+  constexpr int NNO = NDIM + 1;
+  constexpr int BS = B * B;
+  constexpr int STR = NNO * BS + B;
+  __shared__ double sh[64 * STR];
+  const int e0 = (int)blockIdx.x * 64, e = e0 + (int)threadIdx.x;
+  const int cnt = min(64, nelem - e0);
+  ElemIn<KIND, NDIM, B> E;
+  if (e < nelem) load_elem<KIND, NDIM, B>(A, e, E);
+  for (int a = 0; a < NNO; a++) {
+    if (e < nelem) {
+      double em[NNO * BS], er[B];
+      elem_row<KIND, NDIM, B, SAVE>(A, E, a, em, er);
+      double* o = &sh[threadIdx.x * STR];
+#pragma unroll
+      for (int t = 0; t < NNO * BS; t++) o[t] = em[t];
+#pragma unroll
+      for (int d = 0; d < B; d++) o[NNO * BS + d] = er[d];
+    }
+    __syncthreads();
+    double* out = &ebuf[((size_t)a * nelem + e0) * STR];
+    for (int t = (int)threadIdx.x; t < cnt * STR; t += 64) out[t] = sh[t];
+    __syncthreads();
+  }
+}
+
+// Off-diagonal blocks from the element store: one thread per matrix entry q sums the blocks (a, c) of the elements
+// holding both of its nodes, in ascending element order (the list eidx[eptr[q] .. eptr[q+1]) of (e*NNO + a)*NNO + c),
+// and writes the block once: the same sums in the same order as the reference's element loop merges them.
+template <int B, int NNO>
+__global__ void __launch_bounds__(256) k_gather_blocks(long long ncrs, int nelem, const unsigned* __restrict__ eptr,
+                                                      const unsigned* __restrict__ eidx,
+                                                      const double* __restrict__ ebuf, double* __restrict__ crs) {
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= ncrs) return;
+  constexpr int BS = B * B;
+  constexpr int STR = NNO * BS + B;
+  double acc[BS];
+#pragma unroll
+  for (int t = 0; t < BS; t++) acc[t] = 0.0;
+  for (unsigned k = eptr[q]; k < eptr[q + 1]; k++) {
+    const unsigned idx = eidx[k];
+    const unsigned ea = idx / NNO, c = idx - ea * NNO, e = ea / NNO, a = ea - e * NNO;
+    const double* src = &ebuf[((size_t)a * nelem + e) * STR + c * BS];
+#pragma unroll
+    for (int t = 0; t < BS; t++) acc[t] += src[t];
+  }
+#pragma unroll
+  for (int t = 0; t < BS; t++) crs[q * BS + t] = acc[t];
+}
+
+// Diagonal blocks and residual from the element store: one thread per device row sums its incident elements' blocks
+// (a, a) and residual rows in ascending element order (eloc: e*NNO + a per node-element incidence), then the surface
+// loads in face order.
+template <int B, int NNO>
+__global__ void __launch_bounds__(256) k_gather_rows(int n, int nelem, const int* __restrict__ order,
+                                                    const int* __restrict__ elsup_ptr, const unsigned* __restrict__ eloc,
+                                                    const double* __restrict__ ebuf, const int* __restrict__ load_ptr,
+                                                    const double* __restrict__ loads, double load_scale,
+                                                    double* __restrict__ dia, double* __restrict__ res) {
+  const int i0 = blockIdx.x * 256 + threadIdx.x;
+  if (i0 >= n) return;
+  constexpr int BS = B * B;
+  constexpr int STR = NNO * BS + B;
+  const int i1 = order ? order[i0] : i0;
+  double d[BS], r[B];
+#pragma unroll
+  for (int t = 0; t < BS; t++) d[t] = 0.0;
+#pragma unroll
+  for (int t = 0; t < B; t++) r[t] = 0.0;
+  for (int k = elsup_ptr[i1]; k < elsup_ptr[i1 + 1]; k++) {
+    const unsigned ea = eloc[k], e = ea / NNO, a = ea - e * NNO;
+    const double* src = &ebuf[((size_t)a * nelem + e) * STR];
+#pragma unroll
+    for (int t = 0; t < BS; t++) d[t] += src[a * BS + t];
+#pragma unroll
+    for (int t = 0; t < B; t++) r[t] += src[NNO * BS + t];
+  }
+  if (load_ptr)
+    for (int k = load_ptr[i1]; k < load_ptr[i1 + 1]; k++)
+#pragma unroll
+      for (int t = 0; t < B; t++) r[t] += load_scale * loads[(long long)k * B + t];
+#pragma unroll
+  for (int t = 0; t < BS; t++) dia[(size_t)i0 * BS + t] = d[t];
+#pragma unroll
+  for (int t = 0; t < B; t++) res[(size_t)i0 * B + t] = r[t];
+}
+
+// Linear elasticity stores per element only its geometry, measure and shape-function gradients (1 + NNO*NDIM
+// doubles), and the residual row of each of its nodes (NNO*NDIM doubles, from the element routine's full rows,
+// computed once per element): 25 doubles in 3D against 156 for the element rows. The gathers compute each block
+// from the geometry with the element routine's own block function, so the values are those of the full store, bit
+// for bit.
+template <int NDIM>
+struct SolidStore {
+  static constexpr int NNO = NDIM + 1, GS = 1 + NNO * NDIM, STR = GS + NNO * NDIM;
+};
+
+template <int NDIM>
+__global__ void __launch_bounds__(64) k_geom_store(AsmArgs A, int nelem, double* __restrict__ gbuf) {
+  constexpr int NNO = NDIM + 1, B = NDIM, BS = B * B, GS = SolidStore<NDIM>::GS, STR = SolidStore<NDIM>::STR;
+  __shared__ double sh[64 * STR];
+  const int e0 = (int)blockIdx.x * 64, e = e0 + (int)threadIdx.x;
+  const int cnt = min(64, nelem - e0);
+  if (e < nelem) {
+    double co[4][3], u[4][3];
+    for (int c = 0; c < NNO; c++) {
+      const long long nc = A.lnods[(long long)e * NNO + c];
+      for (int d = 0; d < 3; d++) co[c][d] = d < NDIM ? A.coords[nc * NDIM + d] : 0.0;
+      for (int d = 0; d < 3; d++) u[c][d] = (A.u && d < A.len) ? A.u[nc * A.len + d] : 0.0;
+    }
+    const double* p = &A.params[(long long)e * A.pstride];
+    double* o = &sh[threadIdx.x * STR];
+    double em[NNO * BS], er[B];
+    if (NDIM == 3) {
+      double dd[4][3];
+      const double vol = tet_volume(co);
+      tet_dldx(dd, co);
+      o[0] = vol;
+      for (int c = 0; c < 4; c++)
+        for (int k = 0; k < 3; k++) o[1 + c * 3 + k] = dd[c][k];
+      for (int a = 0; a < 4; a++) {
+        solid3d_rows(a, vol, dd, u, p, em, er);
+        for (int k = 0; k < 3; k++) o[GS + a * 3 + k] = er[k];
+      }
+    } else {
+      double dd[3][2];
+      const double area = tri_area(co[0], co[1], co[2]);
+      tri_dldx(dd, co);
+      o[0] = area;
+      for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 2; k++) o[1 + c * 2 + k] = dd[c][k];
+      for (int a = 0; a < 3; a++) {
+        solid2d_rows(a, area, dd, u, p, em, er);
+        for (int k = 0; k < 2; k++) o[GS + a * 2 + k] = er[k];
+      }
+    }
+  }
+  __syncthreads();
+  double* out = &gbuf[(size_t)e0 * STR];
+  for (int t = (int)threadIdx.x; t < cnt * STR; t += 64) out[t] = sh[t];
+}
+
+// block (a, c) of element e from its stored geometry (only the gradients of nodes a and c are read)
+template <int NDIM>
+__device__ inline void solid_block_stored(const double* __restrict__ g, unsigned a, unsigned c, const double* p,
+                                          double* blk) {
+  double da[NDIM], dc[NDIM];
+#pragma unroll
+  for (int k = 0; k < NDIM; k++) {
+    da[k] = g[1 + a * NDIM + k];
+    dc[k] = g[1 + c * NDIM + k];
+  }
+  if constexpr (NDIM == 3) solid3d_block(g[0], da, dc, p[0], p[1], blk);
+  else solid2d_block(g[0], da, dc, p[0], p[1], blk);
+}
+
+// The gathers load the data of up to four contributions before adding any (independent loads in flight together);
+// the sums still run in list order.
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_gather_blocks_solid(long long ncrs, const unsigned* __restrict__ eptr,
+                                                            const unsigned* __restrict__ eidx,
+                                                            const double* __restrict__ gbuf,
+                                                            const double* __restrict__ params, int pstride,
+                                                            double* __restrict__ crs) {
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= ncrs) return;
+  constexpr int NNO = NDIM + 1, B = NDIM, BS = B * B, STR = SolidStore<NDIM>::STR;
+  double acc[BS];
+#pragma unroll
+  for (int t = 0; t < BS; t++) acc[t] = 0.0;
+  const unsigned k0 = eptr[q], k1 = eptr[q + 1];
+  for (unsigned k = k0; k < k1; k += 4) {
+    double blk[4][BS];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (k + u < k1) {
+        const unsigned idx = eidx[k + u];
+        const unsigned ea = idx / NNO, c = idx - ea * NNO, e = ea / NNO, a = ea - e * NNO;
+        solid_block_stored<NDIM>(&gbuf[(size_t)e * STR], a, c, &params[(size_t)e * pstride], blk[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (k + u < k1)
+#pragma unroll
+        for (int t = 0; t < BS; t++) acc[t] += blk[u][t];
+  }
+#pragma unroll
+  for (int t = 0; t < BS; t++) crs[q * BS + t] = acc[t];
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_gather_rows_solid(AsmArgs A, const unsigned* __restrict__ eloc,
+                                                          const double* __restrict__ gbuf) {
+  const int i0 = blockIdx.x * 256 + threadIdx.x;
+  if (i0 >= A.n) return;
+  constexpr int NNO = NDIM + 1, B = NDIM, BS = B * B, GS = SolidStore<NDIM>::GS, STR = SolidStore<NDIM>::STR;
+  const int i1 = A.order ? A.order[i0] : i0;
+  double dg[BS], r[B];
+#pragma unroll
+  for (int t = 0; t < BS; t++) dg[t] = 0.0;
+#pragma unroll
+  for (int t = 0; t < B; t++) r[t] = 0.0;
+  const int k0 = A.elsup_ptr[i1], k1 = A.elsup_ptr[i1 + 1];
+  for (int k = k0; k < k1; k += 4) {
+    double blk[4][BS], er[4][B];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (k + u < k1) {
+        const unsigned ea = eloc[k + u], e = ea / NNO, a = ea - e * NNO;
+        const double* g = &gbuf[(size_t)e * STR];
+        solid_block_stored<NDIM>(g, a, a, &A.params[(long long)e * A.pstride], blk[u]);
+#pragma unroll
+        for (int t = 0; t < B; t++) er[u][t] = g[GS + a * B + t];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (k + u < k1) {
+#pragma unroll
+        for (int t = 0; t < BS; t++) dg[t] += blk[u][t];
+#pragma unroll
+        for (int t = 0; t < B; t++) r[t] += er[u][t];
+      }
+  }
+  if (A.load_ptr)
+    for (int k = A.load_ptr[i1]; k < A.load_ptr[i1 + 1]; k++)
+#pragma unroll
+      for (int t = 0; t < B; t++) r[t] += A.load_scale * A.loads[(long long)k * B + t];
+#pragma unroll
+  for (int t = 0; t < BS; t++) A.dia[(size_t)i0 * BS + t] = dg[t];
+#pragma unroll
+  for (int t = 0; t < B; t++) A.res[(size_t)i0 * B + t] = r[t];
+}
+
+// Linear elasticity without any store: one group of LPR lanes per node, nodes taken in their original numbering (so
+// that the groups running together share elements in cache) and spread over the XCDs in contiguous ranges. Per chunk
+// of up to CH incident elements (ascending index) each of CH lanes computes one element's block row and residual row
+// for the node with the element routine, into LDS; then each lane, holding one entry of the node's block row (its
+// off-diagonal entries in pattern order, then the diagonal block), runs through the chunk in order and adds the
+// element's block for its column where the element holds that column. Every entry and the residual receive the
+// element routine's values in ascending element order, as the reference's element loop merges them, and every entry
+// of the matrix and residual is written once (entries no element touches become zero). Needs row length + 1 <= LPR;
+// the workgroup is one wave, so its barriers only order the LDS traffic. CH is half of LPR from 16 lanes on, which
+// keeps the LDS per wave near 11 KB (3D) so that the register budget, not the LDS, bounds the waves per CU.
+// The field stride of the row-group records (at least nrec doubles): the one whose reads of one chunk position by
+// all lanes (group g, block c of BS doubles) fall on the fewest equal banks of 32 four-byte banks.
+constexpr int rowgroup_field_stride(int nrec, int BS, int NNO, int G) {
+  int best = nrec, best_cost = 1 << 30;
+  for (int f = nrec; f < nrec + 32; f++) {
+    int hits[32] = {};
+    for (int c = 0; c < NNO; c++)
+      for (int g = 0; g < G; g++)
+        for (int w = 0; w < 2; w++) hits[(c * BS * f * 2 + g * 2 + w) % 32]++;
+    int cost = 0;
+    for (int b = 0; b < 32; b++) cost = hits[b] > cost ? hits[b] : cost;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = f;
+    }
+  }
+  return best;
+}
+
+template <int NDIM, int LPR>
+__global__ void __launch_bounds__(64) k_asm_solid_rowgroups(AsmArgs A) {
+  constexpr int NNO = NDIM + 1, B = NDIM, BS = B * B, G = 64 / LPR, CH = LPR >= 16 ? LPR / 2 : LPR;
+  constexpr int FE = NNO * BS, NF = FE + B;  // the node's blocks (a, 0..NNO-1), then its residual row
+  constexpr int FSTR = rowgroup_field_stride(CH * G, BS, NNO, G);
+  __shared__ double rec[NF * FSTR];
+  __shared__ int nod[NNO * CH * G];  // element nodes
+  const unsigned per = gridDim.x / 8;     // the grid is a multiple of 8; workgroup w runs on XCD w % 8
+  const unsigned lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int g = (int)threadIdx.x / LPR, s = (int)threadIdx.x % LPR;
+  const long long i1l = (long long)lb * G + g;
+  const bool active = i1l < A.n;
+  const int i1 = active ? (int)i1l : 0;
+  // the row in the original pattern (column in original numbering, device entry), so that no load waits for the
+  // ordering maps; the device row is needed only for the diagonal block and the residual
+  int k0 = 0, k1 = 0, col = -1;
+  unsigned q = 0;
+  bool diag = false;
+  if (active) {
+    const unsigned q0 = A.rowptr0[i1], len = A.rowptr0[i1 + 1] - q0;
+    if ((unsigned)s < len) {
+      col = (int)A.colind0[q0 + s];
+      q = A.permq[q0 + s];
+    } else if ((unsigned)s == len) {
+      col = i1;
+      diag = true;
+    }
+    k0 = A.elsup_ptr[i1];
+    k1 = A.elsup_ptr[i1 + 1];
+  }
+  // every lane of the wave runs as many chunks as its longest element list needs
+  int nch = (k1 - k0 + CH - 1) / CH;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) nch = max(nch, __shfl_xor(nch, off));
+  auto at = [&](int f, int m) { return f * FSTR + m * G + g; };
+  auto ati = [&](int f, int m) { return (f * CH + m) * G + g; };
+  double acc[BS], r[B];
+#pragma unroll
+  for (int t = 0; t < BS; t++) acc[t] = 0.0;
+#pragma unroll
+  for (int t = 0; t < B; t++) r[t] = 0.0;
+  // a staging lane loads the element index and nodes (per incidence, elnod) of its next chunk one chunk ahead, so
+  // a chunk waits only for coordinates, field values and parameters
+  const int4* elnod = reinterpret_cast<const int4*>(A.elnod);
+  int en = -1;
+  int4 nn = make_int4(0, 0, 0, 0);
+  if (s < CH && k0 + s < k1) {
+    en = A.elsup[k0 + s];
+    nn = elnod[k0 + s];
+  }
+  for (int ch = 0; ch < nch; ch++) {
+    const int kb = k0 + ch * CH;
+    const int e = en;
+    const int no[4] = {nn.x, nn.y, nn.z, nn.w};
+    en = -1;
+    if (s < CH && kb + CH + s < k1) {
+      en = A.elsup[kb + CH + s];
+      nn = elnod[kb + CH + s];
+    }
+    __syncthreads();
+    if (e >= 0) {
+      double co[4][3], u[4][3];
+      int a = 0;
+#pragma unroll
+      for (int c = 0; c < NNO; c++) {
+        const int nc = no[c];
+        nod[ati(c, s)] = nc;
+        if (nc == i1) a = c;
+#pragma unroll
+        for (int d = 0; d < 3; d++) co[c][d] = d < NDIM ? A.coords[(long long)nc * NDIM + d] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) u[c][d] = (A.u && d < A.len) ? A.u[(long long)nc * A.len + d] : 0.0;
+      }
+      const double* p = &A.params[(long long)e * A.pstride];
+      auto put = [&](int j, const double* blk) {
+#pragma unroll
+        for (int t = 0; t < BS; t++) rec[at(j * BS + t, s)] = blk[t];
+      };
+      double er[B];
+      if constexpr (NDIM == 3) {
+        double dd[4][3];
+        const double vol = tet_volume(co);
+        tet_dldx(dd, co);
+        solid3d_rows_put(a, vol, dd, u, p, er, put);
+      } else {
+        double dd[3][2];
+        const double area = tri_area(co[0], co[1], co[2]);
+        tri_dldx(dd, co);
+        solid2d_rows_put(a, area, dd, u, p, er, put);
+      }
+#pragma unroll
+      for (int t = 0; t < B; t++) rec[at(FE + t, s)] = er[t];
+    }
+    __syncthreads();
+    const int cnt = min(CH, k1 - kb);
+    for (int m = 0; m < cnt; m++) {
+      int c = -1;
+#pragma unroll
+      for (int t = 0; t < NNO; t++)
+        if (nod[ati(t, m)] == col) c = t;
+      if (c < 0) continue;
+#pragma unroll
+      for (int t = 0; t < BS; t++) acc[t] += rec[at(c * BS + t, m)];
+      if (diag)
+#pragma unroll
+        for (int t = 0; t < B; t++) r[t] += rec[at(FE + t, m)];
+    }
+  }
+  if (!active || col < 0) return;
+  if (!diag) {
+#pragma unroll
+    for (int t = 0; t < BS; t++) A.crs[(size_t)q * BS + t] = acc[t];
+    return;
+  }
+  if (A.load_ptr)
+    for (int k = A.load_ptr[i1]; k < A.load_ptr[i1 + 1]; k++)
+#pragma unroll
+      for (int t = 0; t < B; t++) r[t] += A.load_scale * A.loads[(long long)k * B + t];
+  const int i0 = A.inv ? A.inv[i1] : i1;
+#pragma unroll
+  for (int t = 0; t < BS; t++) A.dia[(size_t)i0 * BS + t] = acc[t];
+#pragma unroll
+  for (int t = 0; t < B; t++) A.res[(size_t)i0 * B + t] = r[t];
+}
+
+template <int NDIM>
+static hipError_t launch_solid_rowgroups(const AsmArgs& a, int lpr, hipStream_t st) {
+  if (!a.rowptr0 || !a.colind0 || !a.permq || !a.elnod) return hipErrorInvalidValue;
+  const long long groups = 64 / lpr;
+  const long long nblk = (a.n + groups - 1) / groups;
+  const dim3 grid((unsigned)((nblk + 7) / 8 * 8));
+  if (lpr == 8) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 8>), grid, dim3(64), 0, st, a);
+  else if (lpr == 16) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 16>), grid, dim3(64), 0, st, a);
+  else if (lpr == 32) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 32>), grid, dim3(64), 0, st, a);
+  else if (lpr == 64) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 64>), grid, dim3(64), 0, st, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int KIND, int NDIM, int B, bool SAVE>
+static hipError_t assemble_kind(const AsmArgs& a, const ElemStore* es, hipStream_t st) {
+  constexpr int NNO = NDIM + 1;
+  if (!es) {
+    const int nt = 128;
+    hipLaunchKernelGGL((k_assemble<KIND, NDIM, B, SAVE>), dim3((a.n + nt - 1) / nt), dim3(nt), 0, st, a);
+    return hipGetLastError();
+  }
+  const dim3 ge((unsigned)((es->nelem + 63) / 64));
+  const dim3 gq((unsigned)((es->ncrs + 255) / 256));
+  if (!SAVE && (KIND == DFM_SOLID2D || KIND == DFM_SOLID3D)) {
+    if (es->lpr > 0) return launch_solid_rowgroups<NDIM>(a, es->lpr, st);
+    hipLaunchKernelGGL((k_geom_store<NDIM>), ge, dim3(64), 0, st, a, es->nelem, es->ebuf);
+    if (es->ncrs > 0)
+      hipLaunchKernelGGL((k_gather_blocks_solid<NDIM>), gq, dim3(256), 0, st, es->ncrs, es->eptr, es->eidx,
+                         (const double*)es->ebuf, a.params, a.pstride, a.crs);
+    hipLaunchKernelGGL((k_gather_rows_solid<NDIM>), dim3((a.n + 255) / 256), dim3(256), 0, st, a, es->eloc,
+                       (const double*)es->ebuf);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((k_elem_store<KIND, NDIM, B, SAVE>), ge, dim3(64), 0, st, a, es->nelem, es->ebuf);
+  if (es->ncrs > 0)
+    hipLaunchKernelGGL((k_gather_blocks<B, NNO>), gq, dim3(256), 0, st, es->ncrs, es->nelem, es->eptr, es->eidx,
+                       (const double*)es->ebuf, a.crs);
+  hipLaunchKernelGGL((k_gather_rows<B, NNO>), dim3((a.n + 255) / 256), dim3(256), 0, st, a.n, es->nelem, a.order,
+                     a.elsup_ptr, es->eloc, (const double*)es->ebuf, a.load_ptr, a.loads, a.load_scale, a.dia, a.res);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble(const AsmArgs& a, const ElemStore* es, hipStream_t st) {
   if (a.n == 0) return hipSuccess;
 #define DFM_ASM_SAVE(K, D, B) \
-  if (a.kind == K && a.ndim == D && a.len == B) { hipLaunchKernelGGL((k_assemble<K, D, B, true>), dim3(nb), dim3(nt), 0, st, a); return hipGetLastError(); }
+  if (a.kind == K && a.ndim == D && a.len == B) return assemble_kind<K, D, B, true>(a, es, st);
   if (a.save) {
     DFM_ASM_SAVE(DFM_POISSON, 2, 1)
     DFM_ASM_SAVE(DFM_SOLID2D, 2, 2)
@@ -676,7 +1196,7 @@ hipError_t launch_assemble(const AsmArgs& a, hipStream_t st) {
   }
 #undef DFM_ASM_SAVE
 #define DFM_ASM(K, D, B) \
-  if (a.kind == K && a.ndim == D && a.len == B) { hipLaunchKernelGGL((k_assemble<K, D, B, false>), dim3(nb), dim3(nt), 0, st, a); return hipGetLastError(); }
+  if (a.kind == K && a.ndim == D && a.len == B) return assemble_kind<K, D, B, false>(a, es, st);
   DFM_ASM(DFM_POISSON, 2, 1)
   DFM_ASM(DFM_POISSON, 3, 1)
   DFM_ASM(DFM_DIFFUSION2D, 2, 1)

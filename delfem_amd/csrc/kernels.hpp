@@ -26,13 +26,31 @@ struct AsmArgs {
   const double* loads;   // [nentries*len]
   double load_scale;     // multiplies the surface loads (load stepping); 1 = as given
   const unsigned* rowptr;  // device numbering pattern
+  const unsigned* rowptr0;  // original numbering pattern, and per original entry its device entry (may be null)
+  const unsigned* colind0;
+  const unsigned* permq;
+  const int* elnod;  // per node-element incidence (elsup order) the element's nodes, 4 ints (unused -1); may be null
   const unsigned* colind;
   double* crs;
   double* dia;
   double* res;
   bool save;  // the *_SaveStiffMat element loops: external force into res, no internal force
 };
-hipError_t launch_assemble(const AsmArgs& a, hipStream_t st);
+// The element store of an assembly: ebuf holds nelem * NNO rows of NNO*B*B + B doubles; per off-diagonal entry q the
+// list eidx[eptr[q] .. eptr[q+1]) of (e*NNO + a)*NNO + c in ascending element order; per node-element incidence
+// (elsup order) eloc = e*NNO + a. Null: the per-row gather, into a matrix and residual zeroed beforehand.
+// lpr > 0 (linear elasticity): the row-group kernel, lpr lanes per node, runs instead and needs neither lists nor
+// buffer.
+struct ElemStore {
+  int nelem;
+  int lpr;
+  long long ncrs;
+  const unsigned* eptr;
+  const unsigned* eidx;
+  const unsigned* eloc;
+  double* ebuf;
+};
+hipError_t launch_assemble(const AsmArgs& a, const ElemStore* es, hipStream_t st);
 hipError_t launch_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
                          double* res, const int* bc, hipStream_t st);
 hipError_t launch_update_field(int n, int len, const int* order, const double* upd, double* u, hipStream_t st);

@@ -293,8 +293,9 @@ int dfm_sys_last_solve_info(dfm_sys sys, int* reliable_updates, int* fell_back);
  * (FP32) / FP64 loop runs them (forward phases with the r and x update, backward phases with (r, z)), 6 the
  * MakeResidual kernel of a saved system (it rewrites the residual and update handles as dfm_sys_make_residual
  * does), 7 the numeric ILU factorisation of the assembled matrix (its initial values and the factor kernels of
- * every phase; it rewrites the same factor). Also reports the algorithmic bytes of one launch (all phases). No
- * counterpart in the reference. */
+ * every phase; it rewrites the same factor), 8 the assembly kernels of the last dfm_sys_assemble[_resident] (the
+ * system must be assembled again before it is solved). Also reports the algorithmic bytes of one launch (all
+ * phases). No counterpart in the reference. */
 int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, double* bytes);
 /* ---- batches: many systems solved together (serving, instance batches) ------------------------------
  * A batch advances every system added to it with one launch per kernel for all running systems of one block

@@ -163,3 +163,53 @@ def test_reference_orderings_of_a_system(ctx, kind):
     cpu, gs, order, _ = setup(N, ctx, KINDS[0], N.ORDER_NATURAL)
     o, _ = gs.make_ordering(kind)
     assert np.array_equal(o, P.ordering(kind, cpu.rowptr, cpu.colind))
+
+
+def _fan(ndim, k, seed):
+    """k triangles around one centre node (2D), or k tetrahedra around an axis of two nodes (3D): the centre (axis)
+    rows hold k (k + 1) off-diagonal blocks, past the row-group kernel's lanes for k >= 64."""
+    rng = np.random.default_rng(seed)
+    ang = (np.arange(k) + rng.uniform(-0.2, 0.2, k)) * (2 * np.pi / k)
+    rad = rng.uniform(0.8, 1.2, k)
+    ring = np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)
+    if ndim == 2:
+        co = np.concatenate([[[0.0, 0.0]], ring]).ravel()
+        ln = np.array([[0, 1 + i, 1 + (i + 1) % k] for i in range(k)], np.int32).ravel()
+    else:
+        co = np.concatenate([[[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]],
+                             np.concatenate([ring, rng.uniform(-0.1, 0.1, (k, 1))], 1)]).ravel()
+        ln = np.array([[0, 1, 2 + i, 2 + (i + 1) % k] for i in range(k)], np.int32).ravel()
+    return co, ln
+
+
+@pytest.mark.parametrize("ndim,k", [(2, 20), (2, 40), (2, 70), (3, 20), (3, 40), (3, 70)])
+@pytest.mark.parametrize("oname,ordering", [("natural", 0), ("blockcolor", 2)])
+def test_assembly_high_valence_bit_exact(ctx, ndim, k, oname, ordering):
+    """Linear elasticity on rows of every lane count of the row-group kernel (16, 32, 64 lanes) and on rows too long
+    for it (the geometry store and per-entry gathers)."""
+    co, ln = _fan(ndim, k, 40 + k)
+    case = KINDS[3] if ndim == 2 else KINDS[4]
+    _, _, length, kind, pfun, _, _ = case
+    n = co.size // ndim
+    rng = np.random.default_rng(k)
+    params = pfun(rng, ln.size // (ndim + 1))
+    u = rng.uniform(-0.05, 0.05, n * length)
+    fl = np.zeros(n * length, np.int32)
+    fl[-length:] = 1
+    cpu = P.CpuSystem(ndim, co, ln, length)
+    cpu.assemble(kind, params, u, None, (0.05, 0.6))
+    cpu.set_bc(fl)
+    gs = N.System(ctx, ndim, co, ln, length)
+    if ordering != 0:
+        order, _ = gs.make_ordering(ordering)
+        gs.set_ordering(order)
+    gs.set_bc(fl)
+    gs.set_field(u, None)
+    gs.assemble(kind, params, glob=(0.05, 0.6))
+    rp, _ = gs.pattern()
+    assert np.diff(rp).max() == (k if ndim == 2 else k + 1)
+    crs, dia = gs.get_matrix()
+    assert np.array_equal(crs, cpu.crs), np.abs(crs - cpu.crs).max()
+    assert np.array_equal(dia, cpu.dia), np.abs(dia - cpu.dia).max()
+    assert np.array_equal(gs.get_vector(-1), cpu.res)
+    gs.close()
