@@ -503,3 +503,406 @@ class RefSystem:
 
     def update_field_save(self):
         reference().ref_sys_update_field_save(self.h)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Equation systems: CEqnSystem_Solid2D, CEqn_Solid3D_Linear, CEqnSystem_Scalar2D, CEqn_Scalar3D (include/delfem/
+# eqnsys_solid.h, eqnsys_scalar.h; src/femeqn/eqnsys_solid.cpp, eqnsys_scalar.cpp). RefEqn drives the reference's
+# own classes through oracle/ref_capi.cpp; EqnFlow restates their Solve flows over CpuSystem, and is what the GPU
+# equation systems are tested against.
+
+EQN_SOLID2D, EQN_SOLID3D, EQN_SCALAR2D, EQN_SCALAR3D = range(4)
+
+
+def _eqn_lib():
+    lib = reference()
+    if not getattr(lib, "_eqn_ready", False):
+        lib.ref_eqn_create.restype = C.c_void_p
+        lib.ref_eqn_create.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                       C.c_int, C.c_void_p, C.c_int]
+        lib.ref_eqn_destroy.argtypes = [C.c_void_p]
+        lib.ref_eqn_set_young_poisson.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]
+        lib.ref_eqn_set_rho.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        lib.ref_eqn_set_gravitation.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
+        lib.ref_eqn_set_flag.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.ref_eqn_set_scalar.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+        lib.ref_eqn_set_time.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        lib.ref_eqn_solve.argtypes = [C.c_void_p]
+        lib.ref_eqn_itr.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        lib.ref_eqn_get_field.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib._eqn_ready = True
+    return lib
+
+
+def _eqn_dims(type_):
+    ndim = 3 if type_ in (EQN_SOLID3D, EQN_SCALAR3D) else 2
+    length = {EQN_SOLID2D: 2, EQN_SOLID3D: 3}.get(type_, 1)
+    return ndim, length
+
+
+class RefEqn:
+    """The reference's equation system of `type_` on a mesh of element groups (one element array and CEqn_* per
+    group, elem_group per element) whose boundary elements `bnd` (lines in 2D, triangles in 3D) are fixed with
+    AddFixElemAry (component idof, -1 all)."""
+
+    def __init__(self, type_, coords, lnods, elem_group=None, ngroup=1, bnd=None, idof=-1):
+        self.type = type_
+        self.ndim, self.len = _eqn_dims(type_)
+        self.coords, self.lnods = _f64(coords), _i32(lnods)
+        self.nnode = self.coords.size // self.ndim
+        npoel = self.ndim + 1
+        eg = None if elem_group is None else _i32(elem_group)
+        bnd = np.zeros(0, np.int32) if bnd is None else _i32(bnd)
+        nbno = self.ndim
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        self.h = C.c_void_p(_eqn_lib().ref_eqn_create(type_, self.nnode, p(self.coords), self.lnods.size // npoel,
+                                                      p(self.lnods), ngroup, p(eg), bnd.size // nbno,
+                                                      p(bnd) if bnd.size else None, idof))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _eqn_lib().ref_eqn_destroy(self.h)
+            self.h = None
+
+    def set_young_poisson(self, young, poisson, is_plane_stress=True, group=-1):
+        _eqn_lib().ref_eqn_set_young_poisson(self.h, group, young, poisson, 1 if is_plane_stress else 0)
+
+    def set_rho(self, rho, group=-1):
+        _eqn_lib().ref_eqn_set_rho(self.h, group, rho)
+
+    def set_gravitation(self, gx, gy, gz=0.0, group=-1):
+        _eqn_lib().ref_eqn_set_gravitation(self.h, group, gx, gy, gz)
+
+    def set_flag(self, which, on):
+        """which: 0 geometrical nonlinearity, 1 stationary, 2 saved stiffness"""
+        _eqn_lib().ref_eqn_set_flag(self.h, which, 1 if on else 0)
+
+    def set_scalar(self, which, val, group=-1):
+        """which: 0 alpha, 1 capacity, 2 source"""
+        _eqn_lib().ref_eqn_set_scalar(self.h, which, group, val)
+
+    def set_time(self, dt, gamma=0.6, beta=-1.0):
+        _eqn_lib().ref_eqn_set_time(self.h, dt, gamma, beta)
+
+    def solve(self):
+        return bool(_eqn_lib().ref_eqn_solve(self.h))
+
+    def itr(self):
+        it = np.zeros(256, np.uint32)
+        nr = np.zeros(256)
+        n = _eqn_lib().ref_eqn_itr(self.h, it.ctypes.data_as(C.c_void_p), nr.ctypes.data_as(C.c_void_p), 256)
+        return [(int(it[k]), float(nr[k])) for k in range(n)]
+
+    def fields(self):
+        m = self.nnode * self.len
+        u, v, a = np.zeros(m), np.zeros(m), np.zeros(m)
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        _eqn_lib().ref_eqn_get_field(self.h, p(u), p(v), p(a))
+        return u, v, a
+
+
+def _solid_lame(young, poisson, is_plane_stress):
+    """CEqn_Solid2D::SetYoungPoisson (include/delfem/eqnsys_solid.h:190-196), including its flag: the plane-stress
+    reduction of lambda applies when is_plane_stress is false."""
+    lam = young * poisson / ((1.0 + poisson) * (1 - 2.0 * poisson))
+    myu = young / (2.0 * (1.0 + poisson))
+    if not is_plane_stress:
+        lam = 2 * lam * myu / (lam + 2 * myu)
+    return lam, myu
+
+
+class EqnFlow:
+    """The Solve flows of the reference's equation systems, restated over CpuSystem. The same constructor and
+    setters as RefEqn. Kinds: 0 Poisson, 1/2 linear solid 2D/3D, 3/4 St.Venant-Kirchhoff 2D/3D, 5 diffusion."""
+
+    def __init__(self, type_, coords, lnods, elem_group=None, ngroup=1, bnd=None, idof=-1):
+        self.type = type_
+        self.ndim, self.len = _eqn_dims(type_)
+        npoel = self.ndim + 1
+        coords, lnods = _f64(coords), _i32(lnods).reshape(-1, npoel)
+        eg = np.zeros(lnods.shape[0], np.int32) if elem_group is None else _i32(elem_group)
+        # the reference's element arrays: group by group, element order kept inside each group
+        perm = np.argsort(eg, kind="stable")
+        self.coords, self.lnods, self.group = coords, lnods[perm].ravel(), eg[perm]
+        self.ngroup, self.nnode = ngroup, coords.size // self.ndim
+        self.flags = np.zeros(self.nnode * self.len, np.int32)
+        if bnd is not None:
+            nodes = np.unique(_i32(bnd))
+            fl = self.flags.reshape(-1, self.len)
+            if idof < 0:
+                fl[nodes, :] = 1
+            else:
+                fl[nodes, idof] = 1
+        m = self.nnode * self.len
+        self.u, self.v, self.a = np.zeros(m), np.zeros(m), np.zeros(m)
+        self.dt, self.gamma, self.beta = 0.1, 0.6, 0.3025
+        self.itr_ = []
+        self.cs = None  # the linear system (pLS)
+        self.f = None  # the preconditioner's factor (pPrec); solids ILU(1), scalars ILU(0)
+        self.has_prec = False
+        self.cleared_ls = self.cleared_prec = True
+        self.save = False
+        self.nonlin = False
+        if type_ == EQN_SOLID2D:
+            lam, myu = _solid_lame(1.0, 0.0, True)
+            self.gp = [[lam, myu, 1.0, 0.0, 0.0] for _ in range(ngroup)]
+            self.stationary = True
+        elif type_ == EQN_SOLID3D:
+            self.gp = [[0.0, 0.0, 1.0, 0.0, 0.0, 0.0]]
+            self.stationary = False  # CEqn_Solid3D_Linear's constructor
+        else:
+            self.gp = [[1.0, 1.0, 0.0] for _ in range(ngroup)]  # alpha, capacity, source
+            self.stationary = True
+        self.lev = 1 if type_ in (EQN_SOLID2D, EQN_SOLID3D) else 0
+
+    # setters: the values of the reference's setters, and which of them drop the linear system
+    def _groups(self, group):
+        return range(len(self.gp)) if group < 0 else [group]
+
+    def set_young_poisson(self, young, poisson, is_plane_stress=True, group=-1):
+        if self.type == EQN_SOLID3D:
+            lam = young * poisson / ((1.0 + poisson) * (1 - 2.0 * poisson))
+            myu = young / (2.0 * (1.0 + poisson))
+        else:
+            lam, myu = _solid_lame(young, poisson, is_plane_stress)
+        for g in self._groups(group):
+            self.gp[g][0], self.gp[g][1] = lam, myu
+        self._clear_values(group)
+
+    def set_rho(self, rho, group=-1):
+        for g in self._groups(group):
+            self.gp[g][2] = rho
+        self._clear_values(group)
+
+    def set_gravitation(self, gx, gy, gz=0.0, group=-1):
+        for g in self._groups(group):
+            self.gp[g][3], self.gp[g][4] = gx, gy
+            if self.type == EQN_SOLID3D:
+                self.gp[g][5] = gz
+        self._clear_values(group)
+
+    def set_scalar(self, which, val, group=-1):
+        for g in self._groups(group):
+            self.gp[g][which] = val
+        self._clear_values(group)
+
+    def _clear_values(self, group):
+        # the system-level setters flag the values; SetEquation (one group) drops the linear system and the
+        # preconditioner (CEqnSystem_Solid2D::SetEquation, eqnsys_solid.cpp)
+        if group >= 0 and self.type in (EQN_SOLID2D, EQN_SCALAR2D):
+            self._clear_ls()
+        self.cleared_ls = self.cleared_prec = True
+
+    def _clear_ls(self):
+        self.cs, self.f, self.has_prec = None, None, False
+
+    def set_flag(self, which, on):
+        t = self.type
+        if which == 0:  # geometrical nonlinearity
+            if t == EQN_SOLID2D:
+                self.nonlin = bool(on)
+                if on:
+                    self.save = False
+                    self.cs = None
+            elif t == EQN_SOLID3D:
+                if on and not self.nonlin:
+                    self.nonlin, self.save = True, False
+                elif not on:
+                    self.nonlin = False
+            self.cleared_ls = self.cleared_prec = True
+        elif which == 1:  # stationary
+            if t == EQN_SOLID3D:
+                if bool(on) != self.stationary:
+                    self.stationary = bool(on)
+                    if not on:
+                        self.save = False
+                    self.cleared_ls = self.cleared_prec = True
+            else:
+                self.stationary = bool(on)
+                self._clear_ls()
+        elif which == 2:  # saved stiffness
+            if t == EQN_SOLID3D:
+                if on and not (self.save or self.nonlin or not self.stationary):
+                    self.save = True
+                    self.cleared_ls = self.cleared_prec = True
+                elif not on and self.save:
+                    self.save = False
+                    self.cleared_ls = self.cleared_prec = True
+            else:
+                self.save = bool(on) and not (t == EQN_SOLID2D and self.nonlin)
+                self._clear_ls()
+
+    def set_time(self, dt, gamma=0.6, beta=-1.0):
+        """CEqnSystem::SetTimeIntegrationParameter (include/delfem/eqnsys.h:88-93)"""
+        self.dt, self.gamma = dt, gamma
+        self.beta = 0.25 * (self.gamma + 0.5) * (self.gamma + 0.5) if beta < 0.0 else beta
+
+    # the linear system, the preconditioner and the element loops
+    def _params(self):
+        """per element, in the grouped element order"""
+        if self.type in (EQN_SCALAR2D, EQN_SCALAR3D):
+            gp = np.array(self.gp)
+            if self.stationary or self.type == EQN_SCALAR3D:
+                return np.stack([gp[self.group, 0], gp[self.group, 2]], 1)  # alpha, source
+            return np.stack([gp[self.group, 1], gp[self.group, 0], gp[self.group, 2]], 1)  # capacity, alpha, source
+        gp = np.array(self.gp)
+        return gp[self.group if self.type == EQN_SOLID2D else np.zeros_like(self.group)]
+
+    def _save_passes(self):
+        """CEqn_Solid2D::AddLinSys_Save (src/femeqn/eqnsys_solid.cpp:425-432) passes no element array, so each
+        group's equation assembles every element of the field with its own material: ngroup passes over all
+        elements, group by group. Returns the number of passes."""
+        reps = self.ngroup if (self.type == EQN_SOLID2D and self.save) else 1
+        return reps
+
+    def _init_ls(self):
+        reps = self._save_passes()
+        self.cs = CpuSystem(self.ndim, self.coords, np.tile(self.lnods, reps), self.len)
+        self.f, self.has_prec = None, False
+
+    def _make(self, initial):
+        """MakeLinearSystem: InitializeMarge, the element loop of the mode, FinalizeMarge; returns ||r||"""
+        if self.cs is None:
+            self._init_ls()
+        cs, t, prm = self.cs, self.type, self._params()
+        self.cleared_ls = False
+        if self.save:
+            if not self.stationary:
+                raise NotImplementedError("saved stiffness of a dynamic system (CLinearSystem_SaveDiaM_*)")
+            kind = {EQN_SOLID2D: 1, EQN_SOLID3D: 2}.get(t, 0)
+            if self._save_passes() > 1:
+                prm = np.concatenate([np.tile(np.array(self.gp[g]), (prm.shape[0], 1)) for g in range(self.ngroup)])
+            cs.assemble_save(kind, prm, self.flags)
+            return 0.0
+        cs.zero()
+        solid = t in (EQN_SOLID2D, EQN_SOLID3D)
+        if solid and not self.stationary:
+            glob = (self.dt, self.gamma, self.beta, 1.0 if initial else 0.0)
+            if self.nonlin:
+                cs.assemble_stvk3d_newmark(prm, self.u, self.v, self.a, glob)
+            else:
+                cs.assemble_newmark(prm, self.u, self.v, self.a, glob)
+        elif solid:
+            kind = {(EQN_SOLID2D, False): 1, (EQN_SOLID3D, False): 2, (EQN_SOLID2D, True): 3,
+                    (EQN_SOLID3D, True): 4}[(t, self.nonlin)]
+            cs.assemble(kind, prm, self.u)
+        elif t == EQN_SCALAR2D and not self.stationary:
+            cs.assemble(5, prm, self.u, self.v, (self.dt, self.gamma))
+        else:
+            cs.assemble(0, prm, self.u)
+        return cs.set_bc(self.flags)
+
+    def _make_prec(self):
+        self.f = self.cs.ilu(None, self.lev)
+        self.has_prec = True
+        self.cleared_prec = False
+
+    def _pcg(self, res=None):
+        ok, conv, it, x, _ = self.cs.solve(self.f, "pcg", 1.0e-6, 1000, res)
+        self.itr_.append((it, conv))
+        return x
+
+    def _update(self, x, initial=True):
+        if self.save:
+            _, self.u = self.cs.update_field_save(x, self.u)
+        elif self.stationary:
+            self.u = self.u + x
+        elif self.type in (EQN_SOLID2D, EQN_SOLID3D):
+            self.u, self.v, self.a = CpuSystem.update_field_newmark(self.gamma, self.beta, self.dt, initial, x,
+                                                                    self.u, self.v, self.a)
+        else:
+            # UpdateValueOfField_Newmark (src/femls/linearsystem_field.cpp:885-921), VELOCITY
+            u = self.u + self.dt * self.v
+            self.u = u + (self.gamma * self.dt) * x
+            self.v = self.v + 1.0 * x
+
+    def _residual_save(self):
+        _, res, _ = self.cs.make_residual(self.u)
+        return res
+
+    def solve(self):
+        t = self.type
+        if t == EQN_SOLID2D or t == EQN_SOLID3D:
+            return self._solve_solid(10 if t == EQN_SOLID2D else 40)
+        if t == EQN_SCALAR3D:
+            if self.cs is None:
+                self._init_ls()
+            self._make(True)
+            self._make_prec()
+            x = self._pcg()
+            self.itr_ = self.itr_[-1:]
+            self.u = self.u + x
+            return True
+        # CEqnSystem_Scalar2D::Solve (src/femeqn/eqnsys_scalar.cpp:294-359); its MakeLinearSystem also sets the
+        # preconditioner's values
+        res = None
+        if self.cs is None or not self.has_prec:
+            self._init_ls()
+            self._make(True)
+            self._make_prec()
+        elif not self.save:
+            self._make(True)
+            self._make_prec()
+        if self.save:
+            if self.cleared_ls or self.cleared_prec:
+                self._make(True)
+                self._make_prec()
+            res = self._residual_save()
+        x = self._pcg(res)
+        self.itr_ = self.itr_[-1:]
+        self._update(x, True)
+        return True
+
+    def _solve_solid(self, max_newton):
+        """CEqnSystem_Solid2D::Solve (src/femeqn/eqnsys_solid.cpp:671-761) and CEqn_Solid3D_Linear::Solve
+        (:175-247): a Newton loop for geometrical nonlinearity, else one linear solve"""
+        self.itr_ = []
+        d3 = self.type == EQN_SOLID3D
+        if self.nonlin:
+            if self.cs is None or (d3 and not self.has_prec):
+                self._init_ls()
+            ini = 0.0
+            for it in range(max_newton):
+                norm = self._make(it == 0)
+                self._make_prec()
+                if it == 0:
+                    if norm < 1.0e-20:
+                        break
+                    ini = norm
+                if norm < ini * 1.0e-6:
+                    break
+                x = self._pcg()
+                self._update(x, it == 0)
+            return True
+        res = None
+        if d3:
+            if self.cs is None or not self.has_prec:
+                self._init_ls()
+                self._make(True)
+                self._make_prec()
+            elif not self.save:
+                self._make(True)
+                self._make_prec()
+            if self.save:
+                res = self._residual_save()
+        else:
+            if self.cs is None:
+                self._init_ls()
+                self._make(True)
+            elif not self.save:
+                self._make(True)
+            if self.save:
+                if self.cleared_ls:
+                    self._make(True)
+                res = self._residual_save()
+            if not self.has_prec or not self.save or self.cleared_prec:
+                self._make_prec()
+        x = self._pcg(res)
+        self._update(x, True)
+        return True
+
+    def itr(self):
+        return list(self.itr_)
+
+    def fields(self):
+        return self.u.copy(), self.v.copy(), self.a.copy()

@@ -33,6 +33,8 @@
 #include "delfem/femeqn/eqn_linear_solid2d.h"
 #include "delfem/femeqn/eqn_linear_solid3d.h"
 #include "delfem/femeqn/eqn_st_venant.h"
+#include "delfem/eqnsys_solid.h"
+#include "delfem/eqnsys_scalar.h"
 
 namespace {
 
@@ -43,6 +45,7 @@ class CGroupMesh : public Msh::IMesh {
   std::vector<double> coord;
   std::vector<std::vector<int> > groups;  // lnods per group
   Msh::MSH_TYPE type;
+  std::vector<Msh::MSH_TYPE> types;  // per group where set (boundary groups of other element types)
   virtual unsigned int GetDimention() const { return ndim; }
   virtual void GetInfo(unsigned int id_msh, unsigned int& id_cad, unsigned int& id_before_ext,
                        unsigned int& inum_ext, int& ilayer) const {
@@ -51,7 +54,7 @@ class CGroupMesh : public Msh::IMesh {
   virtual void GetCoord(std::vector<double>& c) const { c = coord; }
   virtual Msh::MSH_TYPE GetConnectivity(unsigned int id_msh, std::vector<int>& lnods) const {
     lnods = groups[id_msh - 1];
-    return type;
+    return types.empty() ? type : types[id_msh - 1];
   }
   virtual std::vector<unsigned int> GetAry_ID() const {
     std::vector<unsigned int> ids;
@@ -526,6 +529,207 @@ void ref_sys_get_force(void* h, double* f) {
 void ref_sys_update_field_save(void* h) {
   RefSys* s = (RefSys*)h;
   ((Fem::Ls::CLinearSystem_Save*)s->ls)->UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------------------
+// The reference's own equation-system classes (CEqnSystem_Solid2D, CEqn_Solid3D_Linear, CEqnSystem_Scalar2D,
+// CEqn_Scalar3D, src/femeqn/eqnsys_solid.cpp, eqnsys_scalar.cpp) on a mesh of element groups (one element array
+// and so one CEqn_* per group) plus one boundary group (lines in 2D, triangles in 3D) that AddFixElemAry fixes,
+// as the reference's examples fix CAD edges (test_glut/solid2d/main.cpp there).
+
+namespace {
+struct RefEqn {
+  Fem::Field::CFieldWorld world;
+  CGroupMesh mesh;
+  unsigned int id_base = 0, id_ea_bnd = 0;
+  std::vector<unsigned int> id_ea;  // per domain group
+  int type = 0, ndim = 0, nnode = 0, len = 0;
+  Fem::Eqn::CEqnSystem* eq = 0;
+  ~RefEqn() { delete eq; }
+  unsigned int id_field() const {
+    switch (type) {
+      case 0: return ((Fem::Eqn::CEqnSystem_Solid2D*)eq)->GetIdField_Disp();
+      case 1: return ((Fem::Eqn::CEqn_Solid3D_Linear*)eq)->GetIdField_Disp();
+      case 2: return ((Fem::Eqn::CEqnSystem_Scalar2D*)eq)->GetIdField_Value();
+      default: return ((Fem::Eqn::CEqn_Scalar3D*)eq)->GetIdField_Value();
+    }
+  }
+};
+}  // namespace
+
+extern "C" {
+
+// type 0 CEqnSystem_Solid2D, 1 CEqn_Solid3D_Linear, 2 CEqnSystem_Scalar2D, 3 CEqn_Scalar3D. Domain elements in
+// groups 0..ngroup-1 (elem_group null: one group); nbnd boundary elements of nbnd_no nodes each are fixed with
+// AddFixElemAry(id_ea, world, idof) (idof -1: every component).
+void* ref_eqn_create(int type, int nnode, const double* coords, int nelem, const int* lnods, int ngroup,
+                     const int* elem_group, int nbnd, const int* bnd_lnods, int idof) {
+  RefEqn* r = new RefEqn;
+  r->type = type;
+  r->ndim = (type == 1 || type == 3) ? 3 : 2;
+  r->len = type == 0 ? 2 : type == 1 ? 3 : 1;
+  r->nnode = nnode;
+  const int npoel = r->ndim == 2 ? 3 : 4, nbno = r->ndim == 2 ? 2 : 3;
+  r->mesh.ndim = r->ndim;
+  r->mesh.type = r->ndim == 2 ? Msh::TRI : Msh::TET;
+  r->mesh.coord.assign(coords, coords + (size_t)nnode * r->ndim);
+  r->mesh.groups.resize(ngroup + 1);
+  r->mesh.types.assign(ngroup, r->mesh.type);
+  r->mesh.types.push_back(r->ndim == 2 ? Msh::BAR : Msh::TRI);
+  for (int ie = 0; ie < nelem; ie++) {
+    const int g = elem_group ? elem_group[ie] : 0;
+    for (int k = 0; k < npoel; k++) r->mesh.groups[g].push_back(lnods[(size_t)ie * npoel + k]);
+  }
+  for (int ib = 0; ib < nbnd * nbno; ib++) r->mesh.groups[ngroup].push_back(bnd_lnods[ib]);
+  r->id_base = r->world.AddMesh(r->mesh);
+  const Fem::Field::CIDConvEAMshCad conv = r->world.GetIDConverter(r->id_base);
+  for (int g = 0; g < ngroup; g++) r->id_ea.push_back(conv.GetIdEA_fromMsh(g + 1));
+  r->id_ea_bnd = conv.GetIdEA_fromMsh(ngroup + 1);
+  switch (type) {
+    case 0: r->eq = new Fem::Eqn::CEqnSystem_Solid2D(r->id_base, r->world); break;
+    case 1: r->eq = new Fem::Eqn::CEqn_Solid3D_Linear(r->id_base, r->world); break;
+    case 2: {  // (the constructor taking a field is declared but not defined in the reference)
+      Fem::Eqn::CEqnSystem_Scalar2D* e = new Fem::Eqn::CEqnSystem_Scalar2D;
+      e->SetDomain_Field(r->id_base, r->world);
+      r->eq = e;
+      break;
+    }
+    default: {
+      Fem::Eqn::CEqn_Scalar3D* e = new Fem::Eqn::CEqn_Scalar3D;
+      e->SetDomain(r->id_base, r->world);
+      r->eq = e;
+    }
+  }
+  if (nbnd > 0) r->eq->AddFixElemAry(r->id_ea_bnd, r->world, idof);
+  return r;
+}
+
+void ref_eqn_destroy(void* h) { delete (RefEqn*)h; }
+
+// SetYoungPoisson: Solid2D with is_plane_stress, per group (group >= 0, through GetEquation / SetEquation) or for
+// the system; Solid3D for the system.
+void ref_eqn_set_young_poisson(void* h, int group, double young, double poisson, int is_plane_stress) {
+  RefEqn* r = (RefEqn*)h;
+  if (r->type == 1) {
+    ((Fem::Eqn::CEqn_Solid3D_Linear*)r->eq)->SetYoungPoisson(young, poisson);
+    return;
+  }
+  Fem::Eqn::CEqnSystem_Solid2D* e = (Fem::Eqn::CEqnSystem_Solid2D*)r->eq;
+  if (group < 0) {
+    e->SetYoungPoisson(young, poisson, is_plane_stress != 0);
+    return;
+  }
+  Fem::Eqn::CEqn_Solid2D q = e->GetEquation(r->id_ea[group]);
+  q.SetYoungPoisson(young, poisson, is_plane_stress != 0);
+  e->SetEquation(q);
+}
+
+// Solid2D: SetRho / SetGravitation (system or group); Solid3D: SetGravitation
+void ref_eqn_set_rho(void* h, int group, double rho) {
+  RefEqn* r = (RefEqn*)h;
+  Fem::Eqn::CEqnSystem_Solid2D* e = (Fem::Eqn::CEqnSystem_Solid2D*)r->eq;
+  if (group < 0) {
+    e->SetRho(rho);
+    return;
+  }
+  Fem::Eqn::CEqn_Solid2D q = e->GetEquation(r->id_ea[group]);
+  q.SetRho(rho);
+  e->SetEquation(q);
+}
+
+void ref_eqn_set_gravitation(void* h, int group, double gx, double gy, double gz) {
+  RefEqn* r = (RefEqn*)h;
+  if (r->type == 1) {
+    ((Fem::Eqn::CEqn_Solid3D_Linear*)r->eq)->SetGravitation(gx, gy, gz);
+    return;
+  }
+  Fem::Eqn::CEqnSystem_Solid2D* e = (Fem::Eqn::CEqnSystem_Solid2D*)r->eq;
+  if (group < 0) {
+    e->SetGravitation(gx, gy);
+    return;
+  }
+  Fem::Eqn::CEqn_Solid2D q = e->GetEquation(r->id_ea[group]);
+  q.SetGravitation(gx, gy);
+  e->SetEquation(q);
+}
+
+// which: 0 geometrical nonlinearity, 1 stationary, 2 saved stiffness matrix
+void ref_eqn_set_flag(void* h, int which, int on) {
+  RefEqn* r = (RefEqn*)h;
+  if (r->type == 0) {
+    Fem::Eqn::CEqnSystem_Solid2D* e = (Fem::Eqn::CEqnSystem_Solid2D*)r->eq;
+    if (which == 0) e->SetGeometricalNonlinear(on != 0);
+    if (which == 1) e->SetStationary(on != 0);
+    if (which == 2) e->SetSaveStiffMat(on != 0);
+  } else if (r->type == 1) {
+    Fem::Eqn::CEqn_Solid3D_Linear* e = (Fem::Eqn::CEqn_Solid3D_Linear*)r->eq;
+    if (which == 0) { if (on) e->SetGeometricalNonLinear(); else e->UnSetGeometricalNonLinear(); }
+    if (which == 1) { if (on) e->SetStationary(); else e->UnSetStationary(); }
+    if (which == 2) { if (on) e->SetSaveStiffMat(); else e->UnSetSaveStiffMat(); }
+  } else if (r->type == 2) {
+    Fem::Eqn::CEqnSystem_Scalar2D* e = (Fem::Eqn::CEqnSystem_Scalar2D*)r->eq;
+    if (which == 1) e->SetStationary(on != 0);
+    if (which == 2) e->SetSaveStiffMat(on != 0);
+  }
+}
+
+// scalars: which 0 alpha, 1 capacity, 2 source (Scalar2D: system or group; Scalar3D: alpha and source)
+void ref_eqn_set_scalar(void* h, int which, int group, double val) {
+  RefEqn* r = (RefEqn*)h;
+  if (r->type == 3) {
+    Fem::Eqn::CEqn_Scalar3D* e = (Fem::Eqn::CEqn_Scalar3D*)r->eq;
+    if (which == 0) e->SetAlpha(val);
+    if (which == 2) e->SetSource(val);
+    return;
+  }
+  Fem::Eqn::CEqnSystem_Scalar2D* e = (Fem::Eqn::CEqnSystem_Scalar2D*)r->eq;
+  if (group < 0) {
+    if (which == 0) e->SetAlpha(val);
+    if (which == 1) e->SetCapacity(val);
+    if (which == 2) e->SetSource(val);
+    return;
+  }
+  Fem::Eqn::CEqn_Scalar2D q = e->GetEquation(r->id_ea[group]);
+  if (which == 0) q.SetAlpha(val);
+  if (which == 1) q.SetCapacity(val);
+  if (which == 2) q.SetSource(val);
+  e->SetEquation(q);
+}
+
+void ref_eqn_set_time(void* h, double dt, double gamma, double beta) {
+  ((RefEqn*)h)->eq->SetTimeIntegrationParameter(dt, gamma, beta);
+}
+
+int ref_eqn_solve(void* h) {
+  RefEqn* r = (RefEqn*)h;
+  return r->eq->Solve(r->world) ? 1 : 0;
+}
+
+// GetAry_ItrNormRes into iters / norms (at most cap); returns the count
+int ref_eqn_itr(void* h, unsigned int* iters, double* norms, int cap) {
+  const std::vector<std::pair<unsigned int, double> >& a = ((RefEqn*)h)->eq->GetAry_ItrNormRes();
+  for (int i = 0; i < (int)a.size() && i < cap; i++) {
+    iters[i] = a[i].first;
+    norms[i] = a[i].second;
+  }
+  return (int)a.size();
+}
+
+// VALUE / VELOCITY / ACCELERATION of the equation's field (n*len each; null to skip; a field without the
+// derivative leaves its array untouched)
+void ref_eqn_get_field(void* h, double* u, double* v, double* a) {
+  RefEqn* r = (RefEqn*)h;
+  Fem::Field::CField& field = r->world.GetField(r->id_field());
+  const Fem::Field::FIELD_DERIVATION_TYPE fdt[3] = {Fem::Field::VALUE, Fem::Field::VELOCITY,
+                                                     Fem::Field::ACCELERATION};
+  double* out[3] = {u, v, a};
+  for (int k = 0; k < 3; k++) {
+    if (!out[k] || !(field.GetFieldDerivativeType() & fdt[k])) continue;
+    const Fem::Field::CNodeAry::CNodeSeg& ns = field.GetNodeSeg(Fem::Field::CORNER, true, r->world, fdt[k]);
+    for (int i = 0; i < r->nnode; i++) ns.GetValue(i, &out[k][(size_t)i * r->len]);
+  }
 }
 
 }  // extern "C"
