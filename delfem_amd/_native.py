@@ -384,10 +384,10 @@ class System:
         ok = _check(lib().dfm_solve_pcg(self.h, precision, C.byref(conv), C.byref(it)))
         return bool(ok), conv.value, it.value
 
-    def solve_pbicgstab(self, tol, max_iter):
+    def solve_pbicgstab(self, tol, max_iter, precision=FP64):
         conv = C.c_double(tol)
         it = C.c_uint(max_iter)
-        ok = _check(lib().dfm_solve_pbicgstab(self.h, C.byref(conv), C.byref(it)))
+        ok = _check(lib().dfm_solve_pbicgstab(self.h, precision, C.byref(conv), C.byref(it)))
         return bool(ok), conv.value, it.value
 
     def _solve_fn(self, fn, tol, max_iter):
@@ -503,6 +503,78 @@ class Request(C.Structure):
 
 
 REQ_QUEUED, REQ_SETUP, REQ_SOLVING, REQ_CONVERGED, REQ_MAXITER, REQ_FAILED = range(6)
+
+
+EQN_SOLID2D, EQN_SOLID3D, EQN_SCALAR2D, EQN_SCALAR3D = range(4)
+
+
+class Eqn:
+    """An equation system (CEqnSystem_Solid2D, CEqn_Solid3D_Linear, CEqnSystem_Scalar2D or CEqn_Scalar3D) over the
+    device linear systems: elements in groups (element arrays, elem_group per element), the boundary elements bnd
+    (lines in 2D, triangles in 3D) fixed in component idof (-1: all) as AddFixElemAry fixes them."""
+
+    def __init__(self, ctx, type_, coords, lnods, elem_group=None, ngroup=1, bnd=None, idof=-1):
+        self.type = type_
+        self.ndim = 3 if type_ in (EQN_SOLID3D, EQN_SCALAR3D) else 2
+        self.len = {EQN_SOLID2D: 2, EQN_SOLID3D: 3}.get(type_, 1)
+        coords, lnods = _f64(coords), _i32(lnods)
+        self.nnode = coords.size // self.ndim
+        eg = _i32(elem_group)
+        self.h = C.c_void_p()
+        _check(lib().dfm_eqn_create(ctx.h, type_, self.nnode, _p(coords), lnods.size // (self.ndim + 1), _p(lnods),
+                                    _p(eg), ngroup, C.byref(self.h)))
+        if bnd is not None:
+            b = _i32(bnd)
+            _check(lib().dfm_eqn_add_fix(self.h, b.size // self.ndim, self.ndim, _p(b), idof))
+
+    def close(self):
+        if self.h:
+            lib().dfm_eqn_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_young_poisson(self, young, poisson, is_plane_stress=True, group=-1):
+        _check(lib().dfm_eqn_set_young_poisson(self.h, group, C.c_double(young), C.c_double(poisson),
+                                               1 if is_plane_stress else 0))
+
+    def set_rho(self, rho, group=-1):
+        _check(lib().dfm_eqn_set_rho(self.h, group, C.c_double(rho)))
+
+    def set_gravitation(self, gx, gy, gz=0.0, group=-1):
+        _check(lib().dfm_eqn_set_gravitation(self.h, group, C.c_double(gx), C.c_double(gy), C.c_double(gz)))
+
+    def set_flag(self, which, on):
+        """which: 0 geometrical nonlinearity, 1 stationary, 2 saved stiffness"""
+        _check(lib().dfm_eqn_set_flag(self.h, which, 1 if on else 0))
+
+    def set_scalar(self, which, val, group=-1):
+        """which: 0 alpha, 1 capacity, 2 source"""
+        _check(lib().dfm_eqn_set_scalar(self.h, which, group, C.c_double(val)))
+
+    def set_time(self, dt, gamma=0.6, beta=-1.0):
+        _check(lib().dfm_eqn_set_time(self.h, C.c_double(dt), C.c_double(gamma), C.c_double(beta)))
+
+    def solve(self):
+        _check(lib().dfm_eqn_solve(self.h))
+        return True
+
+    def itr(self):
+        n = C.c_int()
+        it = np.zeros(256, np.uint32)
+        nr = np.zeros(256)
+        _check(lib().dfm_eqn_itr_norm_res(self.h, C.byref(n), _p(it), _p(nr), 256))
+        return [(int(it[k]), float(nr[k])) for k in range(min(n.value, 256))]
+
+    def fields(self):
+        m = self.nnode * self.len
+        u, v, a = np.zeros(m), np.zeros(m), np.zeros(m)
+        _check(lib().dfm_eqn_get_field(self.h, _p(u), _p(v), _p(a)))
+        return u, v, a
 
 
 class Server:

@@ -663,6 +663,16 @@ int dfm_sys_update_field_beuler(dfm_sys sys, double dt, int is_initial) {
   return DFM_OK;
 }
 
+int dfm_sys_update_field_newmark1(dfm_sys sys, double gamma, double dt, int is_initial) {
+  NEED(sys, "dfm_sys_update_field_newmark1");
+  Sys& s = sys->s;
+  const double gdt = gamma * dt;  // the reference's argument gamma*dt, formed before the products
+  DFM_HIP(launch_update_field_newmark1(s.n, s.len, s.d_order.p, s.d_upd.p, gdt, dt, is_initial ? 1 : 0, s.d_u.p,
+                                       s.d_v.p, s.ctx->stream));
+  DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+  return DFM_OK;
+}
+
 int dfm_sys_set_acceleration(dfm_sys sys, const double* a) {
   NEED(sys, "dfm_sys_set_acceleration");
   Sys& s = sys->s;
@@ -1384,16 +1394,73 @@ int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iter
   return solve_pcg(s, precision, conv_ratio, iteration);
 }
 
-int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration) {
+int dfm_solve_pbicgstab(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration) {
   NEED(sys, "dfm_solve_pbicgstab");
   NEED(conv_ratio, "dfm_solve_pbicgstab");
   NEED(iteration, "dfm_solve_pbicgstab");
   Sys& s = sys->s;
+  DFM_CHECK(!s.batch, DFM_ERR_STATE, "dfm_solve_pbicgstab: the system is being solved in a batch");
+  DFM_CHECK(precision >= DFM_FP64 && precision <= DFM_FP64_TRUE, DFM_ERR_ARG, "dfm_solve_pbicgstab: bad precision");
   DFM_CHECK(s.assembled && s.factored, DFM_ERR_STATE, "dfm_solve_pbicgstab: assemble and factor first");
-  const int rc = ensure_work(s, 7, 0);
+  // the mixed modes run the FP64 loop with the FP32 factor (the FP32-vector loop exists for PCG only)
+  const bool f32 = precision == DFM_FP32_ILU || precision == DFM_MIXED || precision == DFM_MIXED_AUTO;
+  const long long m = mdof(s);
+  hipStream_t st = s.ctx->stream;
+  int rc = ensure_work(s, 8, 0);  // the loop's seven vectors and the solution kept over a restart
   if (rc != DFM_OK) return rc;
-  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, mdof(s), s.ctx->stream));
-  return solve_pbicgstab(s, conv_ratio, iteration);
+  if (f32) {
+    rc = ensure_fp32(s);
+    if (rc != DFM_OK) return rc;
+  }
+  const double tol = *conv_ratio;
+  const unsigned maxit = *iteration;
+  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
+  double conv = tol;
+  unsigned it = maxit;
+  rc = solve_pbicgstab(s, f32, &conv, &it);
+  if (rc < 0) return rc;
+  bool ok = rc == 1 && it < maxit;
+  unsigned used = it;
+  if (ok && precision != DFM_FP64) {
+    // as in dfm_solve_pcg: the loop stops on its recursive residual; its true residual is checked and the loop
+    // restarted on r = b - A x (target half the tolerance, relative to ||b||) until the true residual meets the
+    // tolerance; a restart that does not lower it by a fifth ends the solve unconverged
+    double* xs = s.d_work.p + 7 * m;
+    double prev = 0.0;
+    for (int restart = 0;; restart++) {
+      double rel = 0.0;
+      rc = sys_true_residual(s, st, &rel);
+      if (rc != DFM_OK) return rc;
+      conv = rel;
+      if (rel <= tol) break;
+      if (used + 1 >= maxit || restart == kMaxRestarts || (restart > 0 && rel > kRestartGain * prev)) {
+        ok = false;
+        break;
+      }
+      prev = rel;
+      DFM_HIP(launch_copy(s.d_upd.p, xs, m, st));
+      DFM_HIP(launch_copy(s.d_rhs.p, s.d_res.p, m, st));
+      DFM_HIP(launch_spmv(s.n, s.len, spmv_tpr(s.ncrs, s.n), s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, -1.0,
+                          xs, 1.0, s.d_res.p, st));
+      double c2 = std::min(1.0, 0.5 * tol / rel);
+      unsigned it2 = maxit - used - 1;
+      rc = solve_pbicgstab(s, f32, &c2, &it2);
+      if (rc < 0) return rc;
+      DFM_HIP(launch_axpy(1.0, xs, s.d_upd.p, m, st));
+      used += it2 + 1;
+      if (rc != 1 || it2 >= maxit) {
+        rc = sys_true_residual(s, st, &conv);
+        if (rc != DFM_OK) return rc;
+        ok = false;
+        break;
+      }
+    }
+  }
+  *conv_ratio = conv;
+  if (ok) *iteration = used;
+  else if (precision == DFM_FP64) *iteration = it;
+  else *iteration = maxit;
+  return ok || precision == DFM_FP64 ? 1 : 0;
 }
 
 }  // extern "C"

@@ -1324,6 +1324,33 @@ __global__ void k_update_field_beuler(long long m, int len, const int* order, co
   u[i1] = initial ? u[i1] + dt * vv : u[i1] + dt * du;
 }
 
+// CLinearSystem_Field::UpdateValueOfField_Newmark, VELOCITY (src/femls/linearsystem_field.cpp:885-921): on the
+// first step the value takes dt times the velocity, then gamma dt (gdt, formed by the caller) times the update; the
+// velocity takes the update.
+__global__ void k_update_field_newmark1(long long m, int len, const int* order, const double* upd, double gdt,
+                                       double dt, int initial, double* u, double* v) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long long i0 = k / len;
+  const int d = (int)(k - i0 * len);
+  const long long i1 = (order ? order[i0] : i0) * len + d;
+  const double du = upd[k];
+  double uu = u[i1];
+  if (initial) uu += dt * v[i1];
+  u[i1] = uu + gdt * du;
+  v[i1] = v[i1] + 1.0 * du;
+}
+
+hipError_t launch_update_field_newmark1(int n, int len, const int* order, const double* upd, double gdt, double dt,
+                                        int initial, double* u, double* v, hipStream_t st) {
+  const long long m = (long long)n * len;
+  if (m == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_update_field_newmark1, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order,
+                     upd, gdt, dt, initial, u, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_update_field_beuler(int n, int len, const int* order, const double* upd, double dt, int initial,
                                       double* u, double* v, hipStream_t st) {
   const long long m = (long long)n * len;

@@ -54,6 +54,8 @@ hipError_t launch_assemble(const AsmArgs& a, const ElemStore* es, hipStream_t st
 hipError_t launch_set_bc(int n, int len, const unsigned* rowptr, const unsigned* colind, double* crs, double* dia,
                          double* res, const int* bc, hipStream_t st);
 hipError_t launch_update_field(int n, int len, const int* order, const double* upd, double* u, hipStream_t st);
+hipError_t launch_update_field_newmark1(int n, int len, const int* order, const double* upd, double gdt, double dt,
+                                        int initial, double* u, double* v, hipStream_t st);
 hipError_t launch_update_field_beuler(int n, int len, const int* order, const double* upd, double dt, int initial,
                                       double* u, double* v, hipStream_t st);
 hipError_t launch_update_field_newmark(int n, int len, const int* order, const double* upd, double gamma, double beta,

@@ -1224,7 +1224,7 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   return ok ? 1 : 0;
 }
 
-int solve_pbicgstab(Sys& s, double* conv_ratio, unsigned* iteration) {
+int solve_pbicgstab(Sys& s, bool f32ilu, double* conv_ratio, unsigned* iteration) {
   const long long m = (long long)s.n * s.len;
   const double tol = *conv_ratio;
   const unsigned maxit = *iteration;
@@ -1266,10 +1266,12 @@ int solve_pbicgstab(Sys& s, double* conv_ratio, unsigned* iteration) {
     const unsigned k = std::min(chunk, maxit - launched);
     for (unsigned t = 0; t < k; t++) {
       unsigned G2 = 0, G5 = 0;
-      if (precond_apply64(s, vp, vMp, fl, st) != hipSuccess) return DFM_ERR_HIP;
+      if ((f32ilu ? precond_apply_f32ilu(s, vp, vMp, fl, st) : precond_apply64(s, vp, vMp, fl, st)) != hipSuccess)
+        return DFM_ERR_HIP;
       if (bi_spmv(s, vMp, vAMp, vr2, r, 0, slot(s, P_A), G2, st) != hipSuccess) return DFM_ERR_HIP;
       hipLaunchKernelGGL(k_bi_s, vg, dim3(NT), 0, st, m, r, vAMp, vs, slot(s, P_A), (int)G2, fl);
-      if (precond_apply64(s, vs, vMs, fl, st) != hipSuccess) return DFM_ERR_HIP;
+      if ((f32ilu ? precond_apply_f32ilu(s, vs, vMs, fl, st) : precond_apply64(s, vs, vMs, fl, st)) != hipSuccess)
+        return DFM_ERR_HIP;
       if (bi_spmv(s, vMs, vAMs, vs, nullptr, 1, slot(s, P_B), G5, st) != hipSuccess) return DFM_ERR_HIP;
       hipLaunchKernelGGL(k_bi_update, vg, dim3(NT), 0, st, m, x, r, vMp, vMs, vs, vAMs, vr2, slot(s, P_A), (int)G2,
                          slot(s, P_B), (int)G5, slot(s, P_C), fl);

@@ -63,7 +63,8 @@ int pcg_finish(Sys& s, const PcgBufs& v, bool mixed, int done, hipStream_t st);
 constexpr int kMaxRestarts = 8;
 constexpr double kRestartGain = 0.8;
 int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration);
-int solve_pbicgstab(Sys& s, double* conv_ratio, unsigned* iteration);
+// f32ilu: the preconditioner applied with the FP32 copy of the factor (FP64 vectors and matrix)
+int solve_pbicgstab(Sys& s, bool f32ilu, double* conv_ratio, unsigned* iteration);
 // z = M^-1 src with the FP64 factor (src may equal z); kernels skip when *done != 0 (done may be null)
 int bench_kernel(Sys& s, int which, int reps, double* avg_ms, double* bytes);
 hipError_t precond_apply64(const Sys& s, const double* src, double* z, const int* done, hipStream_t st);

@@ -169,6 +169,10 @@ int dfm_sys_update_field_newmark(dfm_sys sys, double gamma, double beta, double 
  * being the velocity increment: v += du, then u += dt v (is_initial) or u += dt du. */
 int dfm_sys_set_velocity_prev(dfm_sys sys, const double* v0);
 int dfm_sys_update_field_beuler(dfm_sys sys, double dt, int is_initial);
+/* CLinearSystem_Field::UpdateValueOfField_Newmark with VELOCITY (src/femls/linearsystem_field.cpp:885-921), the
+ * update being the velocity increment of the diffusion equation: [is_initial: u += dt v], u += (gamma dt) du,
+ * v += du. */
+int dfm_sys_update_field_newmark1(dfm_sys sys, double gamma, double dt, int is_initial);
 
 /* InitializeMarge + Fem::Eqn::AddLinSys_<kind> + [surface loads] + FinalizeMarge in one call
  * (src/ls/linearsystem.cpp:74-128). params: npar doubles per element with stride pstride (0 = shared).
@@ -251,7 +255,11 @@ int dfm_prec_get_factor(dfm_sys sys, unsigned* frowptr, unsigned* fcolind, unsig
  * tolerance for the system the solve ends unconverged. In every mode but DFM_FP64 the returned conv_ratio is the
  * true relative residual ||b - A x|| / ||b|| (FP64 matrix). */
 int dfm_solve_pcg(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration);
-int dfm_solve_pbicgstab(dfm_sys sys, double* conv_ratio, unsigned* iteration);
+/* PBiCGSTAB: DFM_FP64 is the reference loop (it returns 1 also when not converged, with iteration = max and
+ * conv_ratio untouched, as Solve_PBiCGSTAB returns true); DFM_FP32_ILU, and DFM_MIXED / DFM_MIXED_AUTO (which run
+ * as DFM_FP32_ILU: the FP32-vector loop is PCG's), apply the FP32 copy of the factor in the FP64 loop; they and
+ * DFM_FP64_TRUE check the true residual and restart on b - A x as PCG does, returning 0 when it cannot be met. */
+int dfm_solve_pbicgstab(dfm_sys sys, int precision, double* conv_ratio, unsigned* iteration);
 /* Host-driven loops over the handle operations (the C++ layer include/dfm_lssol.hpp), FP64, same contract as
  * the reference functions they follow: LsSol::Solve_CG and Solve_BiCGSTAB without preconditioner
  * (src/ls/solver_ls_iter.cpp:31-97, :177-286; both return 1 also when max_iter is reached), and Solve_PCG /
@@ -381,6 +389,53 @@ int dfm_server_release(dfm_server s, long long ticket);
 int dfm_newton_batch(dfm_ctx ctx, int nsys, const dfm_sys* sys, int kind, const double* glob, int nsteps,
                      const double* load_scale, int max_newton, double newton_tol, int precision, double lin_tol,
                      unsigned lin_maxit, int* newton_iters, unsigned* lin_iters, int* lin_unconverged);
+
+/* ---- equation systems: CEqnSystem_Solid2D, CEqn_Solid3D_Linear, CEqnSystem_Scalar2D, CEqn_Scalar3D
+ * (include/delfem/eqnsys_solid.h, eqnsys_scalar.h, eqnsys.h) ----------------------------------------------------
+ * The reference's classes and their Solve flows (src/femeqn/eqnsys_solid.cpp:175-247 and :671-761,
+ * src/femeqn/eqnsys_scalar.cpp:294-359 and :554-584) over device linear systems: ILU(1) for solids, ILU(0) for
+ * scalars, PCG to conv_ratio 1e-6 in at most 1000 iterations (GetAry_ItrNormRes gets each solve's iterations
+ * and ratio), Newton under geometrical nonlinearity (10 steps in 2D, 40 in 3D, to 1e-6 of the first residual),
+ * the saved stiffness matrix (CLinearSystem_Save) for stationary linear systems, Newmark-beta (solids) and
+ * first-order Newmark (2D scalar) time stepping. The field world is replaced by the mesh: element groups are the
+ * element arrays (one CEqn_* and material each; elements taken group by group), boundary element lists the fixed
+ * element arrays (AddFixElemAry). dfm_eqn_create takes ngroup 1 for the 3D types. */
+typedef struct dfm_eqn_s* dfm_eqn;
+enum { DFM_EQN_SOLID2D = 0, DFM_EQN_SOLID3D = 1, DFM_EQN_SCALAR2D = 2, DFM_EQN_SCALAR3D = 3 };
+/* dfm_eqn_set_flag: SetGeometricalNonlinear / SetGeometricalNonLinear (solids), SetStationary, SetSaveStiffMat */
+enum { DFM_EQN_NONLINEAR = 0, DFM_EQN_STATIONARY = 1, DFM_EQN_SAVE_STIFF_MAT = 2 };
+/* The equation system of `type` on nnode nodes and nelem P1 elements (lnods), element k in group elem_group[k]
+ * (NULL: one group). Defaults are the reference constructors': 2D solid Young 1, Poisson 0, plane-stress flag
+ * set, rho 1, stationary (the reference leaves that flag unset); 3D solid lambda = myu = 0, rho 1, not
+ * stationary; scalars alpha 1, capacity 1, source 0, stationary. */
+int dfm_eqn_create(dfm_ctx ctx, int type, int nnode, const double* coords, int nelem, const int* lnods,
+                   const int* elem_group, int ngroup, dfm_eqn* out);
+int dfm_eqn_destroy(dfm_eqn eq);
+/* AddFixElemAry (eqnsys_solid.cpp:1341-1362): the nodes of nbnd boundary elements of nbno nodes each are fixed in
+ * component idof (-1: all); every component of those nodes is set to zero. dfm_eqn_clear_fix: ClearFixElemAry(). */
+int dfm_eqn_add_fix(dfm_eqn eq, int nbnd, int nbno, const int* bnd_lnods, int idof);
+int dfm_eqn_clear_fix(dfm_eqn eq);
+/* Material setters; group -1 the system's (CEqnSystem_Solid2D::SetYoungPoisson, :925-937, ...), else that
+ * group's CEqn_* through SetEquation. SetYoungPoisson keeps the reference's flag: is_plane_stress != 0 leaves the
+ * 3D lambda (plane strain), 0 reduces it to 2 lambda myu / (lambda + 2 myu) (plane stress), as
+ * include/delfem/eqnsys_solid.h:190-196 does. set_rho: 2D solids; gz: 3D solids; set_scalar which: 0 alpha,
+ * 1 capacity (2D), 2 source. */
+int dfm_eqn_set_young_poisson(dfm_eqn eq, int group, double young, double poisson, int is_plane_stress);
+int dfm_eqn_set_rho(dfm_eqn eq, int group, double rho);
+int dfm_eqn_set_gravitation(dfm_eqn eq, int group, double gx, double gy, double gz);
+int dfm_eqn_set_scalar(dfm_eqn eq, int which, int group, double value);
+int dfm_eqn_set_flag(dfm_eqn eq, int which, int on);
+/* SetTimeIntegrationParameter (include/delfem/eqnsys.h:88-93): beta < 0 gives 0.25 (gamma + 0.5)^2 */
+int dfm_eqn_set_time(dfm_eqn eq, double dt, double gamma, double beta);
+int dfm_eqn_solve(dfm_eqn eq);
+/* GetAry_ItrNormRes (include/delfem/eqnsys.h:69): the (iterations, conv_ratio) of the last Solve's PCG solves */
+int dfm_eqn_itr_norm_res(dfm_eqn eq, int* count, unsigned* iters, double* norms, int cap);
+int dfm_eqn_get_field(dfm_eqn eq, double* value, double* velocity, double* acceleration);
+int dfm_eqn_set_field(dfm_eqn eq, const double* value, const double* velocity, const double* acceleration);
+/* what: 1 ClearValueLinearSystem, 2 ClearValuePreconditioner, 4 ClearLinearSystemPreconditioner */
+int dfm_eqn_clear(dfm_eqn eq, int what);
+/* The current linear system (NULL before the first Solve and after a clear), for the handle operations. */
+int dfm_eqn_system(dfm_eqn eq, dfm_sys* sys);
 
 /* Device memory currently allocated by the library (bytes). */
 int dfm_device_bytes(long long* bytes);

@@ -167,3 +167,28 @@ def test_fp64_true_meets_where_fp64_does_not(ctx):
     st, cb, itb, _ = b.status(s)
     assert st == 1 and cb <= 3e-10 and cb == s.true_residual()
     b.close()
+
+
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+@pytest.mark.parametrize("mode", ["fp32_ilu", "fp64_true", "mixed"])
+def test_pbicgstab_precision(ctx, case, mode):
+    """PBiCGSTAB's precision argument: the FP32 factor (fp32_ilu, and mixed, which runs as fp32_ilu) or the FP64
+    factor, with the true residual checked and restarted on b - A x: its conv_ratio is the true residual, at most
+    the tolerance, and the solution agrees with the FP64 solve's to within the tolerance times cond(A)."""
+    prec = {"fp32_ilu": N.FP32_ILU, "fp64_true": N.FP64_TRUE, "mixed": N.MIXED}[mode]
+    cpu, gs, order, _ = setup(N, ctx, case, N.ORDER_BLOCKCOLOR, small=False)
+    gs.factor()
+    ok64, conv64, it64 = gs.solve_pbicgstab(1e-9, 3000, N.FP64)
+    x64 = gs.get_vector(-2)
+    assert ok64 and conv64 < 1e-9
+    gs.set_vector(-1, cpu.res)
+    ok, conv, it = gs.solve_pbicgstab(1e-9, 3000, prec)
+    tr = gs.true_residual()
+    print(f"{mode}: FP64 {it64} iterations; {it} iterations, conv {conv:.2e}, true {tr:.2e}")
+    assert ok and conv == tr and conv <= 1e-9
+    x = gs.get_vector(-2)
+    assert np.linalg.norm(x - x64) <= 1e-4 * np.linalg.norm(x64)
+    # the reference's contract on the FP64 loop: not converged within max_iter -> true, iteration = max_iter
+    gs.set_vector(-1, cpu.res)
+    ok, conv, it = gs.solve_pbicgstab(1e-14, 3, N.FP64)
+    assert ok and it == 3 and conv == 1e-14
