@@ -170,6 +170,25 @@ class System:
         self.ncrs = nc.value
         _live.add(self)
 
+    @classmethod
+    def from_pattern(cls, ctx, nblk, length, rowptr, colind):
+        """A system on a host block pattern without a mesh (dfm_sys_create_pattern); values from set_matrix."""
+        c = cls.__new__(cls)
+        rowptr, colind = np.ascontiguousarray(rowptr, np.uint32), np.ascontiguousarray(colind, np.uint32)
+        c.ctx, c.ndim, c.len, c.nnode, c.nelem = ctx, 0, length, nblk, 0
+        c.h = C.c_void_p()
+        _check(lib().dfm_sys_create_pattern(ctx.h, nblk, length, rowptr.size - 1, _p(rowptr), _p(colind),
+                                            C.byref(c.h)))
+        n, ln, nc = C.c_int(), C.c_int(), C.c_longlong()
+        _check(lib().dfm_sys_info(c.h, C.byref(n), C.byref(ln), C.byref(nc)))
+        c.ncrs = nc.value
+        _live.add(c)
+        return c
+
+    def set_matrix(self, crs, dia):
+        """Matrix values from the host in get_matrix's layout (dfm_sys_set_matrix)."""
+        _check(lib().dfm_sys_set_matrix(self.h, _p(_f64(crs)), _p(_f64(dia))))
+
     def clone(self):
         """A new system on the same mesh, ordering and Dirichlet flags (dfm_sys_clone)."""
         c = System.__new__(System)

@@ -410,6 +410,55 @@ int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, 
   return DFM_OK;
 }
 
+// CMatDia_BlkCrs::AddPattern(const CIndexedArray&) (src/matvec/matdia_blkcrs.cpp:346-505) on an empty matrix: the
+// block pattern as given (off-diagonal blocks only; rows past the pattern's are empty), no mesh behind it.
+int dfm_sys_create_pattern(dfm_ctx ctx, int nblk, int len, int nrow, const unsigned* rowptr, const unsigned* colind,
+                           dfm_sys* out) {
+  NEED(ctx, "dfm_sys_create_pattern");
+  NEED(out, "dfm_sys_create_pattern");
+  DFM_CHECK(len >= 1 && len <= 3, DFM_ERR_ARG, "dfm_sys_create_pattern: len must be 1, 2 or 3");
+  DFM_CHECK(nblk > 0 && nrow >= 0 && nrow <= nblk, DFM_ERR_ARG, "dfm_sys_create_pattern: 0 <= nrow <= nblk > 0");
+  DFM_CHECK(nrow == 0 || (rowptr && colind), DFM_ERR_ARG, "dfm_sys_create_pattern: null pattern");
+  DFM_CHECK(nrow == 0 || rowptr[0] == 0, DFM_ERR_ARG, "dfm_sys_create_pattern: rowptr[0] must be 0");
+  std::vector<unsigned> rp((size_t)nblk + 1, 0), ci;
+  std::vector<int> mark((size_t)nblk, -1);
+  for (int i = 0; i < nrow; i++) {
+    DFM_CHECK(rowptr[i + 1] >= rowptr[i], DFM_ERR_ARG, "dfm_sys_create_pattern: rowptr must not decrease");
+    for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+      const unsigned j = colind[q];
+      DFM_CHECK(j < (unsigned)nblk, DFM_ERR_ARG, "dfm_sys_create_pattern: column out of range");
+      DFM_CHECK(j != (unsigned)i, DFM_ERR_ARG, "dfm_sys_create_pattern: the diagonal block is kept apart");
+      DFM_CHECK(mark[j] != i, DFM_ERR_ARG, "dfm_sys_create_pattern: a column repeats in a row");
+      mark[j] = i;
+      ci.push_back(j);
+    }
+    rp[i + 1] = rowptr[i + 1];
+  }
+  for (int i = nrow; i < nblk; i++) rp[i + 1] = rp[i];
+  dfm_sys h = new dfm_sys_s;
+  Sys& s = h->s;
+  s.ctx = &ctx->c;
+  s.ndim = 0; s.n = nblk; s.len = len; s.bs = len * len; s.npoel = 0; s.nelem = 0;
+  s.rowptr0 = std::move(rp);
+  s.colind0 = std::move(ci);
+  s.elsup_ptr.assign((size_t)nblk + 1, 0);
+  s.ncrs = s.rowptr0[nblk];
+  std::vector<int> ident(nblk);
+  std::iota(ident.begin(), ident.end(), 0);
+  int rc = apply_order(s, ident, {}, {}, 0);
+  if (rc != DFM_OK) { delete h; return rc; }
+  const hipError_t e = sys_alloc_device(s);
+  if (e != hipSuccess) {
+    set_error(std::string("dfm_sys_create_pattern: ") + hipGetErrorString(e));
+    delete h;
+    return e == hipErrorOutOfMemory ? DFM_ERR_NOMEM : DFM_ERR_HIP;
+  }
+  h->owner = ctx;
+  ctx->c.refs++;
+  *out = h;
+  return DFM_OK;
+}
+
 static void copy_sweep_host(Sweep& d, const Sweep& a) {
   d.phase_gptr = a.phase_gptr; d.phase_nlev = a.phase_nlev; d.grp = a.grp; d.rowlist = a.rowlist; d.lev = a.lev;
   d.phase_wo = a.phase_wo; d.phase_wi = a.phase_wi;
@@ -506,6 +555,8 @@ int dfm_sys_make_ordering(dfm_sys sys, int kind, int tile, int* order_out, int* 
   } else if (kind == DFM_ORDER_MULTICOLOR) {
     make_multicolor(s, p.order, p.color, p.ncolors);
   } else if (kind == DFM_ORDER_BLOCKCOLOR) {
+    DFM_CHECK(!s.coords.empty(), DFM_ERR_ARG, "dfm_sys_make_ordering: the block-colour tiles need node coordinates "
+                                              "(a system made from a block pattern has none)");
     make_blockcolor(s, tile, p.order, p.color, p.tile, p.ncolors);
   } else if (kind == DFM_ORDER_RCM || kind == DFM_ORDER_RCM2 || kind == DFM_ORDER_AMD) {
     DFM_CHECK(make_reference_ordering(kind, s.n, s.rowptr0.data(), s.colind0.data(), p.order), DFM_ERR_ARG,
@@ -772,6 +823,8 @@ int dfm_sys_set_load_scale(dfm_sys sys, double scale) {
 }
 
 static int check_kind(const Sys& s, int kind) {
+  DFM_CHECK(s.nelem > 0, DFM_ERR_STATE, "dfm_sys_assemble: a system made from a block pattern has no elements "
+                                        "(its values come from dfm_sys_set_matrix)");
   static const int need_npar[12] = {2, 5, 6, 5, 6, 3, 5, 6, 5, 6, 5, 5};
   static const int need_len[12] = {1, 2, 3, 2, 3, 1, 2, 3, 2, 3, 2, 2};
   DFM_CHECK(kind >= 0 && kind < 12, DFM_ERR_ARG, "dfm_sys_assemble: unknown element kind");
@@ -921,6 +974,32 @@ int dfm_sys_assemble(dfm_sys sys, int kind, const double* params, int npar, int 
   rc = dfm_sys_assemble_resident(sys, kind, glob, res_norm);
   if (rc != DFM_OK) return rc;
   DFM_HIP(hipStreamSynchronize(sys->s.ctx->stream));
+  return DFM_OK;
+}
+
+// The matrix values from the host (the layout of dfm_sys_get_matrix: the caller's blocks in pattern order, the
+// diagonal blocks by node), as the element routines leave them in a CMatDia_BlkCrs through
+// ILinearSystem_Eqn::GetMatrix (include/delfem/linearsystem_interface_eqnsys.h:48-68); the system counts as
+// assembled (its factor is stale).
+int dfm_sys_set_matrix(dfm_sys sys, const double* crs, const double* dia) {
+  NEED(sys, "dfm_sys_set_matrix");
+  NEED(dia, "dfm_sys_set_matrix");
+  Sys& s = sys->s;
+  DFM_CHECK(crs || s.ncrs == 0, DFM_ERR_ARG, "dfm_sys_set_matrix: null crs");
+  hipStream_t st = s.ctx->stream;
+  std::vector<double> c((size_t)s.ncrs * s.bs), d((size_t)s.n * s.bs);
+  for (long long q = 0; q < s.ncrs; q++)
+    std::memcpy(&c[(size_t)s.perm_crs[q] * s.bs], &crs[(size_t)q * s.bs], sizeof(double) * s.bs);
+  for (int i0 = 0; i0 < s.n; i0++)
+    std::memcpy(&d[(size_t)i0 * s.bs], &dia[(size_t)s.order[i0] * s.bs], sizeof(double) * s.bs);
+  DFM_HIP(hipStreamSynchronize(st));
+  if (!c.empty()) DFM_HIP(hipMemcpyAsync(s.d_crs.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  DFM_HIP(hipMemcpyAsync(s.d_dia.p, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  DFM_HIP(hipStreamSynchronize(st));  // the host arrays go out of scope
+  const int rc = finish_matrix(s, st);
+  if (rc != DFM_OK) return rc;
+  s.saved = false;
+  DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
 

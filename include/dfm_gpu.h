@@ -124,6 +124,13 @@ int dfm_rng_normal(unsigned long long seed, unsigned long long stream, size_t n,
 int dfm_sys_create_mesh(dfm_ctx ctx, int ndim, int nnode, const double* coords, int nelem, const int* lnods,
                         int len, dfm_sys* out);
 int dfm_sys_destroy(dfm_sys sys);
+/* A system from a host block pattern without a mesh: CMatDia_BlkCrs::AddPattern(const CIndexedArray&)
+ * (src/matvec/matdia_blkcrs.cpp:346-505) on an nblk x nblk matrix of len x len blocks. rowptr/colind give the
+ * off-diagonal blocks of rows 0..nrow-1 (nrow <= nblk; later rows empty), no diagonal, no repeated column. Its
+ * values come from dfm_sys_set_matrix; orderings other than block colour, ILU(k), the solvers and the handle
+ * operations apply as to a mesh system; element assembly does not. */
+int dfm_sys_create_pattern(dfm_ctx ctx, int nblk, int len, int nrow, const unsigned* rowptr, const unsigned* colind,
+                           dfm_sys* out);
 /* A new system on the same mesh, ordering, ILU schedule and Dirichlet flags as src (no symbolic work; field,
  * parameters, loads and values are not copied). No counterpart in the reference: instances of one geometry
  * (Newton batches, repeated serving geometries). */
@@ -189,6 +196,11 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
 int dfm_sys_set_load_scale(dfm_sys sys, double scale);
 /* Matrix values in the original numbering: crs[ncrs*len*len], dia[n*len*len]. */
 int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia);
+/* The matrix values from the host, in dfm_sys_get_matrix's layout (the caller's numbering and pattern order): the
+ * values host-side element routines left in a CMatDia_BlkCrs (ILinearSystem_Eqn::GetMatrix,
+ * include/delfem/linearsystem_interface_eqnsys.h:48-68). The system counts as assembled; factor it again. The
+ * right-hand side goes in with dfm_sys_set_vector(-1). */
+int dfm_sys_set_matrix(dfm_sys sys, const double* crs, const double* dia);
 
 /* ---- Saved stiffness (CLinearSystem_Save, include/delfem/femls/linearsystem_fieldsave.h:59-106) -------- */
 /* InitializeMarge + the *_SaveStiffMat element loop + FinalizeMarge of CLinearSystem_Save

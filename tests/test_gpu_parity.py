@@ -213,3 +213,51 @@ def test_assembly_high_valence_bit_exact(ctx, ndim, k, oname, ordering):
     assert np.array_equal(dia, cpu.dia), np.abs(dia - cpu.dia).max()
     assert np.array_equal(gs.get_vector(-1), cpu.res)
     gs.close()
+
+
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+@pytest.mark.parametrize("oname,ordering", [("natural", 0), ("multicolor", 1), ("rcm", 3)])
+def test_system_from_block_pattern(ctx, case, oname, ordering):
+    """dfm_sys_create_pattern + dfm_sys_set_matrix (CMatDia_BlkCrs::AddPattern(crs) and the values of host-side
+    element routines): the oracle's assembled matrix installed on a mesh-free system has the oracle's pattern and
+    ILU factor bit for bit and solves as the mesh system that assembled it on the device (the fused dot partials
+    of the sweeps may group differently without the mesh: iterates equal up to rounding)."""
+    cpu, gs, order, _ = setup(N, ctx, case, ordering, small=False)
+    ps = N.System.from_pattern(ctx, cpu.nnode, cpu.len, cpu.rowptr, cpu.colind)
+    rp, ci = ps.pattern()
+    assert np.array_equal(rp, cpu.rowptr) and np.array_equal(ci, cpu.colind)
+    if ordering != N.ORDER_NATURAL:
+        ps.set_ordering(order)
+    ps.set_matrix(cpu.crs, cpu.dia)
+    crs, dia = ps.get_matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    ps.set_vector(-1, cpu.res)
+    ps.factor()
+    gs.factor()
+    f = cpu.ilu0(None if ordering == N.ORDER_NATURAL else order)
+    assert np.array_equal(ps.get_factor()["fcrs"], f["fcrs"])
+    ok1, c1, i1 = ps.solve_pcg(1e-9, 3000)
+    ok2, c2, i2 = gs.solve_pcg(1e-9, 3000)
+    assert ok1 and ok2 and abs(i1 - i2) <= 1 and c1 < 1e-9
+    x1, x2 = ps.get_vector(-2), gs.get_vector(-2)
+    assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x2)
+    r = cpu.res - cpu.matvec(1.0, x1, 0.0, np.zeros_like(x1))
+    assert np.linalg.norm(r) <= 1.01e-9 * np.linalg.norm(cpu.res)
+    with pytest.raises(N.DfmError):
+        ps.assemble(case[3], np.ones(8))  # no elements behind a pattern system
+    ps.close()
+
+
+def test_block_pattern_rejects_bad_input(ctx):
+    rp = np.array([0, 1, 2, 2], np.uint32)
+    # row 1 holding its diagonal, row 0 holding its diagonal, a column out of range
+    for ci in ([1, 1], [0, 2], [1, 3]):
+        with pytest.raises(N.DfmError):
+            N.System.from_pattern(ctx, 3, 1, rp, np.array(ci, np.uint32))
+    with pytest.raises(N.DfmError):  # a repeated column
+        N.System.from_pattern(ctx, 3, 1, np.array([0, 2, 2, 2], np.uint32), np.array([1, 1], np.uint32))
+    s = N.System.from_pattern(ctx, 3, 2, np.array([0, 1], np.uint32), np.array([2], np.uint32))  # rows 1, 2 empty
+    assert s.pattern()[0].tolist() == [0, 1, 1, 1]
+    with pytest.raises(N.DfmError):
+        s.make_ordering(N.ORDER_BLOCKCOLOR)  # needs coordinates
+    s.close()
