@@ -201,7 +201,7 @@ def run_c5_batch(args, N, W):
     scales = 0.001 + (0.02 - 0.001) * N.rng_uniform(W.SEEDS["c5"], 7, args.instances)
     log(f"[bench] c5 x{args.instances} ndof={base.ndof} setup {time.perf_counter() - t0:.1f}s")
     res = {}
-    # mixed precision per system (DFM_MIXED_AUTO): for these vector systems the FP32 factor in the FP64 loop
+    # precision per system (DFM_MIXED_AUTO): for these vector systems the FP64 loop with its true residual checked
     for prec, name in [(N.MIXED_AUTO, "mixed"), (N.FP64, "fp64")]:
         for s in systems:
             s.set_field(None)
@@ -403,7 +403,8 @@ def served_object(args, N, W, ctx):
         "mixed": mixed, "fp64": fp64,
         "mixed_over_served_fp64": mixed["dofs_per_s"] / fp64["dofs_per_s"],
         "precision": "DFM_MIXED_AUTO (Poisson: FP32 matrix/factor/vectors with FP64 reliable updates; elasticity: "
-                     "the FP32 factor in the FP64 loop); conv_ratio is the true relative residual",
+                     "the FP64 loop with its true residual checked, DFM_FP64_TRUE); conv_ratio is the true relative "
+                     "residual",
     }
     if plain:
         out["plain_fp64"] = plain
