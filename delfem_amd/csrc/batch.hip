@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -368,6 +370,12 @@ struct Batch {
   std::vector<int> free_slots;
   std::vector<BatchSeg> segs;
   bool dirty = true;
+  // one iteration of every group as a hipGraph, valid for the tables of one rebuild (gen), and the argument
+  // values its kernel nodes were made from
+  hipGraphExec_t gexec = nullptr;
+  std::vector<std::shared_ptr<void>> gargs;
+  long long gen = 0, gexec_gen = -1;
+  int gen_steps = 0;  // steps run with the tables of this generation
   struct Group {
     int B = 0;
     std::vector<int> members;  // indices into segs
@@ -387,6 +395,7 @@ struct Batch {
       g.d_fslot.release(); g.d_bslot.release();
       g.d_pq.release(); g.d_true.release(); g.d_rr.release(); g.d_rz.release();
     }
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     d_sc.release();
     d_fl.release();
     if (h_fl) (void)hipHostFree(h_fl);
@@ -407,6 +416,8 @@ static void grow(DBuf<T>& b, size_t n, hipError_t& e) {
 static int rebuild_tables(Batch& bt);
 static int rebuild(Batch& bt) {
   const double t0 = now_s();
+  bt.gen++;  // the tables (and the pointers a captured iteration holds) change
+  bt.gen_steps = 0;
   const int rc = rebuild_tables(bt);
   bt.t_rebuild += now_s() - t0;
   bt.n_rebuild++;
@@ -540,37 +551,79 @@ static int rebuild_tables(Batch& bt) {
   return DFM_OK;
 }
 
+// Where an iteration's kernels go: launched on a stream, or appended as kernel nodes to a graph (each depending
+// on the one before), built explicitly rather than by stream capture, so that other threads' device-wide calls (a
+// server's setup workers allocate and free) cannot invalidate it.
+struct Launcher {
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+  hipError_t err = hipSuccess;
+  // the argument values and their pointer arrays, kept for the life of the graph made from them
+  std::vector<std::shared_ptr<void>> keep;
+  template <typename... KA>
+  struct Args {
+    std::tuple<KA...> vals;
+    void* ptrs[sizeof...(KA) > 0 ? sizeof...(KA) : 1];
+  };
+  template <typename... KA, typename... A>
+  void operator()(void (*kern)(KA...), dim3 grid, dim3 block, A... args) {
+    if (err != hipSuccess) return;
+    if (!graph) {
+      hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
+      err = hipGetLastError();
+      return;
+    }
+    auto a = std::make_shared<Args<KA...>>(Args<KA...>{std::tuple<KA...>(static_cast<KA>(args)...), {}});
+    std::apply([&](auto&... v) {
+      int k = 0;
+      ((a->ptrs[k++] = (void*)&v), ...);
+    }, a->vals);
+    keep.push_back(a);
+    hipKernelNodeParams p{};
+    p.func = (void*)kern;
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = a->ptrs;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    err = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &p);
+    if (err == hipSuccess) last = node;
+  }
+};
+
 // T: vectors; MX: reliable-update loop; TF: factor values
 template <int B, typename T, bool MX, typename TF>
-static hipError_t iterate_group(Batch::Group& g, hipStream_t st) {
+static hipError_t iterate_group(Batch::Group& g, Launcher& L) {
   using TM = std::conditional_t<MX, float, double>;
   const SegDev* segs = g.d_segs.p;
   const dim3 nseg((unsigned)g.members.size());
   if (g.nsb)
-    hipLaunchKernelGGL((k_b_spmv<B, TM, T, MX>), dim3(g.nsb), dim3(64 * kBSellWaves), 0, st, segs, g.d_sblk.p,
-                       g.d_pq.p, g.d_true.p);
+    L(k_b_spmv<B, TM, T, MX>, dim3(g.nsb), dim3(64 * kBSellWaves), segs, (const int*)g.d_sblk.p, g.d_pq.p,
+      g.d_true.p);
   for (size_t p = 0; p + 1 < g.fptr.size(); p++) {
     const int c = g.fptr[p + 1] - g.fptr[p];
     if (c)
       with_slab_caps(g.fwo[p], g.fwi[p], B == 3 && sizeof(T) == 8 && sizeof(TF) == 8, [&]<int MO, int MI>() {
-        hipLaunchKernelGGL((k_b_fwd<B, MO, MI, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
-                           (const int2*)g.d_fent.p + g.fptr[p], (const int*)g.d_fslot.p + g.fptr[p], c, g.d_rr.p,
-                           p == 0 ? (const double*)g.d_pq.p : nullptr);
+        L(k_b_fwd<B, MO, MI, TF, T>, dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), segs,
+          (const int2*)g.d_fent.p + g.fptr[p], (const int*)g.d_fslot.p + g.fptr[p], c, g.d_rr.p,
+          p == 0 ? (const double*)g.d_pq.p : (const double*)nullptr);
       });
   }
   for (size_t p = 0; p + 1 < g.bptr.size(); p++) {
     const int c = g.bptr[p + 1] - g.bptr[p];
     if (c)
       with_slab_caps(g.bwo[p], g.bwi[p], B == 3 && sizeof(T) == 8 && sizeof(TF) == 8, [&]<int MO, int MI>() {
-        hipLaunchKernelGGL((k_b_bwd<B, MO, MI, TF, T>), dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), 0, st, segs,
-                           (const int2*)g.d_bent.p + g.bptr[p], (const int*)g.d_bslot.p + g.bptr[p], c, g.d_rz.p);
+        L(k_b_bwd<B, MO, MI, TF, T>, dim3((c + kWPB - 1) / kWPB), dim3(64 * kWPB), segs,
+          (const int2*)g.d_bent.p + g.bptr[p], (const int*)g.d_bslot.p + g.bptr[p], c, g.d_rz.p);
       });
   }
-  hipLaunchKernelGGL((k_b_judge_beta<MX>), nseg, dim3(NT), 0, st, segs, (const double*)g.d_rr.p,
-                     (const double*)g.d_rz.p, (const double*)g.d_true.p);
+  L(k_b_judge_beta<MX>, nseg, dim3(NT), segs, (const double*)g.d_rr.p, (const double*)g.d_rz.p,
+    (const double*)g.d_true.p);
   // the scalar mixed SpMV forms p itself
-  if (!(MX && B == 1)) hipLaunchKernelGGL((k_b_pupdate<T, MX>), dim3(g.nvb), dim3(NT), 0, st, segs, g.d_vblk.p);
-  return hipGetLastError();
+  if (!(MX && B == 1)) L(k_b_pupdate<T, MX>, dim3(g.nvb), dim3(NT), segs, (const int*)g.d_vblk.p);
+  return L.err;
 }
 
 // A mixed-precision segment whose true residual stalled restarts as an FP64 solve of the same system with the
@@ -803,23 +856,52 @@ int dfm_batch_step(dfm_batch b, int iterations, int* n_running) {
     rc = rebuild(bt);
     if (rc != DFM_OK) return rc;
   }
-  for (int t = 0; t < iterations; t++) {
+  // one iteration of every group (the kernels read every scalar and flag from device memory, so repeating the
+  // same launches is the next iteration)
+  auto iteration = [&](Launcher& L) -> hipError_t {
     for (auto& g : bt.grp) {
       if (g.members.empty()) continue;
       hipError_t e;
       if (g.mode == DFM_MIXED)
-        e = g.B == 1 ? iterate_group<1, float, true, float>(g, st)
-                     : g.B == 2 ? iterate_group<2, float, true, float>(g, st) : iterate_group<3, float, true, float>(g, st);
+        e = g.B == 1 ? iterate_group<1, float, true, float>(g, L)
+                     : g.B == 2 ? iterate_group<2, float, true, float>(g, L) : iterate_group<3, float, true, float>(g, L);
       else if (g.mode == DFM_FP32_ILU)
-        e = g.B == 1 ? iterate_group<1, double, false, float>(g, st)
-                     : g.B == 2 ? iterate_group<2, double, false, float>(g, st)
-                                : iterate_group<3, double, false, float>(g, st);
+        e = g.B == 1 ? iterate_group<1, double, false, float>(g, L)
+                     : g.B == 2 ? iterate_group<2, double, false, float>(g, L)
+                                : iterate_group<3, double, false, float>(g, L);
       else
-        e = g.B == 1 ? iterate_group<1, double, false, double>(g, st)
-                     : g.B == 2 ? iterate_group<2, double, false, double>(g, st)
-                                : iterate_group<3, double, false, double>(g, st);
-      DFM_HIP(e);
+        e = g.B == 1 ? iterate_group<1, double, false, double>(g, L)
+                     : g.B == 2 ? iterate_group<2, double, false, double>(g, L)
+                                : iterate_group<3, double, false, double>(g, L);
+      if (e != hipSuccess) return e;
     }
+    return hipSuccess;
+  };
+  // The iteration as a graph, built once per membership (rebuild) from the second step of that membership on (a
+  // serving stream whose members change nearly every step would pay the build for one step's launches); the
+  // previous step ended with a poll, so the stream is idle when an old graph is destroyed.
+  const bool use_graph = iterations > 1 && bt.gen_steps++ > 0 && getenv("DFM_NO_GRAPH") == nullptr;
+  if (use_graph && bt.gexec_gen != bt.gen) {
+    if (bt.gexec) {
+      (void)hipGraphExecDestroy(bt.gexec);
+      bt.gexec = nullptr;
+    }
+    bt.gargs.clear();
+    Launcher L;
+    DFM_HIP(hipGraphCreate(&L.graph, 0));
+    const hipError_t eb = iteration(L);
+    hipError_t ei = eb;
+    if (eb == hipSuccess) ei = hipGraphInstantiate(&bt.gexec, L.graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(L.graph);
+    DFM_HIP(ei);
+    bt.gargs = std::move(L.keep);
+    bt.gexec_gen = bt.gen;
+  }
+  Launcher plain;
+  plain.st = st;
+  for (int t = 0; t < iterations; t++) {
+    if (use_graph) DFM_HIP(hipGraphLaunch(bt.gexec, st));
+    else DFM_HIP(iteration(plain));
   }
   rc = poll(bt);
   if (rc != DFM_OK) return rc;
