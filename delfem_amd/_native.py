@@ -518,7 +518,8 @@ class Request(C.Structure):
     _fields_ = [("ndim", C.c_int), ("len", C.c_int), ("kind", C.c_int), ("nnode", C.c_int), ("nelem", C.c_int),
                 ("coords", C.c_void_p), ("lnods", C.c_void_p), ("params", C.c_void_p), ("npar", C.c_int),
                 ("pstride", C.c_int), ("glob", C.c_void_p), ("bc", C.c_void_p), ("nface", C.c_int),
-                ("faces", C.c_void_p), ("trac", C.c_void_p), ("tol", C.c_double), ("max_iter", C.c_uint)]
+                ("faces", C.c_void_p), ("trac", C.c_void_p), ("tol", C.c_double), ("max_iter", C.c_uint),
+                ("method", C.c_int)]
 
 
 REQ_QUEUED, REQ_SETUP, REQ_SOLVING, REQ_CONVERGED, REQ_MAXITER, REQ_FAILED = range(6)
@@ -607,7 +608,8 @@ class Server:
         _live.add(self)
 
     def submit(self, ndim, length, kind, coords, lnods, params, bc, tol=1e-8, max_iter=20000, glob=None,
-               faces=None, trac=None):
+               faces=None, trac=None, method=0):
+        """method: 0 PCG (batched), 1 PBiCGSTAB (solved by a setup worker beside the batch)"""
         coords, lnods, bc = _f64(coords), _i32(lnods), _i32(bc)
         params = _f64(params)
         npar, pstride = (params.size, 0) if params.ndim == 1 else (params.shape[1], params.shape[1])
@@ -618,7 +620,7 @@ class Server:
                     lnods.ctypes.data, params.ctypes.data, npar, pstride, None if g is None else g.ctypes.data,
                     bc.ctypes.data, 0 if faces is None else faces.size // 3,
                     None if faces is None else faces.ctypes.data, None if trac is None else trac.ctypes.data,
-                    tol, max_iter)
+                    tol, max_iter, method)
         t = C.c_longlong()
         _check(lib().dfm_server_submit(self.h, C.byref(q), C.byref(t)))
         return t.value

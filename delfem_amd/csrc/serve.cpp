@@ -33,6 +33,7 @@ struct Request {
   std::vector<int> lnods, bc, faces;
   double tol = 1e-8;
   unsigned max_iter = 1000;
+  int method = DFM_METHOD_PCG;
   long long est_bytes = 0;
   int state = Q_QUEUED;
   dfm_sys sys = nullptr;
@@ -106,16 +107,42 @@ struct Server {
       dead.clear();
       if (!r) continue;
       const double t0 = now_s();
-      const int rc = setup(c, *r);
+      int rc = setup(c, *r);
+      const double t1 = now_s();
+      // PBiCGSTAB requests are solved here, on this worker's stream, beside the batch (which runs PCG only)
+      int bst = 0;
+      if (rc == DFM_OK && r->method == DFM_METHOD_PBICGSTAB) {
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          r->state = Q_SOLVING;
+        }
+        double conv = r->tol;
+        unsigned it = r->max_iter;
+        rc = dfm_solve_pbicgstab(r->sys, precision, &conv, &it);
+        if (rc >= 0) {
+          bst = (rc == 1 && it < r->max_iter) ? Q_CONVERGED : Q_MAXITER;
+          double tr = 0.0;
+          rc = sys_true_residual(r->sys->s, r->sys->s.ctx->stream, &tr);
+          std::lock_guard<std::mutex> lk(mu);
+          r->conv = conv;
+          r->iters = it;
+          r->true_rel = tr;
+        }
+      }
       {
         std::lock_guard<std::mutex> lk(mu);
-        t_setup += now_s() - t0;
+        t_setup += t1 - t0;
         n_setup++;
         if (rc != DFM_OK) {
           r->state = Q_FAILED;
           r->err = dfm_last_error();
           r->t_done = now_s();
           active--;
+        } else if (bst) {
+          r->state = bst;
+          r->t_done = now_s();
+          active--;
+          admit_locked();
         } else {
           ready_q.push_back(r);
         }
@@ -310,6 +337,9 @@ int dfm_server_submit(dfm_server s, const dfm_request* q, long long* ticket) {
   auto* r = new Request;
   r->ndim = q->ndim; r->len = q->len; r->kind = q->kind; r->nnode = q->nnode; r->nelem = q->nelem;
   r->npar = q->npar; r->pstride = q->pstride; r->nface = q->nface; r->tol = q->tol; r->max_iter = q->max_iter;
+  DFM_CHECK(q->method == DFM_METHOD_PCG || q->method == DFM_METHOD_PBICGSTAB, DFM_ERR_ARG,
+            "dfm_server_submit: unknown method");
+  r->method = q->method;
   r->coords.assign(q->coords, q->coords + (size_t)q->nnode * q->ndim);
   r->lnods.assign(q->lnods, q->lnods + (size_t)q->nelem * (q->ndim + 1));
   const size_t np = q->pstride ? (size_t)q->pstride * q->nelem : (size_t)q->npar;

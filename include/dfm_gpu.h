@@ -362,7 +362,11 @@ typedef struct {
   const double* trac;
   double tol;                   /* relative residual                                                   */
   unsigned max_iter;
+  int method;                   /* DFM_METHOD_PCG (batched) or DFM_METHOD_PBICGSTAB (solved by a setup worker on
+                                   its own stream beside the batch, dfm_solve_pbicgstab with the server's
+                                   precision)                                                          */
 } dfm_request;
+enum { DFM_METHOD_PCG = 0, DFM_METHOD_PBICGSTAB = 1 };
 enum { DFM_REQ_QUEUED = 0, DFM_REQ_SETUP = 1, DFM_REQ_SOLVING = 2, DFM_REQ_CONVERGED = 3, DFM_REQ_MAXITER = 4,
        DFM_REQ_FAILED = 5 };
 int dfm_server_create(dfm_ctx ctx, int precision, long long budget_bytes, int max_active, int nworkers,

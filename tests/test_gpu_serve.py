@@ -134,3 +134,35 @@ def test_served_newmark_request(ctx):
     ok, _, _, want, _ = cpu.solve(cpu.ilu0(), "pcg", 1e-12, 5000)
     assert ok
     assert np.linalg.norm(got - want) <= 1e-7 * np.linalg.norm(want)
+
+
+def test_served_pbicgstab_requests(ctx):
+    """PBiCGSTAB requests beside batched PCG ones: each is solved by a setup worker on its own stream with the
+    server's precision (mixed: the FP32 factor in the FP64 loop, the true residual checked), bit for bit the result
+    of the same system solved alone, its true residual with the oracle's matrix within the tolerance."""
+    reqs = _requests()[:6]
+    srv = N.Server(ctx, N.MIXED, 1 << 30, max_active=4, nworkers=2)
+    tickets = [srv.submit(q["ndim"], q["len"], q["kind"], q["coords"], q["lnods"], q["params"], q["bc"], tol=1e-8,
+                          method=k % 2) for k, q in enumerate(reqs)]
+    _drain(srv, tickets)
+    for k, (q, t) in enumerate(zip(reqs, tickets)):
+        info = srv.poll(t)
+        assert info["state"] == N.REQ_CONVERGED, info
+        u = srv.result(t, q["ndof"])
+        if k % 2:
+            s = W.build_system(ctx, dict(q, faces=None, trac=None), N.ORDER_BLOCKCOLOR)
+            s.assemble(q["kind"], q["params"])
+            s.factor()
+            ok, conv, it = s.solve_pbicgstab(1e-8, 20000, N.MIXED)
+            assert ok and it == info["iters"] and conv == info["conv"]
+            assert np.array_equal(s.get_vector(-2), u)
+            assert conv == srv.true_residual(t) and conv <= 1e-8
+            s.close()
+        cpu = P.CpuSystem(q["ndim"], q["coords"], q["lnods"], q["len"])
+        cpu.assemble(q["kind"], q["params"])
+        cpu.set_bc(q["bc"])
+        b = cpu.res
+        r = b - cpu.matvec(1.0, u, 0.0, np.zeros_like(b))
+        assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(b)
+        srv.release(t)
+    srv.close()
