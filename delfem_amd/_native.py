@@ -319,14 +319,27 @@ class System:
         npar, pstride = (params.size, 0) if params.ndim == 1 else (params.shape[1], params.shape[1])
         _check(lib().dfm_sys_assemble_save(self.h, kind, _p(params), npar, pstride))
 
+    def assemble_savedia(self, kind, params, glob):
+        """InitializeMarge + element loop + FinalizeMarge of the lumped-mass saved systems: kind DIFFUSION2D
+        (CLinearSystem_SaveDiaM_Newmark, params (rho, alpha, source), glob (dt, gamma)) or SOLID2D
+        (CLinearSystem_SaveDiaM_NewmarkBeta, (lambda, myu, rho, gx, gy), glob (dt, gamma, beta)). Set the Dirichlet
+        flags first; make_residual and update_field_save then follow these classes."""
+        params = _f64(params)
+        npar, pstride = (params.size, 0) if params.ndim == 1 else (params.shape[1], params.shape[1])
+        g = np.zeros(3)
+        g[: len(glob)] = glob
+        _check(lib().dfm_sys_assemble_save_dia(self.h, kind, _p(params), npar, pstride, _p(g)))
+
     def make_residual(self):
-        """CLinearSystem_Save::MakeResidual: residual = f - K_b u_field, update = u_field; returns ||residual||."""
+        """MakeResidual of the saved systems: residual = f - K_b w, update = w (w = u_field for CLinearSystem_Save,
+        the Newmark predictor for the lumped-mass ones); returns ||residual||."""
         rn = C.c_double(0.0)
         _check(lib().dfm_sys_make_residual(self.h, C.byref(rn)))
         return rn.value
 
     def update_field_save(self):
-        """CLinearSystem_Save::UpdateValueOfField (VALUE): u_field = update, Dirichlet values kept."""
+        """UpdateValueOfField of the saved systems: CLinearSystem_Save (VALUE): u_field = update, Dirichlet values
+        kept; the lumped-mass systems: the Newmark update of u (and v), velocity / acceleration = update."""
         _check(lib().dfm_sys_update_field_save(self.h))
 
     def get_force(self):

@@ -258,7 +258,7 @@ static int apply_order(Sys& s, const std::vector<int>& order, const std::vector<
     return DFM_ERR_ARG;
   }
   s.assembled = s.factored = s.factored32 = false;
-  s.saved = false;
+  s.saved = SAVE_NONE;
   const int rc = upload_structure(s);
   if (rc != DFM_OK) return rc;
   return upload_bc(s);
@@ -642,7 +642,7 @@ int dfm_sys_set_bc(dfm_sys sys, const int* flags) {
   Sys& s = sys->s;
   s.bc0.assign((size_t)mdof(s), 0);
   for (long long k = 0; k < mdof(s); k++) s.bc0[(size_t)k] = flags[k] ? 1 : 0;
-  s.saved = false;  // the boundary coupling depends on the flags
+  s.saved = SAVE_NONE;  // the boundary coupling depends on the flags
   return upload_bc(s);
 }
 
@@ -953,7 +953,7 @@ int dfm_sys_assemble_resident(dfm_sys sys, int kind, const double* glob, double*
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
   rc = finish_matrix(s, st);
   if (rc != DFM_OK) return rc;
-  s.saved = false;
+  s.saved = SAVE_NONE;
   if (res_norm) {
     DFM_HIP(launch_dot(s.d_res.p, s.d_res.p, m, s.d_partial.p, s.d_counter.p, s.d_scal.p + S_TMP, st));
     DFM_HIP(hipMemcpyAsync(s.h_pinned_d, s.d_scal.p + S_TMP, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -998,7 +998,7 @@ int dfm_sys_set_matrix(dfm_sys sys, const double* crs, const double* dia) {
   DFM_HIP(hipStreamSynchronize(st));  // the host arrays go out of scope
   const int rc = finish_matrix(s, st);
   if (rc != DFM_OK) return rc;
-  s.saved = false;
+  s.saved = SAVE_NONE;
   DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
@@ -1024,7 +1024,12 @@ int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia) {
 // treatment (CMat_BlkCrs::SetPatternBoundary, src/matvec/mat_blkcrs.cpp:1177-1265, keeps the diagonal block
 // first and the blocks with a fixed column; SetBoundaryConditionInverse_Colum, :876-900, zeroes the free
 // columns; SetBoundaryCondition_Row, :910-953, the fixed rows), less the entries that are zero by that treatment.
-static int build_boundary_terms(Sys& s) {
+//
+// full_rows: the lumped-mass saved systems' boundary matrix instead (CMat_BlkCrs::SetPatternDia,
+// src/matvec/mat_blkcrs.cpp:1267-1319: the whole row of the matrix pattern, the diagonal block placed before the
+// first column below the row, last in a row with none; only its fixed rows are zeroed, :910-953): for a free dof,
+// every column of those blocks in that order.
+static int build_boundary_terms(Sys& s, bool full_rows = false) {
   const int B = s.len, bs = s.bs;
   const long long m = mdof(s), ndia = (long long)s.n * bs;
   std::vector<int> fl = s.bc0;
@@ -1036,7 +1041,25 @@ static int build_boundary_terms(Sys& s) {
     const int i1 = s.order[i0];
     for (int d = 0; d < B; d++) {
       const long long k = (long long)i0 * B + d;
-      if (!fl[(size_t)i1 * B + d]) {
+      if (!fl[(size_t)i1 * B + d] && full_rows) {
+        bool dia_done = false;
+        auto dia_block = [&]() {
+          for (int jd = 0; jd < B; jd++) {
+            bcol.push_back((unsigned)((long long)i1 * B + jd));
+            bsrc.push_back((long long)i0 * bs + d * B + jd);
+          }
+          dia_done = true;
+        };
+        for (unsigned q = s.rowptr0[i1]; q < s.rowptr0[i1 + 1]; q++) {
+          const unsigned j1 = s.colind0[q];
+          if ((unsigned)i1 > j1 && !dia_done) dia_block();
+          for (int jd = 0; jd < B; jd++) {
+            bcol.push_back((unsigned)((long long)j1 * B + jd));
+            bsrc.push_back(ndia + s.perm_crs[q] * bs + d * B + jd);
+          }
+        }
+        if (!dia_done) dia_block();
+      } else if (!fl[(size_t)i1 * B + d]) {
         for (int jd = 0; jd < B; jd++)
           if (fl[(size_t)i1 * B + jd]) {
             bcol.push_back((unsigned)((long long)i1 * B + jd));
@@ -1104,7 +1127,76 @@ int dfm_sys_assemble_save(dfm_sys sys, int kind, const double* params, int npar,
   DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
   rc = finish_matrix(s, st);
   if (rc != DFM_OK) return rc;
-  s.saved = true;
+  s.saved = SAVE_PLAIN;
+  DFM_HIP(hipStreamSynchronize(st));
+  return DFM_OK;
+}
+
+int dfm_sys_assemble_save_dia(dfm_sys sys, int kind, const double* params, int npar, int pstride,
+                              const double* glob) {
+  NEED(sys, "dfm_sys_assemble_save_dia");
+  NEED(params, "dfm_sys_assemble_save_dia");
+  NEED(glob, "dfm_sys_assemble_save_dia");
+  Sys& s = sys->s;
+  DFM_CHECK((kind == DFM_DIFFUSION2D || kind == DFM_SOLID2D) && s.ndim == 2, DFM_ERR_ARG,
+            "dfm_sys_assemble_save_dia: the reference's lumped-mass saved systems take DFM_DIFFUSION2D "
+            "(CLinearSystem_SaveDiaM_Newmark) or DFM_SOLID2D (CLinearSystem_SaveDiaM_NewmarkBeta) on triangles");
+  int rc = dfm_sys_set_params(sys, params, npar, pstride);
+  if (rc != DFM_OK) return rc;
+  rc = check_kind(s, kind);
+  if (rc != DFM_OK) return rc;
+  hipStream_t st = s.ctx->stream;
+  const long long m = mdof(s);
+  if (s.d_force.n != (size_t)m) DFM_HIP(s.d_force.alloc((size_t)m));
+  // InitializeMarge (src/femls/linearsystem_fieldsave.cpp:398-420, :662-686): matrix, residual, update, force and
+  // boundary matrix to zero
+  auto zero = [&]() -> hipError_t {
+    hipError_t e;
+    if ((e = launch_fill(s.d_crs.p, 0.0, (long long)s.ncrs * s.bs, st)) != hipSuccess) return e;
+    if ((e = launch_fill(s.d_dia.p, 0.0, (long long)s.n * s.bs, st)) != hipSuccess) return e;
+    if ((e = launch_fill(s.d_res.p, 0.0, m, st)) != hipSuccess) return e;
+    return launch_fill(s.d_upd.p, 0.0, m, st);
+  };
+  DFM_HIP(zero());
+  DFM_HIP(launch_fill(s.d_force.p, 0.0, m, st));
+  AsmArgs a{};
+  a.ndim = s.ndim; a.npoel = s.npoel; a.len = s.len; a.n = s.n;
+  a.coords = s.d_coords.p; a.lnods = s.d_lnods.p; a.elsup_ptr = s.d_elsup_ptr.p; a.elsup = s.d_elsup.p;
+  a.order = s.d_order.p; a.inv = s.d_inv.p;
+  a.npar = s.npar; a.pstride = s.pstride;
+  a.rowptr = s.d_rowptr.p; a.colind = s.d_colind.p; a.crs = s.d_crs.p; a.dia = s.d_dia.p;
+  // 1. the boundary matrix's values: the stiffness alone, K (diffusion: alpha area grad grad, the saved Poisson
+  // element on (alpha, source); elasticity: the saved static element), in the same element order as the matrix;
+  // its force goes to the residual handle, which is cleared again
+  a.kind = kind == DFM_DIFFUSION2D ? DFM_POISSON : DFM_SOLID2D;
+  a.params = kind == DFM_DIFFUSION2D ? s.d_par.p + 1 : s.d_par.p;
+  a.res = s.d_res.p;
+  a.save = true;
+  ElemStore es{};
+  const bool store = elem_store(s, a.kind, true, es);
+  DFM_HIP(launch_assemble(a, store ? &es : nullptr, st));
+  rc = build_boundary_terms(s, true);
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(launch_gather_terms(s.nbterm, s.d_bsrc.p, (long long)s.n * s.bs, s.d_dia.p, s.d_crs.p, s.d_bval.p, st));
+  // 2. the system matrix and the force: the lumped-mass element loop
+  DFM_HIP(zero());
+  a.kind = kind == DFM_DIFFUSION2D ? KIND_DIFFUSION2D_SAVEDIA : KIND_SOLID2D_SAVEDIA;
+  a.params = s.d_par.p;
+  a.glob0 = glob[0];
+  a.glob1 = glob[1];
+  a.glob2 = kind == DFM_SOLID2D ? glob[2] : 0.0;
+  a.res = s.d_force.p;
+  a.save = false;
+  DFM_HIP(launch_assemble(a, nullptr, st));
+  // FinalizeMarge: the matrix's Dirichlet rows and columns (the residual is still zero)
+  DFM_HIP(launch_set_bc(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_bc.p, st));
+  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, m, st));
+  rc = finish_matrix(s, st);
+  if (rc != DFM_OK) return rc;
+  s.saved = kind == DFM_DIFFUSION2D ? SAVE_DIA_NEWMARK : SAVE_DIA_NEWMARKBETA;
+  s.save_dt = glob[0];
+  s.save_gamma = glob[1];
+  s.save_beta = kind == DFM_SOLID2D ? glob[2] : 0.0;
   DFM_HIP(hipStreamSynchronize(st));
   return DFM_OK;
 }
@@ -1112,10 +1204,20 @@ int dfm_sys_assemble_save(dfm_sys sys, int kind, const double* params, int npar,
 int dfm_sys_make_residual(dfm_sys sys, double* res_norm) {
   NEED(sys, "dfm_sys_make_residual");
   Sys& s = sys->s;
-  DFM_CHECK(s.saved, DFM_ERR_STATE, "dfm_sys_make_residual: assemble with dfm_sys_assemble_save first");
+  DFM_CHECK(s.saved, DFM_ERR_STATE,
+            "dfm_sys_make_residual: assemble with dfm_sys_assemble_save or dfm_sys_assemble_save_dia first");
   hipStream_t st = s.ctx->stream;
   const long long m = mdof(s);
-  DFM_HIP(launch_make_residual(s.n, s.len, s.d_order.p, s.d_bptr.p, s.d_bcol.p, s.d_bval.p, s.d_force.p, s.d_u.p,
+  double c0 = 0.0, c1 = 0.0;
+  if (s.saved == SAVE_DIA_NEWMARK) c0 = s.save_dt * (1 - s.save_gamma);
+  if (s.saved == SAVE_DIA_NEWMARKBETA) {
+    c0 = s.save_dt;
+    c1 = s.save_dt * s.save_dt * (0.5 - s.save_beta);
+    const int rc = ensure_acc(s);
+    if (rc != DFM_OK) return rc;
+  }
+  DFM_HIP(launch_make_residual(s.n, s.len, s.saved, c0, c1, s.d_order.p, s.d_bptr.p, s.d_bcol.p, s.d_bval.p,
+                               s.d_force.p, s.d_u.p, s.d_v.p, s.d_a.p, s.saved == SAVE_PLAIN ? nullptr : s.d_bc.p,
                                s.d_res.p, s.d_upd.p, st));
   s.prepared = -1;
   if (res_norm) {
@@ -1130,6 +1232,14 @@ int dfm_sys_make_residual(dfm_sys sys, double* res_norm) {
 int dfm_sys_update_field_save(dfm_sys sys) {
   NEED(sys, "dfm_sys_update_field_save");
   Sys& s = sys->s;
+  if (s.saved == SAVE_DIA_NEWMARK || s.saved == SAVE_DIA_NEWMARKBETA) {
+    const int rc = ensure_acc(s);
+    if (rc != DFM_OK) return rc;
+    DFM_HIP(launch_update_field_savedia(s.n, s.len, s.saved, s.save_dt, s.save_gamma, s.save_beta, s.d_order.p,
+                                        s.d_bc.p, s.d_upd.p, s.d_u.p, s.d_v.p, s.d_a.p, s.ctx->stream));
+    DFM_HIP(hipStreamSynchronize(s.ctx->stream));
+    return DFM_OK;
+  }
   DFM_HIP(launch_update_field_save(s.n, s.len, s.d_order.p, s.d_bc.p, s.d_upd.p, s.d_u.p, s.ctx->stream));
   DFM_HIP(hipStreamSynchronize(s.ctx->stream));
   return DFM_OK;
@@ -1594,10 +1704,10 @@ int dfm_sys_bench_kernel(dfm_sys sys, int which, int reps, double* avg_ms, doubl
   if (which == 6) {
     // MakeResidual of the saved stiffness: force, residual, update and the field once per dof, the order map,
     // the term ranges, and per term its value, its column and the field value it gathers
-    DFM_CHECK(s.saved, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble with dfm_sys_assemble_save first");
+    DFM_CHECK(s.saved == SAVE_PLAIN, DFM_ERR_STATE, "dfm_sys_bench_kernel: assemble with dfm_sys_assemble_save first");
     auto launch = [&]() {
-      return launch_make_residual(s.n, s.len, s.d_order.p, s.d_bptr.p, s.d_bcol.p, s.d_bval.p, s.d_force.p, s.d_u.p,
-                                  s.d_res.p, s.d_upd.p, st);
+      return launch_make_residual(s.n, s.len, SAVE_PLAIN, 0.0, 0.0, s.d_order.p, s.d_bptr.p, s.d_bcol.p, s.d_bval.p,
+                                  s.d_force.p, s.d_u.p, nullptr, nullptr, nullptr, s.d_res.p, s.d_upd.p, st);
     };
     const int rc = time_launches(st, reps, launch, avg_ms);
     if (rc != DFM_OK) return rc;

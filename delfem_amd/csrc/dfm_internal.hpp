@@ -244,7 +244,8 @@ struct Sys {
   DBuf<long long> d_bsrc;   // matrix value of each term: diagonal block entry (< n*bs) or n*bs + crs entry
   DBuf<double> d_bval;      // K_b values, gathered before the Dirichlet zeroing
   long long nbterm = 0;
-  bool saved = false;
+  int saved = 0;  // SAVE_NONE, SAVE_PLAIN (CLinearSystem_Save), SAVE_DIA_NEWMARK / SAVE_DIA_NEWMARKBETA
+  double save_dt = 0.0, save_gamma = 0.0, save_beta = 0.0;  // the lumped-mass saved systems' Newmark parameters
 
   // resident assembly inputs
   DBuf<double> d_par;

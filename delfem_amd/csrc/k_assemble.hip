@@ -445,6 +445,39 @@ __device__ inline void row_solid_save(int a, const double co[4][3], const double
   for (int r = 0; r < NDIM; r++) er[r] = (NDIM == 2) ? w * rho * p[3 + r] * 0.3333333333333333333 : w * rho * p[3 + r] * 0.25;
 }
 
+// Lumped-mass saved systems (CLinearSystem_SaveDiaM_Newmark / _NewmarkBeta): the rows of the system matrix; the
+// boundary matrix is the stiffness alone, assembled with the saved kinds above.
+// AddLinearSystem_Diffusion2D_P1 on CLinearSystem_SaveDiaM_Newmark, src/femeqn/eqn_diffusion.cpp:565-650: row a of
+// (gamma dt) eC plus the lumped mass rho area / 3 on the diagonal; the source term with its 14-digit literal.
+// p = (rho, alpha, source)
+__device__ inline void row_diffusion2d_savedia(int a, const double co[4][3], const double* p, double dt, double gamma,
+                                               double* em, double* er) {
+  const double rho = p[0], alpha = p[1], source = p[2];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double d[3][2];
+  tri_dldx(d, co);
+  const double gdt = gamma * dt;
+  for (int j = 0; j < 3; j++) em[j] = gdt * (alpha * area * (d[a][0] * d[j][0] + d[a][1] * d[j][1]));
+  em[a] += rho * area / 3.0;
+  er[0] = source * area * 0.33333333333333;
+}
+
+// AddLinSys_LinearSolid2D_NonStatic_Save_NewmarkBeta_P1, src/femeqn/eqn_linear_solid2d.cpp:1051-1180: row a of
+// M + (beta dt dt) K on the diagonal block (M: area rho / 3 on its diagonal, accumulated onto zero as written there)
+// and (beta dt dt) K elsewhere, K as the saved static loop writes it; er = the node's gravity (x, y), which the
+// reference adds to the node's first dof, x then y. p = (lambda, myu, rho, gx, gy)
+__device__ inline void row_solid2d_savedia(int a, const double co[4][3], const double* p, double dt, double beta,
+                                           double* em, double* er) {
+  row_solid_save<2>(a, co, p, em, er);
+  const double m = tri_area(co[0], co[1], co[2]) * p[2] * 0.33333333333333333333333;
+  const double c = beta * dt * dt;
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 4; k++) {
+      const double kv = c * em[j * 4 + k];
+      em[j * 4 + k] = (j == a) ? ((k == 0 || k == 3) ? 0.0 + m : 0.0) + kv : kv;
+    }
+}
+
 // Newmark-beta rows (AddLinSys_LinearSolid2D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_linear_solid2d.cpp:1182-1337;
 // AddLinSys_LinearSolid3D_NonStatic_NewmarkBeta_P1, src/femeqn/eqn_linear_solid3d.cpp:525-690): stiffness K as
 // the saved loops write it, consistent mass M (rho w / 12 in 2D, rho w / 20 in 3D, doubled on the diagonal
@@ -695,6 +728,8 @@ __device__ inline void elem_row(const AsmArgs& A, const ElemIn<KIND, NDIM, B>& E
       row_stvk2d_newmark(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob2, A.glob3 != 0.0, em, er);
     if (KIND == DFM_STVK2D_BEULER) row_stvk2d_beuler(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob1 != 0.0, em, er);
     if (KIND == DFM_SOLID2D_BEULER) row_solid2d_beuler(a, E.co, E.u, E.vv, E.aa, p, A.glob0, A.glob1 != 0.0, em, er);
+    if (KIND == KIND_DIFFUSION2D_SAVEDIA) row_diffusion2d_savedia(a, E.co, p, A.glob0, A.glob1, em, er);
+    if (KIND == KIND_SOLID2D_SAVEDIA) row_solid2d_savedia(a, E.co, p, A.glob0, A.glob2, em, er);
   }
 }
 
@@ -736,7 +771,13 @@ __global__ void __launch_bounds__(128) k_assemble(AsmArgs A) {
         }
       }
     }
-    for (int d = 0; d < B; d++) racc[d] += er[d];
+    if (KIND == KIND_SOLID2D_SAVEDIA) {
+      // both gravity components of the element go to the node's first dof, one after the other
+      racc[0] += er[0];
+      racc[0] += er[1];
+    } else {
+      for (int d = 0; d < B; d++) racc[d] += er[d];
+    }
   }
   // surface loads (a reference extension): added after the element contributions, in face order
   if (A.load_ptr) {
@@ -1158,7 +1199,8 @@ static hipError_t launch_solid_rowgroups(const AsmArgs& a, int lpr, hipStream_t 
 template <int KIND, int NDIM, int B, bool SAVE>
 static hipError_t assemble_kind(const AsmArgs& a, const ElemStore* es, hipStream_t st) {
   constexpr int NNO = NDIM + 1;
-  if (!es) {
+  // the lumped-mass saved kinds add two force values per element row to one dof: the per-row gather only
+  if (!es || KIND == KIND_DIFFUSION2D_SAVEDIA || KIND == KIND_SOLID2D_SAVEDIA) {
     const int nt = 128;
     hipLaunchKernelGGL((k_assemble<KIND, NDIM, B, SAVE>), dim3((a.n + nt - 1) / nt), dim3(nt), 0, st, a);
     return hipGetLastError();
@@ -1210,6 +1252,8 @@ hipError_t launch_assemble(const AsmArgs& a, const ElemStore* es, hipStream_t st
   DFM_ASM(DFM_STVK3D_NEWMARK, 3, 3)
   DFM_ASM(DFM_STVK2D_NEWMARK, 2, 2)
   DFM_ASM(DFM_STVK2D_BEULER, 2, 2)
+  DFM_ASM(KIND_DIFFUSION2D_SAVEDIA, 2, 1)
+  DFM_ASM(KIND_SOLID2D_SAVEDIA, 2, 2)
 #undef DFM_ASM
   return hipErrorInvalidValue;
 }
@@ -1381,33 +1425,95 @@ hipError_t launch_gather_terms(long long nterm, const long long* src, long long 
   return hipGetLastError();
 }
 
-// CLinearSystem_Save::MakeResidual (src/femls/linearsystem_fieldsave.cpp:206-247): update = field values,
-// residual = force, then CMat_BlkCrs::MatVec(-1, update, 1, residual) over the boundary matrix
-// (src/matvec/mat_blkcrs.cpp:955-1170): residual[i] += (-1 * K_b[i][j]) * u[j] entry by entry, which is
-// residual[i] - K_b[i][j] * u[j] exactly. The boundary matrix keeps only fixed columns in free rows; its other
-// entries are zero and leave the sum unchanged, so each dof runs over its kept terms, in the reference's order
-// (diagonal block first, then the row's blocks in pattern order). One thread per scalar dof.
-__global__ void k_make_residual(long long m, int len, const int* order, const unsigned* bptr, const unsigned* bcol,
-                                const double* bval, const double* force, const double* u, double* res, double* upd) {
+// The update vector w the saved systems' MakeResidual forms before its MatVec, dof j of the field (original
+// numbering): mode SAVE_PLAIN u (CLinearSystem_Save); SAVE_DIA_NEWMARK u + v c0 with c0 = dt (1 - gamma)
+// (CLinearSystem_SaveDiaM_Newmark, src/femls/linearsystem_fieldsave.cpp:433-457); SAVE_DIA_NEWMARKBETA
+// u + v c0 + a c1 with c0 = dt, c1 = dt dt (0.5 - beta) (_NewmarkBeta, :705-737). Each term is added as
+// CNodeAry::AddValueFromNodeSegment adds it (value * alpha, src/femfield/node_ary.cpp:302-325).
+__device__ inline double save_w(int mode, const double* u, const double* v, const double* a, long long j, double c0,
+                                double c1) {
+  double w = u[j];
+  if (mode != SAVE_PLAIN) w += v[j] * c0;
+  if (mode == SAVE_DIA_NEWMARKBETA) w += a[j] * c1;
+  return w;
+}
+
+// MakeResidual of the saved systems (CLinearSystem_Save, src/femls/linearsystem_fieldsave.cpp:206-247, and the
+// lumped-mass ones above): update = w, residual = force, then CMat_BlkCrs::MatVec(-1, update, 1, residual) over
+// the boundary matrix (src/matvec/mat_blkcrs.cpp:955-1170): residual[i] += (-1 * K_b[i][j]) * w[j] entry by entry,
+// which is residual[i] - K_b[i][j] * w[j] exactly. Each dof runs over its kept terms in the reference's order. The
+// lumped-mass systems then zero the residual at fixed dofs (bc non-null). One thread per scalar dof.
+__global__ void k_make_residual(long long m, int len, int mode, double c0, double c1, const int* order,
+                                const unsigned* bptr, const unsigned* bcol, const double* bval, const double* force,
+                                const double* u, const double* v, const double* a, const int* bc, double* res,
+                                double* upd) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const long long i0 = k / len;
   const int d = (int)(k - i0 * len);
   const long long i1 = order ? order[i0] : i0;
   double r = force[k];
-  for (unsigned t = bptr[k]; t < bptr[k + 1]; t++) r = r - bval[t] * u[bcol[t]];
+  for (unsigned t = bptr[k]; t < bptr[k + 1]; t++) r = r - bval[t] * save_w(mode, u, v, a, bcol[t], c0, c1);
+  if (bc && bc[k]) r = 0.0;
   res[k] = r;
-  upd[k] = u[i1 * len + d];
+  upd[k] = save_w(mode, u, v, a, i1 * len + d, c0, c1);
 }
 
-hipError_t launch_make_residual(int n, int len, const int* order, const unsigned* bptr, const unsigned* bcol,
-                                const double* bval, const double* force, const double* u, double* res, double* upd,
-                                hipStream_t st) {
+hipError_t launch_make_residual(int n, int len, int mode, double c0, double c1, const int* order,
+                                const unsigned* bptr, const unsigned* bcol, const double* bval, const double* force,
+                                const double* u, const double* v, const double* a, const int* bc, double* res,
+                                double* upd, hipStream_t st) {
   const long long m = (long long)n * len;
   if (m == 0) return hipSuccess;
   const int nt = 256;
-  hipLaunchKernelGGL(k_make_residual, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, order, bptr, bcol,
-                     bval, force, u, res, upd);
+  hipLaunchKernelGGL(k_make_residual, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, mode, c0, c1,
+                     order, bptr, bcol, bval, force, u, v, a, bc, res, upd);
+  return hipGetLastError();
+}
+
+// UpdateValueOfField of the lumped-mass saved systems, per dof (free: bc == 0). SAVE_DIA_NEWMARK, VELOCITY
+// (src/femls/linearsystem_fieldsave.cpp:474-520): u += dt (1 - gamma) v0 + dt gamma v1 (v1 the update); a fixed dof
+// keeps its velocity in the update; velocity = update. SAVE_DIA_NEWMARKBETA, ACCELERATION (:760-860):
+// u += dt v0 + dt dt (0.5 - beta) a0 + dt dt beta a1, v += dt (1 - gamma) a0 + gamma dt a1 (a1 the update); a fixed
+// dof keeps its acceleration in the update; acceleration = update. The expressions are written as there.
+__global__ void k_update_field_savedia(long long m, int len, int mode, double dt, double gamma, double beta,
+                                       const int* order, const int* bc, double* upd, double* u, double* v,
+                                       double* a) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long long i0 = k / len;
+  const int d = (int)(k - i0 * len);
+  const long long f = (order ? order[i0] : i0) * len + d;
+  if (mode == SAVE_DIA_NEWMARK) {
+    const double velo0 = v[f];
+    if (!bc[k]) {
+      const double velo1 = upd[k];
+      u[f] += dt * (1 - gamma) * velo0 + dt * gamma * velo1;
+    } else {
+      upd[k] = velo0;
+    }
+    v[f] = upd[k];
+  } else {
+    const double velo0 = v[f], acc0 = a[f];
+    if (!bc[k]) {
+      const double acc1 = upd[k];
+      u[f] += dt * velo0 + dt * dt * (0.5 - beta) * acc0 + dt * dt * beta * acc1;
+      v[f] += dt * (1 - gamma) * acc0 + gamma * dt * acc1;
+    } else {
+      upd[k] = acc0;
+    }
+    a[f] = upd[k];
+  }
+}
+
+hipError_t launch_update_field_savedia(int n, int len, int mode, double dt, double gamma, double beta,
+                                       const int* order, const int* bc, double* upd, double* u, double* v, double* a,
+                                       hipStream_t st) {
+  const long long m = (long long)n * len;
+  if (m == 0) return hipSuccess;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_update_field_savedia, dim3((unsigned)((m + nt - 1) / nt)), dim3(nt), 0, st, m, len, mode,
+                     dt, gamma, beta, order, bc, upd, u, v, a);
   return hipGetLastError();
 }
 

@@ -64,9 +64,18 @@ hipError_t launch_update_field_newmark(int n, int len, const int* order, const d
 // off-diagonal blocks at src - ndia); MakeResidual; UpdateValueOfField of CLinearSystem_Save
 hipError_t launch_gather_terms(long long nterm, const long long* src, long long ndia, const double* dia,
                                const double* crs, double* val, hipStream_t st);
-hipError_t launch_make_residual(int n, int len, const int* order, const unsigned* bptr, const unsigned* bcol,
-                                const double* bval, const double* force, const double* u, double* res, double* upd,
-                                hipStream_t st);
+// saved-system kinds of MakeResidual / UpdateValueOfField (Sys::saved)
+enum { SAVE_NONE = 0, SAVE_PLAIN = 1, SAVE_DIA_NEWMARK = 2, SAVE_DIA_NEWMARKBETA = 3 };
+// internal element kinds of the lumped-mass saved systems' matrix (dfm_sys_assemble_save_dia; not public kinds)
+constexpr int KIND_DIFFUSION2D_SAVEDIA = 100;
+constexpr int KIND_SOLID2D_SAVEDIA = 101;
+hipError_t launch_make_residual(int n, int len, int mode, double c0, double c1, const int* order,
+                                const unsigned* bptr, const unsigned* bcol, const double* bval, const double* force,
+                                const double* u, const double* v, const double* a, const int* bc, double* res,
+                                double* upd, hipStream_t st);
+hipError_t launch_update_field_savedia(int n, int len, int mode, double dt, double gamma, double beta,
+                                       const int* order, const int* bc, double* upd, double* u, double* v, double* a,
+                                       hipStream_t st);
 hipError_t launch_update_field_save(int n, int len, const int* order, const int* bc, double* upd, double* u,
                                     hipStream_t st);
 

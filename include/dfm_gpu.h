@@ -213,12 +213,33 @@ int dfm_sys_set_matrix(dfm_sys sys, const double* crs, const double* dia);
  * src/femeqn/eqn_linear_solid2d.cpp:861-960), DFM_SOLID3D (AddLinSys_LinearSolid3D_Static_SaveStiffMat,
  * src/femeqn/eqn_linear_solid3d.cpp:102-126, :425-521). Set the Dirichlet flags first. */
 int dfm_sys_assemble_save(dfm_sys sys, int kind, const double* params, int npar, int pstride);
-/* CLinearSystem_Save::MakeResidual (src/femls/linearsystem_fieldsave.cpp:206-247): the update handle takes the
- * field values, the residual handle r = f - K_b * u (the same terms in the same order as the reference), and
- * ||r|| is returned in *res_norm (may be NULL). Re-solving after new Dirichlet values needs only this call. */
+/* The saved systems with a lumped mass (src/femls/linearsystem_fieldsave.cpp:363-870), InitializeMarge + the
+ * element loop + FinalizeMarge:
+ *  kind DFM_DIFFUSION2D, CLinearSystem_SaveDiaM_Newmark (AddLinearSystem_Diffusion2D_P1,
+ *    src/femeqn/eqn_diffusion.cpp:565-650), params (rho, alpha, source), glob = (dt, gamma): matrix
+ *    gamma dt C + M (M lumped, rho area / 3), boundary matrix C, force the source term; the unknown is the new
+ *    velocity;
+ *  kind DFM_SOLID2D, CLinearSystem_SaveDiaM_NewmarkBeta (AddLinSys_LinearSolid2D_NonStatic_Save_NewmarkBeta_P1,
+ *    src/femeqn/eqn_linear_solid2d.cpp:1051-1180), params (lambda, myu, rho, gx, gy), glob = (dt, gamma, beta):
+ *    matrix M + beta dt^2 K, boundary matrix K, force the gravity (both of its components added to a node's first
+ *    dof, as the reference adds them); the unknown is the new acceleration.
+ * The boundary matrix is the whole matrix pattern with the diagonal (CMat_BlkCrs::SetPatternDia,
+ * src/matvec/mat_blkcrs.cpp:1267-1319), its fixed rows zeroed. Triangles only. Set the Dirichlet flags first; the
+ * residual handle is left zero. dfm_sys_make_residual and dfm_sys_update_field_save then follow these classes. */
+int dfm_sys_assemble_save_dia(dfm_sys sys, int kind, const double* params, int npar, int pstride,
+                              const double* glob);
+/* MakeResidual of the saved systems: the update handle takes w, the residual handle r = f - K_b * w (the same
+ * terms in the same order as the reference), and ||r|| is returned in *res_norm (may be NULL). w is the field
+ * values for CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp:206-247); u + dt (1 - gamma) v for
+ * SaveDiaM_Newmark (:425-472), with the residual zeroed at fixed dofs; u + dt v + dt^2 (0.5 - beta) a for
+ * SaveDiaM_NewmarkBeta (:697-758), likewise. Re-solving after new Dirichlet values needs only this call. */
 int dfm_sys_make_residual(dfm_sys sys, double* res_norm);
-/* CLinearSystem_Save::UpdateValueOfField (VALUE, :249-332): the update takes the field values at fixed dofs,
- * then the field is set to the update (u = update, not u += update). */
+/* UpdateValueOfField of the saved systems: CLinearSystem_Save (VALUE, :249-332): the update takes the field
+ * values at fixed dofs, then the field is set to the update (u = update, not u += update). SaveDiaM_Newmark
+ * (VELOCITY, :474-520): u += dt (1 - gamma) v + dt gamma update at free dofs, then v = update (fixed dofs keep
+ * their velocity). SaveDiaM_NewmarkBeta (ACCELERATION, :760-860): u += dt v + dt^2 (0.5 - beta) a +
+ * dt^2 beta update, v += dt (1 - gamma) a + gamma dt update at free dofs, then a = update (fixed dofs keep
+ * their acceleration). */
 int dfm_sys_update_field_save(dfm_sys sys);
 /* The saved external force f (original numbering, n*len) and the number of scalar terms of K_b. */
 int dfm_sys_get_force(dfm_sys sys, double* f);

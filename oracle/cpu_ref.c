@@ -1578,33 +1578,38 @@ void cr_boundary_bc(int n, int len, const unsigned* browptr, const unsigned* bco
     }
 }
 
-double cr_make_residual_save(int n, int len, const unsigned* browptr, const unsigned* bcolind, const double* bval,
-                             const double* force, const double* u, double* res, double* upd) {
+/* CMat_BlkCrs::MatVec(-1, x, 1, res) (src/matvec/mat_blkcrs.cpp:955-1170) over a boundary matrix: 1x1 blocks
+ * (:1007-1020), otherwise the general branch (:1150-1168) */
+static void boundary_matvec(int n, int len, const unsigned* browptr, const unsigned* bcolind, const double* bval,
+                            const double* x, double* res) {
   const double alpha = -1.0, beta = 1.0;
-  const long m = (long)n * len;
   const int bs = len * len;
-  /* update = field values; residual = force (MakeResidual, src/femls/linearsystem_fieldsave.cpp:206-247) */
-  for (long k = 0; k < m; k++) upd[k] = u[k];
-  for (long k = 0; k < m; k++) res[k] = force[k];
-  /* CMat_BlkCrs::MatVec(alpha, update, beta, residual) (:955-1170): 1x1 blocks (:1007-1020), otherwise the
-   * general branch (:1150-1168) */
   if (len == 1) {
     for (long i = 0; i < n; i++) {
       double* lval = &res[i];
       *lval *= beta;
-      for (unsigned q = browptr[i]; q < browptr[i + 1]; q++) *lval += alpha * bval[q] * upd[bcolind[q]];
+      for (unsigned q = browptr[i]; q < browptr[i + 1]; q++) *lval += alpha * bval[q] * x[bcolind[q]];
     }
   } else {
     for (long i = 0; i < n; i++) {
       double* iy = &res[i * len];
       for (int id = 0; id < len; id++) iy[id] *= beta;
       for (unsigned q = browptr[i]; q < browptr[i + 1]; q++) {
-        const double* jx = &upd[(long)bcolind[q] * len];
+        const double* jx = &x[(long)bcolind[q] * len];
         for (int id = 0; id < len; id++)
           for (int jd = 0; jd < len; jd++) iy[id] += alpha * bval[(long)q * bs + id * len + jd] * jx[jd];
       }
     }
   }
+}
+
+double cr_make_residual_save(int n, int len, const unsigned* browptr, const unsigned* bcolind, const double* bval,
+                             const double* force, const double* u, double* res, double* upd) {
+  const long m = (long)n * len;
+  /* update = field values; residual = force (MakeResidual, src/femls/linearsystem_fieldsave.cpp:206-247) */
+  for (long k = 0; k < m; k++) upd[k] = u[k];
+  for (long k = 0; k < m; k++) res[k] = force[k];
+  boundary_matvec(n, len, browptr, bcolind, bval, upd, res);
   /* sqrt(DOT(-1,-1)) */
   double sq = 0.0;
   for (long k = 0; k < m; k++) sq += res[k] * res[k];
@@ -1617,6 +1622,199 @@ void cr_update_field_save(int n, int len, const int* flags, double* upd, double*
   for (long k = 0; k < m; k++)
     if (flags[k] != 0) upd[k] = u[k];
   for (long k = 0; k < m; k++) u[k] = upd[k];
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Saved stiffness with a lumped mass: CLinearSystem_SaveDiaM_Newmark / _NewmarkBeta                     */
+/* (src/femls/linearsystem_fieldsave.cpp:363-870), restated step by step.                                */
+
+long cr_pattern_dia(int n, const unsigned* rowptr, const unsigned* colind, unsigned* browptr, unsigned* bcolind) {
+  /* CMat_BlkCrs::SetPatternDia (src/matvec/mat_blkcrs.cpp:1267-1319): the row of the matrix pattern with the
+   * diagonal placed just before the first column below it; last when no column is below it */
+  long nb = 0;
+  if (browptr) browptr[0] = 0;
+  for (long i = 0; i < n; i++) {
+    int dia_done = 0;
+    for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+      const unsigned j = colind[q];
+      if ((unsigned)i > j && !dia_done) {
+        if (bcolind) bcolind[nb] = (unsigned)i;
+        nb++;
+        dia_done = 1;
+      }
+      if (bcolind) bcolind[nb] = j;
+      nb++;
+    }
+    if (!dia_done) {
+      if (bcolind) bcolind[nb] = (unsigned)i;
+      nb++;
+    }
+    if (browptr) browptr[i + 1] = (unsigned)nb;
+  }
+  return nb;
+}
+
+/* AddLinearSystem_Diffusion2D_P1 on CLinearSystem_SaveDiaM_Newmark, src/femeqn/eqn_diffusion.cpp:565-650:
+ * eC (the boundary matrix's element), emat = (gamma dt) eC + the lumped mass rho area / 3 on the diagonal, and the
+ * source term with its 14-digit literal. p = (rho, alpha, source) */
+static void elem_diffusion2d_savedia(const double co[][3], const double* p, double dt, double gamma, double* emat,
+                                     double* ec, double* eforce) {
+  const double rho = p[0], alpha = p[1], source = p[2];
+  const double area = tri_area(co[0], co[1], co[2]);
+  double dldx[3][2];
+  tri_dldx(dldx, co[0], co[1], co[2]);
+  double em[3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) ec[i * 3 + j] = alpha * area * (dldx[i][0] * dldx[j][0] + dldx[i][1] * dldx[j][1]);
+  {
+    const double dtmp1 = rho * area / 3.0;
+    for (int i = 0; i < 3; i++) em[i] = dtmp1;
+  }
+  for (int i = 0; i < 3; i++) eforce[i] = source * area * 0.33333333333333;
+  {
+    const double dtmp1 = gamma * dt;
+    for (int i = 0; i < 3; i++) {
+      for (int j = 0; j < 3; j++) emat[i * 3 + j] = dtmp1 * ec[i * 3 + j];
+      emat[i * 3 + i] += em[i];
+    }
+  }
+}
+
+/* AddLinSys_LinearSolid2D_NonStatic_Save_NewmarkBeta_P1, src/femeqn/eqn_linear_solid2d.cpp:1051-1180: eK (the
+ * boundary matrix's element, written as the static saved loop writes it), the lumped mass area rho / 3 on the
+ * diagonal of the diagonal blocks, emat = M + (beta dt dt) K on the diagonal blocks and (beta dt dt) K elsewhere,
+ * and the gravity force of each node as two values (x, then y) that the reference adds to the node's first dof.
+ * p = (lambda, myu, rho, gx, gy); eforce[2 a + d] is the value added d-th. */
+static void elem_solid2d_savedia(const double co[][3], const double* p, double dt, double beta, double* emat,
+                                 double* ek, double* eforce) {
+  const double rho = p[2], g_x = p[3], g_y = p[4];
+  double em[3][4], dummy[6];
+  elem_solid_save(2, co, p, ek, dummy);
+  const double area = tri_area(co[0], co[1], co[2]);
+  for (int a = 0; a < 3; a++)
+    for (int k = 0; k < 4; k++) em[a][k] = 0.0;
+  {
+    const double dtmp1 = area * rho * 0.33333333333333333333333;
+    for (int a = 0; a < 3; a++) {
+      em[a][0] += dtmp1;
+      em[a][3] += dtmp1;
+    }
+  }
+  for (int a = 0; a < 3; a++) {
+    eforce[a * 2 + 0] = area * rho * g_x * 0.33333333333333333333333333;
+    eforce[a * 2 + 1] = area * rho * g_y * 0.33333333333333333333333333;
+  }
+  const double dtmp1 = beta * dt * dt;
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++)
+      for (int k = 0; k < 4; k++) {
+        const double kv = ek[(a * 3 + c) * 4 + k];
+        emat[(a * 3 + c) * 4 + k] = (a == c) ? em[a][k] + dtmp1 * kv : dtmp1 * kv;
+      }
+}
+
+int cr_assemble_savedia(int kind, int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                        int pstride, const double* glob, int len, const unsigned* rowptr, const unsigned* colind,
+                        double* crs, double* dia, const unsigned* browptr, const unsigned* bcolind, double* bval,
+                        double* force) {
+  (void)nnode;
+  const double dt = glob[0], gamma = glob[1], beta = glob[2];
+  const int bs = len * len;
+  double emat[36], ek[36], eforce[6], co[4][3];
+  if (!((kind == CR_DIFFUSION2D && len == 1) || (kind == CR_SOLID2D && len == 2))) return -1;
+  for (long e = 0; e < nelem; e++) {
+    const int* no = &lnods[e * 3];
+    const double* p = &params[e * pstride];
+    for (int a = 0; a < 3; a++)
+      for (int d = 0; d < 3; d++) co[a][d] = (d < 2) ? coords[(long)no[a] * 2 + d] : 0.0;
+    if (kind == CR_DIFFUSION2D) elem_diffusion2d_savedia(co, p, dt, gamma, emat, ek, eforce);
+    else elem_solid2d_savedia(co, p, dt, beta, emat, ek, eforce);
+    merge(3, no, bs, emat, rowptr, colind, crs, dia);
+    merge_crs(3, no, bs, ek, browptr, bcolind, bval);
+    if (kind == CR_DIFFUSION2D) {
+      for (int a = 0; a < 3; a++) force[no[a]] += eforce[a];
+    } else {
+      /* force_c.AddValue(noes[ino], 0, eqf_out[ino][idim]) for idim = 0, 1 */
+      for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 2; d++) force[(long)no[a] * 2] += eforce[a * 2 + d];
+    }
+  }
+  return 0;
+}
+
+void cr_boundary_rows(int n, int len, const unsigned* browptr, double* bval, const int* flags) {
+  /* SetBoundaryCondition_Row (src/matvec/mat_blkcrs.cpp:910-953): rows of fixed dofs to zero */
+  const int bs = len * len;
+  for (long i = 0; i < n; i++)
+    for (int id = 0; id < len; id++) {
+      if (flags[i * len + id] == 0) continue;
+      for (unsigned q = browptr[i]; q < browptr[i + 1]; q++)
+        for (int jd = 0; jd < len; jd++) bval[(long)q * bs + id * len + jd] = 0.0;
+    }
+}
+
+double cr_make_residual_savedia(int kind, int n, int len, const unsigned* browptr, const unsigned* bcolind,
+                                const double* bval, const double* force, const double* glob, const int* flags,
+                                const double* u, const double* v, const double* a, double* res, double* upd) {
+  const double dt = glob[0], gamma = glob[1], beta = glob[2];
+  const long m = (long)n * len;
+  /* residual = force; update = u + (dt (1 - gamma)) v (Newmark, :433-457), or u + dt v + (dt dt (0.5 - beta)) a
+   * (NewmarkBeta, :705-737), each term added by CNodeAry::AddValueFromNodeSegment (value * alpha,
+   * src/femfield/node_ary.cpp:302-325) */
+  for (long k = 0; k < m; k++) res[k] = force[k];
+  for (long k = 0; k < m; k++) upd[k] = u[k];
+  if (kind == CR_DIFFUSION2D) {
+    const double c1 = dt * (1 - gamma);
+    for (long k = 0; k < m; k++) upd[k] += v[k] * c1;
+  } else {
+    const double c2 = dt * dt * (0.5 - beta);
+    for (long k = 0; k < m; k++) upd[k] += v[k] * dt;
+    for (long k = 0; k < m; k++) upd[k] += a[k] * c2;
+  }
+  /* the boundary matrix's MatVec(-1, update, 1, residual) (src/matvec/mat_blkcrs.cpp:955-1170) */
+  boundary_matvec(n, len, browptr, bcolind, bval, upd, res);
+  /* SetZeroToBCDof of the residual, then sqrt(DOT(-1,-1)) */
+  for (long k = 0; k < m; k++)
+    if (flags[k] != 0) res[k] = 0.0;
+  double sq = 0.0;
+  for (long k = 0; k < m; k++) sq += res[k] * res[k];
+  return sqrt(sq);
+}
+
+void cr_update_field_savedia(int kind, int n, int len, const double* glob, const int* flags, double* upd, double* u,
+                             double* v, double* a) {
+  const double dt = glob[0], gamma = glob[1], beta = glob[2];
+  const long m = (long)n * len;
+  if (kind == CR_DIFFUSION2D) {
+    /* CLinearSystem_SaveDiaM_Newmark::UpdateValueOfField (:474-520): free dofs u += dt (1 - gamma) v0 +
+     * dt gamma v1 (v1: the update); fixed dofs keep their velocity in the update; then velocity = update */
+    for (long k = 0; k < m; k++) {
+      const double velo0 = v[k];
+      if (flags[k] == 0) {
+        const double velo1 = upd[k];
+        u[k] += dt * (1 - gamma) * velo0 + dt * gamma * velo1;
+      } else {
+        upd[k] = velo0;
+      }
+    }
+    for (long k = 0; k < m; k++) v[k] = upd[k];
+  } else {
+    /* CLinearSystem_SaveDiaM_NewmarkBeta::UpdateValueOfField (:760-860), ACCELERATION: free dofs
+     * u += dt v0 + dt dt (0.5 - beta) a0 + dt dt beta a1, v += dt (1 - gamma) a0 + gamma dt a1 (a1: the update);
+     * fixed dofs keep their acceleration in the update; then acceleration = update */
+    for (long k = 0; k < m; k++) {
+      const double velo0 = v[k], acc0 = a[k];
+      if (flags[k] == 0) {
+        const double acc1 = upd[k];
+        u[k] += dt * velo0 + dt * dt * (0.5 - beta) * acc0 + dt * dt * beta * acc1;
+        v[k] += dt * (1 - gamma) * acc0 + gamma * dt * acc1;
+        a[k] += acc1;
+      } else {
+        upd[k] = acc0;
+      }
+    }
+    for (long k = 0; k < m; k++) a[k] = upd[k];
+  }
 }
 
 /* ---------------------------------------------------------------------------------------------------- */

@@ -111,6 +111,26 @@ double cr_make_residual_save(int n, int len, const unsigned* browptr, const unsi
                              const double* force, const double* u, double* res, double* upd);
 void cr_update_field_save(int n, int len, const int* flags, double* upd, double* u);
 
+/* Saved stiffness with a lumped mass (src/femls/linearsystem_fieldsave.cpp:363-870): CLinearSystem_SaveDiaM_Newmark
+ * (kind CR_DIFFUSION2D, the element loop AddLinearSystem_Diffusion2D_P1 of src/femeqn/eqn_diffusion.cpp:565-650)
+ * and CLinearSystem_SaveDiaM_NewmarkBeta (kind CR_SOLID2D, AddLinSys_LinearSolid2D_NonStatic_Save_NewmarkBeta_P1 of
+ * src/femeqn/eqn_linear_solid2d.cpp:1051-1180). Boundary matrix pattern (CMat_BlkCrs::SetPatternDia: the whole
+ * row with its diagonal; returns the block count, fills browptr/bcolind when non-NULL); the element loop (matrix,
+ * boundary matrix, force; glob = (dt, gamma, beta)); the boundary matrix's fixed rows zeroed (FinalizeMarge);
+ * MakeResidual (u, v, a: VALUE, VELOCITY, ACCELERATION; returns ||res||); UpdateValueOfField (VELOCITY for
+ * CR_DIFFUSION2D, ACCELERATION for CR_SOLID2D). */
+long cr_pattern_dia(int n, const unsigned* rowptr, const unsigned* colind, unsigned* browptr, unsigned* bcolind);
+int cr_assemble_savedia(int kind, int nnode, const double* coords, int nelem, const int* lnods, const double* params,
+                        int pstride, const double* glob, int len, const unsigned* rowptr, const unsigned* colind,
+                        double* crs, double* dia, const unsigned* browptr, const unsigned* bcolind, double* bval,
+                        double* force);
+void cr_boundary_rows(int n, int len, const unsigned* browptr, double* bval, const int* flags);
+double cr_make_residual_savedia(int kind, int n, int len, const unsigned* browptr, const unsigned* bcolind,
+                                const double* bval, const double* force, const double* glob, const int* flags,
+                                const double* u, const double* v, const double* a, double* res, double* upd);
+void cr_update_field_savedia(int kind, int n, int len, const double* glob, const int* flags, double* upd, double* u,
+                             double* v, double* a);
+
 /* Newmark-beta dynamic linear elasticity (2D and 3D P1): element loops into matrix and residual (u, v, a the
  * VALUE, VELOCITY and ACCELERATION fields; glob = (dt, gamma, beta, is_initial)), and UpdateValueOfField_NewmarkBeta
  * over m values. */

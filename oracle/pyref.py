@@ -40,6 +40,8 @@ def cpu_ref():
         _cr.cr_ordering.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _cr.cr_boundary_pattern.restype = C.c_long
         _cr.cr_make_residual_save.restype = C.c_double
+        _cr.cr_pattern_dia.restype = C.c_long
+        _cr.cr_make_residual_savedia.restype = C.c_double
     return _cr
 
 
@@ -147,6 +149,55 @@ class CpuSystem:
         p = lambda a: a.ctypes.data_as(C.c_void_p)
         cpu_ref().cr_update_field_save(self.nnode, self.len, p(self.flags), p(upd), p(u))
         return upd, u
+
+    def assemble_savedia(self, kind, params, flags, glob):
+        """CLinearSystem_SaveDiaM_Newmark (kind DIFFUSION2D, params (rho, alpha, source)) or _NewmarkBeta (kind
+        SOLID2D, (lambda, myu, rho, gx, gy)), restated: the element loop into the matrix, the boundary matrix
+        (SetPatternDia: self.brp / self.bci / self.bval) and the force (self.force); FinalizeMarge: the matrix's
+        Dirichlet treatment and the boundary matrix's fixed rows zeroed. glob = (dt, gamma, beta)."""
+        params = _f64(params)
+        pstride = 0 if params.ndim == 1 else params.shape[1]
+        flags, glob = _i32(flags), _f64(glob)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        lib = cpu_ref()
+        n, bs = self.nnode, self.len * self.len
+        nb = lib.cr_pattern_dia(n, p(self.rowptr), p(self.colind), None, None)
+        self.brp = np.zeros(n + 1, np.uint32)
+        self.bci = np.zeros(nb, np.uint32)
+        lib.cr_pattern_dia(n, p(self.rowptr), p(self.colind), p(self.brp), p(self.bci))
+        self.bval = np.zeros(nb * bs)
+        self.force = np.zeros(n * self.len)
+        self.zero()
+        rc = lib.cr_assemble_savedia(kind, n, p(self.coords), self.nelem, p(self.lnods), p(params), pstride, p(glob),
+                                     self.len, p(self.rowptr), p(self.colind), p(self.crs), p(self.dia), p(self.brp),
+                                     p(self.bci), p(self.bval), p(self.force))
+        assert rc == 0, rc
+        self.set_bc(flags)
+        lib.cr_boundary_rows(n, self.len, p(self.brp), p(self.bval), p(flags))
+        self.flags, self.savedia = flags, (kind, glob)
+
+    def make_residual_savedia(self, u, v, a=None):
+        """MakeResidual of the lumped-mass saved systems (restated): returns (||r||, residual, update); the residual
+        also becomes self.res."""
+        kind, glob = self.savedia
+        u, v = _f64(u), _f64(v)
+        a = np.zeros_like(u) if a is None else _f64(a)
+        res, upd = np.zeros_like(u), np.zeros_like(u)
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        norm = cpu_ref().cr_make_residual_savedia(kind, self.nnode, self.len, p(self.brp), p(self.bci), p(self.bval),
+                                                  p(self.force), p(glob), p(self.flags), p(u), p(v), p(a), p(res),
+                                                  p(upd))
+        self.res = res.copy()
+        return norm, res, upd
+
+    def update_field_savedia(self, upd, u, v, a=None):
+        """UpdateValueOfField of the lumped-mass saved systems (restated): returns (update, u, v, a)."""
+        kind, glob = self.savedia
+        upd, u, v = _f64(upd).copy(), _f64(u).copy(), _f64(v).copy()
+        a = np.zeros_like(u) if a is None else _f64(a).copy()
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        cpu_ref().cr_update_field_savedia(kind, self.nnode, self.len, p(glob), p(self.flags), p(upd), p(u), p(v), p(a))
+        return upd, u, v, a
 
     def assemble_newmark(self, params, u, v, a, glob):
         """Newmark-beta element loop (restated), 2D or 3D by the mesh: matrix and residual accumulate into
@@ -324,7 +375,7 @@ def reference():
 class RefSystem:
     """The reference DelFEM driven through its own API (oracle/ref_capi.cpp)."""
 
-    def __init__(self, ndim, coords, lnods, len_, groups=None, save=False, dynamic=False):
+    def __init__(self, ndim, coords, lnods, len_, groups=None, save=False, dynamic=False, savedia=False):
         self.ndim, self.len = ndim, len_
         self.coords = _f64(coords)
         self.lnods = _i32(lnods)
@@ -337,7 +388,8 @@ class RefSystem:
         lib = reference()
         p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
         self.h = C.c_void_p(lib.ref_sys_create2(ndim, self.nnode, p(self.coords), self.nelem, p(self.lnods), ngroup,
-                                                p(self.groups), len_, (1 if save else 0) | (2 if dynamic else 0)))
+                                                p(self.groups), len_,
+                                                (1 if save else 0) | (2 if dynamic else 0) | (4 if savedia else 0)))
         self.ncrs = lib.ref_sys_ncrs(self.h)
 
     def __del__(self):
@@ -503,6 +555,33 @@ class RefSystem:
 
     def update_field_save(self):
         reference().ref_sys_update_field_save(self.h)
+
+    def assemble_savedia(self, kind, group_params, glob):
+        """CLinearSystem_SaveDiaM_Newmark / _NewmarkBeta (system made with savedia): InitializeMarge,
+        SetNewmarkParameter, the element loop per group, FinalizeMarge. glob = (dt, gamma, beta)."""
+        gp = _f64(group_params)
+        if gp.ndim == 1:
+            gp = gp[None, :]
+        assert gp.shape[0] == self.ngroup
+        dt, gamma, beta = (float(x) for x in glob)
+        rc = reference().ref_sys_assemble_savedia(self.h, kind, gp.ctypes.data_as(C.c_void_p), gp.shape[1],
+                                                  C.c_double(dt), C.c_double(gamma), C.c_double(beta))
+        assert rc == 0, rc
+
+    def update_field_savedia(self):
+        reference().ref_sys_update_field_savedia(self.h)
+
+    def boundary(self):
+        """The boundary matrix: (browptr, bcolind, values)."""
+        lib = reference()
+        lib.ref_sys_get_boundary.restype = C.c_long
+        nb = lib.ref_sys_get_boundary(self.h, None, None, None)
+        rp = np.zeros(self.nnode + 1, np.uint32)
+        ci = np.zeros(max(nb, 1), np.uint32)
+        val = np.zeros(max(nb, 1) * self.len * self.len)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        lib.ref_sys_get_boundary(self.h, p(rp), p(ci), p(val))
+        return rp, ci[:nb], val[: nb * self.len * self.len]
 
 
 # ---------------------------------------------------------------------------------------------------------

@@ -70,6 +70,7 @@ struct RefSys {
   Fem::Field::CFieldWorld world;
   Fem::Ls::CLinearSystem_Field* ls = 0;  // a CLinearSystem_Save for the saved-stiffness path
   bool save = false;
+  bool savedia = false;  // CLinearSystem_SaveDiaM_Newmark (scalar field) or _NewmarkBeta (vector field)
   MatVec::CVector_Blk* velo_pre = 0;  // the backward-Euler loop's previous velocity
   ~RefSys() { delete ls; delete velo_pre; }
   CGroupMesh mesh;
@@ -87,13 +88,17 @@ extern "C" {
 
 // kind of field: len = 1 (scalar) or ndim (vector). Element groups are 0..ngroup-1 per element.
 // mode bit 0: the linear system is a CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp); bit 1: the field
-// also has ACCELERATION values (Newmark). Without bit 1 the field is the one the other drivers use (VALUE |
+// also has ACCELERATION values (Newmark); bit 2: the linear system is a CLinearSystem_SaveDiaM_Newmark (len 1) or
+// CLinearSystem_SaveDiaM_NewmarkBeta (len 2; set bit 1 too). Without bit 1 the field is the one the other drivers use (VALUE |
 // VELOCITY): the reference's matrix-level loops read uninitialised heap memory, whose contents depend on it.
 void* ref_sys_create2(int ndim, int nnode, const double* coords, int nelem, const int* lnods,
                       int ngroup, const int* elem_group, int len, int mode) {
   RefSys* s = new RefSys;
-  s->save = (mode & 1) != 0;
-  if (s->save) s->ls = new Fem::Ls::CLinearSystem_Save;
+  s->save = (mode & 5) != 0;
+  s->savedia = (mode & 4) != 0;
+  if (s->savedia && len == 1) s->ls = new Fem::Ls::CLinearSystem_SaveDiaM_Newmark;
+  else if (s->savedia) s->ls = new Fem::Ls::CLinearSystem_SaveDiaM_NewmarkBeta;
+  else if (s->save) s->ls = new Fem::Ls::CLinearSystem_Save;
   else s->ls = new Fem::Ls::CLinearSystem_Field;
   s->ndim = ndim; s->nnode = nnode; s->len = len;
   const int npoel = (ndim == 2) ? 3 : 4;
@@ -529,6 +534,75 @@ void ref_sys_get_force(void* h, double* f) {
 void ref_sys_update_field_save(void* h) {
   RefSys* s = (RefSys*)h;
   ((Fem::Ls::CLinearSystem_Save*)s->ls)->UpdateValueOfField(s->id_field, s->world, Fem::Field::VALUE);
+}
+
+// The lumped-mass saved systems on a system made with mode bit 2: InitializeMarge, SetNewmarkParameter, the element
+// loop of each group, FinalizeMarge. kind 5: AddLinSys_Diffusion on CLinearSystem_SaveDiaM_Newmark
+// (src/femeqn/eqn_diffusion.cpp:769-805), p = (rho, alpha, source); kind 1:
+// AddLinSys_LinearSolid2D_NonStatic_Save_NewmarkBeta (src/femeqn/eqn_linear_solid2d.cpp:313-353),
+// p = (lambda, myu, rho, gx, gy).
+int ref_sys_assemble_savedia(void* h, int kind, const double* params, int npar, double dt, double gamma,
+                             double beta) {
+  RefSys* s = (RefSys*)h;
+  if (!s->savedia) return -3;
+  const int ngroup = (int)s->id_ea.size();
+  if (kind == 5 && s->len == 1) {
+    Fem::Ls::CLinearSystem_SaveDiaM_Newmark& ls = *(Fem::Ls::CLinearSystem_SaveDiaM_Newmark*)s->ls;
+    ls.InitializeMarge();
+    ls.SetNewmarkParameter(gamma, dt);
+    for (int g = 0; g < ngroup; g++) {
+      const double* p = &params[(size_t)g * npar];
+      if (!Fem::Eqn::AddLinSys_Diffusion(ls, p[0], p[1], p[2], s->world, s->id_field, s->id_ea[g])) return -2;
+    }
+    ls.FinalizeMarge();
+    return 0;
+  }
+  if (kind == 1 && s->len == 2) {
+    Fem::Ls::CLinearSystem_SaveDiaM_NewmarkBeta& ls = *(Fem::Ls::CLinearSystem_SaveDiaM_NewmarkBeta*)s->ls;
+    ls.InitializeMarge();
+    ls.SetNewmarkParameter(beta, gamma, dt);
+    for (int g = 0; g < ngroup; g++) {
+      const double* p = &params[(size_t)g * npar];
+      if (!Fem::Eqn::AddLinSys_LinearSolid2D_NonStatic_Save_NewmarkBeta(ls, p[0], p[1], p[2], p[3], p[4], s->world,
+                                                                       s->id_field, s->id_ea[g]))
+        return -2;
+    }
+    ls.FinalizeMarge();
+    return 0;
+  }
+  return -1;
+}
+
+// UpdateValueOfField of the lumped-mass saved systems: VELOCITY (SaveDiaM_Newmark) or ACCELERATION (_NewmarkBeta)
+void ref_sys_update_field_savedia(void* h) {
+  RefSys* s = (RefSys*)h;
+  s->ls->UpdateValueOfField(s->id_field, s->world,
+                            s->len == 1 ? Fem::Field::VELOCITY : Fem::Field::ACCELERATION);
+}
+
+// The saved system's boundary matrix (GetMatrix_Boundary of the field's corner segment): its pattern (browptr
+// [n+1], bcolind) and values; returns its block count (call with NULL arrays for the count alone).
+long ref_sys_get_boundary(void* h, unsigned* browptr, unsigned* bcolind, double* bval) {
+  RefSys* s = (RefSys*)h;
+  MatVec::CMat_BlkCrs& b = ((Fem::Ls::CLinearSystem_Save*)s->ls)
+                               ->GetMatrix_Boundary(s->id_field, Fem::Field::CORNER, s->id_field, Fem::Field::CORNER,
+                                                    s->world);
+  const unsigned bs = (unsigned)(s->len * s->len);
+  long nb = 0;
+  if (browptr) browptr[0] = 0;
+  for (int i = 0; i < s->nnode; i++) {
+    unsigned np = 0;
+    const unsigned* ind = b.GetPtrIndPSuP((unsigned)i, np);
+    const double* val = b.GetPtrValPSuP((unsigned)i, np);
+    for (unsigned k = 0; k < np; k++) {
+      if (bcolind) bcolind[nb + k] = ind[k];
+      if (bval)
+        for (unsigned t = 0; t < bs; t++) bval[(size_t)(nb + k) * bs + t] = val[(size_t)k * bs + t];
+    }
+    nb += np;
+    if (browptr) browptr[i + 1] = (unsigned)nb;
+  }
+  return nb;
 }
 
 }  // extern "C"

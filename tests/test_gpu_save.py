@@ -217,3 +217,111 @@ def test_saved_errors(ctx):
     g2.assemble(1, [LAM2, MU2, 1.0, 0.0, -0.01])  # a plain assembly ends the saved state
     with pytest.raises(N.DfmError):
         g2.make_residual()
+
+
+# ---- lumped-mass saved systems (CLinearSystem_SaveDiaM_Newmark / _NewmarkBeta) --------------------------------
+SAVEDIA = [
+    # (name, len, kind, params per element, glob)
+    ("diffusion2d", 1, 5, lambda r, n: np.stack([r.uniform(0.5, 2, n), r.uniform(0.5, 2, n), np.full(n, 0.3)], 1),
+     (0.05, 0.6, 0.0)),
+    ("solid2d", 2, 1, lambda r, n: np.stack([LAM2 * r.uniform(0.5, 2, n), np.full(n, MU2), r.uniform(0.5, 2, n),
+                                             np.full(n, 0.02), np.full(n, -0.01)], 1), (0.05, 0.6, 0.3025)),
+]
+
+
+def build_savedia(ctx, case, ordering, small=True):
+    name, length, kind, pfun, glob = case
+    co, ln = mesh(2, small)
+    n = co.size // 2
+    params = pfun(np.random.default_rng(19), ln.size // 3)
+    fl = save_flags(co, 2, length)
+    u = save_field(n, length, fl, 29)
+    rng = np.random.default_rng(31)
+    v = rng.uniform(-0.1, 0.1, n * length)
+    a = rng.uniform(-0.2, 0.2, n * length) if length == 2 else None
+    cpu = P.CpuSystem(2, co, ln, length)
+    cpu.assemble_savedia(kind, params, fl, glob)
+    gs = N.System(ctx, 2, co, ln, length)
+    order = None
+    if ordering != 0:
+        order, _ = gs.make_ordering(ordering)
+        gs.set_ordering(order)
+    gs.set_bc(fl)
+    gs.set_field(u, v)
+    if a is not None:
+        gs.set_acceleration(a)
+    gs.assemble_savedia(kind, params, glob)
+    return cpu, gs, (u, v, a), fl, order, kind
+
+
+@pytest.mark.parametrize("case", SAVEDIA, ids=[c[0] for c in SAVEDIA])
+@pytest.mark.parametrize("oname,ordering", ORDERS, ids=[o[0] for o in ORDERS])
+def test_savedia_assembly_and_residual_bit_exact(ctx, case, oname, ordering):
+    """Matrix (mass + time-scaled stiffness), force, MakeResidual's residual (zero at fixed dofs) and update (the
+    Newmark predictor) bit for bit under every device numbering; one boundary term per column of every block of
+    the row and the diagonal block, per free dof."""
+    print(f"lumped-mass saved system {case[0]} {oname}", flush=True)
+    cpu, gs, (u, v, a), fl, _, _ = build_savedia(ctx, case, ordering)
+    crs, dia = gs.get_matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    assert np.array_equal(gs.get_force(), cpu.force)
+    assert not gs.get_vector(-1).any()
+    norm, res, upd = cpu.make_residual_savedia(u, v, a)
+    gn = gs.make_residual()
+    assert np.array_equal(gs.get_vector(-1), res), np.abs(gs.get_vector(-1) - res).max()
+    assert np.array_equal(gs.get_vector(-2), upd)
+    assert abs(gn - norm) <= 1e-14 * norm
+    b = cpu.len
+    nfree_rows = sum(int((fl[i * b:(i + 1) * b] == 0).sum()) * (int(cpu.rowptr[i + 1] - cpu.rowptr[i]) + 1)
+                     for i in range(cpu.nnode))
+    assert gs.boundary_terms() == nfree_rows * b
+
+
+@pytest.mark.parametrize("case", SAVEDIA, ids=[c[0] for c in SAVEDIA])
+@pytest.mark.parametrize("oname,ordering", ORDERS[:2], ids=[o[0] for o in ORDERS[:2]])
+def test_savedia_time_steps(ctx, case, oname, ordering):
+    """Three time steps of MakeResidual -> PCG (ILU(0) for the diffusion system, ILU(1) for the solid, as
+    CEqnSystem_Scalar2D / CEqnSystem_Solid2D use them) -> UpdateValueOfField on the saved matrix, against the
+    oracle's same steps: solution within the solver tolerance, fixed dofs keep their values (and their velocity or
+    acceleration), residual at each step bit-exact to the oracle's from the GPU's own fields."""
+    print(f"lumped-mass saved time steps {case[0]} {oname}", flush=True)
+    cpu, gs, (u, v, a), fl, order, kind = build_savedia(ctx, case, ordering, small=False)
+    lev = 0 if kind == 5 else 1
+    gs.set_fill_level(lev)
+    gs.factor()
+    f = cpu.ilu(None if ordering == 0 else order, lev)
+    uc, vc, ac = u.copy(), v.copy(), (None if a is None else a.copy())
+    for step in range(3):
+        ug, vg, ag = gs.get_fields()
+        _, res_g, _ = cpu.make_residual_savedia(ug, vg, ag if kind == 1 else None)
+        gs.make_residual()
+        assert np.array_equal(gs.get_vector(-1), res_g)
+        cpu.make_residual_savedia(uc, vc, ac)
+        ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-10, 3000)
+        gok, gconv, git = gs.solve_pcg(1e-10, 3000)
+        assert ok and gok
+        assert abs(git - it) <= 2, (git, it)
+        xg = gs.get_vector(-2)
+        assert np.linalg.norm(xg - x) <= 1e-6 * np.linalg.norm(x)
+        gs.update_field_save()
+        _, uc, vc, ac2 = cpu.update_field_savedia(x, uc, vc, ac)
+        ac = ac2 if kind == 1 else None
+        ug, vg, ag = gs.get_fields()
+        assert np.array_equal(ug[fl == 1], u[fl == 1])
+        if kind == 5:
+            assert np.array_equal(vg[fl == 1], v[fl == 1])
+        else:
+            assert np.array_equal(ag[fl == 1], a[fl == 1])
+        assert np.linalg.norm(ug - uc) <= 1e-6 * np.linalg.norm(uc)
+        assert np.linalg.norm(vg - vc) <= 1e-6 * np.linalg.norm(vc)
+
+
+def test_savedia_rejects_other_kinds(ctx):
+    co, ln = mesh(3, True)
+    gs = N.System(ctx, 3, co, ln, 3)
+    with pytest.raises(N.DfmError):
+        gs.assemble_savedia(2, np.ones(6), (0.1, 0.5, 0.25))
+    co, ln = mesh(2, True)
+    gs = N.System(ctx, 2, co, ln, 1)
+    with pytest.raises(N.DfmError):
+        gs.assemble_savedia(0, np.ones(3), (0.1, 0.5))

@@ -418,6 +418,106 @@ def test_saved_stiffness_solve_and_update_bit_exact(case):
     assert np.array_equal(u2[flags == 1], u[flags == 1])  # Dirichlet values kept
 
 
+# ---- saved stiffness with a lumped mass (CLinearSystem_SaveDiaM_Newmark / _NewmarkBeta) --------------------
+SAVEDIA_CASES = [
+    # (name, len, kind, params, gamma, beta)
+    ("diffusion2d", 1, P.DIFFUSION2D, lambda r: [r.uniform(0.5, 2.0), r.uniform(0.5, 2.0), r.uniform(-1.0, 1.0)],
+     0.6, 0.0),
+    ("solid2d", 2, P.SOLID2D, lambda r: [LAM2 * r.uniform(0.5, 2), MU2, r.uniform(0.5, 2), 0.01, -0.02], 0.6, 0.3025),
+]
+
+
+def _savedia_pair(case, per_element, seed=13):
+    name, length, kind, pfun, gamma, beta = case
+    co, ln = mesh2d(4, 2) if per_element else mesh2d(9, 5)
+    n, nelem = co.size // 2, ln.size // 3
+    rng = np.random.default_rng(seed)
+    if per_element:
+        params = np.array([pfun(rng) for _ in range(nelem)])
+        groups = np.arange(nelem, dtype=np.int32)
+    else:
+        params = np.array(pfun(rng))
+        groups = None
+    ref = P.RefSystem(2, co, ln, length, groups, dynamic=length == 2, savedia=True)
+    cpu = P.CpuSystem(2, co, ln, length)
+    flags = save_flags(co, 2, length)
+    u = save_field(co, 2, length, flags, seed + 1)
+    v = _rand_u(n, length, seed + 2, 0.1)
+    a = _rand_u(n, length, seed + 3, 0.2) if length == 2 else None
+    ref.set_field(u, v)
+    if a is not None:
+        ref.set_acc(a)
+    ref.set_bc(flags)
+    glob = (0.05, gamma, beta)
+    ref.assemble_savedia(kind, params if per_element else params[None, :], glob)
+    cpu.assemble_savedia(kind, params, flags, glob)
+    return ref, cpu, (u, v, a), kind
+
+
+@pytest.mark.parametrize("case", SAVEDIA_CASES, ids=[c[0] for c in SAVEDIA_CASES])
+@pytest.mark.parametrize("per_element", [False, True], ids=["uniform", "per_element"])
+def test_savedia_bit_exact(case, per_element):
+    """Matrix (mass + time-scaled stiffness), force, boundary matrix (SetPatternDia pattern, the stiffness alone,
+    fixed rows zeroed), residual and update of MakeResidual and its norm, element for element."""
+    ref, cpu, (u, v, a), kind = _savedia_pair(case, per_element)
+    crs, dia = ref.matrix()
+    assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
+    assert np.array_equal(ref.force(), cpu.force)
+    if kind == P.SOLID2D:  # both gravity components land on the first dof, as the reference adds them
+        assert not cpu.force[1::2].any() and cpu.force[0::2].any()
+    rp, ci, val = ref.boundary()
+    assert np.array_equal(rp, cpu.brp) and np.array_equal(ci, cpu.bci)
+    assert np.array_equal(val, cpu.bval)
+    rn = ref.make_residual()
+    norm, res, upd = cpu.make_residual_savedia(u, v, a)
+    assert np.array_equal(ref.vector(-1), res), np.abs(ref.vector(-1) - res).max()
+    assert np.array_equal(ref.vector(-2), upd)
+    assert rn == norm
+    assert not res[cpu.flags == 1].any() and res.any()
+
+
+def test_savedia_pattern_places_diagonal():
+    """SetPatternDia: the diagonal before the first column below the row, last in rows with none below."""
+    rp = np.array([0, 2, 4, 5], np.uint32)
+    ci = np.array([1, 2, 0, 2, 1], np.uint32)
+    lib = P.cpu_ref()
+    import ctypes as C
+    brp = np.zeros(4, np.uint32)
+    bci = np.zeros(8, np.uint32)
+    nb = lib.cr_pattern_dia(3, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                            brp.ctypes.data_as(C.c_void_p), bci.ctypes.data_as(C.c_void_p))
+    assert nb == 8
+    assert brp.tolist() == [0, 3, 6, 8]
+    assert bci.tolist() == [1, 2, 0, 1, 0, 2, 2, 1]
+
+
+@pytest.mark.parametrize("case", SAVEDIA_CASES, ids=[c[0] for c in SAVEDIA_CASES])
+def test_savedia_solve_and_update_bit_exact(case):
+    """MakeResidual -> Solve_PCG (ILU(0) for the scalar system, ILU(1) for the solid, as CEqnSystem_Scalar2D and
+    CEqnSystem_Solid2D use them) -> UpdateValueOfField (VELOCITY / ACCELERATION) of the lumped-mass systems,
+    twice in a row (the second step from the first one's fields)."""
+    ref, cpu, (u, v, a), kind = _savedia_pair(case, False)
+    lev = 0 if kind == P.DIFFUSION2D else 1
+    for _ in range(2):
+        ref.make_residual()
+        cpu.make_residual_savedia(u, v, a)
+        f = cpu.ilu(None, lev)
+        ok, conv, it, x, _ = cpu.solve(f, "pcg", 1e-10, 1000)
+        rok, rconv, rit = ref.solve("pcg", 1e-10, 1000, lev, None)
+        assert (ok, conv, it) == (bool(rok), rconv, rit)
+        assert np.array_equal(ref.vector(-2), x)
+        ref.update_field_savedia()
+        upd2, u2, v2, a2 = cpu.update_field_savedia(x, u, v, a)
+        assert np.array_equal(ref.vector(-2), upd2)
+        ru, rv, ra = ref.get_fields(acc=kind == P.SOLID2D)
+        assert np.array_equal(ru, u2) and np.array_equal(rv, v2)
+        if kind == P.SOLID2D:
+            assert np.array_equal(ra, a2)
+        fl = cpu.flags == 1
+        assert np.array_equal(u2[fl], u[fl])  # fixed dofs keep their values
+        u, v, a = u2, v2, (a2 if kind == P.SOLID2D else None)
+
+
 # ---- Newmark-beta dynamic linear elasticity -----------------------------------------------------------------
 NM_CASES = [
     ("solid2d", 2, P.SOLID2D_NEWMARK, lambda r: [LAM2 * r.uniform(0.5, 2), MU2, r.uniform(0.5, 2), 0.0, -0.01]),
