@@ -28,7 +28,7 @@ def _close_all():
         if isinstance(o, (Batch, Server)):
             o.close()
     for o in objs:
-        if isinstance(o, System):
+        if isinstance(o, (System, MultiSegSystem)):
             o.close()
     for o in objs:
         if isinstance(o, Context):
@@ -465,6 +465,103 @@ class System:
         a, b, c = C.c_double(), C.c_double(), C.c_uint()
         _check(lib().dfm_sys_last_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"ms": a.value, "ms_per_iter": b.value, "iters": c.value}
+
+
+class MultiSegSystem:
+    """Multi-segment linear system (dfm_msys_*): CLinearSystem with several segments + CPreconditioner_ILU(0).
+
+    nblk[i], lens[i] per segment; matrices (i, j) from add_pattern / set_matrix; vectors concatenated."""
+
+    def __init__(self, ctx, nblk, lens):
+        self.ctx = ctx
+        self.nblk, self.lens = [int(x) for x in nblk], [int(x) for x in lens]
+        self.nseg = len(self.nblk)
+        self.off = [0]
+        for n, ln in zip(self.nblk, self.lens):
+            self.off.append(self.off[-1] + n * ln)
+        self.m = self.off[-1]
+        nb, ln = _i32(self.nblk), _i32(self.lens)
+        self.h = C.c_void_p()
+        _check(lib().dfm_msys_create(ctx.h, self.nseg, _p(nb), _p(ln), C.byref(self.h)))
+        _live.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().dfm_msys_destroy(self.h))
+            self.h = None
+
+    def add_pattern(self, i, j, rowptr, colind):
+        rowptr = np.ascontiguousarray(rowptr, np.uint32)
+        colind = np.ascontiguousarray(colind, np.uint32)
+        _check(lib().dfm_msys_add_pattern(self.h, i, j, _p(rowptr), _p(colind) if colind.size else None))
+
+    def nnz(self, i, j):
+        n = C.c_longlong()
+        _check(lib().dfm_msys_info(self.h, i, j, C.byref(n)))
+        return n.value
+
+    def set_matrix(self, i, j, val, dia=None):
+        _check(lib().dfm_msys_set_matrix(self.h, i, j, _p(_f64(val)), _p(_f64(dia))))
+
+    def get_matrix(self, i, j):
+        bs = self.lens[i] * self.lens[j]
+        val = np.zeros(max(self.nnz(i, j), 1) * bs)
+        dia = np.zeros(self.nblk[i] * bs) if i == j else None
+        _check(lib().dfm_msys_get_matrix(self.h, i, j, _p(val), _p(dia)))
+        return val[:self.nnz(i, j) * bs], dia
+
+    def get_factor(self, i, j):
+        bs = self.lens[i] * self.lens[j]
+        val = np.zeros(max(self.nnz(i, j), 1) * bs)
+        dia = np.zeros(self.nblk[i] * bs) if i == j else None
+        _check(lib().dfm_msys_get_factor(self.h, i, j, _p(val), _p(dia)))
+        return val[:self.nnz(i, j) * bs], dia
+
+    def set_vector(self, iv, v):
+        _check(lib().dfm_msys_set_vector(self.h, iv, _p(_f64(v))))
+
+    def get_vector(self, iv):
+        v = np.zeros(self.m)
+        _check(lib().dfm_msys_get_vector(self.h, iv, _p(v)))
+        return v
+
+    def finalize(self, flags):
+        nrm = C.c_double()
+        _check(lib().dfm_msys_finalize(self.h, _p(_i32(flags)), C.byref(nrm)))
+        return nrm.value
+
+    def factor(self):
+        _check(lib().dfm_msys_factor(self.h))
+
+    def precond(self, iv):
+        _check(lib().dfm_msys_precond(self.h, iv))
+
+    def resize_tmp(self, n):
+        _check(lib().dfm_msys_resize_tmp(self.h, n))
+
+    def dot(self, a, b):
+        d = C.c_double()
+        _check(lib().dfm_msys_dot(self.h, a, b, C.byref(d)))
+        return d.value
+
+    def matvec(self, alpha, iv1, beta, iv2):
+        _check(lib().dfm_msys_matvec(self.h, C.c_double(alpha), iv1, C.c_double(beta), iv2))
+
+    def solve(self, method, tol, max_iter):
+        """method "pcg" / "pbicgstab": (converged, conv_ratio, iteration)."""
+        conv, it = C.c_double(tol), C.c_uint(max_iter)
+        rc = _check(lib().dfm_msys_solve(self.h, 0 if method == "pcg" else 1, C.byref(conv), C.byref(it)))
+        return rc == 1, conv.value, it.value
+
+    @classmethod
+    def from_arrays(cls, ctx, nblk, lens, mats):
+        """mats: {(i, j): dict(rowptr, colind, val[, dia])} -> patterns and values installed."""
+        ms = cls(ctx, nblk, lens)
+        for (i, j), a in sorted(mats.items()):
+            ms.add_pattern(i, j, a["rowptr"], a["colind"])
+        for (i, j), a in sorted(mats.items()):
+            ms.set_matrix(i, j, a["val"], a.get("dia"))
+        return ms
 
 
 class Batch:

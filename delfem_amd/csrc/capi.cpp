@@ -273,6 +273,8 @@ static double* vec_ptr(Sys& s, int iv) {
 
 static long long mdof(const Sys& s) { return (long long)s.n * s.len; }
 
+double* sys_vec_ptr(Sys& s, int iv) { return vec_ptr(s, iv); }
+
 }  // namespace dfm
 
 using namespace dfm;
@@ -1003,6 +1005,19 @@ int dfm_sys_set_matrix(dfm_sys sys, const double* crs, const double* dia) {
   return DFM_OK;
 }
 
+}  // extern "C"
+
+// FinalizeMarge of a system whose values came from the host: Dirichlet rows and columns by the flags of
+// dfm_sys_set_bc, the residual zeroed at fixed dofs and kept as the right-hand side, the SELL values refreshed.
+int dfm::sys_apply_bc(Sys& s) {
+  hipStream_t st = s.ctx->stream;
+  DFM_HIP(launch_set_bc(s.n, s.len, s.d_rowptr.p, s.d_colind.p, s.d_crs.p, s.d_dia.p, s.d_res.p, s.d_bc.p, st));
+  DFM_HIP(launch_copy(s.d_res.p, s.d_rhs.p, mdof(s), st));
+  return finish_matrix(s, st);
+}
+
+extern "C" {
+
 int dfm_sys_get_matrix(dfm_sys sys, double* crs, double* dia) {
   NEED(sys, "dfm_sys_get_matrix");
   Sys& s = sys->s;
@@ -1388,20 +1403,26 @@ int dfm_sys_set_vector(dfm_sys sys, int iv, const double* in) {
 // (ILU(0): the patterns coincide and the lane-parallel kernel reads the matrix itself; otherwise a scatter into
 // the zeroed fill pattern, factored in place), then the phases of the numeric factorisation; singular pivots
 // are counted in d_iflag[4].
-static hipError_t factor_numeric(Sys& s, hipStream_t st) {
+// prefilled: the factor arrays already hold the values to factor (a multi-segment system's diagonal segment after
+// its coupling updates); they are factored in place.
+static hipError_t factor_numeric(Sys& s, hipStream_t st, bool prefilled = false) {
   hipError_t e;
   const long long fncrs = (long long)s.fcolind.size();
   const bool lanes = s.fplan.ok && std::getenv("DFM_ILU_ROWKERNEL") == nullptr;
   const double* a_crs = s.d_fcrs.p;
   const double* a_dia = s.d_fdia.p;
-  if (s.a2f.empty()) {
+  if (prefilled) {
+    // in place
+  } else if (s.a2f.empty()) {
     if (lanes) a_crs = s.d_crs.p, a_dia = s.d_dia.p;
     else if ((e = launch_copy(s.d_crs.p, s.d_fcrs.p, (long long)s.ncrs * s.bs, st)) != hipSuccess) return e;
   } else {
     if ((e = launch_fill(s.d_fcrs.p, 0.0, fncrs * s.bs, st)) != hipSuccess) return e;
     if ((e = launch_scatter_blocks(s.ncrs, s.bs, s.d_a2f.p, s.d_crs.p, s.d_fcrs.p, st)) != hipSuccess) return e;
   }
-  if (a_dia == s.d_fdia.p && (e = launch_copy(s.d_dia.p, s.d_fdia.p, (long long)s.n * s.bs, st)) != hipSuccess) return e;
+  if (!prefilled && a_dia == s.d_fdia.p &&
+      (e = launch_copy(s.d_dia.p, s.d_fdia.p, (long long)s.n * s.bs, st)) != hipSuccess)
+    return e;
   if ((e = hipMemsetAsync(s.d_iflag.p + 4, 0, sizeof(int), st)) != hipSuccess) return e;
   if (lanes) {
     const FactorPlan& fp = s.fplan;
@@ -1415,16 +1436,40 @@ static hipError_t factor_numeric(Sys& s, hipStream_t st) {
                            s.d_iflag.p + 4, st);
 }
 
-int dfm_prec_factor(dfm_sys sys) {
-  NEED(sys, "dfm_prec_factor");
-  Sys& s = sys->s;
-  DFM_CHECK(s.assembled, DFM_ERR_STATE, "dfm_prec_factor: matrix not assembled");
-  hipStream_t st = s.ctx->stream;
+}  // extern "C"
+
+// The factor arrays, padded by two doubles as the matrix.
+static int factor_alloc(Sys& s) {
   const long long fncrs = (long long)s.fcolind.size();
-  // padded by two doubles, as the matrix
   if (s.d_fcrs.n != (size_t)(fncrs * s.bs + 2)) DFM_HIP(s.d_fcrs.alloc((size_t)(fncrs * s.bs + 2)));
   if (!s.d_fdia.p) DFM_HIP(s.d_fdia.alloc((size_t)s.n * s.bs + 2));
-  DFM_HIP(factor_numeric(s, st));
+  return DFM_OK;
+}
+
+int dfm::sys_factor_prefill(Sys& s) {
+  DFM_CHECK(s.assembled, DFM_ERR_STATE, "factor: matrix not assembled");
+  int rc = factor_alloc(s);
+  if (rc != DFM_OK) return rc;
+  hipStream_t st = s.ctx->stream;
+  const long long fncrs = (long long)s.fcolind.size();
+  if (s.a2f.empty()) {
+    DFM_HIP(launch_copy(s.d_crs.p, s.d_fcrs.p, (long long)s.ncrs * s.bs, st));
+  } else {
+    DFM_HIP(launch_fill(s.d_fcrs.p, 0.0, fncrs * s.bs, st));
+    DFM_HIP(launch_scatter_blocks(s.ncrs, s.bs, s.d_a2f.p, s.d_crs.p, s.d_fcrs.p, st));
+  }
+  DFM_HIP(launch_copy(s.d_dia.p, s.d_fdia.p, (long long)s.n * s.bs, st));
+  s.factored = s.factored32 = false;
+  s.prepared = -1;
+  return DFM_OK;
+}
+
+// The numeric factorisation (in place when prefilled), its sweep slabs and the singular-pivot check.
+int dfm::sys_factor_run(Sys& s, bool prefilled) {
+  hipStream_t st = s.ctx->stream;
+  int rc = factor_alloc(s);
+  if (rc != DFM_OK) return rc;
+  DFM_HIP(factor_numeric(s, st, prefilled));
   int nsing = 0;
   DFM_HIP(hipMemcpyAsync(s.h_pinned + 8, s.d_iflag.p + 4, sizeof(int), hipMemcpyDeviceToHost, st));
   DFM_HIP(hipStreamSynchronize(st));
@@ -1451,6 +1496,15 @@ int dfm_prec_factor(dfm_sys sys) {
   }
   s.factored = true;
   return DFM_OK;
+}
+
+extern "C" {
+
+int dfm_prec_factor(dfm_sys sys) {
+  NEED(sys, "dfm_prec_factor");
+  Sys& s = sys->s;
+  DFM_CHECK(s.assembled, DFM_ERR_STATE, "dfm_prec_factor: matrix not assembled");
+  return sys_factor_run(s, false);
 }
 
 int dfm_prec_apply(dfm_sys sys, int iv) {

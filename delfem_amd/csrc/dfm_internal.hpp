@@ -296,6 +296,17 @@ int sys_solve_setup(Sys& s, int precision);
 int sys_true_residual(Sys& s, hipStream_t st, double* rel);
 // A vector handle (caller's numbering) read on st, the stream whose work last wrote it (capi.cpp).
 int sys_download(Sys& s, int iv, double* out, hipStream_t st);
+// The device vector of a handle (-1 residual, -2 update, >= 0 temporaries), nullptr for a bad handle (capi.cpp).
+double* sys_vec_ptr(Sys& s, int iv);
+// FinalizeMarge of host-installed values: Dirichlet treatment by the flags of dfm_sys_set_bc (capi.cpp).
+int sys_apply_bc(Sys& s);
+// The ILU factor in two halves (capi.cpp): the matrix values copied (ILU(0)) or scattered (ILU(k)) into the factor
+// arrays, then the numeric factorisation, in place when prefilled, with its sweep slabs; DFM_ERR_SINGULAR as
+// dfm_prec_factor. Between the two a caller may update the factor arrays (multi-segment systems).
+int sys_factor_prefill(Sys& s);
+int sys_factor_run(Sys& s, bool prefilled);
+// z := M^-1 z restricted to the forward (dirs bit 0) and/or backward (bit 1) substitution, FP64 (solver.hip).
+hipError_t precond_apply64_dirs(const Sys& s, double* z, int dirs, hipStream_t st);
 // Take a system out of the batch it is solved in (batch.hip).
 void batch_detach(struct Batch* b, Sys* s);
 // Host time of a batch: seconds in table rebuilds, rebuilds, seconds in polls, true-residual restarts (batch.hip).

@@ -650,13 +650,13 @@ static unsigned spmv_grid(const Sys& s, int tpr) {
 // z = M^-1 src over the sweep slabs: TV the factor values (FP64 or the FP32 copy), V the vectors and the
 // arithmetic.
 template <int B, typename TV, typename V>
-static hipError_t precond_T(const Sys& s, const V* src, V* z, const int* done, hipStream_t st) {
+static hipError_t precond_T(const Sys& s, const V* src, V* z, const int* done, hipStream_t st, int dirs = 3) {
   constexpr bool f32 = sizeof(TV) == 4;
   const Sweep& f = s.fwd;
   const SlabArgs fa{f.d_gmeta.p, f.d_srow.p, f.d_scnt.p, f.d_ecol.p};
   const TV* fval = f32 ? (const TV*)f.d_eval32.p : (const TV*)f.d_eval.p;
   const TV* fdval = f32 ? (const TV*)f.d_dval32.p : (const TV*)f.d_dval.p;
-  for (int p = 0; p < f.nphase(); p++) {
+  for (int p = 0; p < f.nphase() && (dirs & 1); p++) {
     const int g0 = f.phase_gptr[p], ng = f.phase_gptr[p + 1] - g0;
     if (!ng) continue;
     with_slab_caps(f.phase_wo[p], f.phase_wi[p], B == 3 && sizeof(V) == 8 && sizeof(TV) == 8, [&]<int MO, int MI>() {
@@ -667,7 +667,7 @@ static hipError_t precond_T(const Sys& s, const V* src, V* z, const int* done, h
   const Sweep& b = s.bwd;
   const SlabArgs ba{b.d_gmeta.p, b.d_srow.p, b.d_scnt.p, b.d_ecol.p};
   const TV* bval = f32 ? (const TV*)b.d_eval32.p : (const TV*)b.d_eval.p;
-  for (int p = 0; p < b.nphase(); p++) {
+  for (int p = 0; p < b.nphase() && (dirs & 2); p++) {
     const int g0 = b.phase_gptr[p], ng = b.phase_gptr[p + 1] - g0;
     if (!ng) continue;
     with_slab_caps(b.phase_wo[p], b.phase_wi[p], B == 3 && sizeof(V) == 8 && sizeof(TV) == 8, [&]<int MO, int MI>() {
@@ -676,6 +676,13 @@ static hipError_t precond_T(const Sys& s, const V* src, V* z, const int* done, h
     });
   }
   return hipGetLastError();
+}
+
+hipError_t precond_apply64_dirs(const Sys& s, double* z, int dirs, hipStream_t st) {
+  if (s.len == 1) return precond_T<1, double, double>(s, z, z, nullptr, st, dirs);
+  if (s.len == 2) return precond_T<2, double, double>(s, z, z, nullptr, st, dirs);
+  if (s.len == 3) return precond_T<3, double, double>(s, z, z, nullptr, st, dirs);
+  return hipErrorInvalidValue;
 }
 
 hipError_t precond_apply64(const Sys& s, const double* src, double* z, const int* done, hipStream_t st) {

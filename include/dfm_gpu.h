@@ -474,6 +474,57 @@ int dfm_eqn_clear(dfm_eqn eq, int what);
 /* The current linear system (NULL before the first Solve and after a clear), for the handle operations. */
 int dfm_eqn_system(dfm_eqn eq, dfm_sys* sys);
 
+/* ---- multi-segment systems (LsSol::CLinearSystem with several segments, src/ls/linearsystem.cpp:74-415) ----
+ * Segments i = 0 .. nseg-1 of nblk[i] blocks of len[i] dofs (CLinearSystem::AddLinSysSeg, :138-167); matrix (i, i)
+ * is a diagonal segment (CMatDia_BlkCrs: off-diagonal blocks in CRS, diagonal blocks apart), matrix (i, j), i != j,
+ * a coupling of len[i] x len[j] blocks (CMat_BlkCrs). Vectors and Dirichlet flags are concatenated over the
+ * segments, segment 0 first; handles as for dfm_sys (-1 residual, -2 update, >= 0 temporaries). The preconditioner
+ * is CPreconditioner_ILU with no fill level (src/ls/preconditioner.cpp:57-335): ILU(0) of every diagonal segment
+ * (CMatDiaFrac_BlkCrs) and of every coupling (CMatFrac_BlkCrs, src/matvec/matfrac_blkcrs.cpp), factored in the
+ * reference's order of segment pairs; factor and preconditioner are the reference's bit for bit. The reference
+ * applies no ordering to such systems, and neither does this (natural order, level-scheduled). Every diagonal
+ * segment needs its pattern before vectors, Dirichlet flags or the factor are used. Values come from the host, as
+ * the element routines leave them through ILinearSystem_Eqn::GetMatrix(id_field1, .., id_field2, ..)
+ * (include/delfem/linearsystem_interface_eqnsys.h:48-68). */
+#define DFM_MSYS_MAXSEG 8
+typedef struct dfm_msys_s* dfm_msys;
+int dfm_msys_create(dfm_ctx ctx, int nseg, const int* nblk, const int* len, dfm_msys* out);
+int dfm_msys_destroy(dfm_msys ms);
+/* CLinearSystem::AddMat_Dia (i == j, the off-diagonal pattern: no diagonal) / AddMat_NonDia (i != j)
+ * (src/ls/linearsystem.cpp:419-500): CRS rowptr[nblk[i]+1], colind ascending in each row; once per matrix. */
+int dfm_msys_add_pattern(dfm_msys ms, int i, int j, const unsigned* rowptr, const unsigned* colind);
+/* Off-diagonal block count of matrix (i, j), -1 when it has no pattern. */
+int dfm_msys_info(dfm_msys ms, int i, int j, long long* nnz);
+/* Values of matrix (i, j): blocks row-major in pattern order; dia (i == j) the diagonal blocks. */
+int dfm_msys_set_matrix(dfm_msys ms, int i, int j, const double* val, const double* dia);
+int dfm_msys_get_matrix(dfm_msys ms, int i, int j, double* val, double* dia);
+int dfm_msys_set_vector(dfm_msys ms, int iv, const double* v);
+int dfm_msys_get_vector(dfm_msys ms, int iv, double* v);
+/* FinalizeMarge (src/ls/linearsystem.cpp:92-128): the Dirichlet flags (concatenated), the diagonal segments'
+ * SetBoundaryCondition, the couplings' fixed rows and columns zeroed, the fixed residual entries zeroed; the
+ * residual norm (the segments' squared norms summed in segment order) in *res_norm when non-NULL. */
+int dfm_msys_finalize(dfm_msys ms, const int* flags, double* res_norm);
+/* CPreconditioner_ILU::SetLinearSystem + SetValue (ILU(0)); DFM_ERR_SINGULAR when a diagonal segment's
+ * DoILUDecomp fails. dfm_msys_get_factor reads factor (i, j) in the layout of dfm_msys_get_matrix (dia: the
+ * inverted diagonal blocks of segment i). */
+int dfm_msys_factor(dfm_msys ms);
+int dfm_msys_get_factor(dfm_msys ms, int i, int j, double* val, double* dia);
+/* CPreconditioner_ILU::SolvePrecond (src/ls/preconditioner.cpp:301-335): v := M^-1 v in place. */
+int dfm_msys_precond(dfm_msys ms, int iv);
+/* Handle operations of CLinearSystem (src/ls/linearsystem.cpp:208-415); DOT sums the segments' products in
+ * segment order, MATVEC adds each row's couplings after its diagonal segment. */
+int dfm_msys_resize_tmp(dfm_msys ms, int ntmp);
+int dfm_msys_ntmp(dfm_msys ms, int* ntmp);
+int dfm_msys_dot(dfm_msys ms, int iv1, int iv2, double* out);
+int dfm_msys_copy(dfm_msys ms, int iv1, int iv2);
+int dfm_msys_scal(dfm_msys ms, double alpha, int iv);
+int dfm_msys_axpy(dfm_msys ms, double alpha, int iv1, int iv2);
+int dfm_msys_matvec(dfm_msys ms, double alpha, int iv1, double beta, int iv2);
+/* LsSol::Solve_PCG (method DFM_METHOD_PCG) or Solve_PBiCGSTAB (DFM_METHOD_PBICGSTAB) with the factored ILU(0),
+ * FP64, host-driven over the handle operations (src/ls/solver_ls_iter.cpp:100-172, 291-408): the reference's
+ * contract (conv_ratio in = tolerance, out = ratio; iteration in = max, out = count; 1 converged, 0 not). */
+int dfm_msys_solve(dfm_msys ms, int method, double* conv_ratio, unsigned* iteration);
+
 /* Device memory currently allocated by the library (bytes). */
 int dfm_device_bytes(long long* bytes);
 

@@ -884,11 +884,12 @@ int cr_ilu0_factor(int n, int len, const unsigned* frowptr, const unsigned* fcol
 
 /* src/matvec/matdiafrac_blkcrs.cpp:332-507 (ForwardSubstitution), :509-669 (BackwardSubstitution),      */
 /* src/ls/preconditioner.cpp:301-335 (SolvePrecond), src/matvec/ordering_blk.cpp:136-166 (permutation).  */
-static void fwd_bwd(int n, int bl, const unsigned* rp, const unsigned* ci, const unsigned* di, const double* fcrs,
-                    const double* fdia, double* v) {
+/* dirs: bit 0 the forward substitution, bit 1 the backward one */
+static void fwd_bwd_dirs(int n, int bl, const unsigned* rp, const unsigned* ci, const unsigned* di,
+                         const double* fcrs, const double* fdia, double* v, int dirs) {
   const int bs = bl * bl;
   double t[8];
-  for (long i = 0; i < n; i++) {
+  for (long i = 0; i < n && (dirs & 1); i++) {
     for (int d = 0; d < bl; d++) t[d] = v[i * bl + d];
     for (unsigned q = rp[i]; q < di[i]; q++) {
       const long j = ci[q];
@@ -927,7 +928,7 @@ static void fwd_bwd(int n, int bl, const unsigned* rp, const unsigned* ci, const
       }
     }
   }
-  for (long i = (long)n - 1; i >= 0; i--) {
+  for (long i = (long)n - 1; i >= 0 && (dirs & 2); i--) {
     for (int d = 0; d < bl; d++) t[d] = v[i * bl + d];
     for (unsigned q = di[i]; q < rp[i + 1]; q++) {
       const long j = ci[q];
@@ -952,6 +953,11 @@ static void fwd_bwd(int n, int bl, const unsigned* rp, const unsigned* ci, const
   }
 }
 
+static void fwd_bwd(int n, int bl, const unsigned* rp, const unsigned* ci, const unsigned* di, const double* fcrs,
+                    const double* fdia, double* v) {
+  fwd_bwd_dirs(n, bl, rp, ci, di, fcrs, fdia, v, 3);
+}
+
 void cr_ilu_solve(int n, int len, const unsigned* frowptr, const unsigned* fcolind, const unsigned* diaind,
                   const double* fcrs, const double* fdia, const int* order, double* vec, double* work) {
   if (!order) {
@@ -963,6 +969,328 @@ void cr_ilu_solve(int n, int len, const unsigned* frowptr, const unsigned* fcoli
   fwd_bwd(n, len, frowptr, fcolind, diaind, fcrs, fdia, work);
   for (long i = 0; i < n; i++)
     for (int d = 0; d < len; d++) vec[(long)order[i] * len + d] = work[i * len + d];
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Multi-segment systems: LsSol::CLinearSystem with several segments (src/ls/linearsystem.cpp:74-415) and  */
+/* CPreconditioner_ILU over them (src/ls/preconditioner.cpp:57-335), ILU(0): the diagonal segments are     */
+/* CMatDiaFrac_BlkCrs, the couplings CMatFrac_BlkCrs (src/matvec/matfrac_blkcrs.cpp).                      */
+
+/* CalcSubMatPr (src/matvec/matfrac_blkcrs.cpp:43-54): out -= a b, one product at a time */
+static void sub_matpr(double* out, const double* a, const double* b, int ni, int nk, int nj) {
+  for (int i = 0; i < ni; i++)
+    for (int j = 0; j < nj; j++)
+      for (int k = 0; k < nk; k++) out[i * nj + j] -= a[i * nk + k] * b[k * nj + j];
+}
+
+/* CalcMatPr (:56-72): out := d out, d square ni x ni, each entry summed from 0.0 */
+static void matpr(double* out, const double* d, int ni, int nj) {
+  double tmp[64];
+  for (int i = 0; i < ni; i++)
+    for (int j = 0; j < nj; j++) {
+      tmp[i * nj + j] = 0.0;
+      for (int k = 0; k < ni; k++) tmp[i * nj + j] += d[i * ni + k] * out[k * nj + j];
+    }
+  for (int i = 0; i < ni * nj; i++) out[i] = tmp[i];
+}
+
+/* CMat_BlkCrs::MatVec(alpha, x, beta, y, isnt_trans = true), src/matvec/mat_blkcrs.cpp:955-1175: the shape
+ * branches sum in different orders (1 x n: alpha (sum of the row's products); n x 1: products with alpha x;
+ * otherwise one alpha a x term at a time). */
+void cr_blk_matvec(const cr_blk* A, double alpha, const double* x, double beta, double* y) {
+  const int bi = A->bi, bj = A->bj, bs = bi * bj;
+  const unsigned *rp = A->rowptr, *ci = A->colind;
+  const double* v = A->val;
+  for (long i = 0; i < A->nrow; i++) {
+    double* iy = y + i * bi;
+    if (bi == 1 && bj == 1 && beta == 0.0) {
+      iy[0] = 0.0;
+    } else {
+      for (int d = 0; d < bi; d++) iy[d] *= beta;
+    }
+    for (unsigned q = rp[i]; q < rp[i + 1]; q++) {
+      const long j = ci[q];
+      const double* a = v + (size_t)q * bs;
+      const double* jx = x + j * bj;
+      if (bi == 1 && bj == 1) {
+        iy[0] += alpha * a[0] * jx[0];
+      } else if (bi == 1 && bj == 2) {
+        iy[0] += alpha * (a[0] * jx[0] + a[1] * jx[1]);
+      } else if (bi == 1 && bj == 3) {
+        iy[0] += alpha * (a[0] * jx[0] + a[1] * jx[1] + a[2] * jx[2]);
+      } else if (bi == 1) {
+        for (int c = 0; c < bj; c++) iy[0] += alpha * a[c] * jx[c];
+      } else if (bj == 1) {
+        const double ax = jx[0] * alpha;
+        for (int r = 0; r < bi; r++) iy[r] += a[r] * ax;
+      } else {
+        for (int r = 0; r < bi; r++)
+          for (int c = 0; c < bj; c++) iy[r] += alpha * a[r * bj + c] * jx[c];
+      }
+    }
+  }
+}
+
+/* CLinearSystem::FinalizeMarge (src/ls/linearsystem.cpp:92-128): SetBoundaryCondition of every diagonal segment,
+ * SetBoundaryCondition_Row of every coupling by its row segment's flags, then _Colum by its column segment's
+ * (src/matvec/mat_blkcrs.cpp:829-874, 910-953), the fixed residual entries zeroed; the norm sums the segments'
+ * squared norms in segment order. flags and res are concatenated over the segments. */
+double cr_ms_set_bc(cr_msys* s, const int* flags, double* res) {
+  const int ns = s->nseg;
+  for (int i = 0; i < ns; i++) {
+    cr_blk* A = s->a[i * ns + i];
+    if (!A) continue;
+    const int bl = A->bi, bs = bl * bl;
+    const int* fl = flags + s->off[i];
+    for (long r = 0; r < A->nrow; r++)
+      for (int id = 0; id < bl; id++) {
+        if (fl[r * bl + id] == 0) continue;
+        for (int k = 0; k < bl; k++) A->dia[r * bs + k * bl + id] = 0.0;
+        for (int j = 0; j < bl; j++) A->dia[r * bs + id * bl + j] = 0.0;
+        A->dia[r * bs + id * bl + id] = 1.0;
+        for (unsigned q = A->rowptr[r]; q < A->rowptr[r + 1]; q++)
+          for (int j = 0; j < bl; j++) A->val[(size_t)q * bs + id * bl + j] = 0.0;
+      }
+    const unsigned ncrs = A->rowptr[A->nrow];
+    for (unsigned q = 0; q < ncrs; q++) {
+      const unsigned j1 = A->colind[q];
+      for (int jd = 0; jd < bl; jd++) {
+        if (fl[(long)j1 * bl + jd] == 0) continue;
+        for (int id = 0; id < bl; id++) A->val[(size_t)q * bs + id * bl + jd] = 0.0;
+      }
+    }
+  }
+  for (int i = 0; i < ns; i++)
+    for (int j = 0; j < ns; j++) {
+      cr_blk* A = s->a[i * ns + j];
+      if (!A || i == j) continue;
+      const int bi = A->bi, bj = A->bj, bs = bi * bj;
+      const int* fl = flags + s->off[i];
+      for (long r = 0; r < A->nrow; r++)
+        for (int id = 0; id < bi; id++) {
+          if (fl[r * bi + id] == 0) continue;
+          for (unsigned q = A->rowptr[r]; q < A->rowptr[r + 1]; q++)
+            for (int jd = 0; jd < bj; jd++) A->val[(size_t)q * bs + id * bj + jd] = 0.0;
+        }
+    }
+  for (int j = 0; j < ns; j++)
+    for (int i = 0; i < ns; i++) {
+      cr_blk* A = s->a[i * ns + j];
+      if (!A || i == j) continue;
+      const int bi = A->bi, bj = A->bj, bs = bi * bj;
+      const int* fl = flags + s->off[j];
+      const unsigned ncrs = A->rowptr[A->nrow];
+      for (unsigned q = 0; q < ncrs; q++) {
+        const unsigned c = A->colind[q];
+        for (int jd = 0; jd < bj; jd++) {
+          if (fl[(long)c * bj + jd] == 0) continue;
+          for (int id = 0; id < bi; id++) A->val[(size_t)q * bs + id * bj + jd] = 0.0;
+        }
+      }
+    }
+  for (long k = 0; k < s->off[ns]; k++)
+    if (flags[k] == 1) res[k] = 0.0;
+  double sq = 0.0;
+  for (int i = 0; i < ns; i++) {
+    double si = 0.0;
+    for (long k = s->off[i]; k < s->off[i + 1]; k++) si += res[k] * res[k];
+    sq += si;
+  }
+  return sqrt(sq);
+}
+
+/* CLinearSystem::MATVEC (src/ls/linearsystem.cpp:250-291): per segment its diagonal matrix
+ * (CMatDia_BlkCrs::MatVec) or y *= beta, then the couplings of its row in segment order with beta 1. */
+void cr_ms_matvec(const cr_msys* s, double alpha, const double* x, double beta, double* y) {
+  const int ns = s->nseg;
+  if (alpha == 0.0) {
+    for (long k = 0; k < s->off[ns]; k++) y[k] *= beta;
+    return;
+  }
+  for (int i = 0; i < ns; i++) {
+    const cr_blk* D = s->a[i * ns + i];
+    double* yi = y + s->off[i];
+    if (D) cr_matvec(D->nrow, D->bi, D->rowptr, D->colind, D->val, D->dia, alpha, x + s->off[i], beta, yi);
+    else
+      for (long k = s->off[i]; k < s->off[i + 1]; k++) y[k] *= beta;
+    for (int j = 0; j < ns; j++) {
+      const cr_blk* A = s->a[i * ns + j];
+      if (!A || i == j) continue;
+      cr_blk_matvec(A, alpha, x + s->off[j], 1.0, yi);
+    }
+  }
+}
+
+static double ms_dot(const cr_msys* s, const double* a, const double* b) {
+  double d = 0.0;
+  for (int i = 0; i < s->nseg; i++) {
+    double di = 0.0;
+    for (long k = s->off[i]; k < s->off[i + 1]; k++) di += a[k] * b[k];
+    d += di;
+  }
+  return d;
+}
+
+/* CMatDiaFrac_BlkCrs::DoILUDecompLowUp (src/matvec/matdiafrac_blkcrs.cpp:671-772, uniform blocks):
+ * D_ij -= L_ik U_kj over the coupling factors low (segment i x k) and up (k x i), rows in order, each row's
+ * low entries ascending, entries outside the factor pattern dropped */
+static void ilu_dia_lowup(cr_blk* D, const cr_blk* low, const cr_blk* up, int* row2crs) {
+  const int bi = D->bi, bk = low->bj;
+  for (long i = 0; i < D->nrow; i++) {
+    for (unsigned q = D->rowptr[i]; q < D->rowptr[i + 1]; q++) row2crs[D->colind[q]] = (int)q;
+    for (unsigned ik = low->rowptr[i]; ik < low->rowptr[i + 1]; ik++) {
+      const unsigned k = low->colind[ik];
+      const double* pik = low->val + (size_t)ik * bi * bk;
+      for (unsigned kj = up->rowptr[k]; kj < up->rowptr[k + 1]; kj++) {
+        const unsigned j = up->colind[kj];
+        const double* pkj = up->val + (size_t)kj * bk * bi;
+        if (j != (unsigned)i) {
+          const int ij = row2crs[j];
+          if (ij == -1) continue;
+          sub_matpr(D->val + (size_t)ij * bi * bi, pik, pkj, bi, bk, bi);
+        } else {
+          sub_matpr(D->dia + (size_t)i * bi * bi, pik, pkj, bi, bk, bi);
+        }
+      }
+    }
+    for (unsigned q = D->rowptr[i]; q < D->rowptr[i + 1]; q++) row2crs[D->colind[q]] = -1;
+  }
+}
+
+/* CMatFrac_BlkCrs::DoILUDecompLowUp (src/matvec/matfrac_blkcrs.cpp:540-599): F_ij -= L_ik U_kj */
+static void ilu_od_lowup(cr_blk* F, const cr_blk* low, const cr_blk* up, int* row2crs) {
+  const int bi = F->bi, bj = F->bj, bk = low->bj;
+  for (long i = 0; i < F->nrow; i++) {
+    for (unsigned q = F->rowptr[i]; q < F->rowptr[i + 1]; q++) row2crs[F->colind[q]] = (int)q;
+    for (unsigned ik = low->rowptr[i]; ik < low->rowptr[i + 1]; ik++) {
+      const unsigned k = low->colind[ik];
+      const double* pik = low->val + (size_t)ik * bi * bk;
+      for (unsigned kj = up->rowptr[k]; kj < up->rowptr[k + 1]; kj++) {
+        const int ij = row2crs[up->colind[kj]];
+        if (ij == -1) continue;
+        sub_matpr(F->val + (size_t)ij * bi * bj, pik, up->val + (size_t)kj * bk * bj, bi, bk, bj);
+      }
+    }
+    for (unsigned q = F->rowptr[i]; q < F->rowptr[i + 1]; q++) row2crs[F->colind[q]] = -1;
+  }
+}
+
+/* CMatFrac_BlkCrs::DoILUDecompUp (:601-701, uniform blocks): the upper coupling F (segment i x j, i < j) is
+ * eliminated with the lower part of the factored diagonal segment D = segment i (its rows in order, reading
+ * the already scaled rows k < i of F), then each row is scaled by D's inverted diagonal block */
+static void ilu_od_up(cr_blk* F, const cr_blk* D, int* row2crs) {
+  const int bi = F->bi, bj = F->bj;
+  for (long i = 0; i < F->nrow; i++) {
+    for (unsigned q = F->rowptr[i]; q < F->rowptr[i + 1]; q++) row2crs[F->colind[q]] = (int)q;
+    for (unsigned ik = D->rowptr[i]; ik < D->diaind[i]; ik++) {
+      const unsigned k = D->colind[ik];
+      const double* pik = D->val + (size_t)ik * bi * bi;
+      for (unsigned kj = F->rowptr[k]; kj < F->rowptr[k + 1]; kj++) {
+        const int ij = row2crs[F->colind[kj]];
+        if (ij == -1) continue;
+        sub_matpr(F->val + (size_t)ij * bi * bj, pik, F->val + (size_t)kj * bi * bj, bi, bi, bj);
+      }
+    }
+    const double* dii = D->dia + (size_t)i * bi * bi;
+    for (unsigned q = F->rowptr[i]; q < F->rowptr[i + 1]; q++) matpr(F->val + (size_t)q * bi * bj, dii, bi, bj);
+    for (unsigned q = F->rowptr[i]; q < F->rowptr[i + 1]; q++) row2crs[F->colind[q]] = -1;
+  }
+}
+
+/* CMatFrac_BlkCrs::DoILUDecompLow (:703-783, uniform blocks): the lower coupling F (segment i x j, i > j)
+ * against the upper part of the factored diagonal segment D = segment j: F_ij -= F_ik U_kj, each row's
+ * entries ascending */
+static void ilu_od_low(cr_blk* F, const cr_blk* D, int* row2crs) {
+  const int bi = F->bi, bj = F->bj;
+  for (long i = 0; i < F->nrow; i++) {
+    for (unsigned q = F->rowptr[i]; q < F->rowptr[i + 1]; q++) row2crs[F->colind[q]] = (int)q;
+    for (unsigned ik = F->rowptr[i]; ik < F->rowptr[i + 1]; ik++) {
+      const unsigned k = F->colind[ik];
+      const double* pik = F->val + (size_t)ik * bi * bj;
+      for (unsigned kj = D->diaind[k]; kj < D->rowptr[k + 1]; kj++) {
+        const int ij = row2crs[D->colind[kj]];
+        if (ij == -1) continue;
+        sub_matpr(F->val + (size_t)ij * bi * bj, pik, D->val + (size_t)kj * bj * bj, bi, bj, bj);
+      }
+    }
+    for (unsigned q = F->rowptr[i]; q < F->rowptr[i + 1]; q++) row2crs[F->colind[q]] = -1;
+  }
+}
+
+/* CPreconditioner_ILU::SetValue (src/ls/preconditioner.cpp:215-297), ILU(0): the factors f take the matrices'
+ * values (SetValue_Initialize), then segment row by segment row: the diagonal segment gets the LowUp updates of
+ * the earlier segments and is factored (DoILUDecomp); a coupling of the row gets LowUp for the segments before
+ * both, then Up (row segment before the column's) or Low (after) against the factored diagonal segment.
+ * Returns 0 when a diagonal factorisation fails (more than 10 singular pivots), else 1. */
+int cr_ms_ilu0(cr_msys* s) {
+  const int ns = s->nseg;
+  int nmax = 1;
+  for (int i = 0; i < ns; i++) nmax = s->nblk[i] > nmax ? s->nblk[i] : nmax;
+  int* row2crs = (int*)malloc(sizeof(int) * (size_t)nmax);
+  for (int k = 0; k < nmax; k++) row2crs[k] = -1;
+  for (int p = 0; p < ns * ns; p++) {
+    const cr_blk* A = s->a[p];
+    cr_blk* F = s->f[p];
+    if (!A) continue;
+    const size_t bs = (size_t)A->bi * A->bj;
+    memcpy(F->val, A->val, sizeof(double) * bs * A->rowptr[A->nrow]);
+    if (A->dia) memcpy(F->dia, A->dia, sizeof(double) * bs * A->nrow);
+  }
+  int ok = 1;
+  for (int i = 0; i < ns && ok; i++) {
+    for (int j = 0; j < ns && ok; j++) {
+      if (i == j) {
+        cr_blk* D = s->f[i * ns + i];
+        for (int k = 0; k < i; k++) {
+          const cr_blk *low = s->f[i * ns + k], *up = s->f[k * ns + i];
+          if (low && up) ilu_dia_lowup(D, low, up, row2crs);
+        }
+        if (!cr_ilu0_factor(D->nrow, D->bi, D->rowptr, D->colind, D->diaind, D->val, D->dia)) ok = 0;
+        continue;
+      }
+      cr_blk* F = s->f[i * ns + j];
+      if (!F) continue;
+      const int kmax = i < j ? i : j;
+      for (int k = 0; k <= kmax; k++) {
+        if (k == i && k < j) {
+          ilu_od_up(F, s->f[i * ns + i], row2crs);
+          continue;
+        }
+        if (k == j && k < i) {
+          ilu_od_low(F, s->f[j * ns + j], row2crs);
+          continue;
+        }
+        const cr_blk *low = s->f[i * ns + k], *up = s->f[k * ns + j];
+        if (!low || !up) continue;
+        ilu_od_lowup(F, low, up, row2crs);
+      }
+    }
+  }
+  free(row2crs);
+  return ok;
+}
+
+/* CPreconditioner_ILU::SolvePrecond (src/ls/preconditioner.cpp:301-335): forward, segment by segment, the
+ * lower couplings' MatVec(-1, v_j, 1, v_i) then the diagonal segment's forward substitution; backward from the
+ * last segment, the upper couplings then the backward substitution. v is concatenated over the segments. */
+void cr_ms_precond(const cr_msys* s, double* v) {
+  const int ns = s->nseg;
+  for (int i = 0; i < ns; i++) {
+    for (int j = 0; j < i; j++) {
+      const cr_blk* F = s->f[i * ns + j];
+      if (F) cr_blk_matvec(F, -1.0, v + s->off[j], 1.0, v + s->off[i]);
+    }
+    const cr_blk* D = s->f[i * ns + i];
+    fwd_bwd_dirs(D->nrow, D->bi, D->rowptr, D->colind, D->diaind, D->val, D->dia, v + s->off[i], 1);
+  }
+  for (int i = ns - 1; i >= 0; i--) {
+    for (int j = i + 1; j < ns; j++) {
+      const cr_blk* F = s->f[i * ns + j];
+      if (F) cr_blk_matvec(F, -1.0, v + s->off[j], 1.0, v + s->off[i]);
+    }
+    const cr_blk* D = s->f[i * ns + i];
+    fwd_bwd_dirs(D->nrow, D->bi, D->rowptr, D->colind, D->diaind, D->val, D->dia, v + s->off[i], 2);
+  }
 }
 
 /* ---------------------------------------------------------------------------------------------------- */
@@ -988,10 +1316,30 @@ static void matvec(const cr_system* s, double alpha, const double* x, double bet
   cr_matvec(s->n, s->len, s->rowptr, s->colind, s->crs, s->dia, alpha, x, beta, y);
 }
 
-int cr_solve_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+/* The operator a preconditioned loop runs on: one segment (cr_system) or several (cr_msys, whose inner product
+ * is the sum of the segments' sequential sums, src/ls/linearsystem.cpp:335-360). */
+typedef struct {
+  long m;
+  const cr_system* s;
+  const cr_msys* ms;
+} kop;
+static double k_dot(const kop* o, const double* a, const double* b) {
+  if (o->s) return v_dot(o->m, a, b);
+  return ms_dot(o->ms, a, b);
+}
+static void k_precond(const kop* o, double* v, double* work) {
+  if (o->s) precond(o->s, v, work);
+  else cr_ms_precond(o->ms, v);
+}
+static void k_matvec(const kop* o, double alpha, const double* x, double beta, double* y) {
+  if (o->s) matvec(o->s, alpha, x, beta, y);
+  else cr_ms_matvec(o->ms, alpha, x, beta, y);
+}
+
+static int pcg_loop(const kop* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
   const double tol = *conv_ratio;
   const unsigned mx_iter = *iter;
-  const long m = (long)s->n * s->len;
+  const long m = s->m;
   double* p = (double*)calloc((size_t)m + 1, sizeof(double));
   double* z = (double*)calloc((size_t)m + 1, sizeof(double));
   double* w = (double*)calloc((size_t)m + 1, sizeof(double));
@@ -999,7 +1347,7 @@ int cr_solve_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, u
   v_scal(m, 0.0, x);
   double sq_inv_norm_res0;
   {
-    const double sq0 = v_dot(m, r, r);
+    const double sq0 = k_dot(s, r, r);
     if (sq0 < 1.0e-30) {
       *conv_ratio = 0.0;
       *iter = 0;
@@ -1009,21 +1357,21 @@ int cr_solve_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, u
     sq_inv_norm_res0 = 1.0 / sq0;
   }
   memcpy(z, r, sizeof(double) * m);
-  precond(s, z, w);
+  k_precond(s, z, w);
   memcpy(p, z, sizeof(double) * m);
   {
-    double inpro_rz = v_dot(m, r, z);
+    double inpro_rz = k_dot(s, r, z);
     for (unsigned it = 0; it < mx_iter; it++) {
       double alpha;
       {
-        matvec(s, 1.0, p, 0.0, z);
-        const double pap = v_dot(m, p, z);
+        k_matvec(s, 1.0, p, 0.0, z);
+        const double pap = k_dot(s, p, z);
         alpha = inpro_rz / pap;
       }
       v_axpy(m, -alpha, z, r);
       v_axpy(m, alpha, p, x);
       {
-        const double sq = v_dot(m, r, r);
+        const double sq = k_dot(s, r, r);
         if (sq * sq_inv_norm_res0 < tol * tol) {
           *conv_ratio = sqrt(sq * sq_inv_norm_res0);
           *iter = it;
@@ -1034,8 +1382,8 @@ int cr_solve_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, u
       double beta;
       {
         memcpy(z, r, sizeof(double) * m);
-        precond(s, z, w);
-        const double rz_new = v_dot(m, r, z);
+        k_precond(s, z, w);
+        const double rz_new = k_dot(s, r, z);
         beta = rz_new / inpro_rz;
         inpro_rz = rz_new;
       }
@@ -1043,7 +1391,7 @@ int cr_solve_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, u
       v_axpy(m, 1.0, z, p);
     }
     {
-      const double sq = v_dot(m, r, r);
+      const double sq = k_dot(s, r, r);
       *conv_ratio = sqrt(sq * sq_inv_norm_res0);
       ret = 0;
     }
@@ -1053,17 +1401,17 @@ done:
   return ret;
 }
 
-int cr_solve_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+static int pbicgstab_loop(const kop* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
   const double tol = *conv_ratio;
   const unsigned max_iter = *iter;
-  const long m = (long)s->n * s->len;
+  const long m = s->m;
   double* buf = (double*)calloc((size_t)m * 8 + 1, sizeof(double));
   double *vs = buf, *vMs = buf + m, *vAMs = buf + 2 * m, *vp = buf + 3 * m, *vMp = buf + 4 * m,
          *vAMp = buf + 5 * m, *vr2 = buf + 6 * m, *w = buf + 7 * m;
   int ret = 1;
   double sq_inv_norm_res_ini;
   {
-    const double sq0 = v_dot(m, r, r);
+    const double sq0 = k_dot(s, r, r);
     if (sq0 < 1.0e-60) {
       *conv_ratio = sqrt(sq0);
       *iter = 0;
@@ -1077,23 +1425,23 @@ int cr_solve_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ra
   *iter = max_iter;
   for (unsigned it = 1; it < max_iter; it++) {
     memcpy(vMp, vp, sizeof(double) * m);
-    precond(s, vMp, w);
-    const double r_r2 = v_dot(m, r, vr2);
-    matvec(s, 1.0, vMp, 0.0, vAMp);
+    k_precond(s, vMp, w);
+    const double r_r2 = k_dot(s, r, vr2);
+    k_matvec(s, 1.0, vMp, 0.0, vAMp);
     double alpha;
     {
-      const double den = v_dot(m, vAMp, vr2);
+      const double den = k_dot(s, vAMp, vr2);
       alpha = r_r2 / den;
     }
     memcpy(vs, r, sizeof(double) * m);
     v_axpy(m, -alpha, vAMp, vs);
     memcpy(vMs, vs, sizeof(double) * m);
-    precond(s, vMs, w);
-    matvec(s, 1.0, vMs, 0.0, vAMs);
+    k_precond(s, vMs, w);
+    k_matvec(s, 1.0, vMs, 0.0, vAMs);
     double omega;
     {
-      const double den = v_dot(m, vAMs, vAMs);
-      const double num = v_dot(m, vs, vAMs);
+      const double den = k_dot(s, vAMs, vAMs);
+      const double num = k_dot(s, vs, vAMs);
       omega = num / den;
     }
     v_axpy(m, alpha, vMp, x);
@@ -1101,7 +1449,7 @@ int cr_solve_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ra
     memcpy(r, vs, sizeof(double) * m);
     v_axpy(m, -omega, vAMs, r);
     {
-      const double sq = v_dot(m, r, r);
+      const double sq = k_dot(s, r, r);
       const double sqc = sq * sq_inv_norm_res_ini;
       if (sqc < tol * tol) {
         *conv_ratio = sqrt(sq * sq_inv_norm_res_ini);
@@ -1111,7 +1459,7 @@ int cr_solve_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ra
     }
     double beta;
     {
-      const double tmp1 = v_dot(m, r, vr2);
+      const double tmp1 = k_dot(s, r, vr2);
       beta = tmp1 * alpha / (r_r2 * omega);
     }
     v_scal(m, beta, vp);
@@ -1121,6 +1469,21 @@ int cr_solve_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ra
 done:
   free(buf);
   return ret;
+}
+
+int cr_solve_pcg(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const kop o = {(long)s->n * s->len, s, 0};
+  return pcg_loop(&o, r, x, conv_ratio, iter);
+}
+
+int cr_solve_pbicgstab(const cr_system* s, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const kop o = {(long)s->n * s->len, s, 0};
+  return pbicgstab_loop(&o, r, x, conv_ratio, iter);
+}
+
+int cr_ms_solve(const cr_msys* ms, int method, double* r, double* x, double* conv_ratio, unsigned* iter) {
+  const kop o = {ms->off[ms->nseg], 0, ms};
+  return method == 0 ? pcg_loop(&o, r, x, conv_ratio, iter) : pbicgstab_loop(&o, r, x, conv_ratio, iter);
 }
 
 /* Solve_CG (src/ls/solver_ls_iter.cpp:31-97): no preconditioner; guard 1e-60 returns ||r0|| as conv_ratio;

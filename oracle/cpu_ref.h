@@ -160,6 +160,40 @@ int cr_assemble_stvk2d_beuler(int nnode, const double* coords, int nelem, const 
                               const double* velo_pre, const unsigned* rowptr, const unsigned* colind, double* crs,
                               double* dia, double* res);
 
+/* Multi-segment systems (LsSol::CLinearSystem with several segments, src/ls/linearsystem.cpp; CPreconditioner_ILU
+ * over them, src/ls/preconditioner.cpp; the couplings' factors CMatFrac_BlkCrs, src/matvec/matfrac_blkcrs.cpp).
+ * A block matrix of the system: nrow x ncol blocks of bi x bj, CRS rows (a diagonal segment: square, the
+ * diagonal kept apart in dia, diaind = first upper entry of each row of its factor). */
+typedef struct {
+  int nrow, ncol, bi, bj;
+  const unsigned *rowptr, *colind;
+  double* val;
+  double* dia;
+  const unsigned* diaind;
+} cr_blk;
+
+#define CR_MAXSEG 8
+typedef struct {
+  int nseg;
+  int nblk[CR_MAXSEG], len[CR_MAXSEG];
+  long off[CR_MAXSEG + 1];           /* dof offset of each segment in a concatenated vector */
+  cr_blk* a[CR_MAXSEG * CR_MAXSEG];  /* a[i*nseg + j]: matrix (i, j), NULL where absent; a[i*nseg+i] diagonal */
+  cr_blk* f[CR_MAXSEG * CR_MAXSEG];  /* their ILU(0) factors (same patterns) */
+} cr_msys;
+
+/* CMat_BlkCrs::MatVec (src/matvec/mat_blkcrs.cpp:955-1175), not transposed */
+void cr_blk_matvec(const cr_blk* A, double alpha, const double* x, double beta, double* y);
+/* FinalizeMarge over the segments; flags/res concatenated. Returns the residual norm. */
+double cr_ms_set_bc(cr_msys* s, const int* flags, double* res);
+/* CLinearSystem::MATVEC over the segments */
+void cr_ms_matvec(const cr_msys* s, double alpha, const double* x, double beta, double* y);
+/* CPreconditioner_ILU::SetValue (ILU(0)): f from a, factored; 1 ok, 0 failed */
+int cr_ms_ilu0(cr_msys* s);
+/* CPreconditioner_ILU::SolvePrecond, in place */
+void cr_ms_precond(const cr_msys* s, double* v);
+/* Solve_PCG (method 0) / Solve_PBiCGSTAB (1) over the segments with the ILU(0) of f */
+int cr_ms_solve(const cr_msys* s, int method, double* res, double* upd, double* conv_ratio, unsigned* iter);
+
 #ifdef __cplusplus
 }
 #endif

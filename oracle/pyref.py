@@ -985,3 +985,249 @@ class EqnFlow:
 
     def fields(self):
         return self.u.copy(), self.v.copy(), self.a.copy()
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# Multi-segment systems (LsSol::CLinearSystem with several segments, CPreconditioner_ILU over them).
+# A system is described by plain arrays: nblk[i], lens[i] per segment and mats = {(i, j): dict(rowptr, colind, val
+# [, dia])}; (i, i) is a diagonal segment (off-diagonal blocks in CRS, diagonal blocks in dia). Vectors and Dirichlet
+# flags are concatenated over the segments.
+
+def ms_offsets(nblk, lens):
+    off = [0]
+    for n, l in zip(nblk, lens):
+        off.append(off[-1] + int(n) * int(l))
+    return off
+
+
+def _diaind(rowptr, colind):
+    n = rowptr.size - 1
+    out = np.zeros(n, np.uint32)
+    for i in range(n):
+        row = colind[rowptr[i]:rowptr[i + 1]]
+        out[i] = rowptr[i] + int(np.searchsorted(row, i))
+    return out
+
+
+def ms_copy(mats):
+    return {k: {f: np.array(v, copy=True) for f, v in m.items()} for k, m in mats.items()}
+
+
+class _CrBlk(C.Structure):
+    _fields_ = [("nrow", C.c_int), ("ncol", C.c_int), ("bi", C.c_int), ("bj", C.c_int), ("rowptr", C.c_void_p),
+                ("colind", C.c_void_p), ("val", C.c_void_p), ("dia", C.c_void_p), ("diaind", C.c_void_p)]
+
+
+class _CrMsys(C.Structure):
+    _fields_ = [("nseg", C.c_int), ("nblk", C.c_int * 8), ("len", C.c_int * 8), ("off", C.c_long * 9),
+                ("a", C.POINTER(_CrBlk) * 64), ("f", C.POINTER(_CrBlk) * 64)]
+
+
+class CpuMultiSeg:
+    """The restatement (cpu_ref.c, cr_ms_*) over a multi-segment system; keeps its own copy of the matrices."""
+
+    def __init__(self, nblk, lens, mats):
+        self.nblk, self.lens = [int(x) for x in nblk], [int(x) for x in lens]
+        self.nseg = len(self.nblk)
+        self.off = ms_offsets(self.nblk, self.lens)
+        self.m = self.off[-1]
+        self.mats = ms_copy(mats)
+        for (i, j), a in self.mats.items():
+            a["rowptr"] = np.ascontiguousarray(a["rowptr"], np.uint32)
+            a["colind"] = np.ascontiguousarray(a["colind"], np.uint32)
+            a["val"] = _f64(a["val"])
+            if i == j:
+                a["dia"] = _f64(a["dia"])
+                a["diaind"] = _diaind(a["rowptr"], a["colind"])
+        self.fac = {k: {"val": np.zeros_like(a["val"]), "dia": (np.zeros_like(a["dia"]) if "dia" in a else None)}
+                    for k, a in self.mats.items()}
+        self._keep = []
+        self.s = _CrMsys()
+        self.s.nseg = self.nseg
+        for i in range(self.nseg):
+            self.s.nblk[i], self.s.len[i] = self.nblk[i], self.lens[i]
+        for i in range(self.nseg + 1):
+            self.s.off[i] = self.off[i]
+        ptr = lambda x: None if x is None else x.ctypes.data
+        for (i, j), a in self.mats.items():
+            for which, vals in (("a", a), ("f", self.fac[(i, j)])):
+                b = _CrBlk(self.nblk[i], self.nblk[j], self.lens[i], self.lens[j], ptr(a["rowptr"]), ptr(a["colind"]),
+                           ptr(vals["val"]), ptr(vals.get("dia")), ptr(a.get("diaind")))
+                self._keep.append(b)
+                getattr(self.s, which)[i * self.nseg + j] = C.pointer(b)
+        lib = cpu_ref()
+        lib.cr_ms_set_bc.restype = C.c_double
+
+    def set_bc(self, flags, res):
+        """FinalizeMarge: Dirichlet treatment of the matrices (in place); returns (norm, residual)."""
+        fl = _i32(flags)
+        r = _f64(res).copy()
+        nrm = cpu_ref().cr_ms_set_bc(C.byref(self.s), fl.ctypes.data_as(C.c_void_p), r.ctypes.data_as(C.c_void_p))
+        return nrm, r
+
+    def matvec(self, alpha, x, beta, y):
+        y = _f64(y).copy()
+        cpu_ref().cr_ms_matvec(C.byref(self.s), C.c_double(alpha), _f64(x).ctypes.data_as(C.c_void_p),
+                               C.c_double(beta), y.ctypes.data_as(C.c_void_p))
+        return y
+
+    def ilu0(self):
+        return bool(cpu_ref().cr_ms_ilu0(C.byref(self.s)))
+
+    def precond(self, v):
+        v = _f64(v).copy()
+        cpu_ref().cr_ms_precond(C.byref(self.s), v.ctypes.data_as(C.c_void_p))
+        return v
+
+    def solve(self, method, res, tol, max_iter):
+        """Solve_PCG ("pcg") / Solve_PBiCGSTAB ("pbicgstab") with the ILU(0) of ilu0(): (ok, conv, iter, x, r)."""
+        r = _f64(res).copy()
+        x = np.zeros_like(r)
+        conv, it = C.c_double(tol), C.c_uint(max_iter)
+        ok = cpu_ref().cr_ms_solve(C.byref(self.s), 0 if method == "pcg" else 1, r.ctypes.data_as(C.c_void_p),
+                                   x.ctypes.data_as(C.c_void_p), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, x, r
+
+
+def _ref_ms_lib():
+    lib = reference()
+    if not getattr(lib, "_ms_ready", False):
+        lib.ref_ms_create.restype = C.c_void_p
+        lib.ref_ms_hyper2d.restype = C.c_void_p
+        lib.ref_ms_hyper2d.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p] + [C.c_double] * 5 + \
+            [C.c_void_p]
+        lib.ref_ms_destroy.argtypes = [C.c_void_p]
+        lib.ref_ms_ncrs.restype = C.c_long
+        lib.ref_ms_ncrs.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.ref_ms_finalize.restype = C.c_double
+        lib.ref_ms_finalize.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ref_ms_matvec.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int]
+        lib.ref_ms_dot.restype = C.c_double
+        lib.ref_ms_dot.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib._ms_ready = True
+    return lib
+
+
+class RefMultiSeg:
+    """The reference's own multi-segment CLinearSystem (oracle/ref_msys.cpp)."""
+
+    def __init__(self, h):
+        self.lib = _ref_ms_lib()
+        self.h = C.c_void_p(h)
+        nblk = np.zeros(8, np.int32)
+        lens = np.zeros(8, np.int32)
+        self.nseg = self.lib.ref_ms_info(self.h, nblk.ctypes.data_as(C.c_void_p), lens.ctypes.data_as(C.c_void_p))
+        self.nblk = [int(x) for x in nblk[:self.nseg]]
+        self.lens = [int(x) for x in lens[:self.nseg]]
+        self.off = ms_offsets(self.nblk, self.lens)
+        self.m = self.off[-1]
+
+    @classmethod
+    def create(cls, nblk, lens, patterns):
+        """patterns: {(i, j): (rowptr, colind)}; (i, i) the off-diagonal pattern of a diagonal segment."""
+        lib = _ref_ms_lib()
+        nb, ln = _i32(nblk), _i32(lens)
+        h = lib.ref_ms_create(len(nblk), nb.ctypes.data_as(C.c_void_p), ln.ctypes.data_as(C.c_void_p))
+        self = cls(h)
+        for (i, j), (rp, ci) in sorted(patterns.items()):
+            rp = np.ascontiguousarray(rp, np.uint32)
+            ci = np.ascontiguousarray(ci, np.uint32)
+            lib.ref_ms_add_pattern(self.h, i, j, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p))
+        lib.ref_ms_initialize(self.h)
+        return self
+
+    @classmethod
+    def hyper2d(cls, coords, lnods, fixed, c1, c2, rho, gx, gy):
+        lib = _ref_ms_lib()
+        co, ln, fx = _f64(coords), _i32(lnods), _i32(fixed)
+        nrm = C.c_double(0.0)
+        h = lib.ref_ms_hyper2d(co.size // 2, co.ctypes.data_as(C.c_void_p), ln.size // 3,
+                               ln.ctypes.data_as(C.c_void_p), fx.ctypes.data_as(C.c_void_p), c1, c2, rho, gx, gy,
+                               C.byref(nrm))
+        self = cls(h)
+        self.res_norm = nrm.value
+        return self
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_ms_destroy(self.h)
+            self.h = None
+
+    def pairs(self):
+        return [(i, j) for i in range(self.nseg) for j in range(self.nseg) if self.lib.ref_ms_ncrs(self.h, i, j) >= 0]
+
+    def matrices(self):
+        """{(i, j): dict(rowptr, colind, val[, dia])} of the current values."""
+        out = {}
+        for (i, j) in self.pairs():
+            ncrs = self.lib.ref_ms_ncrs(self.h, i, j)
+            rp = np.zeros(self.nblk[i] + 1, np.uint32)
+            ci = np.zeros(max(ncrs, 1), np.uint32)
+            self.lib.ref_ms_get_pattern(self.h, i, j, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p))
+            bs = self.lens[i] * self.lens[j]
+            val = np.zeros(max(ncrs, 1) * bs)
+            dia = np.zeros(self.nblk[i] * bs) if i == j else None
+            self.lib.ref_ms_get_values(self.h, i, j, val.ctypes.data_as(C.c_void_p),
+                                       None if dia is None else dia.ctypes.data_as(C.c_void_p))
+            m = {"rowptr": rp, "colind": ci[:ncrs], "val": val[:ncrs * bs]}
+            if dia is not None:
+                m["dia"] = dia
+            out[(i, j)] = m
+        return out
+
+    def set_values(self, mats):
+        for (i, j), m in mats.items():
+            val = _f64(m["val"]) if m["val"].size else np.zeros(1)
+            dia = _f64(m["dia"]) if i == j else None
+            self.lib.ref_ms_set_values(self.h, i, j, val.ctypes.data_as(C.c_void_p),
+                                       None if dia is None else dia.ctypes.data_as(C.c_void_p))
+
+    def set_vector(self, iv, v):
+        v = _f64(v)
+        self.lib.ref_ms_set_vector(self.h, iv, v.ctypes.data_as(C.c_void_p))
+
+    def vector(self, iv):
+        v = np.zeros(self.m)
+        self.lib.ref_ms_get_vector(self.h, iv, v.ctypes.data_as(C.c_void_p))
+        return v
+
+    def finalize(self, flags):
+        fl = _i32(flags)
+        return self.lib.ref_ms_finalize(self.h, fl.ctypes.data_as(C.c_void_p))
+
+    def factor(self):
+        """ILU(0) factor values as CPreconditioner_ILU computes them: (ok, {(i, j): val}, [dia per segment])."""
+        pairs = self.pairs()
+        ns = self.nseg
+        vals = {k: np.zeros(max(self.lib.ref_ms_ncrs(self.h, *k), 1) * self.lens[k[0]] * self.lens[k[1]])
+                for k in pairs}
+        dias = [np.zeros(self.nblk[i] * self.lens[i] ** 2) for i in range(ns)]
+        vp = (C.c_void_p * (ns * ns))()
+        dp = (C.c_void_p * ns)()
+        for (i, j), v in vals.items():
+            vp[i * ns + j] = v.ctypes.data
+        for i in range(ns):
+            dp[i] = dias[i].ctypes.data
+        ok = self.lib.ref_ms_factor(self.h, vp, dp)
+        for k in pairs:
+            vals[k] = vals[k][:self.lib.ref_ms_ncrs(self.h, *k) * self.lens[k[0]] * self.lens[k[1]]]
+        return bool(ok), vals, dias
+
+    def precond(self, v):
+        self.set_vector(0, v)
+        ok = self.lib.ref_ms_precond(self.h, 0)
+        assert ok
+        return self.vector(0)
+
+    def matvec(self, alpha, x, beta, y):
+        self.set_vector(0, x)
+        self.set_vector(1, y)
+        self.lib.ref_ms_matvec(self.h, alpha, 0, beta, 1)
+        return self.vector(1)
+
+    def solve(self, method, res, tol, max_iter):
+        """(ok, conv, iter, x, r) of Solve_PCG / Solve_PBiCGSTAB from the residual res."""
+        self.set_vector(-1, res)
+        conv, it = C.c_double(tol), C.c_uint(max_iter)
+        ok = self.lib.ref_ms_solve(self.h, 0 if method == "pcg" else 1, C.byref(conv), C.byref(it))
+        return ok == 1, conv.value, it.value, self.vector(-2), self.vector(-1)
