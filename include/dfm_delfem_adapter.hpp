@@ -11,6 +11,13 @@
 // (AddLinSys_*) merge into it on the host as they always do; Upload installs its matrix pattern, values and
 // residual in a device system (dfm_sys_create_pattern, dfm_sys_set_matrix), the device factors and solves, and
 // DownloadUpdate puts the solution back in its update vector for UpdateValueOfField.
+//
+// CLinearSystemGPU_MultiSeg is the same seam for a linear system of several segments (LsSol::CLinearSystem,
+// include/delfem/ls/linearsystem.h:51-173), e.g. the displacement (CORNER, BUBBLE) and lambda segments that
+// CLinearSystem_Field::AddPattern_Field(id_disp) / AddPattern_Field(id_lambda, id_disp) make for a P1b/P1 element
+// routine: the reference's element routines fill the host system, Upload mirrors every matrix, the Dirichlet flags
+// and the residual into a device multi-segment system, the device factors (CPreconditioner_ILU, ILU(0)) and solves,
+// DownloadUpdate writes the update back.
 #ifndef DFM_DELFEM_ADAPTER_HPP
 #define DFM_DELFEM_ADAPTER_HPP
 
@@ -18,7 +25,9 @@
 #include <vector>
 
 #include "delfem/femls/linearsystem_field.h"
+#include "delfem/ls/linearsystem.h"
 #include "delfem/ls/linearsystem_interface_solver.h"
+#include "delfem/matvec/bcflag_blk.h"
 #include "delfem/matvec/matdia_blkcrs.h"
 #include "delfem/matvec/vector_blk.h"
 #include "dfm_gpu.h"
@@ -142,6 +151,114 @@ class CLinearSystemGPU_Field : public Fem::Ls::CLinearSystem_Field {
   int lev_;
   int len_ = 1;
   dfm_sys sys_ = nullptr;
+};
+
+class CLinearSystemGPU_MultiSeg {
+ public:
+  explicit CLinearSystemGPU_MultiSeg(dfm_ctx ctx) : ctx_(ctx) {}
+  ~CLinearSystemGPU_MultiSeg() {
+    if (ms_) dfm_msys_destroy(ms_);
+  }
+  CLinearSystemGPU_MultiSeg(const CLinearSystemGPU_MultiSeg&) = delete;
+  CLinearSystemGPU_MultiSeg& operator=(const CLinearSystemGPU_MultiSeg&) = delete;
+  // After InitializeMarge, the element routines and FinalizeMarge on ls: its matrices (patterns at the first call),
+  // Dirichlet flags and residual into the device system
+  void Upload(LsSol::CLinearSystem& ls) {
+    const unsigned int ns = ls.GetNLinSysSeg();
+    if (!ms_) {
+      std::vector<int> nblk(ns), len(ns);
+      for (unsigned int i = 0; i < ns; i++) {
+        nblk[i] = (int)ls.GetVector(-1, i).NBlk();
+        len[i] = ls.GetVector(-1, i).Len();
+      }
+      throw_if(dfm_msys_create(ctx_, (int)ns, nblk.data(), len.data(), &ms_));
+      for (unsigned int i = 0; i < ns; i++)
+        for (unsigned int j = 0; j < ns; j++) {
+          if (!ls.IsMatrix(i, j)) continue;
+          const MatVec::CMat_BlkCrs& a = matrix(ls, i, j);
+          std::vector<unsigned> rp((size_t)nblk[i] + 1, 0), ci;
+          for (int r = 0; r < nblk[i]; r++) {
+            unsigned int np = 0;
+            const unsigned int* ind = a.GetPtrIndPSuP(r, np);
+            ci.insert(ci.end(), ind, ind + np);
+            rp[r + 1] = rp[r] + np;
+          }
+          ci.push_back(0);  // a valid pointer also for an empty pattern
+          throw_if(dfm_msys_add_pattern(ms_, (int)i, (int)j, rp.data(), ci.data()));
+        }
+      len_ = len;
+    }
+    std::vector<double> res, vals, dia;
+    std::vector<int> flags;
+    for (unsigned int i = 0; i < ns; i++) {
+      const MatVec::CVector_Blk& r = ls.GetVector(-1, i);
+      const MatVec::CBCFlag& bc = ls.GetBCFlag(i);
+      for (unsigned int b = 0; b < r.NBlk(); b++)
+        for (int d = 0; d < len_[i]; d++) {
+          res.push_back(r.GetValue(b, d));
+          flags.push_back(bc.GetBCFlag(b, d));
+        }
+      for (unsigned int j = 0; j < ns; j++) {
+        if (!ls.IsMatrix(i, j)) continue;
+        MatVec::CMat_BlkCrs& a = matrix(ls, i, j);
+        const size_t bs = (size_t)len_[i] * len_[j];
+        vals.clear();
+        dia.clear();
+        for (unsigned int b = 0; b < r.NBlk(); b++) {
+          unsigned int np = 0;
+          const double* v = a.GetPtrValPSuP(b, np);
+          vals.insert(vals.end(), v, v + np * bs);
+          if (i == j) {
+            const double* d = ls.GetMatrix(i).GetPtrValDia(b);
+            dia.insert(dia.end(), d, d + bs);
+          }
+        }
+        vals.push_back(0.0);
+        throw_if(dfm_msys_set_matrix(ms_, (int)i, (int)j, vals.data(), i == j ? dia.data() : nullptr));
+      }
+    }
+    throw_if(dfm_msys_set_vector(ms_, -1, res.data()));
+    // FinalizeMarge has run on the host: the device treatment of the same flags leaves the values as they are and
+    // gives the device system its flags
+    throw_if(dfm_msys_finalize(ms_, flags.data(), nullptr));
+  }
+  // CPreconditioner_ILU::SetValue and LsSol::Solve_PBiCGSTAB / Solve_PCG on the device (same in/out meaning)
+  bool SolvePBiCGSTAB(double& conv_ratio, unsigned int& iteration) {
+    throw_if(dfm_msys_factor(ms_));
+    const int rc = dfm_msys_solve(ms_, DFM_METHOD_PBICGSTAB, &conv_ratio, &iteration);
+    throw_if(rc);
+    return rc == 1;
+  }
+  bool SolvePCG(double& conv_ratio, unsigned int& iteration) {
+    throw_if(dfm_msys_factor(ms_));
+    const int rc = dfm_msys_solve(ms_, DFM_METHOD_PCG, &conv_ratio, &iteration);
+    throw_if(rc);
+    return rc == 1;
+  }
+  // the device solution into ls's update vectors (then UpdateValueOfField as usual)
+  void DownloadUpdate(LsSol::CLinearSystem& ls) {
+    std::vector<double> x;
+    size_t m = 0;
+    for (unsigned int i = 0; i < ls.GetNLinSysSeg(); i++) m += (size_t)ls.GetVector(-2, i).NBlk() * len_[i];
+    x.resize(m);
+    throw_if(dfm_msys_get_vector(ms_, -2, x.data()));
+    size_t pos = 0;
+    for (unsigned int i = 0; i < ls.GetNLinSysSeg(); i++) {
+      MatVec::CVector_Blk& u = ls.GetVector(-2, i);
+      for (unsigned int b = 0; b < u.NBlk(); b++)
+        for (int d = 0; d < len_[i]; d++) u.SetValue(b, d, x[pos++]);
+    }
+  }
+  dfm_msys handle() const { return ms_; }
+
+ private:
+  static MatVec::CMat_BlkCrs& matrix(LsSol::CLinearSystem& ls, unsigned int i, unsigned int j) {
+    if (i == j) return ls.GetMatrix(i);
+    return ls.GetMatrix(i, j);
+  }
+  dfm_ctx ctx_;
+  std::vector<int> len_;
+  dfm_msys ms_ = nullptr;
 };
 
 }  // namespace delfem

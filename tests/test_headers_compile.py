@@ -43,6 +43,7 @@ ADAPTER_TU = r"""
 #include "dfm_delfem_adapter.hpp"
 #include "delfem/ls/solver_ls_iter.h"
 #include "delfem/femeqn/eqn_poisson.h"
+#include "delfem/femeqn/eqn_hyper.h"
 #include "delfem/field_world.h"
 
 bool solve_through_the_seams(dfm_ctx ctx, Fem::Field::CFieldWorld& world, unsigned int id_field) {
@@ -65,6 +66,24 @@ bool solve_through_the_seams(dfm_ctx ctx, Fem::Field::CFieldWorld& world, unsign
   iter = 1000;
   ok = LsSol::Solve_PCG(conv, iter, dev) && ok;
   ok = dev.SolvePBiCGSTAB(conv, iter, DFM_FP32_ILU) && ok;
+  return ok;
+}
+
+bool solve_multi_segment(dfm_ctx ctx, Fem::Field::CFieldWorld& world, unsigned int id_disp, unsigned int id_lambda) {
+  // a P1b/P1 system of three segments, assembled by the reference's own Mooney-Rivlin routine on the host
+  Fem::Ls::CLinearSystem_Field ls;
+  ls.AddPattern_Field(id_disp, world);
+  ls.AddPattern_Field(id_lambda, id_disp, world);
+  ls.InitializeMarge();
+  Fem::Eqn::AddLinSys_Hyper2D_Static(ls, 1.0, 0.5, 1.0, 0.0, -0.1, id_disp, id_lambda, world);
+  ls.FinalizeMarge();
+  dfm::delfem::CLinearSystemGPU_MultiSeg dev(ctx);
+  dev.Upload(ls.m_ls);
+  double conv = 1e-9;
+  unsigned int iter = 500;
+  const bool ok = dev.SolvePBiCGSTAB(conv, iter);
+  dev.DownloadUpdate(ls.m_ls);
+  ls.UpdateValueOfField(id_disp, world, Fem::Field::VALUE);
   return ok;
 }
 """
