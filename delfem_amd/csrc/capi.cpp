@@ -853,12 +853,16 @@ static int finish_matrix(Sys& s, hipStream_t st) {
 }  // extern "C"
 
 // The element store of an assembly of s (its lists and its context's buffer, grown as needed); false where the
-// lists are not built or the buffer cannot be had (the per-row gather runs then). Linear elasticity with rows of at
-// most 63 off-diagonal blocks takes the row-group kernel, which needs neither.
+// lists are not built or the buffer cannot be had (the per-row gather runs then). The vector kinds (one dof per
+// dimension: linear elasticity, StVK, their Newmark and backward-Euler forms) with rows of at most 63 off-diagonal
+// blocks take the row-group kernel, which needs neither.
 static bool elem_store(Sys& s, int kind, bool save, ElemStore& es) {
   es = ElemStore{};
-  if (!save && (kind == DFM_SOLID2D || kind == DFM_SOLID3D) && s.nelem > 0 && s.max_rowlen < 64 && s.d_permq.p &&
-      s.d_rowptr0.p && s.d_colind0.p && s.npoel <= 4) {
+  const bool vector_kind = kind == DFM_SOLID2D || kind == DFM_SOLID3D || kind == DFM_STVK2D || kind == DFM_STVK3D ||
+                           kind == DFM_SOLID2D_NEWMARK || kind == DFM_SOLID3D_NEWMARK || kind == DFM_SOLID2D_BEULER ||
+                           kind == DFM_STVK3D_NEWMARK || kind == DFM_STVK2D_NEWMARK || kind == DFM_STVK2D_BEULER;
+  if (!save && vector_kind && s.len == s.ndim && std::getenv("DFM_ASM_ELEMSTORE") == nullptr && s.nelem > 0 &&
+      s.max_rowlen < 64 && s.d_permq.p && s.d_rowptr0.p && s.d_colind0.p && s.npoel <= 4) {
     if (!s.d_elnod.p) {
       std::vector<int> en(s.elsup.size() * 4, -1);
       for (size_t k = 0; k < s.elsup.size(); k++)

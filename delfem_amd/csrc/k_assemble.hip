@@ -686,12 +686,12 @@ struct ElemIn {
   const double* p;
 };
 
+// The node values of an element whose nodes are in E.no.
 template <int KIND, int NDIM, int B>
-__device__ inline void load_elem(const AsmArgs& A, int e, ElemIn<KIND, NDIM, B>& E) {
+__device__ inline void load_elem_nodes(const AsmArgs& A, ElemIn<KIND, NDIM, B>& E) {
   constexpr int NNO = NDIM + 1;
   constexpr bool NM = (KIND == DFM_SOLID2D_NEWMARK || KIND == DFM_SOLID3D_NEWMARK || KIND == DFM_SOLID2D_BEULER ||
                        KIND == DFM_STVK3D_NEWMARK || KIND == DFM_STVK2D_NEWMARK || KIND == DFM_STVK2D_BEULER);
-  for (int c = 0; c < NNO; c++) E.no[c] = A.lnods[(long long)e * NNO + c];
   for (int c = 0; c < NNO; c++) {
     const long long nc = E.no[c];
     for (int d = 0; d < 3; d++) E.co[c][d] = (d < NDIM) ? A.coords[nc * NDIM + d] : 0.0;
@@ -704,6 +704,13 @@ __device__ inline void load_elem(const AsmArgs& A, int e, ElemIn<KIND, NDIM, B>&
         E.aa[c][d] = (d < B) ? A.acc[nc * B + d] : 0.0;
       }
   }
+}
+
+template <int KIND, int NDIM, int B>
+__device__ inline void load_elem(const AsmArgs& A, int e, ElemIn<KIND, NDIM, B>& E) {
+  constexpr int NNO = NDIM + 1;
+  for (int c = 0; c < NNO; c++) E.no[c] = A.lnods[(long long)e * NNO + c];
+  load_elem_nodes<KIND, NDIM, B>(A, E);
   E.p = &A.params[(long long)e * A.pstride];
 }
 
@@ -1057,8 +1064,11 @@ constexpr int rowgroup_field_stride(int nrec, int BS, int NNO, int G) {
   return best;
 }
 
-template <int NDIM, int LPR>
+// KIND: linear elasticity (its rows made block by block, solid*_rows_put) or another vector kind with one degree of
+// freedom per dimension (Newmark, backward Euler, StVK), whose rows come whole from elem_row.
+template <int KIND, int NDIM, int LPR>
 __global__ void __launch_bounds__(64) k_asm_solid_rowgroups(AsmArgs A) {
+  constexpr bool kLinear = KIND == DFM_SOLID2D || KIND == DFM_SOLID3D;
   constexpr int NNO = NDIM + 1, B = NDIM, BS = B * B, G = 64 / LPR, CH = LPR >= 16 ? LPR / 2 : LPR;
   constexpr int FE = NNO * BS, NF = FE + B;  // the node's blocks (a, 0..NNO-1), then its residual row
   constexpr int FSTR = rowgroup_field_stride(CH * G, BS, NNO, G);
@@ -1117,7 +1127,26 @@ __global__ void __launch_bounds__(64) k_asm_solid_rowgroups(AsmArgs A) {
       nn = elnod[kb + CH + s];
     }
     __syncthreads();
-    if (e >= 0) {
+    if (e >= 0 && !kLinear) {
+      // the kind's element row as the element store computes it (same loads, same routine)
+      ElemIn<KIND, NDIM, B> E;
+      int a = 0;
+#pragma unroll
+      for (int c = 0; c < NNO; c++) {
+        E.no[c] = no[c];
+        nod[ati(c, s)] = no[c];
+        if (no[c] == i1) a = c;
+      }
+      load_elem_nodes<KIND, NDIM, B>(A, E);
+      E.p = &A.params[(long long)e * A.pstride];
+      double em[NNO * BS], er[B];
+      elem_row<KIND, NDIM, B, false>(A, E, a, em, er);
+#pragma unroll
+      for (int t = 0; t < NNO * BS; t++) rec[at(t, s)] = em[t];
+#pragma unroll
+      for (int t = 0; t < B; t++) rec[at(FE + t, s)] = er[t];
+    }
+    if (e >= 0 && kLinear) {
       double co[4][3], u[4][3];
       int a = 0;
 #pragma unroll
@@ -1182,16 +1211,16 @@ __global__ void __launch_bounds__(64) k_asm_solid_rowgroups(AsmArgs A) {
   for (int t = 0; t < B; t++) A.res[(size_t)i0 * B + t] = r[t];
 }
 
-template <int NDIM>
+template <int KIND, int NDIM>
 static hipError_t launch_solid_rowgroups(const AsmArgs& a, int lpr, hipStream_t st) {
   if (!a.rowptr0 || !a.colind0 || !a.permq || !a.elnod) return hipErrorInvalidValue;
   const long long groups = 64 / lpr;
   const long long nblk = (a.n + groups - 1) / groups;
   const dim3 grid((unsigned)((nblk + 7) / 8 * 8));
-  if (lpr == 8) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 8>), grid, dim3(64), 0, st, a);
-  else if (lpr == 16) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 16>), grid, dim3(64), 0, st, a);
-  else if (lpr == 32) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 32>), grid, dim3(64), 0, st, a);
-  else if (lpr == 64) hipLaunchKernelGGL((k_asm_solid_rowgroups<NDIM, 64>), grid, dim3(64), 0, st, a);
+  if (lpr == 8) hipLaunchKernelGGL((k_asm_solid_rowgroups<KIND, NDIM, 8>), grid, dim3(64), 0, st, a);
+  else if (lpr == 16) hipLaunchKernelGGL((k_asm_solid_rowgroups<KIND, NDIM, 16>), grid, dim3(64), 0, st, a);
+  else if (lpr == 32) hipLaunchKernelGGL((k_asm_solid_rowgroups<KIND, NDIM, 32>), grid, dim3(64), 0, st, a);
+  else if (lpr == 64) hipLaunchKernelGGL((k_asm_solid_rowgroups<KIND, NDIM, 64>), grid, dim3(64), 0, st, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -1207,8 +1236,9 @@ static hipError_t assemble_kind(const AsmArgs& a, const ElemStore* es, hipStream
   }
   const dim3 ge((unsigned)((es->nelem + 63) / 64));
   const dim3 gq((unsigned)((es->ncrs + 255) / 256));
+  if constexpr (!SAVE && B == NDIM && B > 1)
+    if (es->lpr > 0) return launch_solid_rowgroups<KIND, NDIM>(a, es->lpr, st);
   if (!SAVE && (KIND == DFM_SOLID2D || KIND == DFM_SOLID3D)) {
-    if (es->lpr > 0) return launch_solid_rowgroups<NDIM>(a, es->lpr, st);
     hipLaunchKernelGGL((k_geom_store<NDIM>), ge, dim3(64), 0, st, a, es->nelem, es->ebuf);
     if (es->ncrs > 0)
       hipLaunchKernelGGL((k_gather_blocks_solid<NDIM>), gq, dim3(256), 0, st, es->ncrs, es->eptr, es->eidx,
