@@ -1166,17 +1166,21 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   if (rc != DFM_OK) return rc;
   int nrel = s.h_pinned[F_NREL];
   unsigned before = 0;  // iterations of a mixed-precision loop that was stopped
+  // iterations executed by all loops of this solve (a loop that did not converge ran its whole budget)
+  unsigned total = ok ? it + 1 : maxit;
   if (mixed && s.h_pinned[F_DONE] == 3) {
     // the FP32 matrix limits progress (cond(A) eps32 not small): continue from the FP64 solution with FP64
     // vectors and matrix, keeping the FP32 factor (the preconditioner's precision does not bound the attainable
     // accuracy); max_iter bounds both loops together
     s.last_fallback = 1;
     before = (unsigned)s.h_pinned[F_IT] + 1;
+    total = before;
     PcgBufs v64 = pcg_bufs(s, false);
     v64.f32ilu = true;
     if (before < maxit) {
       rc = pcg_run<double>(s, v64, false, tol, maxit - before, conv, it, ok, true);
       if (rc != DFM_OK) return rc;
+      total += ok ? it + 1 : maxit - before;
     } else {
       ok = false;
     }
@@ -1208,6 +1212,7 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
       double c2 = 0.0;
       rc = pcg_run<double>(s, v64, false, 0.5 * tol, maxit - before, c2, it, ok, true);
       if (rc != DFM_OK) return rc;
+      total += ok ? it + 1 : maxit - before;
       if (!ok) {
         rc = sys_true_residual(s, st, &conv);
         if (rc != DFM_OK) return rc;
@@ -1223,7 +1228,7 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   (void)hipEventDestroy(e1);
   *conv_ratio = conv;
   if (ok) *iteration = it + before;
-  const unsigned its = ok ? it + before + 1 : maxit;
+  const unsigned its = total;
   s.last_ms = ms;
   s.last_inner = its;
   s.last_ms_iter = its ? ms / its : 0.0;
