@@ -678,7 +678,8 @@ def solve_legs(args, N, s, w, ctx):
         nrel, fell_back = s.last_solve_info()
         res[name] = dict(lat=lat, its=its, ok=ok_all, conv=conv, true_rel=true_rel, stats=st, nrel=nrel,
                          fell_back=fell_back, x=s.get_vector(-2))
-        log(f"[bench] {name}: {statistics.median(lat) * 1e3:.1f} ms/solve, iters {its[-1]}, conv {conv:.2e}, "
+        log(f"[bench] {name}: {statistics.median(lat) * 1e3:.1f} ms/solve, iters {its[-1]} (executed {st['iters']}), "
+            f"conv {conv:.2e}, "
             f"true rel residual {true_rel:.2e}, solve {st['ms']:.1f} ms, {st['ms_per_iter'] * 1e3:.1f} us/iter"
             + (f", reliable updates {nrel}" if name == "mixed" else "") + (", fell back to FP64" if fell_back else ""))
     return res
@@ -775,8 +776,9 @@ def run_single(args, N, W, w, ctx, order_kind):
         "p99_ms": pct(head["lat"], 0.99) * 1e3,
         "fp64_p50_ms": pct(fp64["lat"], 0.5) * 1e3,
         "fp64_p99_ms": pct(fp64["lat"], 0.99) * 1e3,
-        "iters_mixed_inner": mixed["its"][-1],
+        "iters_mixed_inner": mixed["stats"]["iters"],
         "iters_fp64": fp64["its"][-1],
+        "iters_executed_fp64": fp64["stats"]["iters"],
         "true_rel_residual_mixed": mixed["true_rel"],
         "true_rel_residual_fp64": fp64["true_rel"],
         "mixed_error_vs_fp64": err,
