@@ -265,6 +265,19 @@ class CpuSystem:
         return cpu_ref().cr_set_bc(self.nnode, self.len, p(self.rowptr), p(self.colind), p(self.crs),
                                    p(self.dia), p(flags), p(self.res))
 
+    def dense(self):
+        """The assembled matrix as a dense array (for checks on small systems)."""
+        n, b = self.nnode, self.len
+        A = np.zeros((n * b, n * b))
+        crs = self.crs.reshape(-1, b, b)
+        dia = self.dia.reshape(-1, b, b)
+        for i in range(n):
+            A[i * b:(i + 1) * b, i * b:(i + 1) * b] = dia[i]
+            for q in range(self.rowptr[i], self.rowptr[i + 1]):
+                j = self.colind[q]
+                A[i * b:(i + 1) * b, j * b:(j + 1) * b] = crs[q]
+        return A
+
     def matvec(self, alpha, x, beta, y):
         y = _f64(y).copy()
         x = _f64(x)

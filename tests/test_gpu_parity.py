@@ -129,11 +129,39 @@ def test_mixed_precision_true_residual(ctx, case):
     b = cpu.res
     r = b - cpu.matvec(1.0, xm, 0.0, np.zeros_like(b))
     assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(b)
-    # the two solutions agree to within cond(A) * tol (cond ~ 1e5 here); both residuals are below 1e-8
-    assert np.linalg.norm(xm - x64) <= 1e-3 * np.linalg.norm(x64)
+    # against the exact solution of the oracle's matrix (dense LU): each error within what its relative residual
+    # allows, kappa_2(A) * ||b - A x|| / ||b||, and the mixed solution no further from it than the FP64 one by more
+    # than that bound
+    A = cpu.dense()
+    xs = np.linalg.solve(A, b)
+    sv = np.linalg.svd(A, compute_uv=False)
+    kappa = sv[0] / sv[-1]
+    for xx in (xm, x64):
+        rel = np.linalg.norm(b - A @ xx) / np.linalg.norm(b)
+        assert np.linalg.norm(xx - xs) <= 1.01 * kappa * rel * np.linalg.norm(xs)
+    e_m = np.linalg.norm(xm - xs) / np.linalg.norm(xs)
+    e_64 = np.linalg.norm(x64 - xs) / np.linalg.norm(xs)
+    print(f"mixed error {e_m:.2e}, fp64 error {e_64:.2e}, kappa {kappa:.2e}", flush=True)
+    assert e_m <= e_64 + kappa * 1.01e-8
     # mixed precision must not cost many more iterations than FP64 (reliable updates keep the Krylov space)
     _, _, it64 = gs.solve_pcg(1e-8, 5000, N.FP64)
     assert it <= 1.5 * it64 + 10, (it, it64)
+
+
+@pytest.mark.parametrize("oname,ordering", ORDERS[:3], ids=[o[0] for o in ORDERS[:3]])
+@pytest.mark.parametrize("case", [KINDS[0], KINDS[3], KINDS[4]], ids=["b1", "b2", "b3"])
+def test_update_field_bit_exact(ctx, case, oname, ordering):
+    """UpdateValueOfField (src/femls/linearsystem_field.cpp:716-794): value += 1.0 * update per dof, which is
+    value + update exactly (the reference's CNodeAry::AddValueToNodeSegment, src/femfield/node_ary.cpp:223-260)."""
+    cpu, gs, order, _ = setup(N, ctx, case, ordering)
+    n = cpu.nnode * cpu.len
+    rng = np.random.default_rng(41)
+    u = rng.standard_normal(n)
+    du = rng.standard_normal(n) * 1e-3
+    gs.set_field(u)
+    gs.set_vector(-2, du)
+    gs.update_field()
+    assert np.array_equal(gs.get_field(), u + du)
 
 
 def test_pcg_not_converged_contract(ctx):

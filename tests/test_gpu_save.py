@@ -181,6 +181,12 @@ def test_saved_equals_incremental_large(ctx):
     u_inc = inc.get_field()
     assert np.array_equal(u_save[fl == 1], u[fl == 1])
     assert np.abs(u_save - u_inc).max() <= 1e-8 * np.abs(u_inc).max()
+    # and against the oracle's own saved solve (its ILU(0) under the same ordering, PCG to the same tolerance)
+    f = cpu.ilu0(order)
+    cok, _, _, xc, _ = cpu.solve(f, "pcg", 1e-12, 20000, res=res)
+    assert cok
+    _, u_cpu = cpu.update_field_save(xc, u)
+    assert np.abs(u_save - u_cpu).max() <= 1e-8 * np.abs(u_cpu).max()
 
 
 def test_saved_edge_cases(ctx):

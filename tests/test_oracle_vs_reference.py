@@ -678,3 +678,20 @@ def test_stvk3d_newmark_assembly_bit_exact(ndim, per_element, initial):
     assert np.array_equal(crs, cpu.crs) and np.array_equal(dia, cpu.dia)
     assert np.array_equal(ref.vector(-1), cpu.res), np.abs(ref.vector(-1) - cpu.res).max()
     assert rn == rn2
+
+
+@pytest.mark.parametrize("ndim,length", [(2, 1), (2, 2), (3, 3)])
+def test_update_value_of_field_is_value_plus_update(ndim, length):
+    """CLinearSystem_Field::UpdateValueOfField (src/femls/linearsystem_field.cpp:716-794) adds 1.0 * update to the
+    field values (CNodeAry::AddValueToNodeSegment, src/femfield/node_ary.cpp:223-260): the reference's result is
+    value + update bit for bit, which is what dfm_sys_update_field is checked against."""
+    co, ln = mesh2d(6, 4) if ndim == 2 else mesh3d(2, 2, 2)
+    ref = P.RefSystem(ndim, co, ln, length)
+    n = co.size // ndim
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(n * length)
+    du = rng.standard_normal(n * length) * 1e-3
+    ref.set_field(u)
+    ref.set_vector(-2, du)
+    ref.update_field()
+    assert np.array_equal(ref.get_field(), u + du)
