@@ -158,6 +158,8 @@ Sys::~Sys() {
   delete last_asm;
   if (h_pinned) DevAlloc::release_host(h_pinned);
   if (h_pinned_d) DevAlloc::release_host(h_pinned_d);
+  for (hipEvent_t ev : poll_ev)
+    if (ev) (void)hipEventDestroy(ev);
 }
 
 template <typename T>
@@ -365,8 +367,8 @@ static hipError_t sys_alloc_device(Sys& s) {
   if ((e = s.d_counter.alloc(8)) != hipSuccess) return e;
   if ((e = s.d_scal.alloc(kNScal)) != hipSuccess) return e;
   if ((e = s.d_iflag.alloc(8)) != hipSuccess) return e;
-  if ((e = DevAlloc::alloc_host((void**)&s.h_pinned, 16 * sizeof(int))) != hipSuccess) return e;
-  if ((e = DevAlloc::alloc_host((void**)&s.h_pinned_d, kNScal * sizeof(double))) != hipSuccess) return e;
+  if ((e = DevAlloc::alloc_host((void**)&s.h_pinned, 2 * 16 * sizeof(int))) != hipSuccess) return e;
+  if ((e = DevAlloc::alloc_host((void**)&s.h_pinned_d, 2 * kNScal * sizeof(double))) != hipSuccess) return e;
   if ((e = hipMemsetAsync(s.d_counter.p, 0, 8 * sizeof(unsigned), st)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(s.d_bc.p, 0, m * sizeof(int), st)) != hipSuccess) return e;
   if ((e = launch_fill(s.d_u.p, 0.0, m, st)) != hipSuccess) return e;

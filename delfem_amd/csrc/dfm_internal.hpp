@@ -230,8 +230,10 @@ struct Sys {
   DBuf<unsigned> d_counter;          // last-block counters
   DBuf<double> d_scal;               // device scalars of the Krylov loops
   DBuf<int> d_iflag;                 // device int scalars (done flag, iteration)
-  int* h_pinned = nullptr;           // host-pinned mirror of the done flag + iteration
-  double* h_pinned_d = nullptr;
+  int* h_pinned = nullptr;           // host-pinned mirror of the done flag + iteration: two slots of 16 ints
+  double* h_pinned_d = nullptr;      // and of the device scalars: two slots of kNScal doubles
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};  // the PCG loop's status writes, one per slot (made on first use)
+  int poll_seq = 0;                            // sequence number of the last status write
 
   // Dirichlet flags in the original numbering (d_bc holds them in the device numbering)
   std::vector<int> bc0;

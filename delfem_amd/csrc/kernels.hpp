@@ -39,7 +39,7 @@ struct AsmArgs {
 // The element store of an assembly: ebuf holds nelem * NNO rows of NNO*B*B + B doubles; per off-diagonal entry q the
 // list eidx[eptr[q] .. eptr[q+1]) of (e*NNO + a)*NNO + c in ascending element order; per node-element incidence
 // (elsup order) eloc = e*NNO + a. Null: the per-row gather, into a matrix and residual zeroed beforehand.
-// lpr > 0 (linear elasticity): the row-group kernel, lpr lanes per node, runs instead and needs neither lists nor
+// lpr > 0 (the vector kinds): the row-group kernel, lpr lanes per node, runs instead and needs neither lists nor
 // buffer.
 struct ElemStore {
   int nelem;

@@ -34,6 +34,18 @@ __global__ void k_solver_init(double* sc, int* fl, double tol2, int maxit, doubl
   }
 }
 
+// The loop status into a pinned host slot: the flags (8 ints) and the scalars, then, after a system-scope fence,
+// the chunk's sequence number at int 15, so that the host takes the slot as the chunk's only when it reads that
+// number. A kernel on the loop's own queue, rather than a copy, which would wait for another engine.
+__global__ void __launch_bounds__(64) k_status_out(const int* fl, const double* sc, int* hi, double* hd, int seq) {
+  const int t = threadIdx.x;
+  if (t < 8) hi[t] = fl[t];
+  if (t < kNScal) hd[t] = sc[t];
+  __threadfence_system();
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(hi + 15, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(NT) k_vfill(T* x, T v, long long m, const int* done) {
   if (done && *done) return;
@@ -1105,30 +1117,79 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
     const int rc = graph_chunk(s, gk, graph_key(s, v), [&]() { return pcg_iteration<T>(s, v, mixed, st); }, st);
     if (rc != DFM_OK) return rc;
   }
-  unsigned launched = 0, chunk = 32, since = 0;
-  double prev_ratio2 = 1.0;
-  while (launched < maxit) {
-    unsigned k = std::min(chunk, maxit - launched);
+  // Chunks of iterations, each followed by a status write into one of two pinned slots (k_status_out) and an
+  // event. The next chunk is enqueued before the status of the current one is waited for, so the device never
+  // idles while the host decides, and a chunk boundary costs one small kernel: the chunks can stay short (one or
+  // two graphs: half the iterations estimated to remain, at most two graphs), which bounds the no-op iterations
+  // (kernels that see the done flag and return, about 11 us each on the 2.7 M-dof elasticity system) queued past
+  // convergence to the two chunks in flight. Longer chunks sized from the early decay rate overshot by hundreds
+  // of iterations: preconditioned CG converges faster later than its first iterations predict.
+  for (hipEvent_t& ev : s.poll_ev)
+    if (!ev) DFM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const unsigned gc = use_graph ? (unsigned)s.gchunk : 1u;
+  unsigned ksz[2] = {0, 0};
+  int seq[2] = {0, 0};
+  auto enqueue_chunk = [&](unsigned k, int slot) -> int {
     unsigned t = 0;
     if (use_graph)
-      for (; t + (unsigned)s.gchunk <= k; t += s.gchunk) DFM_HIP(hipGraphLaunch(s.gexec[gk], st));
+      for (; t + gc <= k; t += gc) DFM_HIP(hipGraphLaunch(s.gexec[gk], st));
     for (; t < k; t++) {
       const int rc = pcg_iteration<T>(s, v, mixed, st);
       if (rc != DFM_OK) return rc;
     }
-    launched += k;
-    since += k;
-    const int rc = poll(s, st);
-    if (rc != DFM_OK) return rc;
-    if (s.h_pinned[F_DONE]) break;
-    const double ratio2 = s.h_pinned_d[S_RR] * s.h_pinned_d[S_INV0];
-    chunk = next_chunk(ratio2, prev_ratio2, since, tol * tol, 32, 1024);
+    seq[slot] = ++s.poll_seq;
+    hipLaunchKernelGGL(k_status_out, dim3(1), dim3(64), 0, st, (const int*)fl, (const double*)sc, s.h_pinned + 16 * slot,
+                       s.h_pinned_d + kNScal * slot, seq[slot]);
+    DFM_HIP(hipGetLastError());
+    DFM_HIP(hipEventRecord(s.poll_ev[slot], st));
+    ksz[slot] = k;
+    return DFM_OK;
+  };
+  auto whole = [&](double k) {
+    const unsigned lo = std::max(gc, 16u), hi = 2 * lo;
+    const unsigned u = (unsigned)std::min((double)hi, std::max((double)lo, k));
+    return std::max(lo, u / gc * gc);
+  };
+  // cur: the slot of the oldest chunk whose status is unread; inflight: such chunks (at most two); next: the size
+  // of the chunk to queue ahead, 0 when the chunk in flight is expected to end the solve (then it is waited for
+  // before anything more is queued)
+  unsigned launched = 0, since = 0, next = whole(0.0);
+  double prev_ratio2 = 1.0;
+  int cur = 0, inflight = 0;
+  for (;;) {
+    if (inflight < 2 && launched < maxit && (inflight == 0 || next > 0)) {
+      const unsigned k = std::min(whole((double)next), maxit - launched);
+      const int rc = enqueue_chunk(k, (cur + inflight) & 1);
+      if (rc != DFM_OK) return rc;
+      launched += k;
+      inflight++;
+    }
+    if (inflight == 0) break;  // max_iter reached, every status read
+    DFM_HIP(hipEventSynchronize(s.poll_ev[cur]));
+    const int* f = s.h_pinned + 16 * cur;
+    const double* d = s.h_pinned_d + kNScal * cur;
+    if (__atomic_load_n(f + 15, __ATOMIC_ACQUIRE) != seq[cur]) {
+      // the slot does not show this chunk's status yet: read it with a copy (everything queued runs first)
+      const int rc = poll(s, st);
+      if (rc != DFM_OK) return rc;
+      f = s.h_pinned;
+      d = s.h_pinned_d;
+    }
+    inflight--;
+    if (f[F_DONE]) break;
+    since += ksz[cur];
+    const double ratio2 = d[S_RR] * d[S_INV0];
+    const double left = (double)next_chunk(ratio2, prev_ratio2, since, tol * tol, 0, 1u << 30) -
+                        (inflight ? (double)ksz[cur ^ 1] : 0.0);
+    next = left > 0.0 ? whole(0.5 * left) : 0u;
     if (ratio2 < prev_ratio2) {
       prev_ratio2 = ratio2;
       since = 0;
     }
+    cur ^= 1;
   }
-  if (launched == 0) {
+  {
+    // everything queued has run: the final status into slot 0, where the callers read it
     const int rc = poll(s, st);
     if (rc != DFM_OK) return rc;
   }
