@@ -202,7 +202,10 @@ def run_c5_batch(args, N, W):
     log(f"[bench] c5 x{args.instances} ndof={base.ndof} setup {time.perf_counter() - t0:.1f}s")
     res = {}
     # precision per system (DFM_MIXED_AUTO): for these vector systems the FP64 loop with its true residual checked
-    for prec, name in [(N.MIXED_AUTO, "mixed"), (N.FP64, "fp64")]:
+    legs = [(N.MIXED_AUTO, "mixed"), (N.FP64, "fp64")]
+    if args.c5_extra:  # a third leg in another precision, reported beside the two
+        legs.append(({"fp32ilu": N.FP32_ILU, "mixed32": N.MIXED}[args.c5_extra], args.c5_extra))
+    for prec, name in legs:
         for s in systems:
             s.set_field(None)
             s.set_params(w["params"])
@@ -247,6 +250,8 @@ def run_c5_batch(args, N, W):
         "fp64_value": work / res["fp64"]["t"],
         "mixed": res["mixed"], "fp64": res["fp64"],
     }
+    if args.c5_extra:
+        out["extra_" + args.c5_extra] = dict(res[args.c5_extra], value=work / res[args.c5_extra]["t"])
     for s in systems:
         s.close()
     ctx.close()
@@ -856,6 +861,8 @@ def main():
     ap.add_argument("--instances", type=int, default=256, help="c5: instances in the batch")
     ap.add_argument("--load-steps", type=int, default=10, help="c5: load steps")
     ap.add_argument("--lev", type=int, default=0, help="c5: ILU fill level of the inner solves")
+    ap.add_argument("--c5-extra", default="", choices=["", "fp32ilu", "mixed32"],
+                    help="c5: a third leg with DFM_FP32_ILU (fp32ilu) or DFM_MIXED (mixed32) inner solves")
     ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=100.0, help="c4: arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=64, help="c4: systems solved at once")
