@@ -191,7 +191,9 @@ __global__ void __launch_bounds__(NT) k_b_judge_beta(const SegDev* __restrict__ 
     }
     rmax = rt;
   }
-  if (it + 1 >= fl[F_MAXIT]) {
+  // the last iteration max_iter allows: stop, unless the recursive residual says converged (its true-residual
+  // check then runs in one more iteration, the solution reported being this one)
+  if (it + 1 >= fl[F_MAXIT] && !(it + 1 == fl[F_MAXIT] && rr * sc[S_INV0] < sc[S_TOL2])) {
     sc[S_CONV] = sqrt(rr * sc[S_INV0]);
     fl[F_DONE] = 2;
     return;

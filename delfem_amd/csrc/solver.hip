@@ -493,7 +493,10 @@ __global__ void __launch_bounds__(kRedNT) k_pcg_judge_beta(const double* slot_rr
       }
       rmax = rt[0];
     }
-    if (it + 1 >= fl[F_MAXIT]) {
+    // the last iteration max_iter allows: stop, unless the recursive residual says converged, which schedules
+    // the true-residual check of this iteration's solution in one more iteration (the solution reported is
+    // still this one, index it < max_iter, as in the reference's counting)
+    if (it + 1 >= fl[F_MAXIT] && !(it + 1 == fl[F_MAXIT] && rr[0] * sc[S_INV0] < sc[S_TOL2])) {
       sc[S_CONV] = sqrt(rr[0] * sc[S_INV0]);
       fl[F_DONE] = 2;
       return;
@@ -1161,9 +1164,12 @@ static int pcg_run(Sys& s, const PcgBufs& v, bool mixed, double tol, unsigned ma
   unsigned launched = 0, since = 0, next = whole(0.0);
   double prev_ratio2 = 1.0;
   int cur = 0, inflight = 0;
+  // the mixed loop may run one iteration past max_iter: the true-residual check of a solution that converged in
+  // the last allowed iteration (k_pcg_judge_beta)
+  const unsigned cap = mixed && maxit > 0 ? maxit + 1 : maxit;
   for (;;) {
-    if (inflight < 2 && launched < maxit && (inflight == 0 || next > 0)) {
-      const unsigned k = std::min(whole((double)next), maxit - launched);
+    if (inflight < 2 && launched < cap && (inflight == 0 || next > 0)) {
+      const unsigned k = std::min(whole((double)next), cap - launched);
       const int rc = enqueue_chunk(k, (cur + inflight) & 1);
       if (rc != DFM_OK) return rc;
       launched += k;
@@ -1233,7 +1239,7 @@ int solve_pcg(Sys& s, int precision, double* conv_ratio, unsigned* iteration) {
   int nrel = s.h_pinned[F_NREL];
   unsigned before = 0;  // iterations of a mixed-precision loop that was stopped
   // iterations executed by all loops of this solve (a loop that did not converge ran its whole budget)
-  unsigned total = ok ? it + 1 : maxit;
+  unsigned total = (unsigned)(s.h_pinned[F_IT] + 1);  // the device's count (a mixed check may run one past it)
   if (mixed && s.h_pinned[F_DONE] == 3) {
     // the FP32 matrix limits progress (cond(A) eps32 not small): continue from the FP64 solution with FP64
     // vectors and matrix, keeping the FP32 factor (the preconditioner's precision does not bound the attainable

@@ -174,6 +174,35 @@ def test_pcg_not_converged_contract(ctx):
     assert np.isclose(conv, cconv, rtol=1e-10)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pcg_max_iter_at_chunk_boundaries(ctx, precision):
+    """The device loop enqueues iterations in chunks of one or two 16-iteration graphs and reads the status of a
+    chunk while the next runs; max_iter must still end the loop exactly as the reference's counter does
+    (src/ls/solver_ls_iter.cpp:134-171): converged iff the converging iteration index is below max_iter (the
+    index returned), else max_iter returned with the last residual ratio, whatever max_iter is against the
+    chunk length. Unconverged FP64 ratios agree with the oracle's loop at the same max_iter."""
+    prec = N.FP64 if precision == "fp64" else N.MIXED
+    cpu, gs, order, _ = setup(N, ctx, KINDS[0], N.ORDER_NATURAL, small=False)
+    gs.factor()
+    b = gs.get_vector(-1)  # the solve leaves its final residual in the residual handle, as the reference's does
+    tol = 1e-10
+    ok, conv, git = gs.solve_pcg(tol, 2000, prec)
+    assert ok and git > 17, git
+    f = cpu.ilu0()
+    near = [1, 2, 15, 16, 17, 31, 32, 33, 47, 48, 49]
+    for maxit in sorted({m for m in near if m < git} | {git - 1, git, git + 1, git + 2, git + 17}):
+        gs.set_vector(-1, b)
+        ok2, conv2, it2 = gs.solve_pcg(tol, maxit, prec)
+        if maxit > git:
+            assert ok2 and it2 == git and conv2 == conv, (maxit, ok2, it2, conv2, conv)
+        else:
+            assert not ok2 and it2 == maxit, (maxit, ok2, it2)
+            if precision == "fp64" and maxit <= 49:
+                cok, cconv, cit, _, _ = cpu.solve(f, "pcg", tol, maxit)
+                assert not cok and cit == maxit
+                assert np.isclose(conv2, cconv, rtol=1e-6), (maxit, conv2, cconv)
+
+
 def test_singular_factor_reports(ctx):
     """An unconstrained Poisson matrix with zero conductivity: every pivot is singular (DoILUDecomp false)."""
     co, ln = _mesh(2)
