@@ -403,8 +403,12 @@ __global__ void __launch_bounds__(64) k_ilu_rows(int gbase, const int* __restric
 #pragma unroll
       for (int t = 0; t < BS; t++) acc[t] = 0.0;
     }
+    // the lane's update (kj, step) and the one after it are held ahead, so that the block A_kj of the next update
+    // is requested as soon as the current one is applied, without first waiting for its index
     unsigned nkj = u < ue ? ukj[u] : 0u;
     int nst = u < ue ? (int)ustep[u] : 255;
+    unsigned nkj2 = u + 1 < ue ? ukj[u + 1] : 0u;
+    int nst2 = u + 1 < ue ? (int)ustep[u + 1] : 255;
     double akj[BS];
     load_block<B>(fcrs, nkj, akj);
     for (int q = 0; q < steps; q++) {
@@ -414,9 +418,11 @@ __global__ void __launch_bounds__(64) k_ilu_rows(int gbase, const int* __restric
       if (nst == q) {
         sub_prod<B>(acc, aik, akj);
         u++;
-        nkj = u < ue ? ukj[u] : 0u;
-        nst = u < ue ? (int)ustep[u] : 255;
+        nkj = nkj2;
+        nst = nst2;
         load_block<B>(fcrs, nkj, akj);
+        nkj2 = u + 1 < ue ? ukj[u + 1] : 0u;
+        nst2 = u + 1 < ue ? (int)ustep[u + 1] : 255;
       }
     }
     if (dia && !invert_dia<B>(acc)) atomicAdd(nsing, 1);
