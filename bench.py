@@ -183,6 +183,101 @@ def cpu_baseline(w, gpu_iters, budget_s):
     return out
 
 
+def _agg_procs():
+    """Worker processes of the aggregate CPU legs: the host's CPU share (at most 16)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _c4_cpu_worker(job):
+    """One process of the C4 CPU aggregate: the oracle's C restatement solves requests (per-element materials as
+    the GPU gets them: pattern, assembly, Dirichlet, ILU(0) natural, PCG to 1e-8) until the deadline."""
+    idx, deadline = job
+    from delfem_amd import workloads as W
+    from oracle import pyref as P
+
+    dofs, lat = 0, []
+    for i in idx:
+        if time.time() > deadline:
+            break
+        w = W.c4_request(i)
+        a = time.perf_counter()
+        cpu = P.CpuSystem(w["ndim"], w["coords"], w["lnods"], w["len"])
+        cpu.assemble(w["kind"], w["params"])
+        cpu.set_bc(w["bc"])
+        f = cpu.ilu0()
+        cpu.solve(f, "pcg", 1e-8, 20000)
+        lat.append(time.perf_counter() - a)
+        dofs += w["ndof"]
+    return dofs, lat, time.time()
+
+
+def c4_cpu_aggregate(budget_s, nreq):
+    """C4 on the host: _agg_procs() independent processes take the first nreq requests of the stream round-robin
+    and solve them with the oracle's C restatement until budget_s has passed; DOF-solves/s over the processes."""
+    import multiprocessing as mp
+
+    P = _agg_procs()
+    t0 = time.time()
+    jobs = [(list(range(k, nreq, P)), t0 + budget_s) for k in range(P)]
+    with mp.get_context("spawn").Pool(P) as pool:
+        res = pool.map(_c4_cpu_worker, jobs)
+    t1 = max(r[2] for r in res)
+    dofs = sum(r[0] for r in res)
+    lat = sorted(x for r in res for x in r[1])
+    return {"value": dofs / (t1 - t0), "unit": "DOF-solves/s", "cores": P, "kind": "port",
+            "sample": f"c4 requests 0..{nreq - 1} round-robin over {P} processes, {len(lat)} solved in {t1 - t0:.1f}s "
+                      f"(budget {budget_s:.0f}s; pool start-up included), the oracle's C restatement, FP64 ILU(0)-PCG "
+                      f"to 1e-8, one thread each",
+            "requests_solved": len(lat), "p50_ms": pct(lat, 0.5) * 1e3 if lat else None,
+            "p99_ms": pct(lat, 0.99) * 1e3 if lat else None}
+
+
+def _c5_cpu_worker(k1k2):
+    """One process of the C5 CPU aggregate: one StVK Newton iteration of an instance with the oracle's C
+    restatement (tangent and residual, Dirichlet, ILU(0) natural) and the time per PCG iteration from runs of k1
+    and k2 iterations."""
+    k1, k2 = k1k2
+    import numpy as np
+
+    from delfem_amd import workloads as W
+    from oracle import pyref as P
+
+    w = W.c5()
+    nelem = w["lnods"].size // 4
+    a = time.perf_counter()
+    cpu = P.CpuSystem(3, w["coords"], w["lnods"], 3)
+    cpu.assemble(w["kind"], np.tile(w["params"], (nelem, 1)), np.zeros(w["coords"].size))
+    cpu.traction(w["faces"], w["trac"])
+    cpu.set_bc(w["bc"])
+    f = cpu.ilu0()
+    t_setup = time.perf_counter() - a
+    a = time.perf_counter()
+    cpu.solve(f, "pcg", 1e-30, k1)
+    b = time.perf_counter()
+    cpu.solve(f, "pcg", 1e-30, k2)
+    c = time.perf_counter()
+    return t_setup, ((c - b) - (b - a)) / (k2 - k1), w["coords"].size
+
+
+def c5_cpu_aggregate(newton_per_inst, pcg_per_inst, load_steps):
+    """C5 on the host: _agg_procs() processes each time one Newton iteration of an instance at once (contention
+    included); an instance's solve is extrapolated with the GPU run's Newton and PCG iterations per instance."""
+    import multiprocessing as mp
+
+    P = _agg_procs()
+    with mp.get_context("spawn").Pool(P) as pool:
+        res = pool.map(_c5_cpu_worker, [(3, 23)] * P)
+    t_setup = statistics.median(r[0] for r in res)
+    t_iter = statistics.median(r[1] for r in res)
+    ndof = res[0][2]
+    t_inst = newton_per_inst * t_setup + pcg_per_inst * t_iter
+    return {"value": P * ndof * load_steps / t_inst, "unit": "DOF-solves/s", "cores": P, "kind": "port",
+            "sample": f"{P} processes at once, each one StVK Newton iteration of one instance ({ndof} dofs) with the "
+                      f"oracle's C restatement: tangent+residual+ILU(0) {t_setup:.2f}s, PCG {t_iter * 1e3:.1f} ms/iteration "
+                      f"(medians); an instance extrapolated to the GPU run's {newton_per_inst:.1f} Newton and "
+                      f"{pcg_per_inst:.0f} PCG iterations -> {t_inst:.0f}s per instance"}
+
+
 def run_c5_batch(args, N, W):
     """StVK cantilever (64x16x16 hexes x 6 tets, 56,355 dofs) as a batch of served instances with seeded load
     magnitudes U[0.001, 0.02], end traction ramped over the load steps, Newton per step to ||r|| < 1e-6 ||r_0||
@@ -252,6 +347,11 @@ def run_c5_batch(args, N, W):
     }
     if args.c5_extra:
         out["extra_" + args.c5_extra] = dict(res[args.c5_extra], value=work / res[args.c5_extra]["t"])
+    if not args.no_cpu:
+        out["cpu_aggregate"] = c5_cpu_aggregate(res["fp64"]["newton"] / args.instances,
+                                                res["fp64"]["pcg"] / args.instances, args.load_steps)
+        log(f"[bench] c5 cpu aggregate: {out['cpu_aggregate']['value'] / 1e6:.3f} M DOF-solves/s "
+            f"({out['cpu_aggregate']['sample']})")
     for s in systems:
         s.close()
     ctx.close()
@@ -444,6 +544,10 @@ def run_c4_serving(args, N, W):
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic", "config": {"workload": sv["workload"]},
         "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"], "served": sv,
     }
+    if not args.no_cpu:
+        out["cpu_aggregate"] = c4_cpu_aggregate(args.cpu_budget * 2.0, min(args.requests, 400))
+        log(f"[bench] c4 cpu aggregate: {out['cpu_aggregate']['value'] / 1e6:.3f} M DOF-solves/s "
+            f"({out['cpu_aggregate']['sample']})")
     print(json.dumps(out), flush=True)
 
 
