@@ -969,7 +969,7 @@ def main():
                     help="c5: a third leg with DFM_FP32_ILU (fp32ilu) or DFM_MIXED (mixed32) inner solves")
     ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=100.0, help="c4: arrival rate (requests/s)")
-    ap.add_argument("--slots", type=int, default=64, help="c4: systems solved at once")
+    ap.add_argument("--slots", type=int, default=128, help="c4: systems solved at once (128: measured against 64 and 256)")
     ap.add_argument("--plain", type=int, default=200,
                     help="c4: requests (the first ones of the stream) also solved by the plain FP64 path one at a "
                          "time; -1: all, 0: none")
