@@ -406,7 +406,7 @@ def run_c4_plain(N, ctx, reqs, n):
             "max_true_rel_residual": float(np.max(trues)), "mean_pcg_iters": float(np.mean(iters))}
 
 
-def serve_stream(N, ctx, reqs, arrivals, slots, precision, ref=None, keep=False):
+def serve_stream(N, ctx, reqs, arrivals, slots, precision, ref=None, keep=False, step_iters=16):
     """The requests submitted at their arrival times (s from the start) to a continuous-batching server of the
     given precision (64 GB admission budget); every result is fetched when it finishes. Aggregate DOF-solves/s
     over the time from the first arrival to the last completion, latency from arrival to completion. ref: the
@@ -434,7 +434,7 @@ def serve_stream(N, ctx, reqs, arrivals, slots, precision, ref=None, keep=False)
             inflight[t] = (nxt, time.perf_counter() - t0)
             nxt += 1
         c1 = time.perf_counter()
-        active, queued = srv.progress(16, 1)
+        active, queued = srv.progress(step_iters, 1)
         c2 = time.perf_counter()
         tc["submit"] += c1 - c0
         tc["progress"] += c2 - c1
@@ -500,11 +500,11 @@ def served_object(args, N, W, ctx):
     total = float(sum(r["ndof"] for r in reqs))
     log(f"[bench] c4: {n} requests ({total / 1e6:.1f} M dofs) generated in {time.perf_counter() - t0:.1f}s, "
         f"arrivals {args.rate}/s, {args.slots} slots")
-    fp64, sols = serve_stream(N, ctx, reqs, arrivals, args.slots, N.FP64, keep=True)
+    fp64, sols = serve_stream(N, ctx, reqs, arrivals, args.slots, N.FP64, keep=True, step_iters=args.step_iters)
     log(f"[bench] c4 served fp64: {fp64['dofs_per_s'] / 1e6:.2f} M DOF-solves/s over {fp64['wall_s']:.1f}s, "
         f"p50 {fp64['p50_ms']:.0f} ms, p99 {fp64['p99_ms']:.0f} ms, {fp64['met_tolerance']}/{n} met 1e-8 (max true "
         f"residual {fp64['max_true_rel_residual']:.2e})")
-    mixed, _ = serve_stream(N, ctx, reqs, arrivals, args.slots, N.MIXED_AUTO, ref=sols)
+    mixed, _ = serve_stream(N, ctx, reqs, arrivals, args.slots, N.MIXED_AUTO, ref=sols, step_iters=args.step_iters)
     del sols
     log(f"[bench] c4 served mixed: {mixed['dofs_per_s'] / 1e6:.2f} M DOF-solves/s over {mixed['wall_s']:.1f}s, "
         f"p50 {mixed['p50_ms']:.0f} ms, p99 {mixed['p99_ms']:.0f} ms, {mixed['met_tolerance']}/{n} met 1e-8 (max true "
@@ -969,6 +969,7 @@ def main():
                     help="c5: a third leg with DFM_FP32_ILU (fp32ilu) or DFM_MIXED (mixed32) inner solves")
     ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=100.0, help="c4: arrival rate (requests/s)")
+    ap.add_argument("--step-iters", type=int, default=16, help="c4: batched iterations per server step")
     ap.add_argument("--slots", type=int, default=128, help="c4: systems solved at once (128: measured against 64 and 256)")
     ap.add_argument("--plain", type=int, default=200,
                     help="c4: requests (the first ones of the stream) also solved by the plain FP64 path one at a "
